@@ -1,0 +1,529 @@
+// vv_api.cpp -- C ABI of libvvmi (declared in include/vvmi.h): run
+// lifetime, host/device field transfer, the reference routines as device
+// operations, and the fused multi-step driver.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vv_internal.h"
+
+namespace vv {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return VV_EHIP;
+}
+
+// rb.f:3046-3064 elimination factors (same operations as the reference)
+TdmaTab make_tdma(const std::vector<double> &a, const std::vector<double> &b,
+                  const std::vector<double> &c)
+{
+    TdmaTab t;
+    const int n = (int)b.size();
+    t.n = n;
+    t.fact.assign(n, 0.0);
+    t.bp = b;
+    t.c = c;
+    for (int j = 1; j < n; ++j) {
+        const double fact = a[j] / t.bp[j - 1];
+        t.fact[j] = fact;
+        t.bp[j] = t.bp[j] - fact * c[j - 1];
+    }
+    return t;
+}
+
+// rb.f:3066-3116 elimination factors; arrays are 1-based (index 0 unused)
+PtdmaTab make_ptdma(const std::vector<double> &a0, const std::vector<double> &b0,
+                    const std::vector<double> &c0)
+{
+    PtdmaTab t;
+    const int n = (int)b0.size();
+    t.n = n;
+    std::vector<double> a(n + 1), b(n + 1), c(n + 1);
+    for (int j = 1; j <= n; ++j) { a[j] = a0[j - 1]; b[j] = b0[j - 1]; c[j] = c0[j - 1]; }
+    t.f1.assign(n + 1, 0.0);
+    t.f2.assign(n + 1, 0.0);
+    double fact;
+    for (int j = 2; j <= n - 2; ++j) {
+        fact = a[j] / b[j - 1];
+        t.f1[j] = fact;
+        b[j] = b[j] - fact * c[j - 1];
+        a[j] = -fact * a[j - 1];
+        fact = c[n] / b[j - 1];
+        t.f2[j] = fact;
+        c[n] = -fact * c[j - 1];
+        b[n] = b[n] - fact * a[j - 1];
+    }
+    fact = a[n - 1] / b[n - 2];
+    t.fA = fact;
+    b[n - 1] = b[n - 1] - fact * c[n - 2];
+    c[n - 1] = c[n - 1] - fact * a[n - 2];
+    fact = c[n] / b[n - 2];
+    t.fB = fact;
+    a[n] = a[n] - fact * c[n - 2];
+    b[n] = b[n] - fact * a[n - 2];
+    fact = a[n] / b[n - 1];
+    t.fC = fact;
+    b[n] = b[n] - fact * c[n - 1];
+    t.a = a;
+    t.b = b;
+    t.c = c;
+    return t;
+}
+
+}  // namespace vv
+
+using namespace vv;
+
+namespace {
+
+struct Pending {
+    bool c = false, t = false;
+};
+
+// pending Robert filter: between the explicit step and update the filtered
+// current level lives in Cf/thf
+Pending &pending(vv_run *r)
+{
+    static thread_local std::vector<std::pair<vv_run *, Pending>> tab;
+    for (auto &e : tab)
+        if (e.first == r) return e.second;
+    tab.emplace_back(r, Pending{});
+    return tab.back().second;
+}
+
+int alloc_field(vv_run *r, double **p)
+{
+    if (hipMalloc(p, r->fbytes) != hipSuccess) {
+        set_error("device allocation failed");
+        return VV_ENOMEM;
+    }
+    VV_HIP(hipMemsetAsync(*p, 0, r->fbytes, r->stream));
+    return VV_OK;
+}
+
+double *field_ptr(vv_run *r, int f)
+{
+    Pending &pd = pending(r);
+    switch (f) {
+    case VV_C: return pd.c ? r->Cf : r->C;
+    case VV_CP: return r->Cp;
+    case VV_CN: return r->Cn;
+    case VV_TH: return pd.t ? r->thf : r->th;
+    case VV_THP: return r->thp;
+    case VV_THN: return r->thn;
+    case VV_UEL: return r->uel;
+    case VV_VEL: return r->vel;
+    case VV_UELP: return r->uelp;
+    case VV_VELP: return r->velp;
+    case VV_VRH: return r->vrh;
+    case VV_URH: return r->urh;
+    case VV_MEAN_T2: return r->mT2;
+    case VV_MEAN_T: return r->mT;
+    case VV_MEAN_U2: return r->mU2;
+    case VV_MEAN_V2: return r->mV2;
+    default: return nullptr;
+    }
+}
+
+int upload_tab(vv_run *r, double **dst, const std::vector<double> &h)
+{
+    VV_HIP(hipMalloc(dst, h.size() * sizeof(double)));
+    VV_HIP(hipMemcpyAsync(*dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    return VV_OK;
+}
+
+int build_tables(vv_run *r)
+{
+    const int nr = r->nr, np = r->np;
+    {   // Cr, thr at vorticity points: 1-4-1, n = nr-1 (rb.f:810-826)
+        const int n = nr - 1;
+        std::vector<double> a(n, 1.0), b(n, 4.0), c(n, 1.0);
+        TdmaTab t = make_tdma(a, b, c);
+        std::vector<double> h;
+        h.insert(h.end(), t.fact.begin(), t.fact.end());
+        h.insert(h.end(), t.bp.begin(), t.bp.end());
+        h.insert(h.end(), t.c.begin(), t.c.end());
+        int st = upload_tab(r, &r->tab_r1, h);
+        if (st) return st;
+    }
+    {   // thr at temperature points with closure rows (rb.f:1244-1262)
+        const int n = nr;
+        std::vector<double> a(n, 1.0), b(n, 4.0), c(n, 1.0);
+        b[0] = 1.0; c[0] = 2.0;
+        a[n - 1] = 2.0; b[n - 1] = 1.0;
+        TdmaTab t = make_tdma(a, b, c);
+        std::vector<double> h;
+        h.insert(h.end(), t.fact.begin(), t.fact.end());
+        h.insert(h.end(), t.bp.begin(), t.bp.end());
+        h.insert(h.end(), t.c.begin(), t.c.end());
+        int st = upload_tab(r, &r->tab_rT, h);
+        if (st) return st;
+    }
+    {   // cyclic 1-4-1 on np+1 points (rb.f:833-845)
+        const int n = np + 1;
+        std::vector<double> a(n, 1.0), b(n, 4.0), c(n, 1.0);
+        PtdmaTab t = make_ptdma(a, b, c);
+        std::vector<double> h;
+        h.insert(h.end(), t.f1.begin(), t.f1.end());
+        h.insert(h.end(), t.f2.begin(), t.f2.end());
+        h.insert(h.end(), t.a.begin(), t.a.end());
+        h.insert(h.end(), t.b.begin(), t.b.end());
+        h.insert(h.end(), t.c.begin(), t.c.end());
+        h.push_back(t.fA);
+        h.push_back(t.fB);
+        h.push_back(t.fC);
+        int st = upload_tab(r, &r->tab_p, h);
+        if (st) return st;
+    }
+    return VV_OK;
+}
+
+void rotate(vv_run *r)
+{
+    double *oC = r->C, *oCp = r->Cp;
+    r->C = r->Cn;
+    r->Cp = r->Cf;
+    r->Cn = oCp;
+    r->Cf = oC;
+    double *oT = r->th, *oTp = r->thp;
+    r->th = r->thn;
+    r->thp = r->thf;
+    r->thn = oTp;
+    r->thf = oT;
+}
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char *vv_last_error(void) { return g_err.c_str(); }
+
+int vv_device_count(int *n)
+{
+    if (!n) return VV_EINVAL;
+    VV_HIP(hipGetDeviceCount(n));
+    return VV_OK;
+}
+
+int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
+{
+    if (!cfg || !out) { set_error("null argument"); return VV_EINVAL; }
+    *out = nullptr;
+    if (cfg->nr < 8 || cfg->np < 8 || !pow2(cfg->np)) {
+        set_error("need nr >= 8 and np a power of two >= 8");
+        return VV_EINVAL;
+    }
+    if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->width > 0) || !(cfg->Pr > 0)) {
+        set_error("bad physical parameters");
+        return VV_EINVAL;
+    }
+    VV_HIP(hipSetDevice(device));
+    vv_run *r = new vv_run();
+    r->cfg = *cfg;
+    if (r->cfg.rtol <= 0) r->cfg.rtol = 1e-10;
+    if (r->cfg.max_cycles <= 0) r->cfg.max_cycles = 100;
+    if (r->cfg.nu1 <= 0) r->cfg.nu1 = 1;
+    if (r->cfg.nu2 <= 0) r->cfg.nu2 = 1;
+    if (r->cfg.robert < 0) r->cfg.robert = 0.02;
+    r->device = device;
+    r->nr = cfg->nr;
+    r->np = cfg->np;
+    r->ld = ((cfg->nr + 2 + 7) / 8) * 8;
+    r->ncols = cfg->np + 1;
+    r->dr = (cfg->rmax - cfg->r0) / cfg->nr;
+    r->dp = cfg->width / cfg->np;
+    r->fbytes = (size_t)r->ld * r->ncols * sizeof(double);
+    int st = VV_OK;
+    auto fail = [&](int s) { vv_destroy(r); return s; };
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return fail(VV_EHIP);
+    if (hipEventCreate(&r->ev0) != hipSuccess || hipEventCreate(&r->ev1) != hipSuccess) return fail(VV_EHIP);
+    double **fields[] = {&r->C, &r->Cp, &r->Cn, &r->Cf, &r->th, &r->thp, &r->thn, &r->thf,
+                         &r->uel, &r->vel, &r->uelp, &r->velp, &r->vrh, &r->urh,
+                         &r->dCr, &r->dCh, &r->dthrC, &r->dthhC, &r->dthrT, &r->dthhT,
+                         &r->mT2, &r->mT, &r->mU2, &r->mV2};
+    for (double **f : fields)
+        if ((st = alloc_field(r, f))) return fail(st);
+    if ((st = build_tables(r))) return fail(st);
+    if (hipMalloc(&r->mg_flags, 4 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->mg_bits, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->mg_relres, 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if ((st = mg_build(r))) return fail(st);
+    r->diag_cap = 1 << 16;
+    if (hipMalloc(&r->diag, (size_t)r->diag_cap * 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->diag_i, 4 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->cfl_bits, 2 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMemsetAsync(r->diag_i, 0, 4 * sizeof(int), r->stream) != hipSuccess) return fail(VV_EHIP);
+    if (hipMemsetAsync(r->cfl_bits, 0, 2 * sizeof(unsigned long long), r->stream) != hipSuccess)
+        return fail(VV_EHIP);
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
+    if (nlev) *nlev = r->nlev_v;
+    *out = r;
+    return VV_OK;
+}
+
+int vv_destroy(vv_run *r)
+{
+    if (!r) return VV_OK;
+    (void)hipSetDevice(r->device);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
+                        r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
+                        r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->tab_r1, r->tab_rT,
+                        r->tab_p, r->mg_pool, r->mg_relres, r->diag};
+    for (double *f : fields)
+        if (f) (void)hipFree(f);
+    if (r->mg_dev) (void)hipFree(r->mg_dev);
+    if (r->mg_flags) (void)hipFree(r->mg_flags);
+    if (r->mg_bits) (void)hipFree(r->mg_bits);
+    if (r->diag_i) (void)hipFree(r->diag_i);
+    if (r->cfl_bits) (void)hipFree(r->cfl_bits);
+    if (r->ev0) (void)hipEventDestroy(r->ev0);
+    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return VV_OK;
+}
+
+int vv_put(vv_run *r, int field, const double *host, int ld)
+{
+    if (!r || !host || ld < r->nr + 2) { set_error("vv_put: bad argument"); return VV_EINVAL; }
+    double *d = field_ptr(r, field);
+    if (!d) { set_error("vv_put: unknown field"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipMemcpy2DAsync(d, (size_t)r->ld * sizeof(double), host, (size_t)ld * sizeof(double),
+                            (size_t)(r->nr + 2) * sizeof(double), r->ncols, hipMemcpyHostToDevice,
+                            r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vv_get(vv_run *r, int field, double *host, int ld)
+{
+    if (!r || !host || ld < r->nr + 2) { set_error("vv_get: bad argument"); return VV_EINVAL; }
+    double *d = field_ptr(r, field);
+    if (!d) { set_error("vv_get: unknown field"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipMemcpy2DAsync(host, (size_t)ld * sizeof(double), d, (size_t)r->ld * sizeof(double),
+                            (size_t)(r->nr + 2) * sizeof(double), r->ncols, hipMemcpyDeviceToHost,
+                            r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vv_start(vv_run *r)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipMemcpyAsync(r->Cp, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    VV_HIP(hipMemcpyAsync(r->thp, r->th, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    VV_HIP(hipMemcpyAsync(r->uelp, r->uel, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    VV_HIP(hipMemcpyAsync(r->velp, r->vel, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    r->steps = 0;
+    r->last_cycles[0] = r->last_cycles[1] = 0;
+    pending(r) = Pending{};
+    return VV_OK;
+}
+
+int vv_cvorticity(vv_run *r, int first)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    int st = launch_compact_mask(r, 1 | 2 | 8 | 16);
+    if (!st) st = launch_df(r, first ? 1 : 0, 1, 0);
+    if (!st) {
+        VV_HIP(hipStreamSynchronize(r->stream));
+        pending(r).c = true;
+    }
+    return st;
+}
+
+int vv_temperature(vv_run *r, int first)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    int st = launch_compact_mask(r, 4 | 32);
+    if (!st) st = launch_df(r, first ? 1 : 0, 0, 1);
+    if (!st) {
+        VV_HIP(hipStreamSynchronize(r->stream));
+        pending(r).t = true;
+    }
+    return st;
+}
+
+int vv_heat(vv_run *r, double *trans1, double *trans2)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    const double *th = pending(r).t ? r->thf : r->th;
+    int st = launch_heat_from(r, th, r->diag);
+    if (st) return st;
+    double h[2];
+    VV_HIP(hipMemcpyAsync(h, r->diag, 2 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    if (trans1) *trans1 = h[0];
+    if (trans2) *trans2 = h[1];
+    return VV_OK;
+}
+
+int vv_rhs(vv_run *r)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    int st = mg_reset(r, 5);
+    if (!st) st = launch_rhs_guess(r, 3);
+    if (!st) VV_HIP(hipStreamSynchronize(r->stream));
+    return st;
+}
+
+int vv_poisson(vv_run *r, int which, int *cycles, double *relres)
+{
+    if (!r || (which != 2 && which != 3 && which != 5)) { set_error("vv_poisson: bad argument"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    const int w = (which == 5) ? 3 : (which == 2 ? 1 : 2);
+    int st = mg_reset(r, which);
+    if (!st) st = launch_rhs_fields(r, w);
+    if (st) return st;
+    int cyc[2] = {0, 0};
+    double rel[2] = {0, 0};
+    st = mg_solve(r, which, cyc, rel);
+    if (st < 0) return st;
+    int st2 = launch_mg_scatter_out(r, which);
+    if (st2) return st2;
+    VV_HIP(hipStreamSynchronize(r->stream));
+    if (cycles) { cycles[0] = (which == 3) ? cyc[1] : cyc[0]; if (which == 5) cycles[1] = cyc[1]; }
+    if (relres) { relres[0] = (which == 3) ? rel[1] : rel[0]; if (which == 5) relres[1] = rel[1]; }
+    return st;
+}
+
+int vv_poisson_host(vv_run *r, int which, double *psi, double *psip, const double *rhs, int ld,
+                    int *cycles, double *relres)
+{
+    if (!r || !psi || !psip || !rhs || (which != 2 && which != 3)) {
+        set_error("vv_poisson_host: bad argument");
+        return VV_EINVAL;
+    }
+    int st;
+    const int fpsi = (which == 2) ? VV_VEL : VV_UEL;
+    const int fpsip = (which == 2) ? VV_VELP : VV_UELP;
+    const int frhs = (which == 2) ? VV_VRH : VV_URH;
+    if ((st = vv_put(r, fpsi, psi, ld))) return st;
+    if ((st = vv_put(r, fpsip, psip, ld))) return st;
+    if ((st = vv_put(r, frhs, rhs, ld))) return st;
+    const int pst = vv_poisson(r, which, cycles, relres);
+    if (pst < 0) return pst;
+    if ((st = vv_get(r, fpsi, psi, ld))) return st;
+    if ((st = vv_get(r, fpsip, psip, ld))) return st;
+    return pst;
+}
+
+int vv_update(vv_run *r, int *nclamp)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    Pending &pd = pending(r);
+    if (!pd.c) VV_HIP(hipMemcpyAsync(r->Cf, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    if (!pd.t) VV_HIP(hipMemcpyAsync(r->thf, r->th, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
+    VV_HIP(hipMemsetAsync(r->diag_i, 0, sizeof(int), r->stream));
+    int st = launch_update(r, (int)(r->steps + 1));
+    if (st) return st;
+    int nc = 0;
+    VV_HIP(hipMemcpyAsync(&nc, r->diag_i, sizeof(int), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    rotate(r);
+    pd = Pending{};
+    r->steps += 1;
+    if (nclamp) *nclamp = nc;
+    return VV_OK;
+}
+
+int vv_step(vv_run *r, int nsteps, vv_step_info *info)
+{
+    if (!r || nsteps < 0) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    pending(r) = Pending{};
+    int status = VV_OK;
+    int cyc[2] = {0, 0};
+    double rel[2] = {0, 0};
+    VV_HIP(hipEventRecord(r->ev0, r->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const int first = (r->steps == 0) ? 1 : 0;
+        int st = launch_compact(r);
+        if (!st) st = launch_df(r, first, 1, 1);
+        if (!st) st = launch_heat(r, r->diag + 2 * (size_t)(r->steps % r->diag_cap));
+        if (!st) st = mg_reset(r, 5);
+        if (!st) st = launch_rhs_guess(r, 3);
+        if (st) return st;
+        st = mg_solve(r, 5, cyc, rel);
+        if (st < 0) return st;
+        if (st == VV_NOT_CONVERGED) status = VV_NOT_CONVERGED;
+        if ((st = launch_mg_scatter_out(r, 5))) return st;
+        if ((st = launch_update(r, (int)(r->steps + 1)))) return st;
+        rotate(r);
+        r->steps += 1;
+    }
+    VV_HIP(hipEventRecord(r->ev1, r->stream));
+    VV_HIP(hipEventSynchronize(r->ev1));
+    float ms = 0.f;
+    VV_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->last_ms = ms;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        if (r->steps > 0) {
+            double tr[2];
+            VV_HIP(hipMemcpy(tr, r->diag + 2 * (size_t)((r->steps - 1) % r->diag_cap),
+                             2 * sizeof(double), hipMemcpyDeviceToHost));
+            info->trans1 = tr[0];
+            info->trans2 = tr[1];
+        }
+        unsigned long long cb[2];
+        int nc = 0;
+        VV_HIP(hipMemcpy(cb, r->cfl_bits, sizeof(cb), hipMemcpyDeviceToHost));
+        VV_HIP(hipMemcpy(&nc, r->diag_i, sizeof(int), hipMemcpyDeviceToHost));
+        std::memcpy(&info->cflu, &cb[0], sizeof(double));
+        std::memcpy(&info->cflv, &cb[1], sizeof(double));
+        info->cycles_v = cyc[0];
+        info->cycles_u = cyc[1];
+        info->relres_v = rel[0];
+        info->relres_u = rel[1];
+        info->nclamp = nc;
+        info->status = status;
+    }
+    return status;
+}
+
+int vv_heat_history(vv_run *r, double *trans, int n)
+{
+    if (!r || !trans || n < 0) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    long long avail = r->steps < r->diag_cap ? r->steps : r->diag_cap;
+    long long cnt = n < avail ? n : avail;
+    for (long long j = 0; j < cnt; ++j) {
+        long long s = r->steps - cnt + j;
+        VV_HIP(hipMemcpy(trans + 2 * j, r->diag + 2 * (size_t)(s % r->diag_cap), 2 * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    }
+    return (int)cnt >= 0 ? VV_OK : VV_EINVAL;
+}
+
+int vv_last_step_ms(vv_run *r, double *ms)
+{
+    if (!r || !ms) return VV_EINVAL;
+    *ms = r->last_ms;
+    return VV_OK;
+}
+
+}  // extern "C"

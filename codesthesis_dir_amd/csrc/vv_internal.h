@@ -1,0 +1,153 @@
+// vv_internal.h -- shared declarations of the HIP library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/vvmi.h"
+
+namespace vv {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define VV_HIP(call)                                             \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::vv::hip_fail(e_, #call);  \
+    } while (0)
+
+// ------------------------------------------------ banded elimination tables
+// Thomas factors of a constant-coefficient tridiagonal system exactly as the
+// reference tdma computes them (rb.f:3046-3064): fact(j) = a(j)/b'(j-1),
+// b'(j) = b(j) - fact(j)*c(j-1).  Index j is 0-based.
+struct TdmaTab {
+    int n = 0;
+    std::vector<double> fact, bp, c;
+};
+TdmaTab make_tdma(const std::vector<double> &a, const std::vector<double> &b,
+                  const std::vector<double> &c);
+
+// Factors of the reference cyclic solver ptdma (rb.f:3066-3116) for a
+// constant-coefficient system: the r-updates use f1(j), f2(j) for
+// j = 2..n-2 (1-based), then fA, fB, fC; back substitution uses the final
+// a', b', c'.
+struct PtdmaTab {
+    int n = 0;
+    std::vector<double> f1, f2, a, b, c;   // 1-based storage, size n+1
+    double fA = 0, fB = 0, fC = 0;
+};
+PtdmaTab make_ptdma(const std::vector<double> &a, const std::vector<double> &b,
+                    const std::vector<double> &c);
+
+// Banded (2 sub, 2 super) LU without pivoting of one line operator.
+struct BandLU {
+    int n = 0;
+    std::vector<double> l1, l2, u1, u2, dinv;
+};
+BandLU make_band_lu(int n, const std::vector<double> &sub2, const std::vector<double> &sub1,
+                    const std::vector<double> &diag, const std::vector<double> &sup1,
+                    const std::vector<double> &sup2);
+
+// ------------------------------------------------------------- multigrid
+// Device layout of one multigrid level: rows i = 0..n-1 (unknown rows of
+// the line operator), columns q = 0..ncol-1 (periodic).  Columns are stored
+// by colour: index = (q&1)*n*h + i*h + (q>>1), h = max(ncol/2,1), so the
+// threads of a half-sweep touch consecutive addresses.
+struct MgLevelDev {
+    double *u, *f;
+    const double *lu;     // 5*n: l1 | l2 | u1 | u2 | dinv
+    double cphi;          // 1/hp^2 at this level
+    int ncol, h;
+};
+
+constexpr int MG_MAXLEV = 16;
+constexpr int MG_MAXPROB = 2;
+
+struct MgProblemDev {
+    int n;                // unknown rows
+    int nlev;
+    const double *sten;   // 5*n stencil of the r operator D_r (i-2..i+2)
+    const double *rphi;   // n: row factor of the phi coupling (1/r^2; 1 in the box)
+    MgLevelDev lev[MG_MAXLEV];
+};
+
+// one batch of problems solved together (v and u of one run)
+struct MgBatchDev {
+    int nprob;
+    MgProblemDev prob[MG_MAXPROB];
+    int *done;            // [nprob] converged / capped flag
+    int *cycles;          // [nprob]
+    unsigned long long *res;    // [nprob] max |r| bits (non-negative doubles)
+    unsigned long long *fnorm;  // [nprob] max |f| bits
+    double *relres;       // [nprob]
+};
+
+}  // namespace vv
+
+// ---------------------------------------------------------------- vv_run
+struct vv_run {
+    vv_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nr = 0, np = 0, ld = 0, ncols = 0;
+    double dr = 0, dp = 0;
+    size_t fbytes = 0;
+
+    // field buffers (device, layout F[k*ld+i]); rotation by pointer swap
+    double *C = nullptr, *Cp = nullptr, *Cn = nullptr, *Cf = nullptr;
+    double *th = nullptr, *thp = nullptr, *thn = nullptr, *thf = nullptr;
+    double *uel = nullptr, *vel = nullptr, *uelp = nullptr, *velp = nullptr;
+    double *vrh = nullptr, *urh = nullptr;
+    // compact-derivative scratch (the reference loans A, B, An, Bn for these)
+    double *dCr = nullptr, *dCh = nullptr, *dthrC = nullptr, *dthhC = nullptr;
+    double *dthrT = nullptr, *dthhT = nullptr;
+    // running means (rb.f:409-417)
+    double *mT2 = nullptr, *mT = nullptr, *mU2 = nullptr, *mV2 = nullptr;
+
+    // elimination tables (device)
+    double *tab_r1 = nullptr;   // tdma 1-4-1, n = nr-1: fact | bp | c
+    double *tab_rT = nullptr;   // tdma with closure rows, n = nr
+    double *tab_p = nullptr;    // ptdma 1-4-1, n = np+1: f1 | f2 | a | b | c | fA fB fC
+
+    // multigrid
+    vv::MgBatchDev mg{};
+    vv::MgBatchDev *mg_dev = nullptr;
+    double *mg_pool = nullptr;
+    int *mg_flags = nullptr;    // done[2], cycles[2]
+    unsigned long long *mg_bits = nullptr;  // res[2], fnorm[2]
+    double *mg_relres = nullptr;
+    int nlev_v = 0, nlev_u = 0;
+
+    // per-step diagnostics ring: [trans1, trans2, cflu, cflv] per step
+    double *diag = nullptr;
+    int *diag_i = nullptr;      // nclamp accumulators
+    unsigned long long *cfl_bits = nullptr;
+    int diag_cap = 0;
+    long long steps = 0;        // completed steps since vv_start
+    int last_cycles[2] = {0, 0};
+    double last_relres[2] = {0, 0};
+    float last_ms = 0.f;
+};
+
+namespace vv {
+// explicit-step launchers (vv_explicit.hip)
+int launch_compact(vv_run *r);
+int launch_df(vv_run *r, int first, int do_c, int do_t);
+int launch_heat(vv_run *r, double *dst2);
+int launch_rhs_guess(vv_run *r, int which);
+int launch_update(vv_run *r, int nstat_kt);
+int launch_copy_prev(vv_run *r);
+// multigrid (vv_mg.hip)
+int mg_build(vv_run *r);
+int mg_solve(vv_run *r, int which, int *cycles, double *relres);
+int launch_mg_scatter_out(vv_run *r, int which);
+int mg_reset(vv_run *r, int which);
+int launch_compact_mask(vv_run *r, int mask);
+int launch_heat_from(vv_run *r, const double *th, double *dst2);
+int launch_rhs_fields(vv_run *r, int which);
+}  // namespace vv

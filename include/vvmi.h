@@ -1,0 +1,136 @@
+/*
+ * vvmi.h -- C ABI of the MI355X velocity-vorticity time-step library
+ * (libvvmi.so, HIP for gfx950).
+ *
+ * The library replaces, behind a thin ISO_C_BINDING shim, the hot path of
+ * the thesis codes: the explicit compact-scheme DuFort-Frankel step of
+ * vorticity and temperature, the multigrid Poisson solves for the velocity
+ * components, and the wall heat-flux diagnostic.  Every entry point below
+ * names the reference routine it stands in for (paths are relative to the
+ * reference repository; rb.f = monashubuntudir_velvor2dperiodicrbconvdir/
+ * velvor2dperiodicrbconv.f).
+ *
+ * Conventions
+ *   - Host arrays use the reference's column-major layout F(0:kr,1,0:kp):
+ *     element (i,k) at F[k*ld + i] with ld = kr+1 (the caller's PARAMETER).
+ *   - i = 0..nr+1 runs across the walls ("r"), k = 0..np along the periodic
+ *     direction ("phi"); k = 0 and k = np are the same physical column.
+ *   - All arithmetic is IEEE binary64.
+ *   - Every function returns VV_OK (0) or a nonzero status; no C++
+ *     exception crosses the boundary.  vv_last_error() returns a message.
+ *   - A vv_run owns device storage for one simulation; calls on one run are
+ *     not thread-safe, calls on different runs are.
+ */
+#ifndef VVMI_H
+#define VVMI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VV_OK            0
+#define VV_NOT_CONVERGED 1    /* Poisson cycle cap hit (reference: STOP after 500 sweeps, rb.f:1843) */
+#define VV_EINVAL       -1    /* bad argument (sizes, null pointers)        */
+#define VV_EHIP         -2    /* HIP runtime error                          */
+#define VV_ENOMEM       -3    /* device allocation failed                   */
+#define VV_ENONFINITE   -4    /* NaN/Inf met in a Poisson right-hand side   */
+
+/* field identifiers for vv_put / vv_get (the reference COMMON arrays,
+ * rb.f:69-76) */
+enum vv_field {
+    VV_C = 0, VV_CP, VV_CN,        /* vorticity, previous, new           */
+    VV_TH, VV_THP, VV_THN,         /* temperature, previous, new         */
+    VV_UEL, VV_VEL, VV_UELP, VV_VELP,
+    VV_VRH, VV_URH,                /* Poisson right-hand sides            */
+    VV_MEAN_T2, VV_MEAN_T, VV_MEAN_U2, VV_MEAN_V2,   /* running means     */
+    VV_NFIELDS
+};
+
+/* Geometry and physics of one run: the reference's COMMON /num/ /val/
+ * /radius/ (rb.f:72-75) plus the knobs it hard-codes. */
+typedef struct vv_config {
+    int    nr, np;        /* intervals across / along (np: power of two)   */
+    double dt;
+    double r0, rmax;      /* walls; dr = (rmax-r0)/nr                       */
+    double width;         /* periodic length; dp = width/np (rb.f:132-133) */
+    double Ra, Pr;
+    double robert;        /* Robert filter coefficient (0.02, rb.f:1043)   */
+    double rtol;          /* Poisson stop: inf-norm residual <= rtol*|rhs| */
+    int    max_cycles;    /* V-cycle cap per solve                          */
+    int    nu1, nu2;      /* zebra line sweeps before/after coarse grid     */
+    int    stats;         /* 1: running means + CFL every step (rb.f:409-458) */
+} vv_config;
+
+/* per-step diagnostics (device-resident until read) */
+typedef struct vv_step_info {
+    double trans1, trans2;     /* wall heat flux, rb.f heat() (rb.f:1452)  */
+    double cflu, cflv;         /* rb.f:421-458                              */
+    double relres_v, relres_u; /* final relative residual of each solve    */
+    int    cycles_v, cycles_u; /* V-cycles used                             */
+    int    nclamp;             /* temperature clamps (rb.f:1614-1621)       */
+    int    status;
+} vv_step_info;
+
+typedef struct vv_run vv_run;
+
+const char *vv_last_error(void);
+int vv_device_count(int *n);
+
+/* Create a run on a device.  Replaces the static COMMON storage and
+ * poissoninit (rb.f:1724): the multigrid hierarchy is built here.
+ * nlev (out, may be NULL) receives the number of multigrid levels. */
+int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev);
+int vv_destroy(vv_run *run);
+
+/* Copy one field between a host array (leading dimension ld >= nr+2,
+ * np+1 columns) and the run. */
+int vv_put(vv_run *run, int field, const double *host, int ld);
+int vv_get(vv_run *run, int field, double *host, int ld);
+
+/* The reference start-up copy Xp = X (rb.f:315-329) and step counter reset. */
+int vv_start(vv_run *run);
+
+/* Explicit parts of the time loop on the device-resident state.
+ * first != 0 selects the forward-Euler first-step forms.
+ *   vv_cvorticity  -> Cvorticity1 / Cvorticity   (rb.f:584, rb.f:786)
+ *   vv_temperature -> temperature1 / temperature (rb.f:1056, rb.f:1220)
+ *   vv_heat        -> heat(trans1,trans2)        (rb.f:1452)
+ *   vv_rhs         -> rhsv(vrh) and rhsu(urh)    (rb.f:1629, rb.f:1679)
+ *   vv_update      -> update                     (rb.f:1557)            */
+int vv_cvorticity(vv_run *run, int first);
+int vv_temperature(vv_run *run, int first);
+int vv_heat(vv_run *run, double *trans1, double *trans2);
+int vv_rhs(vv_run *run);
+int vv_update(vv_run *run, int *nclamp);
+
+/* Poisson solves with the reference's in/out conventions: initial guess
+ * 2*psi - psip, on return psip = previous psi, psi = solution with zero
+ * wall rows and column 0 equal to column np.
+ *   which = 2 -> poisson2(vel,velp,vrh,...)  (rb.f:1796)
+ *   which = 3 -> poisson3(uel,uelp,urh,...)  (rb.f:1930)
+ *   which = 5 -> both, batched in the same launches                      */
+int vv_poisson(vv_run *run, int which, int *cycles, double *relres);
+
+/* Host-array drop-in of poisson2/poisson3 with the reference argument
+ * list: psi, psip, rhs are host arrays with leading dimension ld.
+ * Replaces poisson2 (rb.f:1796) and poisson3 (rb.f:1930). */
+int vv_poisson_host(vv_run *run, int which, double *psi, double *psip,
+                    const double *rhs, int ld, int *cycles, double *relres);
+
+/* nsteps complete time steps (rb.f:341-470 loop body, the first step of a
+ * run in its forward-Euler form) without host copies; info (may be NULL)
+ * receives the diagnostics of the last step. */
+int vv_step(vv_run *run, int nsteps, vv_step_info *info);
+
+/* Diagnostics of the steps run so far: trans1/trans2 per step (rb.f:384-388
+ * multiplies them by (rmax-r0)/width to get the Nusselt numbers).
+ * Copies min(n, steps) pairs of the most recent steps. */
+int vv_heat_history(vv_run *run, double *trans, int n);
+
+/* Timing of the last vv_step call on the run's stream (milliseconds). */
+int vv_last_step_ms(vv_run *run, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
