@@ -1,0 +1,164 @@
+"""Benchmark of the velocity-vorticity time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+A step is one pass of the time loop (compact-scheme DuFort-Frankel step of
+vorticity and temperature, both multigrid Poisson solves to 1e-10 relative
+residual, wall update, heat flux, running means) over the device-resident
+fields.  The timed region runs K steps after W warm-up steps; inputs are
+resident in HBM before it starts.  A second pass of K steps with per-kernel
+HIP events gives the dominant kernel's average duration and its achieved
+algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
+reference's CPU path on a bounded sample of the same workload on this host.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time-steps/s & V-cycle HBM GB/s; served solves/s + p99 at 1e-10 residual"
+
+# name -> (description, BoxConfig kwargs, seed)
+WORKLOADS = {
+    "box-129x65": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 129x65 nodes, 2:1, "
+                   "Ra=1e5, Pr=1, dt=1e-5", dict(nr=64, np=128, width=1.0, dt=1e-5, Ra=1e5, Pr=1.0),
+                   20261202),
+    "box-513x257": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 513x257 nodes "
+                    "(nr=512 across, np=256 periodic), width 1, Ra=1e5, Pr=1, dt=1e-6",
+                    dict(nr=512, np=256, width=1.0, dt=1e-6, Ra=1e5, Pr=1.0), 20261203),
+}
+DEFAULT_WORKLOAD = "box-513x257"
+
+
+def gpu_leg(name, steps, warmup):
+    from codesthesis_dir_amd import BoxConfig, PeriodicBox
+    from codesthesis_dir_amd.ic import box_temperature
+    desc, kw, seed = WORKLOADS[name]
+    cfg = BoxConfig(rtol=1e-10, max_cycles=60, **kw)
+    box = PeriodicBox(cfg, device=0)
+    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, 1e-3, seed))
+    box.start()
+    if warmup:
+        box.step(warmup)
+    t0 = time.perf_counter()
+    info = box.step(steps)          # returns after the stream is idle
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3
+    ev_ms = box.last_step_ms()
+    # second pass with per-kernel events for the roofline
+    box.profile(True)
+    box.step(steps)
+    prof = box.profile_read()
+    box.profile(False)
+    nu = float(box.nusselt(info.trans1))
+    box.close()
+    return cfg, info, wall_ms, ev_ms, prof, nu
+
+
+def cpu_leg(name, budget_s=15.0):
+    """Time the reference CPU path on this host for a bounded sample."""
+    desc, kw, seed = WORKLOADS[name]
+    nr, np_ = kw["nr"], kw["np"]
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "rb_omp")
+    # the reference's static arrays hold nr <= 289, np <= 1609 (rb.f:54)
+    if os.path.exists(ref_bin) and nr <= 289 and np_ <= 1609 and kw["width"] == 3.0:
+        nts = 200
+        with tempfile.TemporaryDirectory() as d:
+            deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", str(kw["dt"]), str(nr),
+                              str(np_), str(nts), "0.5 1.0 1.0", str(kw["Ra"]), "0.0", "0.0",
+                              str(kw["Pr"]), "0.0", "0.0", "1", "no", "n"]) + "\n"
+            env = dict(os.environ, OMP_NUM_THREADS="2")
+            t0 = time.perf_counter()
+            subprocess.run([ref_bin], input=deck, text=True, cwd=d, env=env,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+            el = time.perf_counter() - t0
+        return {"value": nts / el, "unit": "time-steps/s", "cores": 2, "kind": "reference",
+                "sample": f"{nts} steps of the reference executable (flang -O2, 2 OpenMP threads)"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    run = O.Run(nr, np_, kw["dt"], kw["Ra"], kw["Pr"], width=kw["width"])
+    run.seed(r0=0.5, amp=1e-3, seed=seed)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        st, tr, ncl = run.step(mode=0, tolcon=2e-6)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 2000:
+            break
+    return {"value": n / el, "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps from the seeded start, C restatement of the reference time "
+                      f"loop (line-GS FMG Poisson, absolute stop 2e-6 as shipped), 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; the multi-GPU layout "
+              "is described in DESIGN.md and not implemented", file=sys.stderr)
+        sys.exit(2)
+    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup)
+    ms_per_step = wall_ms / a.steps
+    # dominant kernel by device time in the profiled pass
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_ms = d["ms"] / max(d["launches"], 1)
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    vcyc = {k: prof[k] for k in ("zebra", "restrict", "prolong", "norm")}
+    vc_ms = sum(v["ms"] for v in vcyc.values())
+    vc_bytes = sum(v["bytes"] for v in vcyc.values())
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{a.workload}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": a.steps / (wall_ms * 1e-3),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded linear-conduction temperature + U(-1e-3,1e-3) splitmix64 noise",
+        "config": {"workload": a.workload, "description": WORKLOADS[a.workload][0],
+                   "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
+                   "vcycle": f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_us": avg_ms * 1e3,
+                     "bytes_per_launch": d["bytes"] / max(d["launches"], 1)},
+        "vcycle_gbs": vc_bytes / (vc_ms * 1e-3) / 1e9 if vc_ms > 0 else None,
+        "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items() if v["launches"]},
+        "gpu_event_ms_per_step": ev_ms / a.steps,
+        "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
+        "nusselt_last_step": nu,
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_leg(a.workload)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -3,6 +3,6 @@
 The compute path is libvvmi.so (C ABI in include/vvmi.h); this package holds
 its build recipe and a thin host mirror of the reference routines.
 """
-from .velvor import BoxConfig, PeriodicBox  # noqa: F401
+from .velvor import AnnulusConfig, BoxConfig, PeriodicBox  # noqa: F401
 
-__all__ = ["BoxConfig", "PeriodicBox"]
+__all__ = ["AnnulusConfig", "BoxConfig", "PeriodicBox"]

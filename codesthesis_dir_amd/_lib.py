@@ -27,7 +27,10 @@ class VVConfig(ctypes.Structure):
                 ("r0", ctypes.c_double), ("rmax", ctypes.c_double), ("width", ctypes.c_double),
                 ("Ra", ctypes.c_double), ("Pr", ctypes.c_double), ("robert", ctypes.c_double),
                 ("rtol", ctypes.c_double), ("max_cycles", ctypes.c_int), ("nu1", ctypes.c_int),
-                ("nu2", ctypes.c_int), ("stats", ctypes.c_int)]
+                ("nu2", ctypes.c_int), ("stats", ctypes.c_int), ("geom", ctypes.c_int),
+                ("Re", ctypes.c_double)]
+
+VV_GEOM_BOX, VV_GEOM_ANNULUS = 0, 1
 
 
 class VVStepInfo(ctypes.Structure):

@@ -81,6 +81,49 @@ PtdmaTab make_ptdma(const std::vector<double> &a0, const std::vector<double> &b0
     return t;
 }
 
+int prof_begin(vv_run *r, hipEvent_t *a)
+{
+    *a = nullptr;
+    if (!r->prof_on) return VV_OK;
+    if (r->ev_used == r->ev_pool.size()) {
+        hipEvent_t e;
+        VV_HIP(hipEventCreate(&e));
+        r->ev_pool.push_back(e);
+    }
+    *a = r->ev_pool[r->ev_used++];
+    VV_HIP(hipEventRecord(*a, r->stream));
+    return VV_OK;
+}
+
+int prof_end(vv_run *r, int kid, hipEvent_t a, double bytes)
+{
+    if (!r->prof_on || !a) return VV_OK;
+    if (r->ev_used == r->ev_pool.size()) {
+        hipEvent_t e;
+        VV_HIP(hipEventCreate(&e));
+        r->ev_pool.push_back(e);
+    }
+    hipEvent_t b = r->ev_pool[r->ev_used++];
+    VV_HIP(hipEventRecord(b, r->stream));
+    r->prof_pending.push_back({kid, a, b, bytes});
+    return VV_OK;
+}
+
+// fold the recorded event pairs into the per-kernel totals (stream idle)
+static int prof_resolve(vv_run *r)
+{
+    for (const auto &p : r->prof_pending) {
+        float ms = 0.f;
+        VV_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        r->prof_launch[p.kid] += 1;
+        r->prof_ms[p.kid] += ms;
+        r->prof_bytes[p.kid] += p.bytes;
+    }
+    r->prof_pending.clear();
+    r->ev_used = 0;
+    return VV_OK;
+}
+
 }  // namespace vv
 
 using namespace vv;
@@ -226,7 +269,10 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         set_error("need nr >= 8 and np a power of two >= 8");
         return VV_EINVAL;
     }
-    if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->width > 0) || !(cfg->Pr > 0)) {
+    if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->Pr > 0) ||
+        (cfg->geom == VV_GEOM_BOX && !(cfg->width > 0)) ||
+        (cfg->geom == VV_GEOM_ANNULUS && (!(cfg->r0 > 0) || cfg->Re == 0.0)) ||
+        (cfg->geom != VV_GEOM_BOX && cfg->geom != VV_GEOM_ANNULUS)) {
         set_error("bad physical parameters");
         return VV_EINVAL;
     }
@@ -244,7 +290,12 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     r->ld = ((cfg->nr + 2 + 7) / 8) * 8;
     r->ncols = cfg->np + 1;
     r->dr = (cfg->rmax - cfg->r0) / cfg->nr;
-    r->dp = cfg->width / cfg->np;
+    if (cfg->geom == VV_GEOM_ANNULUS) {
+        const double pi = std::acos(-1.0);           // ann.f:121-124
+        r->dp = 2.0 * pi / (double)cfg->np;
+    } else {
+        r->dp = cfg->width / cfg->np;
+    }
     r->fbytes = (size_t)r->ld * r->ncols * sizeof(double);
     int st = VV_OK;
     auto fail = [&](int s) { vv_destroy(r); return s; };
@@ -257,6 +308,15 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     for (double **f : fields)
         if ((st = alloc_field(r, f))) return fail(st);
     if ((st = build_tables(r))) return fail(st);
+    {   // row radii exactly as the reference forms them (ann.f:918, 1331)
+        std::vector<double> rc(r->ld, 1.0), rt(r->ld, 1.0);
+        for (int i = 0; i <= r->nr + 1; ++i) {
+            rc[i] = cfg->r0 + (double)i * r->dr;
+            rt[i] = cfg->r0 + ((double)i - 0.5) * r->dr;
+        }
+        if ((st = upload_tab(r, &r->rC, rc))) return fail(st);
+        if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
+    }
     if (hipMalloc(&r->mg_flags, 4 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMalloc(&r->mg_bits, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMalloc(&r->mg_relres, 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
@@ -281,7 +341,7 @@ int vv_destroy(vv_run *r)
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                         r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
-                        r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->tab_r1, r->tab_rT,
+                        r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
                         r->tab_p, r->mg_pool, r->mg_relres, r->diag};
     for (double *f : fields)
         if (f) (void)hipFree(f);
@@ -290,6 +350,7 @@ int vv_destroy(vv_run *r)
     if (r->mg_bits) (void)hipFree(r->mg_bits);
     if (r->diag_i) (void)hipFree(r->diag_i);
     if (r->cfl_bits) (void)hipFree(r->cfl_bits);
+    for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -474,6 +535,10 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
         if ((st = launch_update(r, (int)(r->steps + 1)))) return st;
         rotate(r);
         r->steps += 1;
+        if (r->prof_on) {
+            VV_HIP(hipStreamSynchronize(r->stream));
+            if ((st = prof_resolve(r))) return st;
+        }
     }
     VV_HIP(hipEventRecord(r->ev1, r->stream));
     VV_HIP(hipEventSynchronize(r->ev1));
@@ -517,6 +582,33 @@ int vv_heat_history(vv_run *r, double *trans, int n)
                          hipMemcpyDeviceToHost));
     }
     return (int)cnt >= 0 ? VV_OK : VV_EINVAL;
+}
+
+int vv_profile(vv_run *r, int enable)
+{
+    if (!r) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    int st = prof_resolve(r);
+    if (st) return st;
+    if (enable) {
+        for (int k = 0; k < 16; ++k) { r->prof_launch[k] = 0; r->prof_ms[k] = 0; r->prof_bytes[k] = 0; }
+    }
+    r->prof_on = enable ? 1 : 0;
+    return VV_OK;
+}
+
+int vv_profile_read(vv_run *r, int kernel, long long *launches, double *ms, double *bytes)
+{
+    if (!r || kernel < 0 || kernel >= VV_K_COUNT) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    int st = prof_resolve(r);
+    if (st) return st;
+    if (launches) *launches = r->prof_launch[kernel];
+    if (ms) *ms = r->prof_ms[kernel];
+    if (bytes) *bytes = r->prof_bytes[kernel];
+    return VV_OK;
 }
 
 int vv_last_step_ms(vv_run *r, double *ms)

@@ -160,16 +160,21 @@ __global__ void __launch_bounds__(WG) k_compact(CompactArgs a)
 
 struct DFArgs {
     const double *C, *Cp, *th, *thp, *uel, *vel;
-    const double *dCr, *dCh, *dthhC, *dthrT, *dthhT;
+    const double *dCr, *dCh, *dthrC, *dthhC, *dthrT, *dthhT;
     double *Cn, *Cf, *thn, *thf;
-    int nr, np, ld, first, do_c, do_t;
-    double dri, dpi, dt, Pr, Ra, robert;
+    const double *rC, *rT;
+    int nr, np, ld, first, do_c, do_t, geom;
+    double dri, dpi, dt, Pr, Ra, robert, Re;
 };
 
 // DuFort-Frankel vorticity (rb.f:900-982, first step rb.f:699-775) and
 // temperature (rb.f:1292-1377, first step rb.f:1124-1204).  The Robert
 // filtered current level goes to Cf/thf (the reference filters in place,
 // rb.f:1041-1045 and 1435-1440, after all neighbours were read).
+// Annulus (geom 1, velvor2dnewsch3.4.f): r = r0+i*dr at vorticity rows
+// (ann.f:918) and r0+(i-1/2)*dr at temperature rows (ann.f:1331); RHS1
+// gains Cra/r and thra/r, the leapfrog term uses Cp, and RHS2 gains the
+// centrifugal buoyancy 2Ra(-thr u - thh v/r)/Re/Pr (ann.f:981-987, 1383).
 __global__ void __launch_bounds__(WG) k_df(DFArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,11 +185,13 @@ __global__ void __launch_bounds__(WG) k_df(DFArgs a)
     const int km = (k == 0) ? np - 1 : k - 1;
     const int ka = (k == np) ? 1 : k + 1;
     const double dri = a.dri, dpi = a.dpi, dt = a.dt, Ptl = a.Pr;
-    const double ri = 1.0;
     const size_t o = ix(i, k, ld);
+    const bool ann = (a.geom == 1);
     if (a.do_c) {
         const double Cc = a.C[o];
         if (i >= 1 && i <= nr - 1) {
+            const double r = ann ? a.rC[i] : 1.0;
+            const double ri = 1.0 / r;
             const double u = (a.uel[ix(i, ka, ld)] + a.uel[o]) * 0.5;
             const double v = (a.vel[o] + a.vel[ix(i + 1, k, ld)]) * 0.5;
             const double Chha = (a.C[ix(i, ka, ld)] + a.C[ix(i, km, ld)]) * (dpi * dpi);
@@ -194,16 +201,29 @@ __global__ void __launch_bounds__(WG) k_df(DFArgs a)
             const double Cr = a.dCr[o];
             const double thh = a.dthhC[o];
             const double cnh = (-1.0 * (dpi * dpi)) * (ri * ri);
-            double RHS1, cf1;
-            if (a.first) {
-                RHS1 = Crra + Chha * (ri * ri) + Cc / (Ptl * dt);
-                cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+            double RHS1, cf1, RHS2;
+            if (ann) {
+                const double Cra = (a.C[ix(i + 1, k, ld)] - a.C[ix(i - 1, k, ld)]) * 0.5 * dri;
+                const double thr = a.dthrC[o];
+                if (a.first) {
+                    RHS1 = Crra + Cra * ri + Chha * (ri * ri) + Cc / (Ptl * dt);
+                    cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+                } else {
+                    RHS1 = Crra + Cra * ri + Chha * (ri * ri) + 0.5 * a.Cp[o] / (Ptl * dt);
+                    cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                }
+                RHS2 = -(a.Ra * thh) + 2.0 * a.Ra * (-(thr * u) - thh * v * ri) / a.Re / Ptl;
             } else {
-                RHS1 = Crra + Chha * (ri * ri) + 0.5 * Cc / (Ptl * dt);
-                cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                if (a.first) {
+                    RHS1 = Crra + Chha * (ri * ri) + Cc / (Ptl * dt);
+                    cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+                } else {
+                    RHS1 = Crra + Chha * (ri * ri) + 0.5 * Cc / (Ptl * dt);
+                    cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                }
+                RHS2 = -(a.Ra * thh);
             }
             const double ULT = (u * Cr + v * Ch * ri) / Ptl;
-            const double RHS2 = -(a.Ra * thh);
             const double cf2 = cnr + cnh;
             const double cn = (cf2 * a.Cp[o] - ULT - RHS2 + RHS1) / cf1;
             a.Cn[o] = cn;
@@ -216,17 +236,22 @@ __global__ void __launch_bounds__(WG) k_df(DFArgs a)
     if (a.do_t) {
         const double T = a.th[o];
         if (i >= 1 && i <= nr) {
+            const double r = ann ? a.rT[i] : 1.0;
+            const double ri = 1.0 / r;
             const double u = (a.uel[o] + a.uel[ix(i - 1, k, ld)]) * 0.5;
             const double v = (a.vel[o] + a.vel[ix(i, km, ld)]) * 0.5;
-            double thrra, cnr;
+            double thrra, cnr, thra;
             if (i == 1) {
                 thrra = (3.2 * a.th[ix(0, k, ld)] + 2.0 * a.th[ix(2, k, ld)] - 0.2 * a.th[ix(3, k, ld)]) * (dri * dri);
+                thra = (-4.0 * a.th[ix(0, k, ld)] / 3.0 + a.th[ix(2, k, ld)] / 3.0) * dri;
                 cnr = -(2.5 * (dri * dri)) + 0.5 * dri * ri;
             } else if (i == nr) {
                 thrra = (3.2 * a.th[ix(nr + 1, k, ld)] + 2.0 * a.th[ix(nr - 1, k, ld)] - 0.2 * a.th[ix(nr - 2, k, ld)]) * (dri * dri);
+                thra = (-a.th[ix(nr - 1, k, ld)] / 3.0 + 4.0 * a.th[ix(nr + 1, k, ld)] / 3.0) * dri;
                 cnr = -(2.5 * (dri * dri)) - 0.5 * dri * ri;
             } else {
                 thrra = (a.th[ix(i - 1, k, ld)] + a.th[ix(i + 1, k, ld)]) * (dri * dri);
+                thra = (a.th[ix(i + 1, k, ld)] - a.th[ix(i - 1, k, ld)]) * 0.5 * dri;
                 cnr = -1.0 * (dri * dri);
             }
             const double thr = a.dthrT[o];
@@ -235,10 +260,12 @@ __global__ void __launch_bounds__(WG) k_df(DFArgs a)
             const double cnp = -(1.0 * (dpi * dpi)) * (ri * ri);
             double RHS1, cf1;
             if (a.first) {
-                RHS1 = thrra + thhha * (ri * ri) + T / dt;
+                RHS1 = ann ? thrra + thra * ri + thhha * (ri * ri) + T / dt
+                           : thrra + thhha * (ri * ri) + T / dt;
                 cf1 = 1.0 / dt - cnr - cnp;
             } else {
-                RHS1 = thrra + thhha * (ri * ri) + 0.5 * a.thp[o] / dt;
+                RHS1 = ann ? thrra + thra * ri + thhha * (ri * ri) + 0.5 * a.thp[o] / dt
+                           : thrra + thhha * (ri * ri) + 0.5 * a.thp[o] / dt;
                 cf1 = 0.5 / dt - cnr - cnp;
             }
             const double ULT = u * thr + v * thh * ri;
@@ -305,7 +332,8 @@ struct RhsArgs {
     unsigned long long *fnorm;     // [2]
     unsigned long long *cfl_bits;  // [2] reset here
     int *nclamp;                   // reset here
-    int nr, np, ld, which, from_fields;
+    const double *rC, *rT;
+    int nr, np, ld, which, from_fields, geom;
     double dr, dp;
 };
 
@@ -336,9 +364,17 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
         if (a.from_fields) {
             v = a.vrh[o];
         } else {
-            if (i == 1) v = a.Cn[ix(1, k, ld)] / a.dr;
-            else if (i == nr) v = -a.Cn[ix(nr - 1, k, ld)] / a.dr;
-            else v = (a.Cn[o] - a.Cn[ix(i - 1, k, ld)]) / a.dr;
+            if (a.geom == 1) {          // ann.f:1663-1670
+                const double r = a.rT[i];
+                const double Cr = (a.Cn[o] - a.Cn[ix(i - 1, k, ld)]) / a.dr;
+                if (i == 1) v = a.Cn[ix(1, k, ld)] / a.dr + a.Cn[ix(1, k, ld)] / r;
+                else if (i == nr) v = -a.Cn[ix(nr - 1, k, ld)] / a.dr + a.Cn[ix(nr - 1, k, ld)] / r;
+                else v = Cr + (a.Cn[o] + a.Cn[ix(i - 1, k, ld)]) / r;
+            } else {
+                if (i == 1) v = a.Cn[ix(1, k, ld)] / a.dr;
+                else if (i == nr) v = -a.Cn[ix(nr - 1, k, ld)] / a.dr;
+                else v = (a.Cn[o] - a.Cn[ix(i - 1, k, ld)]) / a.dr;
+            }
             a.vrh[o] = v;
         }
         if (k >= 1) {
@@ -355,7 +391,7 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
         } else {
             const int km = (k == 0) ? np - 1 : k - 1;
             const double Ch = (a.Cn[o] - a.Cn[ix(i, km, ld)]) / a.dp;
-            u = -Ch / 1.0;
+            u = -Ch / (a.geom == 1 ? a.rC[i] : 1.0);   // ann.f:1715-1719
             a.urh[o] = u;
         }
         if (k >= 1) {
@@ -370,6 +406,8 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
 struct UpdArgs {
     double *Cn, *thn;
     const double *vel, *uel;
+    const double *rT;
+    int geom;
     double *mT2, *mT, *mU2, *mV2;
     unsigned long long *cfl_bits;
     int *nclamp;
@@ -413,7 +451,7 @@ __global__ void __launch_bounds__(WG) k_update(UpdArgs a)
             atomicMax(&a.cfl_bits[0], (unsigned long long)__double_as_longlong(c));
         }
         if (i >= 1 && i <= nr - 1) {
-            const double c = fabs(V * a.dt / (1.0 * a.dp));
+            const double c = fabs(V * a.dt / ((a.geom == 1 ? a.rT[i] : 1.0) * a.dp));
             atomicMax(&a.cfl_bits[1], (unsigned long long)__double_as_longlong(c));
         }
     }
@@ -434,7 +472,11 @@ int launch_compact(vv_run *r)
     a.dri = 1.0 / r->dr; a.dpi = 1.0 / r->dp;
     const int len = (r->np + 1 > r->nr) ? r->np + 1 : r->nr;
     dim3 grid(nblk(len, 64), 6);
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_compact, grid, dim3(64), 0, r->stream, a);
+    // read C and th, write six derivative fields
+    prof_end(r, VV_K_COMPACT, pe_, 8.0 * 8.0 * (double)(r->nr + 2) * (r->np + 1));
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -459,21 +501,30 @@ int launch_df(vv_run *r, int first, int do_c, int do_t)
 {
     DFArgs a;
     a.C = r->C; a.Cp = r->Cp; a.th = r->th; a.thp = r->thp; a.uel = r->uel; a.vel = r->vel;
-    a.dCr = r->dCr; a.dCh = r->dCh; a.dthhC = r->dthhC; a.dthrT = r->dthrT; a.dthhT = r->dthhT;
+    a.dCr = r->dCr; a.dCh = r->dCh; a.dthrC = r->dthrC; a.dthhC = r->dthhC; a.dthrT = r->dthrT;
+    a.dthhT = r->dthhT;
     a.Cn = r->Cn; a.Cf = r->Cf; a.thn = r->thn; a.thf = r->thf;
+    a.rC = r->rC; a.rT = r->rT;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.first = first; a.do_c = do_c; a.do_t = do_t;
+    a.geom = r->cfg.geom;
     a.dri = 1.0 / r->dr; a.dpi = 1.0 / r->dp; a.dt = r->cfg.dt; a.Pr = r->cfg.Pr; a.Ra = r->cfg.Ra;
-    a.robert = r->cfg.robert;
+    a.robert = r->cfg.robert; a.Re = r->cfg.Re;
     const long long n = (long long)(r->nr + 2) * (r->np + 1);
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_df, dim3(nblk(n, WG)), dim3(WG), 0, r->stream, a);
+    prof_end(r, VV_K_DF, pe_, 8.0 * 15.0 * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
 int launch_heat(vv_run *r, double *dst2)
 {
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_heat, dim3(1), dim3(WG), 0, r->stream, (const double *)r->thf, r->nr,
                        r->np, r->ld, 1.0 / r->dr, r->dp, dst2);
+    prof_end(r, VV_K_HEAT, pe_, 8.0 * 10.0 * (r->np + 1));
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -499,8 +550,12 @@ static int launch_rhs_common(vv_run *r, int which, int from_fields)
     a.nclamp = r->diag_i;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.which = which;
     a.dr = r->dr; a.dp = r->dp;
+    a.rC = r->rC; a.rT = r->rT; a.geom = r->cfg.geom;
     const long long n = (long long)(r->nr + 2) * (r->np + 1);
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_rhs_guess, dim3(nblk(n, WG)), dim3(WG), 0, r->stream, a);
+    prof_end(r, VV_K_RHS, pe_, 8.0 * 11.0 * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -512,12 +567,16 @@ int launch_update(vv_run *r, int kt)
 {
     UpdArgs a;
     a.Cn = r->Cn; a.thn = r->thn; a.vel = r->vel; a.uel = r->uel;
+    a.rT = r->rT; a.geom = r->cfg.geom;
     a.mT2 = r->mT2; a.mT = r->mT; a.mU2 = r->mU2; a.mV2 = r->mV2;
     a.cfl_bits = r->cfl_bits; a.nclamp = r->diag_i;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.kt = kt; a.stats = r->cfg.stats;
     a.dr = r->dr; a.dp = r->dp; a.dt = r->cfg.dt;
     const long long n = (long long)(r->nr + 2) * (r->np + 1);
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_update, dim3(nblk(n, WG)), dim3(WG), 0, r->stream, a);
+    prof_end(r, VV_K_UPDATE, pe_, 8.0 * (r->cfg.stats ? 13.0 : 5.0) * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

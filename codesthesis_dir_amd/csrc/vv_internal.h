@@ -108,6 +108,9 @@ struct vv_run {
     double *dthrT = nullptr, *dthhT = nullptr;
     // running means (rb.f:409-417)
     double *mT2 = nullptr, *mT = nullptr, *mU2 = nullptr, *mV2 = nullptr;
+    // radii of the vorticity rows r0+i*dr and temperature rows r0+(i-1/2)*dr
+    // (annulus; unused in the box)
+    double *rC = nullptr, *rT = nullptr;
 
     // elimination tables (device)
     double *tab_r1 = nullptr;   // tdma 1-4-1, n = nr-1: fact | bp | c
@@ -132,9 +135,23 @@ struct vv_run {
     int last_cycles[2] = {0, 0};
     double last_relres[2] = {0, 0};
     float last_ms = 0.f;
+
+    // per-kernel event timing (vv_profile): pending event pairs of this
+    // vv_step call, resolved after its final synchronisation
+    int prof_on = 0;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct ProfRec { int kid; hipEvent_t a, b; double bytes; };
+    std::vector<ProfRec> prof_pending;
+    long long prof_launch[16] = {0};
+    double prof_ms[16] = {0};
+    double prof_bytes[16] = {0};
 };
 
 namespace vv {
+// kernel ids of the timing instrumentation (VV_K_* in include/vvmi.h)
+int prof_begin(vv_run *r, hipEvent_t *a);
+int prof_end(vv_run *r, int kid, hipEvent_t a, double bytes);
 // explicit-step launchers (vv_explicit.hip)
 int launch_compact(vv_run *r);
 int launch_df(vv_run *r, int first, int do_c, int do_t);

@@ -282,17 +282,29 @@ int mg_build(vv_run *r)
         MgProblemDev &P = r->mg.prob[p];
         P.n = n;
         P.nlev = nlev;
-        // r operator stencil (i-2..i+2) and phi row factor
+        // r operator stencil (i-2..i+2) and phi row factor: the level-1
+        // residual operators of relax2 (rb.f:2306-2357, ann.f:2301-2348) and
+        // relax3 (rb.f:2553-2587, ann.f:2555-2583)
         std::vector<double> s2(n, 0), s1(n, 0), dg(n, 0), p1(n, 0), p2(n, 0), rphi(n, 1.0);
+        const bool ann = (r->cfg.geom == VV_GEOM_ANNULUS);
+        const double hri = 1.0 / r->dr;
         for (int j = 0; j < n; ++j) {
+            const int i = j + 1;
+            double rvi = 1.0;
+            if (ann) rvi = (p == 0) ? 1.0 / (r->cfg.r0 + ((double)i - 0.5) * r->dr)
+                                    : 1.0 / ((double)i * r->dr + r->cfg.r0);
+            const double rvs = rvi * rvi;
+            rphi[j] = rvs;
+            dg[j] = -2.0 * hhri + (ann ? rvs : 0.0);
             if (p == 0 && j == 0) {
-                dg[j] = -2.0 * hhri; p1[j] = 5.0 / 3.0 * hhri; p2[j] = -0.2 * hhri;
+                p1[j] = 5.0 / 3.0 * hhri + (ann ? 4.0 / 3.0 * hri * rvi : 0.0);
+                p2[j] = -0.2 * hhri;
             } else if (p == 0 && j == n - 1) {
-                dg[j] = -2.0 * hhri; s1[j] = 5.0 / 3.0 * hhri; s2[j] = -0.2 * hhri;
+                s1[j] = 5.0 / 3.0 * hhri - (ann ? 4.0 / 3.0 * hri * rvi : 0.0);
+                s2[j] = -0.2 * hhri;
             } else {
-                dg[j] = -2.0 * hhri;
-                if (j > 0) s1[j] = hhri;
-                if (j < n - 1) p1[j] = hhri;
+                if (j > 0) s1[j] = hhri - (ann ? 1.5 * hri * rvi : 0.0);
+                if (j < n - 1) p1[j] = hhri + (ann ? 1.5 * hri * rvi : 0.0);
             }
         }
         double *sten_h = host.data() + off;
@@ -340,33 +352,62 @@ int mg_build(vv_run *r)
     return VV_OK;
 }
 
+// algorithmic bytes of one zebra half-sweep of level l over both problems:
+// read f and the other colour, write the own colour (the coarsest level:
+// read f, write u)
+static double zebra_bytes(const vv::MgBatchDev &B, int l)
+{
+    double b = 0.0;
+    for (int p = 0; p < B.nprob; ++p) {
+        const int n = B.prob[p].n, h = B.prob[p].lev[l].h, nc = B.prob[p].lev[l].ncol;
+        b += (nc == 1) ? 16.0 * n : 24.0 * (double)n * h;
+    }
+    return b;
+}
+
 static int enqueue_cycle(vv_run *r)
 {
     vv::MgBatchDev &B = r->mg;
     const vv::MgBatchDev *Bd = r->mg_dev;
     const int nlev = B.prob[0].nlev;
     const int n = r->nr;
+    const int ntot = B.prob[0].n + B.prob[1].n;
     const double rtol = r->cfg.rtol;
+    hipEvent_t pe;
+    auto zebra = [&](int l, int c) {
+        const int h = B.prob[0].lev[l].h;
+        const bool coarse = B.prob[0].lev[l].ncol == 1;
+        prof_begin(r, &pe);
+        hipLaunchKernelGGL(k_mg_zebra, dim3(coarse ? 1 : nblk(h, WG), 2), dim3(WG), 0, r->stream, Bd, l, c);
+        prof_end(r, VV_K_ZEBRA, pe, zebra_bytes(B, l));
+    };
     for (int l = 0; l < nlev - 1; ++l) {
-        const int h = B.prob[0].lev[l].h;
         for (int s = 0; s < r->cfg.nu1; ++s)
-            for (int c = 0; c < 2; ++c)
-                hipLaunchKernelGGL(k_mg_zebra, dim3(nblk(h, WG), 2), dim3(WG), 0, r->stream, Bd, l, c);
-        const long long cnt = (long long)n * B.prob[0].lev[l + 1].ncol;
+            for (int c = 0; c < 2; ++c) zebra(l, c);
+        const int ncf = B.prob[0].lev[l].ncol, ncc = B.prob[0].lev[l + 1].ncol;
+        const long long cnt = (long long)n * ncc;
+        prof_begin(r, &pe);
         hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, l);
+        prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncf + 2.0 * ncc));
     }
-    hipLaunchKernelGGL(k_mg_zebra, dim3(1, 2), dim3(WG), 0, r->stream, Bd, nlev - 1, 0);
+    zebra(nlev - 1, 0);
     for (int l = nlev - 2; l >= 0; --l) {
-        const long long cnt = (long long)n * B.prob[0].lev[l + 1].ncol;
+        const int ncf = B.prob[0].lev[l].ncol, ncc = B.prob[0].lev[l + 1].ncol;
+        const long long cnt = (long long)n * ncc;
+        prof_begin(r, &pe);
         hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, l);
-        const int h = B.prob[0].lev[l].h;
+        prof_end(r, VV_K_PROLONG, pe, 8.0 * ntot * (1.0 * ncc + 2.0 * ncf));
         for (int s = 0; s < r->cfg.nu2; ++s)
-            for (int c = 0; c < 2; ++c)
-                hipLaunchKernelGGL(k_mg_zebra, dim3(nblk(h, WG), 2), dim3(WG), 0, r->stream, Bd, l, c);
+            for (int c = 0; c < 2; ++c) zebra(l, c);
     }
-    const long long cnt0 = (long long)n * B.prob[0].lev[0].ncol;
+    const int nc0 = B.prob[0].lev[0].ncol;
+    const long long cnt0 = (long long)n * nc0;
+    prof_begin(r, &pe);
     hipLaunchKernelGGL(k_mg_norm, dim3(nblk(cnt0, 256), 2), dim3(256), 0, r->stream, Bd);
+    prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
+    prof_begin(r, &pe);
     hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, r->stream, Bd, rtol, r->cfg.max_cycles);
+    prof_end(r, VV_K_CHECK, pe, 0.0);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -434,8 +475,11 @@ int launch_mg_scatter_out(vv_run *r, int which)
 {
     const long long n = (long long)(r->nr + 2) * (r->np + 1);
     const int w = (which == 5) ? 3 : (which == 2 ? 1 : 2);
+    hipEvent_t pe;
+    prof_begin(r, &pe);
     hipLaunchKernelGGL(k_mg_out, dim3(nblk(n, 256)), dim3(256), 0, r->stream, r->mg_dev, r->vel, r->velp,
                        r->uel, r->uelp, r->nr, r->np, r->ld, w);
+    prof_end(r, VV_K_OUT, pe, 8.0 * 6.0 * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

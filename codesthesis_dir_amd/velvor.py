@@ -41,11 +41,20 @@ class BoxConfig:
     nu1: int = 2
     nu2: int = 2
     stats: int = 1
+    geom: int = 0          # 0: periodic box (rb.f); 1: rotating annulus plane (ann.f)
+    Re: float = 0.0        # rotational Reynolds number (annulus)
 
     def c(self) -> L.VVConfig:
         return L.VVConfig(self.nr, self.np, self.dt, self.r0, self.rmax, self.width, self.Ra,
                           self.Pr, self.robert, self.rtol, self.max_cycles, self.nu1, self.nu2,
-                          self.stats)
+                          self.stats, self.geom, self.Re)
+
+
+def AnnulusConfig(**kw) -> BoxConfig:
+    """Configuration of the rotating-annulus plane of velvor2dnewsch3.4.f:
+    dp = 2 pi/np, Robert coefficient 0.05 (ann.f:1055)."""
+    kw.setdefault("robert", 0.05)
+    return BoxConfig(geom=1, **kw)
 
 
 def _P(a: np.ndarray):
@@ -150,6 +159,28 @@ class PeriodicBox:
         L.check(L.lib().vv_last_step_ms(self._run, ctypes.byref(ms)), "vv_last_step_ms")
         return ms.value
 
+    KERNELS = ["compact", "df", "heat", "rhs", "zebra", "restrict", "prolong", "norm",
+               "check", "out", "update"]
+
+    def profile(self, enable: bool):
+        """Start (clearing totals) or stop per-kernel HIP event timing."""
+        L.check(L.lib().vv_profile(self._run, int(enable)), "vv_profile")
+
+    def profile_read(self) -> dict:
+        out = {}
+        for kid, name in enumerate(self.KERNELS):
+            n = ctypes.c_longlong()
+            ms = ctypes.c_double()
+            b = ctypes.c_double()
+            L.check(L.lib().vv_profile_read(self._run, kid, ctypes.byref(n), ctypes.byref(ms),
+                                            ctypes.byref(b)), "vv_profile_read")
+            out[name] = {"launches": n.value, "ms": ms.value, "bytes": b.value}
+        return out
+
     def nusselt(self, trans):
-        """Nu = trans*(rmax-r0)/width (rb.f:386-388)."""
+        """Nu = trans*(rmax-r0)/width in the box (rb.f:386-388);
+        -r0 log(r0/rmax) trans/(2 pi) in the annulus (ann.f:376-377)."""
+        if self.cfg.geom == 1:
+            c = self.cfg
+            return -c.r0 * np.log(c.r0 / c.rmax) * np.asarray(trans) / (2.0 * np.pi)
         return np.asarray(trans) * (self.cfg.rmax - self.cfg.r0) / self.cfg.width

@@ -47,18 +47,28 @@ enum vv_field {
 };
 
 /* Geometry and physics of one run: the reference's COMMON /num/ /val/
- * /radius/ (rb.f:72-75) plus the knobs it hard-codes. */
+ * /radius/ (rb.f:72-75) plus the knobs it hard-codes.
+ * geom = VV_GEOM_BOX: periodic Cartesian box of rb.f (r == 1 in all metric
+ *        terms, dp = width/np);
+ * geom = VV_GEOM_ANNULUS: rotating-annulus (r,phi) plane of
+ *        iarchomedir_.../velvor2dnewsch3.4.f ("ann.f"): r = r0 + i*dr,
+ *        cylindrical terms, centrifugal buoyancy 2Ra(..)/Re/Pr
+ *        (ann.f:981-987), dp = 2 pi/np (ann.f:124), width ignored. */
+#define VV_GEOM_BOX     0
+#define VV_GEOM_ANNULUS 1
 typedef struct vv_config {
     int    nr, np;        /* intervals across / along (np: power of two)   */
     double dt;
     double r0, rmax;      /* walls; dr = (rmax-r0)/nr                       */
     double width;         /* periodic length; dp = width/np (rb.f:132-133) */
     double Ra, Pr;
-    double robert;        /* Robert filter coefficient (0.02, rb.f:1043)   */
+    double robert;        /* Robert filter coefficient (0.02 rb.f:1043, 0.05 ann.f:1055) */
     double rtol;          /* Poisson stop: inf-norm residual <= rtol*|rhs| */
     int    max_cycles;    /* V-cycle cap per solve                          */
     int    nu1, nu2;      /* zebra line sweeps before/after coarse grid     */
     int    stats;         /* 1: running means + CFL every step (rb.f:409-458) */
+    int    geom;          /* VV_GEOM_BOX or VV_GEOM_ANNULUS                 */
+    double Re;            /* rotational Reynolds number (annulus only)     */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */
@@ -129,6 +139,18 @@ int vv_heat_history(vv_run *run, double *trans, int n);
 
 /* Timing of the last vv_step call on the run's stream (milliseconds). */
 int vv_last_step_ms(vv_run *run, double *ms);
+
+/* Per-kernel timing with HIP events on the run's stream (no counterpart in
+ * the reference; used by the benchmark for the roofline figure).
+ * vv_profile(run, 1) clears and starts collection, 0 stops it.
+ * vv_profile_read returns, for one kernel id, the launches, the summed
+ * device time (ms) and the summed algorithmic bytes of those launches. */
+enum vv_kernel_id {
+    VV_K_COMPACT = 0, VV_K_DF, VV_K_HEAT, VV_K_RHS, VV_K_ZEBRA, VV_K_RESTRICT,
+    VV_K_PROLONG, VV_K_NORM, VV_K_CHECK, VV_K_OUT, VV_K_UPDATE, VV_K_COUNT
+};
+int vv_profile(vv_run *run, int enable);
+int vv_profile_read(vv_run *run, int kernel, long long *launches, double *ms, double *bytes);
 
 #ifdef __cplusplus
 }

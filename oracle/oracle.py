@@ -31,6 +31,7 @@ def lib():
             build()
         L = ctypes.CDLL(LIB)
         L.vo_residual.restype = ctypes.c_double
+        L.vo_residual_g.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -43,7 +44,8 @@ def P(a: np.ndarray):
 class VoParams(ctypes.Structure):
     _fields_ = [("nr", ctypes.c_int), ("np", ctypes.c_int), ("ld", ctypes.c_int),
                 ("dt", ctypes.c_double), ("dr", ctypes.c_double), ("dp", ctypes.c_double),
-                ("Ra", ctypes.c_double), ("Pr", ctypes.c_double)]
+                ("Ra", ctypes.c_double), ("Pr", ctypes.c_double), ("geom", ctypes.c_int),
+                ("r0", ctypes.c_double), ("Re", ctypes.c_double), ("robert", ctypes.c_double)]
 
 
 _FIELD_NAMES = ["C", "Cp", "Cn", "th", "thp", "thn", "uel", "vel", "wel",
@@ -64,19 +66,29 @@ class VoMG(ctypes.Structure):
                 ("hhr", ctypes.c_double * MAXLEV), ("hhp", ctypes.c_double * MAXLEV),
                 ("v", _D * MAXLEV), ("f", _D * MAXLEV),
                 ("psif", _D), ("psic", _D), ("rc", _D), ("rf", _D), ("tempa", _D),
-                ("nlev_cap", ctypes.c_int)]
+                ("nlev_cap", ctypes.c_int), ("geom", ctypes.c_int), ("r0", ctypes.c_double),
+                ("nsweep", ctypes.c_int), ("zero_seam", ctypes.c_int)]
 
 
 class Run:
-    """One periodic-box run held by the oracle (mirror of the reference's
-    COMMON state, rb.f:69-76)."""
+    """One run held by the oracle (mirror of the reference's COMMON state,
+    rb.f:69-76).  geom 0: periodic box of velvor2dperiodicrbconv.f (width
+    given, dp = width/np); geom 1: rotating-annulus plane of
+    velvor2dnewsch3.4.f (dp = 2 pi/np, ann.f:124), with Re, Robert 0.05,
+    five multigrid levels at most and three sweeps per relaxation."""
 
-    def __init__(self, nr, np_, dt, Ra, Pr, width=3.0, height=0.5, ld=None, nlev_cap=4):
+    def __init__(self, nr, np_, dt, Ra, Pr, width=3.0, height=0.5, ld=None, nlev_cap=None,
+                 geom=0, r0=0.5, Re=0.0, robert=None):
         self.nr, self.np = nr, np_
+        self.geom = geom
         self.ld = ld if ld is not None else nr + 2
         dr = height / nr
-        dp = width / np_
-        self.p = VoParams(nr, np_, self.ld, dt, dr, dp, Ra, Pr)
+        dp = width / np_ if geom == 0 else 2.0 * np.arccos(-1.0) / np_
+        if robert is None:
+            robert = 0.02 if geom == 0 else 0.05
+        if nlev_cap is None:
+            nlev_cap = 4 if geom == 0 else 5
+        self.p = VoParams(nr, np_, self.ld, dt, dr, dp, Ra, Pr, geom, r0, Re, robert)
         self.arr = {n: np.zeros((np_ + 1, self.ld)) for n in _FIELD_NAMES}
         self.vrh = np.zeros((np_ + 1, self.ld))
         self.urh = np.zeros((np_ + 1, self.ld))
@@ -84,6 +96,8 @@ class Run:
         self.mg = VoMG()
         lib().vo_mg_init(ctypes.byref(self.mg), nr, np_, ctypes.c_double(dr),
                          ctypes.c_double(dp), nlev_cap)
+        if geom == 1:
+            lib().vo_mg_set_geom(ctypes.byref(self.mg), 1, ctypes.c_double(r0), 3, 0)
         self.kt = 0
 
     def __del__(self):
@@ -107,8 +121,11 @@ class Run:
             self.arr[a][...] = self.arr[b]
         self.kt = 0
 
-    def step(self, mode=1, tolcon=2e-6):
-        """mode 0: literal reference Poisson; mode 1: exact Poisson."""
+    def step(self, mode=1, tolcon=None):
+        """mode 0: literal reference Poisson (absolute stop tolcon: 2e-6 in
+        rb.f:2086, 1e-5 in ann.f:2093); mode 1: exact Poisson."""
+        if tolcon is None:
+            tolcon = 2e-6 if self.geom == 0 else 10e-6
         trans = np.zeros(2)
         ncl = ctypes.c_int(0)
         first = 1 if self.kt == 0 else 0
@@ -144,17 +161,17 @@ def pdma(a, b, c, d, e, r):
     return x
 
 
-def poisson_exact(which, psi, psip, rhs, nr, np_, dr, dp):
+def poisson_exact(which, psi, psip, rhs, nr, np_, dr, dp, geom=0, r0=0.0):
     ld = psi.shape[1]
-    lib().vo_poisson_exact(which, P(psi), P(psip), P(rhs), nr, np_, ld,
-                           ctypes.c_double(dr), ctypes.c_double(dp))
+    lib().vo_poisson_exact_g(which, P(psi), P(psip), P(rhs), nr, np_, ld,
+                             ctypes.c_double(dr), ctypes.c_double(dp), geom, ctypes.c_double(r0))
 
 
-def residual(which, psi, rhs, nr, np_, dr, dp, out=None):
+def residual(which, psi, rhs, nr, np_, dr, dp, out=None, geom=0, r0=0.0):
     ld = psi.shape[1]
     o = out if out is not None else np.zeros_like(psi)
-    m = lib().vo_residual(which, P(psi), P(rhs), P(o), nr, np_, ld,
-                          ctypes.c_double(dr), ctypes.c_double(dp))
+    m = lib().vo_residual_g(which, P(psi), P(rhs), P(o), nr, np_, ld,
+                            ctypes.c_double(dr), ctypes.c_double(dp), geom, ctypes.c_double(r0))
     return m, o
 
 

@@ -255,8 +255,11 @@ void vo_cvorticity(const vo_params *p, vo_fields *f, int first)
     compact_thrC(p, th, f->An, wa, wb, wc, wr, ws);
     compact_thhC(p, th, f->Bn, wa, wb, wc, wr, ws);
 
-    const double ri = 1.0;                 /* r = 1 in the Cartesian box */
+    const int ann = (p->geom == 1);
     for (int i = 1; i <= nr - 1; ++i) {
+        /* r = 1 in the box (rb.f:922); r = r0 + i*dr in the annulus (ann.f:918) */
+        const double r = ann ? p->r0 + (double)i * p->dr : 1.0;
+        const double ri = 1.0 / r;
         for (int k = 0; k <= np; ++k) {
             int ip = i + 1;
             int km = k - 1, ka = k + 1;
@@ -264,33 +267,48 @@ void vo_cvorticity(const vo_params *p, vo_fields *f, int first)
             if (k == np) ka = 1;
             double u = (uel[IX(i, ka)] + uel[IX(i, k)]) * 0.5;
             double v = (vel[IX(i, k)] + vel[IX(ip, k)]) * 0.5;
+            double thr = f->An[IX(i, k)];
             double Chha = (C[IX(i, ka)] + C[IX(i, km)]) * (dpi * dpi);
             double Ch = f->B[IX(i, k)];
             double Crra = (C[IX(i - 1, k)] + C[IX(i + 1, k)]) * (dri * dri);
             double cnr = -1.0 * (dri * dri);
+            double Cra = (C[IX(i + 1, k)] - C[IX(i - 1, k)]) * 0.5 * dri;
             double Cr = f->A[IX(i, k)];
             double thh = f->Bn[IX(i, k)];
             double cnh = (-1.0 * (dpi * dpi)) * (ri * ri);
-            double RHS1, cf1;
-            if (first) {
-                RHS1 = Crra + Chha * (ri * ri) + C[IX(i, k)] / (Ptl * dt);
-                cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+            double RHS1, cf1, RHS2;
+            if (ann) {
+                /* ann.f:981-987 (first step ann.f:734-740): Cra/r term, the
+                 * level Cp in the leapfrog term, centrifugal buoyancy */
+                if (first) {
+                    RHS1 = Crra + Cra * ri + Chha * (ri * ri) + C[IX(i, k)] / (Ptl * dt);
+                    cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+                } else {
+                    RHS1 = Crra + Cra * ri + Chha * (ri * ri) + 0.5 * Cp[IX(i, k)] / (Ptl * dt);
+                    cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                }
+                RHS2 = -(Ra * thh) + 2.0 * Ra * (-(thr * u) - thh * v * ri) / p->Re / Ptl;
             } else {
-                RHS1 = Crra + Chha * (ri * ri) + 0.5 * C[IX(i, k)] / (Ptl * dt);
-                cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                if (first) {
+                    RHS1 = Crra + Chha * (ri * ri) + C[IX(i, k)] / (Ptl * dt);
+                    cf1 = 1.0 / (Ptl * dt) - cnr - cnh;
+                } else {
+                    RHS1 = Crra + Chha * (ri * ri) + 0.5 * C[IX(i, k)] / (Ptl * dt);
+                    cf1 = 0.5 / (Ptl * dt) - cnr - cnh;
+                }
+                RHS2 = -(Ra * thh);
             }
             double ULT = (u * Cr + v * Ch * ri) / Ptl;
-            double RHS2 = -(Ra * thh);
             double cf2 = cnr + cnh;
             Cn[IX(i, k)] = (cf2 * Cp[IX(i, k)] - ULT - RHS2 + RHS1) / cf1;
             if (!first) f->Ap[IX(i, k)] = Cn[IX(i, k)];
         }
     }
     if (!first) {
-        /* Robert filter, rb.f:1041-1045 */
+        /* Robert filter, rb.f:1041-1045 (0.02; 0.05 in ann.f:1055) */
         for (int i = 1; i <= nr - 1; ++i)
             for (int k = 0; k <= np; ++k)
-                C[IX(i, k)] = C[IX(i, k)] + 0.02 * (Cn[IX(i, k)] - 2.0 * C[IX(i, k)] + Cp[IX(i, k)]);
+                C[IX(i, k)] = C[IX(i, k)] + p->robert * (Cn[IX(i, k)] - 2.0 * C[IX(i, k)] + Cp[IX(i, k)]);
     }
     free(w);
 }
@@ -310,8 +328,11 @@ void vo_temperature(const vo_params *p, vo_fields *f, int first)
     compact_thrT(p, th, f->A, wa, wb, wc, wr, ws);
     compact_Ch(p, th, 1, nr, f->B, wa, wb, wc, wr, ws);   /* thh, rb.f:1271-1289 */
 
-    const double ri = 1.0;
+    const int ann = (p->geom == 1);
     for (int i = 1; i <= nr; ++i) {
+        /* r = 1 in the box; r = r0 + (i-1/2)*dr in the annulus (ann.f:1331) */
+        const double r = ann ? p->r0 + ((double)i - 0.5) * p->dr : 1.0;
+        const double ri = 1.0 / r;
         for (int k = 0; k <= np; ++k) {
             int im = i - 1;
             int km = k - 1, ka = k + 1;
@@ -319,15 +340,18 @@ void vo_temperature(const vo_params *p, vo_fields *f, int first)
             if (k == np) ka = 1;
             double u = (uel[IX(i, k)] + uel[IX(im, k)]) * 0.5;
             double v = (vel[IX(i, k)] + vel[IX(i, km)]) * 0.5;
-            double thrra, cnr;
+            double thrra, cnr, thra;
             if (i == 1) {
                 thrra = (3.2 * th[IX(i - 1, k)] + 2.0 * th[IX(i + 1, k)] - 0.2 * th[IX(i + 2, k)]) * (dri * dri);
+                thra = (-4.0 * th[IX(i - 1, k)] / 3.0 + th[IX(i + 1, k)] / 3.0) * dri;
                 cnr = -(2.5 * (dri * dri)) + 0.5 * dri * ri;
             } else if (i == nr) {
                 thrra = (3.2 * th[IX(i + 1, k)] + 2.0 * th[IX(i - 1, k)] - 0.2 * th[IX(i - 2, k)]) * (dri * dri);
+                thra = (-th[IX(i - 1, k)] / 3.0 + 4.0 * th[IX(i + 1, k)] / 3.0) * dri;
                 cnr = -(2.5 * (dri * dri)) - 0.5 * dri * ri;
             } else {
                 thrra = (th[IX(i - 1, k)] + th[IX(i + 1, k)]) * (dri * dri);
+                thra = (th[IX(i + 1, k)] - th[IX(i - 1, k)]) * 0.5 * dri;
                 cnr = -1.0 * (dri * dri);
             }
             double thr = f->A[IX(i, k)];
@@ -336,10 +360,12 @@ void vo_temperature(const vo_params *p, vo_fields *f, int first)
             double cnp = -(1.0 * (dpi * dpi)) * (ri * ri);
             double RHS1, cf1;
             if (first) {
-                RHS1 = thrra + thhha * (ri * ri) + th[IX(i, k)] / dt;
+                if (ann) RHS1 = thrra + thra * ri + thhha * (ri * ri) + th[IX(i, k)] / dt;
+                else RHS1 = thrra + thhha * (ri * ri) + th[IX(i, k)] / dt;
                 cf1 = 1.0 / dt - cnr - cnp;
             } else {
-                RHS1 = thrra + thhha * (ri * ri) + 0.5 * thp[IX(i, k)] / dt;
+                if (ann) RHS1 = thrra + thra * ri + thhha * (ri * ri) + 0.5 * thp[IX(i, k)] / dt;
+                else RHS1 = thrra + thhha * (ri * ri) + 0.5 * thp[IX(i, k)] / dt;
                 cf1 = 0.5 / dt - cnr - cnp;
             }
             double ULT = u * thr + v * thh * ri;
@@ -352,7 +378,7 @@ void vo_temperature(const vo_params *p, vo_fields *f, int first)
         /* Robert filter, rb.f:1435-1440 */
         for (int i = 1; i <= nr; ++i)
             for (int k = 0; k <= np; ++k)
-                th[IX(i, k)] = th[IX(i, k)] + 0.02 * (thn[IX(i, k)] - 2.0 * th[IX(i, k)] + thp[IX(i, k)]);
+                th[IX(i, k)] = th[IX(i, k)] + p->robert * (thn[IX(i, k)] - 2.0 * th[IX(i, k)] + thp[IX(i, k)]);
     }
     free(w);
 }
@@ -389,13 +415,22 @@ void vo_rhsv(const vo_params *p, const double *Cn, double *vrh)
 {
     const int nr = p->nr, np = p->np, ld = p->ld;
     const double dr = p->dr;
-    for (int i = 1; i <= nr; ++i)
+    const int ann = (p->geom == 1);
+    for (int i = 1; i <= nr; ++i) {
+        const double r = ann ? p->r0 + ((double)i - 0.5) * dr : 1.0;
         for (int k = 0; k <= np; ++k) {
             double Cr = (Cn[IX(i, k)] - Cn[IX(i - 1, k)]) / dr;
-            if (i == 1) vrh[IX(i, k)] = Cn[IX(1, k)] / dr;
-            else if (i == nr) vrh[IX(i, k)] = -Cn[IX(nr - 1, k)] / dr;
-            else vrh[IX(i, k)] = Cr;
+            if (ann) {   /* ann.f:1663-1670 */
+                if (i == 1) vrh[IX(i, k)] = Cn[IX(1, k)] / dr + Cn[IX(1, k)] / r;
+                else if (i == nr) vrh[IX(i, k)] = -Cn[IX(nr - 1, k)] / dr + Cn[IX(nr - 1, k)] / r;
+                else vrh[IX(i, k)] = Cr + (Cn[IX(i, k)] + Cn[IX(i - 1, k)]) / r;
+            } else {
+                if (i == 1) vrh[IX(i, k)] = Cn[IX(1, k)] / dr;
+                else if (i == nr) vrh[IX(i, k)] = -Cn[IX(nr - 1, k)] / dr;
+                else vrh[IX(i, k)] = Cr;
+            }
         }
+    }
 }
 
 /* rb.f:1679-1721 */
@@ -403,13 +438,16 @@ void vo_rhsu(const vo_params *p, const double *Cn, double *urh)
 {
     const int nr = p->nr, np = p->np, ld = p->ld;
     const double dp = p->dp;
-    for (int i = 1; i <= nr - 1; ++i)
+    const int ann = (p->geom == 1);
+    for (int i = 1; i <= nr - 1; ++i) {
+        const double r = ann ? p->r0 + (double)i * p->dr : 1.0;   /* ann.f:1715 */
         for (int k = 0; k <= np; ++k) {
             int km = k - 1;
             if (k == 0) km = np - 1;
             double Ch = (Cn[IX(i, k)] - Cn[IX(i, km)]) / dp;
-            urh[IX(i, k)] = -Ch / 1.0;
+            urh[IX(i, k)] = -Ch / r;
         }
+    }
 }
 
 /* rb.f:1557-1626: wall vorticity, temperature walls, time-level rotation
@@ -460,6 +498,10 @@ int vo_mg_init(vo_mg *mg, int nr, int np, double dr, double dp, int nlev_cap)
     if (nlev > VO_MAXLEV) nlev = VO_MAXLEV;
     mg->nlev = nlev;
     mg->nlev_cap = nlev_cap;
+    mg->geom = 0;
+    mg->r0 = 0.0;
+    mg->nsweep = 1;
+    mg->zero_seam = 1;
     for (int l = 1; l <= nlev; ++l) {
         int p2 = 1 << l;
         mg->n1[l - 1] = nr * 2 / p2;
@@ -481,6 +523,14 @@ int vo_mg_init(vo_mg *mg, int nr, int np, double dr, double dp, int nlev_cap)
     mg->rf = (double *)calloc(sz, sizeof(double));
     mg->tempa = (double *)calloc(sz, sizeof(double));
     return nlev;
+}
+
+void vo_mg_set_geom(vo_mg *mg, int geom, double r0, int nsweep, int zero_seam)
+{
+    mg->geom = geom;
+    mg->r0 = r0;
+    mg->nsweep = nsweep;
+    mg->zero_seam = zero_seam;
 }
 
 void vo_mg_free(vo_mg *mg)
@@ -506,51 +556,95 @@ static void trans2(const vo_mg *mg, const double *a1, double *a2, int n1, int n3
             a2[IX(i, k)] = a1[(size_t)k * (n1 + 1) + i];
 }
 
-/* relax2 (rb.f:2063-2371) for which == 2, relax3 (rb.f:2374-2600) for 3.
- * klev is 1-based.  Returns nconv. */
+/* relax2 (rb.f:2063-2371; ann.f:2071-2359) for which == 2, relax3
+ * (rb.f:2374-2600; ann.f:2365-2591) for 3.  klev is 1-based.  In the
+ * annulus the operators carry the cylindrical terms with 1/r evaluated at the
+ * level's own cell (v) or face (u) radii, and every call does 3 sweeps
+ * (ann.f:2140, 2411).  Returns nconv. */
 static int relax(vo_mg *mg, int which, int nrel, int klev, double tolcon)
 {
     const int ld = mg->ld;
     const int L = klev - 1;
     const int n1 = mg->n1[L], n3 = mg->n3[L];
+    const int ann = (mg->geom == 1);
     double *psif = mg->psif, *rf = mg->rf, *rc = mg->rc;
     int nconv = 0;
     double resmax = 0.0;
     trans2(mg, mg->f[L], rf, n1, n3);
     const double hhri = 1.0 / mg->hhr[L];
+    const double hri = 1.0 / mg->hr[L];
     const double hhpi = 1.0 / mg->hhp[L];
     const int nn = n1 + 2;
-    double *w = (double *)malloc(sizeof(double) * 13 * nn);
+    double *w = (double *)malloc(sizeof(double) * 14 * nn);
     double *pa = w, *pb = w + nn, *pc = w + 2 * nn, *pd = w + 3 * nn, *pe = w + 4 * nn;
     double *ta = w + 5 * nn, *tb = w + 6 * nn, *tc = w + 7 * nn, *tr = w + 8 * nn;
     double *ts = w + 9 * nn, *pr = w + 10 * nn, *ps = w + 11 * nn, *rvs = w + 12 * nn;
+    double *rvi = w + 13 * nn;
+    const int nsw = ann ? mg->nsweep : nrel;
 
-    for (int irel = 1; irel <= nrel; ++irel) {
+    /* line coefficients (rb.f:2120-2156 at k=1; ann.f:2097-2126) */
+    if (which == 2) {
+        for (int i = 1; i <= n1; ++i) {
+            double rval = ann ? mg->r0 + ((double)i - 0.5) * mg->hr[L] : 1.0;
+            rvi[i] = 1.0 / rval;
+            rvs[i] = rvi[i] * rvi[i];
+            if (i == 1) {
+                pa[i] = 0.0; pb[i] = 0.0;
+                if (ann) {
+                    pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i] + rvs[i];
+                    pd[i] = 5.0 / 3.0 * hhri + 4.0 / 3.0 * hri * rvi[i];
+                } else {
+                    pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i];
+                    pd[i] = 5.0 / 3.0 * hhri;
+                }
+                pe[i] = -0.2 * hhri;
+            } else if (i == n1) {
+                pa[i] = -0.2 * hhri;
+                if (ann) {
+                    pb[i] = 5.0 / 3.0 * hhri - 4.0 / 3.0 * hri * rvi[i];
+                    pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i] + rvs[i];
+                } else {
+                    pb[i] = 5.0 / 3.0 * hhri;
+                    pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i];
+                }
+                pd[i] = 0.0; pe[i] = 0.0;
+            } else {
+                pa[i] = 0.0;
+                if (ann) {
+                    pb[i] = hhri - 1.5 * hri * rvi[i];
+                    pc[i] = -(2.0 * hhri + 2.0 * hhpi * rvs[i]) + rvs[i];
+                    pd[i] = hhri + 1.5 * hri * rvi[i];
+                } else {
+                    pb[i] = hhri;
+                    pc[i] = -(2.0 * hhri + 2.0 * hhpi * rvs[i]);
+                    pd[i] = hhri;
+                }
+                pe[i] = 0.0;
+            }
+        }
+    } else {
+        for (int i = 1; i <= n1 - 1; ++i) {
+            double rval = ann ? (double)i * mg->hr[L] + mg->r0 : 1.0;
+            rvi[i] = 1.0 / rval;
+            rvs[i] = rvi[i] * rvi[i];
+            if (ann) {
+                pa[i] = hhri - 1.5 * hri * rvi[i];
+                pb[i] = -(2.0 * hhri - rvs[i] + 2.0 * hhpi * rvs[i]);
+                pc[i] = hhri + 1.5 * hri * rvi[i];
+            } else {
+                pa[i] = hhri;
+                pb[i] = -(2.0 * hhri + 2.0 * hhpi * rvs[i]);
+                pc[i] = hhri;
+            }
+        }
+    }
+
+    for (int irel = 1; irel <= nsw; ++irel) {
         if (which == 2) {
             for (int k = 1; k <= n3; ++k) {
                 int ka = k + 1, km = k - 1;
                 if (k == n3) ka = 1;
                 for (int i = 1; i <= n1; ++i) {
-                    if (k == 1) {
-                        rvs[i] = 1.0;   /* rval = 1: rvali**2. */
-                        if (i == 1) {
-                            pa[i] = 0.0; pb[i] = 0.0;
-                            pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i];
-                            pd[i] = 5.0 / 3.0 * hhri;
-                            pe[i] = -0.2 * hhri;
-                        } else if (i == n1) {
-                            pa[i] = -0.2 * hhri;
-                            pb[i] = 5.0 / 3.0 * hhri;
-                            pc[i] = -2.0 * hhri - 2.0 * hhpi * rvs[i];
-                            pd[i] = 0.0; pe[i] = 0.0;
-                        } else {
-                            pa[i] = 0.0;
-                            pb[i] = hhri;
-                            pc[i] = -(2.0 * hhri + 2.0 * hhpi * rvs[i]);
-                            pd[i] = hhri;
-                            pe[i] = 0.0;
-                        }
-                    }
                     ta[i] = pa[i]; tb[i] = pb[i]; tc[i] = pc[i]; tr[i] = pd[i]; ts[i] = pe[i];
                     pr[i] = -(psif[IX(i, ka)] + psif[IX(i, km)]) * hhpi * rvs[i] + rf[IX(i, k)];
                 }
@@ -566,14 +660,19 @@ static int relax(vo_mg *mg, int which, int nrel, int klev, double tolcon)
                     int ka = k + 1, km = k - 1;
                     if (k == n3) ka = 1;
                     double temp = (psif[IX(i, ka)] - 2.0 * psif[IX(i, k)] + psif[IX(i, km)]) * hhpi * rvs[i];
-                    if (i == 1)
+                    if (ann) temp = temp + psif[IX(i, k)] * rvs[i];
+                    if (i == 1) {
                         temp = temp + (-2.0 * psif[IX(i, k)] + 5.0 / 3.0 * psif[IX(i + 1, k)]
                                        - 0.2 * psif[IX(i + 2, k)]) * hhri;
-                    else if (i == n1)
+                        if (ann) temp = temp + 4.0 / 3.0 * psif[IX(i + 1, k)] * hri * rvi[i];
+                    } else if (i == n1) {
                         temp = temp + (-2.0 * psif[IX(i, k)] + 5.0 / 3.0 * psif[IX(i - 1, k)]
                                        - 0.2 * psif[IX(i - 2, k)]) * hhri;
-                    else
+                        if (ann) temp = temp - 4.0 / 3.0 * psif[IX(i - 1, k)] * hri * rvi[i];
+                    } else {
                         temp = temp + (psif[IX(i - 1, k)] - 2.0 * psif[IX(i, k)] + psif[IX(i + 1, k)]) * hhri;
+                        if (ann) temp = temp + 1.5 * (psif[IX(i + 1, k)] - psif[IX(i - 1, k)]) * hri * rvi[i];
+                    }
                     temp = rf[IX(i, k)] - temp;
                     rc[IX(i, k)] = temp;
                     temp = fabs(temp);
@@ -584,12 +683,6 @@ static int relax(vo_mg *mg, int which, int nrel, int klev, double tolcon)
                 int ka = k + 1, km = k - 1;
                 if (k == n3) ka = 1;
                 for (int i = 1; i <= n1 - 1; ++i) {
-                    if (k == 1) {
-                        rvs[i] = 1.0;
-                        pa[i] = hhri;
-                        pb[i] = -(2.0 * hhri + 2.0 * hhpi * rvs[i]);
-                        pc[i] = hhri;
-                    }
                     ta[i] = pa[i]; tb[i] = pb[i]; tc[i] = pc[i];
                     tr[i] = -(psif[IX(i, ka)] + psif[IX(i, km)]) * hhpi * rvs[i] + rf[IX(i, k)];
                 }
@@ -604,9 +697,17 @@ static int relax(vo_mg *mg, int which, int nrel, int klev, double tolcon)
                 for (int i = 1; i <= n1 - 1; ++i) {
                     int ka = k + 1, km = k - 1;
                     if (k == n3) ka = 1;
-                    double temp = (psif[IX(i + 1, k)] + psif[IX(i - 1, k)]) * hhri
-                                + (psif[IX(i, ka)] + psif[IX(i, km)]) * hhpi * rvs[i]
-                                - psif[IX(i, k)] * (2.0 * hhri + 2.0 * hhpi * rvs[i]);
+                    double temp;
+                    if (ann)
+                        temp = (psif[IX(i + 1, k)] + psif[IX(i - 1, k)]) * hhri
+                             + 1.5 * (psif[IX(i + 1, k)] - psif[IX(i - 1, k)]) * hri * rvi[i]
+                             + psif[IX(i, k)] * rvs[i]
+                             + (psif[IX(i, ka)] + psif[IX(i, km)]) * hhpi * rvs[i]
+                             - psif[IX(i, k)] * (2.0 * hhri + 2.0 * hhpi * rvs[i]);
+                    else
+                        temp = (psif[IX(i + 1, k)] + psif[IX(i - 1, k)]) * hhri
+                             + (psif[IX(i, ka)] + psif[IX(i, km)]) * hhpi * rvs[i]
+                             - psif[IX(i, k)] * (2.0 * hhri + 2.0 * hhpi * rvs[i]);
                     temp = rf[IX(i, k)] - temp;
                     rc[IX(i, k)] = temp;
                     temp = fabs(temp);
@@ -661,7 +762,7 @@ static void corser(vo_mg *mg, int klev)
         psif[IX(0, k)] = 0.0; psif[IX(n1c, k)] = 0.0;
     }
     for (int i = 0; i <= n1c; ++i) {
-        rc[IX(i, 0)] = 0.0; rc[IX(i, n3c)] = 0.0;
+        if (mg->zero_seam) { rc[IX(i, 0)] = 0.0; rc[IX(i, n3c)] = 0.0; }
         psic[IX(i, 0)] = 0.0; psic[IX(i, n3c)] = 0.0;
         psif[IX(i, 0)] = 0.0; psif[IX(i, n3c)] = 0.0;
     }
@@ -782,33 +883,62 @@ int vo_poisson_ref(vo_mg *mg, int which, double *psi, double *psip,
 
 /* ----------------------------------------------------- exact Poisson */
 
-/* Residual of the operator whose discrete solution poisson2/poisson3
- * converge to: relax2's residual (rb.f:2306-2357) and relax3's
- * (rb.f:2553-2587) at level 1, unknowns k = 1..np (k = 0 == np). */
-double vo_residual(int which, const double *psi, const double *rhs,
-                   double *out, int nr, int np, int ld, double dr, double dp)
+/* Row coefficients of the r part of the converged operators (the level-1
+ * residuals of relax2/relax3): row i couples psi(i-2..i+2) with s[0..4];
+ * rv2 is the factor of the phi second difference (1/r^2, 1 in the box). */
+static void op_row(int which, int geom, int i, int nr, double dr, double r0,
+                   double *s, double *rv2)
 {
-    const double hhri = 1.0 / (dr * dr), hhpi = 1.0 / (dp * dp);
+    const double hhri = 1.0 / (dr * dr), hri = 1.0 / dr;
+    for (int o = 0; o < 5; ++o) s[o] = 0.0;
+    const int ann = (geom == 1);
+    if (which == 2) {
+        const double rvi = ann ? 1.0 / (r0 + ((double)i - 0.5) * dr) : 1.0;
+        const double rvs = rvi * rvi;
+        *rv2 = rvs;
+        if (ann) s[2] += rvs;
+        if (i == 1) {
+            s[2] += -2.0 * hhri; s[3] += 5.0 / 3.0 * hhri; s[4] += -0.2 * hhri;
+            if (ann) s[3] += 4.0 / 3.0 * hri * rvi;
+        } else if (i == nr) {
+            s[2] += -2.0 * hhri; s[1] += 5.0 / 3.0 * hhri; s[0] += -0.2 * hhri;
+            if (ann) s[1] -= 4.0 / 3.0 * hri * rvi;
+        } else {
+            s[1] += hhri; s[2] += -2.0 * hhri; s[3] += hhri;
+            if (ann) { s[3] += 1.5 * hri * rvi; s[1] -= 1.5 * hri * rvi; }
+        }
+    } else {
+        const double rvi = ann ? 1.0 / ((double)i * dr + r0) : 1.0;
+        const double rvs = rvi * rvi;
+        *rv2 = rvs;
+        s[1] = hhri; s[2] = -2.0 * hhri; s[3] = hhri;
+        if (ann) { s[2] += rvs; s[3] += 1.5 * hri * rvi; s[1] -= 1.5 * hri * rvi; }
+        if (i == 1) s[1] = 0.0;
+        if (i == nr - 1) s[3] = 0.0;
+    }
+}
+
+/* Residual of the operator whose discrete solution poisson2/poisson3
+ * converge to: relax2's residual (rb.f:2306-2357, ann.f:2301-2348) and
+ * relax3's (rb.f:2553-2587, ann.f:2555-2583) at level 1, unknowns
+ * k = 1..np (k = 0 == np). */
+double vo_residual_g(int which, const double *psi, const double *rhs, double *out,
+                     int nr, int np, int ld, double dr, double dp, int geom, double r0)
+{
+    const double hhpi = 1.0 / (dp * dp);
     double rmax = 0.0;
-    int ihi = (which == 2) ? nr : nr - 1;
+    const int ihi = (which == 2) ? nr : nr - 1;
     for (int k = 1; k <= np; ++k) {
         int ka = (k == np) ? 1 : k + 1, km = k - 1;
         for (int i = 1; i <= ihi; ++i) {
+            double sr[5], rv2;
+            op_row(which, geom, i, nr, dr, r0, sr, &rv2);
             double pkm = (km == 0) ? psi[IX(i, np)] : psi[IX(i, km)];
-            double t;
-            if (which == 2) {
-                t = (psi[IX(i, ka)] - 2.0 * psi[IX(i, k)] + pkm) * hhpi;
-                if (i == 1)
-                    t += (-2.0 * psi[IX(1, k)] + 5.0 / 3.0 * psi[IX(2, k)] - 0.2 * psi[IX(3, k)]) * hhri;
-                else if (i == nr)
-                    t += (-2.0 * psi[IX(nr, k)] + 5.0 / 3.0 * psi[IX(nr - 1, k)] - 0.2 * psi[IX(nr - 2, k)]) * hhri;
-                else
-                    t += (psi[IX(i - 1, k)] - 2.0 * psi[IX(i, k)] + psi[IX(i + 1, k)]) * hhri;
-            } else {
-                double pim = (i == 1) ? 0.0 : psi[IX(i - 1, k)];
-                double pip = (i == nr - 1) ? 0.0 : psi[IX(i + 1, k)];
-                t = (pip + pim) * hhri + (psi[IX(i, ka)] + pkm) * hhpi
-                  - psi[IX(i, k)] * (2.0 * hhri + 2.0 * hhpi);
+            double t = (psi[IX(i, ka)] - 2.0 * psi[IX(i, k)] + pkm) * hhpi * rv2;
+            for (int o = -2; o <= 2; ++o) {
+                int ii = i + o;
+                if (ii < 1 || ii > ihi || sr[o + 2] == 0.0) continue;
+                t += sr[o + 2] * psi[IX(ii, k)];
             }
             double r = rhs[IX(i, k)] - t;
             if (out) out[IX(i, k)] = r;
@@ -818,25 +948,29 @@ double vo_residual(int which, const double *psi, const double *rhs,
     return rmax;
 }
 
-/* Separable exact solve: the operator is D_phi (x) I + I (x) D_r with D_phi
- * the periodic 3-point second difference on np points.  A real DFT in phi
- * diagonalises D_phi (eigenvalue -(2-2cos(2 pi m/np))/dp^2); each mode is a
- * banded (bandwidth 2) system in r solved by elimination without pivoting
- * (row diagonally dominant). */
-int vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
-                     int nr, int np, int ld, double dr, double dp)
+double vo_residual(int which, const double *psi, const double *rhs,
+                   double *out, int nr, int np, int ld, double dr, double dp)
+{
+    return vo_residual_g(which, psi, rhs, out, nr, np, ld, dr, dp, 0, 0.0);
+}
+
+/* Separable exact solve: the operator is D_r + diag(rv2) (x) D_phi with
+ * D_phi the periodic 3-point second difference on np points.  A real DFT in
+ * phi diagonalises D_phi (eigenvalue -(2-2cos(2 pi m/np))/dp^2); each mode
+ * is a banded (bandwidth 2) system in r solved by elimination without
+ * pivoting (row diagonally dominant). */
+int vo_poisson_exact_g(int which, double *psi, double *psip, const double *rhs,
+                       int nr, int np, int ld, double dr, double dp, int geom, double r0)
 {
     const int n = (which == 2) ? nr : nr - 1;
-    const double hhri = 1.0 / (dr * dr), hhpi = 1.0 / (dp * dp);
+    const double hhpi = 1.0 / (dp * dp);
     const double twopi = 6.283185307179586476925286766559;
     const int nm = np / 2 + 1;
     double *cs = (double *)malloc(sizeof(double) * np);
     double *sn = (double *)malloc(sizeof(double) * np);
     for (int j = 0; j < np; ++j) { cs[j] = cos(twopi * j / np); sn[j] = sin(twopi * j / np); }
-    /* spectral coefficients: fc[m][i], fs[m][i] */
     double *fc = (double *)calloc((size_t)nm * (n + 1), sizeof(double));
     double *fsn = (double *)calloc((size_t)nm * (n + 1), sizeof(double));
-    /* unknown column index q = 0..np-1 corresponds to k = q+1 */
     for (int m = 0; m < nm; ++m)
         for (int i = 1; i <= n; ++i) {
             long double sc = 0.0L, ss = 0.0L;
@@ -849,7 +983,6 @@ int vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
             fc[(size_t)m * (n + 1) + i] = (double)sc;
             fsn[(size_t)m * (n + 1) + i] = (double)ss;
         }
-    /* band matrix rows i=1..n: columns i-2..i+2 */
     double *bm = (double *)malloc(sizeof(double) * 5 * (n + 3));
     double *x = (double *)malloc(sizeof(double) * (n + 3));
     double *y = (double *)malloc(sizeof(double) * (n + 3));
@@ -861,19 +994,13 @@ int vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
 #define BM(i, o) bm[(size_t)(i) * 5 + (o) + 2]
             for (int i = 0; i <= n + 2; ++i) for (int o = -2; o <= 2; ++o) BM(i, o) = 0.0;
             for (int i = 1; i <= n; ++i) {
-                if (which == 2) {
-                    if (i == 1) { BM(1, 0) = -2.0 * hhri; BM(1, 1) = 5.0 / 3.0 * hhri; BM(1, 2) = -0.2 * hhri; }
-                    else if (i == n) { BM(n, 0) = -2.0 * hhri; BM(n, -1) = 5.0 / 3.0 * hhri; BM(n, -2) = -0.2 * hhri; }
-                    else { BM(i, -1) = hhri; BM(i, 0) = -2.0 * hhri; BM(i, 1) = hhri; }
-                } else {
-                    BM(i, 0) = -2.0 * hhri;
-                    if (i > 1) BM(i, -1) = hhri;
-                    if (i < n) BM(i, 1) = hhri;
-                }
-                BM(i, 0) += lam;
+                double sr[5], rv2;
+                op_row(which, geom, i, nr, dr, r0, sr, &rv2);
+                for (int o = -2; o <= 2; ++o)
+                    if (i + o >= 1 && i + o <= n) BM(i, o) = sr[o + 2];
+                BM(i, 0) += lam * rv2;
                 y[i] = rv[(size_t)m * (n + 1) + i];
             }
-            /* forward elimination, bandwidth 2 */
             for (int i = 1; i <= n; ++i) {
                 for (int rr = i + 1; rr <= i + 2 && rr <= n; ++rr) {
                     double a = BM(rr, i - rr);
@@ -885,34 +1012,39 @@ int vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
                 }
             }
             for (int i = n; i >= 1; --i) {
-                double s = y[i];
-                for (int o = 1; o <= 2 && i + o <= n; ++o) s -= BM(i, o) * x[i + o];
-                x[i] = s / BM(i, 0);
+                double sacc = y[i];
+                for (int o = 1; o <= 2 && i + o <= n; ++o) sacc -= BM(i, o) * x[i + o];
+                x[i] = sacc / BM(i, 0);
             }
             for (int i = 1; i <= n; ++i) rv[(size_t)m * (n + 1) + i] = x[i];
 #undef BM
         }
     }
-    /* inverse real DFT */
     for (int kk = 0; kk <= np; ++kk)
         for (int i = 0; i <= ((which == 2) ? nr + 1 : nr); ++i)
             psip[IX(i, kk)] = psi[IX(i, kk)];
     for (int i = 1; i <= n; ++i)
         for (int q = 0; q < np; ++q) {
-            long double s = 0.0L;
+            long double sacc = 0.0L;
             for (int m = 0; m < nm; ++m) {
                 int j = (int)(((long)m * q) % np);
                 double wgt = (m == 0 || 2 * m == np) ? 1.0 : 2.0;
-                s += (long double)wgt * (fc[(size_t)m * (n + 1) + i] * cs[j]
-                                         + fsn[(size_t)m * (n + 1) + i] * sn[j]);
+                sacc += (long double)wgt * (fc[(size_t)m * (n + 1) + i] * cs[j]
+                                            + fsn[(size_t)m * (n + 1) + i] * sn[j]);
             }
-            psi[IX(i, q + 1)] = (double)(s / np);
+            psi[IX(i, q + 1)] = (double)(sacc / np);
         }
     for (int i = 1; i <= n; ++i) psi[IX(i, 0)] = psi[IX(i, np)];
     int iwall = (which == 2) ? nr + 1 : nr;
     for (int kk = 0; kk <= np; ++kk) { psi[IX(0, kk)] = 0.0; psi[IX(iwall, kk)] = 0.0; }
     free(cs); free(sn); free(fc); free(fsn); free(bm); free(x); free(y);
     return 0;
+}
+
+int vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
+                     int nr, int np, int ld, double dr, double dp)
+{
+    return vo_poisson_exact_g(which, psi, psip, rhs, nr, np, ld, dr, dp, 0, 0.0);
 }
 
 /* ------------------------------------------------------------ one step */
@@ -926,11 +1058,14 @@ int vo_step(const vo_params *p, vo_fields *f, vo_mg *mg, double *vrh,
     vo_temperature(p, f, first);
     vo_heat(p, f->th, &trans[0], &trans[1]);
     vo_rhsv(p, f->Cn, vrh);
-    if (mode == 0) st |= vo_poisson_ref(mg, 2, f->vel, f->velp, vrh, p->nr, p->np, tolcon, 500, &it);
-    else st |= vo_poisson_exact(2, f->vel, f->velp, vrh, p->nr, p->np, p->ld, p->dr, p->dp);
+    /* rb.f stops after 500 outer iterations (rb.f:1843); ann.f runs up to
+     * nvs = 10000 (ann.f:1821) */
+    const int maxit = (p->geom == 1) ? 10000 : 500;
+    if (mode == 0) st |= vo_poisson_ref(mg, 2, f->vel, f->velp, vrh, p->nr, p->np, tolcon, maxit, &it);
+    else st |= vo_poisson_exact_g(2, f->vel, f->velp, vrh, p->nr, p->np, p->ld, p->dr, p->dp, p->geom, p->r0);
     vo_rhsu(p, f->Cn, urh);
-    if (mode == 0) st |= vo_poisson_ref(mg, 3, f->uel, f->uelp, urh, p->nr, p->np, tolcon, 500, &it);
-    else st |= vo_poisson_exact(3, f->uel, f->uelp, urh, p->nr, p->np, p->ld, p->dr, p->dp);
+    if (mode == 0) st |= vo_poisson_ref(mg, 3, f->uel, f->uelp, urh, p->nr, p->np, tolcon, maxit, &it);
+    else st |= vo_poisson_exact_g(3, f->uel, f->uelp, urh, p->nr, p->np, p->ld, p->dr, p->dp, p->geom, p->r0);
     int nc = vo_update(p, f);
     if (nclamp) *nclamp = nc;
     return st;

@@ -27,6 +27,12 @@ typedef struct vo_params {
     int nr, np, ld;
     double dt, dr, dp;     /* dp = box width / np (rb.f:130-133)          */
     double Ra, Pr;
+    /* geometry: 0 = Cartesian periodic box (rb.f, r == 1),
+     *           1 = rotating annulus plane (velvor2dnewsch3.4.f, "ann.f"):
+     *               r = r0 + i*dr at vorticity rows, cylindrical terms,
+     *               centrifugal buoyancy with Re, Robert 0.05 */
+    int geom;
+    double r0, Re, robert;
 } vo_params;
 
 /* State of one run: the reference's COMMON /sol1/ /sol2/ /sol3/ (rb.f:69-76).
@@ -49,7 +55,11 @@ typedef struct vo_mg {
     double hr[VO_MAXLEV], hp[VO_MAXLEV], hhr[VO_MAXLEV], hhp[VO_MAXLEV];
     double *v[VO_MAXLEV], *f[VO_MAXLEV];
     double *psif, *psic, *rc, *rf, *tempa;
-    int nlev_cap;          /* 4 in the reference (rb.f:1766) */
+    int nlev_cap;          /* 4 in rb.f (rb.f:1766), 5 in ann.f (ann.f:1778) */
+    int geom;              /* 0 box, 1 annulus */
+    double r0;
+    int nsweep;            /* sweeps per relax: nrel (rb.f) or 3 (ann.f:2140) */
+    int zero_seam;         /* corser zeroes rc at k=0,n3 (rb.f:2688-2689) */
 } vo_mg;
 
 /* banded solvers, 1-based semantics on 0-based arrays of length n */
@@ -68,6 +78,8 @@ int  vo_update(const vo_params *p, vo_fields *f);
 
 /* Poisson */
 int  vo_mg_init(vo_mg *mg, int nr, int np, double dr, double dp, int nlev_cap);
+/* geometry of the Poisson operators (set after vo_mg_init) */
+void vo_mg_set_geom(vo_mg *mg, int geom, double r0, int nsweep, int zero_seam);
 void vo_mg_free(vo_mg *mg);
 /* literal FMG/line-GS of the reference; which = 2 (v, cell rows 1..nr) or
  * 3 (u, face rows 1..nr-1).  tolcon is the absolute max-norm stop
@@ -84,6 +96,11 @@ int  vo_poisson_exact(int which, double *psi, double *psip, const double *rhs,
  * returns max |out| */
 double vo_residual(int which, const double *psi, const double *rhs,
                    double *out, int nr, int np, int ld, double dr, double dp);
+/* the same two routines for either geometry (r0 used when geom == 1) */
+int  vo_poisson_exact_g(int which, double *psi, double *psip, const double *rhs,
+                        int nr, int np, int ld, double dr, double dp, int geom, double r0);
+double vo_residual_g(int which, const double *psi, const double *rhs, double *out,
+                     int nr, int np, int ld, double dr, double dp, int geom, double r0);
 
 /* one full time step of the reference time loop (rb.f:341-382):
  * mode 0 = literal reference Poisson (tolcon absolute),

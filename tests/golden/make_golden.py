@@ -15,6 +15,11 @@ Every fixture holds inputs and the reference's outputs only (numpy .npz):
                      the wall fluxes trans1/trans2 of every step
   traj_64x128.npz    the same at 64x128 for 1000 steps (fields after steps
                      1 and 1000, fluxes of every step)
+  ann_16x32.npz      the rotating-annulus plane (velvor2dnewsch3.4.f built
+                     into oracle/_ref/libann.so), r0=0.5, rmax=1, Ra=1e5,
+                     Re=100, Pr=1, dt=1e-5, 20 steps (fields after 1, 20)
+  ann_64x128.npz     the same at 64x128, Ra=1e7, Re=sqrt(1e5), dt=1e-6,
+                     300 steps (fields after 1, 300)
 The time-loop runs use box width 3 (the reference's hard-coded xx,
 rb.f:132), r0=0.5, rmax=1, Ra=1e5, Pr=1, dt=1e-5, and the temperature
 seed of the oracle (splitmix64, seed 20261202, amplitude 1e-3).
@@ -117,6 +122,28 @@ def trajectory(ref, nr, np_, nsteps, snaps):
     np.savez_compressed(os.path.join(HERE, f"traj_{nr}x{np_}.npz"), **out)
 
 
+def annulus(nr, np_, nsteps, snaps, dt, Ra, Re):
+    ref = refdrv.Reference("ann")
+    Pr = 1.0
+    run = O.Run(nr, np_, dt, Ra, Pr, geom=1, r0=0.5, Re=Re)
+    run.seed()
+    th0 = run["th"].copy()
+    ref.setup(nr, np_, dt, Ra, Pr, Re=Re)
+    ref.load("th", th0)
+    ref.start()
+    out = {"nr": nr, "np": np_, "dt": dt, "Ra": Ra, "Pr": Pr, "Re": Re, "r0": 0.5, "rmax": 1.0,
+           "seed": 20261202, "amp": 1e-3, "th0": th0}
+    trans = np.zeros((nsteps, 2))
+    for s in range(1, nsteps + 1):
+        trans[s - 1] = ref.step()
+        if s in snaps:
+            for name, ni in (("th", nr + 2), ("thp", nr + 2), ("C", nr + 1), ("Cp", nr + 1),
+                             ("uel", nr + 1), ("vel", nr + 2)):
+                out[f"s{s}_{name}"] = ref.get(name, ni)
+    out["trans"] = trans
+    np.savez_compressed(os.path.join(HERE, f"ann_{nr}x{np_}.npz"), **out)
+
+
 def main():
     if not refdrv.available():
         raise SystemExit("oracle/_ref/librb.so missing: run make -C oracle all first")
@@ -125,6 +152,9 @@ def main():
     poisson(ref)
     trajectory(ref, 16, 32, 20, {1, 2, 20})
     trajectory(ref, 64, 128, 1000, {1, 1000})
+    if refdrv.available("ann"):
+        annulus(16, 32, 20, {1, 20}, 1e-5, 1e5, 100.0)
+        annulus(64, 128, 300, {1, 300}, 1e-6, 1e7, float(np.sqrt(1e5)))
     print("fixtures written to", HERE)
 
 

@@ -17,36 +17,42 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBRB = os.path.join(ROOT, "oracle", "_ref", "librb.so")
+LIBANN = os.path.join(ROOT, "oracle", "_ref", "libann.so")
 
 KR1, KP1 = 291, 1611          # rb.f:54 parameter(kr=290, kp=1610)
+ANN_KR1, ANN_KP1 = 421, 3601  # velvor2dnewsch3.4.f: parameter(kr=420, kp=3600)
 _D = ctypes.POINTER(ctypes.c_double)
 
 
-def available() -> bool:
-    return os.path.exists(LIBRB)
+def available(kind="rb") -> bool:
+    return os.path.exists(LIBRB if kind == "rb" else LIBANN)
 
 
 class Reference:
-    """Singleton view of the reference's COMMON state."""
+    """Singleton view of one reference build's COMMON state
+    (kind "rb": velvor2dperiodicrbconv.f, "ann": velvor2dnewsch3.4.f)."""
 
-    _inst = None
+    _inst = {}
 
-    def __new__(cls):
-        if cls._inst is None:
+    def __new__(cls, kind="rb"):
+        if kind not in cls._inst:
             inst = super().__new__(cls)
-            inst._init()
-            cls._inst = inst
-        return cls._inst
+            inst._init(kind)
+            cls._inst[kind] = inst
+        return cls._inst[kind]
 
-    def _init(self):
-        L = ctypes.CDLL(LIBRB)
+    def _init(self, kind):
+        self.kind = kind
+        L = ctypes.CDLL(LIBRB if kind == "rb" else LIBANN)
         self.L = L
-        nf = KR1 * KP1
+        self.KR1, self.KP1 = (KR1, KP1) if kind == "rb" else (ANN_KR1, ANN_KP1)
+        KR1_, KP1_ = self.KR1, self.KP1
+        nf = KR1_ * KP1_
 
         def block(name, n):
             addr = ctypes.addressof(ctypes.c_char.in_dll(L, name))
             buf = (ctypes.c_double * (n * nf)).from_address(addr)
-            return np.frombuffer(buf, dtype=np.float64).reshape(n, KP1, KR1)
+            return np.frombuffer(buf, dtype=np.float64).reshape(n, KP1_, KR1_)
 
         s1 = block("sol1_", 9)
         s2 = block("sol2_", 3)
@@ -63,17 +69,18 @@ class Reference:
         self.num_i = (ctypes.c_int * 3).from_address(numaddr + 32)
         self.val = (ctypes.c_double * 4).in_dll(L, "val_")
         self.radius = (ctypes.c_double * 2).in_dll(L, "radius_")
-        self.vrh = np.zeros((KP1, KR1))
-        self.urh = np.zeros((KP1, KR1))
+        self.vrh = np.zeros((KP1_, KR1_))
+        self.urh = np.zeros((KP1_, KR1_))
 
     # ------------------------------------------------------------ setup
-    def setup(self, nr, np_, dt, Ra, Pr, r0=0.5, rmax=1.0, width=3.0):
-        """Scalars as the main program sets them (rb.f:78-133)."""
+    def setup(self, nr, np_, dt, Ra, Pr, r0=0.5, rmax=1.0, width=3.0, Re=0.0):
+        """Scalars as the main program sets them (rb.f:78-133; the annulus
+        program uses dp = 2 pi/np, ann.f:124)."""
         dr = (rmax - r0) / nr
-        dp = width / np_
+        dp = width / np_ if self.kind == "rb" else 2.0 * np.arccos(-1.0) / np_
         self.num_d[0], self.num_d[1], self.num_d[2], self.num_d[3] = dt, dr, 0.0, dp
         self.num_i[0], self.num_i[1], self.num_i[2] = nr, 1, np_
-        self.val[0], self.val[1], self.val[2], self.val[3] = Ra, 0.0, Pr, 0.0
+        self.val[0], self.val[1], self.val[2], self.val[3] = Ra, Re, Pr, 0.0
         self.radius[0], self.radius[1] = r0, rmax
         for a in self.f.values():
             a[...] = 0.0
@@ -109,7 +116,7 @@ class Reference:
         getattr(self.L, name + "_")(*args)
 
     def poisson(self, which, psi, psip, rhs):
-        """Call poisson2/poisson3 on arrays of shape (KP1, KR1)."""
+        """Call poisson2/poisson3 on arrays of shape (KP1, KR1) of this build."""
         n = ctypes.c_int
         d = ctypes.c_double
         P = lambda a: a.ctypes.data_as(_D)

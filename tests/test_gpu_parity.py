@@ -22,6 +22,20 @@ def _box(nr, np_, width, **kw):
     return PeriodicBox(BoxConfig(nr=nr, np=np_, width=width, **kw), device=0)
 
 
+def _annulus(nr, np_, **kw):
+    from codesthesis_dir_amd import AnnulusConfig, PeriodicBox
+    print(f"[gpu] annulus nr={nr} np={np_} {kw}", flush=True)
+    return PeriodicBox(AnnulusConfig(nr=nr, np=np_, **kw), device=0)
+
+
+def _pair(geom, nr, np_, dt=1e-5, Ra=1e5, Re=100.0, **kw):
+    """An oracle run and a GPU run with the same configuration."""
+    if geom == 0:
+        return O.Run(nr, np_, dt, Ra, 1.0, width=3.0), _box(nr, np_, 3.0, dt=dt, Ra=Ra, **kw)
+    return (O.Run(nr, np_, dt, Ra, 1.0, geom=1, r0=0.5, Re=Re),
+            _annulus(nr, np_, dt=dt, Ra=Ra, Re=Re, **kw))
+
+
 def _random_state(run, seed):
     rng = np.random.default_rng(seed)
     nr, np_ = run.nr, run.np
@@ -41,13 +55,13 @@ def _random_state(run, seed):
     return st
 
 
+@pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False),
                                            (40, 64, False)])
-def test_explicit_step_bitexact(nr, np_, first):
+def test_explicit_step_bitexact(nr, np_, first, geom):
     import ctypes
-    run = O.Run(nr, np_, 1e-5, 1e5, 1.0, width=3.0)
+    run, box = _pair(geom, nr, np_)
     st = _random_state(run, seed=nr + np_)
-    box = _box(nr, np_, 3.0)
     for n, a in st.items():
         box.put(n, a)
     # oracle
@@ -89,11 +103,18 @@ def test_explicit_step_bitexact(nr, np_, first):
 
 
 @pytest.mark.parametrize("nr,np_,width", [(16, 32, 3.0), (64, 128, 1.0), (64, 128, 3.0),
-                                          (128, 256, 1.0), (48, 64, 2.0)])
+                                          (128, 256, 1.0), (48, 64, 2.0), (64, 128, 0.0),
+                                          (256, 256, 0.0)])
 @pytest.mark.parametrize("which", [2, 3])
 def test_poisson_matches_exact_solution(nr, np_, width, which):
-    box = _box(nr, np_, width, rtol=1e-12, max_cycles=60)
-    dr, dp = 0.5 / nr, width / np_
+    """width 0 marks the annulus geometry (dp = 2 pi/np, r0 = 0.5)."""
+    geom = 1 if width == 0.0 else 0
+    if geom:
+        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
+        dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
+    else:
+        box = _box(nr, np_, width, rtol=1e-12, max_cycles=60)
+        dr, dp = 0.5 / nr, width / np_
     rng = np.random.default_rng(nr * 7 + which)
     ld = nr + 2
     k = np.arange(np_ + 1)[:, None]
@@ -110,13 +131,13 @@ def test_poisson_matches_exact_solution(nr, np_, width, which):
     st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
     assert st == 0 and rel <= 1e-12 and cyc < 40, (st, cyc, rel)
     ex = np.zeros_like(psi)
-    O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp)
+    O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
     err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
     assert err < 1e-10, err
     assert np.array_equal(psip[:, :ni], psi0[:, :ni])       # psip <- old psi
     assert np.all(psi[:, 0] == 0) and np.all(psi[:, ni - 1] == 0)
     assert np.array_equal(psi[0], psi[np_])
-    m, _ = O.residual(which, psi, rhs, nr, np_, dr, dp)
+    m, _ = O.residual(which, psi, rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
     assert m <= 2e-12 * np.max(np.abs(rhs[1:, 1:ni - 1]))
     box.close()
 
@@ -170,6 +191,52 @@ def test_trajectory_against_reference_fixture(golden_dir, name, snap):
         assert err < tol, (f, err)
     hist = box.heat_history(snap)
     assert np.max(np.abs(hist - g["trans"]) / np.abs(g["trans"])) < 1e-10
+    box.close()
+
+
+def test_annulus_trajectory_300_steps():
+    """Rotating-annulus plane (velvor2dnewsch3.4.f), 65x129 nodes, Ra=1e7,
+    Re=sqrt(1e5), dt=1e-6, 300 steps at 1e-10: relative L2 <= 1e-9 against
+    the oracle with exact solves."""
+    nr, np_ = 64, 128
+    Re = float(np.sqrt(1e5))
+    run, box = _pair(1, nr, np_, dt=1e-6, Ra=1e7, Re=Re, rtol=1e-10, max_cycles=60)
+    run.seed()
+    box.put("th", run["th"])
+    box.start()
+    info = box.step(300)
+    assert info.status == 0
+    for _ in range(300):
+        st, tr, ncl = run.step(mode=1)
+        assert st == 0
+    for name in ("th", "C", "uel", "vel", "Cp", "thp"):
+        g, o = box.get(name), run[name]
+        err = np.linalg.norm(g - o) / np.linalg.norm(o)
+        print(f"annulus {name}: rel L2 {err:.3e}")
+        assert err <= 1e-9, (name, err)
+    box.close()
+
+
+@pytest.mark.parametrize("name,snap", [("ann_16x32.npz", 20), ("ann_64x128.npz", 300)])
+def test_annulus_against_reference_fixture(golden_dir, name, snap):
+    """Against the annulus reference's own run (absolute Poisson stop 1e-5,
+    ann.f:2093): temperature and wall fluxes to round-off, the velocity and
+    vorticity to the reference's solver tolerance."""
+    g = np.load(os.path.join(golden_dir, name))
+    nr, np_ = int(g["nr"]), int(g["np"])
+    box = _annulus(nr, np_, dt=float(g["dt"]), Ra=float(g["Ra"]), Re=float(g["Re"]), rtol=1e-12,
+                   max_cycles=60)
+    box.put("th", g["th0"])
+    box.start()
+    box.step(snap)
+    for f, tol in (("th", 1e-9), ("C", 1e-5), ("uel", 1e-5), ("vel", 1e-5)):
+        ref = g[f"s{snap}_{f}"]
+        a = box.get(f)[:, : ref.shape[1]]
+        err = np.linalg.norm(a - ref) / np.linalg.norm(ref)
+        print(f"{name} {f}: {err:.3e}")
+        assert err < tol, (f, err)
+    hist = box.heat_history(snap)
+    assert np.max(np.abs(hist - g["trans"]) / np.abs(g["trans"])) < 1e-9
     box.close()
 
 

@@ -62,3 +62,24 @@ def test_trajectory_bitexact(golden_dir, name, snaps):
             for name_f in ("th", "thp", "C", "Cp", "uel", "vel"):
                 ref = g[f"s{s}_{name_f}"]
                 assert np.array_equal(run[name_f][:, : ref.shape[1]], ref), (s, name_f)
+
+
+@pytest.mark.parametrize("name,snaps", [("ann_16x32.npz", (1, 20)), ("ann_64x128.npz", (1, 300))])
+def test_annulus_trajectory_bitexact(golden_dir, name, snaps):
+    """The rotating-annulus plane (velvor2dnewsch3.4.f) restated by the
+    oracle against the reference's own run."""
+    g = _load(golden_dir, name)
+    nr, np_ = int(g["nr"]), int(g["np"])
+    run = O.Run(nr, np_, float(g["dt"]), float(g["Ra"]), float(g["Pr"]), geom=1,
+                r0=float(g["r0"]), Re=float(g["Re"]))
+    run.seed(r0=float(g["r0"]), amp=float(g["amp"]), seed=int(g["seed"]))
+    assert np.array_equal(run["th"], g["th0"])
+    trans = g["trans"]
+    for s in range(1, max(snaps) + 1):
+        st, tr, ncl = run.step(mode=0)
+        assert st == 0 and ncl == 0
+        assert np.array_equal(tr, trans[s - 1]), s
+        if s in snaps:
+            for name_f in ("th", "thp", "C", "Cp", "uel", "vel"):
+                ref = g[f"s{s}_{name_f}"]
+                assert np.array_equal(run[name_f][:, : ref.shape[1]], ref), (s, name_f)

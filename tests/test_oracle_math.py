@@ -236,19 +236,22 @@ def test_exact_poisson_against_dense_solve(which):
     assert np.array_equal(psi[0], psi[np_])
 
 
+@pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("which", [2, 3])
-def test_literal_solver_converges_to_exact(which):
+def test_literal_solver_converges_to_exact(which, geom):
     """Tightening the reference's absolute stop drives its FMG/line-GS
-    iteration to the exact discrete solution."""
+    iteration (box: rb.f; annulus: velvor2dnewsch3.4.f) to the exact
+    discrete solution of the separable solver."""
     nr, np_ = 32, 64
-    run = O.Run(nr, np_, 1e-5, 1e5, 1.0, width=3.0)
+    run = O.Run(nr, np_, 1e-5, 1e5, 1.0, width=3.0, geom=geom, r0=0.5, Re=100.0)
     rng = np.random.default_rng(9)
     k = np.arange(np_ + 1)[:, None]
     i = np.arange(run.ld)[None, :]
     rhs = np.ascontiguousarray(np.cos(2 * np.pi * k / np_) * np.sin(0.2 * i) + 0.05 * rng.standard_normal((np_ + 1, run.ld)))
     rhs[np_] = rhs[0]
     psi_e = np.zeros_like(rhs)
-    O.poisson_exact(which, psi_e, np.zeros_like(rhs), rhs, nr, np_, run.p.dr, run.p.dp)
+    O.poisson_exact(which, psi_e, np.zeros_like(rhs), rhs, nr, np_, run.p.dr, run.p.dp,
+                    geom=geom, r0=0.5)
     psi = np.zeros_like(rhs)
     psip = np.zeros_like(rhs)
     it = ctypes.c_int()
@@ -259,7 +262,7 @@ def test_literal_solver_converges_to_exact(which):
     ni = nr + 2 if which == 2 else nr + 1
     err = np.max(np.abs(psi[:, :ni] - psi_e[:, :ni])) / np.max(np.abs(psi_e))
     assert err < 1e-9
-    m, _ = O.residual(which, psi_e, rhs, nr, np_, run.p.dr, run.p.dp)
+    m, _ = O.residual(which, psi_e, rhs, nr, np_, run.p.dr, run.p.dp, geom=geom, r0=0.5)
     assert m < 1e-11 * np.max(np.abs(rhs))
 
 
