@@ -17,6 +17,39 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-command-line-argument"]
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+DROPIN_SRC = os.path.join(PKG, "fortran", "vv_dropin.F90")
+DROPIN_LIB = os.path.join(PKG, "libvvdropin.so")
+
+
+def build_dropin() -> str:
+    """The Fortran drop-in (reference routine names over the C ABI) as a
+    shared library next to libvvmi.so."""
+    mod = os.path.join(PKG, "fortran", "_build")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-o", DROPIN_LIB + ".tmp",
+           DROPIN_SRC, "-L" + PKG, "-lvvmi", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
+    os.replace(DROPIN_LIB + ".tmp", DROPIN_LIB)
+    return DROPIN_LIB
+
+
+def build_dropin_driver(root: str) -> str:
+    """Test driver (tests/fortran/vv_driver.f90) linked against the drop-in."""
+    src = os.path.join(root, "tests", "fortran", "vv_driver.f90")
+    out_dir = os.path.join(root, "tests", "fortran", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "vv_driver")
+    cmd = [FLANG, "-O2", "-module-dir", out_dir, "-o", exe, src, "-L" + PKG, "-lvvdropin",
+           "-lvvmi", "-Wl,-rpath,$ORIGIN/../../../codesthesis_dir_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
+    return exe
+
+
 def build_library(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
@@ -31,3 +64,4 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_dropin())

@@ -37,3 +37,15 @@ def test_every_declaration_cites_the_reference():
     replaces (file:line) or says it has no counterpart."""
     src = open(HEADER).read()
     assert src.count("rb.f:") >= 10
+
+
+def test_fortran_dropin_exports_reference_routine_names():
+    """libvvdropin.so exports the reference's routine names (flang mangling:
+    lower case plus underscore) so a reference main program links to it."""
+    path = os.path.join(ROOT, "codesthesis_dir_amd", "libvvdropin.so")
+    if not os.path.exists(path):
+        pytest.skip("libvvdropin.so not built")
+    lib = ctypes.CDLL(path)
+    for n in ("poissoninit_", "poisson2_", "poisson3_", "cvorticity1_", "cvorticity_",
+              "temperature1_", "temperature_", "heat_", "rhsv_", "rhsu_", "update_"):
+        assert hasattr(lib, n), n
