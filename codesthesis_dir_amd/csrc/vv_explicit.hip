@@ -337,10 +337,8 @@ struct RhsArgs {
     double dr, dp;
 };
 
-__device__ __forceinline__ size_t mgix(int j, int q, int n, int h)
-{
-    return (size_t)(q & 1) * n * h + (size_t)j * h + (q >> 1);
-}
+// multigrid level 0 is column-major: row j of column q at q*n + j
+__device__ __forceinline__ size_t mgix(int j, int q, int n) { return (size_t)q * n + j; }
 
 // rhsv (rb.f:1629-1676), rhsu (rb.f:1679-1721), the extrapolated initial
 // guess of poisson2/3 (rb.f:1827-1835) scattered into the multigrid level 0,
@@ -358,7 +356,6 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
     if (t >= rows * (np + 1)) return;
     const int k = t / rows, i = t - k * rows;
     const size_t o = ix(i, k, ld);
-    const int h = np / 2 > 0 ? np / 2 : 1;
     if ((a.which & 1) && i >= 1 && i <= nr) {
         double v;
         if (a.from_fields) {
@@ -378,7 +375,7 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
             a.vrh[o] = v;
         }
         if (k >= 1) {
-            const size_t m = mgix(i - 1, k - 1, nr, h);
+            const size_t m = mgix(i - 1, k - 1, nr);
             a.fv[m] = v;
             a.uv[m] = 2.0 * a.vel[o] - a.velp[o];
             atomicMax(&a.fnorm[0], (unsigned long long)__double_as_longlong(fabs(v)));
@@ -395,7 +392,7 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
             a.urh[o] = u;
         }
         if (k >= 1) {
-            const size_t m = mgix(i - 1, k - 1, nr - 1, h);
+            const size_t m = mgix(i - 1, k - 1, nr - 1);
             a.fu[m] = u;
             a.uu[m] = 2.0 * a.uel[o] - a.uelp[o];
             atomicMax(&a.fnorm[1], (unsigned long long)__double_as_longlong(fabs(u)));

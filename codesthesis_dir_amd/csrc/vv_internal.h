@@ -43,27 +43,25 @@ struct PtdmaTab {
 PtdmaTab make_ptdma(const std::vector<double> &a, const std::vector<double> &b,
                     const std::vector<double> &c);
 
-// Banded (2 sub, 2 super) LU without pivoting of one line operator.
-struct BandLU {
-    int n = 0;
-    std::vector<double> l1, l2, u1, u2, dinv;
-};
-BandLU make_band_lu(int n, const std::vector<double> &sub2, const std::vector<double> &sub1,
-                    const std::vector<double> &diag, const std::vector<double> &sup1,
-                    const std::vector<double> &sup2);
+// Parallel-cyclic-reduction tables of one banded line operator (see
+// MgLevelDev); returns the number of reduction steps.
+int make_pcr(int n, const std::vector<double> &sub2, const std::vector<double> &sub1,
+             const std::vector<double> &diag, const std::vector<double> &sup1,
+             const std::vector<double> &sup2, std::vector<double> &tab);
 
 // ------------------------------------------------------------- multigrid
-// Device layout of one multigrid level: rows i = 0..n-1 (unknown rows of
-// the line operator), columns q = 0..ncol-1 (periodic).  Columns are stored
-// by colour: index = (q&1)*n*h + i*h + (q>>1), h = max(ncol/2,1), so the
-// threads of a half-sweep touch consecutive addresses.
+// Device layout of one multigrid level: rows i = 0..n-1 (unknown rows of the
+// line operator) contiguous, columns q = 0..ncol-1 (periodic): u[q*n + i].
+// A line solve (one column) reads and writes consecutive addresses.
+// lu holds the parallel-cyclic-reduction tables of the level's line
+// operator: g0 | gN | alpha[S][n] | beta[S][n] | dinv[n], S = ceil(log2 n)
+// (closure rows of the v operator are folded into rows 0 and n-1 first).
 struct MgLevelDev {
     double *u, *f;
-    const double *lu;     // 5*n: l1 | l2 | u1 | u2 | dinv
+    const double *lu;
     double cphi;          // 1/hp^2 at this level
-    int ncol, h;
+    int ncol, nstep;
 };
-
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;
 

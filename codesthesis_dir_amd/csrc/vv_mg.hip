@@ -2,17 +2,20 @@
 //
 // The converged discrete solution is fixed by the reference's residual
 // operators (rb.f relax2 residual rb.f:2306-2357 for v, relax3 residual
-// rb.f:2553-2587 for u): a periodic 3-point second difference along phi
+// rb.f:2553-2587 for u; the annulus forms ann.f:2301-2348, 2555-2583): a
+// periodic 3-point second difference along phi (times 1/r^2 in the annulus)
 // plus the r operator D_r, which for v carries the one-sided closure rows
 // (-2, 5/3, -1/5)/dr^2 at both walls.  That closure is not preserved under
 // r-coarsening (a point-smoother V-cycle with rediscretised coarse grids
 // diverges on it), so the cycle here never coarsens r:
-//   - smoother: zebra r-line Gauss-Seidel, each line solved exactly with
-//     a banded LU factorised once per level on the host;
-//   - coarsening: phi only, full weighting / linear interpolation, down to
-//     a single column where the line solve is the exact coarse solve.
+//   - smoother: zebra r-line Gauss-Seidel; every line is solved exactly by
+//     parallel cyclic reduction in LDS, one workgroup per line, with the
+//     reduction coefficients of the level's line operator precomputed on the
+//     host (the closure rows are first folded into a tridiagonal system);
+//   - coarsening: phi only, full weighting / linear interpolation, down to a
+//     single column, where the line solve is the exact coarse solve.
 // Both velocity problems of a run are batched into every launch
-// (blockIdx.y = problem).  Kernels return early once their problem has
+// (blockIdx.y = problem).  Kernels return at once when their problem has
 // converged, so several cycles can be queued between host checks.
 
 #include "vv_internal.h"
@@ -25,35 +28,34 @@ namespace vv {
 
 namespace {
 
-constexpr int WG = 64;
+constexpr int MAXN = 2048;     // rows per line held in LDS
 
-__device__ __forceinline__ size_t mix(int j, int q, int n, int h)
-{
-    return (size_t)(q & 1) * n * h + (size_t)j * h + (q >> 1);
-}
+__device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
 
 // residual f - (D_r u + cphi*rphi*(u_{q+1} - 2u + u_{q-1})) at (j, q)
 __device__ __forceinline__ double resid_at(const MgProblemDev &P, const MgLevelDev &L, int j, int q)
 {
-    const int n = P.n, h = L.h, ncol = L.ncol;
+    const int n = P.n, ncol = L.ncol;
     const double *u = L.u;
     const double *s = P.sten + 5 * j;
     double lu = 0.0;
 #pragma unroll
     for (int o = -2; o <= 2; ++o) {
         const int jj = j + o;
-        if (jj >= 0 && jj < n && s[o + 2] != 0.0) lu += s[o + 2] * u[mix(jj, q, n, h)];
+        if (jj >= 0 && jj < n && s[o + 2] != 0.0) lu += s[o + 2] * u[cix(jj, q, n)];
     }
-    const double uc = u[mix(j, q, n, h)];
+    const double uc = u[cix(j, q, n)];
     if (ncol > 1) {
         const int qm = (q == 0) ? ncol - 1 : q - 1;
         const int qp = (q == ncol - 1) ? 0 : q + 1;
-        lu += L.cphi * P.rphi[j] * (u[mix(j, qp, n, h)] - 2.0 * uc + u[mix(j, qm, n, h)]);
+        lu += L.cphi * P.rphi[j] * (u[cix(j, qp, n)] - 2.0 * uc + u[cix(j, qm, n)]);
     }
-    return L.f[mix(j, q, n, h)] - lu;
+    return L.f[cix(j, q, n)] - lu;
 }
 
-__global__ void __launch_bounds__(WG) k_mg_zebra(const MgBatchDev *__restrict__ Bp, int lev, int color)
+// One colour of r-lines of one level: line q = 2*blockIdx.x + colour (the
+// single column of the coarsest level solves D_r u = f).
+__global__ void __launch_bounds__(1024) k_mg_zebra(const MgBatchDev *__restrict__ Bp, int lev, int color)
 {
     const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
@@ -61,45 +63,49 @@ __global__ void __launch_bounds__(WG) k_mg_zebra(const MgBatchDev *__restrict__ 
     const MgProblemDev &P = B.prob[p];
     if (lev >= P.nlev) return;
     const MgLevelDev &L = P.lev[lev];
-    const int n = P.n, h = L.h, ncol = L.ncol;
-    const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = P.n, ncol = L.ncol;
     const bool coarsest = (ncol == 1);
-    if (coarsest) {
-        if (color != 0 || Q != 0) return;
-    } else if (Q >= h) {
-        return;
-    }
-    const int q = coarsest ? 0 : 2 * Q + color;
+    const int line = blockIdx.x;
+    if (coarsest ? (line != 0 || color != 0) : (2 * line + color >= ncol)) return;
+    const int q = coarsest ? 0 : 2 * line + color;
+    __shared__ double buf[2][MAXN];
+    const int tid = threadIdx.x, nt = blockDim.x;
     double *u = L.u;
     const double *f = L.f;
-    const double *l1 = L.lu, *l2 = L.lu + n, *u1 = L.lu + 2 * n, *u2 = L.lu + 3 * n, *dinv = L.lu + 4 * n;
-    const size_t self = (size_t)color * n * h + Q;
-    size_t om = 0, op = 0;
-    if (!coarsest) {
+    const double c = L.cphi;
+    if (coarsest) {
+        for (int i = tid; i < n; i += nt) buf[0][i] = f[cix(i, 0, n)];
+    } else {
         const int qm = (q == 0) ? ncol - 1 : q - 1;
         const int qp = (q == ncol - 1) ? 0 : q + 1;
-        om = (size_t)(qm & 1) * n * h + (qm >> 1);
-        op = (size_t)(qp & 1) * n * h + (qp >> 1);
+        for (int i = tid; i < n; i += nt)
+            buf[0][i] = f[cix(i, q, n)] - c * P.rphi[i] * (u[cix(i, qm, n)] + u[cix(i, qp, n)]);
     }
-    const double c = L.cphi;
-    double y1 = 0.0, y2 = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const size_t row = (size_t)j * h;
-        double r = f[self + row];
-        if (!coarsest) r -= c * P.rphi[j] * (u[om + row] + u[op + row]);
-        const double y = r - l1[j] * y1 - l2[j] * y2;
-        u[self + row] = y;
-        y2 = y1;
-        y1 = y;
+    __syncthreads();
+    const double *tab = L.lu;
+    const int S = L.nstep;
+    if (tid == 0) {      // fold the closure rows: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
+        const double r0 = buf[0][0] - tab[0] * buf[0][1];
+        const double rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
+        buf[0][0] = r0;
+        buf[0][n - 1] = rN;
     }
-    double x1 = 0.0, x2 = 0.0;
-    for (int j = n - 1; j >= 0; --j) {
-        const size_t row = (size_t)j * h;
-        const double x = (u[self + row] - u1[j] * x1 - u2[j] * x2) * dinv[j];
-        u[self + row] = x;
-        x2 = x1;
-        x1 = x;
+    __syncthreads();
+    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
+    int cur = 0;
+    for (int t = 0; t < S; ++t) {
+        const int s = 1 << t;
+        const double *a = alpha + (size_t)t * n, *b = beta + (size_t)t * n;
+        for (int i = tid; i < n; i += nt) {
+            double v = buf[cur][i];
+            if (i >= s) v += a[i] * buf[cur][i - s];
+            if (i + s < n) v += b[i] * buf[cur][i + s];
+            buf[cur ^ 1][i] = v;
+        }
+        __syncthreads();
+        cur ^= 1;
     }
+    for (int i = tid; i < n; i += nt) u[cix(i, q, n)] = buf[cur][i] * dinv[i];
 }
 
 // fine level lev -> coarse lev+1: full weighting of the residual along phi;
@@ -116,14 +122,13 @@ __global__ void __launch_bounds__(256) k_mg_restrict(const MgBatchDev *__restric
     const int n = P.n, nc = Cc.ncol;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * nc) return;
-    const int j = (int)(t / nc), qc = (int)(t - (long long)j * nc);
+    const int qc = (int)(t / n), j = (int)(t - (long long)qc * n);
     const int q0 = 2 * qc, q1 = 2 * qc + 1, q2 = (2 * qc + 2) % F.ncol;
     const double r0 = resid_at(P, F, j, q0);
     const double r1 = resid_at(P, F, j, q1);
     const double r2 = resid_at(P, F, j, q2);
-    const size_t m = mix(j, qc, n, Cc.h);
-    Cc.f[m] = 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
-    Cc.u[m] = 0.0;
+    Cc.f[cix(j, qc, n)] = 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
+    Cc.u[cix(j, qc, n)] = 0.0;
 }
 
 // coarse lev+1 -> fine lev: linear interpolation along phi, added.
@@ -139,12 +144,12 @@ __global__ void __launch_bounds__(256) k_mg_prolong(const MgBatchDev *__restrict
     const int n = P.n, nc = Cc.ncol;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * nc) return;
-    const int j = (int)(t / nc), qc = (int)(t - (long long)j * nc);
+    const int qc = (int)(t / n), j = (int)(t - (long long)qc * n);
     const int qcm = (qc == 0) ? nc - 1 : qc - 1;
-    const double e = Cc.u[mix(j, qc, n, Cc.h)];
-    const double em = Cc.u[mix(j, qcm, n, Cc.h)];
-    F.u[mix(j, 2 * qc + 1, n, F.h)] += e;
-    F.u[mix(j, 2 * qc, n, F.h)] += 0.5 * (e + em);
+    const double e = Cc.u[cix(j, qc, n)];
+    const double em = Cc.u[cix(j, qcm, n)];
+    F.u[cix(j, 2 * qc + 1, n)] += e;
+    F.u[cix(j, 2 * qc, n)] += 0.5 * (e + em);
 }
 
 // max-norm of the level-0 residual
@@ -158,10 +163,9 @@ __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ 
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (t < (long long)P.n * L.ncol) {
-        const int j = (int)(t / L.ncol), q = (int)(t - (long long)j * L.ncol);
+        const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
         m = fabs(resid_at(P, L, j, q));
     }
-    // wavefront reduction, then one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_down(m, off, 64);
         m = (o > m || o != o) ? o : m;
@@ -198,8 +202,8 @@ __global__ void k_mg_reset(const MgBatchDev *__restrict__ Bp, int which)
 }
 
 // level 0 -> fields: psip <- psi, psi <- solution (rb.f:1894-1913)
-__global__ void __launch_bounds__(256) k_mg_out(const MgBatchDev *__restrict__ Bp, double *vel, double *velp, double *uel,
-                                                double *uelp, int nr, int np, int ld, int which)
+__global__ void __launch_bounds__(256) k_mg_out(const MgBatchDev *__restrict__ Bp, double *vel, double *velp,
+                                                double *uel, double *uelp, int nr, int np, int ld, int which)
 {
     const MgBatchDev &B = *Bp;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,19 +212,18 @@ __global__ void __launch_bounds__(256) k_mg_out(const MgBatchDev *__restrict__ B
     const int k = (int)(t / rows), i = (int)(t - (long long)k * rows);
     const size_t o = (size_t)k * ld + i;
     const int q = (k == 0) ? np - 1 : k - 1;
-    const int h = np / 2 > 0 ? np / 2 : 1;
     if (which & 1) {
         const double old = vel[o];
         velp[o] = old;
         double nv = 0.0;
-        if (i >= 1 && i <= nr) nv = B.prob[0].lev[0].u[mix(i - 1, q, nr, h)];
+        if (i >= 1 && i <= nr) nv = B.prob[0].lev[0].u[cix(i - 1, q, nr)];
         vel[o] = nv;
     }
     if ((which & 2) && i <= nr) {
         const double old = uel[o];
         uelp[o] = old;
         double nu = 0.0;
-        if (i >= 1 && i <= nr - 1) nu = B.prob[1].lev[0].u[mix(i - 1, q, nr - 1, h)];
+        if (i >= 1 && i <= nr - 1) nu = B.prob[1].lev[0].u[cix(i - 1, q, nr - 1)];
         uel[o] = nu;
     }
 }
@@ -229,59 +232,75 @@ inline int nblk(long long n, int wg) { return (int)((n + wg - 1) / wg); }
 
 }  // namespace
 
-// banded LU without pivoting (Doolittle, bandwidth 2)
-BandLU make_band_lu(int n, const std::vector<double> &s2, const std::vector<double> &s1,
-                    const std::vector<double> &d, const std::vector<double> &p1,
-                    const std::vector<double> &p2)
+// Parallel-cyclic-reduction tables of a line operator with rows
+// s2 x_{i-2} + s1 x_{i-1} + d x_i + p1 x_{i+1} + p2 x_{i+2} = r_i, where only
+// row 0 has p2 and only row n-1 has s2.  Those two couplings are removed
+// first with rows 1 and n-2 (factors g0, gN); then the tridiagonal system is
+// reduced in S = ceil(log2 n) steps: step t with stride s = 2^t replaces
+// row i by row i + alpha*row(i-s) + beta*row(i+s).  The rhs follows the same
+// linear combinations on the device; x_i = r_i^(S) * dinv_i.
+int make_pcr(int n, const std::vector<double> &s2, const std::vector<double> &s1,
+             const std::vector<double> &d, const std::vector<double> &p1,
+             const std::vector<double> &p2, std::vector<double> &tab)
 {
-    BandLU f;
-    f.n = n;
-    f.l1.assign(n, 0.0); f.l2.assign(n, 0.0); f.u1.assign(n, 0.0); f.u2.assign(n, 0.0);
-    f.dinv.assign(n, 0.0);
-    std::vector<double> dd(n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        double l2 = 0.0, l1 = 0.0;
-        if (i >= 2) l2 = s2[i] / dd[i - 2];
-        if (i >= 1) l1 = (s1[i] - (i >= 2 ? l2 * f.u1[i - 2] : 0.0)) / dd[i - 1];
-        double di = d[i] - (i >= 1 ? l1 * f.u1[i - 1] : 0.0) - (i >= 2 ? l2 * f.u2[i - 2] : 0.0);
-        f.u1[i] = (i + 1 < n) ? p1[i] - (i >= 1 ? l1 * f.u2[i - 1] : 0.0) : 0.0;
-        f.u2[i] = (i + 2 < n) ? p2[i] : 0.0;
-        f.l1[i] = l1;
-        f.l2[i] = l2;
-        dd[i] = di;
-        f.dinv[i] = 1.0 / di;
+    std::vector<double> a(s1), b(d), c(p1);
+    double g0 = 0.0, gN = 0.0;
+    if (n > 2) {
+        g0 = p2[0] / c[1];
+        b[0] = b[0] - g0 * a[1];
+        c[0] = c[0] - g0 * b[1];
+        gN = s2[n - 1] / a[n - 2];
+        a[n - 1] = a[n - 1] - gN * b[n - 2];
+        b[n - 1] = b[n - 1] - gN * c[n - 2];
     }
-    return f;
+    a[0] = 0.0;
+    c[n - 1] = 0.0;
+    int S = 0;
+    while ((1 << S) < n) ++S;
+    tab.assign(2 + 2 * (size_t)S * n + n, 0.0);
+    tab[0] = g0;
+    tab[1] = gN;
+    double *alpha = tab.data() + 2, *beta = tab.data() + 2 + (size_t)S * n;
+    std::vector<double> an(n), bn(n), cn(n);
+    for (int t = 0; t < S; ++t) {
+        const int s = 1 << t;
+        for (int i = 0; i < n; ++i) {
+            const double A = (i - s >= 0) ? -a[i] / b[i - s] : 0.0;
+            const double Bc = (i + s < n) ? -c[i] / b[i + s] : 0.0;
+            alpha[(size_t)t * n + i] = A;
+            beta[(size_t)t * n + i] = Bc;
+            an[i] = (i - s >= 0) ? A * a[i - s] : 0.0;
+            cn[i] = (i + s < n) ? Bc * c[i + s] : 0.0;
+            bn[i] = b[i] + ((i - s >= 0) ? A * c[i - s] : 0.0) + ((i + s < n) ? Bc * a[i + s] : 0.0);
+        }
+        a.swap(an);
+        b.swap(bn);
+        c.swap(cn);
+    }
+    double *dinv = tab.data() + 2 + 2 * (size_t)S * n;
+    for (int i = 0; i < n; ++i) dinv[i] = 1.0 / b[i];
+    return S;
 }
 
 int mg_build(vv_run *r)
 {
     const int nr = r->nr, np = r->np;
     const double hhri = 1.0 / (r->dr * r->dr);
+    if (nr > MAXN) { set_error("nr exceeds the line length held in LDS (2048)"); return VV_EINVAL; }
     int nlev = 0;
     for (int c = np; c >= 1; c /= 2) {
         ++nlev;
         if (c == 1) break;
     }
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
-    // sizes: per problem per level u,f (n*ncol each) + lu (5n) ; sten+rphi (6n)
     const int nvec[2] = {nr, nr - 1};
+    // host-side tables first, then one pool
+    struct LevHost { std::vector<double> tab; int S; double cphi; int ncol; };
+    std::vector<LevHost> lh[2];
+    std::vector<double> sten_h[2], rphi_h[2];
     size_t total = 0;
     for (int p = 0; p < 2; ++p) {
         const int n = nvec[p];
-        total += 6 * (size_t)n;
-        for (int l = 0, c = np; l < nlev; ++l, c /= 2) total += 2 * (size_t)n * c + 5 * (size_t)n;
-    }
-    VV_HIP(hipMalloc(&r->mg_pool, total * sizeof(double)));
-    VV_HIP(hipMemsetAsync(r->mg_pool, 0, total * sizeof(double), r->stream));
-    std::vector<double> host(total, 0.0);
-    size_t off = 0;
-    r->mg.nprob = 2;
-    for (int p = 0; p < 2; ++p) {
-        const int n = nvec[p];
-        MgProblemDev &P = r->mg.prob[p];
-        P.n = n;
-        P.nlev = nlev;
         // r operator stencil (i-2..i+2) and phi row factor: the level-1
         // residual operators of relax2 (rb.f:2306-2357, ann.f:2301-2348) and
         // relax3 (rb.f:2553-2587, ann.f:2555-2583)
@@ -307,35 +326,52 @@ int mg_build(vv_run *r)
                 if (j < n - 1) p1[j] = hhri + (ann ? 1.5 * hri * rvi : 0.0);
             }
         }
-        double *sten_h = host.data() + off;
+        sten_h[p].resize(5 * (size_t)n);
         for (int j = 0; j < n; ++j) {
-            sten_h[5 * j + 0] = s2[j]; sten_h[5 * j + 1] = s1[j]; sten_h[5 * j + 2] = dg[j];
-            sten_h[5 * j + 3] = p1[j]; sten_h[5 * j + 4] = p2[j];
+            sten_h[p][5 * j + 0] = s2[j]; sten_h[p][5 * j + 1] = s1[j]; sten_h[p][5 * j + 2] = dg[j];
+            sten_h[p][5 * j + 3] = p1[j]; sten_h[p][5 * j + 4] = p2[j];
         }
-        P.sten = r->mg_pool + off;
-        off += 5 * (size_t)n;
-        for (int j = 0; j < n; ++j) host[off + j] = rphi[j];
-        P.rphi = r->mg_pool + off;
-        off += n;
+        rphi_h[p] = rphi;
+        total += 6 * (size_t)n;
         double hp = r->dp;
         for (int l = 0, c = np; l < nlev; ++l, c /= 2, hp *= 2.0) {
-            MgLevelDev &L = P.lev[l];
+            LevHost L;
             L.ncol = c;
-            L.h = c / 2 > 0 ? c / 2 : 1;
             L.cphi = 1.0 / (hp * hp);
-            L.u = r->mg_pool + off; off += (size_t)n * c;
-            L.f = r->mg_pool + off; off += (size_t)n * c;
             std::vector<double> d2(dg);
             if (c > 1)
                 for (int j = 0; j < n; ++j) d2[j] = dg[j] - 2.0 * L.cphi * rphi[j];
-            BandLU lu = make_band_lu(n, s2, s1, d2, p1, p2);
-            double *t = host.data() + off;
-            for (int j = 0; j < n; ++j) {
-                t[j] = lu.l1[j]; t[n + j] = lu.l2[j]; t[2 * n + j] = lu.u1[j];
-                t[3 * n + j] = lu.u2[j]; t[4 * n + j] = lu.dinv[j];
-            }
+            L.S = make_pcr(n, s2, s1, d2, p1, p2, L.tab);
+            total += 2 * (size_t)n * c + L.tab.size();
+            lh[p].push_back(std::move(L));
+        }
+    }
+    VV_HIP(hipMalloc(&r->mg_pool, total * sizeof(double)));
+    std::vector<double> host(total, 0.0);
+    size_t off = 0;
+    r->mg.nprob = 2;
+    for (int p = 0; p < 2; ++p) {
+        const int n = nvec[p];
+        MgProblemDev &P = r->mg.prob[p];
+        P.n = n;
+        P.nlev = nlev;
+        std::memcpy(host.data() + off, sten_h[p].data(), 5 * (size_t)n * sizeof(double));
+        P.sten = r->mg_pool + off;
+        off += 5 * (size_t)n;
+        std::memcpy(host.data() + off, rphi_h[p].data(), (size_t)n * sizeof(double));
+        P.rphi = r->mg_pool + off;
+        off += n;
+        for (int l = 0; l < nlev; ++l) {
+            MgLevelDev &L = P.lev[l];
+            const LevHost &H = lh[p][l];
+            L.ncol = H.ncol;
+            L.cphi = H.cphi;
+            L.nstep = H.S;
+            L.u = r->mg_pool + off; off += (size_t)n * H.ncol;
+            L.f = r->mg_pool + off; off += (size_t)n * H.ncol;
+            std::memcpy(host.data() + off, H.tab.data(), H.tab.size() * sizeof(double));
             L.lu = r->mg_pool + off;
-            off += 5 * (size_t)n;
+            off += H.tab.size();
         }
     }
     VV_HIP(hipMemcpyAsync(r->mg_pool, host.data(), total * sizeof(double), hipMemcpyHostToDevice,
@@ -359,10 +395,16 @@ static double zebra_bytes(const vv::MgBatchDev &B, int l)
 {
     double b = 0.0;
     for (int p = 0; p < B.nprob; ++p) {
-        const int n = B.prob[p].n, h = B.prob[p].lev[l].h, nc = B.prob[p].lev[l].ncol;
-        b += (nc == 1) ? 16.0 * n : 24.0 * (double)n * h;
+        const int n = B.prob[p].n, nc = B.prob[p].lev[l].ncol;
+        b += (nc == 1) ? 16.0 * n : 24.0 * (double)n * (nc / 2);
     }
     return b;
+}
+
+static int zebra_threads(int n)
+{
+    int t = ((n + 63) / 64) * 64;
+    return t > 1024 ? 1024 : t;
 }
 
 static int enqueue_cycle(vv_run *r)
@@ -372,13 +414,14 @@ static int enqueue_cycle(vv_run *r)
     const int nlev = B.prob[0].nlev;
     const int n = r->nr;
     const int ntot = B.prob[0].n + B.prob[1].n;
+    const int zt = zebra_threads(n);
     const double rtol = r->cfg.rtol;
     hipEvent_t pe;
     auto zebra = [&](int l, int c) {
-        const int h = B.prob[0].lev[l].h;
-        const bool coarse = B.prob[0].lev[l].ncol == 1;
+        const int nc = B.prob[0].lev[l].ncol;
+        const int lines = (nc == 1) ? 1 : nc / 2;
         prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_zebra, dim3(coarse ? 1 : nblk(h, WG), 2), dim3(WG), 0, r->stream, Bd, l, c);
+        hipLaunchKernelGGL(k_mg_zebra, dim3(lines, 2), dim3(zt), 0, r->stream, Bd, l, c);
         prof_end(r, VV_K_ZEBRA, pe, zebra_bytes(B, l));
     };
     for (int l = 0; l < nlev - 1; ++l) {
