@@ -32,129 +32,180 @@ struct CompactArgs {
     double dri, dpi;
 };
 
-// ---- batched Thomas along r (one thread per column k), rb.f:3046-3064
-// with factors from TdmaTab: fact | bp | c, each of length n.
+// Right-hand sides of the compact systems.  FAM 0: Cr (rb.f:810-824),
+// FAM 1: thr at vorticity points (half derivative, rb.f:854-875), FAM 2: thr
+// at temperature points with one-sided closures (rb.f:1244-1262); row i of
+// column k.
 template <int FAM>
-__device__ void tdma_column(const CompactArgs &a, int k)
+__device__ __forceinline__ double rhs_r(const CompactArgs &a, int i, int k)
 {
     const int nr = a.nr, ld = a.ld, np = a.np;
-    const int n = (FAM == 2) ? nr : nr - 1;
-    const double *tab = (FAM == 2) ? a.tab_rT : a.tab_r1;
-    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n;
     const double dri = a.dri;
-    double *out = (FAM == 0) ? a.dCr : (FAM == 1) ? a.dthrC : a.dthrT;
     const double *C = a.C, *th = a.th;
     const int ka = (k == np) ? 1 : k + 1;
-    double rprev = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const int i = j + 1;
-        double r;
-        if (FAM == 0) {            // Cr, rb.f:810-824
-            if (i == 1)
-                r = 3.0 * (C[ix(2, k, ld)] - C[ix(0, k, ld)]) * dri
-                  - (-1.5 * C[ix(0, k, ld)] + 2.0 * C[ix(1, k, ld)] - 0.5 * C[ix(2, k, ld)]) * dri;
-            else if (i == nr - 1)
-                r = 3.0 * (C[ix(nr, k, ld)] - C[ix(nr - 2, k, ld)]) * dri
-                  - (0.5 * C[ix(nr - 2, k, ld)] - 2.0 * C[ix(nr - 1, k, ld)] + 1.5 * C[ix(nr, k, ld)]) * dri;
-            else
-                r = 3.0 * (C[ix(i + 1, k, ld)] - C[ix(i - 1, k, ld)]) * dri;
-        } else if (FAM == 1) {     // thr at vorticity points, rb.f:854-875
-            if (i == 1)
-                r = 1.5 * (th[ix(2, k, ld)] + th[ix(2, ka, ld)] - th[ix(1, k, ld)] - th[ix(1, ka, ld)]) * dri
-                  - (-f23() * (th[ix(0, k, ld)] + th[ix(0, ka, ld)])
-                     + 0.5 * (th[ix(1, k, ld)] + th[ix(1, ka, ld)])
-                     + f16() * (th[ix(2, k, ld)] + th[ix(2, ka, ld)])) * dri;
-            else if (i == nr - 1)
-                r = 1.5 * (th[ix(nr, k, ld)] + th[ix(nr, ka, ld)] - th[ix(nr - 1, k, ld)] - th[ix(nr - 1, ka, ld)]) * dri
-                  - (-f16() * (th[ix(nr - 1, k, ld)] + th[ix(nr - 1, ka, ld)])
-                     - 0.5 * (th[ix(nr, k, ld)] + th[ix(nr, ka, ld)])
-                     + f23() * (th[ix(nr + 1, k, ld)] + th[ix(nr + 1, ka, ld)])) * dri;
-            else
-                r = 1.5 * (th[ix(i + 1, k, ld)] + th[ix(i + 1, ka, ld)] - th[ix(i, k, ld)] - th[ix(i, ka, ld)]) * dri;
-        } else {                   // thr at temperature points, rb.f:1244-1262
-            if (i == 1)
-                r = dri * (0.533333333 * th[ix(0, k, ld)] - 3.6 * th[ix(1, k, ld)]
-                           + 2.966666667 * th[ix(2, k, ld)] + 0.1 * th[ix(4, k, ld)]);
-            else if (i == nr)
-                r = dri * (-0.1 * th[ix(nr - 3, k, ld)] - 2.966666667 * th[ix(nr - 1, k, ld)]
-                           + 3.6 * th[ix(nr, k, ld)] - 0.533333333 * th[ix(nr + 1, k, ld)]);
-            else
-                r = 3.0 * (th[ix(i + 1, k, ld)] - th[ix(i - 1, k, ld)]) * dri;
-        }
-        if (j > 0) r = r - fact[j] * rprev;
-        out[ix(i, k, ld)] = r;
-        rprev = r;
-    }
-    double y = rprev / bp[n - 1];
-    out[ix(n, k, ld)] = y;
-    for (int j = n - 2; j >= 0; --j) {
-        y = (out[ix(j + 1, k, ld)] - cc[j] * y) / bp[j];
-        out[ix(j + 1, k, ld)] = y;
+    if (FAM == 0) {
+        if (i == 1)
+            return 3.0 * (C[ix(2, k, ld)] - C[ix(0, k, ld)]) * dri
+                 - (-1.5 * C[ix(0, k, ld)] + 2.0 * C[ix(1, k, ld)] - 0.5 * C[ix(2, k, ld)]) * dri;
+        if (i == nr - 1)
+            return 3.0 * (C[ix(nr, k, ld)] - C[ix(nr - 2, k, ld)]) * dri
+                 - (0.5 * C[ix(nr - 2, k, ld)] - 2.0 * C[ix(nr - 1, k, ld)] + 1.5 * C[ix(nr, k, ld)]) * dri;
+        return 3.0 * (C[ix(i + 1, k, ld)] - C[ix(i - 1, k, ld)]) * dri;
+    } else if (FAM == 1) {
+        if (i == 1)
+            return 1.5 * (th[ix(2, k, ld)] + th[ix(2, ka, ld)] - th[ix(1, k, ld)] - th[ix(1, ka, ld)]) * dri
+                 - (-f23() * (th[ix(0, k, ld)] + th[ix(0, ka, ld)])
+                    + 0.5 * (th[ix(1, k, ld)] + th[ix(1, ka, ld)])
+                    + f16() * (th[ix(2, k, ld)] + th[ix(2, ka, ld)])) * dri;
+        if (i == nr - 1)
+            return 1.5 * (th[ix(nr, k, ld)] + th[ix(nr, ka, ld)] - th[ix(nr - 1, k, ld)] - th[ix(nr - 1, ka, ld)]) * dri
+                 - (-f16() * (th[ix(nr - 1, k, ld)] + th[ix(nr - 1, ka, ld)])
+                    - 0.5 * (th[ix(nr, k, ld)] + th[ix(nr, ka, ld)])
+                    + f23() * (th[ix(nr + 1, k, ld)] + th[ix(nr + 1, ka, ld)])) * dri;
+        return 1.5 * (th[ix(i + 1, k, ld)] + th[ix(i + 1, ka, ld)] - th[ix(i, k, ld)] - th[ix(i, ka, ld)]) * dri;
+    } else {
+        if (i == 1)
+            return dri * (0.533333333 * th[ix(0, k, ld)] - 3.6 * th[ix(1, k, ld)]
+                          + 2.966666667 * th[ix(2, k, ld)] + 0.1 * th[ix(4, k, ld)]);
+        if (i == nr)
+            return dri * (-0.1 * th[ix(nr - 3, k, ld)] - 2.966666667 * th[ix(nr - 1, k, ld)]
+                          + 3.6 * th[ix(nr, k, ld)] - 0.533333333 * th[ix(nr + 1, k, ld)]);
+        return 3.0 * (th[ix(i + 1, k, ld)] - th[ix(i - 1, k, ld)]) * dri;
     }
 }
 
-// ---- batched cyclic solve along phi (one thread per row i), the
-// reference ptdma (rb.f:3066-3116) on n = np+1 points.  Table layout
-// (1-based, stride n+1): f1 | f2 | a | b | c | fA fB fC.
+// Right-hand side of entry j (1-based, column k = j-1) of the cyclic systems
+// on np+1 points: FAM 3 Ch (rb.f:836-842), FAM 4 thh at vorticity points
+// (rb.f:884-891), FAM 5 thh at temperature points (rb.f:1276-1282); row i.
 template <int FAM>
-__device__ void ptdma_row(const CompactArgs &a, int i)
+__device__ __forceinline__ double rhs_p(const CompactArgs &a, int i, int j)
+{
+    const int np = a.np, ld = a.ld;
+    const double dpi = a.dpi;
+    const double *X = (FAM == 3) ? a.C : a.th;
+    const int k = j - 1;
+    if (FAM == 4) {
+        const int ka = (k == np) ? 1 : k + 1;
+        return 1.5 * (X[ix(i + 1, ka, ld)] + X[ix(i, ka, ld)] - X[ix(i + 1, k, ld)] - X[ix(i, k, ld)]) * dpi;
+    }
+    if (k == np || k == 0) return 3.0 * (X[ix(i, 1, ld)] - X[ix(i, np - 1, ld)]) * dpi;
+    return 3.0 * (X[ix(i, k + 1, ld)] - X[ix(i, k - 1, ld)]) * dpi;
+}
+
+// Batched Thomas along r, the reference tdma (rb.f:3046-3064) with its
+// elimination factors from TdmaTab (fact | bp | c): the workgroup computes
+// the right-hand sides of its T columns into LDS with coalesced loads, one
+// thread per column then runs the recurrences in the reference's order, and
+// the workgroup writes the solutions back.  LDS row stride nP is odd.
+template <int FAM>
+__device__ void tdma_tile(const CompactArgs &a, double *R, int T)
+{
+    const int nr = a.nr, ld = a.ld, np = a.np;
+    const int n = (FAM == 2) ? nr : nr - 1;
+    const int nP = n | 1;
+    const double *tab = (FAM == 2) ? a.tab_rT : a.tab_r1;
+    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n;
+    double *out = (FAM == 0) ? a.dCr : (FAM == 1) ? a.dthrC : a.dthrT;
+    const int k0 = blockIdx.x * T;
+    if (k0 > np) return;
+    const int tc = (np + 1 - k0 < T) ? np + 1 - k0 : T;
+    for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
+        const int c = idx / n, j = idx - c * n;
+        R[c * nP + j] = rhs_r<FAM>(a, j + 1, k0 + c);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < tc) {
+        double *r = R + threadIdx.x * nP;
+        double rp = r[0];
+        for (int j = 1; j < n; ++j) {
+            rp = r[j] - fact[j] * rp;
+            r[j] = rp;
+        }
+        double y = rp / bp[n - 1];
+        r[n - 1] = y;
+        for (int j = n - 2; j >= 0; --j) {
+            y = (r[j] - cc[j] * y) / bp[j];
+            r[j] = y;
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
+        const int c = idx / n, j = idx - c * n;
+        out[ix(j + 1, k0 + c, ld)] = R[c * nP + j];
+    }
+}
+
+// Batched cyclic solve along phi, the reference ptdma (rb.f:3066-3116) on
+// n = np+1 points with the factors of PtdmaTab (1-based, stride n+1:
+// f1 | f2 | a | b | c | fA fB fC).  LDS holds entry j of row c at
+// (j-1)*T + c, so the per-row recurrences of consecutive threads touch
+// consecutive words.
+template <int FAM>
+__device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
 {
     const int np = a.np, ld = a.ld;
     const int n = np + 1;
     const int s = n + 1;
-    const double *f1 = a.tab_p, *f2 = a.tab_p + s, *aa = a.tab_p + 2 * s;
-    const double *bb = a.tab_p + 3 * s, *cc = a.tab_p + 4 * s;
-    const double fA = a.tab_p[5 * s], fB = a.tab_p[5 * s + 1], fC = a.tab_p[5 * s + 2];
-    const double dpi = a.dpi;
-    const double *X = (FAM == 1) ? a.th : (FAM == 0 ? a.C : a.th);
-    double *out = (FAM == 0) ? a.dCh : (FAM == 1) ? a.dthhC : a.dthhT;
-    // rhs of entry j (1-based), column k = j-1
-    auto rhs = [&](int j) -> double {
-        const int k = j - 1;
-        if (FAM == 1) {            // thh at vorticity points, rb.f:884-891
-            const int ka = (k == np) ? 1 : k + 1;
-            return 1.5 * (X[ix(i + 1, ka, ld)] + X[ix(i, ka, ld)] - X[ix(i + 1, k, ld)] - X[ix(i, k, ld)]) * dpi;
+    const int rows = (FAM == 5) ? a.nr : a.nr - 1;   // rows 1..rows
+    const int i0 = 1 + blockIdx.x * T;
+    if (i0 > rows) return;
+    const int tc = (rows + 1 - i0 < T) ? rows + 1 - i0 : T;
+    double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
+    for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
+        const int j = idx / tc, c = idx - j * tc;
+        R[j * T + c] = rhs_p<FAM>(a, i0 + c, j + 1);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < tc) {
+        const double *f1 = a.tab_p, *f2 = a.tab_p + s, *aa = a.tab_p + 2 * s;
+        const double *bb = a.tab_p + 3 * s, *cc = a.tab_p + 4 * s;
+        const double fA = a.tab_p[5 * s], fB = a.tab_p[5 * s + 1], fC = a.tab_p[5 * s + 2];
+        const int c = threadIdx.x;
+#define R_(j) R[((j) - 1) * T + c]
+        double rn = R_(n);
+        double rm = R_(1);
+        for (int j = 2; j <= n - 2; ++j) {
+            const double r = R_(j) - f1[j] * rm;
+            rn = rn - f2[j] * rm;
+            R_(j) = r;
+            rm = r;
         }
-        // Ch (rb.f:836-842), thh at temperature points (rb.f:1276-1282)
-        if (k == np || k == 0) return 3.0 * (X[ix(i, 1, ld)] - X[ix(i, np - 1, ld)]) * dpi;
-        return 3.0 * (X[ix(i, k + 1, ld)] - X[ix(i, k - 1, ld)]) * dpi;
-    };
-#define R_(j) out[ix(i, (j) - 1, ld)]
-    double rn = rhs(n);
-    double rm = rhs(1);
-    R_(1) = rm;
-    for (int j = 2; j <= n - 2; ++j) {
-        double r = rhs(j) - f1[j] * rm;
-        rn = rn - f2[j] * rm;
-        R_(j) = r;
-        rm = r;
-    }
-    double rn1 = rhs(n - 1) - fA * R_(n - 2);
-    rn = rn - fB * R_(n - 2);
-    rn = rn - fC * rn1;
-    const double yn = rn / bb[n];
-    double y = (rn1 - cc[n - 1] * yn) / bb[n - 1];
-    R_(n) = yn;
-    R_(n - 1) = y;
-    for (int j = n - 2; j >= 1; --j) {
-        y = (R_(j) - cc[j] * y - aa[j] * yn) / bb[j];
-        R_(j) = y;
-    }
+        const double rn1 = R_(n - 1) - fA * R_(n - 2);
+        rn = rn - fB * R_(n - 2);
+        rn = rn - fC * rn1;
+        const double yn = rn / bb[n];
+        double y = (rn1 - cc[n - 1] * yn) / bb[n - 1];
+        R_(n) = yn;
+        R_(n - 1) = y;
+        for (int j = n - 2; j >= 1; --j) {
+            y = (R_(j) - cc[j] * y - aa[j] * yn) / bb[j];
+            R_(j) = y;
+        }
 #undef R_
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
+        const int j = idx / tc, c = idx - j * tc;
+        out[ix(i0 + c, j, ld)] = R[j * T + c];
+    }
 }
 
-__global__ void __launch_bounds__(WG) k_compact(CompactArgs a)
+struct CompactTiles {
+    int Tr, Tp;    // columns per workgroup (r families), rows per workgroup (phi families)
+};
+
+__global__ void __launch_bounds__(WG) k_compact(CompactArgs a, CompactTiles t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ double R[];
     const int fam = blockIdx.y;
     if (!(a.mask & (1 << fam))) return;
     switch (fam) {
-    case 0: if (t <= a.np) tdma_column<0>(a, t); break;
-    case 1: if (t <= a.np) tdma_column<1>(a, t); break;
-    case 2: if (t <= a.np) tdma_column<2>(a, t); break;
-    case 3: if (t + 1 <= a.nr - 1) ptdma_row<0>(a, t + 1); break;
-    case 4: if (t + 1 <= a.nr - 1) ptdma_row<1>(a, t + 1); break;
-    case 5: if (t + 1 <= a.nr) ptdma_row<2>(a, t + 1); break;
+    case 0: tdma_tile<0>(a, R, t.Tr); break;
+    case 1: tdma_tile<1>(a, R, t.Tr); break;
+    case 2: tdma_tile<2>(a, R, t.Tr); break;
+    case 3: ptdma_tile<3>(a, R, t.Tp); break;
+    case 4: ptdma_tile<4>(a, R, t.Tp); break;
+    case 5: ptdma_tile<5>(a, R, t.Tp); break;
     }
 }
 
@@ -458,27 +509,7 @@ inline int nblk(long long n, int wg) { return (int)((n + wg - 1) / wg); }
 
 }  // namespace
 
-int launch_compact(vv_run *r)
-{
-    CompactArgs a;
-    a.C = r->C; a.th = r->th;
-    a.dCr = r->dCr; a.dCh = r->dCh; a.dthrC = r->dthrC; a.dthhC = r->dthhC;
-    a.dthrT = r->dthrT; a.dthhT = r->dthhT;
-    a.tab_r1 = r->tab_r1; a.tab_rT = r->tab_rT; a.tab_p = r->tab_p;
-    a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.mask = 0x3f;
-    a.dri = 1.0 / r->dr; a.dpi = 1.0 / r->dp;
-    const int len = (r->np + 1 > r->nr) ? r->np + 1 : r->nr;
-    dim3 grid(nblk(len, 64), 6);
-    hipEvent_t pe_;
-    prof_begin(r, &pe_);
-    hipLaunchKernelGGL(k_compact, grid, dim3(64), 0, r->stream, a);
-    // read C and th, write six derivative fields
-    prof_end(r, VV_K_COMPACT, pe_, 8.0 * 8.0 * (double)(r->nr + 2) * (r->np + 1));
-    VV_HIP(hipGetLastError());
-    return VV_OK;
-}
-
-int launch_compact_mask(vv_run *r, int mask)
+static int compact_launch(vv_run *r, int mask)
 {
     CompactArgs a;
     a.C = r->C; a.th = r->th;
@@ -487,12 +518,30 @@ int launch_compact_mask(vv_run *r, int mask)
     a.tab_r1 = r->tab_r1; a.tab_rT = r->tab_rT; a.tab_p = r->tab_p;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.mask = mask;
     a.dri = 1.0 / r->dr; a.dpi = 1.0 / r->dp;
-    const int len = (r->np + 1 > r->nr) ? r->np + 1 : r->nr;
-    dim3 grid(nblk(len, 64), 6);
-    hipLaunchKernelGGL(k_compact, grid, dim3(64), 0, r->stream, a);
+    // tiles sized for 32 KB of LDS per workgroup
+    const int budget = 32768 / 8;
+    const int nP = r->nr | 1;
+    CompactTiles t;
+    t.Tr = budget / nP;
+    if (t.Tr > 64) t.Tr = 64;
+    if (t.Tr < 1) t.Tr = 1;
+    t.Tp = budget / (r->np + 1);
+    if (t.Tp > 64) t.Tp = 64;
+    if (t.Tp < 1) t.Tp = 1;
+    const size_t lds = (size_t)budget * 8;
+    const int gr = nblk(r->np + 1, t.Tr), gp = nblk(r->nr, t.Tp);
+    dim3 grid(gr > gp ? gr : gp, 6);
+    hipEvent_t pe_;
+    prof_begin(r, &pe_);
+    hipLaunchKernelGGL(k_compact, grid, dim3(WG), lds, r->stream, a, t);
+    // read C and th, write six derivative fields
+    prof_end(r, VV_K_COMPACT, pe_, 8.0 * 8.0 * (double)(r->nr + 2) * (r->np + 1));
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
+
+int launch_compact(vv_run *r) { return compact_launch(r, 0x3f); }
+int launch_compact_mask(vv_run *r, int mask) { return compact_launch(r, mask); }
 
 int launch_df(vv_run *r, int first, int do_c, int do_t)
 {
