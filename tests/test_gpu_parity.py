@@ -196,11 +196,13 @@ def test_trajectory_against_reference_fixture(golden_dir, name, snap):
 
 def test_annulus_trajectory_300_steps():
     """Rotating-annulus plane (velvor2dnewsch3.4.f), 65x129 nodes, Ra=1e7,
-    Re=sqrt(1e5), dt=1e-6, 300 steps at 1e-10: relative L2 <= 1e-9 against
-    the oracle with exact solves."""
+    Re=sqrt(1e5), dt=1e-6, 300 steps: relative L2 <= 1e-9 against the oracle
+    with exact solves.  The solves run at 1e-12: at 1e-10 the azimuthal
+    velocity differs by 3.8e-9 (measured), the solver tolerance times the
+    conditioning of the annulus operator."""
     nr, np_ = 64, 128
     Re = float(np.sqrt(1e5))
-    run, box = _pair(1, nr, np_, dt=1e-6, Ra=1e7, Re=Re, rtol=1e-10, max_cycles=60)
+    run, box = _pair(1, nr, np_, dt=1e-6, Ra=1e7, Re=Re, rtol=1e-12, max_cycles=60)
     run.seed()
     box.put("th", run["th"])
     box.start()
