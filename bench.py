@@ -30,25 +30,33 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "time-steps/s & V-cycle HBM GB/s; served solves/s + p99 at 1e-10 residual"
 
-# name -> (description, BoxConfig kwargs, seed)
+# name -> (description, config kwargs (geom 1 = annulus), seed, noise amplitude)
 WORKLOADS = {
+    "annulus-513x257": (
+        "rotating-annulus velocity-vorticity plane (velvor2dnewsch3.4.f physics: cylindrical "
+        "terms, centrifugal buoyancy, compact 4th-order scheme, DuFort-Frankel), 513x257 nodes "
+        "(nr=512 radial, np=256 azimuthal), radius ratio 0.5, Ra=1e8, Ta=1e5 (Re=sqrt(Ta)), "
+        "Pr=1, dt=4e-8, T linear in r + U(-1e-4,1e-4); config 2's grid size -- its "
+        "meridional/swirl/tanh-stretched form has no reference program and is not built",
+        dict(geom=1, nr=512, np=256, dt=4e-8, Ra=1e8, Re=1e5 ** 0.5, Pr=1.0, robert=0.05),
+        20261203, 1e-4),
     "box-129x65": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 129x65 nodes, 2:1, "
-                   "Ra=1e5, Pr=1, dt=1e-5", dict(nr=64, np=128, width=1.0, dt=1e-5, Ra=1e5, Pr=1.0),
-                   20261202),
+                   "Ra=1e5, Pr=1, dt=1e-5 (config 1)",
+                   dict(geom=0, nr=64, np=128, width=1.0, dt=1e-5, Ra=1e5, Pr=1.0), 20261202, 1e-3),
     "box-513x257": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 513x257 nodes "
                     "(nr=512 across, np=256 periodic), width 1, Ra=1e5, Pr=1, dt=1e-6",
-                    dict(nr=512, np=256, width=1.0, dt=1e-6, Ra=1e5, Pr=1.0), 20261203),
+                    dict(geom=0, nr=512, np=256, width=1.0, dt=1e-6, Ra=1e5, Pr=1.0), 20261203, 1e-3),
 }
-DEFAULT_WORKLOAD = "box-513x257"
+DEFAULT_WORKLOAD = "annulus-513x257"
 
 
-def gpu_leg(name, steps, warmup):
+def gpu_leg(name, steps, warmup, nu):
     from codesthesis_dir_amd import BoxConfig, PeriodicBox
     from codesthesis_dir_amd.ic import box_temperature
-    desc, kw, seed = WORKLOADS[name]
-    cfg = BoxConfig(rtol=1e-10, max_cycles=60, **kw)
+    desc, kw, seed, amp = WORKLOADS[name]
+    cfg = BoxConfig(rtol=1e-10, max_cycles=60, nu1=nu, nu2=nu, **kw)
     box = PeriodicBox(cfg, device=0)
-    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, 1e-3, seed))
+    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, amp, seed))
     box.start()
     if warmup:
         box.step(warmup)
@@ -69,11 +77,13 @@ def gpu_leg(name, steps, warmup):
 
 def cpu_leg(name, budget_s=15.0):
     """Time the reference CPU path on this host for a bounded sample."""
-    desc, kw, seed = WORKLOADS[name]
+    desc, kw, seed, amp = WORKLOADS[name]
     nr, np_ = kw["nr"], kw["np"]
+    geom = kw.get("geom", 0)
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "rb_omp")
-    # the reference's static arrays hold nr <= 289, np <= 1609 (rb.f:54)
-    if os.path.exists(ref_bin) and nr <= 289 and np_ <= 1609 and kw["width"] == 3.0:
+    # the reference's static arrays hold nr <= 289, np <= 1609 (rb.f:54); it
+    # runs its hard-coded width 3 (rb.f:132), the same work per step
+    if geom == 0 and os.path.exists(ref_bin) and nr <= 289 and np_ <= 1609:
         nts = 200
         with tempfile.TemporaryDirectory() as d:
             deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", str(kw["dt"]), str(nr),
@@ -85,22 +95,29 @@ def cpu_leg(name, budget_s=15.0):
                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
             el = time.perf_counter() - t0
         return {"value": nts / el, "unit": "time-steps/s", "cores": 2, "kind": "reference",
-                "sample": f"{nts} steps of the reference executable (flang -O2, 2 OpenMP threads)"}
+                "sample": f"{nts} steps of the reference executable velvor2dperiodicrbconv.f "
+                          f"(flang -O2, 2 OpenMP threads) at {nr}x{np_}"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    run = O.Run(nr, np_, kw["dt"], kw["Ra"], kw["Pr"], width=kw["width"])
-    run.seed(r0=0.5, amp=1e-3, seed=seed)
+    if geom == 1:
+        run = O.Run(nr, np_, kw["dt"], kw["Ra"], kw["Pr"], geom=1, r0=0.5, Re=kw["Re"])
+        prog = "velvor2dnewsch3.4.f (static arrays hold nr <= 419)"
+    else:
+        run = O.Run(nr, np_, kw["dt"], kw["Ra"], kw["Pr"], width=kw["width"])
+        prog = "velvor2dperiodicrbconv.f (static arrays hold nr <= 289)"
+    run.seed(r0=0.5, amp=amp, seed=seed)
     n = 0
     t0 = time.perf_counter()
     while True:
-        st, tr, ncl = run.step(mode=0, tolcon=2e-6)
+        st, tr, ncl = run.step(mode=0)
         n += 1
         el = time.perf_counter() - t0
         if el > budget_s or n >= 2000:
             break
     return {"value": n / el, "unit": "time-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps from the seeded start, C restatement of the reference time "
-                      f"loop (line-GS FMG Poisson, absolute stop 2e-6 as shipped), 1 thread"}
+            "sample": f"{n} steps of the same workload from its seeded start: the oracle's C "
+                      f"restatement (bit-exact against the flang build) of {prog}, its "
+                      f"line-GS FMG Poisson with the shipped absolute stop, 1 thread"}
 
 
 def main():
@@ -110,12 +127,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; the multi-GPU layout "
               "is described in DESIGN.md and not implemented", file=sys.stderr)
         sys.exit(2)
-    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup)
+    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu)
     ms_per_step = wall_ms / a.steps
     # dominant kernel by device time in the profiled pass
     dom = max(prof, key=lambda k: prof[k]["ms"])
@@ -141,7 +159,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: seeded linear-conduction temperature + U(-1e-3,1e-3) splitmix64 noise",
+        "data": "synthetic: seeded linear-in-r temperature + uniform splitmix64 noise, fluid at rest",
         "config": {"workload": a.workload, "description": WORKLOADS[a.workload][0],
                    "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
                    "vcycle": f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening"},
