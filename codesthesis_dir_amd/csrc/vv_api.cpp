@@ -317,7 +317,9 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rC, rc))) return fail(st);
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
-    if (hipMalloc(&r->mg_flags, 4 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
+    if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 32;
+    if (hipMalloc(&r->mg_flags, 6 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMemsetAsync(r->mg_flags, 0, 6 * sizeof(int), r->stream) != hipSuccess) return fail(VV_EHIP);
     if (hipMalloc(&r->mg_bits, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMalloc(&r->mg_relres, 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if ((st = mg_build(r))) return fail(st);

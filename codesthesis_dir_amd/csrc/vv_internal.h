@@ -61,6 +61,13 @@ struct MgLevelDev {
     const double *lu;
     double cphi;          // 1/hp^2 at this level
     int ncol, nstep;
+    // the coarsest level is solved exactly: lu then holds one PCR table per
+    // phi mode m = 0..ncol/2 (stride tabsz) followed by the twiddles
+    // cos|sin(2 pi j/ncol); xm is the (ncol x n) mode scratch
+    int direct;
+    size_t tabsz;
+    double *xm;
+    const double *twid;
 };
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;
@@ -79,6 +86,7 @@ struct MgBatchDev {
     MgProblemDev prob[MG_MAXPROB];
     int *done;            // [nprob] converged / capped flag
     int *cycles;          // [nprob]
+    int *count;           // [nprob] blocks of the norm kernel finished
     unsigned long long *res;    // [nprob] max |r| bits (non-negative doubles)
     unsigned long long *fnorm;  // [nprob] max |f| bits
     double *relres;       // [nprob]
@@ -119,7 +127,7 @@ struct vv_run {
     vv::MgBatchDev mg{};
     vv::MgBatchDev *mg_dev = nullptr;
     double *mg_pool = nullptr;
-    int *mg_flags = nullptr;    // done[2], cycles[2]
+    int *mg_flags = nullptr;    // done[2], cycles[2], count[2]
     unsigned long long *mg_bits = nullptr;  // res[2], fnorm[2]
     double *mg_relres = nullptr;
     int nlev_v = 0, nlev_u = 0;

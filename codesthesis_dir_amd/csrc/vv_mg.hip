@@ -12,8 +12,11 @@
 //     parallel cyclic reduction in LDS, one workgroup per line, with the
 //     reduction coefficients of the level's line operator precomputed on the
 //     host (the closure rows are first folded into a tridiagonal system);
-//   - coarsening: phi only, full weighting / linear interpolation, down to a
-//     single column, where the line solve is the exact coarse solve.
+//   - coarsening: phi only, full weighting / linear interpolation, down to
+//     direct_cols columns or fewer; that level is solved exactly: the
+//     operator is invariant along phi, so a real DFT over the level's
+//     columns decouples it into one line system per Fourier mode
+//     (k_mg_direct: project, reduce; k_mg_idft: synthesise).
 // Both velocity problems of a run are batched into every launch
 // (blockIdx.y = problem).  Kernels return at once when their problem has
 // converged, so several cycles can be queued between host checks.
@@ -152,8 +155,88 @@ __global__ void __launch_bounds__(256) k_mg_prolong(const MgBatchDev *__restrict
     F.u[cix(j, 2 * qc, n)] += 0.5 * (e + em);
 }
 
-// max-norm of the level-0 residual
-__global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp)
+// Exact solve of the coarsest level with C = ncol columns: one workgroup per
+// real Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
+// ml = C-1: the Nyquist mode), the projection of f on that mode, then the
+// mode's line system D_r + lambda_m diag(rphi) by parallel cyclic reduction.
+__global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict__ Bp, int lev)
+{
+    const MgBatchDev &B = *Bp;
+    const int p = blockIdx.y;
+    if (p >= B.nprob || B.done[p]) return;
+    const MgProblemDev &P = B.prob[p];
+    if (lev >= P.nlev) return;
+    const MgLevelDev &L = P.lev[lev];
+    const int n = P.n, C = L.ncol;
+    const int ml = blockIdx.x;
+    if (ml >= C) return;
+    int m, sinpart;
+    if (ml == 0) { m = 0; sinpart = 0; }
+    else if (C >= 2 && ml == C - 1) { m = C / 2; sinpart = 0; }
+    else { m = (ml + 1) / 2; sinpart = (ml & 1) ? 0 : 1; }
+    __shared__ double buf[2][MAXN];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double *f = L.f;
+    const double *tw = L.twid + (sinpart ? C : 0);
+    for (int i = tid; i < n; i += nt) {
+        double acc = 0.0;
+        for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) % C];
+        buf[0][i] = acc;
+    }
+    __syncthreads();
+    const double *tab = L.lu + (size_t)m * L.tabsz;
+    const int S = L.nstep;
+    if (tid == 0) {
+        const double r0 = buf[0][0] - tab[0] * buf[0][1];
+        const double rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
+        buf[0][0] = r0;
+        buf[0][n - 1] = rN;
+    }
+    __syncthreads();
+    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
+    int cur = 0;
+    for (int t = 0; t < S; ++t) {
+        const int s = 1 << t;
+        const double *a = alpha + (size_t)t * n, *b = beta + (size_t)t * n;
+        for (int i = tid; i < n; i += nt) {
+            double v = buf[cur][i];
+            if (i >= s) v += a[i] * buf[cur][i - s];
+            if (i + s < n) v += b[i] * buf[cur][i + s];
+            buf[cur ^ 1][i] = v;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int i = tid; i < n; i += nt) L.xm[(size_t)ml * n + i] = buf[cur][i] * dinv[i];
+}
+
+// synthesis of the coarsest-level solution from its mode lines
+__global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ Bp, int lev)
+{
+    const MgBatchDev &B = *Bp;
+    const int p = blockIdx.y;
+    if (p >= B.nprob || B.done[p]) return;
+    const MgProblemDev &P = B.prob[p];
+    if (lev >= P.nlev) return;
+    const MgLevelDev &L = P.lev[lev];
+    const int n = P.n, C = L.ncol;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * C) return;
+    const int q = (int)(t / n), i = (int)(t - (long long)q * n);
+    const double *X = L.xm;
+    const double *cs = L.twid, *sn = L.twid + C;
+    double v = X[i];
+    for (int m = 1; m < (C + 1) / 2; ++m) {
+        const int j = (m * q) % C;
+        v += 2.0 * (X[(size_t)(2 * m - 1) * n + i] * cs[j] + X[(size_t)(2 * m) * n + i] * sn[j]);
+    }
+    if (C >= 2) v += X[(size_t)(C - 1) * n + i] * ((q & 1) ? -1.0 : 1.0);
+    L.u[cix(i, q, n)] = v / (double)C;
+}
+
+// max-norm of the level-0 residual; the last block of the problem applies
+// the stop test (relative residual <= rtol, or the cycle cap)
+__global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp, double rtol, int maxc)
 {
     const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
@@ -171,21 +254,23 @@ __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ 
         m = (o > m || o != o) ? o : m;
     }
     if ((threadIdx.x & 63) == 0) atomicMax(&B.res[p], (unsigned long long)__double_as_longlong(m));
-}
-
-__global__ void k_mg_check(const MgBatchDev *__restrict__ Bp, double rtol, int maxc)
-{
-    const MgBatchDev &B = *Bp;
-    const int p = threadIdx.x;
-    if (p >= B.nprob || B.done[p]) return;
-    const double r = __longlong_as_double((long long)B.res[p]);
-    const double fn = __longlong_as_double((long long)B.fnorm[p]);
-    const double rel = r / (fn > 0.0 ? fn : 1.0);
-    const int c = B.cycles[p] + 1;
-    B.cycles[p] = c;
-    B.relres[p] = rel;
-    B.res[p] = 0ull;
-    if (!(rel > rtol) || c >= maxc) B.done[p] = 1;    // NaN also stops
+    __threadfence();
+    __syncthreads();
+    __shared__ int last;
+    if (threadIdx.x == 0) last = (atomicAdd(&B.count[p], 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const double r = __longlong_as_double((long long)atomicMax(&B.res[p], 0ull));
+        const double fn = __longlong_as_double((long long)B.fnorm[p]);
+        const double rel = r / (fn > 0.0 ? fn : 1.0);
+        const int c = B.cycles[p] + 1;
+        B.cycles[p] = c;
+        B.relres[p] = rel;
+        atomicExch(&B.res[p], 0ull);
+        atomicExch(&B.count[p], 0);
+        if (!(rel > rtol) || c >= maxc) B.done[p] = 1;    // NaN also stops
+    }
 }
 
 __global__ void k_mg_reset(const MgBatchDev *__restrict__ Bp, int which)
@@ -196,6 +281,7 @@ __global__ void k_mg_reset(const MgBatchDev *__restrict__ Bp, int which)
     const bool active = (which == 5) || (which == 2 && p == 0) || (which == 3 && p == 1);
     B.done[p] = active ? 0 : 1;
     B.cycles[p] = 0;
+    B.count[p] = 0;
     B.res[p] = 0ull;
     B.fnorm[p] = 0ull;
     B.relres[p] = 0.0;
@@ -287,15 +373,13 @@ int mg_build(vv_run *r)
     const int nr = r->nr, np = r->np;
     const double hhri = 1.0 / (r->dr * r->dr);
     if (nr > MAXN) { set_error("nr exceeds the line length held in LDS (2048)"); return VV_EINVAL; }
-    int nlev = 0;
-    for (int c = np; c >= 1; c /= 2) {
-        ++nlev;
-        if (c == 1) break;
-    }
+    const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 32;
+    int nlev = 1;
+    for (int c = np; c > Cd; c /= 2) ++nlev;
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
     const int nvec[2] = {nr, nr - 1};
     // host-side tables first, then one pool
-    struct LevHost { std::vector<double> tab; int S; double cphi; int ncol; };
+    struct LevHost { std::vector<double> tab; int S; double cphi; int ncol; int direct; size_t tabsz; };
     std::vector<LevHost> lh[2];
     std::vector<double> sten_h[2], rphi_h[2];
     size_t total = 0;
@@ -338,11 +422,30 @@ int mg_build(vv_run *r)
             LevHost L;
             L.ncol = c;
             L.cphi = 1.0 / (hp * hp);
-            std::vector<double> d2(dg);
-            if (c > 1)
+            L.direct = (l == nlev - 1);
+            if (!L.direct) {
+                std::vector<double> d2(dg);
                 for (int j = 0; j < n; ++j) d2[j] = dg[j] - 2.0 * L.cphi * rphi[j];
-            L.S = make_pcr(n, s2, s1, d2, p1, p2, L.tab);
-            total += 2 * (size_t)n * c + L.tab.size();
+                L.S = make_pcr(n, s2, s1, d2, p1, p2, L.tab);
+                L.tabsz = L.tab.size();
+                total += 2 * (size_t)n * c + L.tab.size();
+            } else {
+                // one table per Fourier mode m = 0..c/2: the phi second
+                // difference on c periodic points has eigenvalue
+                // (2 cos(2 pi m/c) - 2)/hp^2
+                const double pi = std::acos(-1.0);
+                for (int m = 0; m <= c / 2; ++m) {
+                    const double lam = (2.0 * std::cos(2.0 * pi * m / c) - 2.0) * L.cphi;
+                    std::vector<double> d2(dg), t;
+                    for (int j = 0; j < n; ++j) d2[j] = dg[j] + lam * rphi[j];
+                    L.S = make_pcr(n, s2, s1, d2, p1, p2, t);
+                    L.tabsz = t.size();
+                    L.tab.insert(L.tab.end(), t.begin(), t.end());
+                }
+                for (int j = 0; j < c; ++j) L.tab.push_back(std::cos(2.0 * pi * j / c));
+                for (int j = 0; j < c; ++j) L.tab.push_back(std::sin(2.0 * pi * j / c));
+                total += 3 * (size_t)n * c + L.tab.size();
+            }
             lh[p].push_back(std::move(L));
         }
     }
@@ -367,17 +470,23 @@ int mg_build(vv_run *r)
             L.ncol = H.ncol;
             L.cphi = H.cphi;
             L.nstep = H.S;
+            L.direct = H.direct;
+            L.tabsz = H.tabsz;
             L.u = r->mg_pool + off; off += (size_t)n * H.ncol;
             L.f = r->mg_pool + off; off += (size_t)n * H.ncol;
             std::memcpy(host.data() + off, H.tab.data(), H.tab.size() * sizeof(double));
             L.lu = r->mg_pool + off;
+            L.twid = H.direct ? L.lu + (size_t)(H.ncol / 2 + 1) * H.tabsz : nullptr;
             off += H.tab.size();
+            L.xm = nullptr;
+            if (H.direct) { L.xm = r->mg_pool + off; off += (size_t)n * H.ncol; }
         }
     }
     VV_HIP(hipMemcpyAsync(r->mg_pool, host.data(), total * sizeof(double), hipMemcpyHostToDevice,
                           r->stream));
     r->mg.done = r->mg_flags;
     r->mg.cycles = r->mg_flags + 2;
+    r->mg.count = r->mg_flags + 4;
     r->mg.res = r->mg_bits;
     r->mg.fnorm = r->mg_bits + 2;
     r->mg.relres = r->mg_relres;
@@ -433,7 +542,16 @@ static int enqueue_cycle(vv_run *r)
         hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, l);
         prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncf + 2.0 * ncc));
     }
-    zebra(nlev - 1, 0);
+    {   // exact solve of the coarsest level
+        const int L = nlev - 1;
+        const int C = B.prob[0].lev[L].ncol;
+        prof_begin(r, &pe);
+        hipLaunchKernelGGL(k_mg_direct, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
+        prof_end(r, VV_K_ZEBRA, pe, 16.0 * ntot * C);
+        prof_begin(r, &pe);
+        hipLaunchKernelGGL(k_mg_idft, dim3(nblk((long long)n * C, 256), 2), dim3(256), 0, r->stream, Bd, L);
+        prof_end(r, VV_K_PROLONG, pe, 16.0 * ntot * C);
+    }
     for (int l = nlev - 2; l >= 0; --l) {
         const int ncf = B.prob[0].lev[l].ncol, ncc = B.prob[0].lev[l + 1].ncol;
         const long long cnt = (long long)n * ncc;
@@ -446,11 +564,9 @@ static int enqueue_cycle(vv_run *r)
     const int nc0 = B.prob[0].lev[0].ncol;
     const long long cnt0 = (long long)n * nc0;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nblk(cnt0, 256), 2), dim3(256), 0, r->stream, Bd);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nblk(cnt0, 256), 2), dim3(256), 0, r->stream, Bd, rtol,
+                       r->cfg.max_cycles);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
-    prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, r->stream, Bd, rtol, r->cfg.max_cycles);
-    prof_end(r, VV_K_CHECK, pe, 0.0);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

@@ -35,6 +35,7 @@ module vv_dropin_c
     real(c_double) :: dt, r0, rmax, width, Ra, Pr, robert, rtol
     integer(c_int) :: max_cycles, nu1, nu2, stats, geom
     real(c_double) :: Re
+    integer(c_int) :: direct_cols
   end type
 
   integer(c_int), parameter :: VV_C = 0, VV_CP = 1, VV_CN = 2, VV_TH = 3, VV_THP = 4, &
@@ -161,6 +162,7 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%stats = 0
   cfg%geom = 0
   cfg%Re = 0.0d0
+  cfg%direct_cols = 32
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl
 end subroutine

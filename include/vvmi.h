@@ -69,6 +69,10 @@ typedef struct vv_config {
     int    stats;         /* 1: running means + CFL every step (rb.f:409-458) */
     int    geom;          /* VV_GEOM_BOX or VV_GEOM_ANNULUS                 */
     double Re;            /* rotational Reynolds number (annulus only)     */
+    int    direct_cols;   /* phi coarsening stops at <= this many columns,
+                             where the level is solved exactly (real DFT in
+                             phi + one line solve per mode); >= np makes the
+                             whole solve direct.  0 selects 32.             */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */

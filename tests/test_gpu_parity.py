@@ -142,6 +142,36 @@ def test_poisson_matches_exact_solution(nr, np_, width, which):
     box.close()
 
 
+@pytest.mark.parametrize("direct_cols", [1, 8, 4096])
+@pytest.mark.parametrize("geom", [0, 1])
+def test_poisson_coarse_solve_variants(direct_cols, geom):
+    """The exact coarse solve (real DFT over the coarsest level's columns,
+    one line solve per mode) at 1 column, at 8 columns, and on the fine
+    level itself (direct_cols >= np: the whole solve is direct)."""
+    nr, np_ = 96, 128
+    if geom:
+        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0, direct_cols=direct_cols)
+        dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
+    else:
+        box = _box(nr, np_, 2.0, rtol=1e-12, max_cycles=60, direct_cols=direct_cols)
+        dr, dp = 0.5 / nr, 2.0 / np_
+    rng = np.random.default_rng(11)
+    rhs = np.ascontiguousarray(rng.standard_normal((np_ + 1, nr + 2)))
+    rhs[0] = rhs[np_]
+    for which in (2, 3):
+        ni = nr + 2 if which == 2 else nr + 1
+        psi, psip = np.zeros_like(rhs), np.zeros_like(rhs)
+        st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
+        assert st == 0 and rel <= 1e-12, (st, cyc, rel)
+        if direct_cols >= np_:
+            assert cyc == 1
+        ex = np.zeros_like(psi)
+        O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
+        err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
+        assert err < 1e-10, (which, err)
+    box.close()
+
+
 def test_trajectory_1000_steps_config1():
     """Periodic RB box 129x65 nodes (nr=64, np=128), 2:1 (width 1, height
     0.5), Ra=1e5, Pr=1, dt=1e-5, seeded temperature, 1000 steps with the
