@@ -157,8 +157,11 @@ __global__ void __launch_bounds__(256) k_mg_prolong(const MgBatchDev *__restrict
 
 // Exact solve of the coarsest level with C = ncol columns: one workgroup per
 // real Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
-// ml = C-1: the Nyquist mode), the projection of f on that mode, then the
-// mode's line system D_r + lambda_m diag(rphi) by parallel cyclic reduction.
+// ml = C-1: the Nyquist mode), the projection of the residual on that mode,
+// then the mode's line system D_r + lambda_m diag(rphi) by parallel cyclic
+// reduction.  Below level 0 the correction starts from zero, so the residual
+// is f; on level 0 (the whole solve is direct) it is f - Lu and each cycle is
+// one step of iterative refinement.
 __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict__ Bp, int lev)
 {
     const MgBatchDev &B = *Bp;
@@ -180,7 +183,11 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
     const double *tw = L.twid + (sinpart ? C : 0);
     for (int i = tid; i < n; i += nt) {
         double acc = 0.0;
-        for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) % C];
+        if (lev == 0) {
+            for (int q = 0; q < C; ++q) acc += resid_at(P, L, i, q) * tw[(m * q) % C];
+        } else {
+            for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) % C];
+        }
         buf[0][i] = acc;
     }
     __syncthreads();
@@ -210,7 +217,7 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
     for (int i = tid; i < n; i += nt) L.xm[(size_t)ml * n + i] = buf[cur][i] * dinv[i];
 }
 
-// synthesis of the coarsest-level solution from its mode lines
+// synthesis of the coarsest-level correction from its mode lines, added
 __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ Bp, int lev)
 {
     const MgBatchDev &B = *Bp;
@@ -231,7 +238,7 @@ __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ 
         v += 2.0 * (X[(size_t)(2 * m - 1) * n + i] * cs[j] + X[(size_t)(2 * m) * n + i] * sn[j]);
     }
     if (C >= 2) v += X[(size_t)(C - 1) * n + i] * ((q & 1) ? -1.0 : 1.0);
-    L.u[cix(i, q, n)] = v / (double)C;
+    L.u[cix(i, q, n)] += v / (double)C;
 }
 
 // max-norm of the level-0 residual; the last block of the problem applies

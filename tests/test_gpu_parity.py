@@ -163,8 +163,8 @@ def test_poisson_coarse_solve_variants(direct_cols, geom):
         psi, psip = np.zeros_like(rhs), np.zeros_like(rhs)
         st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
         assert st == 0 and rel <= 1e-12, (st, cyc, rel)
-        if direct_cols >= np_:
-            assert cyc == 1
+        if direct_cols >= np_:     # direct solve plus iterative refinement
+            assert cyc <= 3, cyc
         ex = np.zeros_like(psi)
         O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
         err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
