@@ -140,7 +140,7 @@ def main():
     d = prof[dom]
     avg_ms = d["ms"] / max(d["launches"], 1)
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    vcyc = {k: prof[k] for k in ("zebra", "restrict", "prolong", "norm")}
+    vcyc = {k: prof[k] for k in ("zebra", "restrict", "prolong", "norm", "direct")}
     vc_ms = sum(v["ms"] for v in vcyc.values())
     vc_bytes = sum(v["bytes"] for v in vcyc.values())
     traffic = None

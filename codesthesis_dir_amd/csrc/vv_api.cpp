@@ -318,10 +318,11 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
     if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 32;
-    if (hipMalloc(&r->mg_flags, 6 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMemsetAsync(r->mg_flags, 0, 6 * sizeof(int), r->stream) != hipSuccess) return fail(VV_EHIP);
-    if (hipMalloc(&r->mg_bits, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMalloc(&r->mg_relres, 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->mg_state, 128) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMemsetAsync(r->mg_state, 0, 128, r->stream) != hipSuccess) return fail(VV_EHIP);
+    r->mg_flags = static_cast<int *>(r->mg_state);
+    r->mg_bits = reinterpret_cast<unsigned long long *>(static_cast<char *>(r->mg_state) + 32);
+    r->mg_relres = reinterpret_cast<double *>(static_cast<char *>(r->mg_state) + 64);
     if ((st = mg_build(r))) return fail(st);
     r->diag_cap = 1 << 16;
     if (hipMalloc(&r->diag, (size_t)r->diag_cap * 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
@@ -344,12 +345,11 @@ int vv_destroy(vv_run *r)
     double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                         r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
                         r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                        r->tab_p, r->mg_pool, r->mg_relres, r->diag};
+                        r->tab_p, r->mg_pool, r->diag};
     for (double *f : fields)
         if (f) (void)hipFree(f);
     if (r->mg_dev) (void)hipFree(r->mg_dev);
-    if (r->mg_flags) (void)hipFree(r->mg_flags);
-    if (r->mg_bits) (void)hipFree(r->mg_bits);
+    if (r->mg_state) (void)hipFree(r->mg_state);
     if (r->diag_i) (void)hipFree(r->diag_i);
     if (r->cfl_bits) (void)hipFree(r->cfl_bits);
     for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);

@@ -127,8 +127,13 @@ struct vv_run {
     vv::MgBatchDev mg{};
     vv::MgBatchDev *mg_dev = nullptr;
     double *mg_pool = nullptr;
-    int *mg_flags = nullptr;    // done[2], cycles[2], count[2]
-    unsigned long long *mg_bits = nullptr;  // res[2], fnorm[2]
+    // solver state, one device block read back with a single copy:
+    //   bytes 0-31 int done[2], cycles[2], count[2], pad[2];
+    //   32-63 u64 res[2], fnorm[2] (bits of non-negative doubles);
+    //   64-79 double relres[2]
+    void *mg_state = nullptr;
+    int *mg_flags = nullptr;
+    unsigned long long *mg_bits = nullptr;
     double *mg_relres = nullptr;
     int nlev_v = 0, nlev_u = 0;
 

@@ -56,9 +56,114 @@ __device__ __forceinline__ double resid_at(const MgProblemDev &P, const MgLevelD
     return L.f[cix(j, q, n)] - lu;
 }
 
-// One colour of r-lines of one level: line q = 2*blockIdx.x + colour (the
-// single column of the coarsest level solves D_r u = f).
-__global__ void __launch_bounds__(1024) k_mg_zebra(const MgBatchDev *__restrict__ Bp, int lev, int color)
+constexpr int MAXS = 11;       // cyclic-reduction steps for n <= MAXN
+
+// Line smoothing modes of k_mg_zebra.
+enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
+
+// Restricted residual of level lev-1 at coarse line qc, row j (full
+// weighting along phi, centred on fine line 2qc+1).
+__device__ __forceinline__ double restricted_at(const MgProblemDev &P, const MgLevelDev &F, int j, int qc)
+{
+    const int q0 = 2 * qc, q1 = 2 * qc + 1, q2 = (2 * qc + 2) % F.ncol;
+    const double r0 = resid_at(P, F, j, q0);
+    const double r1 = resid_at(P, F, j, q1);
+    const double r2 = resid_at(P, F, j, q2);
+    return 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
+}
+
+// Right-hand side of the line solve of row j of line q:
+//   f - c rphi (u_{q-1} + u_{q+1})
+// ZB_RESTRICT: f is the restricted residual of the finer level (returned in
+//   *fr); on colour 0 the neighbours are the zero initial correction.
+// ZB_PROLONG (colour 0 only): the neighbours are odd lines 2qc+1, read with
+//   the coarse correction e(qc) of level lev+1 added.
+template <int MODE>
+__device__ __forceinline__ double line_rhs(const MgProblemDev &P, int lev, int j, int q, int qm, int qp,
+                                           int color, double *fr)
+{
+    const int n = P.n;
+    const MgLevelDev &L = P.lev[lev];
+    const double c = L.cphi;
+    if (MODE == ZB_RESTRICT) {
+        const double f = restricted_at(P, P.lev[lev - 1], j, q);
+        *fr = f;
+        if (color == 0) return f;
+        return f - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
+    }
+    if (MODE == ZB_PROLONG) {
+        const double *e = P.lev[lev + 1].u;
+        const double um = L.u[cix(j, qm, n)] + e[cix(j, qm >> 1, n)];
+        const double up = L.u[cix(j, qp, n)] + e[cix(j, qp >> 1, n)];
+        return L.f[cix(j, q, n)] - c * P.rphi[j] * (um + up);
+    }
+    return L.f[cix(j, q, n)] - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
+}
+
+// Solves one r-line in LDS by parallel cyclic reduction.  Each thread owns
+// rows tid + k*nt (k < R) and holds their reduction coefficients in
+// registers (loaded before the first barrier, so no global load sits on the
+// chain of steps).  rhs[k] is the thread's right-hand side; with FOLD the
+// closure rows are folded here, otherwise the caller has folded them.  The
+// solution is returned in x[k].
+template <int R, bool FOLD>
+__device__ __forceinline__ void pcr_line(const double *__restrict__ tab, int S, int n, double (*buf)[MAXN],
+                                         double rhs[R], double x[R])
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
+    double a[R][MAXS], b[R][MAXS], di[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        const bool in = i < n;
+#pragma unroll
+        for (int t = 0; t < MAXS; ++t) {
+            a[k][t] = (in && t < S) ? alpha[(size_t)t * n + i] : 0.0;
+            b[k][t] = (in && t < S) ? beta[(size_t)t * n + i] : 0.0;
+        }
+        di[k] = in ? dinv[i] : 0.0;
+        if (in) buf[0][i] = rhs[k];
+    }
+    __syncthreads();
+    if (FOLD && n > 2) {   // fold the closure rows here: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
+        double r0 = 0.0, rN = 0.0;
+        if (tid == 0) r0 = buf[0][0] - tab[0] * buf[0][1];
+        if (tid == (n - 1) % nt) rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
+        __syncthreads();
+        if (tid == 0) buf[0][0] = r0;
+        if (tid == (n - 1) % nt) buf[0][n - 1] = rN;
+        __syncthreads();
+    }
+    int cur = 0;
+#pragma unroll
+    for (int t = 0; t < MAXS; ++t) {
+        if (t < S) {            // S is uniform over the block
+            const int s = 1 << t;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int i = tid + k * nt;
+                if (i < n) {
+                    double v = buf[cur][i];
+                    if (i >= s) v += a[k][t] * buf[cur][i - s];
+                    if (i + s < n) v += b[k][t] * buf[cur][i + s];
+                    buf[cur ^ 1][i] = v;
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        x[k] = (i < n) ? buf[cur][i] * di[k] : 0.0;
+    }
+}
+
+// One colour of r-lines of one level: line q = 2*blockIdx.x + colour.
+template <int R>
+__global__ void __launch_bounds__(1024) k_mg_zebra(const MgBatchDev *__restrict__ Bp, int lev, int color, int mode)
 {
     const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
@@ -67,48 +172,45 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(const MgBatchDev *__restrict_
     if (lev >= P.nlev) return;
     const MgLevelDev &L = P.lev[lev];
     const int n = P.n, ncol = L.ncol;
-    const bool coarsest = (ncol == 1);
-    const int line = blockIdx.x;
-    if (coarsest ? (line != 0 || color != 0) : (2 * line + color >= ncol)) return;
-    const int q = coarsest ? 0 : 2 * line + color;
+    const int q = 2 * blockIdx.x + color;
+    if (ncol < 2 || q >= ncol) return;
+    const int qm = (q == 0) ? ncol - 1 : q - 1;
+    const int qp = (q == ncol - 1) ? 0 : q + 1;
     __shared__ double buf[2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
-    double *u = L.u;
-    const double *f = L.f;
-    const double c = L.cphi;
-    if (coarsest) {
-        for (int i = tid; i < n; i += nt) buf[0][i] = f[cix(i, 0, n)];
-    } else {
-        const int qm = (q == 0) ? ncol - 1 : q - 1;
-        const int qp = (q == ncol - 1) ? 0 : q + 1;
-        for (int i = tid; i < n; i += nt)
-            buf[0][i] = f[cix(i, q, n)] - c * P.rphi[i] * (u[cix(i, qm, n)] + u[cix(i, qp, n)]);
-    }
-    __syncthreads();
     const double *tab = L.lu;
-    const int S = L.nstep;
-    if (tid == 0) {      // fold the closure rows: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
-        const double r0 = buf[0][0] - tab[0] * buf[0][1];
-        const double rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
-        buf[0][0] = r0;
-        buf[0][n - 1] = rN;
-    }
-    __syncthreads();
-    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
-    int cur = 0;
-    for (int t = 0; t < S; ++t) {
-        const int s = 1 << t;
-        const double *a = alpha + (size_t)t * n, *b = beta + (size_t)t * n;
-        for (int i = tid; i < n; i += nt) {
-            double v = buf[cur][i];
-            if (i >= s) v += a[i] * buf[cur][i - s];
-            if (i + s < n) v += b[i] * buf[cur][i + s];
-            buf[cur ^ 1][i] = v;
+    double rhs[R], x[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        rhs[k] = 0.0;
+        if (i >= n) continue;
+        double fr = 0.0, fr1 = 0.0, v;
+        if (mode == ZB_RESTRICT) {
+            v = line_rhs<ZB_RESTRICT>(P, lev, i, q, qm, qp, color, &fr);
+            L.f[cix(i, q, n)] = fr;
+        } else if (mode == ZB_PROLONG) {
+            v = line_rhs<ZB_PROLONG>(P, lev, i, q, qm, qp, color, &fr);
+        } else {
+            v = line_rhs<ZB_PLAIN>(P, lev, i, q, qm, qp, color, &fr);
         }
-        __syncthreads();
-        cur ^= 1;
+        // fold the closure rows: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
+        if (n > 2 && (i == 0 || i == n - 1)) {
+            const int j = (i == 0) ? 1 : n - 2;
+            double w;
+            if (mode == ZB_RESTRICT) w = line_rhs<ZB_RESTRICT>(P, lev, j, q, qm, qp, color, &fr1);
+            else if (mode == ZB_PROLONG) w = line_rhs<ZB_PROLONG>(P, lev, j, q, qm, qp, color, &fr1);
+            else w = line_rhs<ZB_PLAIN>(P, lev, j, q, qm, qp, color, &fr1);
+            v = v - ((i == 0) ? tab[0] : tab[1]) * w;
+        }
+        rhs[k] = v;
     }
-    for (int i = tid; i < n; i += nt) u[cix(i, q, n)] = buf[cur][i] * dinv[i];
+    pcr_line<R, false>(tab, L.nstep, n, buf, rhs, x);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        if (i < n) L.u[cix(i, q, n)] = x[k];
+    }
 }
 
 // fine level lev -> coarse lev+1: full weighting of the residual along phi;
@@ -134,27 +236,6 @@ __global__ void __launch_bounds__(256) k_mg_restrict(const MgBatchDev *__restric
     Cc.u[cix(j, qc, n)] = 0.0;
 }
 
-// coarse lev+1 -> fine lev: linear interpolation along phi, added.
-__global__ void __launch_bounds__(256) k_mg_prolong(const MgBatchDev *__restrict__ Bp, int lev)
-{
-    const MgBatchDev &B = *Bp;
-    const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    if (lev + 1 >= P.nlev) return;
-    const MgLevelDev &F = P.lev[lev];
-    const MgLevelDev &Cc = P.lev[lev + 1];
-    const int n = P.n, nc = Cc.ncol;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * nc) return;
-    const int qc = (int)(t / n), j = (int)(t - (long long)qc * n);
-    const int qcm = (qc == 0) ? nc - 1 : qc - 1;
-    const double e = Cc.u[cix(j, qc, n)];
-    const double em = Cc.u[cix(j, qcm, n)];
-    F.u[cix(j, 2 * qc + 1, n)] += e;
-    F.u[cix(j, 2 * qc, n)] += 0.5 * (e + em);
-}
-
 // Exact solve of the coarsest level with C = ncol columns: one workgroup per
 // real Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
 // ml = C-1: the Nyquist mode), the projection of the residual on that mode,
@@ -162,6 +243,7 @@ __global__ void __launch_bounds__(256) k_mg_prolong(const MgBatchDev *__restrict
 // reduction.  Below level 0 the correction starts from zero, so the residual
 // is f; on level 0 (the whole solve is direct) it is f - Lu and each cycle is
 // one step of iterative refinement.
+template <int R>
 __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict__ Bp, int lev)
 {
     const MgBatchDev &B = *Bp;
@@ -170,7 +252,7 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
     const MgProblemDev &P = B.prob[p];
     if (lev >= P.nlev) return;
     const MgLevelDev &L = P.lev[lev];
-    const int n = P.n, C = L.ncol;
+    const int n = P.n, C = L.ncol;      // C is a power of two
     const int ml = blockIdx.x;
     if (ml >= C) return;
     int m, sinpart;
@@ -181,40 +263,28 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
     const int tid = threadIdx.x, nt = blockDim.x;
     const double *f = L.f;
     const double *tw = L.twid + (sinpart ? C : 0);
-    for (int i = tid; i < n; i += nt) {
+    const int msk = C - 1;
+    double rhs[R], x[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
         double acc = 0.0;
-        if (lev == 0) {
-            for (int q = 0; q < C; ++q) acc += resid_at(P, L, i, q) * tw[(m * q) % C];
-        } else {
-            for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) % C];
+        if (i < n) {
+            if (lev == 0) {
+                for (int q = 0; q < C; ++q) acc += resid_at(P, L, i, q) * tw[(m * q) & msk];
+            } else {
+#pragma unroll 8
+                for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) & msk];
+            }
         }
-        buf[0][i] = acc;
+        rhs[k] = acc;
     }
-    __syncthreads();
-    const double *tab = L.lu + (size_t)m * L.tabsz;
-    const int S = L.nstep;
-    if (tid == 0) {
-        const double r0 = buf[0][0] - tab[0] * buf[0][1];
-        const double rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
-        buf[0][0] = r0;
-        buf[0][n - 1] = rN;
+    pcr_line<R, true>(L.lu + (size_t)m * L.tabsz, L.nstep, n, buf, rhs, x);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        if (i < n) L.xm[(size_t)ml * n + i] = x[k];
     }
-    __syncthreads();
-    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
-    int cur = 0;
-    for (int t = 0; t < S; ++t) {
-        const int s = 1 << t;
-        const double *a = alpha + (size_t)t * n, *b = beta + (size_t)t * n;
-        for (int i = tid; i < n; i += nt) {
-            double v = buf[cur][i];
-            if (i >= s) v += a[i] * buf[cur][i - s];
-            if (i + s < n) v += b[i] * buf[cur][i + s];
-            buf[cur ^ 1][i] = v;
-        }
-        __syncthreads();
-        cur ^= 1;
-    }
-    for (int i = tid; i < n; i += nt) L.xm[(size_t)ml * n + i] = buf[cur][i] * dinv[i];
 }
 
 // synthesis of the coarsest-level correction from its mode lines, added
@@ -232,17 +302,20 @@ __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ 
     const int q = (int)(t / n), i = (int)(t - (long long)q * n);
     const double *X = L.xm;
     const double *cs = L.twid, *sn = L.twid + C;
+    const int msk = C - 1;
     double v = X[i];
+#pragma unroll 4
     for (int m = 1; m < (C + 1) / 2; ++m) {
-        const int j = (m * q) % C;
+        const int j = (m * q) & msk;
         v += 2.0 * (X[(size_t)(2 * m - 1) * n + i] * cs[j] + X[(size_t)(2 * m) * n + i] * sn[j]);
     }
     if (C >= 2) v += X[(size_t)(C - 1) * n + i] * ((q & 1) ? -1.0 : 1.0);
     L.u[cix(i, q, n)] += v / (double)C;
 }
 
-// max-norm of the level-0 residual; the last block of the problem applies
-// the stop test (relative residual <= rtol, or the cycle cap)
+// max-norm of the level-0 residual (grid-stride, one atomic per block); the
+// last block of the problem applies the stop test (relative residual <=
+// rtol, or the cycle cap)
 __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp, double rtol, int maxc)
 {
     const MgBatchDev &B = *Bp;
@@ -250,21 +323,29 @@ __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ 
     if (p >= B.nprob || B.done[p]) return;
     const MgProblemDev &P = B.prob[p];
     const MgLevelDev &L = P.lev[0];
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long cnt = (long long)P.n * L.ncol;
+    const long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
-    if (t < (long long)P.n * L.ncol) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
         const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
-        m = fabs(resid_at(P, L, j, q));
+        const double a = fabs(resid_at(P, L, j, q));
+        m = (a > m || a != a) ? a : m;
     }
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_down(m, off, 64);
         m = (o > m || o != o) ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(&B.res[p], (unsigned long long)__double_as_longlong(m));
-    __threadfence();
-    __syncthreads();
+    __shared__ double wm[4];
     __shared__ int last;
-    if (threadIdx.x == 0) last = (atomicAdd(&B.count[p], 1) == (int)gridDim.x - 1);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bm = wm[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = (wm[w] > bm || wm[w] != wm[w]) ? wm[w] : bm;
+        atomicMax(&B.res[p], (unsigned long long)__double_as_longlong(bm));
+        __threadfence();
+        last = (atomicAdd(&B.count[p], 1) == (int)gridDim.x - 1);
+    }
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
@@ -505,15 +586,12 @@ int mg_build(vv_run *r)
 }
 
 // algorithmic bytes of one zebra half-sweep of level l over both problems:
-// read f and the other colour, write the own colour (the coarsest level:
-// read f, write u)
+// read f and the other colour, write the own colour (3 words per line point)
 static double zebra_bytes(const vv::MgBatchDev &B, int l)
 {
     double b = 0.0;
-    for (int p = 0; p < B.nprob; ++p) {
-        const int n = B.prob[p].n, nc = B.prob[p].lev[l].ncol;
-        b += (nc == 1) ? 16.0 * n : 24.0 * (double)n * (nc / 2);
-    }
+    for (int p = 0; p < B.nprob; ++p)
+        b += 24.0 * (double)B.prob[p].n * (B.prob[p].lev[l].ncol / 2);
     return b;
 }
 
@@ -523,56 +601,66 @@ static int zebra_threads(int n)
     return t > 1024 ? 1024 : t;
 }
 
+// One V(nu1,nu2) cycle of both problems.  Levels 0..L-1 are smoothed by
+// zebra line sweeps, level L is solved exactly.  The restriction into a
+// smoothed level is fused into its first sweep (ZB_RESTRICT), the
+// prolongation of a correction into the first post-sweep of the finer level
+// (ZB_PROLONG): a line solve overwrites its own line, so only the
+// neighbour lines read by colour 0 need the correction added.
 static int enqueue_cycle(vv_run *r)
 {
     vv::MgBatchDev &B = r->mg;
     const vv::MgBatchDev *Bd = r->mg_dev;
-    const int nlev = B.prob[0].nlev;
+    const int L = B.prob[0].nlev - 1;
     const int n = r->nr;
-    const int ntot = B.prob[0].n + B.prob[1].n;
+    const double ntot = (double)(B.prob[0].n + B.prob[1].n);
     const int zt = zebra_threads(n);
+    const bool two = n > 1024;
     const double rtol = r->cfg.rtol;
     hipEvent_t pe;
-    auto zebra = [&](int l, int c) {
-        const int nc = B.prob[0].lev[l].ncol;
-        const int lines = (nc == 1) ? 1 : nc / 2;
+    auto ncol = [&](int l) { return B.prob[0].lev[l].ncol; };
+    auto zebra = [&](int l, int c, int mode) {
+        double bytes = zebra_bytes(B, l);
+        if (mode == ZB_RESTRICT) bytes += 8.0 * ntot * (ncol(l - 1) + 0.5 * ncol(l));
+        if (mode == ZB_PROLONG) bytes += 8.0 * ntot * ncol(l + 1);
         prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_zebra, dim3(lines, 2), dim3(zt), 0, r->stream, Bd, l, c);
-        prof_end(r, VV_K_ZEBRA, pe, zebra_bytes(B, l));
+        if (two)
+            hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, Bd, l, c, mode);
+        else
+            hipLaunchKernelGGL(k_mg_zebra<1>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, Bd, l, c, mode);
+        prof_end(r, VV_K_ZEBRA, pe, bytes);
     };
-    for (int l = 0; l < nlev - 1; ++l) {
+    for (int l = 0; l < L; ++l)
         for (int s = 0; s < r->cfg.nu1; ++s)
-            for (int c = 0; c < 2; ++c) zebra(l, c);
-        const int ncf = B.prob[0].lev[l].ncol, ncc = B.prob[0].lev[l + 1].ncol;
-        const long long cnt = (long long)n * ncc;
-        prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, l);
-        prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncf + 2.0 * ncc));
-    }
+            for (int c = 0; c < 2; ++c) zebra(l, c, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN);
     {   // exact solve of the coarsest level
-        const int L = nlev - 1;
-        const int C = B.prob[0].lev[L].ncol;
+        const int C = ncol(L);
+        if (L > 0) {
+            const long long cnt = (long long)n * C;
+            prof_begin(r, &pe);
+            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, L - 1);
+            prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncol(L - 1) + 2.0 * C));
+        }
         prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_direct, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
-        prof_end(r, VV_K_ZEBRA, pe, 16.0 * ntot * C);
+        if (two)
+            hipLaunchKernelGGL(k_mg_direct<2>, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
+        else
+            hipLaunchKernelGGL(k_mg_direct<1>, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
+        prof_end(r, VV_K_DIRECT, pe, (L == 0 ? 24.0 : 16.0) * ntot * C);
         prof_begin(r, &pe);
         hipLaunchKernelGGL(k_mg_idft, dim3(nblk((long long)n * C, 256), 2), dim3(256), 0, r->stream, Bd, L);
-        prof_end(r, VV_K_PROLONG, pe, 16.0 * ntot * C);
+        prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
-    for (int l = nlev - 2; l >= 0; --l) {
-        const int ncf = B.prob[0].lev[l].ncol, ncc = B.prob[0].lev[l + 1].ncol;
-        const long long cnt = (long long)n * ncc;
-        prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, l);
-        prof_end(r, VV_K_PROLONG, pe, 8.0 * ntot * (1.0 * ncc + 2.0 * ncf));
+    for (int l = L - 1; l >= 0; --l)
         for (int s = 0; s < r->cfg.nu2; ++s)
-            for (int c = 0; c < 2; ++c) zebra(l, c);
-    }
-    const int nc0 = B.prob[0].lev[0].ncol;
+            for (int c = 0; c < 2; ++c) zebra(l, c, (s == 0 && c == 0) ? ZB_PROLONG : ZB_PLAIN);
+    const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
+    int nb = (int)((cnt0 + 1023) / 1024);
+    if (nb > 256) nb = 256;
+    if (nb < 1) nb = 1;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nblk(cnt0, 256), 2), dim3(256), 0, r->stream, Bd, rtol,
-                       r->cfg.max_cycles);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, Bd, rtol, r->cfg.max_cycles);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -592,19 +680,18 @@ int mg_solve(vv_run *r, int which, int *cycles, double *relres)
     int chunk = r->last_cycles[0] > r->last_cycles[1] ? r->last_cycles[0] : r->last_cycles[1];
     if (chunk < 1) chunk = 2;
     int queued = 0;
-    int hflags[4];
-    unsigned long long hbits[4];
-    double hrel[2];
+    struct { int flags[8]; unsigned long long bits[4]; double rel[2]; } h;
+    static_assert(sizeof(h) == 80, "solver state block layout");
+    const int *hflags = h.flags;
+    const unsigned long long *hbits = h.bits;
+    const double *hrel = h.rel;
     for (;;) {
         for (int c = 0; c < chunk; ++c) {
             int st = enqueue_cycle(r);
             if (st) return st;
         }
         queued += chunk;
-        VV_HIP(hipMemcpyAsync(hflags, r->mg_flags, 4 * sizeof(int), hipMemcpyDeviceToHost, r->stream));
-        VV_HIP(hipMemcpyAsync(hbits, r->mg_bits, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              r->stream));
-        VV_HIP(hipMemcpyAsync(hrel, r->mg_relres, 2 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+        VV_HIP(hipMemcpyAsync(&h, r->mg_state, sizeof(h), hipMemcpyDeviceToHost, r->stream));
         VV_HIP(hipStreamSynchronize(r->stream));
         for (int p = 0; p < 2; ++p) {
             double fn;

@@ -161,7 +161,7 @@ class PeriodicBox:
         return ms.value
 
     KERNELS = ["compact", "df", "heat", "rhs", "zebra", "restrict", "prolong", "norm",
-               "check", "out", "update"]
+               "direct", "out", "update"]
 
     def profile(self, enable: bool):
         """Start (clearing totals) or stop per-kernel HIP event timing."""

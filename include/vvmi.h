@@ -150,8 +150,19 @@ int vv_last_step_ms(vv_run *run, double *ms);
  * vv_profile_read returns, for one kernel id, the launches, the summed
  * device time (ms) and the summed algorithmic bytes of those launches. */
 enum vv_kernel_id {
-    VV_K_COMPACT = 0, VV_K_DF, VV_K_HEAT, VV_K_RHS, VV_K_ZEBRA, VV_K_RESTRICT,
-    VV_K_PROLONG, VV_K_NORM, VV_K_CHECK, VV_K_OUT, VV_K_UPDATE, VV_K_COUNT
+    VV_K_COMPACT = 0,  /* compact derivatives (Thomas / cyclic line solves)      */
+    VV_K_DF,           /* DuFort-Frankel update of C and T + Robert filter      */
+    VV_K_HEAT,         /* wall heat flux                                          */
+    VV_K_RHS,          /* Poisson right-hand sides and initial guess             */
+    VV_K_ZEBRA,        /* multigrid line sweeps (restriction and prolongation    */
+                       /* fused into the first sweep of a level)                 */
+    VV_K_RESTRICT,     /* restriction onto the coarsest level                    */
+    VV_K_PROLONG,      /* synthesis of the coarsest-level correction             */
+    VV_K_NORM,         /* residual max-norm and stop test                        */
+    VV_K_DIRECT,       /* spectral solve of the coarsest level                   */
+    VV_K_OUT,          /* solution -> velocity fields                            */
+    VV_K_UPDATE,       /* wall update, clamp, running means, CFL                 */
+    VV_K_COUNT
 };
 int vv_profile(vv_run *run, int enable);
 int vv_profile_read(vv_run *run, int kernel, long long *launches, double *ms, double *bytes);
