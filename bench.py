@@ -120,6 +120,32 @@ def cpu_leg(name, budget_s=15.0):
                       f"line-GS FMG Poisson with the shipped absolute stop, 1 thread"}
 
 
+def copy_calibration(mib):
+    """One plain device-to-device copy of `mib` MiB (reads and writes exactly
+    that many bytes): the reference dispatch against which the FETCH_SIZE and
+    WRITE_SIZE counters are scaled."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from its counter summary in profiles/
+    (a line 'traffic_bytes_per_launch: X'), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,11 +154,15 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; the multi-GPU layout "
               "is described in DESIGN.md and not implemented", file=sys.stderr)
         sys.exit(2)
+    if a.copy_mib > 0:
+        copy_calibration(a.copy_mib)
     cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu)
     ms_per_step = wall_ms / a.steps
     # dominant kernel by device time in the profiled pass
@@ -143,10 +173,7 @@ def main():
     vcyc = {k: prof[k] for k in ("zebra", "restrict", "prolong", "norm", "direct")}
     vc_ms = sum(v["ms"] for v in vcyc.values())
     vc_bytes = sum(v["bytes"] for v in vcyc.values())
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{a.workload}.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    traffic = read_traffic(dom, a.workload)
     line = {
         "metric": METRIC,
         "value": a.steps / (wall_ms * 1e-3),

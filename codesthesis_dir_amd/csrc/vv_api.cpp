@@ -318,6 +318,15 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
     if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 32;
+    r->nbx = field_blocks(r);
+    {
+        const size_t np_ = 2 * (size_t)r->nbx + 3 * (size_t)r->nbx + 2 * (size_t)vv::MG_NORM_MAXB;
+        if (hipMalloc(&r->part, np_ * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemsetAsync(r->part, 0, np_ * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+        r->fpart = r->part;
+        r->upart = r->part + 2 * (size_t)r->nbx;
+        r->npart = r->part + 5 * (size_t)r->nbx;
+    }
     if (hipMalloc(&r->mg_state, 128) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMemsetAsync(r->mg_state, 0, 128, r->stream) != hipSuccess) return fail(VV_EHIP);
     r->mg_flags = static_cast<int *>(r->mg_state);
@@ -326,11 +335,6 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     if ((st = mg_build(r))) return fail(st);
     r->diag_cap = 1 << 16;
     if (hipMalloc(&r->diag, (size_t)r->diag_cap * 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMalloc(&r->diag_i, 4 * sizeof(int)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMalloc(&r->cfl_bits, 2 * sizeof(unsigned long long)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMemsetAsync(r->diag_i, 0, 4 * sizeof(int), r->stream) != hipSuccess) return fail(VV_EHIP);
-    if (hipMemsetAsync(r->cfl_bits, 0, 2 * sizeof(unsigned long long), r->stream) != hipSuccess)
-        return fail(VV_EHIP);
     if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
     if (nlev) *nlev = r->nlev_v;
     *out = r;
@@ -345,13 +349,11 @@ int vv_destroy(vv_run *r)
     double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                         r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
                         r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                        r->tab_p, r->mg_pool, r->diag};
+                        r->tab_p, r->mg_pool, r->diag, r->part};
     for (double *f : fields)
         if (f) (void)hipFree(f);
     if (r->mg_dev) (void)hipFree(r->mg_dev);
     if (r->mg_state) (void)hipFree(r->mg_state);
-    if (r->diag_i) (void)hipFree(r->diag_i);
-    if (r->cfl_bits) (void)hipFree(r->cfl_bits);
     for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
@@ -493,6 +495,26 @@ int vv_poisson_host(vv_run *r, int which, double *psi, double *psip, const doubl
     return pst;
 }
 
+// Combines the per-workgroup partials of the last k_update: CFL maxima of u
+// and v (NaN propagates) and the clamp count.
+static int read_update_partials(vv_run *r, double cfl[2], int *nclamp)
+{
+    std::vector<double> h(3 * (size_t)r->nbx);
+    VV_HIP(hipMemcpyAsync(h.data(), r->upart, h.size() * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    double cu = 0.0, cv = 0.0, nc = 0.0;
+    for (int b = 0; b < r->nbx; ++b) {
+        const double a = h[b], c = h[r->nbx + b];
+        if (a > cu || a != a) cu = a;
+        if (c > cv || c != c) cv = c;
+        nc += h[2 * (size_t)r->nbx + b];
+    }
+    cfl[0] = cu;
+    cfl[1] = cv;
+    *nclamp = (int)nc;
+    return VV_OK;
+}
+
 int vv_update(vv_run *r, int *nclamp)
 {
     if (!r) return VV_EINVAL;
@@ -500,12 +522,11 @@ int vv_update(vv_run *r, int *nclamp)
     Pending &pd = pending(r);
     if (!pd.c) VV_HIP(hipMemcpyAsync(r->Cf, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
     if (!pd.t) VV_HIP(hipMemcpyAsync(r->thf, r->th, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
-    VV_HIP(hipMemsetAsync(r->diag_i, 0, sizeof(int), r->stream));
     int st = launch_update(r, (int)(r->steps + 1));
     if (st) return st;
+    double cfl[2];
     int nc = 0;
-    VV_HIP(hipMemcpyAsync(&nc, r->diag_i, sizeof(int), hipMemcpyDeviceToHost, r->stream));
-    VV_HIP(hipStreamSynchronize(r->stream));
+    if ((st = read_update_partials(r, cfl, &nc))) return st;
     rotate(r);
     pd = Pending{};
     r->steps += 1;
@@ -556,12 +577,12 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
             info->trans1 = tr[0];
             info->trans2 = tr[1];
         }
-        unsigned long long cb[2];
+        double cfl[2];
         int nc = 0;
-        VV_HIP(hipMemcpy(cb, r->cfl_bits, sizeof(cb), hipMemcpyDeviceToHost));
-        VV_HIP(hipMemcpy(&nc, r->diag_i, sizeof(int), hipMemcpyDeviceToHost));
-        std::memcpy(&info->cflu, &cb[0], sizeof(double));
-        std::memcpy(&info->cflv, &cb[1], sizeof(double));
+        int st = read_update_partials(r, cfl, &nc);
+        if (st) return st;
+        info->cflu = cfl[0];
+        info->cflv = cfl[1];
         info->cycles_v = cyc[0];
         info->cycles_u = cyc[1];
         info->relres_v = rel[0];

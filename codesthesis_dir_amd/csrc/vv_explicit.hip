@@ -9,6 +9,7 @@
 // Field layout: F[k*ld + i], i = 0..nr+1 across the walls, k = 0..np.
 
 #include "vv_internal.h"
+#include "vv_reduce.h"
 
 #include <cmath>
 
@@ -115,15 +116,34 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, int T)
     }
     __syncthreads();
     if ((int)threadIdx.x < tc) {
+        // the recurrences in blocks of U: the loads of a block are issued
+        // before its dependent chain, so only the arithmetic is serial
+        constexpr int U = 8;
         double *r = R + threadIdx.x * nP;
         double rp = r[0];
-        for (int j = 1; j < n; ++j) {
+        int j = 1;
+        for (; j + U <= n; j += U) {
+            double fr[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { fr[u] = fact[j + u]; rr[u] = r[j + u]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { rp = rr[u] - fr[u] * rp; r[j + u] = rp; }
+        }
+        for (; j < n; ++j) {
             rp = r[j] - fact[j] * rp;
             r[j] = rp;
         }
         double y = rp / bp[n - 1];
         r[n - 1] = y;
-        for (int j = n - 2; j >= 0; --j) {
+        j = n - 2;
+        for (; j - U + 1 >= 0; j -= U) {
+            double cr[U], br[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { cr[u] = cc[j - u]; br[u] = bp[j - u]; rr[u] = r[j - u]; }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { y = (rr[u] - cr[u] * y) / br[u]; r[j - u] = y; }
+        }
+        for (; j >= 0; --j) {
             y = (r[j] - cc[j] * y) / bp[j];
             r[j] = y;
         }
@@ -162,9 +182,23 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
         const double fA = a.tab_p[5 * s], fB = a.tab_p[5 * s + 1], fC = a.tab_p[5 * s + 2];
         const int c = threadIdx.x;
 #define R_(j) R[((j) - 1) * T + c]
+        constexpr int U = 8;   // blocked as in tdma_tile
         double rn = R_(n);
         double rm = R_(1);
-        for (int j = 2; j <= n - 2; ++j) {
+        int j = 2;
+        for (; j + U - 1 <= n - 2; j += U) {
+            double g1[U], g2[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) { g1[u] = f1[j + u]; g2[u] = f2[j + u]; rr[u] = R_(j + u); }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double r = rr[u] - g1[u] * rm;
+                rn = rn - g2[u] * rm;
+                R_(j + u) = r;
+                rm = r;
+            }
+        }
+        for (; j <= n - 2; ++j) {
             const double r = R_(j) - f1[j] * rm;
             rn = rn - f2[j] * rm;
             R_(j) = r;
@@ -177,7 +211,17 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
         double y = (rn1 - cc[n - 1] * yn) / bb[n - 1];
         R_(n) = yn;
         R_(n - 1) = y;
-        for (int j = n - 2; j >= 1; --j) {
+        j = n - 2;
+        for (; j - U + 1 >= 1; j -= U) {
+            double cr[U], ar[U], br[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                cr[u] = cc[j - u]; ar[u] = aa[j - u]; br[u] = bb[j - u]; rr[u] = R_(j - u);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) { y = (rr[u] - cr[u] * y - ar[u] * yn) / br[u]; R_(j - u) = y; }
+        }
+        for (; j >= 1; --j) {
             y = (R_(j) - cc[j] * y - aa[j] * yn) / bb[j];
             R_(j) = y;
         }
@@ -363,6 +407,7 @@ __global__ void __launch_bounds__(WG) k_heat(const double *th, int nr, int np, i
         if (threadIdx.x == 0) {
             if (base == 1) { first_b = xb[0]; first_t = xt[0]; }
             const int cnt = (np - base + 1 < WG) ? np - base + 1 : WG;
+#pragma unroll 8
             for (int j = 0; j < cnt; ++j) {
                 const int k = base + j;
                 const double nb = (k == np) ? first_b : xb[j + 1];
@@ -379,10 +424,8 @@ __global__ void __launch_bounds__(WG) k_heat(const double *th, int nr, int np, i
 struct RhsArgs {
     const double *Cn, *vel, *velp, *uel, *uelp;
     double *vrh, *urh;
-    double *fv, *uv, *fu, *uu;     // multigrid level 0 (coloured layout)
-    unsigned long long *fnorm;     // [2]
-    unsigned long long *cfl_bits;  // [2] reset here
-    int *nclamp;                   // reset here
+    double *fv, *uv, *fu, *uu;     // multigrid level 0
+    double *fpart;                 // [2][gridDim.x] per-workgroup max |f|
     const double *rC, *rT;
     int nr, np, ld, which, from_fields, geom;
     double dr, dp;
@@ -393,18 +436,11 @@ __device__ __forceinline__ size_t mgix(int j, int q, int n) { return (size_t)q *
 
 // rhsv (rb.f:1629-1676), rhsu (rb.f:1679-1721), the extrapolated initial
 // guess of poisson2/3 (rb.f:1827-1835) scattered into the multigrid level 0,
-// and the max-norms of the right-hand sides.
-__global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
+// and per-workgroup maxima of |rhs| (combined by the solver's stop test).
+__device__ __forceinline__ void rhs_point(const RhsArgs &a, int t, double &mv, double &mu)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nr = a.nr, np = a.np, ld = a.ld;
     const int rows = nr + 2;
-    if (t == 0) {
-        a.cfl_bits[0] = 0ull;
-        a.cfl_bits[1] = 0ull;
-        *a.nclamp = 0;
-    }
-    if (t >= rows * (np + 1)) return;
     const int k = t / rows, i = t - k * rows;
     const size_t o = ix(i, k, ld);
     if ((a.which & 1) && i >= 1 && i <= nr) {
@@ -429,7 +465,7 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
             const size_t m = mgix(i - 1, k - 1, nr);
             a.fv[m] = v;
             a.uv[m] = 2.0 * a.vel[o] - a.velp[o];
-            atomicMax(&a.fnorm[0], (unsigned long long)__double_as_longlong(fabs(v)));
+            mv = nanmax(mv, fabs(v));
         }
     }
     if ((a.which & 2) && i >= 1 && i <= nr - 1) {
@@ -446,8 +482,22 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
             const size_t m = mgix(i - 1, k - 1, nr - 1);
             a.fu[m] = u;
             a.uu[m] = 2.0 * a.uel[o] - a.uelp[o];
-            atomicMax(&a.fnorm[1], (unsigned long long)__double_as_longlong(fabs(u)));
+            mu = nanmax(mu, fabs(u));
         }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
+{
+    __shared__ double sh[WG / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double mv = 0.0, mu = 0.0;
+    if (t < (a.nr + 2) * (a.np + 1)) rhs_point(a, t, mv, mu);
+    mv = block_max(mv, sh);
+    mu = block_max(mu, sh);
+    if (threadIdx.x == 0) {
+        a.fpart[blockIdx.x] = mv;
+        a.fpart[gridDim.x + blockIdx.x] = mu;
     }
 }
 
@@ -457,8 +507,7 @@ struct UpdArgs {
     const double *rT;
     int geom;
     double *mT2, *mT, *mU2, *mV2;
-    unsigned long long *cfl_bits;
-    int *nclamp;
+    double *upart;      // [3][gridDim.x]: max CFL u, max CFL v, clamp count
     int nr, np, ld, kt, stats;
     double dr, dp, dt;
 };
@@ -466,12 +515,10 @@ struct UpdArgs {
 // update (rb.f:1557-1626) on the new level; the rotation Cp<-C<-Cn,
 // thp<-th<-thn is a pointer swap on the host.  Running means
 // (rb.f:409-417) and CFL maxima (rb.f:421-458) are fused here.
-__global__ void __launch_bounds__(WG) k_update(UpdArgs a)
+__device__ __forceinline__ void update_point(const UpdArgs &a, int t, double &cu, double &cv, double &ncl)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nr = a.nr, np = a.np, ld = a.ld;
+    const int nr = a.nr, ld = a.ld;
     const int rows = nr + 2;
-    if (t >= rows * (np + 1)) return;
     const int k = t / rows, i = t - k * rows;
     const size_t o = ix(i, k, ld);
     if (i == 0) a.Cn[o] = (3.0 * a.vel[ix(1, k, ld)] - a.vel[ix(2, k, ld)] / 3.0) / a.dr;
@@ -483,7 +530,7 @@ __global__ void __launch_bounds__(WG) k_update(UpdArgs a)
     if (T > 1.0) { T = 0.99; ++clamped; }
     if (T < 0.0) { T = 0.01; ++clamped; }
     a.thn[o] = T;
-    if (clamped) atomicAdd(a.nclamp, clamped);
+    ncl += (double)clamped;
     if (a.stats) {
         const double U = a.uel[o], V = a.vel[o];
         if (a.kt >= 2) {
@@ -494,14 +541,24 @@ __global__ void __launch_bounds__(WG) k_update(UpdArgs a)
             a.mU2[o] = tkti * a.mU2[o] * km2 + tkti * (U * U);
             a.mV2[o] = tkti * a.mV2[o] * km2 + tkti * (V * V);
         }
-        if (i >= 1 && i <= nr) {
-            const double c = fabs(U * a.dt / a.dr);
-            atomicMax(&a.cfl_bits[0], (unsigned long long)__double_as_longlong(c));
-        }
-        if (i >= 1 && i <= nr - 1) {
-            const double c = fabs(V * a.dt / ((a.geom == 1 ? a.rT[i] : 1.0) * a.dp));
-            atomicMax(&a.cfl_bits[1], (unsigned long long)__double_as_longlong(c));
-        }
+        if (i >= 1 && i <= nr) cu = nanmax(cu, fabs(U * a.dt / a.dr));
+        if (i >= 1 && i <= nr - 1) cv = nanmax(cv, fabs(V * a.dt / ((a.geom == 1 ? a.rT[i] : 1.0) * a.dp)));
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_update(UpdArgs a)
+{
+    __shared__ double sh[WG / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double cu = 0.0, cv = 0.0, ncl = 0.0;
+    if (t < (a.nr + 2) * (a.np + 1)) update_point(a, t, cu, cv, ncl);
+    cu = block_max(cu, sh);
+    cv = block_max(cv, sh);
+    ncl = block_sum(ncl, sh);
+    if (threadIdx.x == 0) {
+        a.upart[blockIdx.x] = cu;
+        a.upart[gridDim.x + blockIdx.x] = cv;
+        a.upart[2 * gridDim.x + blockIdx.x] = ncl;
     }
 }
 
@@ -539,6 +596,8 @@ static int compact_launch(vv_run *r, int mask)
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
+
+int field_blocks(const vv_run *r) { return nblk((long long)(r->nr + 2) * (r->np + 1), WG); }
 
 int launch_compact(vv_run *r) { return compact_launch(r, 0x3f); }
 int launch_compact_mask(vv_run *r, int mask) { return compact_launch(r, mask); }
@@ -591,9 +650,7 @@ static int launch_rhs_common(vv_run *r, int which, int from_fields)
     a.vrh = r->vrh; a.urh = r->urh;
     a.fv = r->mg.prob[0].lev[0].f; a.uv = r->mg.prob[0].lev[0].u;
     a.fu = r->mg.prob[1].lev[0].f; a.uu = r->mg.prob[1].lev[0].u;
-    a.fnorm = r->mg.fnorm;
-    a.cfl_bits = r->cfl_bits;
-    a.nclamp = r->diag_i;
+    a.fpart = r->fpart;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.which = which;
     a.dr = r->dr; a.dp = r->dp;
     a.rC = r->rC; a.rT = r->rT; a.geom = r->cfg.geom;
@@ -615,7 +672,7 @@ int launch_update(vv_run *r, int kt)
     a.Cn = r->Cn; a.thn = r->thn; a.vel = r->vel; a.uel = r->uel;
     a.rT = r->rT; a.geom = r->cfg.geom;
     a.mT2 = r->mT2; a.mT = r->mT; a.mU2 = r->mU2; a.mV2 = r->mV2;
-    a.cfl_bits = r->cfl_bits; a.nclamp = r->diag_i;
+    a.upart = r->upart;
     a.nr = r->nr; a.np = r->np; a.ld = r->ld; a.kt = kt; a.stats = r->cfg.stats;
     a.dr = r->dr; a.dp = r->dp; a.dt = r->cfg.dt;
     const long long n = (long long)(r->nr + 2) * (r->np + 1);

@@ -80,16 +80,19 @@ struct MgProblemDev {
     MgLevelDev lev[MG_MAXLEV];
 };
 
+constexpr int MG_NORM_MAXB = 256;   // workgroups of the residual-norm kernel
+
 // one batch of problems solved together (v and u of one run)
 struct MgBatchDev {
     int nprob;
     MgProblemDev prob[MG_MAXPROB];
     int *done;            // [nprob] converged / capped flag
     int *cycles;          // [nprob]
-    int *count;           // [nprob] blocks of the norm kernel finished
-    unsigned long long *res;    // [nprob] max |r| bits (non-negative doubles)
-    unsigned long long *fnorm;  // [nprob] max |f| bits
+    unsigned long long *fnorm;  // [nprob] max |f| bits (non-negative doubles)
     double *relres;       // [nprob]
+    double *npart;        // [nprob][MG_NORM_MAXB] per-workgroup residual maxima
+    const double *fpart;  // [nprob][nbr] per-workgroup |f| maxima of the rhs kernel
+    int nbr;
 };
 
 }  // namespace vv
@@ -128,8 +131,8 @@ struct vv_run {
     vv::MgBatchDev *mg_dev = nullptr;
     double *mg_pool = nullptr;
     // solver state, one device block read back with a single copy:
-    //   bytes 0-31 int done[2], cycles[2], count[2], pad[2];
-    //   32-63 u64 res[2], fnorm[2] (bits of non-negative doubles);
+    //   bytes 0-31 int done[2], cycles[2], unused[4];
+    //   32-63 u64 unused[2], fnorm[2] (bits of non-negative doubles);
     //   64-79 double relres[2]
     void *mg_state = nullptr;
     int *mg_flags = nullptr;
@@ -137,11 +140,16 @@ struct vv_run {
     double *mg_relres = nullptr;
     int nlev_v = 0, nlev_u = 0;
 
-    // per-step diagnostics ring: [trans1, trans2, cflu, cflv] per step
+    // per-step wall heat fluxes [trans1, trans2], a ring of diag_cap steps
     double *diag = nullptr;
-    int *diag_i = nullptr;      // nclamp accumulators
-    unsigned long long *cfl_bits = nullptr;
     int diag_cap = 0;
+    // per-workgroup partial reductions (no atomics): nbx = workgroups of the
+    // field kernels; fpart [2][nbx] |f| maxima of the Poisson right-hand
+    // sides, upart [3][nbx] CFL maxima (u, v) and clamp counts of the last
+    // update, npart [2][MG_NORM_MAXB] residual maxima
+    double *part = nullptr;
+    double *fpart = nullptr, *upart = nullptr, *npart = nullptr;
+    int nbx = 0;
     long long steps = 0;        // completed steps since vv_start
     int last_cycles[2] = {0, 0};
     double last_relres[2] = {0, 0};
@@ -163,7 +171,9 @@ namespace vv {
 // kernel ids of the timing instrumentation (VV_K_* in include/vvmi.h)
 int prof_begin(vv_run *r, hipEvent_t *a);
 int prof_end(vv_run *r, int kid, hipEvent_t a, double bytes);
-// explicit-step launchers (vv_explicit.hip)
+// explicit-step launchers (vv_explicit.hip); field_blocks = workgroups of
+// the pointwise field kernels (the length of the partial-reduction arrays)
+int field_blocks(const vv_run *r);
 int launch_compact(vv_run *r);
 int launch_df(vv_run *r, int first, int do_c, int do_t);
 int launch_heat(vv_run *r, double *dst2);

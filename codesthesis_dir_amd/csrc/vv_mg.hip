@@ -22,6 +22,7 @@
 // converged, so several cycles can be queued between host checks.
 
 #include "vv_internal.h"
+#include "vv_reduce.h"
 
 #include <cmath>
 #include <cstring>
@@ -313,10 +314,8 @@ __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ 
     L.u[cix(i, q, n)] += v / (double)C;
 }
 
-// max-norm of the level-0 residual (grid-stride, one atomic per block); the
-// last block of the problem applies the stop test (relative residual <=
-// rtol, or the cycle cap)
-__global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp, double rtol, int maxc)
+// max-norm of the level-0 residual: grid-stride, one partial per workgroup
+__global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp)
 {
     const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
@@ -328,36 +327,42 @@ __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ 
     double m = 0.0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
         const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
-        const double a = fabs(resid_at(P, L, j, q));
-        m = (a > m || a != a) ? a : m;
+        m = nanmax(m, fabs(resid_at(P, L, j, q)));
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_down(m, off, 64);
-        m = (o > m || o != o) ? o : m;
-    }
-    __shared__ double wm[4];
-    __shared__ int last;
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double bm = wm[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = (wm[w] > bm || wm[w] != wm[w]) ? wm[w] : bm;
-        atomicMax(&B.res[p], (unsigned long long)__double_as_longlong(bm));
-        __threadfence();
-        last = (atomicAdd(&B.count[p], 1) == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        const double r = __longlong_as_double((long long)atomicMax(&B.res[p], 0ull));
-        const double fn = __longlong_as_double((long long)B.fnorm[p]);
-        const double rel = r / (fn > 0.0 ? fn : 1.0);
-        const int c = B.cycles[p] + 1;
-        B.cycles[p] = c;
-        B.relres[p] = rel;
-        atomicExch(&B.res[p], 0ull);
-        atomicExch(&B.count[p], 0);
-        if (!(rel > rtol) || c >= maxc) B.done[p] = 1;    // NaN also stops
+    __shared__ double sh[4];
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) B.npart[p * MG_NORM_MAXB + blockIdx.x] = m;
+}
+
+// Stop test of one cycle (one workgroup): combines the residual partials of
+// k_mg_norm and, at the first cycle of a solve, the |f| partials of the
+// right-hand-side kernel; converged when the relative residual is <= rtol
+// (a NaN residual also stops), capped at maxc cycles.
+__global__ void __launch_bounds__(256) k_mg_stop(const MgBatchDev *__restrict__ Bp, int nbn, double rtol, int maxc)
+{
+    const MgBatchDev &B = *Bp;
+    __shared__ double sh[4];
+    for (int p = 0; p < B.nprob; ++p) {
+        if (B.done[p]) continue;             // uniform over the workgroup
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nbn; b += blockDim.x) m = nanmax(m, B.npart[p * MG_NORM_MAXB + b]);
+        m = block_max(m, sh);
+        const int c0 = B.cycles[p];
+        double fn = 0.0;
+        if (c0 == 0) {
+            for (int b = threadIdx.x; b < B.nbr; b += blockDim.x) fn = nanmax(fn, B.fpart[p * B.nbr + b]);
+            fn = block_max(fn, sh);
+        }
+        if (threadIdx.x == 0) {
+            if (c0 == 0) B.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
+            else fn = __longlong_as_double((long long)B.fnorm[p]);
+            const double rel = m / (fn > 0.0 ? fn : 1.0);
+            const int c = c0 + 1;
+            B.cycles[p] = c;
+            B.relres[p] = rel;
+            if (!(rel > rtol) || c >= maxc) B.done[p] = 1;
+        }
+        __syncthreads();
     }
 }
 
@@ -369,8 +374,6 @@ __global__ void k_mg_reset(const MgBatchDev *__restrict__ Bp, int which)
     const bool active = (which == 5) || (which == 2 && p == 0) || (which == 3 && p == 1);
     B.done[p] = active ? 0 : 1;
     B.cycles[p] = 0;
-    B.count[p] = 0;
-    B.res[p] = 0ull;
     B.fnorm[p] = 0ull;
     B.relres[p] = 0.0;
 }
@@ -574,10 +577,11 @@ int mg_build(vv_run *r)
                           r->stream));
     r->mg.done = r->mg_flags;
     r->mg.cycles = r->mg_flags + 2;
-    r->mg.count = r->mg_flags + 4;
-    r->mg.res = r->mg_bits;
     r->mg.fnorm = r->mg_bits + 2;
     r->mg.relres = r->mg_relres;
+    r->mg.npart = r->npart;
+    r->mg.fpart = r->fpart;
+    r->mg.nbr = r->nbx;
     VV_HIP(hipMalloc(&r->mg_dev, sizeof(vv::MgBatchDev)));
     VV_HIP(hipMemcpyAsync(r->mg_dev, &r->mg, sizeof(vv::MgBatchDev), hipMemcpyHostToDevice, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
@@ -657,10 +661,11 @@ static int enqueue_cycle(vv_run *r)
     const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
     int nb = (int)((cnt0 + 1023) / 1024);
-    if (nb > 256) nb = 256;
+    if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, Bd, rtol, r->cfg.max_cycles);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, Bd);
+    hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, Bd, nb, rtol, r->cfg.max_cycles);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
