@@ -317,7 +317,7 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rC, rc))) return fail(st);
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
-    if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 32;
+    if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 128;
     r->nbx = field_blocks(r);
     {
         const size_t np_ = 2 * (size_t)r->nbx + 3 * (size_t)r->nbx + 2 * (size_t)vv::MG_NORM_MAXB;
@@ -448,8 +448,7 @@ int vv_rhs(vv_run *r)
 {
     if (!r) return VV_EINVAL;
     VV_HIP(hipSetDevice(r->device));
-    int st = mg_reset(r, 5);
-    if (!st) st = launch_rhs_guess(r, 3);
+    int st = launch_rhs_guess(r, 3, 5);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
     return st;
 }
@@ -459,8 +458,7 @@ int vv_poisson(vv_run *r, int which, int *cycles, double *relres)
     if (!r || (which != 2 && which != 3 && which != 5)) { set_error("vv_poisson: bad argument"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
     const int w = (which == 5) ? 3 : (which == 2 ? 1 : 2);
-    int st = mg_reset(r, which);
-    if (!st) st = launch_rhs_fields(r, w);
+    int st = launch_rhs_fields(r, w, which);
     if (st) return st;
     int cyc[2] = {0, 0};
     double rel[2] = {0, 0};
@@ -522,7 +520,7 @@ int vv_update(vv_run *r, int *nclamp)
     Pending &pd = pending(r);
     if (!pd.c) VV_HIP(hipMemcpyAsync(r->Cf, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
     if (!pd.t) VV_HIP(hipMemcpyAsync(r->thf, r->th, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
-    int st = launch_update(r, (int)(r->steps + 1));
+    int st = launch_update(r, (int)(r->steps + 1), 0);
     if (st) return st;
     double cfl[2];
     int nc = 0;
@@ -548,14 +546,12 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
         int st = launch_compact(r);
         if (!st) st = launch_df(r, first, 1, 1);
         if (!st) st = launch_heat(r, r->diag + 2 * (size_t)(r->steps % r->diag_cap));
-        if (!st) st = mg_reset(r, 5);
-        if (!st) st = launch_rhs_guess(r, 3);
+        if (!st) st = launch_rhs_guess(r, 3, 5);
         if (st) return st;
         st = mg_solve(r, 5, cyc, rel);
         if (st < 0) return st;
         if (st == VV_NOT_CONVERGED) status = VV_NOT_CONVERGED;
-        if ((st = launch_mg_scatter_out(r, 5))) return st;
-        if ((st = launch_update(r, (int)(r->steps + 1)))) return st;
+        if ((st = launch_update(r, (int)(r->steps + 1), 1))) return st;
         rotate(r);
         r->steps += 1;
         if (r->prof_on) {

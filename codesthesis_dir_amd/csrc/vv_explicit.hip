@@ -426,8 +426,13 @@ struct RhsArgs {
     double *vrh, *urh;
     double *fv, *uv, *fu, *uu;     // multigrid level 0
     double *fpart;                 // [2][gridDim.x] per-workgroup max |f|
+    // solver state reset for a new solve (workgroup 0): problems active per
+    // `which` (2: v, 3: u, 5: both)
+    int *done, *cycles;
+    unsigned long long *fnorm;
+    double *relres;
     const double *rC, *rT;
-    int nr, np, ld, which, from_fields, geom;
+    int nr, np, ld, which, from_fields, geom, solve;
     double dr, dp;
 };
 
@@ -491,6 +496,13 @@ __global__ void __launch_bounds__(WG) k_rhs_guess(RhsArgs a)
 {
     __shared__ double sh[WG / 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < MG_MAXPROB) {
+        const bool active = (a.solve == 5) || (a.solve == 2 && t == 0) || (a.solve == 3 && t == 1);
+        a.done[t] = active ? 0 : 1;
+        a.cycles[t] = 0;
+        a.fnorm[t] = 0ull;
+        a.relres[t] = 0.0;
+    }
     double mv = 0.0, mu = 0.0;
     if (t < (a.nr + 2) * (a.np + 1)) rhs_point(a, t, mv, mu);
     mv = block_max(mv, sh);
@@ -508,6 +520,11 @@ struct UpdArgs {
     int geom;
     double *mT2, *mT, *mU2, *mV2;
     double *upart;      // [3][gridDim.x]: max CFL u, max CFL v, clamp count
+    // from_mg: first move the solver's level-0 solutions u0v (nr rows) and
+    // u0u (nr-1 rows) into vel/uel with velp/uelp <- old values
+    double *velw, *uelw, *velp, *uelp;
+    const double *u0v, *u0u;
+    int from_mg;
     int nr, np, ld, kt, stats;
     double dr, dp, dt;
 };
@@ -521,8 +538,33 @@ __device__ __forceinline__ void update_point(const UpdArgs &a, int t, double &cu
     const int rows = nr + 2;
     const int k = t / rows, i = t - k * rows;
     const size_t o = ix(i, k, ld);
-    if (i == 0) a.Cn[o] = (3.0 * a.vel[ix(1, k, ld)] - a.vel[ix(2, k, ld)] / 3.0) / a.dr;
-    if (i == nr) a.Cn[o] = (a.vel[ix(nr - 1, k, ld)] / 3.0 - 3.0 * a.vel[ix(nr, k, ld)]) / a.dr;
+    double U, V;
+    if (a.from_mg) {
+        // poisson2/3 output (rb.f:1894-1913): psip <- psi, psi <- solution,
+        // walls 0; column k maps to solver column k-1 (k = 0 is k = np)
+        const int q = (k == 0) ? a.np - 1 : k - 1;
+        const double *sv = a.u0v + (size_t)q * nr;
+        const double *su = a.u0u + (size_t)q * (nr - 1);
+        const double vo = a.velw[o];
+        a.velp[o] = vo;
+        V = (i >= 1 && i <= nr) ? sv[i - 1] : 0.0;
+        a.velw[o] = V;
+        if (i <= nr) {
+            const double uo = a.uelw[o];
+            a.uelp[o] = uo;
+            U = (i >= 1 && i <= nr - 1) ? su[i - 1] : 0.0;
+            a.uelw[o] = U;
+        } else {
+            U = a.uelw[o];
+        }
+        if (i == 0) a.Cn[o] = (3.0 * sv[0] - sv[1] / 3.0) / a.dr;
+        if (i == nr) a.Cn[o] = (sv[nr - 2] / 3.0 - 3.0 * sv[nr - 1]) / a.dr;
+    } else {
+        U = a.uel[o];
+        V = a.vel[o];
+        if (i == 0) a.Cn[o] = (3.0 * a.vel[ix(1, k, ld)] - a.vel[ix(2, k, ld)] / 3.0) / a.dr;
+        if (i == nr) a.Cn[o] = (a.vel[ix(nr - 1, k, ld)] / 3.0 - 3.0 * a.vel[ix(nr, k, ld)]) / a.dr;
+    }
     double T = a.thn[o];
     if (i == 0) T = 0.0;
     if (i == nr + 1) T = 1.0;
@@ -532,7 +574,6 @@ __device__ __forceinline__ void update_point(const UpdArgs &a, int t, double &cu
     a.thn[o] = T;
     ncl += (double)clamped;
     if (a.stats) {
-        const double U = a.uel[o], V = a.vel[o];
         if (a.kt >= 2) {
             const double tkti = 1.0 / (double)(a.kt - 1);
             const double km2 = (double)(a.kt - 2);
@@ -642,10 +683,12 @@ int launch_heat_from(vv_run *r, const double *th, double *dst2)
     return VV_OK;
 }
 
-static int launch_rhs_common(vv_run *r, int which, int from_fields)
+static int launch_rhs_common(vv_run *r, int which, int from_fields, int solve)
 {
     RhsArgs a;
     a.from_fields = from_fields;
+    a.solve = solve;
+    a.done = r->mg.done; a.cycles = r->mg.cycles; a.fnorm = r->mg.fnorm; a.relres = r->mg.relres;
     a.Cn = r->Cn; a.vel = r->vel; a.velp = r->velp; a.uel = r->uel; a.uelp = r->uelp;
     a.vrh = r->vrh; a.urh = r->urh;
     a.fv = r->mg.prob[0].lev[0].f; a.uv = r->mg.prob[0].lev[0].u;
@@ -663,13 +706,16 @@ static int launch_rhs_common(vv_run *r, int which, int from_fields)
     return VV_OK;
 }
 
-int launch_rhs_guess(vv_run *r, int which) { return launch_rhs_common(r, which, 0); }
-int launch_rhs_fields(vv_run *r, int which) { return launch_rhs_common(r, which, 1); }
+int launch_rhs_guess(vv_run *r, int which, int solve) { return launch_rhs_common(r, which, 0, solve); }
+int launch_rhs_fields(vv_run *r, int which, int solve) { return launch_rhs_common(r, which, 1, solve); }
 
-int launch_update(vv_run *r, int kt)
+int launch_update(vv_run *r, int kt, int from_mg)
 {
     UpdArgs a;
     a.Cn = r->Cn; a.thn = r->thn; a.vel = r->vel; a.uel = r->uel;
+    a.velw = r->vel; a.uelw = r->uel; a.velp = r->velp; a.uelp = r->uelp;
+    a.u0v = r->mg.prob[0].lev[0].u; a.u0u = r->mg.prob[1].lev[0].u;
+    a.from_mg = from_mg;
     a.rT = r->rT; a.geom = r->cfg.geom;
     a.mT2 = r->mT2; a.mT = r->mT; a.mU2 = r->mU2; a.mV2 = r->mV2;
     a.upart = r->upart;
@@ -679,7 +725,8 @@ int launch_update(vv_run *r, int kt)
     hipEvent_t pe_;
     prof_begin(r, &pe_);
     hipLaunchKernelGGL(k_update, dim3(nblk(n, WG)), dim3(WG), 0, r->stream, a);
-    prof_end(r, VV_K_UPDATE, pe_, 8.0 * (r->cfg.stats ? 13.0 : 5.0) * (double)n);
+    // + solution scatter: read the solutions and old fields, write both levels
+    prof_end(r, VV_K_UPDATE, pe_, 8.0 * ((r->cfg.stats ? 13.0 : 5.0) + (from_mg ? 6.0 : 0.0)) * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

@@ -80,7 +80,7 @@ struct MgProblemDev {
     MgLevelDev lev[MG_MAXLEV];
 };
 
-constexpr int MG_NORM_MAXB = 256;   // workgroups of the residual-norm kernel
+constexpr int MG_NORM_MAXB = 1024;   // workgroups of the residual-norm kernel
 
 // one batch of problems solved together (v and u of one run)
 struct MgBatchDev {
@@ -177,15 +177,14 @@ int field_blocks(const vv_run *r);
 int launch_compact(vv_run *r);
 int launch_df(vv_run *r, int first, int do_c, int do_t);
 int launch_heat(vv_run *r, double *dst2);
-int launch_rhs_guess(vv_run *r, int which);
-int launch_update(vv_run *r, int nstat_kt);
+int launch_rhs_guess(vv_run *r, int which, int solve);
+int launch_update(vv_run *r, int nstat_kt, int from_mg);
 int launch_copy_prev(vv_run *r);
 // multigrid (vv_mg.hip)
 int mg_build(vv_run *r);
 int mg_solve(vv_run *r, int which, int *cycles, double *relres);
 int launch_mg_scatter_out(vv_run *r, int which);
-int mg_reset(vv_run *r, int which);
 int launch_compact_mask(vv_run *r, int mask);
 int launch_heat_from(vv_run *r, const double *th, double *dst2);
-int launch_rhs_fields(vv_run *r, int which);
+int launch_rhs_fields(vv_run *r, int which, int solve);
 }  // namespace vv

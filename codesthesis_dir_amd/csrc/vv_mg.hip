@@ -242,8 +242,9 @@ __global__ void __launch_bounds__(256) k_mg_restrict(const MgBatchDev *__restric
 // ml = C-1: the Nyquist mode), the projection of the residual on that mode,
 // then the mode's line system D_r + lambda_m diag(rphi) by parallel cyclic
 // reduction.  Below level 0 the correction starts from zero, so the residual
-// is f; on level 0 (the whole solve is direct) it is f - Lu and each cycle is
-// one step of iterative refinement.
+// is f.  On level 0 (the whole solve is direct) the first cycle solves for u
+// from f (the initial guess is not needed by an exact solve) and later
+// cycles project f - Lu: iterative refinement.
 template <int R>
 __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict__ Bp, int lev)
 {
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
         const int i = tid + k * nt;
         double acc = 0.0;
         if (i < n) {
-            if (lev == 0) {
+            if (lev == 0 && B.cycles[p] > 0) {
                 for (int q = 0; q < C; ++q) acc += resid_at(P, L, i, q) * tw[(m * q) & msk];
             } else {
 #pragma unroll 8
@@ -289,6 +290,7 @@ __global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict
 }
 
 // synthesis of the coarsest-level correction from its mode lines, added
+// (stored, at the first cycle of a solve that is direct on level 0)
 __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ Bp, int lev)
 {
     const MgBatchDev &B = *Bp;
@@ -311,7 +313,8 @@ __global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ 
         v += 2.0 * (X[(size_t)(2 * m - 1) * n + i] * cs[j] + X[(size_t)(2 * m) * n + i] * sn[j]);
     }
     if (C >= 2) v += X[(size_t)(C - 1) * n + i] * ((q & 1) ? -1.0 : 1.0);
-    L.u[cix(i, q, n)] += v / (double)C;
+    if (lev == 0 && B.cycles[p] == 0) L.u[cix(i, q, n)] = v / (double)C;   // first direct solve
+    else L.u[cix(i, q, n)] += v / (double)C;
 }
 
 // max-norm of the level-0 residual: grid-stride, one partial per workgroup
@@ -364,18 +367,6 @@ __global__ void __launch_bounds__(256) k_mg_stop(const MgBatchDev *__restrict__ 
         }
         __syncthreads();
     }
-}
-
-__global__ void k_mg_reset(const MgBatchDev *__restrict__ Bp, int which)
-{
-    const MgBatchDev &B = *Bp;
-    const int p = threadIdx.x;
-    if (p >= MG_MAXPROB) return;
-    const bool active = (which == 5) || (which == 2 && p == 0) || (which == 3 && p == 1);
-    B.done[p] = active ? 0 : 1;
-    B.cycles[p] = 0;
-    B.fnorm[p] = 0ull;
-    B.relres[p] = 0.0;
 }
 
 // level 0 -> fields: psip <- psi, psi <- solution (rb.f:1894-1913)
@@ -464,7 +455,7 @@ int mg_build(vv_run *r)
     const int nr = r->nr, np = r->np;
     const double hhri = 1.0 / (r->dr * r->dr);
     if (nr > MAXN) { set_error("nr exceeds the line length held in LDS (2048)"); return VV_EINVAL; }
-    const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 32;
+    const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 128;
     int nlev = 1;
     for (int c = np; c > Cd; c /= 2) ++nlev;
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
@@ -660,20 +651,13 @@ static int enqueue_cycle(vv_run *r)
             for (int c = 0; c < 2; ++c) zebra(l, c, (s == 0 && c == 0) ? ZB_PROLONG : ZB_PLAIN);
     const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
-    int nb = (int)((cnt0 + 1023) / 1024);
+    int nb = (int)((cnt0 + 255) / 256);
     if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
     prof_begin(r, &pe);
     hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, Bd);
     hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, Bd, nb, rtol, r->cfg.max_cycles);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
-    VV_HIP(hipGetLastError());
-    return VV_OK;
-}
-
-int mg_reset(vv_run *r, int which)
-{
-    hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(64), 0, r->stream, r->mg_dev, which);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

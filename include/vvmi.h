@@ -72,7 +72,7 @@ typedef struct vv_config {
     int    direct_cols;   /* phi coarsening stops at <= this many columns,
                              where the level is solved exactly (real DFT in
                              phi + one line solve per mode); >= np makes the
-                             whole solve direct.  0 selects 32.             */
+                             whole solve direct.  0 selects 128.            */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */

@@ -106,14 +106,17 @@ def test_explicit_step_bitexact(nr, np_, first, geom):
                                           (128, 256, 1.0), (48, 64, 2.0), (64, 128, 0.0),
                                           (256, 256, 0.0)])
 @pytest.mark.parametrize("which", [2, 3])
-def test_poisson_matches_exact_solution(nr, np_, width, which):
-    """width 0 marks the annulus geometry (dp = 2 pi/np, r0 = 0.5)."""
+@pytest.mark.parametrize("direct_cols", [8, 128])
+def test_poisson_matches_exact_solution(nr, np_, width, which, direct_cols):
+    """width 0 marks the annulus geometry (dp = 2 pi/np, r0 = 0.5).
+    direct_cols 8: multigrid down to 8 columns; 128: the default (two
+    levels at np = 256, a direct solve at np <= 128)."""
     geom = 1 if width == 0.0 else 0
     if geom:
-        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
+        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0, direct_cols=direct_cols)
         dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
     else:
-        box = _box(nr, np_, width, rtol=1e-12, max_cycles=60)
+        box = _box(nr, np_, width, rtol=1e-12, max_cycles=60, direct_cols=direct_cols)
         dr, dp = 0.5 / nr, width / np_
     rng = np.random.default_rng(nr * 7 + which)
     ld = nr + 2
@@ -172,15 +175,18 @@ def test_poisson_coarse_solve_variants(direct_cols, geom):
     box.close()
 
 
-def test_trajectory_1000_steps_config1():
+@pytest.mark.parametrize("direct_cols", [16, 128])
+def test_trajectory_1000_steps_config1(direct_cols):
     """Periodic RB box 129x65 nodes (nr=64, np=128), 2:1 (width 1, height
     0.5), Ra=1e5, Pr=1, dt=1e-5, seeded temperature, 1000 steps with the
     Poisson solves at 1e-10 relative residual.  Relative L2 of every field
-    against the oracle with exact solves must be <= 1e-9."""
+    against the oracle with exact solves must be <= 1e-9.  direct_cols 16:
+    V-cycles over 128/64/32 columns and a direct 16-column level; 128: the
+    default, a direct solve."""
     nr, np_, width = 64, 128, 1.0
     run = O.Run(nr, np_, 1e-5, 1e5, 1.0, width=width)
     run.seed()
-    box = _box(nr, np_, width, rtol=1e-10, max_cycles=60)
+    box = _box(nr, np_, width, rtol=1e-10, max_cycles=60, direct_cols=direct_cols)
     box.put("th", run["th"])
     box.start()
     info = box.step(1000)
@@ -224,15 +230,17 @@ def test_trajectory_against_reference_fixture(golden_dir, name, snap):
     box.close()
 
 
-def test_annulus_trajectory_300_steps():
+@pytest.mark.parametrize("direct_cols", [16, 128])
+def test_annulus_trajectory_300_steps(direct_cols):
     """Rotating-annulus plane (velvor2dnewsch3.4.f), 65x129 nodes, Ra=1e7,
     Re=sqrt(1e5), dt=1e-6, 300 steps: relative L2 <= 1e-9 against the oracle
     with exact solves.  The solves run at 1e-12: at 1e-10 the azimuthal
-    velocity differs by 3.8e-9 (measured), the solver tolerance times the
-    conditioning of the annulus operator."""
+    velocity differs by 3.8e-9 (measured with V-cycles), the solver
+    tolerance times the conditioning of the annulus operator."""
     nr, np_ = 64, 128
     Re = float(np.sqrt(1e5))
-    run, box = _pair(1, nr, np_, dt=1e-6, Ra=1e7, Re=Re, rtol=1e-12, max_cycles=60)
+    run, box = _pair(1, nr, np_, dt=1e-6, Ra=1e7, Re=Re, rtol=1e-12, max_cycles=60,
+                     direct_cols=direct_cols)
     run.seed()
     box.put("th", run["th"])
     box.start()
@@ -287,7 +295,7 @@ def test_nonfinite_rhs_is_an_error():
 
 def test_cycle_cap_reports_not_converged():
     nr, np_ = 32, 64
-    box = _box(nr, np_, 1.0, rtol=1e-15, max_cycles=2)
+    box = _box(nr, np_, 1.0, rtol=1e-15, max_cycles=2, direct_cols=4)
     rng = np.random.default_rng(0)
     rhs = np.ascontiguousarray(rng.standard_normal((np_ + 1, nr + 2)))
     psi, psip = np.zeros_like(rhs), np.zeros_like(rhs)
