@@ -197,6 +197,7 @@ int build_tables(vv_run *r)
         h.insert(h.end(), t.fact.begin(), t.fact.end());
         h.insert(h.end(), t.bp.begin(), t.bp.end());
         h.insert(h.end(), t.c.begin(), t.c.end());
+        for (double v : t.bp) h.push_back(1.0 / v);
         int st = upload_tab(r, &r->tab_r1, h);
         if (st) return st;
     }
@@ -210,6 +211,7 @@ int build_tables(vv_run *r)
         h.insert(h.end(), t.fact.begin(), t.fact.end());
         h.insert(h.end(), t.bp.begin(), t.bp.end());
         h.insert(h.end(), t.c.begin(), t.c.end());
+        for (double v : t.bp) h.push_back(1.0 / v);
         int st = upload_tab(r, &r->tab_rT, h);
         if (st) return st;
     }
@@ -226,6 +228,7 @@ int build_tables(vv_run *r)
         h.push_back(t.fA);
         h.push_back(t.fB);
         h.push_back(t.fC);
+        for (double v : t.b) h.push_back(v != 0.0 ? 1.0 / v : 0.0);   // entry 0 is unused
         int st = upload_tab(r, &r->tab_p, h);
         if (st) return st;
     }
@@ -352,7 +355,6 @@ int vv_destroy(vv_run *r)
                         r->tab_p, r->mg_pool, r->diag, r->part};
     for (double *f : fields)
         if (f) (void)hipFree(f);
-    if (r->mg_dev) (void)hipFree(r->mg_dev);
     if (r->mg_state) (void)hipFree(r->mg_state);
     for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
     if (r->ev0) (void)hipEventDestroy(r->ev0);

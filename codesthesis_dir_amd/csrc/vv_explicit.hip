@@ -21,6 +21,20 @@ constexpr int WG = 256;
 
 __device__ __forceinline__ size_t ix(int i, int k, int ld) { return (size_t)k * ld + i; }
 
+// x / b, correctly rounded, from rb = 1/b correctly rounded (computed on the
+// host): q0 = x*rb is within one ulp of x/b, the remainder x - b*q0 is exact
+// in one FMA, and q0 + rem*rb rounds to x/b (Markstein's theorem for
+// division with a fused multiply-add; no overflow or underflow occurs for
+// these operands).  A zero remainder means q0 is exact, which also keeps the
+// sign of a zero quotient.  Three dependent operations replace the IEEE
+// division sequence on the recurrence's critical path, with the same result.
+__device__ __forceinline__ double div_rb(double x, double b, double rb)
+{
+    const double q0 = x * rb;
+    const double e = __builtin_fma(-b, q0, x);
+    return (e == 0.0) ? q0 : __builtin_fma(e, rb, q0);
+}
+
 // single-precision literals of rb.f:660-668 promoted to double
 __device__ __forceinline__ double f23() { return (double)(2.f / 3.f); }
 __device__ __forceinline__ double f16() { return (double)(1.f / 6.f); }
@@ -105,7 +119,7 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, int T)
     const int n = (FAM == 2) ? nr : nr - 1;
     const int nP = n | 1;
     const double *tab = (FAM == 2) ? a.tab_rT : a.tab_r1;
-    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n;
+    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n, *rbp = tab + 3 * n;
     double *out = (FAM == 0) ? a.dCr : (FAM == 1) ? a.dthrC : a.dthrT;
     const int k0 = blockIdx.x * T;
     if (k0 > np) return;
@@ -133,18 +147,20 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, int T)
             rp = r[j] - fact[j] * rp;
             r[j] = rp;
         }
-        double y = rp / bp[n - 1];
+        double y = div_rb(rp, bp[n - 1], rbp[n - 1]);
         r[n - 1] = y;
         j = n - 2;
         for (; j - U + 1 >= 0; j -= U) {
-            double cr[U], br[U], rr[U];
+            double cr[U], br[U], ir[U], rr[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) { cr[u] = cc[j - u]; br[u] = bp[j - u]; rr[u] = r[j - u]; }
+            for (int u = 0; u < U; ++u) {
+                cr[u] = cc[j - u]; br[u] = bp[j - u]; ir[u] = rbp[j - u]; rr[u] = r[j - u];
+            }
 #pragma unroll
-            for (int u = 0; u < U; ++u) { y = (rr[u] - cr[u] * y) / br[u]; r[j - u] = y; }
+            for (int u = 0; u < U; ++u) { y = div_rb(rr[u] - cr[u] * y, br[u], ir[u]); r[j - u] = y; }
         }
         for (; j >= 0; --j) {
-            y = (r[j] - cc[j] * y) / bp[j];
+            y = div_rb(r[j] - cc[j] * y, bp[j], rbp[j]);
             r[j] = y;
         }
     }
@@ -180,6 +196,7 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
         const double *f1 = a.tab_p, *f2 = a.tab_p + s, *aa = a.tab_p + 2 * s;
         const double *bb = a.tab_p + 3 * s, *cc = a.tab_p + 4 * s;
         const double fA = a.tab_p[5 * s], fB = a.tab_p[5 * s + 1], fC = a.tab_p[5 * s + 2];
+        const double *rbb = a.tab_p + 5 * s + 3;
         const int c = threadIdx.x;
 #define R_(j) R[((j) - 1) * T + c]
         constexpr int U = 8;   // blocked as in tdma_tile
@@ -207,22 +224,25 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
         const double rn1 = R_(n - 1) - fA * R_(n - 2);
         rn = rn - fB * R_(n - 2);
         rn = rn - fC * rn1;
-        const double yn = rn / bb[n];
-        double y = (rn1 - cc[n - 1] * yn) / bb[n - 1];
+        const double yn = div_rb(rn, bb[n], rbb[n]);
+        double y = div_rb(rn1 - cc[n - 1] * yn, bb[n - 1], rbb[n - 1]);
         R_(n) = yn;
         R_(n - 1) = y;
         j = n - 2;
         for (; j - U + 1 >= 1; j -= U) {
-            double cr[U], ar[U], br[U], rr[U];
+            double cr[U], ar[U], br[U], ir[U], rr[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                cr[u] = cc[j - u]; ar[u] = aa[j - u]; br[u] = bb[j - u]; rr[u] = R_(j - u);
+                cr[u] = cc[j - u]; ar[u] = aa[j - u]; br[u] = bb[j - u]; ir[u] = rbb[j - u]; rr[u] = R_(j - u);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) { y = (rr[u] - cr[u] * y - ar[u] * yn) / br[u]; R_(j - u) = y; }
+            for (int u = 0; u < U; ++u) {
+                y = div_rb(rr[u] - cr[u] * y - ar[u] * yn, br[u], ir[u]);
+                R_(j - u) = y;
+            }
         }
         for (; j >= 1; --j) {
-            y = (R_(j) - cc[j] * y - aa[j] * yn) / bb[j];
+            y = div_rb(R_(j) - cc[j] * y - aa[j] * yn, bb[j], rbb[j]);
             R_(j) = y;
         }
 #undef R_
@@ -407,13 +427,26 @@ __global__ void __launch_bounds__(WG) k_heat(const double *th, int nr, int np, i
         if (threadIdx.x == 0) {
             if (base == 1) { first_b = xb[0]; first_t = xt[0]; }
             const int cnt = (np - base + 1 < WG) ? np - base + 1 : WG;
-#pragma unroll 8
-            for (int j = 0; j < cnt; ++j) {
-                const int k = base + j;
-                const double nb = (k == np) ? first_b : xb[j + 1];
-                const double nt = (k == np) ? first_t : xt[j + 1];
-                t1 = t1 + xb[j] + nb;
-                t2 = t2 + xt[j] + nt;
+            // the neighbour of column np is column 1 (periodic); slot cnt of
+            // the last chunk is free for it
+            if (base + cnt - 1 == np) { xb[cnt] = first_b; xt[cnt] = first_t; }
+            // the reference's order, t1 = (t1 + x_k) + x_{k+1}; blocks of U
+            // terms are read from LDS before their dependent additions
+            constexpr int U = 16;
+            int j = 0;
+            for (; j + U <= cnt; j += U) {
+                double b[U + 1], t[U + 1];
+#pragma unroll
+                for (int u = 0; u <= U; ++u) { b[u] = xb[j + u]; t[u] = xt[j + u]; }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    t1 = t1 + b[u] + b[u + 1];
+                    t2 = t2 + t[u] + t[u + 1];
+                }
+            }
+            for (; j < cnt; ++j) {
+                t1 = t1 + xb[j] + xb[j + 1];
+                t2 = t2 + xt[j] + xt[j + 1];
             }
         }
         __syncthreads();

@@ -82,7 +82,8 @@ struct MgProblemDev {
 
 constexpr int MG_NORM_MAXB = 1024;   // workgroups of the residual-norm kernel
 
-// one batch of problems solved together (v and u of one run)
+// one batch of problems solved together (v and u of one run); host-side
+// description of the hierarchy and the solver state (device pointers)
 struct MgBatchDev {
     int nprob;
     MgProblemDev prob[MG_MAXPROB];
@@ -122,13 +123,12 @@ struct vv_run {
     double *rC = nullptr, *rT = nullptr;
 
     // elimination tables (device)
-    double *tab_r1 = nullptr;   // tdma 1-4-1, n = nr-1: fact | bp | c
+    double *tab_r1 = nullptr;   // tdma 1-4-1, n = nr-1: fact | bp | c | 1/bp
     double *tab_rT = nullptr;   // tdma with closure rows, n = nr
-    double *tab_p = nullptr;    // ptdma 1-4-1, n = np+1: f1 | f2 | a | b | c | fA fB fC
+    double *tab_p = nullptr;    // ptdma 1-4-1, n = np+1: f1 | f2 | a | b | c | fA fB fC | 1/b
 
     // multigrid
     vv::MgBatchDev mg{};
-    vv::MgBatchDev *mg_dev = nullptr;
     double *mg_pool = nullptr;
     // solver state, one device block read back with a single copy:
     //   bytes 0-31 int done[2], cycles[2], unused[4];

@@ -33,11 +33,35 @@ namespace vv {
 namespace {
 
 constexpr int MAXN = 2048;     // rows per line held in LDS
+constexpr int MAXS = 11;       // cyclic-reduction steps for n <= MAXN
+
+// Line smoothing modes of k_mg_zebra.
+enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
+
+// Level and problem descriptors are kernel arguments (passed by value), so
+// no dependent load precedes a kernel's first data load.
+struct KLev {
+    double *u, *f;
+    const double *lu, *twid;
+    double *xm;
+    double cphi;
+    int ncol, nstep, tabsz;
+};
+struct KProb {
+    const double *sten, *rphi;
+    int n;
+    KLev L, F, Cc;     // the launch's level, the finer level, the coarser level
+};
+struct KArgs {
+    KProb P[MG_MAXPROB];
+    const int *done, *cycles;
+    int color, mode, lev0;
+};
 
 __device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
 
 // residual f - (D_r u + cphi*rphi*(u_{q+1} - 2u + u_{q-1})) at (j, q)
-__device__ __forceinline__ double resid_at(const MgProblemDev &P, const MgLevelDev &L, int j, int q)
+__device__ __forceinline__ double resid_at(const KProb &P, const KLev &L, int j, int q)
 {
     const int n = P.n, ncol = L.ncol;
     const double *u = L.u;
@@ -57,14 +81,9 @@ __device__ __forceinline__ double resid_at(const MgProblemDev &P, const MgLevelD
     return L.f[cix(j, q, n)] - lu;
 }
 
-constexpr int MAXS = 11;       // cyclic-reduction steps for n <= MAXN
-
-// Line smoothing modes of k_mg_zebra.
-enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
-
-// Restricted residual of level lev-1 at coarse line qc, row j (full
+// Restricted residual of the finer level F at coarse line qc, row j (full
 // weighting along phi, centred on fine line 2qc+1).
-__device__ __forceinline__ double restricted_at(const MgProblemDev &P, const MgLevelDev &F, int j, int qc)
+__device__ __forceinline__ double restricted_at(const KProb &P, const KLev &F, int j, int qc)
 {
     const int q0 = 2 * qc, q1 = 2 * qc + 1, q2 = (2 * qc + 2) % F.ncol;
     const double r0 = resid_at(P, F, j, q0);
@@ -78,22 +97,21 @@ __device__ __forceinline__ double restricted_at(const MgProblemDev &P, const MgL
 // ZB_RESTRICT: f is the restricted residual of the finer level (returned in
 //   *fr); on colour 0 the neighbours are the zero initial correction.
 // ZB_PROLONG (colour 0 only): the neighbours are odd lines 2qc+1, read with
-//   the coarse correction e(qc) of level lev+1 added.
+//   the coarse correction e(qc) of the coarser level added.
 template <int MODE>
-__device__ __forceinline__ double line_rhs(const MgProblemDev &P, int lev, int j, int q, int qm, int qp,
-                                           int color, double *fr)
+__device__ __forceinline__ double line_rhs(const KProb &P, int j, int q, int qm, int qp, int color, double *fr)
 {
     const int n = P.n;
-    const MgLevelDev &L = P.lev[lev];
+    const KLev &L = P.L;
     const double c = L.cphi;
     if (MODE == ZB_RESTRICT) {
-        const double f = restricted_at(P, P.lev[lev - 1], j, q);
+        const double f = restricted_at(P, P.F, j, q);
         *fr = f;
         if (color == 0) return f;
         return f - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
     }
     if (MODE == ZB_PROLONG) {
-        const double *e = P.lev[lev + 1].u;
+        const double *e = P.Cc.u;
         const double um = L.u[cix(j, qm, n)] + e[cix(j, qm >> 1, n)];
         const double up = L.u[cix(j, qp, n)] + e[cix(j, qp >> 1, n)];
         return L.f[cix(j, q, n)] - c * P.rphi[j] * (um + up);
@@ -101,15 +119,15 @@ __device__ __forceinline__ double line_rhs(const MgProblemDev &P, int lev, int j
     return L.f[cix(j, q, n)] - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
 }
 
-// Solves one r-line in LDS by parallel cyclic reduction.  Each thread owns
-// rows tid + k*nt (k < R) and holds their reduction coefficients in
-// registers (loaded before the first barrier, so no global load sits on the
-// chain of steps).  rhs[k] is the thread's right-hand side; with FOLD the
-// closure rows are folded here, otherwise the caller has folded them.  The
-// solution is returned in x[k].
-template <int R, bool FOLD>
-__device__ __forceinline__ void pcr_line(const double *__restrict__ tab, int S, int n, double (*buf)[MAXN],
-                                         double rhs[R], double x[R])
+// Solves NL r-lines that share one operator, in LDS, by parallel cyclic
+// reduction.  Each thread owns rows tid + k*nt (k < R) of every line and
+// holds their reduction coefficients in registers (loaded before the first
+// barrier, so no global load sits on the chain of steps).  rhs[l][k] is the
+// thread's right-hand side of line l; with FOLD the closure rows are folded
+// here, otherwise the caller has folded them.  Solutions return in x[l][k].
+template <int R, int NL, bool FOLD>
+__device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S, int n,
+                                          double (*buf)[2][MAXN], double rhs[NL][R], double x[NL][R])
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
@@ -124,16 +142,25 @@ __device__ __forceinline__ void pcr_line(const double *__restrict__ tab, int S, 
             b[k][t] = (in && t < S) ? beta[(size_t)t * n + i] : 0.0;
         }
         di[k] = in ? dinv[i] : 0.0;
-        if (in) buf[0][i] = rhs[k];
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+            if (in) buf[l][0][i] = rhs[l][k];
     }
     __syncthreads();
-    if (FOLD && n > 2) {   // fold the closure rows here: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
-        double r0 = 0.0, rN = 0.0;
-        if (tid == 0) r0 = buf[0][0] - tab[0] * buf[0][1];
-        if (tid == (n - 1) % nt) rN = buf[0][n - 1] - tab[1] * buf[0][n - 2];
+    if (FOLD && n > 2) {   // r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
+        const bool o0 = (tid == 0), oN = (tid == (n - 1) % nt);
+        double r0[NL], rN[NL];
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            r0[l] = o0 ? buf[l][0][0] - tab[0] * buf[l][0][1] : 0.0;
+            rN[l] = oN ? buf[l][0][n - 1] - tab[1] * buf[l][0][n - 2] : 0.0;
+        }
         __syncthreads();
-        if (tid == 0) buf[0][0] = r0;
-        if (tid == (n - 1) % nt) buf[0][n - 1] = rN;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            if (o0) buf[l][0][0] = r0[l];
+            if (oN) buf[l][0][n - 1] = rN[l];
+        }
         __syncthreads();
     }
     int cur = 0;
@@ -145,10 +172,13 @@ __device__ __forceinline__ void pcr_line(const double *__restrict__ tab, int S, 
             for (int k = 0; k < R; ++k) {
                 const int i = tid + k * nt;
                 if (i < n) {
-                    double v = buf[cur][i];
-                    if (i >= s) v += a[k][t] * buf[cur][i - s];
-                    if (i + s < n) v += b[k][t] * buf[cur][i + s];
-                    buf[cur ^ 1][i] = v;
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) {
+                        double v = buf[l][cur][i];
+                        if (i >= s) v += a[k][t] * buf[l][cur][i - s];
+                        if (i + s < n) v += b[k][t] * buf[l][cur][i + s];
+                        buf[l][cur ^ 1][i] = v;
+                    }
                 }
             }
             __syncthreads();
@@ -158,173 +188,210 @@ __device__ __forceinline__ void pcr_line(const double *__restrict__ tab, int S, 
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        x[k] = (i < n) ? buf[cur][i] * di[k] : 0.0;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) x[l][k] = (i < n) ? buf[l][cur][i] * di[k] : 0.0;
     }
 }
 
 // One colour of r-lines of one level: line q = 2*blockIdx.x + colour.
 template <int R>
-__global__ void __launch_bounds__(1024) k_mg_zebra(const MgBatchDev *__restrict__ Bp, int lev, int color, int mode)
+__global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
 {
-    const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    if (lev >= P.nlev) return;
-    const MgLevelDev &L = P.lev[lev];
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
     const int n = P.n, ncol = L.ncol;
-    const int q = 2 * blockIdx.x + color;
+    const int q = 2 * blockIdx.x + a.color;
     if (ncol < 2 || q >= ncol) return;
+    const int done = a.done[p];            // its latency overlaps the loads below
     const int qm = (q == 0) ? ncol - 1 : q - 1;
     const int qp = (q == ncol - 1) ? 0 : q + 1;
-    __shared__ double buf[2][MAXN];
+    __shared__ double buf[1][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
     const double *tab = L.lu;
-    double rhs[R], x[R];
+    const int mode = a.mode, color = a.color;
+    double rhs[1][R], x[1][R], fo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rhs[k] = 0.0;
+        rhs[0][k] = 0.0;
+        fo[k] = 0.0;
         if (i >= n) continue;
-        double fr = 0.0, fr1 = 0.0, v;
-        if (mode == ZB_RESTRICT) {
-            v = line_rhs<ZB_RESTRICT>(P, lev, i, q, qm, qp, color, &fr);
-            L.f[cix(i, q, n)] = fr;
-        } else if (mode == ZB_PROLONG) {
-            v = line_rhs<ZB_PROLONG>(P, lev, i, q, qm, qp, color, &fr);
-        } else {
-            v = line_rhs<ZB_PLAIN>(P, lev, i, q, qm, qp, color, &fr);
-        }
+        double fr1 = 0.0, v;
+        if (mode == ZB_RESTRICT) v = line_rhs<ZB_RESTRICT>(P, i, q, qm, qp, color, &fo[k]);
+        else if (mode == ZB_PROLONG) v = line_rhs<ZB_PROLONG>(P, i, q, qm, qp, color, &fo[k]);
+        else v = line_rhs<ZB_PLAIN>(P, i, q, qm, qp, color, &fo[k]);
         // fold the closure rows: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
         if (n > 2 && (i == 0 || i == n - 1)) {
             const int j = (i == 0) ? 1 : n - 2;
             double w;
-            if (mode == ZB_RESTRICT) w = line_rhs<ZB_RESTRICT>(P, lev, j, q, qm, qp, color, &fr1);
-            else if (mode == ZB_PROLONG) w = line_rhs<ZB_PROLONG>(P, lev, j, q, qm, qp, color, &fr1);
-            else w = line_rhs<ZB_PLAIN>(P, lev, j, q, qm, qp, color, &fr1);
+            if (mode == ZB_RESTRICT) w = line_rhs<ZB_RESTRICT>(P, j, q, qm, qp, color, &fr1);
+            else if (mode == ZB_PROLONG) w = line_rhs<ZB_PROLONG>(P, j, q, qm, qp, color, &fr1);
+            else w = line_rhs<ZB_PLAIN>(P, j, q, qm, qp, color, &fr1);
             v = v - ((i == 0) ? tab[0] : tab[1]) * w;
         }
-        rhs[k] = v;
+        rhs[0][k] = v;
     }
-    pcr_line<R, false>(tab, L.nstep, n, buf, rhs, x);
+    if (done) return;                       // uniform over the workgroup
+    if (mode == ZB_RESTRICT) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int i = tid + k * nt;
+            if (i < n) L.f[cix(i, q, n)] = fo[k];
+        }
+    }
+    pcr_lines<R, 1, false>(tab, L.nstep, n, buf, rhs, x);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        if (i < n) L.u[cix(i, q, n)] = x[k];
+        if (i < n) L.u[cix(i, q, n)] = x[0][k];
     }
 }
 
-// fine level lev -> coarse lev+1: full weighting of the residual along phi;
-// the coarse correction starts from zero.
-__global__ void __launch_bounds__(256) k_mg_restrict(const MgBatchDev *__restrict__ Bp, int lev)
+// finer level F -> this level L (the coarsest): full weighting of the
+// residual along phi; the correction starts from zero.
+__global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
 {
-    const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    if (lev + 1 >= P.nlev) return;
-    const MgLevelDev &F = P.lev[lev];
-    const MgLevelDev &Cc = P.lev[lev + 1];
-    const int n = P.n, nc = Cc.ncol;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, nc = L.ncol;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * nc) return;
+    const int done = a.done[p];
     const int qc = (int)(t / n), j = (int)(t - (long long)qc * n);
-    const int q0 = 2 * qc, q1 = 2 * qc + 1, q2 = (2 * qc + 2) % F.ncol;
-    const double r0 = resid_at(P, F, j, q0);
-    const double r1 = resid_at(P, F, j, q1);
-    const double r2 = resid_at(P, F, j, q2);
-    Cc.f[cix(j, qc, n)] = 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
-    Cc.u[cix(j, qc, n)] = 0.0;
+    const double v = restricted_at(P, P.F, j, qc);
+    if (done) return;
+    L.f[cix(j, qc, n)] = v;
+    L.u[cix(j, qc, n)] = 0.0;
 }
 
-// Exact solve of the coarsest level with C = ncol columns: one workgroup per
-// real Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
-// ml = C-1: the Nyquist mode), the projection of the residual on that mode,
-// then the mode's line system D_r + lambda_m diag(rphi) by parallel cyclic
-// reduction.  Below level 0 the correction starts from zero, so the residual
-// is f.  On level 0 (the whole solve is direct) the first cycle solves for u
-// from f (the initial guess is not needed by an exact solve) and later
-// cycles project f - Lu: iterative refinement.
+// Exact solve of the coarsest level with C = ncol columns (a power of two).
+// Workgroup w handles Fourier mode m = w: the real lines cos (ml = 2m-1)
+// and sin (ml = 2m) of 0 < m < C/2 share the mode's line operator
+// D_r + lambda_m diag(rphi); w = 0 is m = 0 (ml = 0) and w = C/2 the
+// Nyquist mode (ml = C-1).  The residual is projected on the lines, which
+// are then solved by parallel cyclic reduction.  Below level 0 the
+// correction starts from zero, so the residual is f.  On level 0 (the whole
+// solve is direct) the first cycle solves for u from f (the initial guess
+// is not needed by an exact solve) and later cycles project f - Lu:
+// iterative refinement.
 template <int R>
-__global__ void __launch_bounds__(1024) k_mg_direct(const MgBatchDev *__restrict__ Bp, int lev)
+__global__ void __launch_bounds__(1024) k_mg_direct(KArgs a)
 {
-    const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    if (lev >= P.nlev) return;
-    const MgLevelDev &L = P.lev[lev];
-    const int n = P.n, C = L.ncol;      // C is a power of two
-    const int ml = blockIdx.x;
-    if (ml >= C) return;
-    int m, sinpart;
-    if (ml == 0) { m = 0; sinpart = 0; }
-    else if (C >= 2 && ml == C - 1) { m = C / 2; sinpart = 0; }
-    else { m = (ml + 1) / 2; sinpart = (ml & 1) ? 0 : 1; }
-    __shared__ double buf[2][MAXN];
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, C = L.ncol;
+    const int m = blockIdx.x;
+    if (m > C / 2) return;
+    const int done = a.done[p];
+    const bool pair = (m > 0 && 2 * m < C);
+    const int ml0 = (m == 0) ? 0 : (pair ? 2 * m - 1 : C - 1);
+    const bool from_res = a.lev0 && a.cycles[p] > 0;
+    __shared__ double buf[2][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
     const double *f = L.f;
-    const double *tw = L.twid + (sinpart ? C : 0);
+    const double *cs = L.twid, *sn = L.twid + C;
     const int msk = C - 1;
-    double rhs[R], x[R];
+    double rhs[2][R], x[2][R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        double acc = 0.0;
+        double ac = 0.0, as = 0.0;
         if (i < n) {
-            if (lev == 0 && B.cycles[p] > 0) {
-                for (int q = 0; q < C; ++q) acc += resid_at(P, L, i, q) * tw[(m * q) & msk];
+            if (from_res) {
+                for (int q = 0; q < C; ++q) {
+                    const double v = resid_at(P, L, i, q);
+                    const int j = (m * q) & msk;
+                    ac += v * cs[j];
+                    as += v * sn[j];
+                }
             } else {
-#pragma unroll 8
-                for (int q = 0; q < C; ++q) acc += f[cix(i, q, n)] * tw[(m * q) & msk];
+#pragma unroll 16
+                for (int q = 0; q < C; ++q) {
+                    const double v = f[cix(i, q, n)];
+                    const int j = (m * q) & msk;
+                    ac += v * cs[j];
+                    as += v * sn[j];
+                }
             }
         }
-        rhs[k] = acc;
+        rhs[0][k] = ac;
+        rhs[1][k] = as;
     }
-    pcr_line<R, true>(L.lu + (size_t)m * L.tabsz, L.nstep, n, buf, rhs, x);
+    if (done) return;
+    const double *tab = L.lu + (size_t)m * L.tabsz;
+    if (pair) {
+        pcr_lines<R, 2, true>(tab, L.nstep, n, buf, rhs, x);
+    } else {
+        pcr_lines<R, 1, true>(tab, L.nstep, n, buf, rhs, x);
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        if (i < n) L.xm[(size_t)ml * n + i] = x[k];
+        if (i < n) {
+            L.xm[(size_t)ml0 * n + i] = x[0][k];
+            if (pair) L.xm[(size_t)(ml0 + 1) * n + i] = x[1][k];
+        }
     }
 }
 
-// synthesis of the coarsest-level correction from its mode lines, added
-// (stored, at the first cycle of a solve that is direct on level 0)
-__global__ void __launch_bounds__(256) k_mg_idft(const MgBatchDev *__restrict__ Bp, int lev)
+// Synthesis of the coarsest-level correction from its mode lines, added
+// (stored, at the first cycle of a solve that is direct on level 0).  A
+// thread computes Q consecutive columns of one row from one pass over the
+// mode lines.
+template <int Q>
+__global__ void __launch_bounds__(256) k_mg_idft(KArgs a)
 {
-    const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    if (lev >= P.nlev) return;
-    const MgLevelDev &L = P.lev[lev];
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
     const int n = P.n, C = L.ncol;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * C) return;
-    const int q = (int)(t / n), i = (int)(t - (long long)q * n);
+    if (t >= (long long)n * (C / Q)) return;
+    const int done = a.done[p];
+    const bool store = a.lev0 && a.cycles[p] == 0;
+    const int qg = (int)(t / n), i = (int)(t - (long long)qg * n);
+    const int q0 = qg * Q;
     const double *X = L.xm;
     const double *cs = L.twid, *sn = L.twid + C;
     const int msk = C - 1;
-    double v = X[i];
-#pragma unroll 4
+    double v[Q];
+    const double x0 = X[i];
+#pragma unroll
+    for (int u = 0; u < Q; ++u) v[u] = x0;
+#pragma unroll 8
     for (int m = 1; m < (C + 1) / 2; ++m) {
-        const int j = (m * q) & msk;
-        v += 2.0 * (X[(size_t)(2 * m - 1) * n + i] * cs[j] + X[(size_t)(2 * m) * n + i] * sn[j]);
+        const double xc = X[(size_t)(2 * m - 1) * n + i];
+        const double xs = X[(size_t)(2 * m) * n + i];
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+            const int j = (m * (q0 + u)) & msk;
+            v[u] += 2.0 * (xc * cs[j] + xs * sn[j]);
+        }
     }
-    if (C >= 2) v += X[(size_t)(C - 1) * n + i] * ((q & 1) ? -1.0 : 1.0);
-    if (lev == 0 && B.cycles[p] == 0) L.u[cix(i, q, n)] = v / (double)C;   // first direct solve
-    else L.u[cix(i, q, n)] += v / (double)C;
+    if (C >= 2) {
+        const double xn = X[(size_t)(C - 1) * n + i];
+#pragma unroll
+        for (int u = 0; u < Q; ++u) v[u] += xn * (((q0 + u) & 1) ? -1.0 : 1.0);
+    }
+    if (done) return;
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+        double *o = L.u + cix(i, q0 + u, n);
+        if (store) *o = v[u] / (double)C;
+        else *o += v[u] / (double)C;
+    }
 }
 
 // max-norm of the level-0 residual: grid-stride, one partial per workgroup
-__global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ Bp)
+__global__ void __launch_bounds__(256) k_mg_norm(KArgs a, double *npart)
 {
-    const MgBatchDev &B = *Bp;
     const int p = blockIdx.y;
-    if (p >= B.nprob || B.done[p]) return;
-    const MgProblemDev &P = B.prob[p];
-    const MgLevelDev &L = P.lev[0];
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    if (a.done[p]) return;
     const long long cnt = (long long)P.n * L.ncol;
     const long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
@@ -334,46 +401,64 @@ __global__ void __launch_bounds__(256) k_mg_norm(const MgBatchDev *__restrict__ 
     }
     __shared__ double sh[4];
     m = block_max(m, sh);
-    if (threadIdx.x == 0) B.npart[p * MG_NORM_MAXB + blockIdx.x] = m;
+    if (threadIdx.x == 0) npart[p * MG_NORM_MAXB + blockIdx.x] = m;
 }
+
+struct StopArgs {
+    int *done, *cycles;
+    unsigned long long *fnorm;
+    double *relres;
+    const double *npart, *fpart;
+    int nbn, nbr, maxc;
+    double rtol;
+};
 
 // Stop test of one cycle (one workgroup): combines the residual partials of
 // k_mg_norm and, at the first cycle of a solve, the |f| partials of the
 // right-hand-side kernel; converged when the relative residual is <= rtol
-// (a NaN residual also stops), capped at maxc cycles.
-__global__ void __launch_bounds__(256) k_mg_stop(const MgBatchDev *__restrict__ Bp, int nbn, double rtol, int maxc)
+// (a NaN residual also stops), capped at maxc cycles.  All loads are issued
+// before the reductions.
+__global__ void __launch_bounds__(256) k_mg_stop(StopArgs a)
 {
-    const MgBatchDev &B = *Bp;
     __shared__ double sh[4];
-    for (int p = 0; p < B.nprob; ++p) {
-        if (B.done[p]) continue;             // uniform over the workgroup
-        double m = 0.0;
-        for (int b = threadIdx.x; b < nbn; b += blockDim.x) m = nanmax(m, B.npart[p * MG_NORM_MAXB + b]);
-        m = block_max(m, sh);
-        const int c0 = B.cycles[p];
-        double fn = 0.0;
-        if (c0 == 0) {
-            for (int b = threadIdx.x; b < B.nbr; b += blockDim.x) fn = nanmax(fn, B.fpart[p * B.nbr + b]);
-            fn = block_max(fn, sh);
+    int dn[MG_MAXPROB], cy[MG_MAXPROB];
+    double fo[MG_MAXPROB], mr[MG_MAXPROB], mf[MG_MAXPROB];
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        dn[p] = a.done[p];
+        cy[p] = a.cycles[p];
+        fo[p] = __longlong_as_double((long long)a.fnorm[p]);
+        mr[p] = 0.0;
+        mf[p] = 0.0;
+        for (int b = threadIdx.x; b < a.nbn; b += blockDim.x) mr[p] = nanmax(mr[p], a.npart[p * MG_NORM_MAXB + b]);
+        for (int b = threadIdx.x; b < a.nbr; b += blockDim.x) mf[p] = nanmax(mf[p], a.fpart[p * a.nbr + b]);
+    }
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        mr[p] = block_max(mr[p], sh);
+        mf[p] = block_max(mf[p], sh);
+    }
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        if (dn[p]) continue;
+        double fn = fo[p];
+        if (cy[p] == 0) {
+            fn = mf[p];
+            a.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
         }
-        if (threadIdx.x == 0) {
-            if (c0 == 0) B.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
-            else fn = __longlong_as_double((long long)B.fnorm[p]);
-            const double rel = m / (fn > 0.0 ? fn : 1.0);
-            const int c = c0 + 1;
-            B.cycles[p] = c;
-            B.relres[p] = rel;
-            if (!(rel > rtol) || c >= maxc) B.done[p] = 1;
-        }
-        __syncthreads();
+        const double rel = mr[p] / (fn > 0.0 ? fn : 1.0);
+        const int c = cy[p] + 1;
+        a.cycles[p] = c;
+        a.relres[p] = rel;
+        if (!(rel > a.rtol) || c >= a.maxc) a.done[p] = 1;
     }
 }
 
 // level 0 -> fields: psip <- psi, psi <- solution (rb.f:1894-1913)
-__global__ void __launch_bounds__(256) k_mg_out(const MgBatchDev *__restrict__ Bp, double *vel, double *velp,
+__global__ void __launch_bounds__(256) k_mg_out(const double *uv, const double *uu, double *vel, double *velp,
                                                 double *uel, double *uelp, int nr, int np, int ld, int which)
 {
-    const MgBatchDev &B = *Bp;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int rows = nr + 2;
     if (t >= (long long)rows * (np + 1)) return;
@@ -384,14 +469,14 @@ __global__ void __launch_bounds__(256) k_mg_out(const MgBatchDev *__restrict__ B
         const double old = vel[o];
         velp[o] = old;
         double nv = 0.0;
-        if (i >= 1 && i <= nr) nv = B.prob[0].lev[0].u[cix(i - 1, q, nr)];
+        if (i >= 1 && i <= nr) nv = uv[cix(i - 1, q, nr)];
         vel[o] = nv;
     }
     if ((which & 2) && i <= nr) {
         const double old = uel[o];
         uelp[o] = old;
         double nu = 0.0;
-        if (i >= 1 && i <= nr - 1) nu = B.prob[1].lev[0].u[cix(i - 1, q, nr - 1)];
+        if (i >= 1 && i <= nr - 1) nu = uu[cix(i - 1, q, nr - 1)];
         uel[o] = nu;
     }
 }
@@ -573,8 +658,6 @@ int mg_build(vv_run *r)
     r->mg.npart = r->npart;
     r->mg.fpart = r->fpart;
     r->mg.nbr = r->nbx;
-    VV_HIP(hipMalloc(&r->mg_dev, sizeof(vv::MgBatchDev)));
-    VV_HIP(hipMemcpyAsync(r->mg_dev, &r->mg, sizeof(vv::MgBatchDev), hipMemcpyHostToDevice, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     r->nlev_v = r->nlev_u = nlev;
     return VV_OK;
@@ -596,6 +679,38 @@ static int zebra_threads(int n)
     return t > 1024 ? 1024 : t;
 }
 
+static KLev klev(const MgLevelDev &L)
+{
+    KLev k;
+    k.u = L.u; k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm;
+    k.cphi = L.cphi; k.ncol = L.ncol; k.nstep = L.nstep; k.tabsz = (int)L.tabsz;
+    return k;
+}
+
+// kernel arguments of a launch on level lev (with its finer and coarser
+// neighbours where they exist)
+static KArgs kargs(const vv_run *r, int lev, int color, int mode)
+{
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        const MgProblemDev &Q = r->mg.prob[p];
+        KProb &P = a.P[p];
+        P.sten = Q.sten;
+        P.rphi = Q.rphi;
+        P.n = Q.n;
+        P.L = klev(Q.lev[lev]);
+        if (lev > 0) P.F = klev(Q.lev[lev - 1]);
+        if (lev + 1 < Q.nlev) P.Cc = klev(Q.lev[lev + 1]);
+    }
+    a.done = r->mg.done;
+    a.cycles = r->mg.cycles;
+    a.color = color;
+    a.mode = mode;
+    a.lev0 = (lev == 0);
+    return a;
+}
+
 // One V(nu1,nu2) cycle of both problems.  Levels 0..L-1 are smoothed by
 // zebra line sweeps, level L is solved exactly.  The restriction into a
 // smoothed level is fused into its first sweep (ZB_RESTRICT), the
@@ -605,24 +720,23 @@ static int zebra_threads(int n)
 static int enqueue_cycle(vv_run *r)
 {
     vv::MgBatchDev &B = r->mg;
-    const vv::MgBatchDev *Bd = r->mg_dev;
     const int L = B.prob[0].nlev - 1;
     const int n = r->nr;
     const double ntot = (double)(B.prob[0].n + B.prob[1].n);
     const int zt = zebra_threads(n);
     const bool two = n > 1024;
-    const double rtol = r->cfg.rtol;
     hipEvent_t pe;
     auto ncol = [&](int l) { return B.prob[0].lev[l].ncol; };
     auto zebra = [&](int l, int c, int mode) {
         double bytes = zebra_bytes(B, l);
         if (mode == ZB_RESTRICT) bytes += 8.0 * ntot * (ncol(l - 1) + 0.5 * ncol(l));
         if (mode == ZB_PROLONG) bytes += 8.0 * ntot * ncol(l + 1);
+        const KArgs a = kargs(r, l, c, mode);
         prof_begin(r, &pe);
         if (two)
-            hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, Bd, l, c, mode);
+            hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
         else
-            hipLaunchKernelGGL(k_mg_zebra<1>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, Bd, l, c, mode);
+            hipLaunchKernelGGL(k_mg_zebra<1>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_ZEBRA, pe, bytes);
     };
     for (int l = 0; l < L; ++l)
@@ -630,20 +744,28 @@ static int enqueue_cycle(vv_run *r)
             for (int c = 0; c < 2; ++c) zebra(l, c, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN);
     {   // exact solve of the coarsest level
         const int C = ncol(L);
+        const KArgs a = kargs(r, L, 0, ZB_PLAIN);
         if (L > 0) {
             const long long cnt = (long long)n * C;
             prof_begin(r, &pe);
-            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, Bd, L - 1);
+            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, a);
             prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncol(L - 1) + 2.0 * C));
         }
         prof_begin(r, &pe);
         if (two)
-            hipLaunchKernelGGL(k_mg_direct<2>, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
+            hipLaunchKernelGGL(k_mg_direct<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         else
-            hipLaunchKernelGGL(k_mg_direct<1>, dim3(C, 2), dim3(zt), 0, r->stream, Bd, L);
+            hipLaunchKernelGGL(k_mg_direct<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_DIRECT, pe, (L == 0 ? 24.0 : 16.0) * ntot * C);
         prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_idft, dim3(nblk((long long)n * C, 256), 2), dim3(256), 0, r->stream, Bd, L);
+        const int Q = C >= 8 ? 8 : C;
+        const dim3 g(nblk((long long)n * (C / Q), 256), 2);
+        switch (Q) {
+        case 8: hipLaunchKernelGGL(k_mg_idft<8>, g, dim3(256), 0, r->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_mg_idft<4>, g, dim3(256), 0, r->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_mg_idft<2>, g, dim3(256), 0, r->stream, a); break;
+        default: hipLaunchKernelGGL(k_mg_idft<1>, g, dim3(256), 0, r->stream, a); break;
+        }
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)
@@ -654,9 +776,13 @@ static int enqueue_cycle(vv_run *r)
     int nb = (int)((cnt0 + 255) / 256);
     if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
+    StopArgs sa;
+    sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres;
+    sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
+    sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, Bd);
-    hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, Bd, nb, rtol, r->cfg.max_cycles);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, 0, ZB_PLAIN), B.npart);
+    hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, sa);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -719,8 +845,9 @@ int launch_mg_scatter_out(vv_run *r, int which)
     const int w = (which == 5) ? 3 : (which == 2 ? 1 : 2);
     hipEvent_t pe;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_out, dim3(nblk(n, 256)), dim3(256), 0, r->stream, r->mg_dev, r->vel, r->velp,
-                       r->uel, r->uelp, r->nr, r->np, r->ld, w);
+    hipLaunchKernelGGL(k_mg_out, dim3(nblk(n, 256)), dim3(256), 0, r->stream,
+                       (const double *)r->mg.prob[0].lev[0].u, (const double *)r->mg.prob[1].lev[0].u,
+                       r->vel, r->velp, r->uel, r->uelp, r->nr, r->np, r->ld, w);
     prof_end(r, VV_K_OUT, pe, 8.0 * 6.0 * (double)n);
     VV_HIP(hipGetLastError());
     return VV_OK;
