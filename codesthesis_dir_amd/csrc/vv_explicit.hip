@@ -113,17 +113,20 @@ __device__ __forceinline__ double rhs_p(const CompactArgs &a, int i, int j)
 // thread per column then runs the recurrences in the reference's order, and
 // the workgroup writes the solutions back.  LDS row stride nP is odd.
 template <int FAM>
-__device__ void tdma_tile(const CompactArgs &a, double *R, int T)
+__device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 {
     const int nr = a.nr, ld = a.ld, np = a.np;
     const int n = (FAM == 2) ? nr : nr - 1;
     const int nP = n | 1;
     const double *tab = (FAM == 2) ? a.tab_rT : a.tab_r1;
-    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n, *rbp = tab + 3 * n;
     double *out = (FAM == 0) ? a.dCr : (FAM == 1) ? a.dthrC : a.dthrT;
     const int k0 = blockIdx.x * T;
     if (k0 > np) return;
     const int tc = (np + 1 - k0 < T) ? np + 1 - k0 : T;
+    // the elimination factors go to LDS with the right-hand sides, so the
+    // recurrences below read nothing from global memory
+    for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) Tab[idx] = tab[idx];
+    const double *fact = Tab, *bp = Tab + n, *cc = Tab + 2 * n, *rbp = Tab + 3 * n;
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int c = idx / n, j = idx - c * n;
         R[c * nP + j] = rhs_r<FAM>(a, j + 1, k0 + c);
@@ -177,7 +180,7 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, int T)
 // (j-1)*T + c, so the per-row recurrences of consecutive threads touch
 // consecutive words.
 template <int FAM>
-__device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
+__device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 {
     const int np = a.np, ld = a.ld;
     const int n = np + 1;
@@ -187,16 +190,18 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
     if (i0 > rows) return;
     const int tc = (rows + 1 - i0 < T) ? rows + 1 - i0 : T;
     double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
+    // elimination factors to LDS (see tdma_tile)
+    for (int idx = threadIdx.x; idx < 6 * s + 3; idx += blockDim.x) Tab[idx] = a.tab_p[idx];
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int j = idx / tc, c = idx - j * tc;
         R[j * T + c] = rhs_p<FAM>(a, i0 + c, j + 1);
     }
     __syncthreads();
     if ((int)threadIdx.x < tc) {
-        const double *f1 = a.tab_p, *f2 = a.tab_p + s, *aa = a.tab_p + 2 * s;
-        const double *bb = a.tab_p + 3 * s, *cc = a.tab_p + 4 * s;
-        const double fA = a.tab_p[5 * s], fB = a.tab_p[5 * s + 1], fC = a.tab_p[5 * s + 2];
-        const double *rbb = a.tab_p + 5 * s + 3;
+        const double *f1 = Tab, *f2 = Tab + s, *aa = Tab + 2 * s;
+        const double *bb = Tab + 3 * s, *cc = Tab + 4 * s;
+        const double fA = Tab[5 * s], fB = Tab[5 * s + 1], fC = Tab[5 * s + 2];
+        const double *rbb = Tab + 5 * s + 3;
         const int c = threadIdx.x;
 #define R_(j) R[((j) - 1) * T + c]
         constexpr int U = 8;   // blocked as in tdma_tile
@@ -256,20 +261,22 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, int T)
 
 struct CompactTiles {
     int Tr, Tp;    // columns per workgroup (r families), rows per workgroup (phi families)
+    int rwords;    // LDS words of the right-hand-side tile; the tables follow
 };
 
 __global__ void __launch_bounds__(WG) k_compact(CompactArgs a, CompactTiles t)
 {
     extern __shared__ double R[];
+    double *Tab = R + t.rwords;
     const int fam = blockIdx.y;
     if (!(a.mask & (1 << fam))) return;
     switch (fam) {
-    case 0: tdma_tile<0>(a, R, t.Tr); break;
-    case 1: tdma_tile<1>(a, R, t.Tr); break;
-    case 2: tdma_tile<2>(a, R, t.Tr); break;
-    case 3: ptdma_tile<3>(a, R, t.Tp); break;
-    case 4: ptdma_tile<4>(a, R, t.Tp); break;
-    case 5: ptdma_tile<5>(a, R, t.Tp); break;
+    case 0: tdma_tile<0>(a, R, Tab, t.Tr); break;
+    case 1: tdma_tile<1>(a, R, Tab, t.Tr); break;
+    case 2: tdma_tile<2>(a, R, Tab, t.Tr); break;
+    case 3: ptdma_tile<3>(a, R, Tab, t.Tp); break;
+    case 4: ptdma_tile<4>(a, R, Tab, t.Tp); break;
+    case 5: ptdma_tile<5>(a, R, Tab, t.Tp); break;
     }
 }
 
@@ -659,7 +666,14 @@ static int compact_launch(vv_run *r, int mask)
     t.Tp = budget / (r->np + 1);
     if (t.Tp > 64) t.Tp = 64;
     if (t.Tp < 1) t.Tp = 1;
-    const size_t lds = (size_t)budget * 8;
+    t.rwords = budget;
+    // + the elimination tables: 4n words (r families, n <= nr), 6(np+2)+3 (phi)
+    const int twords = (4 * r->nr > 6 * (r->np + 2) + 3) ? 4 * r->nr : 6 * (r->np + 2) + 3;
+    const size_t lds = (size_t)(budget + twords) * 8;
+    if (lds > 65536) {   // gfx950 has 160 KB of LDS per CU; above 64 KB it is requested explicitly
+        VV_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_compact),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     const int gr = nblk(r->np + 1, t.Tr), gp = nblk(r->nr, t.Tp);
     dim3 grid(gr > gp ? gr : gp, 6);
     hipEvent_t pe_;

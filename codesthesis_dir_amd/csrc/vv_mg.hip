@@ -340,7 +340,8 @@ __global__ void __launch_bounds__(1024) k_mg_direct(KArgs a)
 // Synthesis of the coarsest-level correction from its mode lines, added
 // (stored, at the first cycle of a solve that is direct on level 0).  A
 // thread computes Q consecutive columns of one row from one pass over the
-// mode lines.
+// mode lines; the column group is blockIdx.z, so the twiddle indices are
+// uniform over the workgroup.
 template <int Q>
 __global__ void __launch_bounds__(256) k_mg_idft(KArgs a)
 {
@@ -348,12 +349,11 @@ __global__ void __launch_bounds__(256) k_mg_idft(KArgs a)
     const KProb &P = a.P[p];
     const KLev &L = P.L;
     const int n = P.n, C = L.ncol;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * (C / Q)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     const int done = a.done[p];
     const bool store = a.lev0 && a.cycles[p] == 0;
-    const int qg = (int)(t / n), i = (int)(t - (long long)qg * n);
-    const int q0 = qg * Q;
+    const int q0 = blockIdx.z * Q;
     const double *X = L.xm;
     const double *cs = L.twid, *sn = L.twid + C;
     const int msk = C - 1;
@@ -759,7 +759,7 @@ static int enqueue_cycle(vv_run *r)
         prof_end(r, VV_K_DIRECT, pe, (L == 0 ? 24.0 : 16.0) * ntot * C);
         prof_begin(r, &pe);
         const int Q = C >= 8 ? 8 : C;
-        const dim3 g(nblk((long long)n * (C / Q), 256), 2);
+        const dim3 g(nblk(n, 256), 2, C / Q);
         switch (Q) {
         case 8: hipLaunchKernelGGL(k_mg_idft<8>, g, dim3(256), 0, r->stream, a); break;
         case 4: hipLaunchKernelGGL(k_mg_idft<4>, g, dim3(256), 0, r->stream, a); break;
