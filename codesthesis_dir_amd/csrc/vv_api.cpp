@@ -320,7 +320,8 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rC, rc))) return fail(st);
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
-    if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = 128;
+    if (r->cfg.direct_cols <= 0)   // default: a V-cycle with a direct level of <= 128 columns
+        r->cfg.direct_cols = (cfg->np / 2 < 128) ? (cfg->np / 2 > 1 ? cfg->np / 2 : 1) : 128;
     r->nbx = field_blocks(r);
     {
         const size_t np_ = 2 * (size_t)r->nbx + 3 * (size_t)r->nbx + 2 * (size_t)vv::MG_NORM_MAXB;

@@ -266,18 +266,96 @@ __global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
     L.u[cix(j, qc, n)] = 0.0;
 }
 
-// Exact solve of the coarsest level with C = ncol columns (a power of two).
-// Workgroup w handles Fourier mode m = w: the real lines cos (ml = 2m-1)
-// and sin (ml = 2m) of 0 < m < C/2 share the mode's line operator
-// D_r + lambda_m diag(rphi); w = 0 is m = 0 (ml = 0) and w = C/2 the
-// Nyquist mode (ml = C-1).  The residual is projected on the lines, which
-// are then solved by parallel cyclic reduction.  Below level 0 the
+// Exact solve of the coarsest level with C = ncol columns (a power of two):
+// the operator is invariant along phi, so the real DFT over the C columns
+// decouples it into C line systems D_r + lambda_m diag(rphi), one per real
+// Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
+// ml = C-1: the Nyquist mode).  Three kernels: k_mg_dft<0> projects the
+// residual on the lines, k_mg_modes solves each line by parallel cyclic
+// reduction, k_mg_dft<1> synthesises the correction.  Below level 0 the
 // correction starts from zero, so the residual is f.  On level 0 (the whole
-// solve is direct) the first cycle solves for u from f (the initial guess
-// is not needed by an exact solve) and later cycles project f - Lu:
-// iterative refinement.
+// solve is direct) the first cycle solves for u from f (the initial guess is
+// not needed by an exact solve) and later cycles project f - Lu: iterative
+// refinement.
+
+constexpr int DFT_TN = 64;     // rows per workgroup
+constexpr int DFT_TM = 16;     // output lines (columns) per workgroup
+constexpr int DFT_TK = 64;     // inner-dimension chunk staged in LDS
+
+// Dense transform over the C columns of DFT_TN rows:
+//   INV = 0: xm[o][i] = sum_k Wf[o][k] y[k][i], y = f (or the residual);
+//   INV = 1: u[o][i] (+)= (1/C) sum_k Wi[o][k] xm[k][i].
+// The weights (Wf | Wi, C x C each) are tabulated on the host; each
+// workgroup stages DFT_TK x DFT_TN inputs and DFT_TM x DFT_TK weights in
+// LDS per chunk.  Wave w computes outputs o0 + w*opt .. + opt - 1 of row
+// i0 + lane.
+template <int INV>
+__global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
+{
+    const int p = blockIdx.y;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, C = L.ncol;
+    const int i0 = blockIdx.x * DFT_TN, o0 = blockIdx.z * DFT_TM;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int TM = C < DFT_TM ? C : DFT_TM;
+    const int opt = (TM + 3) / 4;
+    const int done = a.done[p];
+    const bool from_res = !INV && a.lev0 && a.cycles[p] > 0;
+    const bool store = INV && a.lev0 && a.cycles[p] == 0;
+    __shared__ double Ys[DFT_TK][DFT_TN];
+    __shared__ double Ws[DFT_TM][DFT_TK];
+    const double *W = INV ? L.twid + (size_t)C * C : L.twid;
+    const double *Y = INV ? L.xm : L.f;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < C; k0 += DFT_TK) {
+        const int tk = (C - k0 < DFT_TK) ? C - k0 : DFT_TK;
+        for (int idx = threadIdx.x; idx < tk * DFT_TN; idx += blockDim.x) {
+            const int kk = idx / DFT_TN, ii = idx - kk * DFT_TN;
+            const int gi = i0 + ii;
+            double v = 0.0;
+            if (gi < n) v = from_res ? resid_at(P, L, gi, k0 + kk) : Y[(size_t)(k0 + kk) * n + gi];
+            Ys[kk][ii] = v;
+        }
+        for (int idx = threadIdx.x; idx < TM * tk; idx += blockDim.x) {
+            const int oo = idx / tk, kk = idx - oo * tk;
+            Ws[oo][kk] = W[(size_t)(o0 + oo) * C + k0 + kk];
+        }
+        __syncthreads();
+        for (int kk = 0; kk < tk; ++kk) {
+            const double y = Ys[kk][lane];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int oo = w * opt + j;
+                if (j < opt && oo < TM) acc[j] += Ws[oo][kk] * y;
+            }
+        }
+        __syncthreads();
+    }
+    const int i = i0 + lane;
+    if (done || i >= n) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int oo = w * opt + j;
+        if (j >= opt || oo >= TM) continue;
+        const int o = o0 + oo;
+        if (!INV) {
+            L.xm[(size_t)o * n + i] = acc[j];
+        } else {
+            double *d = L.u + cix(i, o, n);
+            const double v = acc[j] / (double)C;
+            if (store) *d = v;
+            else *d += v;
+        }
+    }
+}
+
+// Line solves of the coarsest level: workgroup w handles Fourier mode
+// m = w; the cos and sin lines of 0 < m < C/2 share the mode's operator
+// (one coefficient table), m = 0 and the Nyquist mode m = C/2 have one line.
+// Reads the projections from xm and overwrites them with the solutions.
 template <int R>
-__global__ void __launch_bounds__(1024) k_mg_direct(KArgs a)
+__global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
 {
     const int p = blockIdx.y;
     const KProb &P = a.P[p];
@@ -288,45 +366,19 @@ __global__ void __launch_bounds__(1024) k_mg_direct(KArgs a)
     const int done = a.done[p];
     const bool pair = (m > 0 && 2 * m < C);
     const int ml0 = (m == 0) ? 0 : (pair ? 2 * m - 1 : C - 1);
-    const bool from_res = a.lev0 && a.cycles[p] > 0;
     __shared__ double buf[2][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double *f = L.f;
-    const double *cs = L.twid, *sn = L.twid + C;
-    const int msk = C - 1;
     double rhs[2][R], x[2][R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        double ac = 0.0, as = 0.0;
-        if (i < n) {
-            if (from_res) {
-                for (int q = 0; q < C; ++q) {
-                    const double v = resid_at(P, L, i, q);
-                    const int j = (m * q) & msk;
-                    ac += v * cs[j];
-                    as += v * sn[j];
-                }
-            } else {
-#pragma unroll 16
-                for (int q = 0; q < C; ++q) {
-                    const double v = f[cix(i, q, n)];
-                    const int j = (m * q) & msk;
-                    ac += v * cs[j];
-                    as += v * sn[j];
-                }
-            }
-        }
-        rhs[0][k] = ac;
-        rhs[1][k] = as;
+        rhs[0][k] = (i < n) ? L.xm[(size_t)ml0 * n + i] : 0.0;
+        rhs[1][k] = (i < n && pair) ? L.xm[(size_t)(ml0 + 1) * n + i] : 0.0;
     }
     if (done) return;
     const double *tab = L.lu + (size_t)m * L.tabsz;
-    if (pair) {
-        pcr_lines<R, 2, true>(tab, L.nstep, n, buf, rhs, x);
-    } else {
-        pcr_lines<R, 1, true>(tab, L.nstep, n, buf, rhs, x);
-    }
+    if (pair) pcr_lines<R, 2, true>(tab, L.nstep, n, buf, rhs, x);
+    else pcr_lines<R, 1, true>(tab, L.nstep, n, buf, rhs, x);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
@@ -334,54 +386,6 @@ __global__ void __launch_bounds__(1024) k_mg_direct(KArgs a)
             L.xm[(size_t)ml0 * n + i] = x[0][k];
             if (pair) L.xm[(size_t)(ml0 + 1) * n + i] = x[1][k];
         }
-    }
-}
-
-// Synthesis of the coarsest-level correction from its mode lines, added
-// (stored, at the first cycle of a solve that is direct on level 0).  A
-// thread computes Q consecutive columns of one row from one pass over the
-// mode lines; the column group is blockIdx.z, so the twiddle indices are
-// uniform over the workgroup.
-template <int Q>
-__global__ void __launch_bounds__(256) k_mg_idft(KArgs a)
-{
-    const int p = blockIdx.y;
-    const KProb &P = a.P[p];
-    const KLev &L = P.L;
-    const int n = P.n, C = L.ncol;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int done = a.done[p];
-    const bool store = a.lev0 && a.cycles[p] == 0;
-    const int q0 = blockIdx.z * Q;
-    const double *X = L.xm;
-    const double *cs = L.twid, *sn = L.twid + C;
-    const int msk = C - 1;
-    double v[Q];
-    const double x0 = X[i];
-#pragma unroll
-    for (int u = 0; u < Q; ++u) v[u] = x0;
-#pragma unroll 8
-    for (int m = 1; m < (C + 1) / 2; ++m) {
-        const double xc = X[(size_t)(2 * m - 1) * n + i];
-        const double xs = X[(size_t)(2 * m) * n + i];
-#pragma unroll
-        for (int u = 0; u < Q; ++u) {
-            const int j = (m * (q0 + u)) & msk;
-            v[u] += 2.0 * (xc * cs[j] + xs * sn[j]);
-        }
-    }
-    if (C >= 2) {
-        const double xn = X[(size_t)(C - 1) * n + i];
-#pragma unroll
-        for (int u = 0; u < Q; ++u) v[u] += xn * (((q0 + u) & 1) ? -1.0 : 1.0);
-    }
-    if (done) return;
-#pragma unroll
-    for (int u = 0; u < Q; ++u) {
-        double *o = L.u + cix(i, q0 + u, n);
-        if (store) *o = v[u] / (double)C;
-        else *o += v[u] / (double)C;
     }
 }
 
@@ -540,7 +544,7 @@ int mg_build(vv_run *r)
     const int nr = r->nr, np = r->np;
     const double hhri = 1.0 / (r->dr * r->dr);
     if (nr > MAXN) { set_error("nr exceeds the line length held in LDS (2048)"); return VV_EINVAL; }
-    const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 128;
+    const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 1;
     int nlev = 1;
     for (int c = np; c > Cd; c /= 2) ++nlev;
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
@@ -609,8 +613,21 @@ int mg_build(vv_run *r)
                     L.tabsz = t.size();
                     L.tab.insert(L.tab.end(), t.begin(), t.end());
                 }
-                for (int j = 0; j < c; ++j) L.tab.push_back(std::cos(2.0 * pi * j / c));
-                for (int j = 0; j < c; ++j) L.tab.push_back(std::sin(2.0 * pi * j / c));
+                // transform weights: Wf[ml][k] projects column k on real
+                // line ml; Wi[q][ml] synthesises column q (times 1/c on
+                // the device).  Angles are reduced mod c before the cos/sin.
+                auto w = [&](int ml, int k, bool inv) {
+                    if (ml == 0) return 1.0;
+                    if (c >= 2 && ml == c - 1) return (k & 1) ? -1.0 : 1.0;
+                    const int mm = (ml + 1) / 2;
+                    const double ang = 2.0 * pi * (double)(((long long)mm * k) % c) / c;
+                    const double v = (ml & 1) ? std::cos(ang) : std::sin(ang);
+                    return inv ? 2.0 * v : v;
+                };
+                for (int ml = 0; ml < c; ++ml)
+                    for (int k = 0; k < c; ++k) L.tab.push_back(w(ml, k, false));
+                for (int q = 0; q < c; ++q)
+                    for (int ml = 0; ml < c; ++ml) L.tab.push_back(w(ml, q, true));
                 total += 3 * (size_t)n * c + L.tab.size();
             }
             lh[p].push_back(std::move(L));
@@ -751,21 +768,19 @@ static int enqueue_cycle(vv_run *r)
             hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, a);
             prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncol(L - 1) + 2.0 * C));
         }
+        const int TM = C < DFT_TM ? C : DFT_TM;
+        const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
         prof_begin(r, &pe);
+        // projection (read the level, write the mode lines) and line solves
+        // (read and write the mode lines)
+        hipLaunchKernelGGL(k_mg_dft<0>, gd, dim3(256), 0, r->stream, a);
         if (two)
-            hipLaunchKernelGGL(k_mg_direct<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
+            hipLaunchKernelGGL(k_mg_modes<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         else
-            hipLaunchKernelGGL(k_mg_direct<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
-        prof_end(r, VV_K_DIRECT, pe, (L == 0 ? 24.0 : 16.0) * ntot * C);
+            hipLaunchKernelGGL(k_mg_modes<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
+        prof_end(r, VV_K_DIRECT, pe, 32.0 * ntot * C);
         prof_begin(r, &pe);
-        const int Q = C >= 8 ? 8 : C;
-        const dim3 g(nblk(n, 256), 2, C / Q);
-        switch (Q) {
-        case 8: hipLaunchKernelGGL(k_mg_idft<8>, g, dim3(256), 0, r->stream, a); break;
-        case 4: hipLaunchKernelGGL(k_mg_idft<4>, g, dim3(256), 0, r->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_mg_idft<2>, g, dim3(256), 0, r->stream, a); break;
-        default: hipLaunchKernelGGL(k_mg_idft<1>, g, dim3(256), 0, r->stream, a); break;
-        }
+        hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)

@@ -162,7 +162,7 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%stats = 0
   cfg%geom = 0
   cfg%Re = 0.0d0
-  cfg%direct_cols = 128
+  cfg%direct_cols = 0
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl
 end subroutine

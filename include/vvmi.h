@@ -72,7 +72,8 @@ typedef struct vv_config {
     int    direct_cols;   /* phi coarsening stops at <= this many columns,
                              where the level is solved exactly (real DFT in
                              phi + one line solve per mode); >= np makes the
-                             whole solve direct.  0 selects 128.            */
+                             whole solve direct.  0 selects min(128, np/2):
+                             a V-cycle with at least one smoothed level.    */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */
