@@ -125,8 +125,10 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     const int tc = (np + 1 - k0 < T) ? np + 1 - k0 : T;
     // the elimination factors go to LDS with the right-hand sides, so the
     // recurrences below read nothing from global memory
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) Tab[idx] = tab[idx];
     const double *fact = Tab, *bp = Tab + n, *cc = Tab + 2 * n, *rbp = Tab + 3 * n;
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int c = idx / n, j = idx - c * n;
         R[c * nP + j] = rhs_r<FAM>(a, j + 1, k0 + c);
@@ -168,6 +170,7 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
         }
     }
     __syncthreads();
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int c = idx / n, j = idx - c * n;
         out[ix(j + 1, k0 + c, ld)] = R[c * nP + j];
@@ -191,7 +194,9 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     const int tc = (rows + 1 - i0 < T) ? rows + 1 - i0 : T;
     double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
     // elimination factors to LDS (see tdma_tile)
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < 6 * s + 3; idx += blockDim.x) Tab[idx] = a.tab_p[idx];
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int j = idx / tc, c = idx - j * tc;
         R[j * T + c] = rhs_p<FAM>(a, i0 + c, j + 1);
@@ -253,6 +258,7 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 #undef R_
     }
     __syncthreads();
+#pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int j = idx / tc, c = idx - j * tc;
         out[ix(i0 + c, j, ld)] = R[j * T + c];

@@ -308,20 +308,41 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
     const double *W = INV ? L.twid + (size_t)C * C : L.twid;
     const double *Y = INV ? L.xm : L.f;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // staging: every thread issues all its loads into registers before the
+    // LDS stores, so the loads of a chunk are in flight together
+    constexpr int YPT = DFT_TK * DFT_TN / 256, WPT = DFT_TM * DFT_TK / 256;
+    const int gi = i0 + lane;
     for (int k0 = 0; k0 < C; k0 += DFT_TK) {
         const int tk = (C - k0 < DFT_TK) ? C - k0 : DFT_TK;
-        for (int idx = threadIdx.x; idx < tk * DFT_TN; idx += blockDim.x) {
-            const int kk = idx / DFT_TN, ii = idx - kk * DFT_TN;
-            const int gi = i0 + ii;
-            double v = 0.0;
-            if (gi < n) v = from_res ? resid_at(P, L, gi, k0 + kk) : Y[(size_t)(k0 + kk) * n + gi];
-            Ys[kk][ii] = v;
+        double yv[YPT], wv[WPT];
+        if (from_res) {
+#pragma unroll
+            for (int r = 0; r < YPT; ++r) {
+                const int kk = r * 4 + w;
+                yv[r] = (kk < tk && gi < n) ? resid_at(P, L, gi, k0 + kk) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < YPT; ++r) {
+                const int kk = r * 4 + w;
+                yv[r] = (kk < tk && gi < n) ? Y[(size_t)(k0 + kk) * n + gi] : 0.0;
+            }
         }
-        for (int idx = threadIdx.x; idx < TM * tk; idx += blockDim.x) {
-            const int oo = idx / tk, kk = idx - oo * tk;
-            Ws[oo][kk] = W[(size_t)(o0 + oo) * C + k0 + kk];
+#pragma unroll
+        for (int r = 0; r < WPT; ++r) {
+            const int idx = threadIdx.x + r * 256;
+            const int oo = idx / DFT_TK, kk = idx - oo * DFT_TK;
+            wv[r] = (oo < TM && kk < tk) ? W[(size_t)(o0 + oo) * C + k0 + kk] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < YPT; ++r) Ys[r * 4 + w][lane] = yv[r];
+#pragma unroll
+        for (int r = 0; r < WPT; ++r) {
+            const int idx = threadIdx.x + r * 256;
+            Ws[idx / DFT_TK][idx % DFT_TK] = wv[r];
         }
         __syncthreads();
+#pragma unroll 8
         for (int kk = 0; kk < tk; ++kk) {
             const double y = Ys[kk][lane];
 #pragma unroll
@@ -332,7 +353,7 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
         }
         __syncthreads();
     }
-    const int i = i0 + lane;
+    const int i = gi;
     if (done || i >= n) return;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
