@@ -286,6 +286,13 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     if (r->cfg.max_cycles <= 0) r->cfg.max_cycles = 100;
     if (r->cfg.nu1 <= 0) r->cfg.nu1 = 1;
     if (r->cfg.nu2 <= 0) r->cfg.nu2 = 1;
+    if ((r->cfg.nu1 + r->cfg.nu2) % 2 != 0) {
+        // the sweeps alternate between two buffers per level; an even count
+        // returns the solution to the first one at the end of each cycle
+        set_error("nu1 + nu2 must be even");
+        delete r;
+        return VV_EINVAL;
+    }
     if (r->cfg.robert < 0) r->cfg.robert = 0.02;
     r->device = device;
     r->nr = cfg->nr;

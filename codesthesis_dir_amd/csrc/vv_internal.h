@@ -58,6 +58,7 @@ int make_pcr(int n, const std::vector<double> &sub2, const std::vector<double> &
 // (closure rows of the v operator are folded into rows 0 and n-1 first).
 struct MgLevelDev {
     double *u, *f;
+    double *u2;           // second iterate buffer of a smoothed level (sweeps ping-pong)
     const double *lu;
     double cphi;          // 1/hp^2 at this level
     int ncol, nstep;

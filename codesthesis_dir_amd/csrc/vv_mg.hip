@@ -9,9 +9,10 @@
 // r-coarsening (a point-smoother V-cycle with rediscretised coarse grids
 // diverges on it), so the cycle here never coarsens r:
 //   - smoother: zebra r-line Gauss-Seidel; every line is solved exactly by
-//     parallel cyclic reduction in LDS, one workgroup per line, with the
-//     reduction coefficients of the level's line operator precomputed on the
-//     host (the closure rows are first folded into a tridiagonal system);
+//     parallel cyclic reduction in LDS, with the reduction coefficients of
+//     the level's line operator precomputed on the host (the closure rows
+//     are first folded into a tridiagonal system); both colours of a sweep
+//     run in one launch (k_mg_zebra);
 //   - coarsening: phi only, full weighting / linear interpolation, down to
 //     direct_cols columns or fewer; that level is solved exactly: the
 //     operator is invariant along phi, so a real DFT over the level's
@@ -47,7 +48,7 @@ enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
 // Level and problem descriptors are kernel arguments (passed by value), so
 // no dependent load precedes a kernel's first data load.
 struct KLev {
-    double *u, *f;
+    double *u, *uo, *f;     // current iterate, output buffer of a sweep, rhs
     const double *lu, *twid;
     double *xm;
     double cphi;
@@ -61,7 +62,7 @@ struct KProb {
 struct KArgs {
     KProb P[MG_MAXPROB];
     const int *done, *cycles;
-    int color, mode, lev0;
+    int mode, lev0;
 };
 
 __device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
@@ -96,33 +97,6 @@ __device__ __forceinline__ double restricted_at(const KProb &P, const KLev &F, i
     const double r1 = resid_at(P, F, j, q1);
     const double r2 = resid_at(P, F, j, q2);
     return 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
-}
-
-// Right-hand side of the line solve of row j of line q:
-//   f - c rphi (u_{q-1} + u_{q+1})
-// ZB_RESTRICT: f is the restricted residual of the finer level (returned in
-//   *fr); on colour 0 the neighbours are the zero initial correction.
-// ZB_PROLONG (colour 0 only): the neighbours are odd lines 2qc+1, read with
-//   the coarse correction e(qc) of the coarser level added.
-template <int MODE>
-__device__ __forceinline__ double line_rhs(const KProb &P, int j, int q, int qm, int qp, int color, double *fr)
-{
-    const int n = P.n;
-    const KLev &L = P.L;
-    const double c = L.cphi;
-    if (MODE == ZB_RESTRICT) {
-        const double f = restricted_at(P, P.F, j, q);
-        *fr = f;
-        if (color == 0) return f;
-        return f - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
-    }
-    if (MODE == ZB_PROLONG) {
-        const double *e = P.Cc.u;
-        const double um = L.u[cix(j, qm, n)] + e[cix(j, qm >> 1, n)];
-        const double up = L.u[cix(j, qp, n)] + e[cix(j, qp >> 1, n)];
-        return L.f[cix(j, q, n)] - c * P.rphi[j] * (um + up);
-    }
-    return L.f[cix(j, q, n)] - c * P.rphi[j] * (L.u[cix(j, qm, n)] + L.u[cix(j, qp, n)]);
 }
 
 // Solves NL r-lines that share one operator, in LDS, by parallel cyclic
@@ -199,7 +173,19 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
     }
 }
 
-// One colour of r-lines of one level: line q = 2*blockIdx.x + colour.
+// One zebra sweep (both colours) of one level, from buffer u to buffer uo.
+// Workgroup bx owns black (odd) line qb = 2bx+1 and red (even) line
+// qr0 = 2bx; it also solves the red line qr1 = qb+1 that the black line
+// needs (the workgroup that owns qr1 computes the same values from the same
+// inputs), so both colours run in one launch:
+//   red lines qr0, qr1 from the old black lines (u), in one reduction pass
+//   (the lines share the level's operator); black line qb from the new red
+//   values, which stay in registers.
+// Modes: ZB_RESTRICT (first pre-sweep of a coarse level): f is the restricted
+// residual of the finer level (stored for the owned lines) and the old
+// iterate is the zero initial correction; ZB_PROLONG (first post-sweep): the
+// old black lines are read with the coarser level's correction added (a line
+// solve overwrites its own line, so only these reads need it).
 template <int R>
 __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
 {
@@ -207,50 +193,73 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
     const KProb &P = a.P[p];
     const KLev &L = P.L;
     const int n = P.n, ncol = L.ncol;
-    const int q = 2 * blockIdx.x + a.color;
-    if (ncol < 2 || q >= ncol) return;
+    const int qb = 2 * blockIdx.x + 1;
+    if (ncol < 2 || qb >= ncol) return;
     const int done = a.done[p];            // its latency overlaps the loads below
-    const int qm = (q == 0) ? ncol - 1 : q - 1;
-    const int qp = (q == ncol - 1) ? 0 : q + 1;
-    __shared__ double buf[1][2][MAXN];
+    const int qr0 = qb - 1;
+    const int qr1 = (qb + 1 == ncol) ? 0 : qb + 1;
+    const int qbm = (qr0 == 0) ? ncol - 1 : qr0 - 1;          // black left of qr0
+    const int qbp = (qr1 == ncol - 1) ? 0 : qr1 + 1;          // black right of qr1
+    __shared__ double buf[2][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double *tab = L.lu;
-    const int mode = a.mode, color = a.color;
-    double rhs[1][R], x[1][R], fo[R];
+    const double c = L.cphi;
+    const int mode = a.mode;
+    const double *u = L.u;
+    const double *e = P.Cc.u;
+    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rhs[0][k] = 0.0;
-        fo[k] = 0.0;
+        rr[0][k] = rr[1][k] = 0.0;
+        fb[k] = fr0[k] = 0.0;
         if (i >= n) continue;
-        double fr1 = 0.0, v;
-        if (mode == ZB_RESTRICT) v = line_rhs<ZB_RESTRICT>(P, i, q, qm, qp, color, &fo[k]);
-        else if (mode == ZB_PROLONG) v = line_rhs<ZB_PROLONG>(P, i, q, qm, qp, color, &fo[k]);
-        else v = line_rhs<ZB_PLAIN>(P, i, q, qm, qp, color, &fo[k]);
-        // fold the closure rows: r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
-        if (n > 2 && (i == 0 || i == n - 1)) {
-            const int j = (i == 0) ? 1 : n - 2;
-            double w;
-            if (mode == ZB_RESTRICT) w = line_rhs<ZB_RESTRICT>(P, j, q, qm, qp, color, &fr1);
-            else if (mode == ZB_PROLONG) w = line_rhs<ZB_PROLONG>(P, j, q, qm, qp, color, &fr1);
-            else w = line_rhs<ZB_PLAIN>(P, j, q, qm, qp, color, &fr1);
-            v = v - ((i == 0) ? tab[0] : tab[1]) * w;
+        const double rp = P.rphi[i];
+        if (mode == ZB_RESTRICT) {
+            const double f0 = restricted_at(P, P.F, i, qr0);
+            const double f1 = restricted_at(P, P.F, i, qr1);
+            fb[k] = restricted_at(P, P.F, i, qb);
+            fr0[k] = f0;
+            rr[0][k] = f0;
+            rr[1][k] = f1;
+        } else if (mode == ZB_PROLONG) {
+            const double um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
+            const double ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
+            const double up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
+            rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (um0 + ub);
+            rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + up1);
+            fb[k] = L.f[cix(i, qb, n)];
+        } else {
+            const double ub = u[cix(i, qb, n)];
+            rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (u[cix(i, qbm, n)] + ub);
+            rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + u[cix(i, qbp, n)]);
+            fb[k] = L.f[cix(i, qb, n)];
         }
-        rhs[0][k] = v;
     }
     if (done) return;                       // uniform over the workgroup
     if (mode == ZB_RESTRICT) {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int i = tid + k * nt;
-            if (i < n) L.f[cix(i, q, n)] = fo[k];
+            if (i < n) {
+                L.f[cix(i, qr0, n)] = fr0[k];
+                L.f[cix(i, qb, n)] = fb[k];
+            }
         }
     }
-    pcr_lines<R, 1, false>(tab, L.nstep, n, buf, rhs, x);
+    pcr_lines<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        if (i < n) L.u[cix(i, q, n)] = x[0][k];
+        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : 0.0;
+    }
+    pcr_lines<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        if (i < n) {
+            L.uo[cix(i, qr0, n)] = xr[0][k];
+            L.uo[cix(i, qb, n)] = xb[0][k];
+        }
     }
 }
 
@@ -624,7 +633,7 @@ int mg_build(vv_run *r)
                 for (int j = 0; j < n; ++j) d2[j] = dg[j] - 2.0 * L.cphi * rphi[j];
                 L.S = make_pcr(n, s2, s1, d2, p1, p2, L.tab);
                 L.tabsz = L.tab.size();
-                total += 2 * (size_t)n * c + L.tab.size();
+                total += 3 * (size_t)n * c + L.tab.size();
             } else {
                 // one table per Fourier mode m = 0..c/2: the phi second
                 // difference on c periodic points has eigenvalue
@@ -683,6 +692,8 @@ int mg_build(vv_run *r)
             L.tabsz = H.tabsz;
             L.u = r->mg_pool + off; off += (size_t)n * H.ncol;
             L.f = r->mg_pool + off; off += (size_t)n * H.ncol;
+            L.u2 = nullptr;
+            if (!H.direct) { L.u2 = r->mg_pool + off; off += (size_t)n * H.ncol; }
             std::memcpy(host.data() + off, H.tab.data(), H.tab.size() * sizeof(double));
             L.lu = r->mg_pool + off;
             L.twid = H.direct ? L.lu + (size_t)(H.ncol / 2 + 1) * H.tabsz : nullptr;
@@ -705,33 +716,26 @@ int mg_build(vv_run *r)
     return VV_OK;
 }
 
-// algorithmic bytes of one zebra half-sweep of level l over both problems:
-// read f and the other colour, write the own colour (3 words per line point)
-static double zebra_bytes(const vv::MgBatchDev &B, int l)
-{
-    double b = 0.0;
-    for (int p = 0; p < B.nprob; ++p)
-        b += 24.0 * (double)B.prob[p].n * (B.prob[p].lev[l].ncol / 2);
-    return b;
-}
-
 static int zebra_threads(int n)
 {
     int t = ((n + 63) / 64) * 64;
     return t > 1024 ? 1024 : t;
 }
 
-static KLev klev(const MgLevelDev &L)
+// descriptor of level L whose current iterate is buffer `cur` (0: u, 1: u2)
+static KLev klev(const MgLevelDev &L, int cur)
 {
     KLev k;
-    k.u = L.u; k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm;
+    k.u = cur ? L.u2 : L.u;
+    k.uo = cur ? L.u : L.u2;
+    k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm;
     k.cphi = L.cphi; k.ncol = L.ncol; k.nstep = L.nstep; k.tabsz = (int)L.tabsz;
     return k;
 }
 
 // kernel arguments of a launch on level lev (with its finer and coarser
-// neighbours where they exist)
-static KArgs kargs(const vv_run *r, int lev, int color, int mode)
+// neighbours where they exist); cur[l] is level l's current buffer
+static KArgs kargs(const vv_run *r, int lev, int mode, const int *cur)
 {
     KArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -741,24 +745,24 @@ static KArgs kargs(const vv_run *r, int lev, int color, int mode)
         P.sten = Q.sten;
         P.rphi = Q.rphi;
         P.n = Q.n;
-        P.L = klev(Q.lev[lev]);
-        if (lev > 0) P.F = klev(Q.lev[lev - 1]);
-        if (lev + 1 < Q.nlev) P.Cc = klev(Q.lev[lev + 1]);
+        P.L = klev(Q.lev[lev], cur[lev]);
+        if (lev > 0) P.F = klev(Q.lev[lev - 1], cur[lev - 1]);
+        if (lev + 1 < Q.nlev) P.Cc = klev(Q.lev[lev + 1], cur[lev + 1]);
     }
     a.done = r->mg.done;
     a.cycles = r->mg.cycles;
-    a.color = color;
     a.mode = mode;
     a.lev0 = (lev == 0);
     return a;
 }
 
 // One V(nu1,nu2) cycle of both problems.  Levels 0..L-1 are smoothed by
-// zebra line sweeps, level L is solved exactly.  The restriction into a
-// smoothed level is fused into its first sweep (ZB_RESTRICT), the
+// zebra sweeps (both colours per launch, buffer u -> uo, so a level's
+// current buffer alternates), level L is solved exactly.  The restriction
+// into a smoothed level is fused into its first sweep (ZB_RESTRICT), the
 // prolongation of a correction into the first post-sweep of the finer level
-// (ZB_PROLONG): a line solve overwrites its own line, so only the
-// neighbour lines read by colour 0 need the correction added.
+// (ZB_PROLONG).  nu1 + nu2 is even, so every level is back in buffer 0 at
+// the end of the cycle and all cycles enqueue the same launches.
 static int enqueue_cycle(vv_run *r)
 {
     vv::MgBatchDev &B = r->mg;
@@ -767,26 +771,30 @@ static int enqueue_cycle(vv_run *r)
     const double ntot = (double)(B.prob[0].n + B.prob[1].n);
     const int zt = zebra_threads(n);
     const bool two = n > 1024;
+    int cur[MG_MAXLEV] = {0};
     hipEvent_t pe;
     auto ncol = [&](int l) { return B.prob[0].lev[l].ncol; };
-    auto zebra = [&](int l, int c, int mode) {
-        double bytes = zebra_bytes(B, l);
-        if (mode == ZB_RESTRICT) bytes += 8.0 * ntot * (ncol(l - 1) + 0.5 * ncol(l));
-        if (mode == ZB_PROLONG) bytes += 8.0 * ntot * ncol(l + 1);
-        const KArgs a = kargs(r, l, c, mode);
+    auto sweep = [&](int l, int mode) {
+        // algorithmic words per point N of the level: plain 2.5 (read f and
+        // the old black lines, write u); restriction 6 (read the finer u and
+        // f, write f and u); prolongation 3 (plain + the coarse correction
+        // at the black lines)
+        const double w = (mode == ZB_RESTRICT) ? 6.0 : (mode == ZB_PROLONG) ? 3.0 : 2.5;
+        const double bytes = 8.0 * w * ntot * ncol(l);
+        const KArgs a = kargs(r, l, mode, cur);
         prof_begin(r, &pe);
         if (two)
             hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
         else
             hipLaunchKernelGGL(k_mg_zebra<1>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_ZEBRA, pe, bytes);
+        cur[l] ^= 1;
     };
     for (int l = 0; l < L; ++l)
-        for (int s = 0; s < r->cfg.nu1; ++s)
-            for (int c = 0; c < 2; ++c) zebra(l, c, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN);
+        for (int s = 0; s < r->cfg.nu1; ++s) sweep(l, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN);
     {   // exact solve of the coarsest level
         const int C = ncol(L);
-        const KArgs a = kargs(r, L, 0, ZB_PLAIN);
+        const KArgs a = kargs(r, L, ZB_PLAIN, cur);
         if (L > 0) {
             const long long cnt = (long long)n * C;
             prof_begin(r, &pe);
@@ -809,8 +817,7 @@ static int enqueue_cycle(vv_run *r)
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)
-        for (int s = 0; s < r->cfg.nu2; ++s)
-            for (int c = 0; c < 2; ++c) zebra(l, c, (s == 0 && c == 0) ? ZB_PROLONG : ZB_PLAIN);
+        for (int s = 0; s < r->cfg.nu2; ++s) sweep(l, s == 0 ? ZB_PROLONG : ZB_PLAIN);
     const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
     int nb = (int)((cnt0 + 255) / 256);
@@ -821,7 +828,7 @@ static int enqueue_cycle(vv_run *r)
     sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, 0, ZB_PLAIN), B.npart);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), B.npart);
     hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, sa);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());

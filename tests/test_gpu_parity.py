@@ -302,3 +302,12 @@ def test_cycle_cap_reports_not_converged():
     st, cyc, rel = box.poisson_host(3, psi, psip, rhs)
     assert st == 1 and cyc == 2 and rel > 1e-15
     box.close()
+
+
+def test_odd_sweep_count_is_rejected():
+    """The two-colour sweeps alternate between two buffers per level, so
+    nu1 + nu2 must be even."""
+    from codesthesis_dir_amd import _lib as L
+    with pytest.raises(L.VVError) as e:
+        _box(16, 32, 1.0, nu1=1, nu2=2)
+    assert e.value.code == L.VV_EINVAL
