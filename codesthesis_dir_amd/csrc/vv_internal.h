@@ -59,6 +59,7 @@ int make_pcr(int n, const std::vector<double> &sub2, const std::vector<double> &
 struct MgLevelDev {
     double *u, *f;
     double *u2;           // second iterate buffer of a smoothed level (sweeps ping-pong)
+    double *dlt;          // level nlev-2: black-line changes of the last pre-sweep (n x ncol/2)
     const double *lu;
     double cphi;          // 1/hp^2 at this level
     int ncol, nstep;
@@ -81,7 +82,7 @@ struct MgProblemDev {
     MgLevelDev lev[MG_MAXLEV];
 };
 
-constexpr int MG_NORM_MAXB = 1024;   // workgroups of the residual-norm kernel
+constexpr int MG_NORM_MAXB = 16;     // workgroups of the residual-norm kernel
 
 // one batch of problems solved together (v and u of one run); host-side
 // description of the hierarchy and the solver state (device pointers)
@@ -90,6 +91,7 @@ struct MgBatchDev {
     MgProblemDev prob[MG_MAXPROB];
     int *done;            // [nprob] converged / capped flag
     int *cycles;          // [nprob]
+    int *ticket;          // [nprob] workgroups of the norm kernel finished (reset by the last)
     unsigned long long *fnorm;  // [nprob] max |f| bits (non-negative doubles)
     double *relres;       // [nprob]
     double *npart;        // [nprob][MG_NORM_MAXB] per-workgroup residual maxima
@@ -132,7 +134,7 @@ struct vv_run {
     vv::MgBatchDev mg{};
     double *mg_pool = nullptr;
     // solver state, one device block read back with a single copy:
-    //   bytes 0-31 int done[2], cycles[2], unused[4];
+    //   bytes 0-31 int done[2], cycles[2], ticket[2], unused[2];
     //   32-63 u64 unused[2], fnorm[2] (bits of non-negative doubles);
     //   64-79 double relres[2]
     void *mg_state = nullptr;

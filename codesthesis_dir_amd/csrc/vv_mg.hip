@@ -51,6 +51,7 @@ struct KLev {
     double *u, *uo, *f;     // current iterate, output buffer of a sweep, rhs
     const double *lu, *twid;
     double *xm;
+    double *dlt;            // black-line changes of the last pre-sweep (level L-1)
     double cphi;
     int ncol, nstep, tabsz;
 };
@@ -63,6 +64,8 @@ struct KArgs {
     KProb P[MG_MAXPROB];
     const int *done, *cycles;
     int mode, lev0;
+    int store_delta;        // k_mg_zebra: store the black-line changes in L.dlt
+    int f_from_delta;       // k_mg_dft<0>: restrict from the finer level's dlt
 };
 
 __device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
@@ -186,6 +189,11 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
 // iterate is the zero initial correction; ZB_PROLONG (first post-sweep): the
 // old black lines are read with the coarser level's correction added (a line
 // solve overwrites its own line, so only these reads need it).
+// store_delta (last pre-sweep above the coarsest level): the change of each
+// black line is kept in L.dlt.  After the sweep the black lines satisfy their
+// line equations and a red line e has the residual
+//   r_e = -c rphi (delta_{e-1} + delta_{e+1}),
+// so the restriction onto the coarsest level needs only these changes.
 template <int R>
 __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
 {
@@ -206,12 +214,12 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
     const int mode = a.mode;
     const double *u = L.u;
     const double *e = P.Cc.u;
-    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R];
+    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
         rr[0][k] = rr[1][k] = 0.0;
-        fb[k] = fr0[k] = 0.0;
+        fb[k] = fr0[k] = ubo[k] = 0.0;
         if (i >= n) continue;
         const double rp = P.rphi[i];
         if (mode == ZB_RESTRICT) {
@@ -228,11 +236,13 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (um0 + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + up1);
             fb[k] = L.f[cix(i, qb, n)];
+            ubo[k] = ub;
         } else {
             const double ub = u[cix(i, qb, n)];
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (u[cix(i, qbm, n)] + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + u[cix(i, qbp, n)]);
             fb[k] = L.f[cix(i, qb, n)];
+            ubo[k] = ub;
         }
     }
     if (done) return;                       // uniform over the workgroup
@@ -253,32 +263,16 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
         rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : 0.0;
     }
     pcr_lines<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb);
+    const bool sd = a.store_delta;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
         if (i < n) {
             L.uo[cix(i, qr0, n)] = xr[0][k];
             L.uo[cix(i, qb, n)] = xb[0][k];
+            if (sd) L.dlt[cix(i, qb >> 1, n)] = xb[0][k] - ubo[k];
         }
     }
-}
-
-// finer level F -> this level L (the coarsest): full weighting of the
-// residual along phi; the correction starts from zero.
-__global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
-{
-    const int p = blockIdx.y;
-    const KProb &P = a.P[p];
-    const KLev &L = P.L;
-    const int n = P.n, nc = L.ncol;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n * nc) return;
-    const int done = a.done[p];
-    const int qc = (int)(t / n), j = (int)(t - (long long)qc * n);
-    const double v = restricted_at(P, P.F, j, qc);
-    if (done) return;
-    L.f[cix(j, qc, n)] = v;
-    L.u[cix(j, qc, n)] = 0.0;
 }
 
 // Exact solve of the coarsest level with C = ncol columns (a power of two):
@@ -317,7 +311,9 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
     const int TM = C < DFT_TM ? C : DFT_TM;
     const int done = a.done[p];
     const bool from_res = !INV && a.lev0 && a.cycles[p] > 0;
-    const bool store = INV && a.lev0 && a.cycles[p] == 0;
+    const bool from_dlt = !INV && a.f_from_delta;
+    // below level 0 the correction is computed from zero in every cycle
+    const bool store = INV && (!a.lev0 || a.cycles[p] == 0);
     __shared__ double Ys[DFT_TK][DFT_TN];
     __shared__ double Ws[DFT_TM][DFT_TK];
     const double *W = INV ? L.twid + (size_t)C * C : L.twid;
@@ -335,6 +331,22 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
             for (int r = 0; r < YPT; ++r) {
                 const int kk = r * 4 + w;
                 yv[r] = (kk < tk && gi < n) ? resid_at(P, L, gi, k0 + kk) : 0.0;
+            }
+        } else if (from_dlt) {
+            // full weighting of the finer level's residual: coarse point k
+            // sits on black line 2k+1, whose red neighbours carry
+            // -c rphi (delta_{k-1} + delta_k) and -c rphi (delta_k + delta_{k+1})
+            const double *d = P.F.dlt;
+            const double cr = (gi < n) ? -0.25 * P.F.cphi * P.rphi[gi] : 0.0;
+#pragma unroll
+            for (int r = 0; r < YPT; ++r) {
+                const int kk = r * 4 + w, k = k0 + kk;
+                double v = 0.0;
+                if (kk < tk && gi < n) {
+                    const int km = (k == 0) ? C - 1 : k - 1, kp = (k == C - 1) ? 0 : k + 1;
+                    v = cr * (d[cix(gi, km, n)] + 2.0 * d[cix(gi, k, n)] + d[cix(gi, kp, n)]);
+                }
+                yv[r] = v;
             }
         } else {
 #pragma unroll
@@ -423,73 +435,64 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
 }
 
-// max-norm of the level-0 residual: grid-stride, one partial per workgroup
-__global__ void __launch_bounds__(256) k_mg_norm(KArgs a, double *npart)
+struct StopArgs {
+    int *done, *cycles, *ticket;
+    unsigned long long *fnorm;
+    double *relres;
+    double *npart;
+    const double *fpart;
+    int nbr, maxc;
+    double rtol;
+};
+
+// Max-norm of the level-0 residual and the stop test of one cycle.  Every
+// workgroup stores its partial maximum and takes a ticket; the last
+// workgroup of a problem combines the partials (and, at the first cycle of a
+// solve, the |f| partials of the right-hand-side kernel) and decides:
+// converged when the relative residual is <= rtol (a NaN residual also
+// stops), capped at maxc cycles.
+__global__ void __launch_bounds__(1024) k_mg_norm(KArgs a, StopArgs s)
 {
     const int p = blockIdx.y;
     const KProb &P = a.P[p];
     const KLev &L = P.L;
-    if (a.done[p]) return;
+    if (a.done[p]) return;      // set only by this problem's last workgroup, after every ticket
     const long long cnt = (long long)P.n * L.ncol;
     const long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
+#pragma unroll 8
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
         const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
         m = nanmax(m, fabs(resid_at(P, L, j, q)));
     }
-    __shared__ double sh[4];
+    __shared__ double sh[16];
+    __shared__ int last;
     m = block_max(m, sh);
-    if (threadIdx.x == 0) npart[p * MG_NORM_MAXB + blockIdx.x] = m;
-}
-
-struct StopArgs {
-    int *done, *cycles;
-    unsigned long long *fnorm;
-    double *relres;
-    const double *npart, *fpart;
-    int nbn, nbr, maxc;
-    double rtol;
-};
-
-// Stop test of one cycle (one workgroup): combines the residual partials of
-// k_mg_norm and, at the first cycle of a solve, the |f| partials of the
-// right-hand-side kernel; converged when the relative residual is <= rtol
-// (a NaN residual also stops), capped at maxc cycles.  All loads are issued
-// before the reductions.
-__global__ void __launch_bounds__(256) k_mg_stop(StopArgs a)
-{
-    __shared__ double sh[4];
-    int dn[MG_MAXPROB], cy[MG_MAXPROB];
-    double fo[MG_MAXPROB], mr[MG_MAXPROB], mf[MG_MAXPROB];
-#pragma unroll
-    for (int p = 0; p < MG_MAXPROB; ++p) {
-        dn[p] = a.done[p];
-        cy[p] = a.cycles[p];
-        fo[p] = __longlong_as_double((long long)a.fnorm[p]);
-        mr[p] = 0.0;
-        mf[p] = 0.0;
-        for (int b = threadIdx.x; b < a.nbn; b += blockDim.x) mr[p] = nanmax(mr[p], a.npart[p * MG_NORM_MAXB + b]);
-        for (int b = threadIdx.x; b < a.nbr; b += blockDim.x) mf[p] = nanmax(mf[p], a.fpart[p * a.nbr + b]);
+    if (threadIdx.x == 0) {
+        s.npart[p * MG_NORM_MAXB + blockIdx.x] = m;
+        __threadfence();
+        last = (atomicAdd(&s.ticket[p], 1) == (int)gridDim.x - 1);
     }
-#pragma unroll
-    for (int p = 0; p < MG_MAXPROB; ++p) {
-        mr[p] = block_max(mr[p], sh);
-        mf[p] = block_max(mf[p], sh);
-    }
-    if (threadIdx.x != 0) return;
-#pragma unroll
-    for (int p = 0; p < MG_MAXPROB; ++p) {
-        if (dn[p]) continue;
-        double fn = fo[p];
-        if (cy[p] == 0) {
-            fn = mf[p];
-            a.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
-        }
-        const double rel = mr[p] / (fn > 0.0 ? fn : 1.0);
-        const int c = cy[p] + 1;
-        a.cycles[p] = c;
-        a.relres[p] = rel;
-        if (!(rel > a.rtol) || c >= a.maxc) a.done[p] = 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int c0 = a.cycles[p];
+    double mr = 0.0, mf = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) mr = nanmax(mr, s.npart[p * MG_NORM_MAXB + b]);
+    if (c0 == 0)
+        for (int b = threadIdx.x; b < s.nbr; b += blockDim.x) mf = nanmax(mf, s.fpart[p * s.nbr + b]);
+    mr = block_max(mr, sh);
+    mf = block_max(mf, sh);
+    if (threadIdx.x == 0) {
+        double fn = mf;
+        if (c0 == 0) s.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
+        else fn = __longlong_as_double((long long)s.fnorm[p]);
+        const double rel = mr / (fn > 0.0 ? fn : 1.0);
+        const int c = c0 + 1;
+        s.cycles[p] = c;
+        s.relres[p] = rel;
+        s.ticket[p] = 0;
+        if (!(rel > s.rtol) || c >= s.maxc) s.done[p] = 1;
     }
 }
 
@@ -634,6 +637,7 @@ int mg_build(vv_run *r)
                 L.S = make_pcr(n, s2, s1, d2, p1, p2, L.tab);
                 L.tabsz = L.tab.size();
                 total += 3 * (size_t)n * c + L.tab.size();
+                if (l == nlev - 2) total += (size_t)n * (c / 2);   // black-line changes
             } else {
                 // one table per Fourier mode m = 0..c/2: the phi second
                 // difference on c periodic points has eigenvalue
@@ -694,6 +698,8 @@ int mg_build(vv_run *r)
             L.f = r->mg_pool + off; off += (size_t)n * H.ncol;
             L.u2 = nullptr;
             if (!H.direct) { L.u2 = r->mg_pool + off; off += (size_t)n * H.ncol; }
+            L.dlt = nullptr;
+            if (l == nlev - 2) { L.dlt = r->mg_pool + off; off += (size_t)n * (H.ncol / 2); }
             std::memcpy(host.data() + off, H.tab.data(), H.tab.size() * sizeof(double));
             L.lu = r->mg_pool + off;
             L.twid = H.direct ? L.lu + (size_t)(H.ncol / 2 + 1) * H.tabsz : nullptr;
@@ -706,6 +712,7 @@ int mg_build(vv_run *r)
                           r->stream));
     r->mg.done = r->mg_flags;
     r->mg.cycles = r->mg_flags + 2;
+    r->mg.ticket = r->mg_flags + 4;
     r->mg.fnorm = r->mg_bits + 2;
     r->mg.relres = r->mg_relres;
     r->mg.npart = r->npart;
@@ -728,7 +735,7 @@ static KLev klev(const MgLevelDev &L, int cur)
     KLev k;
     k.u = cur ? L.u2 : L.u;
     k.uo = cur ? L.u : L.u2;
-    k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm;
+    k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm; k.dlt = L.dlt;
     k.cphi = L.cphi; k.ncol = L.ncol; k.nstep = L.nstep; k.tabsz = (int)L.tabsz;
     return k;
 }
@@ -774,14 +781,15 @@ static int enqueue_cycle(vv_run *r)
     int cur[MG_MAXLEV] = {0};
     hipEvent_t pe;
     auto ncol = [&](int l) { return B.prob[0].lev[l].ncol; };
-    auto sweep = [&](int l, int mode) {
+    auto sweep = [&](int l, int mode, int store_delta) {
         // algorithmic words per point N of the level: plain 2.5 (read f and
         // the old black lines, write u); restriction 6 (read the finer u and
         // f, write f and u); prolongation 3 (plain + the coarse correction
         // at the black lines)
         const double w = (mode == ZB_RESTRICT) ? 6.0 : (mode == ZB_PROLONG) ? 3.0 : 2.5;
         const double bytes = 8.0 * w * ntot * ncol(l);
-        const KArgs a = kargs(r, l, mode, cur);
+        KArgs a = kargs(r, l, mode, cur);
+        a.store_delta = store_delta;
         prof_begin(r, &pe);
         if (two)
             hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
@@ -791,16 +799,13 @@ static int enqueue_cycle(vv_run *r)
         cur[l] ^= 1;
     };
     for (int l = 0; l < L; ++l)
-        for (int s = 0; s < r->cfg.nu1; ++s) sweep(l, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN);
-    {   // exact solve of the coarsest level
+        for (int s = 0; s < r->cfg.nu1; ++s)
+            sweep(l, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN, (l == L - 1 && s == r->cfg.nu1 - 1) ? 1 : 0);
+    {   // exact solve of the coarsest level; its right-hand side is the
+        // restriction of the last pre-sweep's black-line changes
         const int C = ncol(L);
-        const KArgs a = kargs(r, L, ZB_PLAIN, cur);
-        if (L > 0) {
-            const long long cnt = (long long)n * C;
-            prof_begin(r, &pe);
-            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(cnt, 256), 2), dim3(256), 0, r->stream, a);
-            prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * (2.0 * ncol(L - 1) + 2.0 * C));
-        }
+        KArgs a = kargs(r, L, ZB_PLAIN, cur);
+        a.f_from_delta = (L > 0) ? 1 : 0;
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
         prof_begin(r, &pe);
@@ -817,19 +822,18 @@ static int enqueue_cycle(vv_run *r)
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)
-        for (int s = 0; s < r->cfg.nu2; ++s) sweep(l, s == 0 ? ZB_PROLONG : ZB_PLAIN);
+        for (int s = 0; s < r->cfg.nu2; ++s) sweep(l, s == 0 ? ZB_PROLONG : ZB_PLAIN, 0);
     const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
-    int nb = (int)((cnt0 + 255) / 256);
-    if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
+    int nb = (int)((cnt0 + 8 * 1024 - 1) / (8 * 1024));   // up to 8 points per thread per pass
+    if (nb > 16) nb = 16;
     if (nb < 1) nb = 1;
     StopArgs sa;
-    sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres;
-    sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
+    sa.done = B.done; sa.cycles = B.cycles; sa.ticket = B.ticket; sa.fnorm = B.fnorm; sa.relres = B.relres;
+    sa.npart = B.npart; sa.fpart = B.fpart; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), B.npart);
-    hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, sa);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(1024), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), sa);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
