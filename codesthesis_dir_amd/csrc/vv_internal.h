@@ -82,7 +82,7 @@ struct MgProblemDev {
     MgLevelDev lev[MG_MAXLEV];
 };
 
-constexpr int MG_NORM_MAXB = 16;     // workgroups of the residual-norm kernel
+constexpr int MG_NORM_MAXB = 1024;   // workgroups of the residual-norm kernel
 
 // one batch of problems solved together (v and u of one run); host-side
 // description of the hierarchy and the solver state (device pointers)

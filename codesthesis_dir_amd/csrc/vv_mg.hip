@@ -65,7 +65,6 @@ struct KArgs {
     const int *done, *cycles;
     int mode, lev0;
     int store_delta;        // k_mg_zebra: store the black-line changes in L.dlt
-    int f_from_delta;       // k_mg_dft<0>: restrict from the finer level's dlt
 };
 
 __device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
@@ -275,6 +274,28 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
     }
 }
 
+// Restriction onto the coarsest level L from the black-line changes of the
+// last pre-sweep of level L-1 (see k_mg_zebra): coarse point k sits on black
+// line 2k+1, whose red neighbours carry -c rphi (delta_{k-1} + delta_k) and
+// -c rphi (delta_k + delta_{k+1}), and the black lines have no residual, so
+// full weighting gives f_L = -c/4 rphi (delta_{k-1} + 2 delta_k + delta_{k+1}).
+__global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
+{
+    const int p = blockIdx.y;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, C = L.ncol;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * C) return;
+    const int done = a.done[p];
+    const int k = (int)(t / n), j = (int)(t - (long long)k * n);
+    const int km = (k == 0) ? C - 1 : k - 1, kp = (k == C - 1) ? 0 : k + 1;
+    const double *d = P.F.dlt;
+    const double v = -0.25 * P.F.cphi * P.rphi[j] * (d[cix(j, km, n)] + 2.0 * d[cix(j, k, n)] + d[cix(j, kp, n)]);
+    if (done) return;
+    L.f[cix(j, k, n)] = v;
+}
+
 // Exact solve of the coarsest level with C = ncol columns (a power of two):
 // the operator is invariant along phi, so the real DFT over the C columns
 // decouples it into C line systems D_r + lambda_m diag(rphi), one per real
@@ -311,7 +332,6 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
     const int TM = C < DFT_TM ? C : DFT_TM;
     const int done = a.done[p];
     const bool from_res = !INV && a.lev0 && a.cycles[p] > 0;
-    const bool from_dlt = !INV && a.f_from_delta;
     // below level 0 the correction is computed from zero in every cycle
     const bool store = INV && (!a.lev0 || a.cycles[p] == 0);
     __shared__ double Ys[DFT_TK][DFT_TN];
@@ -331,22 +351,6 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
             for (int r = 0; r < YPT; ++r) {
                 const int kk = r * 4 + w;
                 yv[r] = (kk < tk && gi < n) ? resid_at(P, L, gi, k0 + kk) : 0.0;
-            }
-        } else if (from_dlt) {
-            // full weighting of the finer level's residual: coarse point k
-            // sits on black line 2k+1, whose red neighbours carry
-            // -c rphi (delta_{k-1} + delta_k) and -c rphi (delta_k + delta_{k+1})
-            const double *d = P.F.dlt;
-            const double cr = (gi < n) ? -0.25 * P.F.cphi * P.rphi[gi] : 0.0;
-#pragma unroll
-            for (int r = 0; r < YPT; ++r) {
-                const int kk = r * 4 + w, k = k0 + kk;
-                double v = 0.0;
-                if (kk < tk && gi < n) {
-                    const int km = (k == 0) ? C - 1 : k - 1, kp = (k == C - 1) ? 0 : k + 1;
-                    v = cr * (d[cix(gi, km, n)] + 2.0 * d[cix(gi, k, n)] + d[cix(gi, kp, n)]);
-                }
-                yv[r] = v;
             }
         } else {
 #pragma unroll
@@ -451,7 +455,7 @@ struct StopArgs {
 // solve, the |f| partials of the right-hand-side kernel) and decides:
 // converged when the relative residual is <= rtol (a NaN residual also
 // stops), capped at maxc cycles.
-__global__ void __launch_bounds__(1024) k_mg_norm(KArgs a, StopArgs s)
+__global__ void __launch_bounds__(256) k_mg_norm(KArgs a, StopArgs s)
 {
     const int p = blockIdx.y;
     const KProb &P = a.P[p];
@@ -460,12 +464,11 @@ __global__ void __launch_bounds__(1024) k_mg_norm(KArgs a, StopArgs s)
     const long long cnt = (long long)P.n * L.ncol;
     const long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
-#pragma unroll 8
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += stride) {
         const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
         m = nanmax(m, fabs(resid_at(P, L, j, q)));
     }
-    __shared__ double sh[16];
+    __shared__ double sh[4];
     __shared__ int last;
     m = block_max(m, sh);
     if (threadIdx.x == 0) {
@@ -804,8 +807,12 @@ static int enqueue_cycle(vv_run *r)
     {   // exact solve of the coarsest level; its right-hand side is the
         // restriction of the last pre-sweep's black-line changes
         const int C = ncol(L);
-        KArgs a = kargs(r, L, ZB_PLAIN, cur);
-        a.f_from_delta = (L > 0) ? 1 : 0;
+        const KArgs a = kargs(r, L, ZB_PLAIN, cur);
+        if (L > 0) {
+            prof_begin(r, &pe);
+            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk((long long)n * C, 256), 2), dim3(256), 0, r->stream, a);
+            prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * 2.0 * C);
+        }
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
         prof_begin(r, &pe);
@@ -825,15 +832,15 @@ static int enqueue_cycle(vv_run *r)
         for (int s = 0; s < r->cfg.nu2; ++s) sweep(l, s == 0 ? ZB_PROLONG : ZB_PLAIN, 0);
     const int nc0 = ncol(0);
     const long long cnt0 = (long long)n * nc0;
-    int nb = (int)((cnt0 + 8 * 1024 - 1) / (8 * 1024));   // up to 8 points per thread per pass
-    if (nb > 16) nb = 16;
+    int nb = (int)((cnt0 + 255) / 256);    // one point per thread
+    if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
     StopArgs sa;
     sa.done = B.done; sa.cycles = B.cycles; sa.ticket = B.ticket; sa.fnorm = B.fnorm; sa.relres = B.relres;
     sa.npart = B.npart; sa.fpart = B.fpart; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(1024), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), sa);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), sa);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
