@@ -75,6 +75,57 @@ def gpu_leg(name, steps, warmup, nu):
     return cfg, info, wall_ms, ev_ms, prof, nu
 
 
+MG_KERNELS = ("zebra", "restrict", "prolong", "norm", "direct")
+
+
+def direct_leg(name, steps, warmup):
+    """The same workload with the Poisson solves done entirely by the direct
+    (spectral) level: direct_cols = np."""
+    from codesthesis_dir_amd import BoxConfig, PeriodicBox
+    from codesthesis_dir_amd.ic import box_temperature
+    desc, kw, seed, amp = WORKLOADS[name]
+    cfg = BoxConfig(rtol=1e-10, max_cycles=60, direct_cols=kw["np"], **kw)
+    box = PeriodicBox(cfg, device=0)
+    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, amp, seed))
+    box.start()
+    box.step(warmup)
+    t0 = time.perf_counter()
+    info = box.step(steps)
+    el = time.perf_counter() - t0
+    box.close()
+    return {"value": steps / el, "unit": "time-steps/s", "direct_cols": cfg.np,
+            "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
+
+
+def vcycle_large_leg(steps=6, nu=1):
+    """V-cycle bandwidth on a large plane: periodic box, 2049 x 4097 nodes
+    (nr=2048, np=4096), V(nu,nu) with five smoothed levels and the direct
+    level at 128 columns; per-kernel HIP events over `steps` time steps."""
+    from codesthesis_dir_amd import BoxConfig, PeriodicBox
+    from codesthesis_dir_amd.ic import box_temperature
+    cfg = BoxConfig(nr=2048, np=4096, width=2.0, dt=1e-7, Ra=1e5, Pr=1.0, rtol=1e-10, max_cycles=60,
+                    nu1=nu, nu2=nu)
+    box = PeriodicBox(cfg, device=0)
+    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, 1e-3, 20261206))
+    box.start()
+    box.step(2)
+    box.profile(True)
+    info = box.step(steps)
+    prof = box.profile_read()
+    box.profile(False)
+    box.close()
+    vb = sum(prof[k]["bytes"] for k in MG_KERNELS)
+    vt = sum(prof[k]["ms"] for k in MG_KERNELS)
+    z = prof["zebra"]
+    return {"workload": "box-2049x4097 (nr=2048, np=4096), V(%d,%d), direct level 128 columns" % (nu, nu),
+            "vcycle_gbs": vb / (vt * 1e-3) / 1e9 if vt > 0 else None,
+            "vcycle_frac": (vb / (vt * 1e-3) / 1e9) / HBM_PEAK_GBS if vt > 0 else None,
+            "zebra_gbs": z["bytes"] / (z["ms"] * 1e-3) / 1e9 if z["ms"] > 0 else None,
+            "zebra_avg_launch_us": 1e3 * z["ms"] / max(z["launches"], 1),
+            "vcycle_ms_per_step": vt / steps,
+            "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
+
+
 def cpu_leg(name, budget_s=15.0):
     """Time the reference CPU path on this host for a bounded sample."""
     desc, kw, seed, amp = WORKLOADS[name]
@@ -154,6 +205,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the direct-solve and large V-cycle measurements")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -170,7 +223,7 @@ def main():
     d = prof[dom]
     avg_ms = d["ms"] / max(d["launches"], 1)
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    vcyc = {k: prof[k] for k in ("zebra", "restrict", "prolong", "norm", "direct")}
+    vcyc = {k: prof[k] for k in MG_KERNELS}
     vc_ms = sum(v["ms"] for v in vcyc.values())
     vc_bytes = sum(v["bytes"] for v in vcyc.values())
     traffic = read_traffic(dom, a.workload)
@@ -200,6 +253,9 @@ def main():
         "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
         "nusselt_last_step": nu,
     }
+    if not a.no_extra:
+        line["alternatives"] = {"poisson_direct": direct_leg(a.workload, a.steps, a.warmup)}
+        line["vcycle_large"] = vcycle_large_leg()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.workload)
     print(json.dumps(line))

@@ -91,7 +91,6 @@ struct MgBatchDev {
     MgProblemDev prob[MG_MAXPROB];
     int *done;            // [nprob] converged / capped flag
     int *cycles;          // [nprob]
-    int *ticket;          // [nprob] workgroups of the norm kernel finished (reset by the last)
     unsigned long long *fnorm;  // [nprob] max |f| bits (non-negative doubles)
     double *relres;       // [nprob]
     double *npart;        // [nprob][MG_NORM_MAXB] per-workgroup residual maxima
@@ -134,7 +133,7 @@ struct vv_run {
     vv::MgBatchDev mg{};
     double *mg_pool = nullptr;
     // solver state, one device block read back with a single copy:
-    //   bytes 0-31 int done[2], cycles[2], ticket[2], unused[2];
+    //   bytes 0-31 int done[2], cycles[2], unused[4];
     //   32-63 u64 unused[2], fnorm[2] (bits of non-negative doubles);
     //   64-79 double relres[2]
     void *mg_state = nullptr;

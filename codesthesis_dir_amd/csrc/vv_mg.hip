@@ -439,28 +439,15 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
 }
 
-struct StopArgs {
-    int *done, *cycles, *ticket;
-    unsigned long long *fnorm;
-    double *relres;
-    double *npart;
-    const double *fpart;
-    int nbr, maxc;
-    double rtol;
-};
-
-// Max-norm of the level-0 residual and the stop test of one cycle.  Every
-// workgroup stores its partial maximum and takes a ticket; the last
-// workgroup of a problem combines the partials (and, at the first cycle of a
-// solve, the |f| partials of the right-hand-side kernel) and decides:
-// converged when the relative residual is <= rtol (a NaN residual also
-// stops), capped at maxc cycles.
-__global__ void __launch_bounds__(256) k_mg_norm(KArgs a, StopArgs s)
+// max-norm of the level-0 residual: one point per thread, one partial per
+// workgroup (no atomics: same-address atomics from hundreds of workgroups
+// cost more than a second launch)
+__global__ void __launch_bounds__(256) k_mg_norm(KArgs a, double *npart)
 {
     const int p = blockIdx.y;
     const KProb &P = a.P[p];
     const KLev &L = P.L;
-    if (a.done[p]) return;      // set only by this problem's last workgroup, after every ticket
+    if (a.done[p]) return;
     const long long cnt = (long long)P.n * L.ncol;
     const long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
@@ -469,33 +456,58 @@ __global__ void __launch_bounds__(256) k_mg_norm(KArgs a, StopArgs s)
         m = nanmax(m, fabs(resid_at(P, L, j, q)));
     }
     __shared__ double sh[4];
-    __shared__ int last;
     m = block_max(m, sh);
-    if (threadIdx.x == 0) {
-        s.npart[p * MG_NORM_MAXB + blockIdx.x] = m;
-        __threadfence();
-        last = (atomicAdd(&s.ticket[p], 1) == (int)gridDim.x - 1);
+    if (threadIdx.x == 0) npart[p * MG_NORM_MAXB + blockIdx.x] = m;
+}
+
+struct StopArgs {
+    int *done, *cycles;
+    unsigned long long *fnorm;
+    double *relres;
+    const double *npart, *fpart;
+    int nbn, nbr, maxc;
+    double rtol;
+};
+
+// Stop test of one cycle (one workgroup): combines the residual partials of
+// k_mg_norm and, at the first cycle of a solve, the |f| partials of the
+// right-hand-side kernel; converged when the relative residual is <= rtol
+// (a NaN residual also stops), capped at maxc cycles.  All loads are issued
+// before the reductions.
+__global__ void __launch_bounds__(256) k_mg_stop(StopArgs a)
+{
+    __shared__ double sh[4];
+    int dn[MG_MAXPROB], cy[MG_MAXPROB];
+    double fo[MG_MAXPROB], mr[MG_MAXPROB], mf[MG_MAXPROB];
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        dn[p] = a.done[p];
+        cy[p] = a.cycles[p];
+        fo[p] = __longlong_as_double((long long)a.fnorm[p]);
+        mr[p] = 0.0;
+        mf[p] = 0.0;
+        for (int b = threadIdx.x; b < a.nbn; b += blockDim.x) mr[p] = nanmax(mr[p], a.npart[p * MG_NORM_MAXB + b]);
+        for (int b = threadIdx.x; b < a.nbr; b += blockDim.x) mf[p] = nanmax(mf[p], a.fpart[p * a.nbr + b]);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const int c0 = a.cycles[p];
-    double mr = 0.0, mf = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) mr = nanmax(mr, s.npart[p * MG_NORM_MAXB + b]);
-    if (c0 == 0)
-        for (int b = threadIdx.x; b < s.nbr; b += blockDim.x) mf = nanmax(mf, s.fpart[p * s.nbr + b]);
-    mr = block_max(mr, sh);
-    mf = block_max(mf, sh);
-    if (threadIdx.x == 0) {
-        double fn = mf;
-        if (c0 == 0) s.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
-        else fn = __longlong_as_double((long long)s.fnorm[p]);
-        const double rel = mr / (fn > 0.0 ? fn : 1.0);
-        const int c = c0 + 1;
-        s.cycles[p] = c;
-        s.relres[p] = rel;
-        s.ticket[p] = 0;
-        if (!(rel > s.rtol) || c >= s.maxc) s.done[p] = 1;
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        mr[p] = block_max(mr[p], sh);
+        mf[p] = block_max(mf[p], sh);
+    }
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        if (dn[p]) continue;
+        double fn = fo[p];
+        if (cy[p] == 0) {
+            fn = mf[p];
+            a.fnorm[p] = (unsigned long long)__double_as_longlong(fn);
+        }
+        const double rel = mr[p] / (fn > 0.0 ? fn : 1.0);
+        const int c = cy[p] + 1;
+        a.cycles[p] = c;
+        a.relres[p] = rel;
+        if (!(rel > a.rtol) || c >= a.maxc) a.done[p] = 1;
     }
 }
 
@@ -715,7 +727,6 @@ int mg_build(vv_run *r)
                           r->stream));
     r->mg.done = r->mg_flags;
     r->mg.cycles = r->mg_flags + 2;
-    r->mg.ticket = r->mg_flags + 4;
     r->mg.fnorm = r->mg_bits + 2;
     r->mg.relres = r->mg_relres;
     r->mg.npart = r->npart;
@@ -836,11 +847,12 @@ static int enqueue_cycle(vv_run *r)
     if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
     StopArgs sa;
-    sa.done = B.done; sa.cycles = B.cycles; sa.ticket = B.ticket; sa.fnorm = B.fnorm; sa.relres = B.relres;
-    sa.npart = B.npart; sa.fpart = B.fpart; sa.nbr = B.nbr;
+    sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres;
+    sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     prof_begin(r, &pe);
-    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), sa);
+    hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), B.npart);
+    hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, sa);
     prof_end(r, VV_K_NORM, pe, 16.0 * ntot * nc0);
     VV_HIP(hipGetLastError());
     return VV_OK;
