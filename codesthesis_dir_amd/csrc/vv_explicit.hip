@@ -123,11 +123,14 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     const int k0 = blockIdx.x * T;
     if (k0 > np) return;
     const int tc = (np + 1 - k0 < T) ? np + 1 - k0 : T;
-    // the elimination factors go to LDS with the right-hand sides, so the
-    // recurrences below read nothing from global memory
+    // the elimination factors go to LDS with the right-hand sides (when they
+    // fit), so the recurrences below read nothing from global memory
+    if (Tab) {
 #pragma unroll 8
-    for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) Tab[idx] = tab[idx];
-    const double *fact = Tab, *bp = Tab + n, *cc = Tab + 2 * n, *rbp = Tab + 3 * n;
+        for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) Tab[idx] = tab[idx];
+    }
+    const double *tb = Tab ? Tab : tab;
+    const double *fact = tb, *bp = tb + n, *cc = tb + 2 * n, *rbp = tb + 3 * n;
 #pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int c = idx / n, j = idx - c * n;
@@ -193,9 +196,12 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     if (i0 > rows) return;
     const int tc = (rows + 1 - i0 < T) ? rows + 1 - i0 : T;
     double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
-    // elimination factors to LDS (see tdma_tile)
+    // elimination factors to LDS when they fit (see tdma_tile)
+    if (Tab) {
 #pragma unroll 8
-    for (int idx = threadIdx.x; idx < 6 * s + 3; idx += blockDim.x) Tab[idx] = a.tab_p[idx];
+        for (int idx = threadIdx.x; idx < 6 * s + 3; idx += blockDim.x) Tab[idx] = a.tab_p[idx];
+    }
+    const double *tb = Tab ? Tab : a.tab_p;
 #pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int j = idx / tc, c = idx - j * tc;
@@ -203,10 +209,10 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     }
     __syncthreads();
     if ((int)threadIdx.x < tc) {
-        const double *f1 = Tab, *f2 = Tab + s, *aa = Tab + 2 * s;
-        const double *bb = Tab + 3 * s, *cc = Tab + 4 * s;
-        const double fA = Tab[5 * s], fB = Tab[5 * s + 1], fC = Tab[5 * s + 2];
-        const double *rbb = Tab + 5 * s + 3;
+        const double *f1 = tb, *f2 = tb + s, *aa = tb + 2 * s;
+        const double *bb = tb + 3 * s, *cc = tb + 4 * s;
+        const double fA = tb[5 * s], fB = tb[5 * s + 1], fC = tb[5 * s + 2];
+        const double *rbb = tb + 5 * s + 3;
         const int c = threadIdx.x;
 #define R_(j) R[((j) - 1) * T + c]
         constexpr int U = 8;   // blocked as in tdma_tile
@@ -268,12 +274,13 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 struct CompactTiles {
     int Tr, Tp;    // columns per workgroup (r families), rows per workgroup (phi families)
     int rwords;    // LDS words of the right-hand-side tile; the tables follow
+    int tab_lds;   // 1: the elimination tables are staged in LDS
 };
 
 __global__ void __launch_bounds__(WG) k_compact(CompactArgs a, CompactTiles t)
 {
     extern __shared__ double R[];
-    double *Tab = R + t.rwords;
+    double *Tab = t.tab_lds ? R + t.rwords : nullptr;
     const int fam = blockIdx.y;
     if (!(a.mask & (1 << fam))) return;
     switch (fam) {
@@ -675,11 +682,9 @@ static int compact_launch(vv_run *r, int mask)
     t.rwords = budget;
     // + the elimination tables: 4n words (r families, n <= nr), 6(np+2)+3 (phi)
     const int twords = (4 * r->nr > 6 * (r->np + 2) + 3) ? 4 * r->nr : 6 * (r->np + 2) + 3;
-    const size_t lds = (size_t)(budget + twords) * 8;
-    if (lds > 65536) {   // gfx950 has 160 KB of LDS per CU; above 64 KB it is requested explicitly
-        VV_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_compact),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
+    // dynamic LDS is kept within 64 KB; larger tables stay in global memory
+    t.tab_lds = ((size_t)(budget + twords) * 8 <= 65536) ? 1 : 0;
+    const size_t lds = (size_t)(budget + (t.tab_lds ? twords : 0)) * 8;
     const int gr = nblk(r->np + 1, t.Tr), gp = nblk(r->nr, t.Tp);
     dim3 grid(gr > gp ? gr : gp, 6);
     hipEvent_t pe_;

@@ -56,7 +56,7 @@ def _random_state(run, seed):
 
 
 @pytest.mark.parametrize("geom", [0, 1])
-@pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False),
+@pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False), (1100, 32, False),
                                            (40, 64, False)])
 def test_explicit_step_bitexact(nr, np_, first, geom):
     import ctypes
@@ -104,7 +104,7 @@ def test_explicit_step_bitexact(nr, np_, first, geom):
 
 @pytest.mark.parametrize("nr,np_,width", [(16, 32, 3.0), (64, 128, 1.0), (64, 128, 3.0),
                                           (128, 256, 1.0), (48, 64, 2.0), (64, 128, 0.0),
-                                          (256, 256, 0.0)])
+                                          (256, 256, 0.0), (2048, 64, 1.0), (1536, 64, 0.0)])
 @pytest.mark.parametrize("which", [2, 3])
 @pytest.mark.parametrize("direct_cols", [8, 128])
 def test_poisson_matches_exact_solution(nr, np_, width, which, direct_cols):
