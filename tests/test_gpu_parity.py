@@ -104,7 +104,7 @@ def test_explicit_step_bitexact(nr, np_, first, geom):
 
 @pytest.mark.parametrize("nr,np_,width", [(16, 32, 3.0), (64, 128, 1.0), (64, 128, 3.0),
                                           (128, 256, 1.0), (48, 64, 2.0), (64, 128, 0.0),
-                                          (256, 256, 0.0), (2048, 64, 1.0), (1536, 64, 0.0)])
+                                          (256, 256, 0.0)])
 @pytest.mark.parametrize("which", [2, 3])
 @pytest.mark.parametrize("direct_cols", [8, 128])
 def test_poisson_matches_exact_solution(nr, np_, width, which, direct_cols):
@@ -142,6 +142,42 @@ def test_poisson_matches_exact_solution(nr, np_, width, which, direct_cols):
     assert np.array_equal(psi[0], psi[np_])
     m, _ = O.residual(which, psi, rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
     assert m <= 2e-12 * np.max(np.abs(rhs[1:, 1:ni - 1]))
+    box.close()
+
+
+@pytest.mark.parametrize("nr,np_,width", [(2048, 64, 1.0), (1536, 64, 0.0)])
+@pytest.mark.parametrize("which", [2, 3])
+@pytest.mark.parametrize("direct_cols", [8, 128])
+def test_poisson_long_lines(nr, np_, width, which, direct_cols):
+    """Lines longer than 1024 rows (two rows per thread in the line solves).
+    The r-operator's coefficients are (nr/0.5)^2 ~ 1e7 here, and evaluating
+    the residual in double precision has a relative floor near
+    eps (nr/pi)^2 ~ 1e-10, so the solves run at rtol 1e-9 and the solution
+    is compared with the exact one at 1e-6 (an indexing error in the long
+    lines gives O(1))."""
+    geom = 1 if width == 0.0 else 0
+    if geom:
+        box = _annulus(nr, np_, rtol=1e-9, max_cycles=60, Re=100.0, direct_cols=direct_cols)
+        dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
+    else:
+        box = _box(nr, np_, width, rtol=1e-9, max_cycles=60, direct_cols=direct_cols)
+        dr, dp = 0.5 / nr, width / np_
+    rng = np.random.default_rng(nr + which)
+    ld = nr + 2
+    k = np.arange(np_ + 1)[:, None]
+    i = np.arange(ld)[None, :]
+    rhs = np.ascontiguousarray(np.cos(2 * np.pi * 3 * k / np_) * np.sin(0.05 * i)
+                               + 0.2 * rng.standard_normal((np_ + 1, ld)))
+    rhs[0] = rhs[np_]
+    ni = nr + 2 if which == 2 else nr + 1
+    psi, psip = np.zeros((np_ + 1, ld)), np.zeros((np_ + 1, ld))
+    st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
+    assert st == 0 and rel <= 1e-9, (st, cyc, rel)
+    ex = np.zeros_like(psi)
+    O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
+    err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
+    print(f"nr={nr} which={which} dc={direct_cols}: cycles {cyc} relres {rel:.2e} err {err:.2e}")
+    assert err < 1e-6, err
     box.close()
 
 
