@@ -1,0 +1,53 @@
+/* vv3_oracle.h -- CPU oracle (test infrastructure only) for the 3D rotating-
+ * annulus time step of rot3dmg.f: a literal C restatement of its explicit
+ * routines and its three Poisson residual operators, plus an exact solve of
+ * those operators for tiny grids.  Only the tests, the smoke check and the
+ * benchmark's CPU leg use it; no product code links it.
+ *
+ * Layout: X(i,j,k) at X[(k*ldj + j)*ldi + i], i = 0..nr+1, j = 0..nz+1,
+ * k = 0..np (k = 0 and k = np are the same azimuth; the reference keeps both).
+ */
+#ifndef VV3_ORACLE_H
+#define VV3_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nr, nz, np, ldi, ldj;
+    double dt, dr, dz, dp, Ra, Re, Ptl, Gr, r0, rmax;
+} v3_params;
+
+typedef struct {
+    double *An, *A, *Ap, *Bn, *B, *Bp, *Cn, *C, *Cp;
+    double *thn, *th, *thp;
+    double *uel, *vel, *wel, *uelp, *velp, *welp;
+    double *urh, *vrh, *wrh;
+} v3_fields;
+
+void v3_avorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:341-468 */
+void v3_bvorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:477-593 */
+void v3_cvorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:601-728 */
+void v3_temperature(const v3_params *P, v3_fields *F);  /* rot3dmg.f:736-874 */
+double v3_heat(const v3_params *P, const v3_fields *F); /* rot3dmg.f:882-905 */
+void v3_update(const v3_params *P, v3_fields *F);       /* rot3dmg.f:964-1018 */
+void v3_rhsw(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1026-1048 */
+void v3_rhsv(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1066-1093 */
+void v3_rhsu(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1111-1134 */
+
+/* residual f - L psi of operator which (1: w, 2: v, 3: u) at the finest
+ * level, into res (same layout; zero outside the unknowns); returns the
+ * max-norm.  rot3dmg.f:1695-1723 (w), 1927-1965 (v), 2130-2165 (u). */
+double v3_residual(const v3_params *P, int which, const double *psi, const double *rhs, double *res);
+
+/* exact solution of L psi = rhs on the unknowns of operator `which`, by a
+ * real DFT in phi (long double sums) and a dense LU of each mode's (r, z)
+ * system; psi is zero outside the unknowns, psi(.,.,0) = psi(.,.,np).
+ * Returns 0, or -1 if a mode's system is singular.  For tiny grids only
+ * ((unknowns per plane)^2 doubles of workspace). */
+int v3_poisson_exact(const v3_params *P, int which, double *psi, const double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+"""In-process driver for the 3D reference build oracle/_ref/librot3d.so.
+
+librot3d.so is rot3dmg.f compiled unmodified by oracle/Makefile from the
+reference sources (only in the container that holds them).  Its COMMON
+blocks (/sol1/ /sol2/ /sol3/ /num/ /val/ /radius/, and the multigrid
+/arrays/ /backup/ /step/ /ints/) are exported symbols; this module maps them
+onto numpy arrays with the reference's PARAMETER extents (kr=kz=33,
+kp=160, mgen=202590; rot3dmg.f:23) so a test can set a state, call one
+reference subroutine and read the result.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
+KR1, KZ1, KP1 = 34, 34, 161
+MGEN = 202590
+SHAPE = (KP1, KZ1, KR1)
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def available() -> bool:
+    return os.path.exists(LIB)
+
+
+class Reference3:
+    _inst = None
+
+    def __new__(cls):
+        if cls._inst is None:
+            inst = super().__new__(cls)
+            inst._init()
+            cls._inst = inst
+        return cls._inst
+
+    def _init(self):
+        L = ctypes.CDLL(LIB)
+        self.L = L
+        nf = KR1 * KZ1 * KP1
+
+        def block(name, n, count=None):
+            addr = ctypes.addressof(ctypes.c_char.in_dll(L, name))
+            buf = (ctypes.c_double * (n * nf)).from_address(addr)
+            return np.frombuffer(buf, dtype=np.float64).reshape((n,) + SHAPE)
+
+        s1, s2, s3, ar = block("sol1_", 9), block("sol2_", 3), block("sol3_", 6), block("arrays_", 5)
+        self.f = {}
+        for j, n in enumerate(["An", "A", "Ap", "Bn", "B", "Bp", "Cn", "C", "Cp"]):
+            self.f[n] = s1[j]
+        for j, n in enumerate(["thn", "th", "thp"]):
+            self.f[n] = s2[j]
+        for j, n in enumerate(["uel", "vel", "wel", "uelp", "velp", "welp"]):
+            self.f[n] = s3[j]
+        for j, n in enumerate(["psic", "psif", "rc", "rf", "tempa"]):
+            self.f[n] = ar[j]
+        baddr = ctypes.addressof(ctypes.c_char.in_dll(L, "backup_"))
+        bk = np.frombuffer((ctypes.c_double * (2 * MGEN)).from_address(baddr), dtype=np.float64)
+        self.v, self.fl = bk[:MGEN], bk[MGEN:]
+        numaddr = ctypes.addressof(ctypes.c_char.in_dll(L, "num_"))
+        self.num_d = (ctypes.c_double * 4).from_address(numaddr)
+        self.num_i = (ctypes.c_int * 3).from_address(numaddr + 32)
+        self.val = (ctypes.c_double * 4).in_dll(L, "val_")
+        self.radius = (ctypes.c_double * 2).in_dll(L, "radius_")
+        self.ints = (ctypes.c_int * 24).in_dll(L, "ints_")
+        for n in ("urh", "vrh", "wrh"):
+            self.f[n] = np.zeros(SHAPE)
+
+    def setup(self, p):
+        """Scalars from a V3Params (the main program's rot3dmg.f:73-93)."""
+        self.num_d[:] = [p.dt, p.dr, p.dz, p.dp]
+        self.num_i[:] = [p.nr, p.nz, p.np]
+        self.val[:] = [p.Ra, p.Re, p.Ptl, p.Gr]
+        self.radius[:] = [p.r0, p.rmax]
+
+    def call(self, name):
+        getattr(self.L, name + "_")()
+
+    def heat(self):
+        t = ctypes.c_double(0.0)
+        self.L.heat_(ctypes.byref(t))
+        return t.value
+
+    def rhs(self, which):
+        name = {1: "wrh", 2: "vrh", 3: "urh"}[which]
+        fn = {1: "rhsw_", 2: "rhsv_", 3: "rhsu_"}[which]
+        getattr(self.L, fn)(self.f[name].ctypes.data_as(_D))
+
+    def relax_residual(self, which, p, psi, rhs):
+        """One pass of relax1/2/3 on the finest level (rot3dmg.f:1543,
+        1739, 1986): psi and rhs go through trans1 into the flat
+        multigrid storage, the routine relaxes once and stores the residual
+        of the relaxed iterate in /arrays/ rc.  Returns (psif, rc)."""
+        nlev = ctypes.c_int(0)
+        ni, nz, np_ = ctypes.c_int(p.nr), ctypes.c_int(p.nz), ctypes.c_int(p.np)
+        dr, dz, dp = ctypes.c_double(p.dr), ctypes.c_double(p.dz), ctypes.c_double(p.dp)
+        self.L.poissoninit_(ctypes.byref(ni), ctypes.byref(nz), ctypes.byref(np_), ctypes.byref(dr),
+                            ctypes.byref(dz), ctypes.byref(dp), ctypes.byref(nlev))
+        ibeg, n1, n2, n3 = self.ints[0], self.ints[6], self.ints[12], self.ints[18]
+        self.v[:] = 0.0
+        self.fl[:] = 0.0
+        self.f["rf"][...] = rhs
+        self.f["psif"][...] = psi
+        args = [ctypes.byref(ctypes.c_int(x)) for x in (ibeg, n1, n2, n3)]
+        self.L.trans1_(self.f["rf"].ctypes.data_as(_D), self.fl.ctypes.data_as(_D), *args)
+        self.L.trans1_(self.f["psif"].ctypes.data_as(_D), self.v.ctypes.data_as(_D), *args)
+        nrel, klev, nconv = ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(0)
+        fn = {1: "relax1_", 2: "relax2_", 3: "relax3_"}[which]
+        getattr(self.L, fn)(ctypes.byref(nrel), ctypes.byref(klev), ctypes.byref(nconv), ctypes.byref(nlev))
+        return self.f["psif"].copy(), self.f["rc"].copy()
