@@ -1,0 +1,154 @@
+"""Host interface of the 3D rotating-annulus step (rot3dmg.f) on the HIP
+library: one object per run, methods named after the reference's routines
+(Avorticity, ..., poisson1/2/3, update), fields as numpy arrays of shape
+(np+1, nz+2, nr+2) (X[k, j, i] is X(i, j, k) of rot3dmg.f).
+
+The compute path is libvvmi.so (include/vvmi3.h); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, lib
+
+FIELDS = ["An", "A", "Ap", "Bn", "B", "Bp", "Cn", "C", "Cp", "thn", "th", "thp",
+          "uel", "vel", "wel", "uelp", "velp", "welp", "urh", "vrh", "wrh"]
+FIELD_IDS = {n: q for q, n in enumerate(FIELDS)}
+
+
+class VV3Config(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("nz", ctypes.c_int), ("np", ctypes.c_int),
+                ("dt", ctypes.c_double), ("r0", ctypes.c_double), ("rmax", ctypes.c_double),
+                ("zmax", ctypes.c_double), ("Ra", ctypes.c_double), ("Re", ctypes.c_double),
+                ("Ptl", ctypes.c_double), ("Gr", ctypes.c_double)]
+
+
+@dataclass
+class Annulus3DConfig:
+    """The scalars of rot3dmg.f's input deck (rot3dmg.f:59-86)."""
+    nr: int
+    nz: int
+    np: int
+    dt: float
+    Ra: float
+    Re: float
+    Ptl: float
+    Gr: float
+    r0: float = 0.5
+    rmax: float = 1.0
+    zmax: float = 0.5
+
+    def c(self) -> VV3Config:
+        return VV3Config(self.nr, self.nz, self.np, self.dt, self.r0, self.rmax, self.zmax,
+                         self.Ra, self.Re, self.Ptl, self.Gr)
+
+
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_vv3_bound", False):
+        return L
+    vp = ctypes.c_void_p
+    L.vv3_create.argtypes = [ctypes.POINTER(VV3Config), ctypes.c_int, ctypes.POINTER(vp)]
+    L.vv3_destroy.argtypes = [vp]
+    L.vv3_put.argtypes = [vp, ctypes.c_int, _P]
+    L.vv3_get.argtypes = [vp, ctypes.c_int, _P]
+    L.vv3_vorticity.argtypes = [vp, ctypes.c_int]
+    L.vv3_temperature.argtypes = [vp]
+    L.vv3_heat.argtypes = [vp, _P]
+    L.vv3_rhs.argtypes = [vp, ctypes.c_int]
+    L.vv3_poisson.argtypes = [vp, ctypes.c_int]
+    L.vv3_update.argtypes = [vp]
+    L.vv3_step.argtypes = [vp, ctypes.c_int, _P]
+    L._vv3_bound = True
+    return L
+
+
+class Annulus3D:
+    """rot3dmg.f's state on one GPU.  The routines act on the device fields;
+    put/get move whole fields."""
+
+    def __init__(self, cfg: Annulus3DConfig, device: int = 0):
+        self.cfg = cfg
+        self.shape = (cfg.np + 1, cfg.nz + 2, cfg.nr + 2)
+        self._L = _bind()
+        self._h = ctypes.c_void_p()
+        c = cfg.c()
+        check(self._L.vv3_create(ctypes.byref(c), device, ctypes.byref(self._h)), "vv3_create")
+
+    def close(self):
+        if self._h:
+            self._L.vv3_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def put(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"{name}: shape {a.shape}, expected {self.shape}")
+        check(self._L.vv3_put(self._h, FIELD_IDS[name], a.ctypes.data_as(_P)), "vv3_put")
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape)
+        check(self._L.vv3_get(self._h, FIELD_IDS[name], out.ctypes.data_as(_P)), "vv3_get")
+        return out
+
+    # rot3dmg.f routine names
+    def Avorticity(self):
+        check(self._L.vv3_vorticity(self._h, 1), "vv3_vorticity")
+
+    def Bvorticity(self):
+        check(self._L.vv3_vorticity(self._h, 2), "vv3_vorticity")
+
+    def Cvorticity(self):
+        check(self._L.vv3_vorticity(self._h, 3), "vv3_vorticity")
+
+    def vorticity(self):
+        """Avorticity, Bvorticity and Cvorticity in one launch."""
+        check(self._L.vv3_vorticity(self._h, 0), "vv3_vorticity")
+
+    def temperature(self):
+        check(self._L.vv3_temperature(self._h), "vv3_temperature")
+
+    def heat(self) -> float:
+        t = ctypes.c_double()
+        check(self._L.vv3_heat(self._h, ctypes.byref(t)), "vv3_heat")
+        return t.value
+
+    def rhsw(self):
+        check(self._L.vv3_rhs(self._h, 1), "vv3_rhs")
+
+    def rhsv(self):
+        check(self._L.vv3_rhs(self._h, 2), "vv3_rhs")
+
+    def rhsu(self):
+        check(self._L.vv3_rhs(self._h, 3), "vv3_rhs")
+
+    def poisson1(self):
+        check(self._L.vv3_poisson(self._h, 1), "vv3_poisson")
+
+    def poisson2(self):
+        check(self._L.vv3_poisson(self._h, 2), "vv3_poisson")
+
+    def poisson3(self):
+        check(self._L.vv3_poisson(self._h, 3), "vv3_poisson")
+
+    def update(self):
+        check(self._L.vv3_update(self._h), "vv3_update")
+
+    def step(self, n: int = 1) -> float:
+        """n passes of the reference time loop (rot3dmg.f:240-253); returns
+        trans1 of the last pass."""
+        t = ctypes.c_double()
+        check(self._L.vv3_step(self._h, n, ctypes.byref(t)), "vv3_step")
+        return t.value

@@ -1,0 +1,74 @@
+/* vvmi3.h -- C ABI of the MI355X 3D rotating-annulus time step
+ * (rot3dmg.f: 3D velocity-vorticity model on a staggered grid, periodic in
+ * phi, walls in r and z).  FP64 throughout; status codes and error text are
+ * shared with vvmi.h (VV_OK, VV_EINVAL, ..., vv_last_error()).
+ *
+ * Fields are (np+1) x (nz+2) x (nr+2) doubles, i fastest, then j, then k:
+ * X(i,j,k) at host[(k*(nz+2) + j)*(nr+2) + i], i = 0..nr+1, j = 0..nz+1,
+ * k = 0..np, as the reference's (0:kr,0:kz,0:kp) arrays with kr = nr+1,
+ * kz = nz+1.  k = 0 and k = np are the same azimuth (both are kept).
+ */
+#ifndef VVMI3_H
+#define VVMI3_H
+#include "vvmi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vv3_run vv3_run;
+
+/* the scalars of rot3dmg.f's input deck (rot3dmg.f:59-86) */
+typedef struct {
+    int    nr, nz, np;          /* intervals in r, z, phi (np even)          */
+    double dt;
+    double r0, rmax, zmax;      /* inner and outer radius, height            */
+    double Ra, Re, Ptl, Gr;     /* Rayleigh, Reynolds, Prandtl, Grashof      */
+} vv3_config;
+
+enum vv3_field {
+    VV3_AN = 0, VV3_A, VV3_AP, VV3_BN, VV3_B, VV3_BP, VV3_CN, VV3_C, VV3_CP,
+    VV3_THN, VV3_TH, VV3_THP, VV3_UEL, VV3_VEL, VV3_WEL, VV3_UELP, VV3_VELP,
+    VV3_WELP, VV3_URH, VV3_VRH, VV3_WRH, VV3_NFIELD
+};
+
+/* Allocates the fields and the Poisson solver on `device`.  The three
+ * Poisson operators are solved exactly: a real DFT in phi decouples each
+ * into one (r,z) system per Fourier mode, whose inverse is formed here (the
+ * reference builds its multigrid levels in poissoninit, rot3dmg.f:1153).
+ * This direct solver holds (np/2+1) * m^2 doubles per operator, m = unknowns
+ * of an (r,z) plane, and is limited to m <= 1024 (VV_EINVAL beyond). */
+int vv3_create(const vv3_config *cfg, int device, vv3_run **out);
+int vv3_destroy(vv3_run *run);
+
+int vv3_put(vv3_run *run, int field, const double *host);
+int vv3_get(vv3_run *run, int field, double *host);
+
+/* Avorticity, Bvorticity, Cvorticity (rot3dmg.f:341, 477, 601): An, Bn, Cn
+ * on their interior points, with the hybrid upwind switch of the reference.
+ * which = 1 / 2 / 3 runs one routine, 0 all three in one launch (they read
+ * only the old levels, so the order does not matter) */
+int vv3_vorticity(vv3_run *run, int which);
+/* temperature (rot3dmg.f:736): thn on i = 1..nr, j = 0..nz+1 */
+int vv3_temperature(vv3_run *run);
+/* heat (rot3dmg.f:882): inner-wall heat flux trans1 */
+int vv3_heat(vv3_run *run, double *trans1);
+/* rhsw / rhsv / rhsu (rot3dmg.f:1026, 1066, 1111): which = 1 / 2 / 3 */
+int vv3_rhs(vv3_run *run, int which);
+/* poisson1 / poisson2 / poisson3 (rot3dmg.f:1216, 1325, 1434): solves the
+ * operator of relax1/2/3's residual (rot3dmg.f:1695, 1927, 2130) for wel /
+ * vel / uel from wrh / vrh / urh, exactly; psip <- old psi, walls 0 as the
+ * reference writes them.  The reference iterates to an absolute residual of
+ * 1e-5 (rot3dmg.f:1548); this solve is exact to rounding. */
+int vv3_poisson(vv3_run *run, int which);
+/* update (rot3dmg.f:964): wall vorticity and wall temperatures, then the
+ * time levels rotate (Ap <- A <- An, ... for A, B, C, th) */
+int vv3_update(vv3_run *run);
+/* n steps of the reference time loop (rot3dmg.f:240-253); trans1 of the
+ * last step is returned in *trans1 (may be NULL) */
+int vv3_step(vv3_run *run, int n, double *trans1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -8,13 +8,16 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vvmi.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h")]
 LIB = os.path.join(ROOT, "codesthesis_dir_amd", "libvvmi.so")
 
 
 def declared():
-    src = open(HEADER).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(vv_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    for h in HEADERS:
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(vv3?_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
@@ -30,6 +33,15 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_3d_header_declares_entry_points():
+    names = declared()
+    for n in ("vv3_create", "vv3_destroy", "vv3_put", "vv3_get", "vv3_vorticity", "vv3_temperature",
+              "vv3_heat", "vv3_rhs", "vv3_poisson", "vv3_update", "vv3_step"):
+        assert n in names
+    src = open(HEADERS[1]).read()
+    assert src.count("rot3dmg.f:") >= 8
 
 
 def test_every_declaration_cites_the_reference():
