@@ -6,24 +6,11 @@
 // (the library is compiled with -ffp-contract=off), so the results equal
 // the reference bit for bit.
 //
-// Poisson: each of the three operators (the residuals of relax1/2/3,
-// rot3dmg.f:1695-1723, 1927-1965, 2130-2165) is periodic and invariant in
-// phi, so a real DFT over k = 1..np decouples it into one (r,z) system per
-// Fourier mode m = 0..np/2 (cos and sin parts share the system).  The
-// systems' inverses are formed once on the host; a solve is a projection
-// kernel, one dense matrix-vector product per (mode, part), a synthesis
-// kernel and the reference's output conventions.
+// The three Poisson solves are exact separable solves (vv3_poisson.hip).
 
-#include "vv_internal.h"
-#include "vv_reduce.h"
+#include "vv3_internal.h"
 
-#include "../../include/vvmi3.h"
-
-#include <algorithm>
-#include <atomic>
 #include <cmath>
-#include <cstring>
-#include <thread>
 #include <vector>
 
 struct vv3_run {
@@ -34,14 +21,7 @@ struct vv3_run {
     double dr = 0, dz = 0, dp = 0;
     size_t nf = 0;                       // doubles per field
     double *f[VV3_NFIELD] = {nullptr};   // device fields (rotated by pointer swap)
-    // direct Poisson solver, per operator w (0), v (1), u (2)
-    struct Op {
-        int i1 = 0, j1 = 0, m = 0;       // unknowns i = 1..i1, j = 1..j1
-        double *inv = nullptr;           // (np/2+1) x m x m, column-major per mode
-        double *spec = nullptr;          // (np/2+1) x 2 x m: projections of the rhs
-        double *sol = nullptr;           // (np/2+1) x 2 x m: per-mode solutions
-    } op[3];
-    double *twid = nullptr;              // cos | sin (2 pi q/np), q = 0..np-1
+    vv3::Poisson3 pois;                  // the three Poisson operators
     double *heat = nullptr;              // per-k partial fluxes (np+1) and the total
 };
 
@@ -50,11 +30,6 @@ namespace vv3 {
 namespace {
 
 constexpr int WG = 256;
-
-struct Geo {
-    int nr, nz, np, ldi, ldj;
-    double dt, dr, dz, dp, Ra, Re, Ptl, Gr, r0;
-};
 
 #define X(a, i, j, k) (a)[((size_t)(k) * g.ldj + (size_t)(j)) * g.ldi + (size_t)(i)]
 
@@ -503,206 +478,7 @@ __global__ void __launch_bounds__(WG) k3_update(Geo g, UpdF F)
     F.thp[p] = th;
 }
 
-// Poisson: projection of the right-hand side on the Fourier modes,
-// spec[(mode*2 + part)*m + u] (u = (j-1)*i1 + (i-1)); thread per (u, mode)
-__global__ void __launch_bounds__(WG) k3_dft_fwd(Geo g, const double *rhs, double *spec, const double *twid,
-                                                  int i1, int j1)
-{
-    const int m = i1 * j1, nm = g.np / 2 + 1, np = g.np;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)m * nm) return;
-    const int u = (int)(t % m), mode = (int)(t / m);
-    const int i = 1 + u % i1, j = 1 + u / i1;
-    const double *cs = twid, *sn = twid + np;
-    double ac = 0.0, as = 0.0;
-    for (int k = 1; k <= np; ++k) {
-        const double f = X(rhs, i, j, k);
-        const int q = (int)(((long long)mode * k) % np);
-        ac += f * cs[q];
-        as += f * sn[q];
-    }
-    spec[((size_t)mode * 2) * m + u] = ac;
-    spec[((size_t)mode * 2 + 1) * m + u] = as;
-}
-
-// x <- inv_mode * x for the cos and sin parts of a mode: a thread per row
-// (blockIdx.x: rows, blockIdx.y: mode), the two vectors in LDS, the inverse
-// column-major so a column is read with consecutive rows at consecutive
-// addresses
-__global__ void __launch_bounds__(WG) k3_modes(const double *spec, double *sol, const double *inv, int m)
-{
-    const int mode = blockIdx.y;
-    const double *xc = spec + (size_t)mode * 2 * m;
-    const double *Mi = inv + (size_t)mode * m * m;
-    extern __shared__ double xs[];
-    for (int c = threadIdx.x; c < 2 * m; c += blockDim.x) xs[c] = xc[c];
-    __syncthreads();
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    double yc = 0.0, ys = 0.0;
-    int c = 0;
-    for (; c + 4 <= m; c += 4) {
-        const double a0 = Mi[(size_t)c * m + row], a1 = Mi[(size_t)(c + 1) * m + row];
-        const double a2 = Mi[(size_t)(c + 2) * m + row], a3 = Mi[(size_t)(c + 3) * m + row];
-        yc += a0 * xs[c] + a1 * xs[c + 1] + a2 * xs[c + 2] + a3 * xs[c + 3];
-        ys += a0 * xs[m + c] + a1 * xs[m + c + 1] + a2 * xs[m + c + 2] + a3 * xs[m + c + 3];
-    }
-    for (; c < m; ++c) {
-        const double a0 = Mi[(size_t)c * m + row];
-        yc += a0 * xs[c];
-        ys += a0 * xs[m + c];
-    }
-    sol[(size_t)mode * 2 * m + row] = yc;
-    sol[(size_t)mode * 2 * m + m + row] = ys;
-}
-
-// synthesis and the reference's output: thread per (i, j, k) of the whole
-// field; psip <- old psi and psi <- solution / 0 on the ranges poisson1/2/3
-// write (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)
-__global__ void __launch_bounds__(WG) k3_dft_out(Geo g, const double *spec, const double *twid, int which, int i1, int j1,
-                                                  double *psi, double *psip)
-{
-    const int nr = g.nr, nz = g.nz, np = g.np;
-    const int m = i1 * j1, nm = np / 2 + 1;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long cnt = (long long)(nr + 2) * (nz + 2) * (np + 1);
-    if (t >= cnt) return;
-    const int i = (int)(t % (nr + 2));
-    const int j = (int)((t / (nr + 2)) % (nz + 2));
-    const int k = (int)(t / ((long long)(nr + 2) * (nz + 2)));
-    // ranges of psip <- psi and of the new psi
-    int pi1, pj1;
-    if (which == 1) { pi1 = nr + 1; pj1 = nz; }
-    else if (which == 2) { pi1 = nr + 1; pj1 = nz + 1; }
-    else { pi1 = nr; pj1 = nz + 1; }
-    if (i > pi1 || j > pj1) return;
-    const double old = X(psi, i, j, k);
-    X(psip, i, j, k) = old;
-    double nv = 0.0;
-    const bool interior = (i >= 1 && i <= i1 && j >= 1 && j <= j1);
-    if (interior) {
-        const int u = (j - 1) * i1 + (i - 1);
-        const int kk = (k == 0) ? np : k;
-        const double *cs = twid, *sn = twid + np;
-        double s = 0.0;
-        for (int mode = 0; mode < nm; ++mode) {
-            const int q = (int)(((long long)mode * kk) % np);
-            const double w = (mode == 0 || 2 * mode == np) ? 1.0 : 2.0;
-            s += w * (spec[((size_t)mode * 2) * m + u] * cs[q] + spec[((size_t)mode * 2 + 1) * m + u] * sn[q]);
-        }
-        nv = s / (double)np;
-    }
-    X(psi, i, j, k) = nv;
-}
-
 inline int nblk(long long n, int wg) { return (int)((n + wg - 1) / wg); }
-
-// L psi at (i, j, k) for operator `which` (1 w, 2 v, 3 u) on the host, the
-// residual operators of relax1/2/3 (rot3dmg.f:1698-1719, 1935-1961,
-// 2133-2154), used to assemble each mode's (r,z) matrix
-struct HostOp {
-    int nr, nz, np;
-    double r0, dr, dz, dp;
-    // coefficient of psi(ic, jc, kc) in (L psi)(i, j, k) with psi = e_(ic,jc) cos(mode phi)
-    double entry(int which, int i, int j, int ic, int jc, int mode) const
-    {
-        const double hhr = dr * dr, hhz = dz * dz, hhp = dp * dp;
-        const double lam = 2.0 * std::cos(2.0 * std::acos(-1.0) * (double)mode / np) - 2.0;
-        double a = 0.0;
-        auto add = [&](int ii, int jj, double c) { if (ii == ic && jj == jc) a += c; };
-        if (which == 1) {
-            const double rval = r0 + ((double)i - 0.5) * dr;
-            add(i, j + 1, 1.0 / hhz); add(i, j - 1, 1.0 / hhz);
-            add(i, j, lam / hhp / (rval * rval) - 2.0 / hhz);
-            if (i == 1) {
-                add(i, j, -2.0 / hhr - 0.5 / dr / rval); add(i + 1, j, 5.0 / 3.0 / hhr + 0.5 / dr / rval);
-                add(i + 2, j, -0.2 / hhr);
-            } else if (i == nr) {
-                add(i, j, -2.0 / hhr + 0.5 / dr / rval); add(i - 1, j, 5.0 / 3.0 / hhr - 0.5 / dr / rval);
-                add(i - 2, j, -0.2 / hhr);
-            } else {
-                add(i - 1, j, 1.0 / hhr - 0.5 / dr / rval); add(i, j, -2.0 / hhr); add(i + 1, j, 1.0 / hhr + 0.5 / dr / rval);
-            }
-        } else if (which == 2) {
-            const double rval = r0 + ((double)i - 0.5) * dr;
-            add(i, j, lam / hhp / (rval * rval) + 1.0 / (rval * rval));
-            if (i == 1) {
-                add(i, j, -2.0 / hhr); add(i + 1, j, 5.0 / 3.0 / hhr + 4.0 / 3.0 / dr / rval); add(i + 2, j, -0.2 / hhr);
-            } else if (i == nr) {
-                add(i, j, -2.0 / hhr); add(i - 1, j, 5.0 / 3.0 / hhr - 4.0 / 3.0 / dr / rval); add(i - 2, j, -0.2 / hhr);
-            } else {
-                add(i - 1, j, 1.0 / hhr - 1.5 / dr / rval); add(i, j, -2.0 / hhr); add(i + 1, j, 1.0 / hhr + 1.5 / dr / rval);
-            }
-            if (j == 1) {
-                add(i, j, -2.0 / hhz); add(i, j + 1, 5.0 / 3.0 / hhz); add(i, j + 2, -0.2 / hhz);
-            } else if (j == nz) {
-                add(i, j, -2.0 / hhz); add(i, j - 1, 5.0 / 3.0 / hhz); add(i, j - 2, -0.2 / hhz);
-            } else {
-                add(i, j - 1, 1.0 / hhz); add(i, j, -2.0 / hhz); add(i, j + 1, 1.0 / hhz);
-            }
-        } else {
-            const double rval = (double)i * dr + r0;
-            add(i + 1, j, 1.0 / hhr + 1.5 / dr / rval); add(i - 1, j, 1.0 / hhr - 1.5 / dr / rval);
-            add(i, j, 1.0 / (rval * rval) + lam / hhp / (rval * rval) - 2.0 / hhr);
-            if (j == 1) {
-                add(i, j, -2.0 / hhz); add(i, j + 1, 5.0 / 3.0 / hhz); add(i, j + 2, -0.2 / hhz);
-            } else if (j == nz) {
-                add(i, j, -2.0 / hhz); add(i, j - 1, 5.0 / 3.0 / hhz); add(i, j - 2, -0.2 / hhz);
-            } else {
-                add(i, j - 1, 1.0 / hhz); add(i, j, -2.0 / hhz); add(i, j + 1, 1.0 / hhz);
-            }
-        }
-        return a;
-    }
-};
-
-// Inverse of a banded n x n matrix (row-major, lower and upper bandwidths
-// kl, ku), column-major into inv: LU with partial pivoting inside the band
-// (rows swapped only from the pivot column on; the solves apply the swaps in
-// order), then one forward and back substitution per unit vector.  Returns
-// false if a pivot is zero.
-bool band_inverse(std::vector<double> &a, int n, int kl, int ku, double *inv)
-{
-    const int kw = kl + ku;   // upper bandwidth of U after the swaps
-    std::vector<int> piv(n);
-    for (int c = 0; c < n; ++c) {
-        const int rlast = std::min(n - 1, c + kl), qlast = std::min(n - 1, c + kw);
-        int p = c;
-        for (int r = c + 1; r <= rlast; ++r)
-            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)p * n + c])) p = r;
-        if (a[(size_t)p * n + c] == 0.0) return false;
-        piv[c] = p;
-        if (p != c)
-            for (int q = c; q <= qlast; ++q) std::swap(a[(size_t)c * n + q], a[(size_t)p * n + q]);
-        const double d = a[(size_t)c * n + c];
-        for (int r = c + 1; r <= rlast; ++r) {
-            const double f = a[(size_t)r * n + c] / d;
-            a[(size_t)r * n + c] = f;
-            if (f == 0.0) continue;
-            for (int q = c + 1; q <= qlast; ++q) a[(size_t)r * n + q] -= f * a[(size_t)c * n + q];
-        }
-    }
-    std::vector<double> b(n);
-    for (int e = 0; e < n; ++e) {
-        std::fill(b.begin(), b.end(), 0.0);
-        b[e] = 1.0;
-        for (int c = std::max(0, e - kl); c < n; ++c) {
-            if (piv[c] != c) std::swap(b[c], b[piv[c]]);
-            const double bc = b[c];
-            if (bc == 0.0) continue;
-            const int rlast = std::min(n - 1, c + kl);
-            for (int r = c + 1; r <= rlast; ++r) b[r] -= a[(size_t)r * n + c] * bc;
-        }
-        for (int c = n - 1; c >= 0; --c) {
-            double s = b[c];
-            const int qlast = std::min(n - 1, c + kw);
-            for (int q = c + 1; q <= qlast; ++q) s -= a[(size_t)c * n + q] * b[q];
-            b[c] = s / a[(size_t)c * n + c];
-        }
-        std::copy(b.begin(), b.end(), inv + (size_t)e * n);
-    }
-    return true;
-}
 
 }  // namespace
 }  // namespace vv3
@@ -734,12 +510,6 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         set_error("vv3_create: bad physical parameters");
         return VV_EINVAL;
     }
-    const int mmax = cfg->nr * cfg->nz;
-    if (mmax > 1024) {
-        set_error("vv3_create: the direct 3D Poisson solve holds dense per-mode inverses and is limited to "
-                  "nr*nz <= 1024 unknowns per (r,z) plane");
-        return VV_EINVAL;
-    }
     VV_HIP(hipSetDevice(device));
     vv3_run *r = new vv3_run();
     r->cfg = *cfg;
@@ -756,67 +526,10 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         if (hipMalloc(&r->f[q], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
         if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
     }
+    if (hipMalloc(&r->heat, (size_t)(cfg->np + 2) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     {
-        std::vector<double> tw(2 * (size_t)cfg->np);
-        const double pi = std::acos(-1.0);
-        for (int q = 0; q < cfg->np; ++q) {
-            tw[q] = std::cos(2.0 * pi * q / cfg->np);
-            tw[cfg->np + q] = std::sin(2.0 * pi * q / cfg->np);
-        }
-        if (hipMalloc(&r->twid, tw.size() * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
-        if (hipMemcpy(r->twid, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(VV_EHIP);
-        if (hipMalloc(&r->heat, (size_t)(cfg->np + 2) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
-    }
-    // each mode's (r,z) matrix, assembled from the residual operator and
-    // inverted on the host, the (operator, mode) pairs spread over threads
-    HostOp H{cfg->nr, cfg->nz, cfg->np, cfg->r0, r->dr, r->dz, r->dp};
-    const int nm = cfg->np / 2 + 1;
-    std::vector<std::vector<double>> all(3);
-    for (int w = 1; w <= 3; ++w) {
-        auto &O = r->op[w - 1];
-        O.i1 = (w == 3) ? cfg->nr - 1 : cfg->nr;
-        O.j1 = (w == 1) ? cfg->nz - 1 : cfg->nz;
-        O.m = O.i1 * O.j1;
-        all[w - 1].assign((size_t)nm * O.m * O.m, 0.0);
-    }
-    std::atomic<int> next{0}, bad{0};
-    auto worker = [&]() {
-        std::vector<double> a;
-        for (int task; (task = next.fetch_add(1)) < 3 * nm && !bad.load();) {
-            const int w = 1 + task / nm, mode = task % nm;
-            const auto &O = r->op[w - 1];
-            const int m = O.m;
-            a.assign((size_t)m * m, 0.0);
-            for (int rr = 0; rr < m; ++rr) {
-                const int i = 1 + rr % O.i1, j = 1 + rr / O.i1;
-                for (int dj = -2; dj <= 2; ++dj)
-                    for (int di = -2; di <= 2; ++di) {
-                        const int ic = i + di, jc = j + dj;
-                        if (ic < 1 || ic > O.i1 || jc < 1 || jc > O.j1) continue;
-                        a[(size_t)rr * m + (size_t)(jc - 1) * O.i1 + (ic - 1)] = H.entry(w, i, j, ic, jc, mode);
-                    }
-            }
-            if (!band_inverse(a, m, 2 * O.i1, 2 * O.i1, all[w - 1].data() + (size_t)mode * m * m)) bad = 1;
-        }
-    };
-    {
-        const unsigned hw = std::thread::hardware_concurrency();
-        const int nth = (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
-        std::vector<std::thread> pool;
-        for (int q = 1; q < nth; ++q) pool.emplace_back(worker);
-        worker();
-        for (auto &th : pool) th.join();
-    }
-    if (bad) { set_error("vv3_create: singular Poisson mode system"); return fail(VV_EINVAL); }
-    for (int w = 1; w <= 3; ++w) {
-        auto &O = r->op[w - 1];
-        const size_t bytes = all[w - 1].size() * sizeof(double);
-        if (hipMalloc(&O.inv, bytes) != hipSuccess) return fail(VV_ENOMEM);
-        if (hipMemcpy(O.inv, all[w - 1].data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(VV_EHIP);
-        std::vector<double>().swap(all[w - 1]);
-        if (hipMalloc(&O.spec, (size_t)nm * 2 * O.m * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
-        if (hipMalloc(&O.sol, (size_t)nm * 2 * O.m * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        const int st = poisson3_setup(r->pois, geo(r));
+        if (st) return fail(st);
     }
     if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
     *out = r;
@@ -830,12 +543,7 @@ int vv3_destroy(vv3_run *r)
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (int q = 0; q < VV3_NFIELD; ++q)
         if (r->f[q]) (void)hipFree(r->f[q]);
-    for (auto &O : r->op) {
-        if (O.inv) (void)hipFree(O.inv);
-        if (O.spec) (void)hipFree(O.spec);
-        if (O.sol) (void)hipFree(O.sol);
-    }
-    if (r->twid) (void)hipFree(r->twid);
+    poisson3_free(r->pois);
     if (r->heat) (void)hipFree(r->heat);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
@@ -892,20 +600,10 @@ static int launch_rhs(vv3_run *r, int which)
 
 static int launch_poisson(vv3_run *r, int which)
 {
-    auto &O = r->op[which - 1];
-    const int nm = r->np / 2 + 1;
     const int rhs = (which == 1) ? VV3_WRH : (which == 2) ? VV3_VRH : VV3_URH;
     const int psi = (which == 1) ? VV3_WEL : (which == 2) ? VV3_VEL : VV3_UEL;
     const int psip = (which == 1) ? VV3_WELP : (which == 2) ? VV3_VELP : VV3_UELP;
-    const Geo g = geo(r);
-    hipLaunchKernelGGL(k3_dft_fwd, dim3(nblk((long long)O.m * nm, WG)), dim3(WG), 0, r->stream, g,
-                       (const double *)r->f[rhs], O.spec, (const double *)r->twid, O.i1, O.j1);
-    hipLaunchKernelGGL(k3_modes, dim3(nblk(O.m, WG), nm), dim3(WG), 2 * O.m * sizeof(double), r->stream,
-                       (const double *)O.spec, O.sol, (const double *)O.inv, O.m);
-    hipLaunchKernelGGL(k3_dft_out, dim3(nblk((long long)r->nf, WG)), dim3(WG), 0, r->stream, g,
-                       (const double *)O.sol, (const double *)r->twid, which, O.i1, O.j1, r->f[psi], r->f[psip]);
-    VV_HIP(hipGetLastError());
-    return VV_OK;
+    return poisson3_launch(r->pois, geo(r), which, r->f[psi], r->f[psip], r->f[rhs], r->stream);
 }
 
 static int launch_update(vv3_run *r)

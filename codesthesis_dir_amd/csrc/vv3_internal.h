@@ -1,0 +1,68 @@
+// vv3_internal.h -- shared declarations of the 3D step (not part of the ABI).
+#pragma once
+
+#include "vv_internal.h"
+
+#include "../../include/vvmi3.h"
+
+#include <vector>
+
+namespace vv3 {
+
+// grid and parameters handed to every kernel by value
+struct Geo {
+    int nr, nz, np, ldi, ldj;
+    double dt, dr, dz, dp, Ra, Re, Ptl, Gr, r0;
+};
+
+// One of the three Poisson operators (1: w, 2: v, 3: u).  Unknowns are
+// i = 1..i1, j = 1..j1, k = 1..np; an (r,z) plane holds m = i1*j1 of them.
+// The operator is L = D_phi + I_z (x) R + Z (x) I_r: a real DFT in phi turns
+// D_phi into a shift per Fourier mode, the z matrix Z = V diag(lz) V^-1 is
+// diagonalised once, and what remains per (mode, z eigenvalue) is a banded
+// system in r.
+struct PoissonOp {
+    int which = 0, i1 = 0, j1 = 0, m = 0;
+    double e1 = 0.0, en = 0.0;   // r-rows 1 and i1: coefficient two points inward
+    double *V = nullptr;         // j1 x j1, row-major: eigenvectors of Z
+    double *Vinv = nullptr;      // j1 x j1, row-major
+    double *rt = nullptr;        // 4 x i1: sub, diagonal, super, 1/(dp^2 r^2)
+    double *lz = nullptr;        // j1 eigenvalues of Z
+};
+
+struct Poisson3 {
+    int nr = 0, nz = 0, np = 0, nm = 0, ldi = 0, ldj = 0;
+    PoissonOp op[3];
+    double *W = nullptr;     // 2nm x np: cos / sin rows of the forward DFT over k = 1..np
+    double *Wi = nullptr;    // np x 2nm: the inverse (weights and 1/np folded in)
+    double *lphi = nullptr;  // nm: 2 cos(2 pi mode/np) - 2
+    double *S = nullptr;     // 2nm x mmax: phi spectra, then the z-inverse result
+    double *T = nullptr;     // 2nm x mmax: z spectra, (i, l) with l fastest
+    double *cp = nullptr;    // nm x mmax: elimination factors of the r solves
+};
+
+// Host-side pieces, exposed so that they can be checked without a GPU.
+// Rows of the r operator of `which` (rot3dmg.f:1698-1719, 1935-1961,
+// 2133-2154 without their z and phi terms): sub, diagonal and super
+// coefficients of row i (1..i1), and the coefficient two points inward on
+// rows 1 and i1.
+void r_rows(int which, int nr, double r0, double dr, std::vector<double> &sub, std::vector<double> &diag,
+            std::vector<double> &sup, double &e1, double &en);
+// The z matrix of `which` (scaled by dz^2): Dirichlet (1,-2,1) for w, with
+// the (-2, 5/3, -0.2) wall closures for v and u; j1 x j1 row-major.
+std::vector<double> z_matrix(int which, int j1);
+// Real eigen-decomposition of a pentadiagonal n x n matrix with real,
+// distinct eigenvalues: eigenvalues ascending, V (columns = eigenvectors,
+// row-major), V^-1.  Returns false if the eigenvalues cannot be separated or
+// the decomposition does not reproduce the matrix.
+bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam, std::vector<double> &V,
+                  std::vector<double> &Vinv);
+
+int poisson3_setup(Poisson3 &P, const Geo &g);
+void poisson3_free(Poisson3 &P);
+// solve L psi = rhs for operator `which`; psip <- old psi and walls 0 on the
+// range the reference writes (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)
+int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, double *psip, const double *rhs,
+                    hipStream_t s);
+
+}  // namespace vv3

@@ -1,0 +1,547 @@
+// vv3_poisson.hip -- exact solves of rot3dmg.f's three Poisson operators
+// (the residuals of relax1/2/3, rot3dmg.f:1695-1723, 1927-1965, 2130-2165)
+// on gfx950.
+//
+// Each operator is L = D_phi + I_z (x) R + Z (x) I_r on the unknowns
+// i = 1..i1, j = 1..j1, k = 1..np (k periodic): D_phi is the periodic second
+// difference in phi divided by (dp r_i)^2, R the r rows, Z the z rows.  A
+// solve is
+//   1. a real DFT over k (a GEMM with a cos/sin table): one (r,z) problem
+//      per Fourier mode and part,
+//   2. the z eigenbasis: y = V^-1 x along j (a batched GEMM),
+//   3. per (mode, z eigenvalue) one banded system in r, eliminated by a
+//      thread (Thomas, with the two wall rows' extra coefficient folded in),
+//   4. x = V y along j, the inverse DFT into psi.
+// The result is the exact solution of the reference's discrete equations to
+// rounding (the reference iterates multigrid V-cycles to an absolute
+// residual of 1e-5, rot3dmg.f:1548).
+
+#include "vv3_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(fast)
+
+namespace vv3 {
+
+namespace {
+
+constexpr int GT = 64;    // GEMM output tile (rows and columns)
+constexpr int GK = 16;    // GEMM k step
+constexpr int GW = 256;   // GEMM workgroup
+
+// C[q] = A * B[q] with A a dense row-major table (M x K) and B, C reached
+// through the functor: op.b(q, k, n) loads, op.c(q, m, n, v) stores.  Each
+// thread accumulates a 4 x 4 block.  BKFAST: B tiles are loaded with k
+// fastest (B contiguous along k); MFAST: a thread's outputs run along m so a
+// store instruction writes consecutive m (C contiguous along m).
+template <class Op, bool BKFAST, bool MFAST>
+__global__ void __launch_bounds__(GW) k3_gemm(Op op)
+{
+    __shared__ double As[GK][GT];
+    __shared__ double Bs[GK][GT];
+    const int q = blockIdx.z;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int tid = threadIdx.x;
+    const int tm = MFAST ? (tid % 16) : (tid / 16);
+    const int tn = MFAST ? (tid / 16) : (tid % 16);
+    double acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+    for (int k0 = 0; k0 < op.K; k0 += GK) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int e = tid + GW * t;
+            const int kk = e % GK, mm = e / GK;
+            const int m = m0 + mm, k = k0 + kk;
+            As[kk][mm] = (m < op.M && k < op.K) ? op.A[(size_t)m * op.K + k] : 0.0;
+            const int kb = BKFAST ? (e % GK) : (e / GT);
+            const int nb = BKFAST ? (e / GK) : (e % GT);
+            const int n = n0 + nb, k2 = k0 + kb;
+            Bs[kb][nb] = (n < op.N && k2 < op.K) ? op.b(q, k2, n) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = MFAST ? As[kk][tm + 16 * r] : As[kk][tm * 4 + r];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = MFAST ? Bs[kk][tn * 4 + c] : Bs[kk][tn + 16 * c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int m = m0 + (MFAST ? tm + 16 * r : tm * 4 + r);
+            const int n = n0 + (MFAST ? tn * 4 + c : tn + 16 * c);
+            if (m < op.M && n < op.N) op.c(q, m, n, acc[r][c]);
+        }
+}
+
+// 1. spectra S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
+struct PhiFwd {
+    const double *A;
+    int M, N, K;
+    const double *rhs;
+    double *S;
+    int i1, ldi, ldj;
+    __device__ double b(int, int k, int n) const
+    {
+        const int j = n / i1, i = n - j * i1;
+        return rhs[((size_t)(k + 1) * ldj + (size_t)(j + 1)) * ldi + (size_t)(i + 1)];
+    }
+    __device__ void c(int, int m, int n, double v) const { S[(size_t)m * N + n] = v; }
+};
+
+// 2. T[q][i][l] = sum_j Vinv[l][j] S[q][j][i]
+struct ZFwd {
+    const double *A;
+    int M, N, K;
+    const double *S;
+    double *T;
+    int mt;
+    __device__ double b(int q, int k, int n) const { return S[(size_t)q * mt + (size_t)k * N + n]; }
+    __device__ void c(int q, int m, int n, double v) const { T[(size_t)q * mt + (size_t)n * M + m] = v; }
+};
+
+// 4. S[q][j][i] = sum_l V[j][l] T[q][i][l]
+struct ZInv {
+    const double *A;
+    int M, N, K;
+    const double *T;
+    double *S;
+    int mt;
+    __device__ double b(int q, int k, int n) const { return T[(size_t)q * mt + (size_t)n * K + k]; }
+    __device__ void c(int q, int m, int n, double v) const { S[(size_t)q * mt + (size_t)m * N + n] = v; }
+};
+
+// 5. psi(i, j, k+1) = sum_q Wi[k][q] S[q][u]; k = np also fills k = 0
+struct PhiInv {
+    const double *A;
+    int M, N, K;
+    const double *S;
+    double *psi;
+    int i1, ldi, ldj;
+    __device__ double b(int, int k, int n) const { return S[(size_t)k * N + n]; }
+    __device__ void c(int, int m, int n, double v) const
+    {
+        const int j = n / i1, i = n - j * i1;
+        const size_t off = (size_t)(j + 1) * ldi + (size_t)(i + 1);
+        psi[(size_t)(m + 1) * ldj * ldi + off] = v;
+        if (m + 1 == M) psi[off] = v;
+    }
+};
+
+template <bool BKFAST, bool MFAST, class Op>
+void gemm(const Op &op, int batch, hipStream_t s)
+{
+    dim3 grid((op.N + GT - 1) / GT, (op.M + GT - 1) / GT, batch);
+    hipLaunchKernelGGL((k3_gemm<Op, BKFAST, MFAST>), grid, dim3(GW), 0, s, op);
+}
+
+// psip <- psi on the range the reference's routine writes, and psi <- 0 on
+// that range's wall points (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)
+__global__ void __launch_bounds__(256) k3_psi_pre(Geo g, int which, int i1, int j1, double *psi, double *psip)
+{
+    const int nr = g.nr, nz = g.nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)(nr + 2) * (nz + 2) * (g.np + 1)) return;
+    const int i = (int)(t % (nr + 2));
+    const int j = (int)((t / (nr + 2)) % (nz + 2));
+    const int ri = (which == 3) ? nr : nr + 1;
+    const int rj = (which == 1) ? nz : nz + 1;
+    if (i > ri || j > rj) return;
+    psip[t] = psi[t];
+    if (!(i >= 1 && i <= i1 && j >= 1 && j <= j1)) psi[t] = 0.0;
+}
+
+// 3. per (mode, l): (R + lphi(mode) diag(1/(dp r)^2) + lz(l)) x = y for the
+// cos and sin parts, x overwriting y in T (layout [q][i][l]).  Row 1 has an
+// extra coefficient e1 at i = 3, removed with row 2; row i1 has en at
+// i1 - 2, removed with the eliminated row i1 - 2.  Thomas factors go to cp.
+__global__ void __launch_bounds__(256) k3_rsolve(double *T, double *cp, const double *rt, const double *lz,
+                                                  const double *lphi, int i1, int j1, int nm, double e1, double en)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm * j1) return;
+    const int mode = t / j1, l = t - mode * j1;
+    const size_t mt = (size_t)i1 * j1;
+    double *xc = T + (size_t)(2 * mode) * mt + l;
+    double *xs = xc + mt;
+    double *c = cp + (size_t)mode * mt + l;
+    const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1, *ir2 = rt + 3 * i1;
+    const double mu = lphi[mode], lam = lz[l];
+    auto diag = [&](int i) { return dia[i] + mu * ir2[i] + lam; };
+    double d0 = diag(0), s0 = sup[0], yc = xc[0], ys = xs[0];
+    if (e1 != 0.0) {
+        const double f = e1 / sup[1];
+        d0 -= f * sub[1];
+        s0 -= f * diag(1);
+        yc -= f * xc[j1];
+        ys -= f * xs[j1];
+    }
+    double cprev = s0 / d0;
+    yc /= d0;
+    ys /= d0;
+    c[0] = cprev;
+    xc[0] = yc;
+    xs[0] = ys;
+    for (int i = 1; i < i1; ++i) {
+        const size_t o = (size_t)i * j1;
+        double a = sub[i], rc = xc[o], rs = xs[o];
+        if (i == i1 - 1 && en != 0.0) {
+            const size_t o2 = (size_t)(i - 2) * j1;
+            a -= en * c[o2];
+            rc -= en * xc[o2];
+            rs -= en * xs[o2];
+        }
+        const double den = diag(i) - a * cprev;
+        cprev = sup[i] / den;
+        yc = (rc - a * yc) / den;
+        ys = (rs - a * ys) / den;
+        c[o] = cprev;
+        xc[o] = yc;
+        xs[o] = ys;
+    }
+    for (int i = i1 - 2; i >= 0; --i) {
+        const size_t o = (size_t)i * j1;
+        const double f = c[o];
+        yc = xc[o] - f * yc;
+        ys = xs[o] - f * ys;
+        xc[o] = yc;
+        xs[o] = ys;
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+// LU with partial pivoting of (M - x I) for a pentadiagonal M, in band
+// storage: row r keeps columns r-2 .. r+4 (fill from the row swaps).
+struct BandLU {
+    int n = 0;
+    std::vector<double> b;
+    std::vector<int> piv;
+    double &at(int r, int c) { return b[(size_t)r * 7 + (c - r + 2)]; }
+    // returns the sign of det(M - x I) (0 if a pivot vanishes)
+    int factor(const std::vector<double> &M, int nn, double x)
+    {
+        n = nn;
+        b.assign((size_t)n * 7, 0.0);
+        piv.assign(n, 0);
+        for (int r = 0; r < n; ++r)
+            for (int c = std::max(0, r - 2); c <= std::min(n - 1, r + 2); ++c)
+                at(r, c) = M[(size_t)r * n + c] - (r == c ? x : 0.0);
+        int sign = 1;
+        for (int c = 0; c < n; ++c) {
+            int p = c;
+            for (int r = c + 1; r <= std::min(n - 1, c + 2); ++r)
+                if (std::fabs(at(r, c)) > std::fabs(at(p, c))) p = r;
+            piv[c] = p;
+            if (at(p, c) == 0.0) return 0;
+            if (p != c) {
+                for (int q = c; q <= std::min(n - 1, c + 4); ++q) std::swap(at(c, q), at(p, q));
+                sign = -sign;
+            }
+            if (at(c, c) < 0.0) sign = -sign;
+            for (int r = c + 1; r <= std::min(n - 1, c + 2); ++r) {
+                const double f = at(r, c) / at(c, c);
+                at(r, c) = f;
+                for (int q = c + 1; q <= std::min(n - 1, c + 4); ++q) at(r, q) -= f * at(c, q);
+            }
+        }
+        return sign;
+    }
+    void solve(std::vector<double> &y)
+    {
+        for (int c = 0; c < n; ++c) {
+            if (piv[c] != c) std::swap(y[c], y[piv[c]]);
+            for (int r = c + 1; r <= std::min(n - 1, c + 2); ++r) y[r] -= at(r, c) * y[c];
+        }
+        for (int c = n - 1; c >= 0; --c) {
+            double s = y[c];
+            for (int q = c + 1; q <= std::min(n - 1, c + 4); ++q) s -= at(c, q) * y[q];
+            y[c] = s / at(c, c);
+        }
+    }
+};
+
+// Gauss-Jordan inverse with partial pivoting (row-major, n x n)
+bool dense_inverse(std::vector<double> a, int n, std::vector<double> &inv)
+{
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)p * n + c])) p = r;
+        if (a[(size_t)p * n + c] == 0.0) return false;
+        if (p != c)
+            for (int q = 0; q < n; ++q) {
+                std::swap(a[(size_t)c * n + q], a[(size_t)p * n + q]);
+                std::swap(inv[(size_t)c * n + q], inv[(size_t)p * n + q]);
+            }
+        const double d = 1.0 / a[(size_t)c * n + c];
+        for (int q = 0; q < n; ++q) {
+            a[(size_t)c * n + q] *= d;
+            inv[(size_t)c * n + q] *= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int q = 0; q < n; ++q) {
+                a[(size_t)r * n + q] -= f * a[(size_t)c * n + q];
+                inv[(size_t)r * n + q] -= f * inv[(size_t)c * n + q];
+            }
+        }
+    }
+    return true;
+}
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src)
+{
+    hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+void r_rows(int which, int nr, double r0, double dr, std::vector<double> &sub, std::vector<double> &diag,
+            std::vector<double> &sup, double &e1, double &en)
+{
+    const int i1 = (which == 3) ? nr - 1 : nr;
+    const double hhr = dr * dr;
+    sub.assign(i1, 0.0);
+    diag.assign(i1, 0.0);
+    sup.assign(i1, 0.0);
+    e1 = en = 0.0;
+    for (int i = 1; i <= i1; ++i) {
+        double a = 0, d = 0, c = 0;
+        if (which == 1) {   // rot3dmg.f:1698-1712
+            const double rv = r0 + ((double)i - 0.5) * dr;
+            if (i == 1) { d = -2.0 / hhr - 0.5 / dr / rv; c = 5.0 / 3.0 / hhr + 0.5 / dr / rv; e1 = -0.2 / hhr; }
+            else if (i == nr) { a = 5.0 / 3.0 / hhr - 0.5 / dr / rv; d = -2.0 / hhr + 0.5 / dr / rv; en = -0.2 / hhr; }
+            else { a = 1.0 / hhr - 0.5 / dr / rv; d = -2.0 / hhr; c = 1.0 / hhr + 0.5 / dr / rv; }
+        } else if (which == 2) {   // rot3dmg.f:1935-1950
+            const double rv = r0 + ((double)i - 0.5) * dr;
+            const double rr = 1.0 / (rv * rv);
+            if (i == 1) { d = -2.0 / hhr + rr; c = 5.0 / 3.0 / hhr + 4.0 / 3.0 / dr / rv; e1 = -0.2 / hhr; }
+            else if (i == nr) { a = 5.0 / 3.0 / hhr - 4.0 / 3.0 / dr / rv; d = -2.0 / hhr + rr; en = -0.2 / hhr; }
+            else { a = 1.0 / hhr - 1.5 / dr / rv; d = -2.0 / hhr + rr; c = 1.0 / hhr + 1.5 / dr / rv; }
+        } else {   // rot3dmg.f:2133-2136
+            const double rv = (double)i * dr + r0;
+            a = 1.0 / hhr - 1.5 / dr / rv;
+            d = 1.0 / (rv * rv) - 2.0 / hhr;
+            c = 1.0 / hhr + 1.5 / dr / rv;
+        }
+        sub[i - 1] = (i == 1) ? 0.0 : a;
+        diag[i - 1] = d;
+        sup[i - 1] = (i == i1) ? 0.0 : c;
+    }
+}
+
+std::vector<double> z_matrix(int which, int j1)
+{
+    std::vector<double> M((size_t)j1 * j1, 0.0);
+    for (int j = 0; j < j1; ++j) {
+        M[(size_t)j * j1 + j] = -2.0;
+        if (which != 1 && j == 0) {   // rot3dmg.f:1951-1952, 2143-2144
+            M[1] = 5.0 / 3.0;
+            M[2] = -0.2;
+        } else if (which != 1 && j == j1 - 1) {
+            M[(size_t)j * j1 + j - 1] = 5.0 / 3.0;
+            M[(size_t)j * j1 + j - 2] = -0.2;
+        } else {
+            if (j > 0) M[(size_t)j * j1 + j - 1] = 1.0;
+            if (j < j1 - 1) M[(size_t)j * j1 + j + 1] = 1.0;
+        }
+    }
+    return M;
+}
+
+bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam, std::vector<double> &V,
+                  std::vector<double> &Vinv)
+{
+    // Gershgorin interval [lo, hi]
+    double lo = 0.0, hi = 0.0;
+    for (int r = 0; r < n; ++r) {
+        double rad = 0.0;
+        for (int c = 0; c < n; ++c)
+            if (c != r) rad += std::fabs(M[(size_t)r * n + c]);
+        const double d = M[(size_t)r * n + r];
+        lo = (r == 0) ? d - rad : std::min(lo, d - rad);
+        hi = (r == 0) ? d + rad : std::max(hi, d + rad);
+    }
+    const double pad = 1e-3 * (hi - lo) + 1e-300;
+    lo -= pad;
+    hi += pad;
+    const double R = std::max(std::fabs(lo), std::fabs(hi));
+    const double cen = 0.5 * (lo + hi), half = 0.5 * (hi - lo);
+    // the sign of det(M - x I) changes exactly at each (real, simple)
+    // eigenvalue: scan a Chebyshev-spaced grid over the Gershgorin interval
+    // (denser at its ends, where the eigenvalues of these second-difference
+    // matrices crowd), refining until n changes are seen, then bisect each
+    // bracket
+    BandLU lu;
+    const double pi = std::acos(-1.0);
+    lam.clear();
+    for (int G = 8 * n + 64; G <= (1 << 22); G *= 4) {
+        lam.clear();
+        std::vector<double> xs(G + 1);
+        std::vector<int> sg(G + 1);
+        for (int g = 0; g <= G; ++g) {
+            xs[g] = cen - half * std::cos(pi * g / G);
+            sg[g] = lu.factor(M, n, xs[g]);
+        }
+        int prev = sg[0], gprev = 0;
+        for (int g = 1; g <= G; ++g) {
+            if (sg[g] == 0) continue;
+            if (prev != 0 && sg[g] != prev) {
+                double a = xs[gprev], b = xs[g];
+                const int sa = prev;
+                for (int it = 0; it < 200 && b - a > 4e-16 * std::max(std::fabs(a), std::fabs(b)); ++it) {
+                    const double mid = 0.5 * (a + b);
+                    const int sm = lu.factor(M, n, mid);
+                    if (sm == 0) { a = b = mid; break; }
+                    if (sm == sa) a = mid; else b = mid;
+                }
+                lam.push_back(0.5 * (a + b));
+            }
+            prev = sg[g];
+            gprev = g;
+        }
+        if ((int)lam.size() == n) break;
+    }
+    if ((int)lam.size() != n) return false;
+    // eigenvectors by inverse iteration at a slightly shifted eigenvalue
+    V.assign((size_t)n * n, 0.0);
+    std::vector<double> y(n);
+    for (int l = 0; l < n; ++l) {
+        const double x = lam[l] + 1e-11 * R;
+        if (lu.factor(M, n, x) == 0) return false;
+        for (int i = 0; i < n; ++i) y[i] = 1.0 + 0.01 * std::sin(1.0 + i);
+        for (int it = 0; it < 3; ++it) {
+            lu.solve(y);
+            double nrm = 0.0;
+            for (double v : y) nrm += v * v;
+            nrm = std::sqrt(nrm);
+            if (!(nrm > 0.0)) return false;
+            for (double &v : y) v /= nrm;
+        }
+        for (int i = 0; i < n; ++i) V[(size_t)i * n + l] = y[i];
+    }
+    if (!dense_inverse(V, n, Vinv)) return false;
+    double e1 = 0.0, e2 = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int l = 0; l < n; ++l) {
+            double mv = 0.0, iv = 0.0;
+            for (int q = std::max(0, i - 2); q <= std::min(n - 1, i + 2); ++q) mv += M[(size_t)i * n + q] * V[(size_t)q * n + l];
+            for (int q = 0; q < n; ++q) iv += Vinv[(size_t)i * n + q] * V[(size_t)q * n + l];
+            e1 = std::max(e1, std::fabs(mv - lam[l] * V[(size_t)i * n + l]));
+            e2 = std::max(e2, std::fabs(iv - (i == l ? 1.0 : 0.0)));
+        }
+    return e1 <= 1e-11 * R && e2 <= 1e-10;
+}
+
+int poisson3_setup(Poisson3 &P, const Geo &g)
+{
+    using vv::set_error;
+    P.nr = g.nr; P.nz = g.nz; P.np = g.np; P.ldi = g.ldi; P.ldj = g.ldj;
+    const int np = g.np, nm = np / 2 + 1;
+    P.nm = nm;
+    const double pi = std::acos(-1.0);
+    // phi tables: the angle of mode m at k is 2 pi ((m k) mod np) / np
+    std::vector<double> W((size_t)2 * nm * np), Wi((size_t)np * 2 * nm), lphi(nm);
+    for (int m = 0; m < nm; ++m) {
+        const double wgt = (m == 0 || 2 * m == np) ? 1.0 : 2.0;
+        lphi[m] = 2.0 * std::cos(2.0 * pi * m / np) - 2.0;
+        for (int k = 1; k <= np; ++k) {
+            const double ang = 2.0 * pi * (double)(((long long)m * k) % np) / np;
+            const double cs = std::cos(ang), sn = std::sin(ang);
+            W[(size_t)(2 * m) * np + (k - 1)] = cs;
+            W[(size_t)(2 * m + 1) * np + (k - 1)] = sn;
+            Wi[(size_t)(k - 1) * 2 * nm + 2 * m] = wgt * cs / np;
+            Wi[(size_t)(k - 1) * 2 * nm + 2 * m + 1] = wgt * sn / np;
+        }
+    }
+    VV_HIP(upload(&P.W, W));
+    VV_HIP(upload(&P.Wi, Wi));
+    VV_HIP(upload(&P.lphi, lphi));
+    const double hhz = g.dz * g.dz, hhp = g.dp * g.dp;
+    size_t mmax = 0;
+    for (int w = 1; w <= 3; ++w) {
+        PoissonOp &O = P.op[w - 1];
+        O.which = w;
+        O.i1 = (w == 3) ? g.nr - 1 : g.nr;
+        O.j1 = (w == 1) ? g.nz - 1 : g.nz;
+        O.m = O.i1 * O.j1;
+        mmax = std::max(mmax, (size_t)O.m);
+        std::vector<double> sub, dia, sup;
+        r_rows(w, g.nr, g.r0, g.dr, sub, dia, sup, O.e1, O.en);
+        std::vector<double> rt((size_t)4 * O.i1);
+        for (int i = 0; i < O.i1; ++i) {
+            const double rv = (w == 3) ? (double)(i + 1) * g.dr + g.r0 : g.r0 + ((double)(i + 1) - 0.5) * g.dr;
+            rt[i] = sub[i];
+            rt[O.i1 + i] = dia[i];
+            rt[2 * O.i1 + i] = sup[i];
+            rt[3 * O.i1 + i] = 1.0 / hhp / (rv * rv);
+        }
+        std::vector<double> lam, V, Vinv;
+        if (!eigen_banded(z_matrix(w, O.j1), O.j1, lam, V, Vinv)) {
+            set_error("vv3_create: z eigen-decomposition of a Poisson operator failed");
+            return VV_EINVAL;
+        }
+        for (double &v : lam) v /= hhz;
+        VV_HIP(upload(&O.V, V));
+        VV_HIP(upload(&O.Vinv, Vinv));
+        VV_HIP(upload(&O.rt, rt));
+        VV_HIP(upload(&O.lz, lam));
+    }
+    VV_HIP(hipMalloc(&P.S, (size_t)2 * nm * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&P.T, (size_t)2 * nm * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&P.cp, (size_t)nm * mmax * sizeof(double)));
+    return VV_OK;
+}
+
+void poisson3_free(Poisson3 &P)
+{
+    for (double *p : {P.W, P.Wi, P.lphi, P.S, P.T, P.cp})
+        if (p) (void)hipFree(p);
+    P.W = P.Wi = P.lphi = P.S = P.T = P.cp = nullptr;
+    for (auto &O : P.op) {
+        for (double *p : {O.V, O.Vinv, O.rt, O.lz})
+            if (p) (void)hipFree(p);
+        O.V = O.Vinv = O.rt = O.lz = nullptr;
+    }
+}
+
+int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, double *psip, const double *rhs,
+                    hipStream_t s)
+{
+    const PoissonOp &O = P.op[which - 1];
+    const int nm = P.nm, np = P.np, i1 = O.i1, j1 = O.j1, mt = O.m;
+    const long long nf = (long long)g.ldi * g.ldj * (np + 1);
+    hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, g, which, i1, j1, psi, psip);
+    gemm<false, false>(PhiFwd{P.W, 2 * nm, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
+    gemm<false, true>(ZFwd{O.Vinv, j1, i1, j1, P.S, P.T, mt}, 2 * nm, s);
+    hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 255) / 256)), dim3(256), 0, s, P.T, P.cp,
+                       (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, nm, O.e1, O.en);
+    gemm<true, false>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, 2 * nm, s);
+    gemm<false, false>(PhiInv{P.Wi, np, mt, 2 * nm, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+}  // namespace vv3

@@ -32,12 +32,11 @@ enum vv3_field {
     VV3_WELP, VV3_URH, VV3_VRH, VV3_WRH, VV3_NFIELD
 };
 
-/* Allocates the fields and the Poisson solver on `device`.  The three
- * Poisson operators are solved exactly: a real DFT in phi decouples each
- * into one (r,z) system per Fourier mode, whose inverse is formed here (the
- * reference builds its multigrid levels in poissoninit, rot3dmg.f:1153).
- * This direct solver holds (np/2+1) * m^2 doubles per operator, m = unknowns
- * of an (r,z) plane, and is limited to m <= 1024 (VV_EINVAL beyond). */
+/* Allocates the fields and the Poisson solver on `device` (the reference
+ * builds its multigrid levels in poissoninit, rot3dmg.f:1153).  The three
+ * Poisson operators are solved exactly and separably: a real DFT in phi, the
+ * eigenbasis of the z rows, a banded solve in r per (mode, z eigenvalue).
+ * Needs nr, nz >= 4 and an even np >= 4 (VV_EINVAL otherwise). */
 int vv3_create(const vv3_config *cfg, int device, vv3_run **out);
 int vv3_destroy(vv3_run *run);
 

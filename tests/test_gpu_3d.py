@@ -172,6 +172,49 @@ def test_poisson_matches_exact_solve(which, shape):
     run.close()
 
 
+@pytest.mark.parametrize("which", [1, 2, 3])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10)])
+def test_poisson_residual_at_larger_sizes(which, shape):
+    """Grids beyond the oracle's dense solve: the solution satisfies the
+    reference's residual operator (rot3dmg.f:1695-1723, 1927-1965,
+    2130-2165) to rounding, relative to the operator's scale."""
+    nr, nz, np_ = shape
+    run = _gpu(nr, nz, np_)
+    ora = O3.Run3(nr, nz, np_, **PARAMS)
+    rng = np.random.default_rng(200 + which)
+    psi, psip, rhs = PSI[which]
+    f = rng.standard_normal(ora.shape)
+    f[0] = f[-1]
+    run.put(rhs, f)
+    getattr(run, f"poisson{which}")()
+    got = run.get(psi)
+    m, _ = ora.residual(which, got, f)
+    h = min(ora.p.dr, ora.p.dz, ora.p.r0 * ora.p.dp)
+    scale = np.max(np.abs(f)) + np.max(np.abs(got)) * 8.0 / h ** 2
+    assert m < 1e-13 * scale, (m, scale)
+    assert np.array_equal(got[0], got[np_])
+    run.close()
+
+
+def test_solver_is_linear():
+    """poisson2(a f + g) = a poisson2(f) + poisson2(g) to rounding, on a
+    reference-sized grid (kr = kz = 33, kp = 160: nr = nz = 32, np = 160)."""
+    nr, nz, np_ = 32, 32, 160
+    run = _gpu(nr, nz, np_)
+    rng = np.random.default_rng(3)
+    shape = (np_ + 1, nz + 2, nr + 2)
+    f, g = rng.standard_normal(shape), rng.standard_normal(shape)
+    f[0], g[0] = f[-1], g[-1]
+    out = []
+    for rhs in (f, g, 2.5 * f + g):
+        run.put("vrh", rhs)
+        run.poisson2()
+        out.append(run.get("vel"))
+    lin = 2.5 * out[0] + out[1]
+    assert np.max(np.abs(out[2] - lin)) <= 1e-12 * np.max(np.abs(lin))
+    run.close()
+
+
 def _smooth_state(nr, nz, np_, seed):
     """A perturbed conduction state: th linear in r with walls 0 / 1, small
     velocities and vorticities."""
