@@ -22,7 +22,7 @@ struct vv3_run {
     size_t nf = 0;                       // doubles per field
     double *f[VV3_NFIELD] = {nullptr};   // device fields (rotated by pointer swap)
     vv3::Poisson3 pois;                  // the three Poisson operators
-    double *heat = nullptr;              // per-k partial fluxes (np+1) and the total
+    double *heat = nullptr;              // flux terms (nz+2) x np, per-k sums (np), the total
 };
 
 namespace vv3 {
@@ -31,23 +31,74 @@ namespace {
 
 constexpr int WG = 256;
 
-#define X(a, i, j, k) (a)[((size_t)(k) * g.ldj + (size_t)(j)) * g.ldi + (size_t)(i)]
+// field element (i, j, k); fields stay below 2^32 elements, so offsets are
+// 32-bit (uniform base + per-lane offset addressing)
+#define X(a, i, j, k) (a)[(unsigned)(((k) * g.ldj + (j)) * g.ldi + (i))]
 
 struct Fields {
     const double *A, *B, *C, *Ap, *Bp, *Cp, *th, *thp, *uel, *vel, *wel;
     double *An, *Bn, *Cn, *thn;
 };
 
+// A positive divisor with its correctly rounded reciprocal.  x / b is
+// evaluated as q0 = x rb, t = fma(b, q0, -x) = b q0 - x (exact),
+// q = fma(-t, rb, q0): for rb = RN(1/b) this is the correctly rounded
+// quotient (Markstein's correction), so the results equal those of IEEE
+// division.  Forming -t rather than x - b q0 gives a zero quotient the sign
+// IEEE division gives it when b > 0 (t = +0, -t = -0 for x = +-0).
+struct Rc {
+    double v, r;
+};
+__device__ __forceinline__ double dv(double x, Rc b)
+{
+    const double q0 = x * b.r;
+    const double t = __builtin_fma(b.v, q0, -x);
+    return __builtin_fma(-t, b.r, q0);
+}
+// the same for a divisor of either sign (the zero case kept apart)
+__device__ __forceinline__ double dvs(double x, Rc b)
+{
+    const double q0 = x * b.r;
+    const double e = __builtin_fma(-b.v, q0, x);
+    return (e == 0.0) ? q0 : __builtin_fma(e, b.r, q0);
+}
+
+// constants of the explicit routines, each evaluated on the host exactly as
+// the reference's expression writes it
+struct Kc {
+    Rc dr, dz, dp, dr2, dz2, dp2, ptl, ptl2, ptldt, three, dt;   // positive
+    Rc re;                                                       // either sign
+    double cnr_int, cnz_int;   // -1/(dr dr), -1/(dz dz)
+    double cnr_w, cnz_w;       // -2.5/(dr dr), -2.5/(dz dz)
+    double cnz_t;              // -4/(dz dz)
+    double hdr;                // 0.5/dr
+    double m1dp2;              // -1/(dp dp)
+    double ptldt_inv, dt_inv;  // 1/(Ptl dt), 1/dt
+    double grptl;              // Gr Ptl
+};
+
+// per-point radius with 1/r and 1/r^2
+struct Rad {
+    Rc r, r2, r2h;   // r, r*r, 2*(r*r)
+    __device__ explicit Rad(double rv)
+    {
+        const double q = rv * rv, iq = 1.0 / q;
+        r = Rc{rv, 1.0 / rv};
+        r2 = Rc{q, iq};
+        r2h = Rc{2.0 * q, 0.5 * iq};
+    }
+};
+
 // Avorticity at (i, j, k), rot3dmg.f:359-465
-__device__ double avort(const Geo &g, const Fields &F, int i, int j, int k)
+__device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int nr = g.nr, np = g.np;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Ra = g.Ra, Re = g.Re, Gr = g.Gr;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
     const int im = i - 1, jp = j + 1;
     const int km = (k == 0) ? np - 1 : k - 1;
     const int ka = (k == np) ? 1 : k + 1;
-    const double r = g.r0 + ((double)i - 0.5) * dr;
+    const Rad R(g.r0 + ((double)i - 0.5) * dr);
     const double u = (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, k) +
                       X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.125;
     const double v = (X(vel, i, jp, k) + X(vel, i, j, k)) * 0.5;
@@ -55,68 +106,73 @@ __device__ double avort(const Geo &g, const Fields &F, int i, int j, int k)
     const double tem = (X(th, i, j, k) + X(th, i, j, ka) + X(th, i, jp, k) + X(th, i, jp, ka)) * 0.25;
     const double BB = (X(B, i, j, k) + X(B, im, j, k) + X(B, i, j, ka) + X(B, im, j, ka)) * 0.25;
     const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, im, j, k) + X(C, im, jp, k)) * 0.25;
+    const double Ac = X(A, i, j, k);
     double Arra, Ara, Ar, cnr;
     if (i == 1) {
-        Arra = (3.2 * X(A, i - 1, j, k) + 2.0 * X(A, i + 1, j, k) - 0.2 * X(A, i + 2, j, k)) / (dr * dr);
-        Ara = (-4.0 * X(A, i - 1, j, k) / 3.0 + X(A, i + 1, j, k) / 3.0) / dr;
-        Ar = (-4.0 * X(A, i - 1, j, k) / 3.0 + X(A, i, j, k) + X(A, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) + 0.5 / dr / r;
+        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
+        Arra = dv(3.2 * Am + 2.0 * A1 - 0.2 * X(A, i + 2, j, k), K.dr2);
+        Ara = dv(dv(-4.0 * Am, K.three) + dv(A1, K.three), K.dr);
+        Ar = dv(dv(-4.0 * Am, K.three) + Ac + dv(A1, K.three), K.dr);
+        cnr = K.cnr_w + dv(K.hdr, R.r);
     } else if (i == nr) {
-        Arra = (3.2 * X(A, i + 1, j, k) + 2.0 * X(A, i - 1, j, k) - 0.2 * X(A, i - 2, j, k)) / (dr * dr);
-        Ara = (-X(A, i - 1, j, k) / 3.0 + 4.0 * X(A, i + 1, j, k) / 3.0) / dr;
-        Ar = (-4.0 * X(A, i - 1, j, k) / 3.0 + X(A, i, j, k) + X(A, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) - 0.5 / dr / r;
+        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
+        Arra = dv(3.2 * A1 + 2.0 * Am - 0.2 * X(A, i - 2, j, k), K.dr2);
+        Ara = dv(dv(-Am, K.three) + dv(4.0 * A1, K.three), K.dr);
+        Ar = dv(dv(-4.0 * Am, K.three) + Ac + dv(A1, K.three), K.dr);
+        cnr = K.cnr_w - dv(K.hdr, R.r);
     } else {
-        Arra = (X(A, i - 1, j, k) + X(A, i + 1, j, k)) / (dr * dr);
-        Ara = (X(A, i + 1, j, k) - X(A, i - 1, j, k)) * 0.5 / dr;
+        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
+        Arra = dv(Am + A1, K.dr2);
+        Ara = dv((A1 - Am) * 0.5, K.dr);
         Ar = Ara;
-        cnr = -1.0 / (dr * dr);
-        if (fabs(u * dr / Ptl) > 2) {          // hybrid upwind, rot3dmg.f:397-406
-            if (u > 0) Ar = (X(A, i, j, k) - X(A, i - 1, j, k)) / dr;
-            else if (u < 0) Ar = (X(A, i + 1, j, k) - X(A, i, j, k)) / dr;
+        cnr = K.cnr_int;
+        if (fabs(dv(u * dr, K.ptl)) > 2) {          // hybrid upwind, rot3dmg.f:397-406
+            if (u > 0) Ar = dv(Ac - Am, K.dr);
+            else if (u < 0) Ar = dv(A1 - Ac, K.dr);
         }
     }
-    const double uelr = ((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
-                         (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25) / dr;
-    const double Azza = (X(A, i, j - 1, k) + X(A, i, j + 1, k)) / (dz * dz);
-    double Az = (X(A, i, j + 1, k) - X(A, i, j - 1, k)) * 0.5 / dz;
-    if (fabs(w * dz / Ptl) > 2) {
-        if (w > 0) Az = (X(A, i, j, k) - X(A, i, j - 1, k)) / dz;
-        else if (w < 0) Az = (X(A, i, j + 1, k) - X(A, i, j, k)) / dz;
+    const double uelr = dv((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
+                           (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25, K.dr);
+    const double Azza = dv(X(A, i, j - 1, k) + X(A, i, j + 1, k), K.dz2);
+    double Az = dv((X(A, i, j + 1, k) - X(A, i, j - 1, k)) * 0.5, K.dz);
+    if (fabs(dv(w * dz, K.ptl)) > 2) {
+        if (w > 0) Az = dv(Ac - X(A, i, j - 1, k), K.dz);
+        else if (w < 0) Az = dv(X(A, i, j + 1, k) - Ac, K.dz);
     }
-    const double uelz = ((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
-                         (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25) / dz;
-    const double thz = (X(th, i, jp, k) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / dz;
-    const double thh = (X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5 / dp;
-    const double Ahha = (X(A, i, j, ka) + X(A, i, j, km)) / (dp * dp);
-    double Ah = (X(A, i, j, ka) - X(A, i, j, km)) * 0.5 / dp;
-    if (fabs(v * dp * r / Ptl) > 2) {
-        if (v > 0) Ah = (X(A, i, j, k) - X(A, i, j, km)) / dp;
-        else if (v < 0) Ah = (X(A, i, j, ka) - X(A, i, j, k)) / dp;
+    const double uelz = dv((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
+                           (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25, K.dz);
+    const double thz = dv((X(th, i, jp, k) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5, K.dz);
+    const double thh = dv((X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5, K.dp);
+    const double Ahha = dv(X(A, i, j, ka) + X(A, i, j, km), K.dp2);
+    double Ah = dv((X(A, i, j, ka) - X(A, i, j, km)) * 0.5, K.dp);
+    if (fabs(dv(v * dp * R.r.v, K.ptl)) > 2) {
+        if (v > 0) Ah = dv(Ac - X(A, i, j, km), K.dp);
+        else if (v < 0) Ah = dv(X(A, i, j, ka) - Ac, K.dp);
     }
-    const double Bh = (X(B, i, j, ka) + X(B, im, j, ka) - X(B, i, j, k) - X(B, im, j, k)) * 0.5 / dp;
-    const double uelh = ((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
-                         (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25) / dp;
-    const double cnz = -1.0 / (dz * dz);
-    const double cnh = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = Arra + Ara / r + Ahha / (r * r) + Azza - 2.0 * Bh / (r * r) + X(A, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Ar + v * Ah / r + w * Az - uelh * BB / r - uelz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * uelz + 2.0 * Ra * (tem * uelz + thz * u) / Re / Ptl - Gr * Ptl / r * thh;
-    const double cf1 = 1.0 / (Ptl * dt) - uelr / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
-    const double cf2 = uelr / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
+    const double Bh = dv((X(B, i, j, ka) + X(B, im, j, ka) - X(B, i, j, k) - X(B, im, j, k)) * 0.5, K.dp);
+    const double uelh = dv((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
+                           (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25, K.dp);
+    const double cnz = K.cnz_int;
+    const double cnh = dv(K.m1dp2, R.r2);
+    const double RHS1 = Arra + dv(Ara, R.r) + dv(Ahha, R.r2) + Azza - dv(2.0 * Bh, R.r2) + dv(Ac, K.ptldt);
+    const double ULT = dv(u * Ar + dv(v * Ah, R.r) + w * Az - dv(uelh * BB, R.r) - uelz * CC, K.ptl);
+    const double RHS2 = -2.0 * Re * uelz + dv(dvs(2.0 * Ra * (tem * uelz + thz * u), K.re), K.ptl) - dv(K.grptl, R.r) * thh;
+    const double cf1 = K.ptldt_inv - dv(uelr, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
+    const double cf2 = dv(uelr, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
     return (cf2 * X(F.Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // Bvorticity at (i, j, k), rot3dmg.f:495-590
-__device__ double bvort(const Geo &g, const Fields &F, int i, int j, int k)
+__device__ double bvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int np = g.np;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Ra = g.Ra, Re = g.Re, Gr = g.Gr;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
     const int ip = i + 1, jp = j + 1;
     const int km = (k == 0) ? np - 1 : k - 1;
     const int ka = (k == np) ? 1 : k + 1;
-    const double r = g.r0 + (double)i * dr;
+    const Rad R(g.r0 + (double)i * dr);
+    const double r = R.r.v;
     const double u = (X(uel, i, j, k) + X(uel, i, jp, k)) * 0.5;
     const double v = (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k) + X(vel, ip, j, km) +
                       X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.125;
@@ -124,55 +180,58 @@ __device__ double bvort(const Geo &g, const Fields &F, int i, int j, int k)
     const double tem = (X(th, i, jp, k) + X(th, ip, jp, k) + X(th, i, j, k) + X(th, ip, j, k)) * 0.25;
     const double AA = (X(A, i, j, k) + X(A, ip, j, k) + X(A, i, j, km) + X(A, ip, j, km)) * 0.25;
     const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, i, j, km) + X(C, i, jp, km)) * 0.25;
-    const double Brra = (X(B, i - 1, j, k) + X(B, i + 1, j, k)) / (dr * dr);
-    const double Bra = (X(B, i + 1, j, k) - X(B, i - 1, j, k)) * 0.5 / dr;
+    const double Bc = X(B, i, j, k);
+    const double Brra = dv(X(B, i - 1, j, k) + X(B, i + 1, j, k), K.dr2);
+    const double Bra = dv((X(B, i + 1, j, k) - X(B, i - 1, j, k)) * 0.5, K.dr);
     double Br = Bra;
-    if (fabs(u * dr / Ptl) > 2) {
-        if (u > 0) Br = (X(B, i, j, k) - X(B, i - 1, j, k)) / dr;
-        else if (u < 0) Br = (X(B, i + 1, j, k) - X(B, i, j, k)) / dr;
+    if (fabs(dv(u * dr, K.ptl)) > 2) {
+        if (u > 0) Br = dv(Bc - X(B, i - 1, j, k), K.dr);
+        else if (u < 0) Br = dv(X(B, i + 1, j, k) - Bc, K.dr);
     }
-    const double velr = ((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25) / dr;
-    const double Bzza = (X(B, i, j - 1, k) + X(B, i, j + 1, k)) / (dz * dz);
-    double Bz = (X(B, i, j + 1, k) - X(B, i, j - 1, k)) * 0.5 / dz;
-    if (fabs(w * dz / Ptl) > 2) {
-        if (w > 0) Bz = (X(B, i, j, k) - X(B, i, j - 1, k)) / dz;
-        else if (w < 0) Bz = (X(B, i, j + 1, k) - X(B, i, j, k)) / dz;
+    const double velr = dv((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
+                           (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25, K.dr);
+    const double Bzza = dv(X(B, i, j - 1, k) + X(B, i, j + 1, k), K.dz2);
+    double Bz = dv((X(B, i, j + 1, k) - X(B, i, j - 1, k)) * 0.5, K.dz);
+    if (fabs(dv(w * dz, K.ptl)) > 2) {
+        if (w > 0) Bz = dv(Bc - X(B, i, j - 1, k), K.dz);
+        else if (w < 0) Bz = dv(X(B, i, j + 1, k) - Bc, K.dz);
     }
-    const double velz = ((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25) / dz;
-    const double thz = (X(th, i, jp, k) + X(th, ip, jp, k) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / dz;
-    const double thr = (X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5 / dr;
-    const double Bhha = (X(B, i, j, ka) + X(B, i, j, km)) / (dp * dp);
-    double Bh = (X(B, i, j, ka) - X(B, i, j, km)) * 0.5 / dp;
-    if (fabs(v * dp * r / Ptl) > 2) {
-        if (v > 0) Bh = (X(B, i, j, k) - X(B, i, j, km)) / dp;
-        else if (v < 0) Bh = (X(B, i, j, ka) - X(B, i, j, k)) / dp;
+    const double velz = dv((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
+                           (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25, K.dz);
+    const double thz = dv((X(th, i, jp, k) + X(th, ip, jp, k) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5, K.dz);
+    const double thr = dv((X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5, K.dr);
+    const double Bhha = dv(X(B, i, j, ka) + X(B, i, j, km), K.dp2);
+    double Bh = dv((X(B, i, j, ka) - X(B, i, j, km)) * 0.5, K.dp);
+    if (fabs(dv(v * dp * r, K.ptl)) > 2) {
+        if (v > 0) Bh = dv(Bc - X(B, i, j, km), K.dp);
+        else if (v < 0) Bh = dv(X(B, i, j, ka) - Bc, K.dp);
     }
-    const double Ah = (X(A, i, j, k) + X(A, ip, j, k) - X(A, i, j, km) - X(A, ip, j, km)) * 0.5 / dp;
-    const double velh = ((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25) / dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnz = -1.0 / (dz * dz);
-    const double cnh = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = Brra + Bra / r + Bhha / (r * r) + Bzza + 2.0 * Ah / (r * r) + X(B, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Br + v * Bh / r + w * Bz - (velr - v / r) * AA - velz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * velz + r * Ra * thz + 2.0 * Ra * (tem * velz + thz * v) / Re / Ptl + Gr * Ptl * thr;
-    const double cf1 = 1.0 / (Ptl * dt) - (u / r + velh / r) / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
-    const double cf2 = (u / r + velh / r) / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
+    const double Ah = dv((X(A, i, j, k) + X(A, ip, j, k) - X(A, i, j, km) - X(A, ip, j, km)) * 0.5, K.dp);
+    const double velh = dv((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
+                           (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25, K.dp);
+    const double cnr = K.cnr_int;
+    const double cnz = K.cnz_int;
+    const double cnh = dv(K.m1dp2, R.r2);
+    const double RHS1 = Brra + dv(Bra, R.r) + dv(Bhha, R.r2) + Bzza + dv(2.0 * Ah, R.r2) + dv(Bc, K.ptldt);
+    const double ULT = dv(u * Br + dv(v * Bh, R.r) + w * Bz - (velr - dv(v, R.r)) * AA - velz * CC, K.ptl);
+    const double RHS2 = -2.0 * Re * velz + r * Ra * thz + dv(dvs(2.0 * Ra * (tem * velz + thz * v), K.re), K.ptl) + K.grptl * thr;
+    const double uvh = dv(u, R.r) + dv(velh, R.r);
+    const double cf1 = K.ptldt_inv - dv(uvh, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
+    const double cf2 = dv(uvh, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
     return (cf2 * X(F.Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // Cvorticity at (i, j, k), rot3dmg.f:619-725
-__device__ double cvort(const Geo &g, const Fields &F, int i, int j, int k)
+__device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int nz = g.nz, np = g.np;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Ra = g.Ra, Re = g.Re;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
     const int ip = i + 1, jm = j - 1;
     const int km = (k == 0) ? np - 1 : k - 1;
     const int ka = (k == np) ? 1 : k + 1;
-    const double r = g.r0 + (double)i * dr;
+    const Rad R(g.r0 + (double)i * dr);
+    const double r = R.r.v;
     const double u = (X(uel, i, j, ka) + X(uel, i, j, k)) * 0.5;
     const double v = (X(vel, i, j, k) + X(vel, ip, j, k)) * 0.5;
     const double w = (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, k) +
@@ -180,65 +239,70 @@ __device__ double cvort(const Geo &g, const Fields &F, int i, int j, int k)
     const double AA = (X(A, i, j, k) + X(A, i, jm, k) + X(A, ip, j, k) + X(A, ip, jm, k)) * 0.25;
     const double BB = (X(B, i, j, k) + X(B, i, jm, k) + X(B, i, j, ka) + X(B, i, jm, ka)) * 0.25;
     const double tem = (X(th, i, j, k) + X(th, ip, j, k) + X(th, ip, j, ka) + X(th, i, j, ka)) * 0.25;
-    const double Crra = (X(C, i - 1, j, k) + X(C, i + 1, j, k)) / (dr * dr);
-    const double Cra = (X(C, i + 1, j, k) - X(C, i - 1, j, k)) * 0.5 / dr;
+    const double Cc = X(C, i, j, k);
+    const double Crra = dv(X(C, i - 1, j, k) + X(C, i + 1, j, k), K.dr2);
+    const double Cra = dv((X(C, i + 1, j, k) - X(C, i - 1, j, k)) * 0.5, K.dr);
     double Cr = Cra;
-    if (fabs(u * dr / Ptl) > 2) {
-        if (u > 0) Cr = (X(C, i, j, k) - X(C, i - 1, j, k)) / dr;
-        else if (u < 0) Cr = (X(C, i + 1, j, k) - X(C, i, j, k)) / dr;
+    if (fabs(dv(u * dr, K.ptl)) > 2) {
+        if (u > 0) Cr = dv(Cc - X(C, i - 1, j, k), K.dr);
+        else if (u < 0) Cr = dv(X(C, i + 1, j, k) - Cc, K.dr);
     }
-    const double welr = ((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
-                         (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25) / dr;
-    const double thr = (X(th, ip, j, k) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / dr;
+    const double welr = dv((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
+                           (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25, K.dr);
+    const double thr = dv((X(th, ip, j, k) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5, K.dr);
     double Czza, Cz, cnz;
     if (j == 1) {
-        Czza = (3.2 * X(C, i, j - 1, k) + 2.0 * X(C, i, j + 1, k) - 0.2 * X(C, i, j + 2, k)) / (dz * dz);
-        Cz = (-4.0 * X(C, i, j - 1, k) / 3.0 + X(C, i, j, k) + X(C, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
+        Czza = dv(3.2 * Cm + 2.0 * C1 - 0.2 * X(C, i, j + 2, k), K.dz2);
+        Cz = dv(dv(-4.0 * Cm, K.three) + Cc + dv(C1, K.three), K.dz);
+        cnz = K.cnz_w;
     } else if (j == nz) {
-        Czza = (3.2 * X(C, i, j + 1, k) + 2.0 * X(C, i, j - 1, k) - 0.2 * X(C, i, j - 2, k)) / (dz * dz);
-        Cz = (-X(C, i, j - 1, k) / 3.0 - X(C, i, j, k) + 4.0 * X(C, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
+        Czza = dv(3.2 * C1 + 2.0 * Cm - 0.2 * X(C, i, j - 2, k), K.dz2);
+        Cz = dv(dv(-Cm, K.three) - Cc + dv(4.0 * C1, K.three), K.dz);
+        cnz = K.cnz_w;
     } else {
-        Czza = (X(C, i, j - 1, k) + X(C, i, j + 1, k)) / (dz * dz);
-        Cz = (X(C, i, j + 1, k) - X(C, i, j - 1, k)) * 0.5 / dz;
-        if (fabs(w * dz / Ptl) > 2) {
-            if (w > 0) Cz = (X(C, i, j, k) - X(C, i, j - 1, k)) / dz;
-            else if (w < 0) Cz = (X(C, i, j + 1, k) - X(C, i, j, k)) / dz;
+        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
+        Czza = dv(Cm + C1, K.dz2);
+        Cz = dv((C1 - Cm) * 0.5, K.dz);
+        if (fabs(dv(w * dz, K.ptl)) > 2) {
+            if (w > 0) Cz = dv(Cc - Cm, K.dz);
+            else if (w < 0) Cz = dv(C1 - Cc, K.dz);
         }
-        cnz = -1.0 / (dz * dz);
+        cnz = K.cnz_int;
     }
-    const double welz = ((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
-                         (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25) / dz;
-    const double Chha = (X(C, i, j, ka) + X(C, i, j, km)) / (dp * dp);
-    double Ch = (X(C, i, j, ka) - X(C, i, j, km)) * 0.5 / dp;
-    if (fabs(v * dp * r / Ptl) > 2) {
-        if (v > 0) Ch = (X(C, i, j, k) - X(C, i, j, km)) / dp;
-        else if (v < 0) Ch = (X(C, i, j, ka) - X(C, i, j, k)) / dp;
+    const double welz = dv((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
+                           (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25, K.dz);
+    const double Chha = dv(X(C, i, j, ka) + X(C, i, j, km), K.dp2);
+    double Ch = dv((X(C, i, j, ka) - X(C, i, j, km)) * 0.5, K.dp);
+    if (fabs(dv(v * dp * r, K.ptl)) > 2) {
+        if (v > 0) Ch = dv(Cc - X(C, i, j, km), K.dp);
+        else if (v < 0) Ch = dv(X(C, i, j, ka) - Cc, K.dp);
     }
-    const double welh = ((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
-                         (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25) / dp;
-    const double thh = (X(th, i, j, ka) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnh = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = Crra + Cra / r + Chha / (r * r) + Czza + X(C, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Cr + v * Ch / r + w * Cz - welr * AA - welh * BB / r) / Ptl;
-    const double RHS2 = -2.0 * Re * welz - Ra * thh + 2.0 * Ra * (tem * welz - thr * u - thh * v / r) / Re / Ptl;
-    const double cf1 = 1.0 / (Ptl * dt) - welz / (2.0 * Ptl) - cnr - cnz - cnh;
-    const double cf2 = welz / (2.0 * Ptl) + cnr + cnz + cnh;
+    const double welh = dv((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
+                           (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25, K.dp);
+    const double thh = dv((X(th, i, j, ka) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5, K.dp);
+    const double cnr = K.cnr_int;
+    const double cnh = dv(K.m1dp2, R.r2);
+    const double RHS1 = Crra + dv(Cra, R.r) + dv(Chha, R.r2) + Czza + dv(Cc, K.ptldt);
+    const double ULT = dv(u * Cr + dv(v * Ch, R.r) + w * Cz - welr * AA - dv(welh * BB, R.r), K.ptl);
+    const double RHS2 = -2.0 * Re * welz - Ra * thh + dv(dvs(2.0 * Ra * (tem * welz - thr * u - dv(thh * v, R.r)), K.re), K.ptl);
+    const double cf1 = K.ptldt_inv - dv(welz, K.ptl2) - cnr - cnz - cnh;
+    const double cf2 = dv(welz, K.ptl2) + cnr + cnz + cnh;
     return (cf2 * X(F.Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // temperature at (i, j, k), rot3dmg.f:754-871
-__device__ double temper(const Geo &g, const Fields &F, int i, int j, int k)
+__device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int nr = g.nr, nz = g.nz, np = g.np;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl;
+    const double dr = g.dr, dz = g.dz, dp = g.dp;
     const int im = i - 1, jm = j - 1;
     const int km = (k == 0) ? np - 1 : k - 1;
     const int ka = (k == np) ? 1 : k + 1;
-    const double r = g.r0 + ((double)i - 0.5) * dr;
+    const Rad R(g.r0 + ((double)i - 0.5) * dr);
+    const double r = R.r.v;
     double u, v, w;
     if (j == 0 || j == nz + 1) {
         u = 0.0;
@@ -249,115 +313,135 @@ __device__ double temper(const Geo &g, const Fields &F, int i, int j, int k)
         v = (X(vel, i, j, k) + X(vel, i, j, km)) * 0.5;
         w = (X(wel, i, j, k) + X(wel, i, jm, k)) * 0.5;
     }
+    const double Tc = X(th, i, j, k);
     double thrra, thra, thr, cnr;
     if (i == 1) {
-        thrra = (3.2 * X(th, i - 1, j, k) + 2.0 * X(th, i + 1, j, k) - 0.2 * X(th, i + 2, j, k)) / (dr * dr);
-        thra = (-4.0 * X(th, i - 1, j, k) / 3.0 + X(th, i + 1, j, k) / 3.0) / dr;
-        thr = (-4.0 * X(th, i - 1, j, k) / 3.0 + X(th, i, j, k) + X(th, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) + 0.5 / dr / r;
+        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
+        thrra = dv(3.2 * Tm + 2.0 * T1 - 0.2 * X(th, i + 2, j, k), K.dr2);
+        thra = dv(dv(-4.0 * Tm, K.three) + dv(T1, K.three), K.dr);
+        thr = dv(dv(-4.0 * Tm, K.three) + Tc + dv(T1, K.three), K.dr);
+        cnr = K.cnr_w + dv(K.hdr, R.r);
     } else if (i == nr) {
-        thrra = (3.2 * X(th, i + 1, j, k) + 2.0 * X(th, i - 1, j, k) - 0.2 * X(th, i - 2, j, k)) / (dr * dr);
-        thra = (-X(th, i - 1, j, k) / 3.0 + 4.0 * X(th, i + 1, j, k) / 3.0) / dr;
-        thr = (-X(th, i - 1, j, k) / 3.0 - X(th, i, j, k) + 4.0 * X(th, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) - 0.5 / dr / r;
+        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
+        thrra = dv(3.2 * T1 + 2.0 * Tm - 0.2 * X(th, i - 2, j, k), K.dr2);
+        thra = dv(dv(-Tm, K.three) + dv(4.0 * T1, K.three), K.dr);
+        thr = dv(dv(-Tm, K.three) - Tc + dv(4.0 * T1, K.three), K.dr);
+        cnr = K.cnr_w - dv(K.hdr, R.r);
     } else {
-        thrra = (X(th, i - 1, j, k) + X(th, i + 1, j, k)) / (dr * dr);
-        thra = (X(th, i + 1, j, k) - X(th, i - 1, j, k)) * 0.5 / dr;
+        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
+        thrra = dv(Tm + T1, K.dr2);
+        thra = dv((T1 - Tm) * 0.5, K.dr);
         thr = thra;
-        if (fabs(u * dr / Ptl) > 2) {
-            if (u > 0) thr = (X(th, i, j, k) - X(th, i - 1, j, k)) / dr;
-            else if (u < 0) thr = (X(th, i + 1, j, k) - X(th, i, j, k)) / dr;
+        if (fabs(dv(u * dr, K.ptl)) > 2) {
+            if (u > 0) thr = dv(Tc - Tm, K.dr);
+            else if (u < 0) thr = dv(T1 - Tc, K.dr);
         }
-        cnr = -1.0 / (dr * dr);
+        cnr = K.cnr_int;
     }
     double thzza, thz, cnz;
     if (j == 1) {
-        thzza = (3.2 * X(th, i, j - 1, k) + 2.0 * X(th, i, j + 1, k) - 0.2 * X(th, i, j + 2, k)) / (dz * dz);
-        thz = (-4.0 * X(th, i, j - 1, k) / 3.0 + X(th, i, j, k) + X(th, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
+        thzza = dv(3.2 * Tm + 2.0 * T1 - 0.2 * X(th, i, j + 2, k), K.dz2);
+        thz = dv(dv(-4.0 * Tm, K.three) + Tc + dv(T1, K.three), K.dz);
+        cnz = K.cnz_w;
     } else if (j == nz) {
-        thzza = (3.2 * X(th, i, j + 1, k) + 2.0 * X(th, i, j - 1, k) - 0.2 * X(th, i, j - 2, k)) / (dz * dz);
-        thz = (-X(th, i, j - 1, k) / 3.0 - X(th, i, j, k) + 4.0 * X(th, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
+        thzza = dv(3.2 * T1 + 2.0 * Tm - 0.2 * X(th, i, j - 2, k), K.dz2);
+        thz = dv(dv(-Tm, K.three) - Tc + dv(4.0 * T1, K.three), K.dz);
+        cnz = K.cnz_w;
     } else if (j == 0) {
-        thzza = 8.0 * X(th, i, j + 1, k) / (dz * dz);
+        thzza = dv(8.0 * X(th, i, j + 1, k), K.dz2);
         thz = 0.0;
-        cnz = -4.0 / (dz * dz);
+        cnz = K.cnz_t;
     } else if (j == nz + 1) {
-        thzza = 8.0 * X(th, i, j - 1, k) / (dz * dz);
+        thzza = dv(8.0 * X(th, i, j - 1, k), K.dz2);
         thz = 0.0;
-        cnz = -4.0 / (dz * dz);
+        cnz = K.cnz_t;
     } else {
-        thzza = (X(th, i, j - 1, k) + X(th, i, j + 1, k)) / (dz * dz);
-        thz = (X(th, i, j + 1, k) - X(th, i, j - 1, k)) * 0.5 / dz;
-        if (fabs(w * dz / Ptl) > 2) {
-            if (w > 0) thz = (X(th, i, j, k) - X(th, i, j - 1, k)) / dz;
-            else if (w < 0) thz = (X(th, i, j + 1, k) - X(th, i, j, k)) / dz;
+        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
+        thzza = dv(Tm + T1, K.dz2);
+        thz = dv((T1 - Tm) * 0.5, K.dz);
+        if (fabs(dv(w * dz, K.ptl)) > 2) {
+            if (w > 0) thz = dv(Tc - Tm, K.dz);
+            else if (w < 0) thz = dv(T1 - Tc, K.dz);
         }
-        cnz = -1.0 / (dz * dz);
+        cnz = K.cnz_int;
     }
-    const double thhha = (X(th, i, j, ka) + X(th, i, j, km)) / (dp * dp);
-    double thh = (X(th, i, j, ka) - X(th, i, j, km)) * 0.5 / dp;
-    if (fabs(v * dp * r / Ptl) > 2) {
-        if (v > 0) thh = (X(th, i, j, k) - X(th, i, j, km)) / dp;
-        else if (v < 0) thh = (X(th, i, j, ka) - X(th, i, j, k)) / dp;
+    const double thhha = dv(X(th, i, j, ka) + X(th, i, j, km), K.dp2);
+    double thh = dv((X(th, i, j, ka) - X(th, i, j, km)) * 0.5, K.dp);
+    if (fabs(dv(v * dp * r, K.ptl)) > 2) {
+        if (v > 0) thh = dv(Tc - X(th, i, j, km), K.dp);
+        else if (v < 0) thh = dv(X(th, i, j, ka) - Tc, K.dp);
     }
-    const double cnp = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = thrra + thra / r + thhha / (r * r) + thzza + X(th, i, j, k) / dt;
-    const double ULT = u * thr + v * thh / r + w * thz;
-    const double cf1 = 1.0 / dt - cnr - cnz - cnp;
+    const double cnp = dv(K.m1dp2, R.r2);
+    const double RHS1 = thrra + dv(thra, R.r) + dv(thhha, R.r2) + thzza + dv(Tc, K.dt);
+    const double ULT = u * thr + dv(v * thh, R.r) + w * thz;
+    const double cf1 = K.dt_inv - cnr - cnz - cnp;
     const double cf2 = cnr + cnz + cnp;
     return (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
 }
 
-// One launch for A, B, C and T: blockIdx.y = 0..3, a thread per point of
-// that routine's loop range.
-__global__ void __launch_bounds__(WG) k3_explicit(Geo g, Fields F, int mask)
+// One routine (W = 0: A, 1: B, 2: C, 3: T) per launch, blockIdx.y = k, a
+// thread per (i, j) point of the routine's loop range.  A kernel per routine
+// keeps its constants and field pointers within the scalar registers.
+template <int W>
+__global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
 {
-    const int which = blockIdx.y;
-    if (!(mask & (1 << which))) return;
+    constexpr int which = W;
     const int nr = g.nr, nz = g.nz;
     int i0 = 1, ni, j0 = 1, nj;
     if (which == 0) { ni = nr; nj = nz - 1; }
     else if (which == 1) { ni = nr - 1; nj = nz - 1; }
     else if (which == 2) { ni = nr - 1; nj = nz; }
     else { ni = nr; j0 = 0; nj = nz + 2; }
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long cnt = (long long)ni * nj * (g.np + 1);
-    if (t >= cnt) return;
-    const int i = i0 + (int)(t % ni);
-    const int j = j0 + (int)((t / ni) % nj);
-    const int k = (int)(t / ((long long)ni * nj));
-    if (which == 0) X(F.An, i, j, k) = avort(g, F, i, j, k);
-    else if (which == 1) X(F.Bn, i, j, k) = bvort(g, F, i, j, k);
-    else if (which == 2) X(F.Cn, i, j, k) = cvort(g, F, i, j, k);
-    else X(F.thn, i, j, k) = temper(g, F, i, j, k);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj) return;
+    const int jj = t / ni;
+    const int i = i0 + (t - jj * ni), j = j0 + jj, k = blockIdx.y;
+    if constexpr (W == 0) X(F.An, i, j, k) = avort(g, K, F, i, j, k);
+    else if constexpr (W == 1) X(F.Bn, i, j, k) = bvort(g, K, F, i, j, k);
+    else if constexpr (W == 2) X(F.Cn, i, j, k) = cvort(g, K, F, i, j, k);
+    else X(F.thn, i, j, k) = temper(g, K, F, i, j, k);
 }
 
-// heat, rot3dmg.f:882-905: the flux of each azimuth k in parallel (with the
-// reference's order of the z sum), then trans1 summed over k in order.
-__global__ void __launch_bounds__(WG) k3_heat(Geo g, const double *th, double *part, double *out)
+// heat, rot3dmg.f:882-905.  The flux terms of every (j, k) in parallel,
+// stored in the reference's summation order over j (0, 1, nz, nz+1, 2, ...,
+// nz-1): hs[p * np + k-1].
+__global__ void __launch_bounds__(WG) k3_heat_terms(Geo g, const double *th, double *hs)
 {
     const int nz = g.nz, np = g.np;
-    const double dr = g.dr, dz = g.dz, dp = g.dp;
-    for (int k = 1 + threadIdx.x; k <= np; k += blockDim.x) {
-        double trans = dz * (-184.0 * X(th, 0, 0, k) + 225.0 * X(th, 1, 0, k) - 50.0 * X(th, 2, 0, k) +
-                             9.0 * X(th, 3, 0, k)) / (240.0 * dr);
-        trans = trans + dz * (-184.0 * X(th, 0, 1, k) + 225.0 * X(th, 1, 1, k) - 50.0 * X(th, 2, 1, k) +
-                              9.0 * X(th, 3, 1, k)) / (80.0 * dr);
-        trans = trans + dz * (-184.0 * X(th, 0, nz, k) + 225.0 * X(th, 1, nz, k) - 50.0 * X(th, 2, nz, k) +
-                              9.0 * X(th, 3, nz, k)) / (80.0 * dr);
-        trans = trans + dz * (-184.0 * X(th, 0, nz + 1, k) + 225.0 * X(th, 1, nz + 1, k) - 50.0 * X(th, 2, nz + 1, k) +
-                              9.0 * X(th, 3, nz + 1, k)) / (240.0 * dr);
-        for (int j = 2; j <= nz - 1; ++j)
-            trans = trans + dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) +
-                                  9.0 * X(th, 3, j, k)) / (60.0 * dr);
+    const double dr = g.dr, dz = g.dz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (nz + 2) * np) return;
+    const int k = 1 + t % np, j = t / np;
+    const double w = (j == 0 || j == nz + 1) ? 240.0 : (j == 1 || j == nz) ? 80.0 : 60.0;
+    const int p = (j == 0) ? 0 : (j == 1) ? 1 : (j == nz) ? 2 : (j == nz + 1) ? 3 : j + 2;
+    hs[(size_t)p * np + (k - 1)] =
+        dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) + 9.0 * X(th, 3, j, k)) /
+        (w * dr);
+}
+
+// the sums in the reference's order: trans per k over j, then trans1 over k
+__global__ void __launch_bounds__(WG) k3_heat_sum(Geo g, const double *hs, double *part, double *out)
+{
+    const int nz = g.nz, np = g.np;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        double trans = hs[k];
+        int p = 1;
+        for (; p + 8 <= nz + 2; p += 8) {   // loads issued ahead of the in-order sum
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = hs[(size_t)(p + u) * np + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) trans = trans + v[u];
+        }
+        for (; p < nz + 2; ++p) trans = trans + hs[(size_t)p * np + k];
         part[k] = trans;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double trans1 = 0.0;
-        for (int k = 1; k <= np; ++k) trans1 = trans1 + part[k] * dp;
+        for (int k = 0; k < np; ++k) trans1 = trans1 + part[k] * g.dp;
         *out = trans1;
     }
 }
@@ -374,11 +458,10 @@ __global__ void __launch_bounds__(WG) k3_rhs(Geo g, RhsF F, int which)
     const double dr = g.dr, dz = g.dz, dp = g.dp;
     const int ni = (which == 3) ? nr - 1 : nr;
     const int nj = (which == 1) ? nz - 1 : nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)ni * nj * (np + 1)) return;
-    const int i = 1 + (int)(t % ni);
-    const int j = 1 + (int)((t / ni) % nj);
-    const int k = (int)(t / ((long long)ni * nj));
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj) return;
+    const int jj = t / ni;
+    const int i = 1 + (t - jj * ni), j = 1 + jj, k = blockIdx.y;
     const int km = (k == 0) ? np - 1 : k - 1;
     const double *An = F.An, *Bn = F.Bn, *Cn = F.Cn, *wel = F.wel;
     if (which == 1) {
@@ -431,12 +514,10 @@ __global__ void __launch_bounds__(WG) k3_update(Geo g, UpdF F)
 {
     const int nr = g.nr, nz = g.nz;
     const double dr = g.dr, dz = g.dz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)(nr + 2) * (nz + 2) * (g.np + 1)) return;
-    const int i = (int)(t % (nr + 2));
-    const int j = (int)((t / (nr + 2)) % (nz + 2));
-    const int k = (int)(t / ((long long)(nr + 2) * (nz + 2)));
-    const size_t p = ((size_t)k * g.ldj + (size_t)j) * g.ldi + (size_t)i;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ldi * g.ldj) return;
+    const int j = t / g.ldi, i = t - j * g.ldi, k = blockIdx.y;
+    const unsigned p = (unsigned)(k * g.ldi * g.ldj + t);
     const double *uel = F.uel, *vel = F.vel, *wel = F.wel;
     // A: endwalls j = 0, nz for i = 1..nr (rot3dmg.f:968-971)
     double a = F.An[p];
@@ -526,7 +607,8 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         if (hipMalloc(&r->f[q], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
         if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
     }
-    if (hipMalloc(&r->heat, (size_t)(cfg->np + 2) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->heat, ((size_t)(cfg->nz + 3) * cfg->np + 1) * sizeof(double)) != hipSuccess)
+        return fail(VV_ENOMEM);
     {
         const int st = poisson3_setup(r->pois, geo(r));
         if (st) return fail(st);
@@ -579,10 +661,55 @@ static Fields fields(vv3_run *r)
     return F;
 }
 
+// heat's terms and sums; the total lands at heat_total(r)
+static double *heat_total(vv3_run *r) { return r->heat + (size_t)(r->nz + 3) * r->np; }
+
+static int launch_heat(vv3_run *r)
+{
+    const Geo g = geo(r);
+    double *hs = r->heat, *part = r->heat + (size_t)(r->nz + 2) * r->np;
+    hipLaunchKernelGGL(k3_heat_terms, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, r->stream, g,
+                       (const double *)r->f[VV3_TH], hs);
+    hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), 0, r->stream, g, (const double *)hs, part, heat_total(r));
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+// the explicit routines' constants, evaluated as the reference writes them
+static Kc constants(const vv3_run *r)
+{
+    const double dr = r->dr, dz = r->dz, dp = r->dp, dt = r->cfg.dt, Ptl = r->cfg.Ptl;
+    auto rc = [](double v) { return Rc{v, 1.0 / v}; };
+    Kc K;
+    K.dr = rc(dr); K.dz = rc(dz); K.dp = rc(dp);
+    K.dr2 = rc(dr * dr); K.dz2 = rc(dz * dz); K.dp2 = rc(dp * dp);
+    K.ptl = rc(Ptl); K.ptl2 = rc(2.0 * Ptl); K.ptldt = rc(Ptl * dt);
+    K.re = rc(r->cfg.Re); K.three = rc(3.0); K.dt = rc(dt);
+    K.cnr_int = -1.0 / (dr * dr);
+    K.cnz_int = -1.0 / (dz * dz);
+    K.cnr_w = -2.5 / (dr * dr);
+    K.cnz_w = -2.5 / (dz * dz);
+    K.cnz_t = -4.0 / (dz * dz);
+    K.hdr = 0.5 / dr;
+    K.m1dp2 = -1.0 / (dp * dp);
+    K.ptldt_inv = 1.0 / (Ptl * dt);
+    K.dt_inv = 1.0 / dt;
+    K.grptl = r->cfg.Gr * Ptl;
+    return K;
+}
+
 static int launch_explicit(vv3_run *r, int mask)
 {
-    const long long cnt = (long long)(r->nr) * (r->nz + 2) * (r->np + 1);   // the largest range (temperature)
-    hipLaunchKernelGGL(k3_explicit, dim3(nblk(cnt, WG), 4), dim3(WG), 0, r->stream, geo(r), fields(r), mask);
+    const Geo g = geo(r);
+    const Kc K = constants(r);
+    const Fields F = fields(r);
+    const int np1 = r->np + 1;
+    const long long pa = (long long)r->nr * (r->nz - 1), pb = (long long)(r->nr - 1) * (r->nz - 1);
+    const long long pc = (long long)(r->nr - 1) * r->nz, pt = (long long)r->nr * (r->nz + 2);
+    if (mask & 1) hipLaunchKernelGGL(k3_explicit<0>, dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    if (mask & 2) hipLaunchKernelGGL(k3_explicit<1>, dim3(nblk(pb, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    if (mask & 4) hipLaunchKernelGGL(k3_explicit<2>, dim3(nblk(pc, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    if (mask & 8) hipLaunchKernelGGL(k3_explicit<3>, dim3(nblk(pt, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -592,8 +719,8 @@ static int launch_rhs(vv3_run *r, int which)
     RhsF F;
     F.An = r->f[VV3_AN]; F.Bn = r->f[VV3_BN]; F.Cn = r->f[VV3_CN]; F.wel = r->f[VV3_WEL];
     F.wrh = r->f[VV3_WRH]; F.vrh = r->f[VV3_VRH]; F.urh = r->f[VV3_URH];
-    const long long cnt = (long long)r->nr * r->nz * (r->np + 1);
-    hipLaunchKernelGGL(k3_rhs, dim3(nblk(cnt, WG)), dim3(WG), 0, r->stream, geo(r), F, which);
+    const long long cnt = (long long)r->nr * r->nz;
+    hipLaunchKernelGGL(k3_rhs, dim3(nblk(cnt, WG), r->np + 1), dim3(WG), 0, r->stream, geo(r), F, which);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -612,7 +739,8 @@ static int launch_update(vv3_run *r)
     F.uel = r->f[VV3_UEL]; F.vel = r->f[VV3_VEL]; F.wel = r->f[VV3_WEL];
     F.An = r->f[VV3_AN]; F.Bn = r->f[VV3_BN]; F.Cn = r->f[VV3_CN]; F.thn = r->f[VV3_THN];
     F.Ap = r->f[VV3_AP]; F.Bp = r->f[VV3_BP]; F.Cp = r->f[VV3_CP]; F.thp = r->f[VV3_THP];
-    hipLaunchKernelGGL(k3_update, dim3(nblk((long long)r->nf, WG)), dim3(WG), 0, r->stream, geo(r), F);
+    hipLaunchKernelGGL(k3_update, dim3(nblk((long long)r->ldi * r->ldj, WG), r->np + 1), dim3(WG), 0, r->stream,
+                       geo(r), F);
     VV_HIP(hipGetLastError());
     // rot3dmg.f:1003-1015: Ap <- A <- An as a rotation of the buffers
     const int trip[4][3] = {{VV3_AP, VV3_A, VV3_AN}, {VV3_BP, VV3_B, VV3_BN}, {VV3_CP, VV3_C, VV3_CN},
@@ -648,11 +776,10 @@ int vv3_heat(vv3_run *r, double *trans1)
 {
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
-    hipLaunchKernelGGL(k3_heat, dim3(1), dim3(WG), 0, r->stream, geo(r), (const double *)r->f[VV3_TH], r->heat,
-                       r->heat + r->np + 1);
-    VV_HIP(hipGetLastError());
+    const int st = launch_heat(r);
+    if (st) return st;
     double t = 0.0;
-    VV_HIP(hipMemcpyAsync(&t, r->heat + r->np + 1, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipMemcpyAsync(&t, heat_total(r), sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     if (trans1) *trans1 = t;
     return VV_OK;
@@ -691,11 +818,7 @@ int vv3_step(vv3_run *r, int n, double *trans1)
     VV_HIP(hipSetDevice(r->device));
     for (int s = 0; s < n; ++s) {
         int st = launch_explicit(r, 15);
-        if (!st) {
-            hipLaunchKernelGGL(k3_heat, dim3(1), dim3(WG), 0, r->stream, geo(r), (const double *)r->f[VV3_TH],
-                               r->heat, r->heat + r->np + 1);
-            VV_HIP(hipGetLastError());
-        }
+        if (!st) st = launch_heat(r);
         if (!st) st = launch_rhs(r, 1);
         if (!st) st = launch_poisson(r, 1);
         if (!st) st = launch_rhs(r, 2);
@@ -706,7 +829,7 @@ int vv3_step(vv3_run *r, int n, double *trans1)
         if (st) return st;
     }
     double t = 0.0;
-    VV_HIP(hipMemcpyAsync(&t, r->heat + r->np + 1, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipMemcpyAsync(&t, heat_total(r), sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     if (trans1) *trans1 = t;
     return VV_OK;

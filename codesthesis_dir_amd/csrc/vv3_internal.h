@@ -9,6 +9,16 @@
 
 namespace vv3 {
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) in turn:
+// xcd_block maps the dispatch index b of nb workgroups to a logical index so
+// that each XCD works through one contiguous range, and neighbouring rows
+// and planes of a field are read through the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned x = b & 7u, q = b >> 3, per = nb >> 3, rem = nb & 7u;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 // grid and parameters handed to every kernel by value
 struct Geo {
     int nr, nz, np, ldi, ldj;
@@ -33,11 +43,12 @@ struct PoissonOp {
 struct Poisson3 {
     int nr = 0, nz = 0, np = 0, nm = 0, ldi = 0, ldj = 0;
     PoissonOp op[3];
-    double *W = nullptr;     // 2nm x np: cos / sin rows of the forward DFT over k = 1..np
-    double *Wi = nullptr;    // np x 2nm: the inverse (weights and 1/np folded in)
+    double *W = nullptr;     // np x np: forward real DFT over k = 1..np; rows m = 0..np/2
+                             // are cos(m phi), rows np/2 + m (0 < m < np/2) sin(m phi)
+    double *Wi = nullptr;    // np x np: the inverse (weights and 1/np folded in)
     double *lphi = nullptr;  // nm: 2 cos(2 pi mode/np) - 2
-    double *S = nullptr;     // 2nm x mmax: phi spectra, then the z-inverse result
-    double *T = nullptr;     // 2nm x mmax: z spectra, (i, l) with l fastest
+    double *S = nullptr;     // np x mmax: phi spectra, then the z-inverse result
+    double *T = nullptr;     // np x mmax: z spectra, (i, l) with l fastest
     double *cp = nullptr;    // nm x mmax: elimination factors of the r solves
 };
 

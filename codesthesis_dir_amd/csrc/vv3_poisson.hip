@@ -31,33 +31,44 @@ constexpr int GT = 64;    // GEMM output tile (rows and columns)
 constexpr int GK = 16;    // GEMM k step
 constexpr int GW = 256;   // GEMM workgroup
 
-// C[q] = A * B[q] with A a dense row-major table (M x K) and B, C reached
-// through the functor: op.b(q, k, n) loads, op.c(q, m, n, v) stores.  Each
-// thread accumulates a 4 x 4 block.  BKFAST: B tiles are loaded with k
-// fastest (B contiguous along k); MFAST: a thread's outputs run along m so a
-// store instruction writes consecutive m (C contiguous along m).
-template <class Op, bool BKFAST, bool MFAST>
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// C[q] = A[q] * B[q] (FP64, exact products and sums, no reordering beyond
+// the MFMA accumulation), operands reached through the functor:
+// op.a(q, m, k), op.b(q, k, n) load, op.c(q, m, n, v) stores; C is written
+// with consecutive n in consecutive lanes.  A workgroup computes a 64 x 64
+// tile, each wave a 32 x 32 quarter as 2 x 2 v_mfma_f64_16x16x4f64 blocks.
+// MFMA lane layout: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// element e of D in lane l is D[l/16 + 4e][l%16].  AKFAST / BKFAST: the
+// tile loads run along k (operand contiguous in k) rather than along m / n.
+template <class Op, bool AKFAST, bool BKFAST>
 __global__ void __launch_bounds__(GW) k3_gemm(Op op)
 {
     __shared__ double As[GK][GT];
     __shared__ double Bs[GK][GT];
-    const int q = blockIdx.z;
-    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-    const int tid = threadIdx.x;
-    const int tm = MFAST ? (tid % 16) : (tid / 16);
-    const int tn = MFAST ? (tid / 16) : (tid % 16);
-    double acc[4][4];
+    // logical tile order: m tiles fastest, then n tiles, then the batch, so
+    // the m tiles that share a B tile run together on one XCD
+    const unsigned nmt = gridDim.y, nnt = gridDim.x;
+    const unsigned lb = xcd_block((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+                                  gridDim.x * gridDim.y * gridDim.z);
+    const int q = (int)(lb / (nmt * nnt));
+    const int m0 = (int)(lb % nmt) * GT, n0 = (int)((lb / nmt) % nnt) * GT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    d4 acc[2][2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+        for (int c = 0; c < 2; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < op.K; k0 += GK) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int e = tid + GW * t;
-            const int kk = e % GK, mm = e / GK;
-            const int m = m0 + mm, k = k0 + kk;
-            As[kk][mm] = (m < op.M && k < op.K) ? op.A[(size_t)m * op.K + k] : 0.0;
+            const int ka = AKFAST ? (e % GK) : (e / GT);
+            const int ma = AKFAST ? (e / GK) : (e % GT);
+            const int m = m0 + ma, k = k0 + ka;
+            As[ka][ma] = (m < op.M && k < op.K) ? op.a(q, m, k) : 0.0;
             const int kb = BKFAST ? (e % GK) : (e / GT);
             const int nb = BKFAST ? (e / GK) : (e % GT);
             const int n = n0 + nb, k2 = k0 + kb;
@@ -65,36 +76,39 @@ __global__ void __launch_bounds__(GW) k3_gemm(Op op)
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GK; ++kk) {
-            double a[4], b[4];
+        for (int ks = 0; ks < GK; ks += 4) {
+            double a[2], b[2];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = MFAST ? As[kk][tm + 16 * r] : As[kk][tm * 4 + r];
+            for (int r = 0; r < 2; ++r) a[r] = As[ks + lk][wr + 16 * r + li];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = MFAST ? Bs[kk][tn * 4 + c] : Bs[kk][tn + 16 * c];
+            for (int c = 0; c < 2; ++c) b[c] = Bs[ks + lk][wc + 16 * c + li];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+                for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int m = m0 + (MFAST ? tm + 16 * r : tm * 4 + r);
-            const int n = n0 + (MFAST ? tn * 4 + c : tn + 16 * c);
-            if (m < op.M && n < op.N) op.c(q, m, n, acc[r][c]);
-        }
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + wr + 16 * r + lk + 4 * e;
+                const int n = n0 + wc + 16 * c + li;
+                if (m < op.M && n < op.N) op.c(q, m, n, acc[r][c][e]);
+            }
 }
 
 // 1. spectra S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
 struct PhiFwd {
-    const double *A;
+    const double *W;
     int M, N, K;
     const double *rhs;
     double *S;
     int i1, ldi, ldj;
+    __device__ double a(int, int m, int k) const { return W[(size_t)m * K + k]; }
     __device__ double b(int, int k, int n) const
     {
         const int j = n / i1, i = n - j * i1;
@@ -103,35 +117,38 @@ struct PhiFwd {
     __device__ void c(int, int m, int n, double v) const { S[(size_t)m * N + n] = v; }
 };
 
-// 2. T[q][i][l] = sum_j Vinv[l][j] S[q][j][i]
+// 2. T[q][i][l] = sum_j S[q][j][i] Vinv[l][j]  (m = i, k = j, n = l)
 struct ZFwd {
-    const double *A;
+    const double *Vinv;
     int M, N, K;
     const double *S;
     double *T;
     int mt;
-    __device__ double b(int q, int k, int n) const { return S[(size_t)q * mt + (size_t)k * N + n]; }
-    __device__ void c(int q, int m, int n, double v) const { T[(size_t)q * mt + (size_t)n * M + m] = v; }
+    __device__ double a(int q, int m, int k) const { return S[(size_t)q * mt + (size_t)k * M + m]; }
+    __device__ double b(int, int k, int n) const { return Vinv[(size_t)n * K + k]; }
+    __device__ void c(int q, int m, int n, double v) const { T[(size_t)q * mt + (size_t)m * N + n] = v; }
 };
 
-// 4. S[q][j][i] = sum_l V[j][l] T[q][i][l]
+// 4. S[q][j][i] = sum_l V[j][l] T[q][i][l]  (m = j, k = l, n = i)
 struct ZInv {
-    const double *A;
+    const double *V;
     int M, N, K;
     const double *T;
     double *S;
     int mt;
+    __device__ double a(int, int m, int k) const { return V[(size_t)m * K + k]; }
     __device__ double b(int q, int k, int n) const { return T[(size_t)q * mt + (size_t)n * K + k]; }
     __device__ void c(int q, int m, int n, double v) const { S[(size_t)q * mt + (size_t)m * N + n] = v; }
 };
 
 // 5. psi(i, j, k+1) = sum_q Wi[k][q] S[q][u]; k = np also fills k = 0
 struct PhiInv {
-    const double *A;
+    const double *Wi;
     int M, N, K;
     const double *S;
     double *psi;
     int i1, ldi, ldj;
+    __device__ double a(int, int m, int k) const { return Wi[(size_t)m * K + k]; }
     __device__ double b(int, int k, int n) const { return S[(size_t)k * N + n]; }
     __device__ void c(int, int m, int n, double v) const
     {
@@ -142,11 +159,11 @@ struct PhiInv {
     }
 };
 
-template <bool BKFAST, bool MFAST, class Op>
+template <bool AKFAST, bool BKFAST, class Op>
 void gemm(const Op &op, int batch, hipStream_t s)
 {
     dim3 grid((op.N + GT - 1) / GT, (op.M + GT - 1) / GT, batch);
-    hipLaunchKernelGGL((k3_gemm<Op, BKFAST, MFAST>), grid, dim3(GW), 0, s, op);
+    hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST>), grid, dim3(GW), 0, s, op);
 }
 
 // psip <- psi on the range the reference's routine writes, and psi <- 0 on
@@ -154,10 +171,10 @@ void gemm(const Op &op, int batch, hipStream_t s)
 __global__ void __launch_bounds__(256) k3_psi_pre(Geo g, int which, int i1, int j1, double *psi, double *psip)
 {
     const int nr = g.nr, nz = g.nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)(nr + 2) * (nz + 2) * (g.np + 1)) return;
-    const int i = (int)(t % (nr + 2));
-    const int j = (int)((t / (nr + 2)) % (nz + 2));
+    const int tp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tp >= g.ldi * g.ldj) return;
+    const int j = tp / g.ldi, i = tp - j * g.ldi;
+    const unsigned t = (unsigned)(blockIdx.y * g.ldi * g.ldj + tp);
     const int ri = (which == 3) ? nr : nr + 1;
     const int rj = (which == 1) ? nz : nz + 1;
     if (i > ri || j > rj) return;
@@ -166,44 +183,73 @@ __global__ void __launch_bounds__(256) k3_psi_pre(Geo g, int which, int i1, int 
 }
 
 // 3. per (mode, l): (R + lphi(mode) diag(1/(dp r)^2) + lz(l)) x = y for the
-// cos and sin parts, x overwriting y in T (layout [q][i][l]).  Row 1 has an
-// extra coefficient e1 at i = 3, removed with row 2; row i1 has en at
-// i1 - 2, removed with the eliminated row i1 - 2.  Thomas factors go to cp.
-__global__ void __launch_bounds__(256) k3_rsolve(double *T, double *cp, const double *rt, const double *lz,
-                                                  const double *lphi, int i1, int j1, int nm, double e1, double en)
+// cos part (spectral row mode) and, for 0 < mode < np/2, the sin part (row
+// np/2 + mode), x overwriting y in T (layout [q][i][l]).  Row 1 has an extra
+// coefficient e1 at i = 3, removed with row 2; row i1 has en at i1 - 2,
+// removed with the eliminated row i1 - 2.  Thomas factors go to cp.
+__global__ void __launch_bounds__(64) k3_rsolve(double *__restrict__ T, double *__restrict__ cp,
+                                                 const double *__restrict__ rt, const double *__restrict__ lz,
+                                                 const double *__restrict__ lphi, int i1, int j1, int np, double e1,
+                                                 double en)
 {
+    const int nm = np / 2 + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm * j1) return;
     const int mode = t / j1, l = t - mode * j1;
     const size_t mt = (size_t)i1 * j1;
-    double *xc = T + (size_t)(2 * mode) * mt + l;
-    double *xs = xc + mt;
-    double *c = cp + (size_t)mode * mt + l;
+    const bool two = (mode > 0 && 2 * mode < np);
+    double *__restrict__ xc = T + (size_t)mode * mt + l;
+    double *__restrict__ xs = T + (size_t)(two ? np / 2 + mode : mode) * mt + l;
+    double *__restrict__ c = cp + (size_t)mode * mt + l;
     const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1, *ir2 = rt + 3 * i1;
     const double mu = lphi[mode], lam = lz[l];
     auto diag = [&](int i) { return dia[i] + mu * ir2[i] + lam; };
-    double d0 = diag(0), s0 = sup[0], yc = xc[0], ys = xs[0];
+    double d0 = diag(0), s0 = sup[0], yc = xc[0], ys = two ? xs[0] : 0.0;
     if (e1 != 0.0) {
         const double f = e1 / sup[1];
         d0 -= f * sub[1];
         s0 -= f * diag(1);
         yc -= f * xc[j1];
-        ys -= f * xs[j1];
+        if (two) ys -= f * xs[j1];
     }
     double cprev = s0 / d0;
     yc /= d0;
     ys /= d0;
     c[0] = cprev;
     xc[0] = yc;
-    xs[0] = ys;
-    for (int i = 1; i < i1; ++i) {
+    if (two) xs[0] = ys;
+    // rows 1 .. i1-2 in groups of U with their loads issued ahead
+    constexpr int U = 8;
+    int i = 1;
+    for (; i + U <= i1 - 1; i += U) {
+        double rcu[U], rsu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rcu[u] = xc[(size_t)(i + u) * j1];
+            rsu[u] = two ? xs[(size_t)(i + u) * j1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ii = i + u;
+            const size_t o = (size_t)ii * j1;
+            const double a = sub[ii];
+            const double den = diag(ii) - a * cprev;
+            cprev = sup[ii] / den;
+            yc = (rcu[u] - a * yc) / den;
+            ys = (rsu[u] - a * ys) / den;
+            c[o] = cprev;
+            xc[o] = yc;
+            if (two) xs[o] = ys;
+        }
+    }
+    for (; i < i1; ++i) {
         const size_t o = (size_t)i * j1;
-        double a = sub[i], rc = xc[o], rs = xs[o];
+        double a = sub[i], rc = xc[o], rs = two ? xs[o] : 0.0;
         if (i == i1 - 1 && en != 0.0) {
             const size_t o2 = (size_t)(i - 2) * j1;
             a -= en * c[o2];
             rc -= en * xc[o2];
-            rs -= en * xs[o2];
+            if (two) rs -= en * xs[o2];
         }
         const double den = diag(i) - a * cprev;
         cprev = sup[i] / den;
@@ -211,15 +257,38 @@ __global__ void __launch_bounds__(256) k3_rsolve(double *T, double *cp, const do
         ys = (rs - a * ys) / den;
         c[o] = cprev;
         xc[o] = yc;
-        xs[o] = ys;
+        if (two) xs[o] = ys;
     }
-    for (int i = i1 - 2; i >= 0; --i) {
+    i = i1 - 2;
+    for (; i - U + 1 >= 0; i -= U) {
+        double fu[U], xcu[U], xsu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t o = (size_t)(i - u) * j1;
+            fu[u] = c[o];
+            xcu[u] = xc[o];
+            xsu[u] = two ? xs[o] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t o = (size_t)(i - u) * j1;
+            yc = xcu[u] - fu[u] * yc;
+            xc[o] = yc;
+            if (two) {
+                ys = xsu[u] - fu[u] * ys;
+                xs[o] = ys;
+            }
+        }
+    }
+    for (; i >= 0; --i) {
         const size_t o = (size_t)i * j1;
         const double f = c[o];
         yc = xc[o] - f * yc;
-        ys = xs[o] - f * ys;
         xc[o] = yc;
-        xs[o] = ys;
+        if (two) {
+            ys = xs[o] - f * ys;
+            xs[o] = ys;
+        }
     }
 }
 
@@ -463,17 +532,21 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
     P.nm = nm;
     const double pi = std::acos(-1.0);
     // phi tables: the angle of mode m at k is 2 pi ((m k) mod np) / np
-    std::vector<double> W((size_t)2 * nm * np), Wi((size_t)np * 2 * nm), lphi(nm);
+    // spectral rows: q = m (cos, m = 0..np/2), q = np/2 + m (sin, m = 1..np/2-1)
+    std::vector<double> W((size_t)np * np), Wi((size_t)np * np), lphi(nm);
     for (int m = 0; m < nm; ++m) {
         const double wgt = (m == 0 || 2 * m == np) ? 1.0 : 2.0;
         lphi[m] = 2.0 * std::cos(2.0 * pi * m / np) - 2.0;
+        const bool sin_row = (m > 0 && 2 * m < np);
         for (int k = 1; k <= np; ++k) {
             const double ang = 2.0 * pi * (double)(((long long)m * k) % np) / np;
             const double cs = std::cos(ang), sn = std::sin(ang);
-            W[(size_t)(2 * m) * np + (k - 1)] = cs;
-            W[(size_t)(2 * m + 1) * np + (k - 1)] = sn;
-            Wi[(size_t)(k - 1) * 2 * nm + 2 * m] = wgt * cs / np;
-            Wi[(size_t)(k - 1) * 2 * nm + 2 * m + 1] = wgt * sn / np;
+            W[(size_t)m * np + (k - 1)] = cs;
+            Wi[(size_t)(k - 1) * np + m] = wgt * cs / np;
+            if (sin_row) {
+                W[(size_t)(np / 2 + m) * np + (k - 1)] = sn;
+                Wi[(size_t)(k - 1) * np + np / 2 + m] = wgt * sn / np;
+            }
         }
     }
     VV_HIP(upload(&P.W, W));
@@ -509,8 +582,8 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
         VV_HIP(upload(&O.rt, rt));
         VV_HIP(upload(&O.lz, lam));
     }
-    VV_HIP(hipMalloc(&P.S, (size_t)2 * nm * mmax * sizeof(double)));
-    VV_HIP(hipMalloc(&P.T, (size_t)2 * nm * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&P.S, (size_t)np * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&P.T, (size_t)np * mmax * sizeof(double)));
     VV_HIP(hipMalloc(&P.cp, (size_t)nm * mmax * sizeof(double)));
     return VV_OK;
 }
@@ -532,14 +605,14 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
 {
     const PoissonOp &O = P.op[which - 1];
     const int nm = P.nm, np = P.np, i1 = O.i1, j1 = O.j1, mt = O.m;
-    const long long nf = (long long)g.ldi * g.ldj * (np + 1);
-    hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, g, which, i1, j1, psi, psip);
-    gemm<false, false>(PhiFwd{P.W, 2 * nm, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
-    gemm<false, true>(ZFwd{O.Vinv, j1, i1, j1, P.S, P.T, mt}, 2 * nm, s);
-    hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 255) / 256)), dim3(256), 0, s, P.T, P.cp,
-                       (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, nm, O.e1, O.en);
-    gemm<true, false>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, 2 * nm, s);
-    gemm<false, false>(PhiInv{P.Wi, np, mt, 2 * nm, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+    hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((g.ldi * g.ldj + 255) / 256), np + 1), dim3(256), 0, s, g, which,
+                       i1, j1, psi, psip);
+    gemm<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
+    gemm<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
+    hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
+                       (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
+    gemm<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
+    gemm<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
