@@ -126,6 +126,77 @@ def vcycle_large_leg(steps=6, nu=1):
             "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
 
 
+# 3D rotating annulus (rot3dmg.f): the reference program's largest grid
+# (PARAMETER kr = kz = 33, kp = 160) and config 3's grid
+ROT3D = {
+    "rot3d-33x33x161": (32, 32, 160),
+    "rot3d-129x129x257": (128, 128, 256),
+}
+ROT3D_PHYS = dict(dt=1e-6, Ra=1e7, Re=1000.0, Ptl=7.0, Gr=0.0, r0=0.5, rmax=1.0, zmax=0.5)
+
+
+def rot3d_state(nr, nz, np_, seed=20261204):
+    """Conduction profile T = 1 - ln(r)/ln(r0) (rot3dmg.f:120, rmax = 1) plus
+    sum_{m=1..8} a_m cos(m phi + phi_m), a_m ~ U(0, 1e-3), on the interior;
+    walls T = 0 (inner) and 1 (outer); fluid at rest."""
+    import numpy as np
+    P = ROT3D_PHYS
+    r0, rmax = P["r0"], P["rmax"]
+    dr = (rmax - r0) / nr
+    i = np.arange(nr + 2)
+    r = np.where(i == 0, r0, np.where(i == nr + 1, rmax, r0 + (i - 0.5) * dr))
+    phi = 2.0 * np.pi * np.arange(np_ + 1) / np_
+    rng = np.random.default_rng(seed)
+    pert = np.zeros(np_ + 1)
+    for m in range(1, 9):
+        pert += 1e-3 * rng.uniform() * np.cos(m * phi + rng.uniform(0.0, 2.0 * np.pi))
+    th = np.broadcast_to(1.0 - np.log(r) / np.log(r0), (np_ + 1, nz + 2, nr + 2)).copy()
+    th[:, :, 1:nr + 1] += pert[:, None, None]
+    th[:, :, 0], th[:, :, nr + 1] = 0.0, 1.0
+    th[0] = th[np_]
+    return {"th": th, "thp": th.copy()}
+
+
+def rot3d_leg(cpu=True, steps=20, warmup=3):
+    """The 3D step (vorticity A/B/C, temperature, heat, three exact Poisson
+    solves, wall update) on the GPU at both grids, and the reference build
+    itself (its multigrid Poisson to the shipped absolute 1e-5) on one core
+    at its largest grid when oracle/_ref/librot3d.so is present."""
+    import numpy as np
+    from codesthesis_dir_amd.velvor3 import Annulus3D, Annulus3DConfig
+    P = ROT3D_PHYS
+    out = {"physics": "rot3dmg.f, Ra=1e7, Re=1000 (Ta=1e6), Pr=7, Gr=0, dt=1e-6, r0=0.5, rmax=1, zmax=0.5",
+           "data": "synthetic: conduction + 8 seeded azimuthal modes (amplitude 1e-3), fluid at rest"}
+    for name, (nr, nz, np_) in ROT3D.items():
+        run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, **P))
+        for k, v in rot3d_state(nr, nz, np_).items():
+            run.put(k, v)
+        run.step(warmup)
+        n = steps if nr >= 128 else 5 * steps
+        t0 = time.perf_counter()
+        t1 = run.step(n)
+        el = time.perf_counter() - t0
+        run.close()
+        out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
+    lib = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
+    if cpu and os.path.exists(lib):
+        nr, nz, np_ = ROT3D["rot3d-33x33x161"]
+        with tempfile.TemporaryDirectory() as d:
+            st = rot3d_state(nr, nz, np_)
+            np.savez(os.path.join(d, "s.npz"), nr=nr, nz=nz, np=np_, **P, f_th=st["th"], f_thp=st["thp"])
+            res = os.path.join(d, "o.json")
+            subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "refdrv3.py"), os.path.join(d, "s.npz"),
+                            res, "8", "10"], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+            r = json.load(open(res))
+        out["cpu_reference_33x33x161"] = {
+            "value": r["steps"] / r["seconds"], "unit": "time-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{r['steps']} steps of rot3dmg.f compiled as-is (flang -O2), its {r['nlev']}-level "
+                      "multigrid to the shipped absolute residual 1e-5, from the same start"}
+        g = out["rot3d-33x33x161"]["steps_per_s"]
+        out["gpu_over_cpu_reference_33x33x161"] = g / out["cpu_reference_33x33x161"]["value"]
+    return out
+
+
 def cpu_leg(name, budget_s=15.0):
     """Time the reference CPU path on this host for a bounded sample."""
     desc, kw, seed, amp = WORKLOADS[name]
@@ -256,6 +327,7 @@ def main():
     if not a.no_extra:
         line["alternatives"] = {"poisson_direct": direct_leg(a.workload, a.steps, a.warmup)}
         line["vcycle_large"] = vcycle_large_leg()
+        line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.workload)
     print(json.dumps(line))

@@ -1,4 +1,5 @@
-"""In-process driver for the 3D reference build oracle/_ref/librot3d.so.
+"""In-process driver for the 3D reference build oracle/_ref/librot3d.so
+(test and CPU-baseline infrastructure; no product code imports it).
 
 librot3d.so is rot3dmg.f compiled unmodified by oracle/Makefile from the
 reference sources (only in the container that holds them).  Its COMMON
@@ -112,3 +113,68 @@ class Reference3:
         fn = {1: "relax1_", 2: "relax2_", 3: "relax3_"}[which]
         getattr(self.L, fn)(ctypes.byref(nrel), ctypes.byref(klev), ctypes.byref(nconv), ctypes.byref(nlev))
         return self.f["psif"].copy(), self.f["rc"].copy()
+
+    def poissoninit(self, p) -> int:
+        """poissoninit (rot3dmg.f:1153): the multigrid levels; returns nlev."""
+        nlev = ctypes.c_int(0)
+        ni, nz, np_ = ctypes.c_int(p.nr), ctypes.c_int(p.nz), ctypes.c_int(p.np)
+        dr, dz, dp = ctypes.c_double(p.dr), ctypes.c_double(p.dz), ctypes.c_double(p.dp)
+        self.L.poissoninit_(ctypes.byref(ni), ctypes.byref(nz), ctypes.byref(np_), ctypes.byref(dr),
+                            ctypes.byref(dz), ctypes.byref(dp), ctypes.byref(nlev))
+        return nlev.value
+
+    def step(self, p, nlev) -> float:
+        """One pass of the reference time loop (rot3dmg.f:241-253) with the
+        reference's own multigrid solves; returns trans1."""
+        for rt in ("avorticity", "bvorticity", "cvorticity", "temperature"):
+            self.call(rt)
+        t = self.heat()
+        refs = lambda: [ctypes.byref(ctypes.c_int(p.nr)), ctypes.byref(ctypes.c_int(p.nz)),
+                        ctypes.byref(ctypes.c_int(p.np)), ctypes.byref(ctypes.c_double(p.dr)),
+                        ctypes.byref(ctypes.c_double(p.dz)), ctypes.byref(ctypes.c_double(p.dp)),
+                        ctypes.byref(ctypes.c_int(nlev))]
+        for which, (psi, psip, rhs) in ((1, ("wel", "welp", "wrh")), (2, ("vel", "velp", "vrh")),
+                                        (3, ("uel", "uelp", "urh"))):
+            self.rhs(which)
+            getattr(self.L, f"poisson{which}_")(self.f[psi].ctypes.data_as(_D), self.f[psip].ctypes.data_as(_D),
+                                                 self.f[rhs].ctypes.data_as(_D), *refs())
+        self.call("update")
+        return t
+
+
+def time_reference_steps(state_npz: str, out_json: str, nsteps: int, budget_s: float) -> None:
+    """Times the reference build's time step on the state saved in
+    state_npz (scalars nr, nz, np, dt, Ra, Re, Ptl, Gr, r0, rmax, zmax and
+    the fields) and writes {"steps", "seconds", "trans1"} to out_json.  The
+    reference prints to stdout, so this runs in a process of its own."""
+    import json
+    import oracle3 as O3
+    z = np.load(state_npz)
+    nr, nz, np_ = int(z["nr"]), int(z["nz"]), int(z["np"])
+    run = O3.Run3(nr, nz, np_, float(z["dt"]), float(z["Ra"]), float(z["Re"]), float(z["Ptl"]), float(z["Gr"]),
+                  r0=float(z["r0"]), rmax=float(z["rmax"]), zmax=float(z["zmax"]))
+    ref = Reference3()
+    ref.setup(run.p)
+    K = np_ + 1
+    for n in O3.FIELDS:
+        ref.f[n][...] = 0.0
+        if "f_" + n in z:
+            ref.f[n][:K, :nz + 2, :nr + 2] = z["f_" + n]
+    nlev = ref.poissoninit(run.p)
+    import time
+    n, t = 0, 0.0
+    t0 = time.perf_counter()
+    while n < nsteps:
+        t = ref.step(run.p, nlev)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    with open(out_json, "w") as f:
+        json.dump({"steps": n, "seconds": el, "trans1": t, "nlev": nlev}, f)
+
+
+if __name__ == "__main__":
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    time_reference_steps(sys.argv[1], sys.argv[2], int(sys.argv[3]), float(sys.argv[4]))
