@@ -36,6 +36,40 @@ def build_dropin() -> str:
     return DROPIN_LIB
 
 
+DROPIN3_SRC = os.path.join(PKG, "fortran", "vv_dropin3.F90")
+DROPIN3_LIB = os.path.join(PKG, "libvvdropin3.so")
+
+
+def build_dropin3() -> str:
+    """The 3D Fortran drop-in (rot3dmg.f routine names over include/vvmi3.h)
+    as a shared library next to libvvmi.so; its routine names overlap the
+    2D drop-in's, so it is a library of its own."""
+    mod = os.path.join(PKG, "fortran", "_build", "d3")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-I" + os.path.join(PKG, "fortran"),
+           "-o", DROPIN3_LIB + ".tmp", DROPIN3_SRC, "-L" + PKG, "-lvvmi", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
+    os.replace(DROPIN3_LIB + ".tmp", DROPIN3_LIB)
+    return DROPIN3_LIB
+
+
+def build_dropin3_driver(root: str) -> str:
+    """Test driver (tests/fortran/vv3_driver.f90) linked against the 3D
+    drop-in."""
+    src = os.path.join(root, "tests", "fortran", "vv3_driver.f90")
+    out_dir = os.path.join(root, "tests", "fortran", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "vv3_driver")
+    cmd = [FLANG, "-O2", "-module-dir", out_dir, "-o", exe, src, "-L" + PKG, "-lvvdropin3",
+           "-lvvmi", "-Wl,-rpath,$ORIGIN/../../../codesthesis_dir_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
+    return exe
+
+
 def build_dropin_driver(root: str) -> str:
     """Test driver (tests/fortran/vv_driver.f90) linked against the drop-in."""
     src = os.path.join(root, "tests", "fortran", "vv_driver.f90")
@@ -65,3 +99,4 @@ def build_library(verbose: bool = False) -> str:
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_dropin())
+    print(build_dropin3())

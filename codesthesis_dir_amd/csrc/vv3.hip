@@ -597,9 +597,10 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
     r->device = device;
     r->nr = cfg->nr; r->nz = cfg->nz; r->np = cfg->np;
     r->ldi = cfg->nr + 2; r->ldj = cfg->nz + 2;
-    r->dr = (cfg->rmax - cfg->r0) / cfg->nr;
-    r->dz = cfg->zmax / cfg->nz;
-    r->dp = 2.0 * std::acos(-1.0) / cfg->np;           // rot3dmg.f:91-93
+    // rot3dmg.f:91-93, unless the caller passes its own values
+    r->dr = (cfg->dr > 0.0) ? cfg->dr : (cfg->rmax - cfg->r0) / cfg->nr;
+    r->dz = (cfg->dz > 0.0) ? cfg->dz : cfg->zmax / cfg->nz;
+    r->dp = (cfg->dp > 0.0) ? cfg->dp : 2.0 * std::acos(-1.0) / cfg->np;
     r->nf = (size_t)r->ldi * r->ldj * (cfg->np + 1);
     auto fail = [&](int s) { vv3_destroy(r); return s; };
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return fail(VV_EHIP);
@@ -632,22 +633,49 @@ int vv3_destroy(vv3_run *r)
     return VV_OK;
 }
 
-int vv3_put(vv3_run *r, int field, const double *host)
+// host (ldi x ldj x (np+1)) <-> device ((nr+2) x (nz+2) x (np+1))
+static int copy3(vv3_run *r, int field, double *host, int ldi, int ldj, bool to_dev)
 {
-    if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vv3_put: bad argument"); return VV_EINVAL; }
+    const int di = r->ldi, dj = r->ldj, nk = r->np + 1;
+    if (ldi == 0) ldi = di;
+    if (ldj == 0) ldj = dj;
+    if (ldi < di || ldj < dj) {
+        vv::set_error("vv3_put/vv3_get: ldi must be >= nr+2 and ldj >= nz+2");
+        return VV_EINVAL;
+    }
     VV_HIP(hipSetDevice(r->device));
-    VV_HIP(hipMemcpyAsync(r->f[field], host, r->nf * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    double *dev = r->f[field];
+    const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (ldi == di && ldj == dj) {
+        if (to_dev) VV_HIP(hipMemcpyAsync(dev, host, r->nf * sizeof(double), kind, r->stream));
+        else VV_HIP(hipMemcpyAsync(host, dev, r->nf * sizeof(double), kind, r->stream));
+    } else {
+        // one pitched copy per k plane (rows of nr+2 doubles)
+        for (int k = 0; k < nk; ++k) {
+            double *h = host + (size_t)k * ldi * ldj;
+            double *d = dev + (size_t)k * di * dj;
+            if (to_dev)
+                VV_HIP(hipMemcpy2DAsync(d, di * sizeof(double), h, ldi * sizeof(double), di * sizeof(double), dj,
+                                        kind, r->stream));
+            else
+                VV_HIP(hipMemcpy2DAsync(h, ldi * sizeof(double), d, di * sizeof(double), di * sizeof(double), dj,
+                                        kind, r->stream));
+        }
+    }
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
 
-int vv3_get(vv3_run *r, int field, double *host)
+int vv3_put(vv3_run *r, int field, const double *host, int ldi, int ldj)
+{
+    if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vv3_put: bad argument"); return VV_EINVAL; }
+    return copy3(r, field, const_cast<double *>(host), ldi, ldj, true);
+}
+
+int vv3_get(vv3_run *r, int field, double *host, int ldi, int ldj)
 {
     if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vv3_get: bad argument"); return VV_EINVAL; }
-    VV_HIP(hipSetDevice(r->device));
-    VV_HIP(hipMemcpyAsync(host, r->f[field], r->nf * sizeof(double), hipMemcpyDeviceToHost, r->stream));
-    VV_HIP(hipStreamSynchronize(r->stream));
-    return VV_OK;
+    return copy3(r, field, host, ldi, ldj, false);
 }
 
 static Fields fields(vv3_run *r)

@@ -23,7 +23,8 @@ class VV3Config(ctypes.Structure):
     _fields_ = [("nr", ctypes.c_int), ("nz", ctypes.c_int), ("np", ctypes.c_int),
                 ("dt", ctypes.c_double), ("r0", ctypes.c_double), ("rmax", ctypes.c_double),
                 ("zmax", ctypes.c_double), ("Ra", ctypes.c_double), ("Re", ctypes.c_double),
-                ("Ptl", ctypes.c_double), ("Gr", ctypes.c_double)]
+                ("Ptl", ctypes.c_double), ("Gr", ctypes.c_double), ("dr", ctypes.c_double),
+                ("dz", ctypes.c_double), ("dp", ctypes.c_double)]
 
 
 @dataclass
@@ -43,7 +44,7 @@ class Annulus3DConfig:
 
     def c(self) -> VV3Config:
         return VV3Config(self.nr, self.nz, self.np, self.dt, self.r0, self.rmax, self.zmax,
-                         self.Ra, self.Re, self.Ptl, self.Gr)
+                         self.Ra, self.Re, self.Ptl, self.Gr, 0.0, 0.0, 0.0)
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -56,8 +57,8 @@ def _bind():
     vp = ctypes.c_void_p
     L.vv3_create.argtypes = [ctypes.POINTER(VV3Config), ctypes.c_int, ctypes.POINTER(vp)]
     L.vv3_destroy.argtypes = [vp]
-    L.vv3_put.argtypes = [vp, ctypes.c_int, _P]
-    L.vv3_get.argtypes = [vp, ctypes.c_int, _P]
+    L.vv3_put.argtypes = [vp, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]
+    L.vv3_get.argtypes = [vp, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]
     L.vv3_vorticity.argtypes = [vp, ctypes.c_int]
     L.vv3_temperature.argtypes = [vp]
     L.vv3_heat.argtypes = [vp, _P]
@@ -96,11 +97,11 @@ class Annulus3D:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.shape != self.shape:
             raise ValueError(f"{name}: shape {a.shape}, expected {self.shape}")
-        check(self._L.vv3_put(self._h, FIELD_IDS[name], a.ctypes.data_as(_P)), "vv3_put")
+        check(self._L.vv3_put(self._h, FIELD_IDS[name], a.ctypes.data_as(_P), 0, 0), "vv3_put")
 
     def get(self, name: str) -> np.ndarray:
         out = np.empty(self.shape)
-        check(self._L.vv3_get(self._h, FIELD_IDS[name], out.ctypes.data_as(_P)), "vv3_get")
+        check(self._L.vv3_get(self._h, FIELD_IDS[name], out.ctypes.data_as(_P), 0, 0), "vv3_get")
         return out
 
     # rot3dmg.f routine names

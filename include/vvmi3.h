@@ -24,6 +24,9 @@ typedef struct {
     double dt;
     double r0, rmax, zmax;      /* inner and outer radius, height            */
     double Ra, Re, Ptl, Gr;     /* Rayleigh, Reynolds, Prandtl, Grashof      */
+    double dr, dz, dp;          /* grid spacings as the caller computed them
+                                   (COMMON /num/); 0 derives them as
+                                   rot3dmg.f:91-93 does                       */
 } vv3_config;
 
 enum vv3_field {
@@ -40,8 +43,12 @@ enum vv3_field {
 int vv3_create(const vv3_config *cfg, int device, vv3_run **out);
 int vv3_destroy(vv3_run *run);
 
-int vv3_put(vv3_run *run, int field, const double *host);
-int vv3_get(vv3_run *run, int field, double *host);
+/* whole-field copies; the host array is X(i,j,k) at
+ * host[(k*ldj + j)*ldi + i] (ldi >= nr+2, ldj >= nz+2; 0 selects nr+2 / nz+2),
+ * so a COMMON array of rot3dmg.f, real*8 X(0:kr,0:kz,0:kp), is passed with
+ * ldi = kr+1, ldj = kz+1 (rot3dmg.f:23-29) */
+int vv3_put(vv3_run *run, int field, const double *host, int ldi, int ldj);
+int vv3_get(vv3_run *run, int field, double *host, int ldi, int ldj);
 
 /* Avorticity, Bvorticity, Cvorticity (rot3dmg.f:341, 477, 601): An, Bn, Cn
  * on their interior points, with the hybrid upwind switch of the reference.

@@ -61,3 +61,14 @@ def test_fortran_dropin_exports_reference_routine_names():
     for n in ("poissoninit_", "poisson2_", "poisson3_", "cvorticity1_", "cvorticity_",
               "temperature1_", "temperature_", "heat_", "rhsv_", "rhsu_", "update_"):
         assert hasattr(lib, n), n
+
+
+def test_fortran_dropin3_exports_reference_routine_names():
+    """libvvdropin3.so exports rot3dmg.f's routine names."""
+    path = os.path.join(ROOT, "codesthesis_dir_amd", "libvvdropin3.so")
+    if not os.path.exists(path):
+        pytest.skip("libvvdropin3.so not built")
+    lib = ctypes.CDLL(path)
+    for n in ("poissoninit_", "poisson1_", "poisson2_", "poisson3_", "avorticity_", "bvorticity_",
+              "cvorticity_", "temperature_", "heat_", "rhsw_", "rhsv_", "rhsu_", "update_"):
+        assert hasattr(lib, n), n

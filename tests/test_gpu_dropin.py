@@ -43,3 +43,44 @@ def test_fortran_dropin_time_loop():
         print(f"drop-in {name}: rel L2 {err:.3e}")
         assert err <= 1e-9, (name, err)
     assert np.max(np.abs(trans - np.array(tr_o)) / np.abs(np.array(tr_o))) < 1e-10
+
+
+DRIVER3 = os.path.join(ROOT, "tests", "fortran", "_build", "vv3_driver")
+
+
+@pytest.mark.gpu
+def test_fortran_dropin3_time_loop():
+    """The 3D drop-in (vv_dropin3.F90): a Fortran program with rot3dmg.f's
+    COMMON blocks (kr = kz = 33, kp = 160) replays the reference's time loop
+    (rot3dmg.f:241-253) through the routine names; the fields match the
+    oracle's routines with exact solves."""
+    import oracle3 as O3
+    from test_gpu_3d import oracle_step
+    assert os.path.exists(DRIVER3), "build the drop-in first (__graft_entry__.build())"
+    nr, nz, np_, nts = 12, 10, 16, 4
+    p = dict(dt=1e-4, Ra=1e5, Re=50.0, Ptl=1.0, Gr=3.0)
+    ora = O3.Run3(nr, nz, np_, **p)
+    rng = np.random.default_rng(8)
+    th0 = np.broadcast_to(np.linspace(0.0, 1.0, nr + 2), ora.shape).copy()
+    th0 += 0.05 * rng.standard_normal(ora.shape)
+    th0[:, :, 0], th0[:, :, -1] = 0.0, 1.0
+    th0[0] = th0[-1]
+    ora["th"], ora["thp"] = th0, th0
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "th0.bin"), os.path.join(d, "out.bin")
+        np.ascontiguousarray(th0).tofile(fin)
+        deck = f"{nr} {nz} {np_} {nts} {p['dt']} {p['Ra']} {p['Re']} {p['Ptl']} {p['Gr']}\n{fin}\n{fout}\n"
+        print(f"[gpu] fortran 3D drop-in driver nr={nr} nz={nz} np={np_} steps={nts}", flush=True)
+        res = subprocess.run([DRIVER3], input=deck, text=True, capture_output=True, timeout=240)
+        assert res.returncode == 0, res.stdout + res.stderr
+        raw = np.fromfile(fout, dtype=np.float64)
+    n = int(np.prod(ora.shape))
+    th, uel, vel, wel = (raw[j * n:(j + 1) * n].reshape(ora.shape) for j in range(4))
+    trans = raw[4 * n:]
+    tr_o = [oracle_step(ora) for _ in range(nts)]
+    for name, g in (("th", th), ("uel", uel), ("vel", vel), ("wel", wel)):
+        o = ora[name]
+        err = np.max(np.abs(g - o)) / np.max(np.abs(o))
+        print(f"3D drop-in {name}: rel max {err:.3e}")
+        assert err <= 1e-10, (name, err)
+    assert np.max(np.abs(trans - np.array(tr_o)) / np.abs(np.array(tr_o))) < 1e-10
