@@ -1,6 +1,7 @@
 """Benchmark of the velocity-vorticity time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py --rot3d-only [--no-cpu]      # the 3D step (rot3dmg.f) alone
 
 A step is one pass of the time loop (compact-scheme DuFort-Frankel step of
 vorticity and temperature, both multigrid Poisson solves to 1e-10 relative
@@ -10,6 +11,10 @@ resident in HBM before it starts.  A second pass of K steps with per-kernel
 HIP events gives the dominant kernel's average duration and its achieved
 algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
 reference's CPU path on a bounded sample of the same workload on this host.
+Unless --no-extra is given, the line also carries the direct-solve
+alternative, a large V-cycle measurement and the 3D step of rot3dmg.f
+("rot3d": the reference program's largest grid and config 3's grid, with the
+reference build's own step timed on one core).
 
 Prints one JSON line.
 """
@@ -157,6 +162,43 @@ def rot3d_state(nr, nz, np_, seed=20261204):
     return {"th": th, "thp": th.copy()}
 
 
+def rot3d_routines(nr, nz, np_, reps=10):
+    """Per-routine device time at one grid: each routine called `reps` times
+    back to back (each call synchronises), with its algorithmic bytes (the
+    fields it reads and writes) or, for the solves, the transform flops
+    (four dense transforms: 2 np^2 m each along phi, 2 nz^2 nr np each
+    along z)."""
+    from codesthesis_dir_amd.velvor3 import Annulus3D, Annulus3DConfig
+    P = ROT3D_PHYS
+    run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, **P))
+    for k, v in rot3d_state(nr, nz, np_).items():
+        run.put(k, v)
+    run.step(2)
+    N = (nr + 2) * (nz + 2) * (np_ + 1)
+    words = {"Avorticity": 9, "Bvorticity": 9, "Cvorticity": 9, "temperature": 6, "rhsw": 3, "rhsv": 3, "rhsu": 4}
+    out = {}
+    for name, w in words.items():
+        f = getattr(run, name)
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        out[name] = {"ms": ms, "GB_s": 8.0 * w * N / (ms * 1e-3) / 1e9}
+    m = nr * nz
+    for which in (1, 2, 3):
+        f = getattr(run, f"poisson{which}")
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        flops = 2 * (2.0 * np_ * np_ * m) + 2 * (2.0 * nz * nz * nr * np_)
+        out[f"poisson{which}"] = {"ms": ms, "transform_TFLOP_s": flops / (ms * 1e-3) / 1e12}
+    run.close()
+    return out
+
+
 def rot3d_leg(cpu=True, steps=20, warmup=3):
     """The 3D step (vorticity A/B/C, temperature, heat, three exact Poisson
     solves, wall update) on the GPU at both grids, and the reference build
@@ -178,6 +220,8 @@ def rot3d_leg(cpu=True, steps=20, warmup=3):
         el = time.perf_counter() - t0
         run.close()
         out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
+        if nr >= 128:
+            out[name]["routines"] = rot3d_routines(nr, nz, np_)
     lib = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
     if cpu and os.path.exists(lib):
         nr, nz, np_ = ROT3D["rot3d-33x33x161"]
@@ -278,6 +322,8 @@ def main():
     ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the direct-solve and large V-cycle measurements")
+    ap.add_argument("--rot3d-only", action="store_true",
+                    help="measure only the 3D step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -285,6 +331,9 @@ def main():
         print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; the multi-GPU layout "
               "is described in DESIGN.md and not implemented", file=sys.stderr)
         sys.exit(2)
+    if a.rot3d_only:
+        print(json.dumps(rot3d_leg(cpu=not a.no_cpu)))
+        return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
     cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu)

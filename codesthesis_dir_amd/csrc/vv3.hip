@@ -422,8 +422,10 @@ __global__ void __launch_bounds__(WG) k3_heat_terms(Geo g, const double *th, dou
 }
 
 // the sums in the reference's order: trans per k over j, then trans1 over k
-__global__ void __launch_bounds__(WG) k3_heat_sum(Geo g, const double *hs, double *part, double *out)
+// (the per-k sums kept in LDS, np doubles of dynamic shared memory)
+__global__ void __launch_bounds__(WG) k3_heat_sum(Geo g, const double *hs, double *out)
 {
+    extern __shared__ double part[];
     const int nz = g.nz, np = g.np;
     for (int k = threadIdx.x; k < np; k += blockDim.x) {
         double trans = hs[k];
@@ -441,7 +443,15 @@ __global__ void __launch_bounds__(WG) k3_heat_sum(Geo g, const double *hs, doubl
     __syncthreads();
     if (threadIdx.x == 0) {
         double trans1 = 0.0;
-        for (int k = 0; k < np; ++k) trans1 = trans1 + part[k] * g.dp;
+        int k = 0;
+        for (; k + 8 <= np; k += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[k + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) trans1 = trans1 + v[u] * g.dp;
+        }
+        for (; k < np; ++k) trans1 = trans1 + part[k] * g.dp;
         *out = trans1;
     }
 }
@@ -582,8 +592,8 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
     using vv::set_error;
     if (!cfg || !out) { set_error("vv3_create: null argument"); return VV_EINVAL; }
     *out = nullptr;
-    if (cfg->nr < 4 || cfg->nz < 4 || cfg->np < 4 || (cfg->np & 1)) {
-        set_error("vv3_create: need nr, nz >= 4 and an even np >= 4");
+    if (cfg->nr < 4 || cfg->nz < 4 || cfg->np < 4 || (cfg->np & 1) || cfg->np > 8192) {
+        set_error("vv3_create: need nr, nz >= 4 and an even np, 4 <= np <= 8192");
         return VV_EINVAL;
     }
     if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->r0 > 0) || !(cfg->zmax > 0) || !(cfg->Ptl > 0) ||
@@ -695,10 +705,11 @@ static double *heat_total(vv3_run *r) { return r->heat + (size_t)(r->nz + 3) * r
 static int launch_heat(vv3_run *r)
 {
     const Geo g = geo(r);
-    double *hs = r->heat, *part = r->heat + (size_t)(r->nz + 2) * r->np;
+    double *hs = r->heat;
     hipLaunchKernelGGL(k3_heat_terms, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, r->stream, g,
                        (const double *)r->f[VV3_TH], hs);
-    hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), 0, r->stream, g, (const double *)hs, part, heat_total(r));
+    hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), r->np * sizeof(double), r->stream, g, (const double *)hs,
+                       heat_total(r));
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

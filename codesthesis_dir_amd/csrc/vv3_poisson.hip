@@ -14,7 +14,7 @@
 //   4. x = V y along j, the inverse DFT into psi.
 // The result is the exact solution of the reference's discrete equations to
 // rounding (the reference iterates multigrid V-cycles to an absolute
-// residual of 1e-5, rot3dmg.f:1548).
+// residual of 1e-5, tolcon at rot3dmg.f:1562).
 
 #include "vv3_internal.h"
 
@@ -27,54 +27,80 @@ namespace vv3 {
 
 namespace {
 
-constexpr int GT = 64;    // GEMM output tile (rows and columns)
 constexpr int GK = 16;    // GEMM k step
-constexpr int GW = 256;   // GEMM workgroup
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// C[q] = A[q] * B[q] (FP64, exact products and sums, no reordering beyond
-// the MFMA accumulation), operands reached through the functor:
-// op.a(q, m, k), op.b(q, k, n) load, op.c(q, m, n, v) stores; C is written
-// with consecutive n in consecutive lanes.  A workgroup computes a 64 x 64
-// tile, each wave a 32 x 32 quarter as 2 x 2 v_mfma_f64_16x16x4f64 blocks.
-// MFMA lane layout: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
-// element e of D in lane l is D[l/16 + 4e][l%16].  AKFAST / BKFAST: the
-// tile loads run along k (operand contiguous in k) rather than along m / n.
-template <class Op, bool AKFAST, bool BKFAST>
-__global__ void __launch_bounds__(GW) k3_gemm(Op op)
+// C[q] = A[q] * B[q] in FP64 on the matrix cores, operands reached through
+// the functor: op.a(q, m, k), op.b(q, k, n) load, op.c(q, m, n, v) stores; C
+// is written with consecutive n in consecutive lanes.  A workgroup of WAVES
+// waves computes a (32 WR) x (32 WC) tile (WR x WC = WAVES), each wave a
+// 32 x 32 block as 2 x 2 v_mfma_f64_16x16x4f64 products.  MFMA lane layout:
+// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, element e of D in
+// lane l is D[l/16 + 4e][l%16].  The next k step's operands are loaded into
+// registers while the current one is multiplied.  AKFAST / BKFAST: the tile
+// loads run along k (operand contiguous in k) rather than along m / n.
+template <class Op, bool AKFAST, bool BKFAST, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k3_gemm(Op op)
 {
-    __shared__ double As[GK][GT];
-    __shared__ double Bs[GK][GT];
+    constexpr int WR = (WAVES == 4) ? 2 : 1, WC = WAVES / WR;
+    constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WAVES;
+    constexpr int LA = GK * TM / NT, LB = GK * TN / NT;   // tile elements per thread
+    // rows padded by one double: k-fast tile stores (consecutive threads one
+    // row apart) then fall on different banks
+    __shared__ double As[GK][TM + 1];
+    __shared__ double Bs[GK][TN + 1];
     // logical tile order: m tiles fastest, then n tiles, then the batch, so
     // the m tiles that share a B tile run together on one XCD
     const unsigned nmt = gridDim.y, nnt = gridDim.x;
     const unsigned lb = xcd_block((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
                                   gridDim.x * gridDim.y * gridDim.z);
     const int q = (int)(lb / (nmt * nnt));
-    const int m0 = (int)(lb % nmt) * GT, n0 = (int)((lb / nmt) % nnt) * GT;
+    const int m0 = (int)(lb % nmt) * TM, n0 = (int)((lb / nmt) % nnt) * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int wr = (wave / WC) * 32, wc = (wave % WC) * 32;
     const int li = lane & 15, lk = lane >> 4;
     d4 acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < op.K; k0 += GK) {
+    double ra[LA], rb[LB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = tid + GW * t;
-            const int ka = AKFAST ? (e % GK) : (e / GT);
-            const int ma = AKFAST ? (e / GK) : (e % GT);
+        for (int t = 0; t < LA; ++t) {
+            const int e = tid + NT * t;
+            const int ka = AKFAST ? (e % GK) : (e / TM);
+            const int ma = AKFAST ? (e / GK) : (e % TM);
             const int m = m0 + ma, k = k0 + ka;
-            As[ka][ma] = (m < op.M && k < op.K) ? op.a(q, m, k) : 0.0;
-            const int kb = BKFAST ? (e % GK) : (e / GT);
-            const int nb = BKFAST ? (e / GK) : (e % GT);
-            const int n = n0 + nb, k2 = k0 + kb;
-            Bs[kb][nb] = (n < op.N && k2 < op.K) ? op.b(q, k2, n) : 0.0;
+            ra[t] = (m < op.M && k < op.K) ? op.a(q, m, k) : 0.0;
         }
+#pragma unroll
+        for (int t = 0; t < LB; ++t) {
+            const int e = tid + NT * t;
+            const int kb = BKFAST ? (e % GK) : (e / TN);
+            const int nb = BKFAST ? (e / GK) : (e % TN);
+            const int n = n0 + nb, k = k0 + kb;
+            rb[t] = (n < op.N && k < op.K) ? op.b(q, k, n) : 0.0;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int t = 0; t < LA; ++t) {
+            const int e = tid + NT * t;
+            As[AKFAST ? (e % GK) : (e / TM)][AKFAST ? (e / GK) : (e % TM)] = ra[t];
+        }
+#pragma unroll
+        for (int t = 0; t < LB; ++t) {
+            const int e = tid + NT * t;
+            Bs[BKFAST ? (e % GK) : (e / TN)][BKFAST ? (e / GK) : (e % TN)] = rb[t];
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < op.K; k0 += GK) {
+        stash();
         __syncthreads();
+        if (k0 + GK < op.K) fetch(k0 + GK);
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4) {
             double a[2], b[2];
@@ -159,11 +185,19 @@ struct PhiInv {
     }
 };
 
+// 64 x 64 tiles of four waves when they fill the chip, else 32 x 32 tiles of
+// one wave (four times the workgroups for small planes)
 template <bool AKFAST, bool BKFAST, class Op>
 void gemm(const Op &op, int batch, hipStream_t s)
 {
-    dim3 grid((op.N + GT - 1) / GT, (op.M + GT - 1) / GT, batch);
-    hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST>), grid, dim3(GW), 0, s, op);
+    const long long big = (long long)((op.N + 63) / 64) * ((op.M + 63) / 64) * batch;
+    if (big >= 256) {
+        dim3 grid((op.N + 63) / 64, (op.M + 63) / 64, batch);
+        hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST, 4>), grid, dim3(256), 0, s, op);
+    } else {
+        dim3 grid((op.N + 31) / 32, (op.M + 31) / 32, batch);
+        hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST, 1>), grid, dim3(64), 0, s, op);
+    }
 }
 
 // psip <- psi on the range the reference's routine writes, and psi <- 0 on

@@ -11,7 +11,7 @@
 ! COMMON is the same as with the reference routines.  poissoninit creates
 ! the device run from COMMON /num/ /val/ /radius/, with the caller's dr, dz,
 ! dp.  The three Poisson solves are exact (the reference iterates its
-! multigrid to an absolute residual of 1e-5, rot3dmg.f:1548).
+! multigrid to an absolute residual of 1e-5, tolcon at rot3dmg.f:1562).
 !
 ! Replaced routines (rot3dmg.f): Avorticity 341, Bvorticity 477,
 ! Cvorticity 601, temperature 736, heat 882, update 964, rhsw 1026,

@@ -39,7 +39,7 @@ enum vv3_field {
  * builds its multigrid levels in poissoninit, rot3dmg.f:1153).  The three
  * Poisson operators are solved exactly and separably: a real DFT in phi, the
  * eigenbasis of the z rows, a banded solve in r per (mode, z eigenvalue).
- * Needs nr, nz >= 4 and an even np >= 4 (VV_EINVAL otherwise). */
+ * Needs nr, nz >= 4 and an even np, 4 <= np <= 8192 (VV_EINVAL otherwise). */
 int vv3_create(const vv3_config *cfg, int device, vv3_run **out);
 int vv3_destroy(vv3_run *run);
 
@@ -65,7 +65,7 @@ int vv3_rhs(vv3_run *run, int which);
  * operator of relax1/2/3's residual (rot3dmg.f:1695, 1927, 2130) for wel /
  * vel / uel from wrh / vrh / urh, exactly; psip <- old psi, walls 0 as the
  * reference writes them.  The reference iterates to an absolute residual of
- * 1e-5 (rot3dmg.f:1548); this solve is exact to rounding. */
+ * 1e-5 (tolcon, rot3dmg.f:1562); this solve is exact to rounding. */
 int vv3_poisson(vv3_run *run, int which);
 /* update (rot3dmg.f:964): wall vorticity and wall temperatures, then the
  * time levels rotate (Ap <- A <- An, ... for A, B, C, th) */
