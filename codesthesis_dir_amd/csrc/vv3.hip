@@ -381,14 +381,27 @@ __device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int 
     return (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
 }
 
-// One routine (W = 0: A, 1: B, 2: C, 3: T) per launch, blockIdx.y = k, a
-// thread per (i, j) point of the routine's loop range.  A kernel per routine
-// keeps its constants and field pointers within the scalar registers.
+// One routine (W = 0: A, 1: B, 2: C, 3: T) per launch, or (W = 4) A, B and
+// C together, which read the same fields; blockIdx.y = k, a thread per
+// (i, j) point of the routine's loop range (for W = 4 the union i = 1..nr,
+// j = 1..nz, each routine evaluated where its own range holds).  A kernel per
+// routine set keeps its constants and field pointers within the scalar
+// registers.
 template <int W>
 __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
 {
-    constexpr int which = W;
     const int nr = g.nr, nz = g.nz;
+    if constexpr (W == 4) {
+        const int t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= nr * nz) return;
+        const int jj = t / nr;
+        const int i = 1 + (t - jj * nr), j = 1 + jj, k = blockIdx.y;
+        if (j <= nz - 1) X(F.An, i, j, k) = avort(g, K, F, i, j, k);
+        if (i <= nr - 1 && j <= nz - 1) X(F.Bn, i, j, k) = bvort(g, K, F, i, j, k);
+        if (i <= nr - 1) X(F.Cn, i, j, k) = cvort(g, K, F, i, j, k);
+        return;
+    }
+    constexpr int which = W;
     int i0 = 1, ni, j0 = 1, nj;
     if (which == 0) { ni = nr; nj = nz - 1; }
     else if (which == 1) { ni = nr - 1; nj = nz - 1; }
@@ -745,9 +758,14 @@ static int launch_explicit(vv3_run *r, int mask)
     const int np1 = r->np + 1;
     const long long pa = (long long)r->nr * (r->nz - 1), pb = (long long)(r->nr - 1) * (r->nz - 1);
     const long long pc = (long long)(r->nr - 1) * r->nz, pt = (long long)r->nr * (r->nz + 2);
-    if (mask & 1) hipLaunchKernelGGL(k3_explicit<0>, dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);
-    if (mask & 2) hipLaunchKernelGGL(k3_explicit<1>, dim3(nblk(pb, WG), np1), dim3(WG), 0, r->stream, g, K, F);
-    if (mask & 4) hipLaunchKernelGGL(k3_explicit<2>, dim3(nblk(pc, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    if ((mask & 7) == 7) {
+        const long long pu = (long long)r->nr * r->nz;
+        hipLaunchKernelGGL(k3_explicit<4>, dim3(nblk(pu, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    } else {
+        if (mask & 1) hipLaunchKernelGGL(k3_explicit<0>, dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (mask & 2) hipLaunchKernelGGL(k3_explicit<1>, dim3(nblk(pb, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (mask & 4) hipLaunchKernelGGL(k3_explicit<2>, dim3(nblk(pc, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    }
     if (mask & 8) hipLaunchKernelGGL(k3_explicit<3>, dim3(nblk(pt, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     VV_HIP(hipGetLastError());
     return VV_OK;
