@@ -22,18 +22,25 @@ DROPIN_SRC = os.path.join(PKG, "fortran", "vv_dropin.F90")
 DROPIN_LIB = os.path.join(PKG, "libvvdropin.so")
 
 
-def build_dropin() -> str:
+DROPIN_ANN_LIB = os.path.join(PKG, "libvvdropin_ann.so")
+
+
+def build_dropin(annulus: bool = False) -> str:
     """The Fortran drop-in (reference routine names over the C ABI) as a
-    shared library next to libvvmi.so."""
-    mod = os.path.join(PKG, "fortran", "_build")
+    shared library next to libvvmi.so: velvor2dperiodicrbconv.f's routines,
+    or with annulus=True velvor2dnewsch3.4.f's (same file, -DVV_GEOM=1 and
+    that program's array extents)."""
+    mod = os.path.join(PKG, "fortran", "_build", "ann" if annulus else "box")
     os.makedirs(mod, exist_ok=True)
-    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-o", DROPIN_LIB + ".tmp",
+    lib = DROPIN_ANN_LIB if annulus else DROPIN_LIB
+    defs = ["-DVV_GEOM=1", "-DVV_KR=420", "-DVV_KP=3600"] if annulus else []
+    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, *defs, "-o", lib + ".tmp",
            DROPIN_SRC, "-L" + PKG, "-lvvmi", "-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
-    os.replace(DROPIN_LIB + ".tmp", DROPIN_LIB)
-    return DROPIN_LIB
+    os.replace(lib + ".tmp", lib)
+    return lib
 
 
 DROPIN3_SRC = os.path.join(PKG, "fortran", "vv_dropin3.F90")
@@ -99,4 +106,5 @@ def build_library(verbose: bool = False) -> str:
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_dropin())
+    print(build_dropin(annulus=True))
     print(build_dropin3())

@@ -18,12 +18,22 @@
 !
 ! Knobs the reference hard-codes are set through vv_dropin_options: the
 ! periodic width (xx, line 132) and the Poisson tolerance.
+!
+! Built with -DVV_GEOM=1 (and the extents -DVV_KR=420 -DVV_KP=3600 of
+! velvor2dnewsch3.4.f:47) the same file is the drop-in for the rotating-
+! annulus plane velvor2dnewsch3.4.f: its routines have the same names and
+! arguments (Cvorticity1 596, Cvorticity 797, temperature1 1068, ...); the
+! run then takes the annulus geometry with Re from COMMON /val/ and the
+! program's Robert coefficient 0.05 (velvor2dnewsch3.4.f:1055, 1454).
 
 #ifndef VV_KR
 #define VV_KR 290
 #endif
 #ifndef VV_KP
 #define VV_KP 1610
+#endif
+#ifndef VV_GEOM
+#define VV_GEOM 0
 #endif
 
 module vv_dropin_c
@@ -154,14 +164,20 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%width = g_width
   cfg%Ra = Ra
   cfg%Pr = Ptl
-  cfg%robert = 0.02d0
   cfg%rtol = g_rtol
   cfg%max_cycles = 100
   cfg%nu1 = 2
   cfg%nu2 = 2
   cfg%stats = 0
+#if VV_GEOM == 1
+  cfg%robert = 0.05d0
+  cfg%geom = 1
+  cfg%Re = Re
+#else
+  cfg%robert = 0.02d0
   cfg%geom = 0
   cfg%Re = 0.0d0
+#endif
   cfg%direct_cols = 0
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl

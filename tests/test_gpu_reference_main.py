@@ -85,6 +85,46 @@ def test_rot3dmg_main_program_on_the_gpu():
         assert errs[n] < 1e-4, (n, errs[n])
 
 
+def _plane_main(ref_exe, gpu_exe, deck, nts, formatted=False):
+    """Runs a 2D program's executable and its main program on the GPU with
+    one deck; returns the heat-series error, the field-file errors and the
+    error of the running-mean file zmtmtempvar.dat.  The field files are
+    unformatted records, or (formatted=True) one E16.10 value per line."""
+    read = (lambda path: np.loadtxt(path)) if formatted else (lambda path: _field(path, 0))
+    env_one = dict(os.environ, OMP_NUM_THREADS="1")
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        print(f"[gpu] {os.path.basename(ref_exe)} in {dr}", flush=True)
+        res = subprocess.run([ref_exe], input=deck, text=True, capture_output=True, cwd=dr, timeout=300,
+                             env=env_one)
+        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+        out_g = _run(gpu_exe, deck, dg)
+        hr, hg = _series(res.stdout, 2), _series(out_g, 2)
+        assert hr.shape == hg.shape == (nts // 50, 2), (hr.shape, hg.shape)
+        errs = {n: _rel(read(os.path.join(dg, n)), read(os.path.join(dr, n))) for n in ("T", "C", "U", "V")}
+        mr = np.loadtxt(os.path.join(dr, "zmtmtempvar.dat"))
+        mg = np.loadtxt(os.path.join(dg, "zmtmtempvar.dat"))
+        return _rel(hg, hr), errs, _rel(mg, mr)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "ann_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_annulus_main_program_on_the_gpu():
+    """velvor2dnewsch3.4.f's main program (the rotating-annulus plane)
+    through the annulus build of the 2D drop-in against the reference
+    executable."""
+    nts = 200   # the program prints its heat fluxes every 50 steps (velvor2dnewsch3.4.f:391)
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "32", "64", str(nts), "0.5 1.0 1.0",
+                      "1e5", "100.0", "0.0", "1.0", "0.0", "0.0", "1", "no", "n"]) + "\n"
+    eh, errs, em = _plane_main(os.path.join(REF, "ann_ref"), os.path.join(REF, "ann_main_gpu"), deck, nts,
+                               formatted=True)
+    print(f"annulus main: heat series rel {eh:.3e}; fields {errs}; running means {em:.3e}")
+    assert eh < 1e-9
+    assert errs["T"] < 1e-9
+    for n in ("C", "U", "V"):
+        assert errs[n] < 1e-4, (n, errs[n])
+    assert em < 1e-4
+
+
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF, "rb_main_gpu")),
                     reason="reference main programs not linked (reference sources were absent at build)")
 def test_periodic_box_main_program_on_the_gpu():
@@ -93,21 +133,11 @@ def test_periodic_box_main_program_on_the_gpu():
     nts = 200   # the program prints its heat fluxes every 50 steps (rb.f:391)
     deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "32", "64", str(nts), "0.5 1.0 1.0",
                       "1e5", "0.0", "0.0", "1.0", "0.0", "0.0", "1", "no", "n"]) + "\n"
-    env_one = dict(os.environ, OMP_NUM_THREADS="1")
-    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
-        print(f"[gpu] rb_omp in {dr}", flush=True)
-        res = subprocess.run([os.path.join(REF, "rb_omp")], input=deck, text=True, capture_output=True, cwd=dr,
-                             timeout=300, env=env_one)
-        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
-        out_g = _run(os.path.join(REF, "rb_main_gpu"), deck, dg)
-        hr, hg = _series(res.stdout, 2), _series(out_g, 2)
-        assert hr.shape == hg.shape == (nts // 50, 2), (hr.shape, hg.shape)
-        eh = _rel(hg, hr)
-        errs = {n: _rel(_field(os.path.join(dg, n), 0), _field(os.path.join(dr, n), 0))
-                for n in ("T", "C", "U", "V")}
-    print(f"rb main: heat series rel {eh:.3e}; fields {errs}")
+    eh, errs, em = _plane_main(os.path.join(REF, "rb_omp"), os.path.join(REF, "rb_main_gpu"), deck, nts)
+    print(f"rb main: heat series rel {eh:.3e}; fields {errs}; running means {em:.3e}")
     # measured on the MI355X: heat 1.5e-12, T 4.6e-12, C, U, V <= 1.4e-7
     assert eh < 1e-9
     assert errs["T"] < 1e-9
     for n in ("C", "U", "V"):
         assert errs[n] < 1e-5, (n, errs[n])
+    assert em < 1e-5
