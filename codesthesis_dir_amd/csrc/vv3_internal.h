@@ -50,6 +50,8 @@ struct Poisson3 {
     double *S = nullptr;     // np x mmax: phi spectra, then the z-inverse result
     double *T = nullptr;     // np x mmax: z spectra, (i, l) with l fastest
     double *cp = nullptr;    // nm x mmax: elimination factors of the r solves
+    double *tw = nullptr;    // np/2 complex (cos, -sin)(2 pi t/np): FFT twiddles (np a power of two)
+    bool fft = false;        // phi transforms by FFT (np a power of two, 8..4096) instead of GEMM
 };
 
 // Host-side pieces, exposed so that they can be checked without a GPU.

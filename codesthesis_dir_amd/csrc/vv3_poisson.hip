@@ -200,6 +200,115 @@ void gemm(const Op &op, int batch, hipStream_t s)
     }
 }
 
+// ---------------------------------------------------------------- phi FFT
+// For np a power of two the phi transforms are radix-2 FFTs in LDS.  Two real
+// lines (consecutive i at one j) share one complex transform, z = a + i b;
+// Z_m and Z_{np-m} separate them afterwards.  A workgroup holds LP such pairs
+// (LP np complex values, at most 64 KB); k' = k mod np indexes the line, so
+// k = np is position 0.  Stage s pairs positions p and p + h (h = 2^s) with
+// the twiddle e^{-i 2 pi (p mod h)/(2h)} (conjugated for the inverse).
+
+__device__ __forceinline__ unsigned bitrev(unsigned x, int bits) { return __builtin_bitreverse32(x) >> (32 - bits); }
+
+template <bool INV>
+__device__ void fft_stages(double2 *z, const double2 *tw, int np, int bits, int LP)
+{
+    for (int s = 0; s < bits; ++s) {
+        const int h = 1 << s;
+        const int nb = LP * (np >> 1);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int pr = b / (np >> 1), t = b - pr * (np >> 1);
+            const int pos = t & (h - 1), grp = t >> s;
+            const int i0 = pr * np + grp * 2 * h + pos, i1 = i0 + h;
+            double2 w = tw[pos << (bits - 1 - s)];
+            if (INV) w.y = -w.y;
+            const double2 u = z[i0], v = z[i1];
+            const double2 vw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+            z[i0] = make_double2(u.x + vw.x, u.y + vw.y);
+            z[i1] = make_double2(u.x - vw.x, u.y - vw.y);
+        }
+        __syncthreads();
+    }
+}
+
+// forward: S[m][u] = sum_k rhs(i,j,k) cos(2 pi m k/np) (m = 0..np/2) and
+// S[np/2+m][u] = sum_k rhs(i,j,k) sin(...) (0 < m < np/2), u = (j-1) i1 + (i-1)
+__global__ void __launch_bounds__(256) k3_fft_fwd(const double *rhs, double *S, const double2 *tw, int np, int bits,
+                                                   int LP, int i1, int j1, int ldi, int ldj, int mt)
+{
+    extern __shared__ double2 z[];
+    const int j = blockIdx.y, ib = blockIdx.x * 2 * LP, nl = 2 * LP;
+    for (int e = threadIdx.x; e < nl * np; e += blockDim.x) {
+        const int l = e % nl, kq = e / nl;
+        const int i = ib + l, k = (kq == 0) ? np : kq;
+        const double v = (i < i1) ? rhs[((size_t)k * ldj + (size_t)(j + 1)) * ldi + (size_t)(i + 1)] : 0.0;
+        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * np + bitrev(kq, bits)]);
+        zz[l & 1] = v;
+    }
+    __syncthreads();
+    fft_stages<false>(z, tw, np, bits, LP);
+    const int half = np >> 1;
+    for (int e = threadIdx.x; e < nl * (half + 1); e += blockDim.x) {
+        const int l = e % nl, m = e / nl;
+        const int i = ib + l;
+        if (i >= i1) continue;
+        const double2 zm = z[(l >> 1) * np + m], zn = z[(l >> 1) * np + ((np - m) & (np - 1))];
+        // A = (Z_m + conj Z_{np-m})/2, B = (Z_m - conj Z_{np-m})/(2i)
+        double re, im;
+        if ((l & 1) == 0) { re = 0.5 * (zm.x + zn.x); im = 0.5 * (zm.y - zn.y); }
+        else { re = 0.5 * (zm.y + zn.y); im = -0.5 * (zm.x - zn.x); }
+        const size_t u = (size_t)j * i1 + i;
+        S[(size_t)m * mt + u] = re;                                  // sum x cos
+        if (m > 0 && m < half) S[(size_t)(half + m) * mt + u] = -im;  // sum x sin
+    }
+}
+
+// inverse: psi(i,j,k) = (1/np) sum_m w_m (C_m cos + S_m sin), k = 1..np, and
+// k = 0 copied from k = np
+__global__ void __launch_bounds__(256) k3_fft_inv(const double *S, double *psi, const double2 *tw, int np, int bits,
+                                                   int LP, int i1, int j1, int ldi, int ldj, int mt)
+{
+    extern __shared__ double2 z[];
+    const int j = blockIdx.y, ib = blockIdx.x * 2 * LP, nl = 2 * LP, half = np >> 1;
+    // Z_m = X_m + i Y_m with X_m = C_m - i S_m (m <= np/2), X_{np-m} = conj X_m
+    for (int e = threadIdx.x; e < LP * np; e += blockDim.x) {
+        const int pr = e % LP, m = e / LP;   // neighbouring threads: neighbouring lines, one spectral row
+        const int mm = (m <= half) ? m : np - m;
+        const double sg = (m <= half) ? -1.0 : 1.0;   // imaginary part of X_m is sg * S_mm
+        double xr = 0, xi = 0, yr = 0, yi = 0;
+        for (int h2 = 0; h2 < 2; ++h2) {
+            const int i = ib + 2 * pr + h2;
+            if (i >= i1) continue;
+            const size_t u = (size_t)j * i1 + i;
+            const double c = S[(size_t)mm * mt + u];
+            const double sn = (mm > 0 && mm < half) ? S[(size_t)(half + mm) * mt + u] : 0.0;
+            if (h2 == 0) { xr = c; xi = sg * sn; } else { yr = c; yi = sg * sn; }
+        }
+        z[pr * np + bitrev(m, bits)] = make_double2(xr - yi, xi + yr);
+    }
+    __syncthreads();
+    fft_stages<true>(z, tw, np, bits, LP);
+    const double inv = 1.0 / np;
+    for (int e = threadIdx.x; e < nl * np; e += blockDim.x) {
+        const int l = e % nl, kq = e / nl;
+        const int i = ib + l;
+        if (i >= i1) continue;
+        const double2 v = z[(l >> 1) * np + kq];
+        const double x = ((l & 1) ? v.y : v.x) * inv;
+        const size_t off = (size_t)(j + 1) * ldi + (size_t)(i + 1);
+        if (kq == 0) {
+            psi[(size_t)np * ldj * ldi + off] = x;
+            psi[off] = x;
+        } else {
+            psi[(size_t)kq * ldj * ldi + off] = x;
+        }
+    }
+}
+
+// pairs of lines per workgroup: LP np complex values within 32 KB (64 KB for
+// the longest lines)
+inline int fft_pairs(int np) { return np >= 4096 ? 1 : std::max(1, 2048 / np); }
+
 // psip <- psi on the range the reference's routine writes, and psi <- 0 on
 // that range's wall points (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)
 __global__ void __launch_bounds__(256) k3_psi_pre(Geo g, int which, int i1, int j1, double *psi, double *psip)
@@ -585,6 +694,15 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
     }
     VV_HIP(upload(&P.W, W));
     VV_HIP(upload(&P.Wi, Wi));
+    P.fft = (np >= 8 && np <= 4096 && (np & (np - 1)) == 0);
+    if (P.fft) {
+        std::vector<double> tw(np);   // (cos, -sin)(2 pi t/np), t = 0..np/2-1
+        for (int t = 0; t < np / 2; ++t) {
+            tw[2 * t] = std::cos(2.0 * pi * t / np);
+            tw[2 * t + 1] = -std::sin(2.0 * pi * t / np);
+        }
+        VV_HIP(upload(&P.tw, tw));
+    }
     VV_HIP(upload(&P.lphi, lphi));
     const double hhz = g.dz * g.dz, hhp = g.dp * g.dp;
     size_t mmax = 0;
@@ -624,9 +742,9 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
 
 void poisson3_free(Poisson3 &P)
 {
-    for (double *p : {P.W, P.Wi, P.lphi, P.S, P.T, P.cp})
+    for (double *p : {P.W, P.Wi, P.lphi, P.S, P.T, P.cp, P.tw})
         if (p) (void)hipFree(p);
-    P.W = P.Wi = P.lphi = P.S = P.T = P.cp = nullptr;
+    P.W = P.Wi = P.lphi = P.S = P.T = P.cp = P.tw = nullptr;
     for (auto &O : P.op) {
         for (double *p : {O.V, O.Vinv, O.rt, O.lz})
             if (p) (void)hipFree(p);
@@ -641,12 +759,23 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
     const int nm = P.nm, np = P.np, i1 = O.i1, j1 = O.j1, mt = O.m;
     hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((g.ldi * g.ldj + 255) / 256), np + 1), dim3(256), 0, s, g, which,
                        i1, j1, psi, psip);
-    gemm<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
+    const int bits = P.fft ? __builtin_ctz((unsigned)np) : 0, LP = fft_pairs(np);
+    const dim3 fgrid((unsigned)((i1 + 2 * LP - 1) / (2 * LP)), (unsigned)j1);
+    const size_t flds = (size_t)LP * np * sizeof(double2);
+    if (P.fft)
+        hipLaunchKernelGGL(k3_fft_fwd, fgrid, dim3(256), flds, s, rhs, P.S, (const double2 *)P.tw, np, bits, LP, i1,
+                           j1, g.ldi, g.ldj, mt);
+    else
+        gemm<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
     gemm<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
     hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
     gemm<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
-    gemm<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+    if (P.fft)
+        hipLaunchKernelGGL(k3_fft_inv, fgrid, dim3(256), flds, s, (const double *)P.S, psi, (const double2 *)P.tw, np,
+                           bits, LP, i1, j1, g.ldi, g.ldj, mt);
+    else
+        gemm<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

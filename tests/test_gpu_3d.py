@@ -137,11 +137,13 @@ def test_explicit_routines_match_oracle(seed):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(NR, NZ, NP), (8, 10, 6)])
+@pytest.mark.parametrize("shape", [(NR, NZ, NP), (8, 10, 6), (12, 9, 16), (7, 6, 8)])
 def test_poisson_matches_exact_solve(which, shape):
     """poisson1/2/3 against the oracle's exact solve of the reference's
     residual operator (pinned in the oracle's tests), with the reference's
-    output conventions (psip, walls, untouched points) bit for bit."""
+    output conventions (psip, walls, untouched points) bit for bit.  np = 16
+    and 8 take the FFT path of the phi transforms, np = 24 and 6 the dense
+    one."""
     nr, nz, np_ = shape
     run = _gpu(nr, nz, np_)
     ora = O3.Run3(nr, nz, np_, **PARAMS)
@@ -173,7 +175,7 @@ def test_poisson_matches_exact_solve(which, shape):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10)])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10), (40, 36, 64)])
 def test_poisson_residual_at_larger_sizes(which, shape):
     """Grids beyond the oracle's dense solve: the solution satisfies the
     reference's residual operator (rot3dmg.f:1695-1723, 1927-1965,
