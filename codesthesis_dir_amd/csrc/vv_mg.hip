@@ -101,19 +101,15 @@ __device__ __forceinline__ double restricted_at(const KProb &P, const KLev &F, i
     return 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
 }
 
-// Solves NL r-lines that share one operator, in LDS, by parallel cyclic
-// reduction.  Each thread owns rows tid + k*nt (k < R) of every line and
-// holds their reduction coefficients in registers (loaded before the first
-// barrier, so no global load sits on the chain of steps).  rhs[l][k] is the
-// thread's right-hand side of line l; with FOLD the closure rows are folded
-// here, otherwise the caller has folded them.  Solutions return in x[l][k].
-template <int R, int NL, bool FOLD>
-__device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S, int n,
-                                          double (*buf)[2][MAXN], double rhs[NL][R], double x[NL][R])
+// The reduction coefficients of one line operator for the rows a thread owns
+// (rows tid + k*nt, k < R): alpha and beta of each of the S steps and the
+// final 1/diagonal, in registers.
+template <int R>
+__device__ __forceinline__ void pcr_coef(const double *__restrict__ tab, int S, int n, double a[R][MAXS],
+                                         double b[R][MAXS], double di[R])
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
-    double a[R][MAXS], b[R][MAXS], di[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
@@ -124,9 +120,27 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
             b[k][t] = (in && t < S) ? beta[(size_t)t * n + i] : 0.0;
         }
         di[k] = in ? dinv[i] : 0.0;
+    }
+}
+
+// Solves NL r-lines that share one operator, in LDS, by parallel cyclic
+// reduction with the coefficients a, b, di of pcr_coef.  rhs[l][k] is the
+// thread's right-hand side of line l for row tid + k*nt; with FOLD the
+// closure rows are folded here (tab[0], tab[1]), otherwise the caller has
+// folded them.  Solutions return in x[l][k].  The caller separates two calls
+// that reuse buf by a barrier.
+template <int R, int NL, bool FOLD>
+__device__ __forceinline__ void pcr_solve(const double *__restrict__ tab, int S, int n, double (*buf)[2][MAXN],
+                                          double rhs[NL][R], double x[NL][R], const double a[R][MAXS],
+                                          const double b[R][MAXS], const double di[R])
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
 #pragma unroll
         for (int l = 0; l < NL; ++l)
-            if (in) buf[l][0][i] = rhs[l][k];
+            if (i < n) buf[l][0][i] = rhs[l][k];
     }
     __syncthreads();
     if (FOLD && n > 2) {   // r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
@@ -175,9 +189,21 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
     }
 }
 
+// pcr_coef + pcr_solve for a single use of an operator
+template <int R, int NL, bool FOLD>
+__device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S, int n,
+                                          double (*buf)[2][MAXN], double rhs[NL][R], double x[NL][R])
+{
+    double a[R][MAXS], b[R][MAXS], di[R];
+    pcr_coef<R>(tab, S, n, a, b, di);
+    pcr_solve<R, NL, FOLD>(tab, S, n, buf, rhs, x, a, b, di);
+}
+
 // One zebra sweep (both colours) of one level, from buffer u to buffer uo.
-// Workgroup bx owns black (odd) line qb = 2bx+1 and red (even) line
-// qr0 = 2bx; it also solves the red line qr1 = qb+1 that the black line
+// Workgroup bx works through G consecutive line groups (G = ncol/2 over the
+// grid size, 1 on small levels), loading the level's reduction coefficients
+// once for all of them.  Group g owns black (odd) line qb = 2(bx G + g)+1 and
+// red (even) line qr0 = qb-1; it also solves the red line qr1 = qb+1 that the black line
 // needs (the workgroup that owns qr1 computes the same values from the same
 // inputs), so both colours run in one launch:
 //   red lines qr0, qr1 from the old black lines (u), in one reduction pass
@@ -195,6 +221,104 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
 // so the restriction onto the coarsest level needs only these changes.
 template <int R>
 __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
+{
+    const int p = blockIdx.y;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, ncol = L.ncol;
+    // With one row per thread the coefficients are loaded once for all line
+    // groups; with two rows per thread they leave no registers for that (the
+    // launch keeps one group per workgroup and each solve loads them).
+    constexpr bool HOIST = (R == 1);
+    const int G = HOIST ? (ncol / 2 + (int)gridDim.x - 1) / (int)gridDim.x : 1;
+    const int g0 = (int)blockIdx.x * G;
+    if (ncol < 2 || 2 * g0 + 1 >= ncol) return;
+    const int done = a.done[p];            // its latency overlaps the loads below
+    double ca[R][MAXS], cb[R][MAXS], cdi[R];
+    if constexpr (HOIST) pcr_coef<R>(L.lu, L.nstep, n, ca, cb, cdi);
+    if (done) return;                       // uniform over the workgroup
+    for (int g = 0; g < G; ++g) {
+    const int qb = 2 * (g0 + g) + 1;
+    if (qb >= ncol) break;                  // uniform
+    if (g > 0) __syncthreads();             // the previous group's last reads of buf
+    const int qr0 = qb - 1;
+    const int qr1 = (qb + 1 == ncol) ? 0 : qb + 1;
+    const int qbm = (qr0 == 0) ? ncol - 1 : qr0 - 1;          // black left of qr0
+    const int qbp = (qr1 == ncol - 1) ? 0 : qr1 + 1;          // black right of qr1
+    __shared__ double buf[2][2][MAXN];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double c = L.cphi;
+    const int mode = a.mode;
+    const double *u = L.u;
+    const double *e = P.Cc.u;
+    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        rr[0][k] = rr[1][k] = 0.0;
+        fb[k] = fr0[k] = ubo[k] = 0.0;
+        if (i >= n) continue;
+        const double rp = P.rphi[i];
+        if (mode == ZB_RESTRICT) {
+            const double f0 = restricted_at(P, P.F, i, qr0);
+            const double f1 = restricted_at(P, P.F, i, qr1);
+            fb[k] = restricted_at(P, P.F, i, qb);
+            fr0[k] = f0;
+            rr[0][k] = f0;
+            rr[1][k] = f1;
+        } else if (mode == ZB_PROLONG) {
+            const double um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
+            const double ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
+            const double up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
+            rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (um0 + ub);
+            rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + up1);
+            fb[k] = L.f[cix(i, qb, n)];
+            ubo[k] = ub;
+        } else {
+            const double ub = u[cix(i, qb, n)];
+            rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (u[cix(i, qbm, n)] + ub);
+            rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + u[cix(i, qbp, n)]);
+            fb[k] = L.f[cix(i, qb, n)];
+            ubo[k] = ub;
+        }
+    }
+    if (mode == ZB_RESTRICT) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int i = tid + k * nt;
+            if (i < n) {
+                L.f[cix(i, qr0, n)] = fr0[k];
+                L.f[cix(i, qb, n)] = fb[k];
+            }
+        }
+    }
+    if constexpr (HOIST) pcr_solve<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr, ca, cb, cdi);
+    else pcr_lines<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : 0.0;
+    }
+    if constexpr (HOIST) pcr_solve<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb, ca, cb, cdi);
+    else pcr_lines<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb);
+    const bool sd = a.store_delta;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * nt;
+        if (i < n) {
+            L.uo[cix(i, qr0, n)] = xr[0][k];
+            L.uo[cix(i, qb, n)] = xb[0][k];
+            if (sd) L.dlt[cix(i, qb >> 1, n)] = xb[0][k] - ubo[k];
+        }
+    }
+    }   // line groups
+}
+
+// The same sweep with one line group per workgroup and the coefficients
+// loaded by each solve: for lines of more than 1024 rows (two rows per
+// thread), where holding the coefficients across both solves spills.
+template <int R>
+__global__ void __launch_bounds__(1024) k_mg_zebra1(KArgs a)
 {
     const int p = blockIdx.y;
     const KProb &P = a.P[p];
@@ -805,10 +929,14 @@ static int enqueue_cycle(vv_run *r)
         KArgs a = kargs(r, l, mode, cur);
         a.store_delta = store_delta;
         prof_begin(r, &pe);
+        // line groups per workgroup: one on small levels, enough to keep
+        // about 512 workgroups per problem on large ones
+        const int nb = ncol(l) / 2, G = two ? 1 : (nb + 511) / 512;
+        const dim3 grid((unsigned)((nb + G - 1) / G), 2);
         if (two)
-            hipLaunchKernelGGL(k_mg_zebra<2>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
+            hipLaunchKernelGGL(k_mg_zebra1<2>, grid, dim3(zt), 0, r->stream, a);
         else
-            hipLaunchKernelGGL(k_mg_zebra<1>, dim3(ncol(l) / 2, 2), dim3(zt), 0, r->stream, a);
+            hipLaunchKernelGGL(k_mg_zebra<1>, grid, dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_ZEBRA, pe, bytes);
         cur[l] ^= 1;
     };
