@@ -132,11 +132,14 @@ def vcycle_large_leg(steps=6, nu=1):
 
 
 # 3D rotating annulus (rot3dmg.f): the reference program's largest grid
-# (PARAMETER kr = kz = 33, kp = 160) and config 3's grid
+# (PARAMETER kr = kz = 33, kp = 160), config 3's grid and config 5's grid
+# size (its tank physics is not built; the rot3dmg.f step at that size)
 ROT3D = {
     "rot3d-33x33x161": (32, 32, 160),
     "rot3d-129x129x257": (128, 128, 256),
+    "rot3d-257x257x513": (256, 256, 512),
 }
+ROT3D_STEPS = {"rot3d-33x33x161": 100, "rot3d-129x129x257": 20, "rot3d-257x257x513": 5}
 ROT3D_PHYS = dict(dt=1e-6, Ra=1e7, Re=1000.0, Ptl=7.0, Gr=0.0, r0=0.5, rmax=1.0, zmax=0.5)
 
 
@@ -199,7 +202,7 @@ def rot3d_routines(nr, nz, np_, reps=10):
     return out
 
 
-def rot3d_leg(cpu=True, steps=20, warmup=3):
+def rot3d_leg(cpu=True, warmup=3):
     """The 3D step (vorticity A/B/C, temperature, heat, three exact Poisson
     solves, wall update) on the GPU at both grids, and the reference build
     itself (its multigrid Poisson to the shipped absolute 1e-5) on one core
@@ -214,13 +217,13 @@ def rot3d_leg(cpu=True, steps=20, warmup=3):
         for k, v in rot3d_state(nr, nz, np_).items():
             run.put(k, v)
         run.step(warmup)
-        n = steps if nr >= 128 else 5 * steps
+        n = ROT3D_STEPS[name]
         t0 = time.perf_counter()
         t1 = run.step(n)
         el = time.perf_counter() - t0
         run.close()
         out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
-        if nr >= 128:
+        if nr == 128:
             out[name]["routines"] = rot3d_routines(nr, nz, np_)
     lib = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
     if cpu and os.path.exists(lib):

@@ -175,7 +175,7 @@ def test_poisson_matches_exact_solve(which, shape):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10), (40, 36, 64)])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10), (40, 36, 64), (160, 192, 32)])
 def test_poisson_residual_at_larger_sizes(which, shape):
     """Grids beyond the oracle's dense solve: the solution satisfies the
     reference's residual operator (rot3dmg.f:1695-1723, 1927-1965,
