@@ -13,8 +13,8 @@ algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
 reference's CPU path on a bounded sample of the same workload on this host.
 Unless --no-extra is given, the line also carries the direct-solve
 alternative, a large V-cycle measurement and the 3D step of rot3dmg.f
-("rot3d": the reference program's largest grid and config 3's grid, with the
-reference build's own step timed on one core).
+("rot3d": the reference program's largest grid, 33x33x161, and 129x129x257,
+with the reference build's own step timed on one core).
 
 Prints one JSON line.
 """
@@ -41,12 +41,12 @@ WORKLOADS = {
         "rotating-annulus velocity-vorticity plane (velvor2dnewsch3.4.f physics: cylindrical "
         "terms, centrifugal buoyancy, compact 4th-order scheme, DuFort-Frankel), 513x257 nodes "
         "(nr=512 radial, np=256 azimuthal), radius ratio 0.5, Ra=1e8, Ta=1e5 (Re=sqrt(Ta)), "
-        "Pr=1, dt=4e-8, T linear in r + U(-1e-4,1e-4); config 2's grid size -- its "
-        "meridional/swirl/tanh-stretched form has no reference program and is not built",
+        "Pr=1, dt=4e-8, T linear in r + U(-1e-4,1e-4); the 513x257 grid of the meridional-plane "
+        "annulus, whose swirl/tanh-stretched form has no reference program and is not built",
         dict(geom=1, nr=512, np=256, dt=4e-8, Ra=1e8, Re=1e5 ** 0.5, Pr=1.0, robert=0.05),
         20261203, 1e-4),
     "box-129x65": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 129x65 nodes, 2:1, "
-                   "Ra=1e5, Pr=1, dt=1e-5 (config 1)",
+                   "Ra=1e5, Pr=1, dt=1e-5 (the reference's CPU-runnable case)",
                    dict(geom=0, nr=64, np=128, width=1.0, dt=1e-5, Ra=1e5, Pr=1.0), 20261202, 1e-3),
     "box-513x257": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f), 513x257 nodes "
                     "(nr=512 across, np=256 periodic), width 1, Ra=1e5, Pr=1, dt=1e-6",
@@ -132,8 +132,9 @@ def vcycle_large_leg(steps=6, nu=1):
 
 
 # 3D rotating annulus (rot3dmg.f): the reference program's largest grid
-# (PARAMETER kr = kz = 33, kp = 160), config 3's grid and config 5's grid
-# size (its tank physics is not built; the rot3dmg.f step at that size)
+# (PARAMETER kr = kz = 33, kp = 160), 129x129x257 and 257x257x513 (the
+# size of a 512x256x256 tank; tank physics is not built, the rot3dmg.f step
+# at that size is)
 ROT3D = {
     "rot3d-33x33x161": (32, 32, 160),
     "rot3d-129x129x257": (128, 128, 256),
@@ -244,6 +245,84 @@ def rot3d_leg(cpu=True, warmup=3):
     return out
 
 
+def sweep_requests(n=64, seed=20261205, steps=2000):
+    """The served mixed-size sweep: n requests arriving as a Poisson stream
+    (mean 20/s); grid sides from {65, 129, 257, 513, 1025} x half height
+    (np = side-1, nr = np/2); Ra log-uniform in [1e4, 1e8], Pr uniform in
+    [0.7, 7]; Ta = 0 (periodic box, rb.f) or, with probability 1/2,
+    log-uniform in [1e3, 1e6] (rotating-annulus plane, Re = sqrt(Ta)); T
+    linear in r plus seeded U(-1e-3, 1e-3).
+
+    dt = min(3e-4/sqrt(Ra), 0.05 dr^2), dr = 0.5/nr.  The explicit step has
+    a diffusive bound besides the advective one: with the oracle, the
+    annulus at nr=128 (Ra=1.1e4, Re=128) grows without bound at dt = 0.18
+    dr^2 and stays bounded over 2000 steps at 0.09 dr^2; box runs at nr=512
+    diverged at 0.21 dr^2 and above."""
+    import numpy as np
+    from codesthesis_dir_amd import BoxConfig
+    from codesthesis_dir_amd.ic import box_temperature
+    from codesthesis_dir_amd.velvor import AnnulusConfig
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    for q in range(n):
+        t += rng.exponential(1.0 / 20.0)
+        np_ = int(rng.choice([64, 128, 256, 512, 1024]))
+        nr = np_ // 2
+        Ra = 10.0 ** rng.uniform(4.0, 8.0)
+        Pr = rng.uniform(0.7, 7.0)
+        Ta = 0.0 if rng.uniform() < 0.5 else 10.0 ** rng.uniform(3.0, 6.0)
+        dt = min(3e-4 / Ra ** 0.5, 0.05 * (0.5 / nr) ** 2)
+        kw = dict(nr=nr, np=np_, dt=dt, Ra=Ra, Pr=Pr, rtol=1e-10, max_cycles=60, nu1=1, nu2=1,
+                  stats=0)
+        cfg = AnnulusConfig(Re=Ta ** 0.5, **kw) if Ta > 0 else BoxConfig(width=1.0, **kw)
+        th0 = box_temperature(nr, np_, cfg.r0, cfg.rmax, 1e-3, seed + q)
+        out.append((t, cfg, th0, steps))
+    return out
+
+
+def serve_leg(n=64, steps=2000, slots=8):
+    """The served sweep through the request loop (include/vvserve.h): the
+    requests arrive on their schedule and are admitted by slot and device
+    footprint; then the same requests one at a time (one slot, all queued
+    at once), the plain drop-in's throughput."""
+    import numpy as np
+    from codesthesis_dir_amd.serve import Server
+    reqs = sweep_requests(n, steps=steps)
+    res = {}
+    with Server(slots=slots, mem_budget=64e9) as srv:
+        t_start = time.perf_counter()
+        for t_arr, cfg, th0, st in reqs:
+            d = t_arr - (time.perf_counter() - t_start)
+            if d > 0:
+                time.sleep(d)
+            srv.submit(cfg, st, th0)
+        got = [srv.wait() for _ in reqs]
+        wall = time.perf_counter() - t_start
+        stats = srv.stats()
+    lat = np.array([g["t_done"] - g["t_submit"] for g in got])
+    bad = sum(1 for g in got if g["status"] < 0)
+    res["failures"] = [{"id": g["id"], "status": g["status"], "error": g["error"], "nr": reqs[g["id"]][1].nr,
+                        "np": reqs[g["id"]][1].np, "geom": reqs[g["id"]][1].geom, "Ra": reqs[g["id"]][1].Ra,
+                        "Re": reqs[g["id"]][1].Re, "Pr": reqs[g["id"]][1].Pr}
+                       for g in got if g["status"] < 0]
+    res["served"] = {"slots": slots, "solves_per_s": n / wall, "wall_s": wall, "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)), "peak_running": stats["peak_running"],
+                     "peak_mem_GB": stats["peak_mem"] / 1e9, "failed": bad,
+                     "not_converged": sum(1 for g in got if g["status"] == 1)}
+    with Server(slots=1, mem_budget=64e9) as srv:
+        t_start = time.perf_counter()
+        for _, cfg, th0, st in reqs:
+            srv.submit(cfg, st, th0)
+        got1 = [srv.wait() for _ in reqs]
+        wall1 = time.perf_counter() - t_start
+    res["one_at_a_time"] = {"solves_per_s": n / wall1, "wall_s": wall1,
+                            "failed": sum(1 for g in got1 if g["status"] < 0)}
+    res["served_over_one_at_a_time"] = res["served"]["solves_per_s"] / res["one_at_a_time"]["solves_per_s"]
+    res["workload"] = (f"{n} requests (served mixed-size sweep), {steps} steps each at Poisson rtol 1e-10, arrivals "
+                       "Poisson 20/s, sides {65..1025} x half height, box or annulus plane, FP64")
+    return res
+
+
 def cpu_leg(name, budget_s=15.0):
     """Time the reference CPU path on this host for a bounded sample."""
     desc, kw, seed, amp = WORKLOADS[name]
@@ -325,6 +404,8 @@ def main():
     ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the direct-solve and large V-cycle measurements")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="measure only the served mixed-size sweep and print its JSON object")
     ap.add_argument("--rot3d-only", action="store_true",
                     help="measure only the 3D step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -336,6 +417,9 @@ def main():
         sys.exit(2)
     if a.rot3d_only:
         print(json.dumps(rot3d_leg(cpu=not a.no_cpu)))
+        return
+    if a.serve_only:
+        print(json.dumps(serve_leg()))
         return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
@@ -380,6 +464,7 @@ def main():
         line["alternatives"] = {"poisson_direct": direct_leg(a.workload, a.steps, a.warmup)}
         line["vcycle_large"] = vcycle_large_leg()
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
+        line["served_sweep"] = serve_leg()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.workload)
     print(json.dumps(line))
