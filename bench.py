@@ -315,8 +315,14 @@ def serve_leg(n=64, steps=2000, slots=8):
             srv.submit(cfg, st, th0)
         got1 = [srv.wait() for _ in reqs]
         wall1 = time.perf_counter() - t_start
+    by_side = {}
+    for g in got1:
+        side = reqs[g["id"]][1].np + 1
+        by_side.setdefault(side, []).append(g["t_done"] - g["t_admit"])
     res["one_at_a_time"] = {"solves_per_s": n / wall1, "wall_s": wall1,
-                            "failed": sum(1 for g in got1 if g["status"] < 0)}
+                            "failed": sum(1 for g in got1 if g["status"] < 0),
+                            "service_s_by_side": {str(k): {"runs": len(v), "mean_s": float(np.mean(v))}
+                                                  for k, v in sorted(by_side.items())}}
     res["served_over_one_at_a_time"] = res["served"]["solves_per_s"] / res["one_at_a_time"]["solves_per_s"]
     res["workload"] = (f"{n} requests (served mixed-size sweep), {steps} steps each at Poisson rtol 1e-10, arrivals "
                        "Poisson 20/s, sides {65..1025} x half height, box or annulus plane, FP64")
