@@ -23,7 +23,7 @@ class V3Params(ctypes.Structure):
                 ("ldj", ctypes.c_int), ("dt", ctypes.c_double), ("dr", ctypes.c_double),
                 ("dz", ctypes.c_double), ("dp", ctypes.c_double), ("Ra", ctypes.c_double),
                 ("Re", ctypes.c_double), ("Ptl", ctypes.c_double), ("Gr", ctypes.c_double),
-                ("r0", ctypes.c_double), ("rmax", ctypes.c_double)]
+                ("r0", ctypes.c_double), ("rmax", ctypes.c_double), ("model", ctypes.c_int)]
 
 
 class V3Fields(ctypes.Structure):
@@ -38,6 +38,7 @@ def lib():
                    "v3_rhsw", "v3_rhsv", "v3_rhsu"):
             getattr(_LIB, fn).restype = None
         _LIB.v3_heat.restype = ctypes.c_double
+        _LIB.v3_heat2.restype = None
         _LIB.v3_residual.restype = ctypes.c_double
         _LIB.v3_poisson_exact.restype = ctypes.c_int
     return _LIB
@@ -49,14 +50,16 @@ def _P(a):
 
 class Run3:
     """One 3D state: parameters and the 21 fields of rot3dmg.f's COMMON
-    blocks (plus the three Poisson right-hand sides)."""
+    blocks (plus the three Poisson right-hand sides).  model 0: rot3dmg.f;
+    1: rot3dmghor.f (stationary horizontal annulus)."""
 
-    def __init__(self, nr, nz, np_, dt, Ra, Re, Ptl, Gr, r0=0.5, rmax=1.0, zmax=0.5, ldi=None, ldj=None):
+    def __init__(self, nr, nz, np_, dt, Ra, Re, Ptl, Gr, r0=0.5, rmax=1.0, zmax=0.5, ldi=None, ldj=None,
+                 model=0):
         ldi = ldi or nr + 2
         ldj = ldj or nz + 2
         pi = np.arccos(-1.0)
         self.p = V3Params(nr, nz, np_, ldi, ldj, dt, (rmax - r0) / nr, zmax / nz, 2.0 * pi / np_,
-                          Ra, Re, Ptl, Gr, r0, rmax)
+                          Ra, Re, Ptl, Gr, r0, rmax, model)
         self.shape = (np_ + 1, ldj, ldi)
         self.f = {n: np.zeros(self.shape) for n in FIELDS}
         self._sync()
@@ -75,6 +78,13 @@ class Run3:
 
     def heat(self):
         return lib().v3_heat(ctypes.byref(self.p), ctypes.byref(self.c))
+
+    def heat2(self):
+        """(trans1, trans2) of the model's heat routine (trans2 = 0 for
+        rot3dmg.f)."""
+        out = np.zeros(2)
+        lib().v3_heat2(ctypes.byref(self.p), ctypes.byref(self.c), _P(out))
+        return float(out[0]), float(out[1])
 
     def residual(self, which, psi, rhs):
         res = np.zeros(self.shape)

@@ -1,8 +1,11 @@
-"""In-process driver for the 3D reference build oracle/_ref/librot3d.so
-(test and CPU-baseline infrastructure; no product code imports it).
+"""In-process driver for the 3D reference builds oracle/_ref/librot3d.so and
+librot3dhor.so (test and CPU-baseline infrastructure; no product code
+imports it).
 
-librot3d.so is rot3dmg.f compiled unmodified by oracle/Makefile from the
-reference sources (only in the container that holds them).  Its COMMON
+librot3d.so is rot3dmg.f, librot3dhor.so rot3dmghor.f (the stationary
+horizontal annulus), each compiled unmodified by oracle/Makefile from the
+reference sources (only in the container that holds them); both declare the
+same COMMON blocks.  Its COMMON
 blocks (/sol1/ /sol2/ /sol3/ /num/ /val/ /radius/, and the multigrid
 /arrays/ /backup/ /step/ /ints/) are exported symbols; this module maps them
 onto numpy arrays with the reference's PARAMETER extents (kr=kz=33,
@@ -18,6 +21,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
+LIBS = {0: LIB, 1: os.path.join(ROOT, "oracle", "_ref", "librot3dhor.so")}
 KR1, KZ1, KP1 = 34, 34, 161
 MGEN = 202590
 SHAPE = (KP1, KZ1, KR1)
@@ -25,22 +29,25 @@ _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 
 
-def available() -> bool:
-    return os.path.exists(LIB)
+def available(model: int = 0) -> bool:
+    return os.path.exists(LIBS[model])
 
 
 class Reference3:
-    _inst = None
+    """The reference build of model 0 (rot3dmg.f) or 1 (rot3dmghor.f); one
+    instance per library, since its COMMON blocks are process-wide."""
+    _inst = {}
 
-    def __new__(cls):
-        if cls._inst is None:
+    def __new__(cls, model: int = 0):
+        if model not in cls._inst:
             inst = super().__new__(cls)
+            inst.model = model
             inst._init()
-            cls._inst = inst
-        return cls._inst
+            cls._inst[model] = inst
+        return cls._inst[model]
 
     def _init(self):
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(LIBS[self.model])
         self.L = L
         nf = KR1 * KZ1 * KP1
 
@@ -82,9 +89,17 @@ class Reference3:
         getattr(self.L, name + "_")()
 
     def heat(self):
-        t = ctypes.c_double(0.0)
-        self.L.heat_(ctypes.byref(t))
-        return t.value
+        return self.heat2()[0]
+
+    def heat2(self):
+        """(trans1, trans2): heat(trans1) of rot3dmg.f (trans2 = 0), or
+        heat(trans1,trans2) of rot3dmghor.f (rot3dmghor.f:913)."""
+        t1, t2 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        if self.model == 1:
+            self.L.heat_(ctypes.byref(t1), ctypes.byref(t2))
+        else:
+            self.L.heat_(ctypes.byref(t1))
+        return t1.value, t2.value
 
     def rhs(self, which):
         name = {1: "wrh", 2: "vrh", 3: "urh"}[which]

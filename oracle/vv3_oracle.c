@@ -1,5 +1,6 @@
 /* vv3_oracle.c -- CPU oracle (test infrastructure only) for the 3D time step
- * of rot3dmg.f (3D rotating annulus, periodic phi).  Each routine restates
+ * of rot3dmg.f (3D rotating annulus, periodic phi) and of its stationary
+ * horizontal-annulus variant rot3dmghor.f (P->model = 1).  Each routine restates
  * the reference subroutine's loops and expressions in the reference's
  * operation order (compiled with -ffp-contract=off, it reproduces the
  * reference bit for bit); the file:line of each follows its name.  The exact
@@ -14,6 +15,10 @@
 
 #define IX(i, j, k) (((size_t)(k) * P->ldj + (size_t)(j)) * P->ldi + (size_t)(i))
 #define X(a, i, j, k) ((a)[IX(i, j, k)])
+
+/* rot3dmghor.f's azimuth of plane k, theta=6.283185307d0/np*k
+ * (rot3dmghor.f:374, 515, 644) */
+static double hor_theta(const v3_params *P, int k) { return 6.283185307 / P->np * k; }
 
 /* Avorticity, rot3dmg.f:341-468 (hybrid upwind 397-406, 421-430, 444-453) */
 void v3_avorticity(const v3_params *P, v3_fields *F)
@@ -95,7 +100,10 @@ void v3_avorticity(const v3_params *P, v3_fields *F)
                 const double cnh = -1.0 / (dp * dp) / (r * r);
                 const double RHS1 = Arra + Ara / r + Ahha / (r * r) + Azza - 2.0 * Bh / (r * r) + X(A, i, j, k) / (Ptl * dt);
                 const double ULT = (u * Ar + v * Ah / r + w * Az - uelh * BB / r - uelz * CC) / Ptl;
-                const double RHS2 = -2.0 * Re * uelz + 2.0 * Ra * (tem * uelz + thz * u) / Re / Ptl - Gr * Ptl / r * thh;
+                /* rot3dmghor.f:479 replaces the Coriolis and centrifugal terms */
+                const double RHS2 = (P->model == 1)
+                                        ? sin(hor_theta(P, k)) * Gr * Ptl * thz
+                                        : -2.0 * Re * uelz + 2.0 * Ra * (tem * uelz + thz * u) / Re / Ptl - Gr * Ptl / r * thh;
                 const double cf1 = 1.0 / (Ptl * dt) - uelr / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
                 const double cf2 = uelr / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
                 X(F->An, i, j, k) = (cf2 * X(Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
@@ -170,7 +178,10 @@ void v3_bvorticity(const v3_params *P, v3_fields *F)
                 const double cnh = -1.0 / (dp * dp) / (r * r);
                 const double RHS1 = Brra + Bra / r + Bhha / (r * r) + Bzza + 2.0 * Ah / (r * r) + X(B, i, j, k) / (Ptl * dt);
                 const double ULT = (u * Br + v * Bh / r + w * Bz - (velr - v / r) * AA - velz * CC) / Ptl;
-                const double RHS2 = -2.0 * Re * velz + r * Ra * thz + 2.0 * Ra * (tem * velz + thz * v) / Re / Ptl + Gr * Ptl * thr;
+                /* rot3dmghor.f:606 */
+                const double RHS2 = (P->model == 1) ? cos(hor_theta(P, k)) * Gr * Ptl * thz
+                                                    : -2.0 * Re * velz + r * Ra * thz +
+                                                          2.0 * Ra * (tem * velz + thz * v) / Re / Ptl + Gr * Ptl * thr;
                 const double cf1 = 1.0 / (Ptl * dt) - (u / r + velh / r) / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
                 const double cf2 = (u / r + velh / r) / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
                 X(F->Bn, i, j, k) = (cf2 * X(Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
@@ -182,7 +193,7 @@ void v3_cvorticity(const v3_params *P, v3_fields *F)
 {
     const int nr = P->nr, nz = P->nz, np = P->np;
     const double dr = P->dr, dz = P->dz, dp = P->dp, dt = P->dt;
-    const double Ptl = P->Ptl, Ra = P->Ra, Re = P->Re;
+    const double Ptl = P->Ptl, Ra = P->Ra, Re = P->Re, Gr = P->Gr;
     const double *A = F->A, *B = F->B, *C = F->C, *Cp = F->Cp, *th = F->th, *uel = F->uel, *vel = F->vel,
                  *wel = F->wel;
     for (int k = 0; k <= np; ++k)
@@ -257,7 +268,12 @@ void v3_cvorticity(const v3_params *P, v3_fields *F)
                 const double cnh = -1.0 / (dp * dp) / (r * r);
                 const double RHS1 = Crra + Cra / r + Chha / (r * r) + Czza + X(C, i, j, k) / (Ptl * dt);
                 const double ULT = (u * Cr + v * Ch / r + w * Cz - welr * AA - welh * BB / r) / Ptl;
-                const double RHS2 = -2.0 * Re * welz - Ra * thh + 2.0 * Ra * (tem * welz - thr * u - thh * v / r) / Re / Ptl;
+                /* rot3dmghor.f:749 */
+                const double RHS2 = (P->model == 1)
+                                        ? -1.0 / r * Gr * Ptl * cos(hor_theta(P, k)) * thh -
+                                              Gr * Ptl * sin(hor_theta(P, k)) * thr
+                                        : -2.0 * Re * welz - Ra * thh +
+                                              2.0 * Ra * (tem * welz - thr * u - thh * v / r) / Re / Ptl;
                 const double cf1 = 1.0 / (Ptl * dt) - welz / (2.0 * Ptl) - cnr - cnz - cnh;
                 const double cf2 = welz / (2.0 * Ptl) + cnr + cnz + cnh;
                 X(F->Cn, i, j, k) = (cf2 * X(Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
@@ -355,10 +371,58 @@ void v3_temperature(const v3_params *P, v3_fields *F)
             }
 }
 
+/* heat of rot3dmghor.f:913-946: the fluxes through the inner (trans1) and
+ * outer (trans2) cylinders over nz/2-3 <= j <= nz/2+3 (weights 1/120 at the
+ * two ends, 1/60 between) and phi */
+static void heat_hor(const v3_params *P, const v3_fields *F, double out[2])
+{
+    const int nr = P->nr, nz = P->nz, np = P->np;
+    const double dr = P->dr, dz = P->dz, dp = P->dp;
+    const double *th = F->th;
+    const int nza = nz / 2 - 3, nzb = nz / 2 + 3;
+    double trans1 = 0.0, trans2 = 0.0;
+    for (int k = 1; k <= np; ++k) {
+        double transa = dz * (-184.0 * X(th, 0, nza, k) + 225.0 * X(th, 1, nza, k) - 50.0 * X(th, 2, nza, k) +
+                              9.0 * X(th, 3, nza, k)) / (120.0 * dr);
+        double transb = dz * (184.0 * X(th, nr + 1, nza, k) - 225.0 * X(th, nr, nza, k) +
+                              50.0 * X(th, nr - 1, nza, k) - 9.0 * X(th, nr - 2, nza, k)) / (120.0 * dr);
+        transa = transa + dz * (-184.0 * X(th, 0, nzb, k) + 225.0 * X(th, 1, nzb, k) - 50.0 * X(th, 2, nzb, k) +
+                                9.0 * X(th, 3, nzb, k)) / (120.0 * dr);
+        transb = transb + dz * (184.0 * X(th, nr + 1, nzb, k) - 225.0 * X(th, nr, nzb, k) +
+                                50.0 * X(th, nr - 1, nzb, k) - 9.0 * X(th, nr - 2, nzb, k)) / (120.0 * dr);
+        for (int j = nza + 1; j <= nzb - 1; ++j) {
+            transa = transa + dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) +
+                                    9.0 * X(th, 3, j, k)) / (60.0 * dr);
+            transb = transb + dz * (184.0 * X(th, nr + 1, j, k) - 225.0 * X(th, nr, j, k) +
+                                    50.0 * X(th, nr - 1, j, k) - 9.0 * X(th, nr - 2, j, k)) / (60.0 * dr);
+        }
+        trans1 = trans1 + transa * dp;
+        trans2 = trans2 + transb * dp;
+    }
+    out[0] = trans1;
+    out[1] = trans2;
+}
+
+void v3_heat2(const v3_params *P, const v3_fields *F, double out[2])
+{
+    if (P->model == 1) {
+        heat_hor(P, F, out);
+    } else {
+        out[0] = v3_heat(P, F);
+        out[1] = 0.0;
+    }
+}
+
 /* heat, rot3dmg.f:882-905: inner-wall flux integrated over z (trapezoid-
- * like weights 1/240, 1/80, 1/60) and phi */
+ * like weights 1/240, 1/80, 1/60) and phi; rot3dmghor.f's trans1 for
+ * model 1 */
 double v3_heat(const v3_params *P, const v3_fields *F)
 {
+    if (P->model == 1) {
+        double out[2];
+        heat_hor(P, F, out);
+        return out[0];
+    }
     const int nz = P->nz, np = P->np;
     const double dr = P->dr, dz = P->dz, dp = P->dp;
     const double *th = F->th;
@@ -409,8 +473,9 @@ void v3_update(const v3_params *P, v3_fields *F)
         }
     for (int k = 0; k <= np; ++k)
         for (int j = 0; j <= nz + 1; ++j) {
-            X(F->thn, 0, j, k) = 0.0;
-            X(F->thn, nr + 1, j, k) = 1.0;
+            /* rot3dmghor.f:1042-1043 swaps the wall temperatures */
+            X(F->thn, 0, j, k) = (P->model == 1) ? 1.0 : 0.0;
+            X(F->thn, nr + 1, j, k) = (P->model == 1) ? 0.0 : 1.0;
         }
     for (int k = 0; k <= np; ++k)
         for (int j = 0; j <= nz + 1; ++j)

@@ -16,6 +16,10 @@ extern "C" {
 typedef struct {
     int nr, nz, np, ldi, ldj;
     double dt, dr, dz, dp, Ra, Re, Ptl, Gr, r0, rmax;
+    int model;   /* 0: rot3dmg.f; 1: rot3dmghor.f (stationary horizontal
+                    annulus: gravity in the r-theta plane through Gr, no
+                    Coriolis terms, T = 1 inside / 0 outside, heat flux at
+                    both cylinders near mid-axis) */
 } v3_params;
 
 typedef struct {
@@ -29,7 +33,10 @@ void v3_avorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:341-468 */
 void v3_bvorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:477-593 */
 void v3_cvorticity(const v3_params *P, v3_fields *F);   /* rot3dmg.f:601-728 */
 void v3_temperature(const v3_params *P, v3_fields *F);  /* rot3dmg.f:736-874 */
-double v3_heat(const v3_params *P, const v3_fields *F); /* rot3dmg.f:882-905 */
+double v3_heat(const v3_params *P, const v3_fields *F); /* rot3dmg.f:882-905; trans1 */
+/* heat of the model: out[0] = trans1, out[1] = trans2 (rot3dmghor.f:913-946,
+ * the two cylinders between nz/2-3 and nz/2+3; 0 for rot3dmg.f) */
+void v3_heat2(const v3_params *P, const v3_fields *F, double out[2]);
 void v3_update(const v3_params *P, v3_fields *F);       /* rot3dmg.f:964-1018 */
 void v3_rhsw(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1026-1048 */
 void v3_rhsv(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1066-1093 */

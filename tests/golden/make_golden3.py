@@ -10,6 +10,9 @@ the reference's outputs only.
   r3_resid.npz: for each Poisson operator, a seeded iterate and right-hand
     side, the iterate after one relaxation pass and the residual the
     reference stores for it (relax1/2/3 on the finest level).
+  r3h_explicit.npz: as r3_explicit.npz for rot3dmghor.f
+    (oracle/_ref/librot3dhor.so, the stationary horizontal annulus), from
+    its own seeded state; "heat" holds (trans1, trans2).
 
     python tests/golden/make_golden3.py
 """
@@ -79,7 +82,32 @@ def main():
         res[f"psif{w}"] = psif[sl]
         res[f"rc{w}"] = rc[sl]
     np.savez_compressed(os.path.join(HERE, "r3_resid.npz"), **res)
-    print("wrote r3_explicit.npz, r3_resid.npz")
+    hor_fixture()
+    print("wrote r3_explicit.npz, r3_resid.npz, r3h_explicit.npz")
+
+
+def hor_fixture():
+    ref = R3.Reference3(1)
+    run = O3.Run3(NR, NZ, NP, model=1, **PARAMS)
+    ref.setup(run.p)
+    st = state(np.random.default_rng(SEED + 1))
+    K = NP + 1
+    sl = (slice(0, K), slice(0, NZ + 2), slice(0, NR + 2))
+    for n, a in st.items():
+        ref.f[n][...] = 0.0
+        ref.f[n][sl] = a
+    out = {"in_" + n: a for n, a in st.items()}
+    for rt, name in [("avorticity", "An"), ("bvorticity", "Bn"), ("cvorticity", "Cn"), ("temperature", "thn")]:
+        ref.call(rt)
+        out[rt] = ref.f[name][sl].copy()
+    out["heat"] = np.array(ref.heat2())
+    for w, name in [(1, "wrh"), (2, "vrh"), (3, "urh")]:
+        ref.rhs(w)
+        out[name] = ref.f[name][sl].copy()
+    ref.call("update")
+    for n in ("A", "Ap", "B", "Bp", "C", "Cp", "th", "thp", "An", "Bn", "Cn", "thn"):
+        out["upd_" + n] = ref.f[n][sl].copy()
+    np.savez_compressed(os.path.join(HERE, "r3h_explicit.npz"), **out)
 
 
 if __name__ == "__main__":

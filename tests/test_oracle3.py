@@ -20,20 +20,27 @@ def _load(name):
     return np.load(os.path.join(HERE, "golden", name))
 
 
-def _run_from(g):
-    run = O3.Run3(NR, NZ, NP, **PARAMS)
+def _run_from(g, model=0):
+    run = O3.Run3(NR, NZ, NP, model=model, **PARAMS)
     for n in O3.FIELDS:
         run[n] = g["in_" + n]
     return run
 
 
-def test_explicit_routines_match_reference_fixture():
-    g = _load("r3_explicit.npz")
-    run = _run_from(g)
+@pytest.mark.parametrize("fixture,model", [("r3_explicit.npz", 0), ("r3h_explicit.npz", 1)])
+def test_explicit_routines_match_reference_fixture(fixture, model):
+    """rot3dmg.f (model 0) and rot3dmghor.f (model 1, its gravity terms in
+    all three vorticity components, swapped wall temperatures and two-wall
+    heat flux): every routine's output equals the reference build's."""
+    g = _load(fixture)
+    run = _run_from(g, model)
     for rt, name in [("avorticity", "An"), ("bvorticity", "Bn"), ("cvorticity", "Cn"), ("temperature", "thn")]:
         run.call(rt)
         assert np.array_equal(run[name], g[rt]), rt
-    assert run.heat() == float(g["heat"])
+    if model == 0:
+        assert run.heat() == float(g["heat"])
+    else:
+        assert run.heat2() == tuple(float(x) for x in g["heat"])
     for fn, name in [("rhsw", "wrh"), ("rhsv", "vrh"), ("rhsu", "urh")]:
         run.call(fn)
         assert np.array_equal(run[name], g[name]), fn
@@ -108,11 +115,14 @@ def test_exact_solve_against_dense_solve():
         assert np.max(np.abs(got - x)) < 1e-12 * np.max(np.abs(x)), which
 
 
-@pytest.mark.skipif(not R3.available(), reason="reference build oracle/_ref/librot3d.so absent")
-def test_explicit_routines_match_live_reference():
-    """A second seeded state, checked against the reference build itself."""
-    run = O3.Run3(NR, NZ, NP, ldi=R3.KR1, ldj=R3.KZ1, **PARAMS)
-    ref = R3.Reference3()
+@pytest.mark.parametrize("model", [0, 1])
+def test_explicit_routines_match_live_reference(model):
+    """A second seeded state, checked against the reference build itself
+    (librot3d.so or librot3dhor.so)."""
+    if not R3.available(model):
+        pytest.skip("reference build under oracle/_ref absent")
+    run = O3.Run3(NR, NZ, NP, ldi=R3.KR1, ldj=R3.KZ1, model=model, **PARAMS)
+    ref = R3.Reference3(model)
     ref.setup(run.p)
     rng = np.random.default_rng(99)
     K = NP + 1
@@ -127,4 +137,4 @@ def test_explicit_routines_match_live_reference():
         run.call(rt)
         ref.call(rt)
         assert np.array_equal(run[name], ref.f[name][:K]), rt
-    assert run.heat() == ref.heat()
+    assert run.heat2() == ref.heat2()
