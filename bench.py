@@ -12,9 +12,10 @@ HIP events gives the dominant kernel's average duration and its achieved
 algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
 reference's CPU path on a bounded sample of the same workload on this host.
 Unless --no-extra is given, the line also carries the direct-solve
-alternative, a large V-cycle measurement and the 3D step of rot3dmg.f
-("rot3d": the reference program's largest grid, 33x33x161, and 129x129x257,
-with the reference build's own step timed on one core).
+alternative, a large V-cycle measurement and the 3D step ("rot3d": rot3dmg.f
+at the reference program's largest grid, 33x33x161, at 129x129x257 and at
+257x257x513, with the reference build's own step timed on one core; and
+rot3dmghor.f, the stationary horizontal annulus, at 257x257x513).
 
 Prints one JSON line.
 """
@@ -133,8 +134,7 @@ def vcycle_large_leg(steps=6, nu=1):
 
 # 3D rotating annulus (rot3dmg.f): the reference program's largest grid
 # (PARAMETER kr = kz = 33, kp = 160), 129x129x257 and 257x257x513 (the
-# size of a 512x256x256 tank; tank physics is not built, the rot3dmg.f step
-# at that size is)
+# size of a 512x256x256 tank)
 ROT3D = {
     "rot3d-33x33x161": (32, 32, 160),
     "rot3d-129x129x257": (128, 128, 256),
@@ -162,6 +162,30 @@ def rot3d_state(nr, nz, np_, seed=20261204):
     th = np.broadcast_to(1.0 - np.log(r) / np.log(r0), (np_ + 1, nz + 2, nr + 2)).copy()
     th[:, :, 1:nr + 1] += pert[:, None, None]
     th[:, :, 0], th[:, :, nr + 1] = 0.0, 1.0
+    th[0] = th[np_]
+    return {"th": th, "thp": th.copy()}
+
+
+# rot3dmghor.f (stationary horizontal annulus: gravity in the r-theta plane)
+# at 257x257x513, the size of a 512x256x256 tank; Gr Ptl = 1e9, Pr = 5
+HOR3D = ("rot3dhor-257x257x513", (256, 256, 512), 5)
+HOR3D_PHYS = dict(dt=1e-7, Ra=1e9, Re=0.0, Ptl=5.0, Gr=2e8, r0=0.5, rmax=1.0, zmax=0.5, model=1)
+
+
+def hor3d_state(nr, nz, np_, seed=20261206):
+    """rot3dmghor.f's conduction profile T = ln(r)/ln(r0) (rot3dmghor.f:133:
+    1 on the inner, 0 on the outer cylinder) plus seeded U(-1e-4, 1e-4) on
+    the interior; fluid at rest."""
+    import numpy as np
+    P = HOR3D_PHYS
+    r0, rmax = P["r0"], P["rmax"]
+    dr = (rmax - r0) / nr
+    i = np.arange(nr + 2)
+    r = np.where(i == 0, r0, np.where(i == nr + 1, rmax, r0 + (i - 0.5) * dr))
+    th = np.broadcast_to(np.log(r) / np.log(r0), (np_ + 1, nz + 2, nr + 2)).copy()
+    rng = np.random.default_rng(seed)
+    th[:, :, 1:nr + 1] += rng.uniform(-1e-4, 1e-4, (np_ + 1, nz + 2, nr))
+    th[:, :, 0], th[:, :, nr + 1] = 1.0, 0.0
     th[0] = th[np_]
     return {"th": th, "thp": th.copy()}
 
@@ -226,6 +250,20 @@ def rot3d_leg(cpu=True, warmup=3):
         out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
         if nr == 128:
             out[name]["routines"] = rot3d_routines(nr, nz, np_)
+    name, (nr, nz, np_), n = HOR3D
+    run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, **HOR3D_PHYS))
+    for k, v in hor3d_state(nr, nz, np_).items():
+        run.put(k, v)
+    run.step(warmup)
+    t0 = time.perf_counter()
+    run.step(n)
+    el = time.perf_counter() - t0
+    t1, t2 = run.fluxes()
+    run.close()
+    out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1,
+                 "trans2_last": t2,
+                 "physics": "rot3dmghor.f (stationary horizontal annulus), Gr=2e8, Pr=5 (Gr Pr = 1e9), dt=1e-7",
+                 "data": "synthetic: conduction ln(r)/ln(r0) + seeded U(-1e-4, 1e-4), fluid at rest"}
     lib = os.path.join(ROOT, "oracle", "_ref", "librot3d.so")
     if cpu and os.path.exists(lib):
         nr, nz, np_ = ROT3D["rot3d-33x33x161"]

@@ -47,19 +47,23 @@ DROPIN3_SRC = os.path.join(PKG, "fortran", "vv_dropin3.F90")
 DROPIN3_LIB = os.path.join(PKG, "libvvdropin3.so")
 
 
-def build_dropin3() -> str:
+def build_dropin3(hor: bool = False) -> str:
     """The 3D Fortran drop-in (rot3dmg.f routine names over include/vvmi3.h)
     as a shared library next to libvvmi.so; its routine names overlap the
-    2D drop-in's, so it is a library of its own."""
-    mod = os.path.join(PKG, "fortran", "_build", "d3")
+    2D drop-in's, so it is a library of its own.  hor=True builds the
+    drop-in of rot3dmghor.f (libvvdropin3h.so: model VV3_MODEL_HOR and
+    heat(trans1,trans2))."""
+    lib = os.path.join(PKG, "libvvdropin3h.so") if hor else DROPIN3_LIB
+    mod = os.path.join(PKG, "fortran", "_build", "d3h" if hor else "d3")
     os.makedirs(mod, exist_ok=True)
-    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-I" + os.path.join(PKG, "fortran"),
-           "-o", DROPIN3_LIB + ".tmp", DROPIN3_SRC, "-L" + PKG, "-lvvmi", "-Wl,-rpath,$ORIGIN"]
+    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-cpp", "-DVV3_HOR=%d" % int(hor), "-module-dir", mod,
+           "-I" + os.path.join(PKG, "fortran"), "-o", lib + ".tmp", DROPIN3_SRC, "-L" + PKG, "-lvvmi",
+           "-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("flang failed:\n" + res.stdout + res.stderr)
-    os.replace(DROPIN3_LIB + ".tmp", DROPIN3_LIB)
-    return DROPIN3_LIB
+    os.replace(lib + ".tmp", lib)
+    return lib
 
 
 def build_dropin3_driver(root: str) -> str:
@@ -108,3 +112,4 @@ if __name__ == "__main__":
     print(build_dropin())
     print(build_dropin(annulus=True))
     print(build_dropin3())
+    print(build_dropin3(hor=True))

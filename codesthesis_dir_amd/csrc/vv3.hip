@@ -1,4 +1,7 @@
-// vv3.hip -- the 3D rotating-annulus time step of rot3dmg.f on gfx950.
+// vv3.hip -- the 3D annulus time step of rot3dmg.f on gfx950, and of its
+// stationary horizontal-annulus variant rot3dmghor.f (model VV3_MODEL_HOR:
+// gravity in the r-theta plane instead of the rotation terms, swapped wall
+// temperatures, heat flux through both cylinders near mid-axis).
 //
 // Explicit part: one thread per point for Avorticity / Bvorticity /
 // Cvorticity / temperature, the Poisson right-hand sides and the wall
@@ -22,7 +25,8 @@ struct vv3_run {
     size_t nf = 0;                       // doubles per field
     double *f[VV3_NFIELD] = {nullptr};   // device fields (rotated by pointer swap)
     vv3::Poisson3 pois;                  // the three Poisson operators
-    double *heat = nullptr;              // flux terms (nz+2) x np, per-k sums (np), the total
+    double *heat = nullptr;              // flux terms (nz+2) x np, then the totals trans1, trans2
+    double *trig = nullptr;              // VV3_MODEL_HOR: sin, cos of each plane's azimuth (2 x (np+1))
 };
 
 namespace vv3 {
@@ -75,6 +79,7 @@ struct Kc {
     double m1dp2;              // -1/(dp dp)
     double ptldt_inv, dt_inv;  // 1/(Ptl dt), 1/dt
     double grptl;              // Gr Ptl
+    const double *trig;        // VV3_MODEL_HOR: sin(theta_k) at [k], cos at [np+1+k]
 };
 
 // per-point radius with 1/r and 1/r^2
@@ -89,7 +94,8 @@ struct Rad {
     }
 };
 
-// Avorticity at (i, j, k), rot3dmg.f:359-465
+// Avorticity at (i, j, k), rot3dmg.f:359-465; M = 1: rot3dmghor.f:373-483
+template <int M>
 __device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
@@ -156,13 +162,16 @@ __device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j
     const double cnh = dv(K.m1dp2, R.r2);
     const double RHS1 = Arra + dv(Ara, R.r) + dv(Ahha, R.r2) + Azza - dv(2.0 * Bh, R.r2) + dv(Ac, K.ptldt);
     const double ULT = dv(u * Ar + dv(v * Ah, R.r) + w * Az - dv(uelh * BB, R.r) - uelz * CC, K.ptl);
-    const double RHS2 = -2.0 * Re * uelz + dv(dvs(2.0 * Ra * (tem * uelz + thz * u), K.re), K.ptl) - dv(K.grptl, R.r) * thh;
+    double RHS2;
+    if constexpr (M == 1) RHS2 = K.trig[k] * g.Gr * g.Ptl * thz;   // sintheta*Gr*Ptl*thz
+    else RHS2 = -2.0 * Re * uelz + dv(dvs(2.0 * Ra * (tem * uelz + thz * u), K.re), K.ptl) - dv(K.grptl, R.r) * thh;
     const double cf1 = K.ptldt_inv - dv(uelr, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
     const double cf2 = dv(uelr, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
     return (cf2 * X(F.Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
-// Bvorticity at (i, j, k), rot3dmg.f:495-590
+// Bvorticity at (i, j, k), rot3dmg.f:495-590; M = 1: rot3dmghor.f:514-613
+template <int M>
 __device__ double bvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
@@ -214,14 +223,17 @@ __device__ double bvort(const Geo &g, const Kc &K, const Fields &F, int i, int j
     const double cnh = dv(K.m1dp2, R.r2);
     const double RHS1 = Brra + dv(Bra, R.r) + dv(Bhha, R.r2) + Bzza + dv(2.0 * Ah, R.r2) + dv(Bc, K.ptldt);
     const double ULT = dv(u * Br + dv(v * Bh, R.r) + w * Bz - (velr - dv(v, R.r)) * AA - velz * CC, K.ptl);
-    const double RHS2 = -2.0 * Re * velz + r * Ra * thz + dv(dvs(2.0 * Ra * (tem * velz + thz * v), K.re), K.ptl) + K.grptl * thr;
+    double RHS2;
+    if constexpr (M == 1) RHS2 = K.trig[np + 1 + k] * g.Gr * g.Ptl * thz;   // costheta*Gr*Ptl*thz
+    else RHS2 = -2.0 * Re * velz + r * Ra * thz + dv(dvs(2.0 * Ra * (tem * velz + thz * v), K.re), K.ptl) + K.grptl * thr;
     const double uvh = dv(u, R.r) + dv(velh, R.r);
     const double cf1 = K.ptldt_inv - dv(uvh, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
     const double cf2 = dv(uvh, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
     return (cf2 * X(F.Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
-// Cvorticity at (i, j, k), rot3dmg.f:619-725
+// Cvorticity at (i, j, k), rot3dmg.f:619-725; M = 1: rot3dmghor.f:643-754
+template <int M>
 __device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
 {
     const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
@@ -286,7 +298,11 @@ __device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j
     const double cnh = dv(K.m1dp2, R.r2);
     const double RHS1 = Crra + dv(Cra, R.r) + dv(Chha, R.r2) + Czza + dv(Cc, K.ptldt);
     const double ULT = dv(u * Cr + dv(v * Ch, R.r) + w * Cz - welr * AA - dv(welh * BB, R.r), K.ptl);
-    const double RHS2 = -2.0 * Re * welz - Ra * thh + dv(dvs(2.0 * Ra * (tem * welz - thr * u - dv(thh * v, R.r)), K.re), K.ptl);
+    double RHS2;
+    if constexpr (M == 1)   // -1.d0/r*Gr*Ptl*costheta*thh-Gr*Ptl*sintheta*thr
+        RHS2 = -(dv(1.0, R.r) * g.Gr * g.Ptl * K.trig[np + 1 + k] * thh) - K.grptl * K.trig[k] * thr;
+    else
+        RHS2 = -2.0 * Re * welz - Ra * thh + dv(dvs(2.0 * Ra * (tem * welz - thr * u - dv(thh * v, R.r)), K.re), K.ptl);
     const double cf1 = K.ptldt_inv - dv(welz, K.ptl2) - cnr - cnz - cnh;
     const double cf2 = dv(welz, K.ptl2) + cnr + cnz + cnh;
     return (cf2 * X(F.Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
@@ -386,8 +402,8 @@ __device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int 
 // (i, j) point of the routine's loop range (for W = 4 the union i = 1..nr,
 // j = 1..nz, each routine evaluated where its own range holds).  A kernel per
 // routine set keeps its constants and field pointers within the scalar
-// registers.
-template <int W>
+// registers.  M is the model (enum vv3_model).
+template <int W, int M>
 __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
 {
     const int nr = g.nr, nz = g.nz;
@@ -396,9 +412,9 @@ __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
         if (t >= nr * nz) return;
         const int jj = t / nr;
         const int i = 1 + (t - jj * nr), j = 1 + jj, k = blockIdx.y;
-        if (j <= nz - 1) X(F.An, i, j, k) = avort(g, K, F, i, j, k);
-        if (i <= nr - 1 && j <= nz - 1) X(F.Bn, i, j, k) = bvort(g, K, F, i, j, k);
-        if (i <= nr - 1) X(F.Cn, i, j, k) = cvort(g, K, F, i, j, k);
+        if (j <= nz - 1) X(F.An, i, j, k) = avort<M>(g, K, F, i, j, k);
+        if (i <= nr - 1 && j <= nz - 1) X(F.Bn, i, j, k) = bvort<M>(g, K, F, i, j, k);
+        if (i <= nr - 1) X(F.Cn, i, j, k) = cvort<M>(g, K, F, i, j, k);
         return;
     }
     constexpr int which = W;
@@ -411,9 +427,9 @@ __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
     if (t >= ni * nj) return;
     const int jj = t / ni;
     const int i = i0 + (t - jj * ni), j = j0 + jj, k = blockIdx.y;
-    if constexpr (W == 0) X(F.An, i, j, k) = avort(g, K, F, i, j, k);
-    else if constexpr (W == 1) X(F.Bn, i, j, k) = bvort(g, K, F, i, j, k);
-    else if constexpr (W == 2) X(F.Cn, i, j, k) = cvort(g, K, F, i, j, k);
+    if constexpr (W == 0) X(F.An, i, j, k) = avort<M>(g, K, F, i, j, k);
+    else if constexpr (W == 1) X(F.Bn, i, j, k) = bvort<M>(g, K, F, i, j, k);
+    else if constexpr (W == 2) X(F.Cn, i, j, k) = cvort<M>(g, K, F, i, j, k);
     else X(F.thn, i, j, k) = temper(g, K, F, i, j, k);
 }
 
@@ -467,6 +483,57 @@ __global__ void __launch_bounds__(WG) k3_heat_sum(Geo g, const double *hs, doubl
         for (; k < np; ++k) trans1 = trans1 + part[k] * g.dp;
         *out = trans1;
     }
+}
+
+// heat of rot3dmghor.f:913-946.  Per plane k (a thread each) the
+// inner-cylinder sum transa and the outer-cylinder sum transb over
+// j = nz/2-3 .. nz/2+3 in the reference's order (the two end rows with
+// weight 1/120 first, then the rows between with 1/60): hs[k-1] and
+// hs[np + k-1].
+__global__ void __launch_bounds__(WG) k3_heat_hor_terms(Geo g, const double *th, double *hs)
+{
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const double dr = g.dr, dz = g.dz;
+    const int k = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k > np) return;
+    const int nza = nz / 2 - 3, nzb = nz / 2 + 3;
+    auto ta = [&](int j, double w) {
+        return dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) + 9.0 * X(th, 3, j, k)) /
+               (w * dr);
+    };
+    auto tb = [&](int j, double w) {
+        return dz * (184.0 * X(th, nr + 1, j, k) - 225.0 * X(th, nr, j, k) + 50.0 * X(th, nr - 1, j, k) -
+                     9.0 * X(th, nr - 2, j, k)) / (w * dr);
+    };
+    double a = ta(nza, 120.0), b = tb(nza, 120.0);
+    a = a + ta(nzb, 120.0);
+    b = b + tb(nzb, 120.0);
+    for (int j = nza + 1; j <= nzb - 1; ++j) {
+        a = a + ta(j, 60.0);
+        b = b + tb(j, 60.0);
+    }
+    hs[k - 1] = a;
+    hs[np + k - 1] = b;
+}
+
+// trans1 = sum over k of transa*dp and trans2 of transb*dp, in k order
+// (one lane of wave 0 and one of wave 1; 128 threads)
+__global__ void __launch_bounds__(128) k3_heat_hor_sum(Geo g, const double *hs, double *out)
+{
+    const int np = g.np, w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) != 0 || w > 1) return;
+    const double *h = hs + (size_t)w * np;
+    double t = 0.0;
+    int k = 0;
+    for (; k + 8 <= np; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = h[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t = t + v[u] * g.dp;
+    }
+    for (; k < np; ++k) t = t + h[k] * g.dp;
+    out[w] = t;
 }
 
 struct RhsF {
@@ -573,10 +640,11 @@ __global__ void __launch_bounds__(WG) k3_update(Geo g, UpdF F)
         F.Cn[p] = c;
     }
     F.Cp[p] = c;
-    // th: 0 on the inner, 1 on the outer cylinder (rot3dmg.f:987-990)
+    // th: 0 on the inner, 1 on the outer cylinder (rot3dmg.f:987-990);
+    // 1 inside, 0 outside in rot3dmghor.f:1042-1043
     double th = F.thn[p];
     if (i == 0 || i == nr + 1) {
-        th = (i == 0) ? 0.0 : 1.0;
+        th = ((i == 0) == (g.model == VV3_MODEL_HOR)) ? 1.0 : 0.0;
         F.thn[p] = th;
     }
     F.thp[p] = th;
@@ -595,6 +663,7 @@ static Geo geo(const vv3_run *r)
     g.nr = r->nr; g.nz = r->nz; g.np = r->np; g.ldi = r->ldi; g.ldj = r->ldj;
     g.dt = r->cfg.dt; g.dr = r->dr; g.dz = r->dz; g.dp = r->dp;
     g.Ra = r->cfg.Ra; g.Re = r->cfg.Re; g.Ptl = r->cfg.Ptl; g.Gr = r->cfg.Gr; g.r0 = r->cfg.r0;
+    g.model = r->cfg.model;
     return g;
 }
 
@@ -605,12 +674,19 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
     using vv::set_error;
     if (!cfg || !out) { set_error("vv3_create: null argument"); return VV_EINVAL; }
     *out = nullptr;
-    if (cfg->nr < 4 || cfg->nz < 4 || cfg->np < 4 || (cfg->np & 1) || cfg->np > 8192) {
-        set_error("vv3_create: need nr, nz >= 4 and an even np, 4 <= np <= 8192");
+    if (cfg->model != VV3_MODEL_ROT3DMG && cfg->model != VV3_MODEL_HOR) {
+        set_error("vv3_create: unknown model");
         return VV_EINVAL;
     }
+    const bool hor = cfg->model == VV3_MODEL_HOR;
+    if (cfg->nr < 4 || cfg->nz < (hor ? 6 : 4) || cfg->np < 4 || (cfg->np & 1) || cfg->np > 8192) {
+        set_error("vv3_create: need nr >= 4, nz >= 4 (6 for the horizontal annulus) and an even np, "
+                  "4 <= np <= 8192");
+        return VV_EINVAL;
+    }
+    // rot3dmg.f divides by Re; rot3dmghor.f has no rotation terms
     if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->r0 > 0) || !(cfg->zmax > 0) || !(cfg->Ptl > 0) ||
-        cfg->Re == 0.0) {
+        (!hor && cfg->Re == 0.0)) {
         set_error("vv3_create: bad physical parameters");
         return VV_EINVAL;
     }
@@ -631,8 +707,28 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         if (hipMalloc(&r->f[q], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
         if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
     }
-    if (hipMalloc(&r->heat, ((size_t)(cfg->nz + 3) * cfg->np + 1) * sizeof(double)) != hipSuccess)
-        return fail(VV_ENOMEM);
+    {
+        const size_t nh = (size_t)(cfg->nz + 3) * cfg->np + 2;
+        if (hipMalloc(&r->heat, nh * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemsetAsync(r->heat, 0, nh * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+    }
+    if (hor) {
+        // the azimuth of plane k as rot3dmghor.f:374 forms it,
+        // theta=6.283185307d0/np*k, and its sine and cosine from the host's
+        // libm (the library the reference's dsin/dcos call)
+        const int n1 = cfg->np + 1;
+        std::vector<double> t(2 * (size_t)n1);
+        for (int k = 0; k < n1; ++k) {
+            const double theta = 6.283185307 / cfg->np * k;
+            t[k] = std::sin(theta);
+            t[n1 + k] = std::cos(theta);
+        }
+        if (hipMalloc(&r->trig, t.size() * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemcpyAsync(r->trig, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, r->stream) !=
+            hipSuccess)
+            return fail(VV_EHIP);
+        if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
+    }
     {
         const int st = poisson3_setup(r->pois, geo(r));
         if (st) return fail(st);
@@ -651,6 +747,7 @@ int vv3_destroy(vv3_run *r)
         if (r->f[q]) (void)hipFree(r->f[q]);
     poisson3_free(r->pois);
     if (r->heat) (void)hipFree(r->heat);
+    if (r->trig) (void)hipFree(r->trig);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return VV_OK;
@@ -719,6 +816,13 @@ static int launch_heat(vv3_run *r)
 {
     const Geo g = geo(r);
     double *hs = r->heat;
+    if (g.model == VV3_MODEL_HOR) {
+        hipLaunchKernelGGL(k3_heat_hor_terms, dim3(nblk(r->np, WG)), dim3(WG), 0, r->stream, g,
+                           (const double *)r->f[VV3_TH], hs);
+        hipLaunchKernelGGL(k3_heat_hor_sum, dim3(1), dim3(128), 0, r->stream, g, (const double *)hs, heat_total(r));
+        VV_HIP(hipGetLastError());
+        return VV_OK;
+    }
     hipLaunchKernelGGL(k3_heat_terms, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, r->stream, g,
                        (const double *)r->f[VV3_TH], hs);
     hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), r->np * sizeof(double), r->stream, g, (const double *)hs,
@@ -747,10 +851,13 @@ static Kc constants(const vv3_run *r)
     K.ptldt_inv = 1.0 / (Ptl * dt);
     K.dt_inv = 1.0 / dt;
     K.grptl = r->cfg.Gr * Ptl;
+    K.trig = r->trig;
     return K;
 }
 
-static int launch_explicit(vv3_run *r, int mask)
+extern "C++" {
+template <int M>
+static int launch_explicit_m(vv3_run *r, int mask)
 {
     const Geo g = geo(r);
     const Kc K = constants(r);
@@ -760,15 +867,25 @@ static int launch_explicit(vv3_run *r, int mask)
     const long long pc = (long long)(r->nr - 1) * r->nz, pt = (long long)r->nr * (r->nz + 2);
     if ((mask & 7) == 7) {
         const long long pu = (long long)r->nr * r->nz;
-        hipLaunchKernelGGL(k3_explicit<4>, dim3(nblk(pu, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        hipLaunchKernelGGL((k3_explicit<4, M>), dim3(nblk(pu, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     } else {
-        if (mask & 1) hipLaunchKernelGGL(k3_explicit<0>, dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);
-        if (mask & 2) hipLaunchKernelGGL(k3_explicit<1>, dim3(nblk(pb, WG), np1), dim3(WG), 0, r->stream, g, K, F);
-        if (mask & 4) hipLaunchKernelGGL(k3_explicit<2>, dim3(nblk(pc, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (mask & 1)
+            hipLaunchKernelGGL((k3_explicit<0, M>), dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (mask & 2)
+            hipLaunchKernelGGL((k3_explicit<1, M>), dim3(nblk(pb, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (mask & 4)
+            hipLaunchKernelGGL((k3_explicit<2, M>), dim3(nblk(pc, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     }
-    if (mask & 8) hipLaunchKernelGGL(k3_explicit<3>, dim3(nblk(pt, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+    // temperature is the same in both programs
+    if (mask & 8) hipLaunchKernelGGL((k3_explicit<3, 0>), dim3(nblk(pt, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     VV_HIP(hipGetLastError());
     return VV_OK;
+}
+}  // extern "C++"
+
+static int launch_explicit(vv3_run *r, int mask)
+{
+    return (r->cfg.model == VV3_MODEL_HOR) ? launch_explicit_m<1>(r, mask) : launch_explicit_m<0>(r, mask);
 }
 
 static int launch_rhs(vv3_run *r, int which)
@@ -839,6 +956,18 @@ int vv3_heat(vv3_run *r, double *trans1)
     VV_HIP(hipMemcpyAsync(&t, heat_total(r), sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     if (trans1) *trans1 = t;
+    return VV_OK;
+}
+
+int vv3_fluxes(vv3_run *r, double *trans1, double *trans2)
+{
+    if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    double t[2] = {0.0, 0.0};
+    VV_HIP(hipMemcpyAsync(t, heat_total(r), sizeof(t), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    if (trans1) *trans1 = t[0];
+    if (trans2) *trans2 = t[1];
     return VV_OK;
 }
 

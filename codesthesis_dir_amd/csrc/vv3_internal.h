@@ -23,6 +23,7 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
 struct Geo {
     int nr, nz, np, ldi, ldj;
     double dt, dr, dz, dp, Ra, Re, Ptl, Gr, r0;
+    int model;   // enum vv3_model
 };
 
 // One of the three Poisson operators (1: w, 2: v, 3: u).  Unknowns are

@@ -17,6 +17,10 @@
 ! Cvorticity 601, temperature 736, heat 882, update 964, rhsw 1026,
 ! rhsv 1066, rhsu 1111, poissoninit 1153, poisson1 1216, poisson2 1325,
 ! poisson3 1434.
+!
+! Built with -DVV3_HOR=1 it is the drop-in of rot3dmghor.f (the stationary
+! horizontal annulus, model VV3_MODEL_HOR): the same routine names, with
+! heat(trans1,trans2) (rot3dmghor.f:913) in place of heat(trans1).
 
 #ifndef VV3_KR
 #define VV3_KR 33
@@ -27,6 +31,9 @@
 #ifndef VV3_KP
 #define VV3_KP 160
 #endif
+#ifndef VV3_HOR
+#define VV3_HOR 0
+#endif
 
 module vv_dropin3_c
   use iso_c_binding
@@ -35,6 +42,7 @@ module vv_dropin3_c
   type, bind(C) :: vv3_config_t
     integer(c_int) :: nr, nz, np
     real(c_double) :: dt, r0, rmax, zmax, Ra, Re, Ptl, Gr, dr, dz, dp
+    integer(c_int) :: model
   end type
 
   integer(c_int), parameter :: F_AN = 0, F_A = 1, F_AP = 2, F_BN = 3, F_B = 4, F_BP = 5, &
@@ -73,6 +81,11 @@ module vv_dropin3_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: run
       real(c_double), intent(out) :: t1
+    end function
+    integer(c_int) function vv3_fluxes(run, t1, t2) bind(C, name="vv3_fluxes")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: run
+      real(c_double), intent(out) :: t1, t2
     end function
     integer(c_int) function vv3_rhs(run, which) bind(C, name="vv3_rhs")
       import :: c_int, c_ptr
@@ -158,6 +171,7 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%dr = dr
   cfg%dz = dz
   cfg%dp = dp
+  cfg%model = VV3_HOR
   call vv3_check(vv3_create(cfg, 0_c_int, g_run), 'poissoninit')
   nlev = 1
 end subroutine
@@ -203,6 +217,18 @@ subroutine temperature
   call get(F_THN, thn)
 end subroutine
 
+#if VV3_HOR
+! heat(trans1,trans2) of rot3dmghor.f:913: inner- and outer-cylinder fluxes
+subroutine heat(trans1, trans2)
+  use vv_dropin3_c
+  implicit none
+  real(c_double) :: trans1, trans2
+#include "vv_dropin3_state.inc"
+  call put(F_TH, th)
+  call vv3_check(vv3_heat(g_run, trans1), 'heat')
+  call vv3_check(vv3_fluxes(g_run, trans1, trans2), 'heat')
+end subroutine
+#else
 subroutine heat(trans1)
   use vv_dropin3_c
   implicit none
@@ -211,6 +237,7 @@ subroutine heat(trans1)
   call put(F_TH, th)
   call vv3_check(vv3_heat(g_run, trans1), 'heat')
 end subroutine
+#endif
 
 subroutine rhsw(wrh)
   use vv_dropin3_c

@@ -1,5 +1,5 @@
-"""Host interface of the 3D rotating-annulus step (rot3dmg.f) on the HIP
-library: one object per run, methods named after the reference's routines
+"""Host interface of the 3D annulus step (rot3dmg.f, and rot3dmghor.f with
+model=1) on the HIP library: one object per run, methods named after the reference's routines
 (Avorticity, ..., poisson1/2/3, update), fields as numpy arrays of shape
 (np+1, nz+2, nr+2) (X[k, j, i] is X(i, j, k) of rot3dmg.f).
 
@@ -24,12 +24,16 @@ class VV3Config(ctypes.Structure):
                 ("dt", ctypes.c_double), ("r0", ctypes.c_double), ("rmax", ctypes.c_double),
                 ("zmax", ctypes.c_double), ("Ra", ctypes.c_double), ("Re", ctypes.c_double),
                 ("Ptl", ctypes.c_double), ("Gr", ctypes.c_double), ("dr", ctypes.c_double),
-                ("dz", ctypes.c_double), ("dp", ctypes.c_double)]
+                ("dz", ctypes.c_double), ("dp", ctypes.c_double), ("model", ctypes.c_int)]
+
+MODEL_ROT3DMG = 0   # rot3dmg.f: rotating annulus
+MODEL_HOR = 1       # rot3dmghor.f: stationary horizontal annulus
 
 
 @dataclass
 class Annulus3DConfig:
-    """The scalars of rot3dmg.f's input deck (rot3dmg.f:59-86)."""
+    """The scalars of rot3dmg.f's input deck (rot3dmg.f:59-86); model 1
+    steps rot3dmghor.f's physics instead (Re unused)."""
     nr: int
     nz: int
     np: int
@@ -41,10 +45,11 @@ class Annulus3DConfig:
     r0: float = 0.5
     rmax: float = 1.0
     zmax: float = 0.5
+    model: int = MODEL_ROT3DMG
 
     def c(self) -> VV3Config:
         return VV3Config(self.nr, self.nz, self.np, self.dt, self.r0, self.rmax, self.zmax,
-                         self.Ra, self.Re, self.Ptl, self.Gr, 0.0, 0.0, 0.0)
+                         self.Ra, self.Re, self.Ptl, self.Gr, 0.0, 0.0, 0.0, self.model)
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -62,6 +67,7 @@ def _bind():
     L.vv3_vorticity.argtypes = [vp, ctypes.c_int]
     L.vv3_temperature.argtypes = [vp]
     L.vv3_heat.argtypes = [vp, _P]
+    L.vv3_fluxes.argtypes = [vp, _P, _P]
     L.vv3_rhs.argtypes = [vp, ctypes.c_int]
     L.vv3_poisson.argtypes = [vp, ctypes.c_int]
     L.vv3_update.argtypes = [vp]
@@ -125,6 +131,13 @@ class Annulus3D:
         t = ctypes.c_double()
         check(self._L.vv3_heat(self._h, ctypes.byref(t)), "vv3_heat")
         return t.value
+
+    def fluxes(self) -> tuple[float, float]:
+        """(trans1, trans2) of the last heat evaluation: rot3dmghor.f's
+        heat(trans1,trans2) (trans2 = 0 for rot3dmg.f)."""
+        t1, t2 = ctypes.c_double(), ctypes.c_double()
+        check(self._L.vv3_fluxes(self._h, ctypes.byref(t1), ctypes.byref(t2)), "vv3_fluxes")
+        return t1.value, t2.value
 
     def rhsw(self):
         check(self._L.vv3_rhs(self._h, 1), "vv3_rhs")

@@ -1,7 +1,8 @@
-/* vvmi3.h -- C ABI of the MI355X 3D rotating-annulus time step
- * (rot3dmg.f: 3D velocity-vorticity model on a staggered grid, periodic in
- * phi, walls in r and z).  FP64 throughout; status codes and error text are
- * shared with vvmi.h (VV_OK, VV_EINVAL, ..., vv_last_error()).
+/* vvmi3.h -- C ABI of the MI355X 3D annulus time step (rot3dmg.f: 3D
+ * velocity-vorticity model on a staggered grid, periodic in phi, walls in r
+ * and z; and its stationary horizontal-annulus variant rot3dmghor.f, model
+ * VV3_MODEL_HOR).  FP64 throughout; status codes and error text are shared
+ * with vvmi.h (VV_OK, VV_EINVAL, ..., vv_last_error()).
  *
  * Fields are (np+1) x (nz+2) x (nr+2) doubles, i fastest, then j, then k:
  * X(i,j,k) at host[(k*(nz+2) + j)*(nr+2) + i], i = 0..nr+1, j = 0..nz+1,
@@ -18,6 +19,19 @@ extern "C" {
 
 typedef struct vv3_run vv3_run;
 
+/* which program's physics the run steps */
+enum vv3_model {
+    VV3_MODEL_ROT3DMG = 0,   /* rot3dmg.f: rotating annulus (Coriolis and
+                                centrifugal terms through Re, axial gravity
+                                through Gr), T = 0 inside / 1 outside      */
+    VV3_MODEL_HOR = 1        /* rot3dmghor.f: stationary horizontal annulus,
+                                gravity in the r-theta plane through Gr
+                                (rot3dmghor.f:479, 606, 749), no Coriolis
+                                terms, T = 1 inside / 0 outside
+                                (rot3dmghor.f:1042), heat flux through both
+                                cylinders near mid-axis (rot3dmghor.f:913) */
+};
+
 /* the scalars of rot3dmg.f's input deck (rot3dmg.f:59-86) */
 typedef struct {
     int    nr, nz, np;          /* intervals in r, z, phi (np even)          */
@@ -27,6 +41,7 @@ typedef struct {
     double dr, dz, dp;          /* grid spacings as the caller computed them
                                    (COMMON /num/); 0 derives them as
                                    rot3dmg.f:91-93 does                       */
+    int    model;               /* enum vv3_model (0: rot3dmg.f)             */
 } vv3_config;
 
 enum vv3_field {
@@ -39,7 +54,8 @@ enum vv3_field {
  * builds its multigrid levels in poissoninit, rot3dmg.f:1153).  The three
  * Poisson operators are solved exactly and separably: a real DFT in phi, the
  * eigenbasis of the z rows, a banded solve in r per (mode, z eigenvalue).
- * Needs nr, nz >= 4 and an even np, 4 <= np <= 8192 (VV_EINVAL otherwise). */
+ * Needs nr, nz >= 4 (nz >= 6 for VV3_MODEL_HOR) and an even np,
+ * 4 <= np <= 8192 (VV_EINVAL otherwise). */
 int vv3_create(const vv3_config *cfg, int device, vv3_run **out);
 int vv3_destroy(vv3_run *run);
 
@@ -57,8 +73,13 @@ int vv3_get(vv3_run *run, int field, double *host, int ldi, int ldj);
 int vv3_vorticity(vv3_run *run, int which);
 /* temperature (rot3dmg.f:736): thn on i = 1..nr, j = 0..nz+1 */
 int vv3_temperature(vv3_run *run);
-/* heat (rot3dmg.f:882): inner-wall heat flux trans1 */
+/* heat (rot3dmg.f:882): inner-wall heat flux trans1; for VV3_MODEL_HOR
+ * the inner-cylinder flux of heat(trans1,trans2) (rot3dmghor.f:913), whose
+ * outer-cylinder flux vv3_fluxes returns */
 int vv3_heat(vv3_run *run, double *trans1);
+/* the wall fluxes of the last heat evaluation (vv3_heat or vv3_step):
+ * trans1, and trans2 of rot3dmghor.f (0 for rot3dmg.f); either may be NULL */
+int vv3_fluxes(vv3_run *run, double *trans1, double *trans2);
 /* rhsw / rhsv / rhsu (rot3dmg.f:1026, 1066, 1111): which = 1 / 2 / 3 */
 int vv3_rhs(vv3_run *run, int which);
 /* poisson1 / poisson2 / poisson3 (rot3dmg.f:1216, 1325, 1434): solves the

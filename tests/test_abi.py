@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def test_3d_header_declares_entry_points():
     names = declared()
     for n in ("vv3_create", "vv3_destroy", "vv3_put", "vv3_get", "vv3_vorticity", "vv3_temperature",
-              "vv3_heat", "vv3_rhs", "vv3_poisson", "vv3_update", "vv3_step"):
+              "vv3_heat", "vv3_fluxes", "vv3_rhs", "vv3_poisson", "vv3_update", "vv3_step"):
         assert n in names
     src = open(HEADERS[1]).read()
     assert src.count("rot3dmg.f:") >= 8
@@ -63,11 +63,13 @@ def test_fortran_dropin_exports_reference_routine_names():
         assert hasattr(lib, n), n
 
 
-def test_fortran_dropin3_exports_reference_routine_names():
-    """libvvdropin3.so exports rot3dmg.f's routine names."""
-    path = os.path.join(ROOT, "codesthesis_dir_amd", "libvvdropin3.so")
+@pytest.mark.parametrize("name", ["libvvdropin3.so", "libvvdropin3h.so"])
+def test_fortran_dropin3_exports_reference_routine_names(name):
+    """libvvdropin3.so exports rot3dmg.f's routine names, libvvdropin3h.so
+    rot3dmghor.f's (the same names)."""
+    path = os.path.join(ROOT, "codesthesis_dir_amd", name)
     if not os.path.exists(path):
-        pytest.skip("libvvdropin3.so not built")
+        pytest.skip(name + " not built")
     lib = ctypes.CDLL(path)
     for n in ("poissoninit_", "poisson1_", "poisson2_", "poisson3_", "avorticity_", "bvorticity_",
               "cvorticity_", "temperature_", "heat_", "rhsw_", "rhsv_", "rhsu_", "update_"):

@@ -276,3 +276,93 @@ def test_step_equals_routine_calls():
         assert np.array_equal(a.get(n), b.get(n)), n
     a.close()
     b.close()
+
+
+# ---- rot3dmghor.f (stationary horizontal annulus, model 1) ----
+
+def test_hor_explicit_routines_match_reference_fixture():
+    """rot3dmghor.f's Avorticity, Bvorticity, Cvorticity (gravity in the
+    r-theta plane, rot3dmghor.f:479, 606, 749), temperature, heat(trans1,
+    trans2), rhsw/v/u and update (T = 1 inside, 0 outside) on the fixture
+    written from the rot3dmghor.f build: equal bit for bit, separately and
+    with the three vorticity routines in one launch."""
+    g = np.load(os.path.join(HERE, "golden", "r3h_explicit.npz"))
+    run = _gpu(model=1)
+    fused = _gpu(model=1)
+    for n in O3.FIELDS:
+        run.put(n, g["in_" + n])
+        fused.put(n, g["in_" + n])
+    run.Avorticity()
+    run.Bvorticity()
+    run.Cvorticity()
+    run.temperature()
+    fused.vorticity()
+    for rt, name in [("avorticity", "An"), ("bvorticity", "Bn"), ("cvorticity", "Cn"), ("temperature", "thn")]:
+        assert np.array_equal(run.get(name), g[rt]), rt
+        if name != "thn":
+            assert np.array_equal(fused.get(name), g[rt]), rt
+    t1 = run.heat()
+    assert (t1,) + run.fluxes()[1:] == tuple(float(x) for x in g["heat"])
+    for fn, name in [("rhsw", "wrh"), ("rhsv", "vrh"), ("rhsu", "urh")]:
+        getattr(run, fn)()
+        assert np.array_equal(run.get(name), g[name]), fn
+    run.update()
+    for n in ("A", "Ap", "B", "Bp", "C", "Cp", "th", "thp", "An", "Bn", "Cn", "thn"):
+        assert np.array_equal(run.get(n), g["upd_" + n]), n
+    run.close()
+    fused.close()
+
+
+@pytest.mark.parametrize("shape", [(9, 7, 14), (12, 12, 16)])
+def test_hor_explicit_routines_match_oracle(shape):
+    """Other grid shapes (nz = 7: the heat band starts at the wall row)
+    against the oracle's rot3dmghor.f restatement, bit for bit."""
+    nr, nz, np_ = shape
+    run = _gpu(nr, nz, np_, model=1)
+    ora = O3.Run3(nr, nz, np_, model=1, **PARAMS)
+    f = _state(np.random.default_rng(17), ora.shape, 150.0)
+    for n, a in f.items():
+        run.put(n, a)
+        ora[n] = a
+    run.vorticity()
+    run.temperature()
+    for rt in ("avorticity", "bvorticity", "cvorticity", "temperature"):
+        ora.call(rt)
+    for n in ("An", "Bn", "Cn", "thn"):
+        assert np.array_equal(run.get(n), ora[n]), n
+    run.heat()
+    assert run.fluxes() == ora.heat2()
+    run.update()
+    ora.call("update")
+    for n in ("A", "Ap", "B", "Bp", "C", "Cp", "th", "thp", "An", "Bn", "Cn", "thn"):
+        assert np.array_equal(run.get(n), ora[n]), n
+    run.close()
+
+
+def test_hor_step_trajectory_matches_oracle():
+    """Four whole steps of rot3dmghor.f's time loop (rot3dmghor.f:251-263):
+    both wall fluxes and the fields agree with the oracle to rounding."""
+    nr, nz, np_ = 10, 8, 12
+    p = dict(PARAMS, dt=2e-4, Gr=2e3)
+    run = _gpu(nr, nz, np_, model=1, dt=2e-4, Gr=2e3)
+    ora = O3.Run3(nr, nz, np_, model=1, **p)
+    for n, a in _smooth_state(nr, nz, np_, 12).items():
+        run.put(n, a)
+        ora[n] = a
+    for s in range(4):
+        run.step(1)
+        tg = run.fluxes()
+        for rt in ("avorticity", "bvorticity", "cvorticity", "temperature"):
+            ora.call(rt)
+        to = ora.heat2()
+        for which, fn in ((1, "rhsw"), (2, "rhsv"), (3, "rhsu")):
+            ora.call(fn)
+            oracle_poisson(ora, which)
+        ora.call("update")
+        for a, b in zip(tg, to):
+            assert abs(a - b) <= 1e-10 * abs(b), (s, tg, to)
+    for n in ("A", "B", "C", "th", "uel", "vel", "wel", "Ap", "thp", "welp"):
+        got, want = run.get(n), ora[n]
+        scale = max(np.max(np.abs(want)), 1e-300)
+        assert np.max(np.abs(got - want)) <= 1e-10 * scale, n
+    run.close()

@@ -1,8 +1,10 @@
 """The reference's own main programs, unmodified, on the GPU.
 
 oracle/Makefile links the main programs of velvor2dperiodicrbconv.f (lines
-1-577) and rot3dmg.f (lines 1-333) against the Fortran drop-ins (libvvdropin,
-libvvdropin3) into oracle/_ref/, next to the reference's own executables.
+1-577), velvor2dnewsch3.4.f (1-589), rot3dmg.f (1-333) and rot3dmghor.f
+(1-347) against the Fortran drop-ins (libvvdropin, libvvdropin_ann,
+libvvdropin3, libvvdropin3h) into oracle/_ref/, next to the reference's own
+executables.
 Both run the same input deck; their printed heat-transfer series and the
 field files they write are compared.  The drop-ins solve the Poisson
 problems exactly (2D: to a relative residual of 1e-10), the reference stops
@@ -79,6 +81,32 @@ def test_rot3dmg_main_program_on_the_gpu():
                 for n in ("T", "A", "B", "C", "U", "V", "W")}
     print(f"rot3dmg main: heat series rel {eh:.3e}; fields {errs}")
     # measured on the MI355X: heat 1.4e-12, T 7.5e-12, A..W <= 6e-6
+    assert eh < 1e-9
+    assert errs["T"] < 1e-9
+    for n in ("A", "B", "C", "U", "V", "W"):
+        assert errs[n] < 1e-4, (n, errs[n])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "r3h_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_rot3dmghor_main_program_on_the_gpu():
+    """rot3dmghor.f's main program (the stationary horizontal annulus: its
+    deck, initial condition, time loop with heat(trans1,trans2) and output
+    files) through the drop-in built for it (libvvdropin3h.so) against the
+    reference executable.  The reference stops its solves at an absolute
+    residual of 1e-6 (rot3dmghor.f:1599)."""
+    nts = 10
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "16", "16", "32", str(nts),
+                      "0.5 1.0 0.5", "1e5", "100.0", "1e4", "1.0", "0.0", "0.1", "4", "no", "n"]) + "\n"
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        out_r = _run(os.path.join(REF, "r3h_ref"), deck, dr)
+        out_g = _run(os.path.join(REF, "r3h_main_gpu"), deck, dg)
+        hr, hg = _series(out_r, 2), _series(out_g, 2)
+        assert hr.shape == hg.shape == (nts, 2), (hr.shape, hg.shape)
+        eh = _rel(hg, hr)
+        errs = {n: _rel(_field(os.path.join(dg, n), 2), _field(os.path.join(dr, n), 2))
+                for n in ("T", "A", "B", "C", "U", "V", "W")}
+    print(f"rot3dmghor main: heat series rel {eh:.3e}; fields {errs}")
     assert eh < 1e-9
     assert errs["T"] < 1e-9
     for n in ("A", "B", "C", "U", "V", "W"):
