@@ -12,6 +12,7 @@
 // The three Poisson solves are exact separable solves (vv3_poisson.hip).
 
 #include "vv3_internal.h"
+#include "vv_reduce.h"
 
 #include <cmath>
 #include <vector>
@@ -27,7 +28,10 @@ struct vv3_run {
     vv3::Poisson3 pois;                  // the three Poisson operators
     double *heat = nullptr;              // flux terms (nz+2) x np, then the totals trans1, trans2
     double *trig = nullptr;              // VV3_MODEL_HOR: sin, cos of each plane's azimuth (2 x (np+1))
+    double *kepart = nullptr;            // kinetic energy: partials [KE3_MAXB], then the total
 };
+
+constexpr int KE3_MAXB = 1024;
 
 namespace vv3 {
 
@@ -536,6 +540,47 @@ __global__ void __launch_bounds__(128) k3_heat_hor_sum(Geo g, const double *hs, 
     out[w] = t;
 }
 
+// Kinetic energy (no counterpart in the reference; vvmi3.h gives the
+// definition).  Workgroup b sums the planes k = b, b + gridDim.x, ... over
+// their (i, j) points; partials are reduced by wavefront shuffles and
+// k3_energy_sum adds them in index order, so the result repeats bit for bit.
+__global__ void __launch_bounds__(WG) k3_energy(Geo g, const double *uel, const double *vel, const double *wel,
+                                                double *part)
+{
+    __shared__ double sh[WG / 64];
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const int plane = g.ldi * g.ldj;
+    double s = 0.0;
+    for (int k = blockIdx.x; k < np; k += gridDim.x) {
+        for (int t = threadIdx.x; t < plane; t += WG) {
+            const int j = t / g.ldi, i = t - j * g.ldi;
+            if (i < 1 || i > nr || j < 1 || j > nz) continue;
+            const double rc = g.r0 + ((double)i - 0.5) * g.dr;
+            const double v = X(vel, i, j, k);
+            s += rc * v * v;
+            if (j <= nz - 1) {
+                const double w = X(wel, i, j, k);
+                s += rc * w * w;
+            }
+            if (i <= nr - 1) {
+                const double u = X(uel, i, j, k);
+                s += (g.r0 + (double)i * g.dr) * u * u;
+            }
+        }
+    }
+    s = vv::block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(WG) k3_energy_sum(const double *part, int nb, double scale, double *out)
+{
+    __shared__ double sh[WG / 64];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += WG) s += part[b];
+    s = vv::block_sum(s, sh);
+    if (threadIdx.x == 0) *out = scale * s;
+}
+
 struct RhsF {
     const double *An, *Bn, *Cn, *wel;
     double *wrh, *vrh, *urh;
@@ -712,6 +757,7 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         if (hipMalloc(&r->heat, nh * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
         if (hipMemsetAsync(r->heat, 0, nh * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
     }
+    if (hipMalloc(&r->kepart, (KE3_MAXB + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if (hor) {
         // the azimuth of plane k as rot3dmghor.f:374 forms it,
         // theta=6.283185307d0/np*k, and its sine and cosine from the host's
@@ -748,6 +794,7 @@ int vv3_destroy(vv3_run *r)
     poisson3_free(r->pois);
     if (r->heat) (void)hipFree(r->heat);
     if (r->trig) (void)hipFree(r->trig);
+    if (r->kepart) (void)hipFree(r->kepart);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return VV_OK;
@@ -956,6 +1003,22 @@ int vv3_heat(vv3_run *r, double *trans1)
     VV_HIP(hipMemcpyAsync(&t, heat_total(r), sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     if (trans1) *trans1 = t;
+    return VV_OK;
+}
+
+int vv3_energy(vv3_run *r, double *ke)
+{
+    if (!r || !ke) { vv::set_error("vv3_energy: null argument"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    const int nb = r->np < KE3_MAXB ? r->np : KE3_MAXB;
+    const Geo g = geo(r);
+    hipLaunchKernelGGL(k3_energy, dim3(nb), dim3(WG), 0, r->stream, g, (const double *)r->f[VV3_UEL],
+                       (const double *)r->f[VV3_VEL], (const double *)r->f[VV3_WEL], r->kepart);
+    hipLaunchKernelGGL(k3_energy_sum, dim3(1), dim3(WG), 0, r->stream, (const double *)r->kepart, nb,
+                       0.5 * r->dr * r->dz * r->dp, r->kepart + KE3_MAXB);
+    VV_HIP(hipGetLastError());
+    VV_HIP(hipMemcpyAsync(ke, r->kepart + KE3_MAXB, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
 

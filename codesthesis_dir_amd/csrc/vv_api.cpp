@@ -346,6 +346,7 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     if ((st = mg_build(r))) return fail(st);
     r->diag_cap = 1 << 16;
     if (hipMalloc(&r->diag, (size_t)r->diag_cap * 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->kepart, (vv::KE_MAXB + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
     if (nlev) *nlev = r->nlev_v;
     *out = r;
@@ -360,7 +361,7 @@ int vv_destroy(vv_run *r)
     double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                         r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
                         r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                        r->tab_p, r->mg_pool, r->diag, r->part};
+                        r->tab_p, r->mg_pool, r->diag, r->part, r->kepart};
     for (double *f : fields)
         if (f) (void)hipFree(f);
     if (r->mg_state) (void)hipFree(r->mg_state);
@@ -451,6 +452,17 @@ int vv_heat(vv_run *r, double *trans1, double *trans2)
     VV_HIP(hipStreamSynchronize(r->stream));
     if (trans1) *trans1 = h[0];
     if (trans2) *trans2 = h[1];
+    return VV_OK;
+}
+
+int vv_energy(vv_run *r, double *ke)
+{
+    if (!r || !ke) return VV_EINVAL;
+    VV_HIP(hipSetDevice(r->device));
+    const int st = vv::launch_energy(r);
+    if (st) return st;
+    VV_HIP(hipMemcpyAsync(ke, r->kepart + vv::KE_MAXB, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
 

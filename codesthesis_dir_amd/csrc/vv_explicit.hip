@@ -658,6 +658,41 @@ __global__ void __launch_bounds__(WG) k_update(UpdArgs a)
 
 inline int nblk(long long n, int wg) { return (int)((n + wg - 1) / wg); }
 
+// Kinetic energy 1/2 dr dp sum_k [sum_{i=1..nr} wT vel^2 + sum_{i=1..nr-1}
+// wC uel^2] over the columns k = 0..np-1 (no counterpart in the reference;
+// the weights are the row radii in the annulus, 1 in the box: wr = 1 / 0
+// selects).  Workgroup b sums the columns b, b + gridDim.x, ... (rows across
+// the threads), reduced by wavefront shuffles; k_energy_sum adds the
+// partials.  The order is fixed, so the result repeats bit for bit.
+__global__ void __launch_bounds__(WG) k_energy(const double *uel, const double *vel, const double *rC,
+                                               const double *rT, double wr, int nr, int np, int ld, double *part)
+{
+    __shared__ double sh[WG / 64];
+    double s = 0.0;
+    for (int k = blockIdx.x; k < np; k += gridDim.x) {
+        const double *u = uel + (size_t)k * ld, *v = vel + (size_t)k * ld;
+        for (int i = 1 + (int)threadIdx.x; i <= nr; i += WG) {
+            const double vv = v[i];
+            s += (wr != 0.0 ? rT[i] : 1.0) * vv * vv;
+            if (i <= nr - 1) {
+                const double uu = u[i];
+                s += (wr != 0.0 ? rC[i] : 1.0) * uu * uu;
+            }
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(WG) k_energy_sum(const double *part, int nb, double scale, double *out)
+{
+    __shared__ double sh[WG / 64];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += WG) s += part[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = scale * s;
+}
+
 }  // namespace
 
 static int compact_launch(vv_run *r, int mask)
@@ -729,6 +764,18 @@ int launch_heat(vv_run *r, double *dst2)
     hipLaunchKernelGGL(k_heat, dim3(1), dim3(WG), 0, r->stream, (const double *)r->thf, r->nr,
                        r->np, r->ld, 1.0 / r->dr, r->dp, dst2);
     prof_end(r, VV_K_HEAT, pe_, 8.0 * 10.0 * (r->np + 1));
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_energy(vv_run *r)
+{
+    const int nb = r->np < KE_MAXB ? r->np : KE_MAXB;
+    const double w = (r->cfg.geom == VV_GEOM_ANNULUS) ? 1.0 : 0.0;
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(WG), 0, r->stream, (const double *)r->uel, (const double *)r->vel,
+                       (const double *)r->rC, (const double *)r->rT, w, r->nr, r->np, r->ld, r->kepart);
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(WG), 0, r->stream, (const double *)r->kepart, nb,
+                       0.5 * r->dr * r->dp, r->kepart + KE_MAXB);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

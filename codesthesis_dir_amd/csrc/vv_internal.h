@@ -151,6 +151,8 @@ struct vv_run {
     // update, npart [2][MG_NORM_MAXB] residual maxima
     double *part = nullptr;
     double *fpart = nullptr, *upart = nullptr, *npart = nullptr;
+    // kinetic energy: per-workgroup partial sums [KE_MAXB], then the total
+    double *kepart = nullptr;
     int nbx = 0;
     long long steps = 0;        // completed steps since vv_start
     int last_cycles[2] = {0, 0};
@@ -189,4 +191,7 @@ int launch_mg_scatter_out(vv_run *r, int which);
 int launch_compact_mask(vv_run *r, int mask);
 int launch_heat_from(vv_run *r, const double *th, double *dst2);
 int launch_rhs_fields(vv_run *r, int which, int solve);
+// kinetic energy of uel, vel into kepart[KE_MAXB]
+constexpr int KE_MAXB = 1024;
+int launch_energy(vv_run *r);
 }  // namespace vv

@@ -116,6 +116,12 @@ class PeriodicBox:
         L.check(L.lib().vv_heat(self._run, ctypes.byref(t1), ctypes.byref(t2)), "vv_heat")
         return t1.value, t2.value
 
+    def energy(self) -> float:
+        """Kinetic energy of the current velocities (vv_energy)."""
+        ke = ctypes.c_double()
+        L.check(L.lib().vv_energy(self._run, ctypes.byref(ke)), "vv_energy")
+        return ke.value
+
     def rhs(self):
         L.check(L.lib().vv_rhs(self._run), "vv_rhs")
 

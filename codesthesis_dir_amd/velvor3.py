@@ -68,6 +68,7 @@ def _bind():
     L.vv3_temperature.argtypes = [vp]
     L.vv3_heat.argtypes = [vp, _P]
     L.vv3_fluxes.argtypes = [vp, _P, _P]
+    L.vv3_energy.argtypes = [vp, _P]
     L.vv3_rhs.argtypes = [vp, ctypes.c_int]
     L.vv3_poisson.argtypes = [vp, ctypes.c_int]
     L.vv3_update.argtypes = [vp]
@@ -138,6 +139,12 @@ class Annulus3D:
         t1, t2 = ctypes.c_double(), ctypes.c_double()
         check(self._L.vv3_fluxes(self._h, ctypes.byref(t1), ctypes.byref(t2)), "vv3_fluxes")
         return t1.value, t2.value
+
+    def energy(self) -> float:
+        """Kinetic energy of the current velocities (vv3_energy)."""
+        t = ctypes.c_double()
+        check(self._L.vv3_energy(self._h, ctypes.byref(t)), "vv3_energy")
+        return t.value
 
     def rhsw(self):
         check(self._L.vv3_rhs(self._h, 1), "vv3_rhs")

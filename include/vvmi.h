@@ -118,6 +118,15 @@ int vv_heat(vv_run *run, double *trans1, double *trans2);
 int vv_rhs(vv_run *run);
 int vv_update(vv_run *run, int *nclamp);
 
+/* Kinetic energy of the current velocities.  No counterpart in the
+ * reference (its diagnostics are the wall heat fluxes); the definition is
+ *   ke = 1/2 dr dp sum_{k=0..np-1} [ sum_{i=1..nr}   wT(i) vel(i,k)^2
+ *                                  + sum_{i=1..nr-1} wC(i) uel(i,k)^2 ]
+ * over the unknowns of poisson2 / poisson3, with w = 1 in the box and the
+ * row radius in the annulus (wT = r0 + (i-1/2) dr, wC = r0 + i dr), i.e.
+ * 1/2 of the integral of |u|^2 over the domain by the midpoint rule. */
+int vv_energy(vv_run *run, double *ke);
+
 /* Poisson solves with the reference's in/out conventions: initial guess
  * 2*psi - psip, on return psip = previous psi, psi = solution with zero
  * wall rows and column 0 equal to column np.

@@ -80,6 +80,14 @@ int vv3_heat(vv3_run *run, double *trans1);
 /* the wall fluxes of the last heat evaluation (vv3_heat or vv3_step):
  * trans1, and trans2 of rot3dmghor.f (0 for rot3dmg.f); either may be NULL */
 int vv3_fluxes(vv3_run *run, double *trans1, double *trans2);
+/* Kinetic energy of the current velocities (no counterpart in the
+ * reference):
+ *   ke = 1/2 dr dz dp sum_{k=0..np-1} [ sum_{j=1..nz,   i=1..nr-1} (r0 + i dr) uel^2
+ *                                     + sum_{j=1..nz,   i=1..nr} (r0 + (i-1/2) dr) vel^2
+ *                                     + sum_{j=1..nz-1, i=1..nr} (r0 + (i-1/2) dr) wel^2 ],
+ * each over the unknowns of its Poisson solve: 1/2 of the volume integral
+ * of |u|^2 by the midpoint rule. */
+int vv3_energy(vv3_run *run, double *ke);
 /* rhsw / rhsv / rhsu (rot3dmg.f:1026, 1066, 1111): which = 1 / 2 / 3 */
 int vv3_rhs(vv3_run *run, int which);
 /* poisson1 / poisson2 / poisson3 (rot3dmg.f:1216, 1325, 1434): solves the

@@ -32,6 +32,7 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.vo_residual.restype = ctypes.c_double
         L.vo_residual_g.restype = ctypes.c_double
+        L.vo_energy.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -138,6 +139,10 @@ class Run:
     def nusselt(self, trans, width):
         height = self.p.dr * self.nr
         return trans * height / width
+
+    def energy(self):
+        """Kinetic energy of uel, vel (the build's definition, vv_oracle.h)."""
+        return lib().vo_energy(ctypes.byref(self.p), P(self.arr["uel"]), P(self.arr["vel"]))
 
 
 def tdma(a, b, c, r):

@@ -39,6 +39,7 @@ def lib():
             getattr(_LIB, fn).restype = None
         _LIB.v3_heat.restype = ctypes.c_double
         _LIB.v3_heat2.restype = None
+        _LIB.v3_energy.restype = ctypes.c_double
         _LIB.v3_residual.restype = ctypes.c_double
         _LIB.v3_poisson_exact.restype = ctypes.c_int
     return _LIB
@@ -85,6 +86,10 @@ class Run3:
         out = np.zeros(2)
         lib().v3_heat2(ctypes.byref(self.p), ctypes.byref(self.c), _P(out))
         return float(out[0]), float(out[1])
+
+    def energy(self):
+        """Kinetic energy of the velocities (the build's definition)."""
+        return lib().v3_energy(ctypes.byref(self.p), ctypes.byref(self.c))
 
     def residual(self, which, psi, rhs):
         res = np.zeros(self.shape)

@@ -491,6 +491,31 @@ void v3_update(const v3_params *P, v3_fields *F)
             }
 }
 
+/* kinetic energy, the build's definition (vv3_oracle.h) */
+double v3_energy(const v3_params *P, const v3_fields *F)
+{
+    const int nr = P->nr, nz = P->nz, np = P->np;
+    double s = 0.0;
+    for (int k = 0; k < np; ++k) {
+        for (int j = 1; j <= nz; ++j)
+            for (int i = 1; i <= nr - 1; ++i) {
+                const double u = X(F->uel, i, j, k);
+                s += (P->r0 + (double)i * P->dr) * u * u;
+            }
+        for (int j = 1; j <= nz; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                const double v = X(F->vel, i, j, k);
+                s += (P->r0 + ((double)i - 0.5) * P->dr) * v * v;
+            }
+        for (int j = 1; j <= nz - 1; ++j)
+            for (int i = 1; i <= nr; ++i) {
+                const double w = X(F->wel, i, j, k);
+                s += (P->r0 + ((double)i - 0.5) * P->dr) * w * w;
+            }
+    }
+    return 0.5 * P->dr * P->dz * P->dp * s;
+}
+
 /* rhsw, rot3dmg.f:1026-1048 */
 void v3_rhsw(const v3_params *P, v3_fields *F)
 {

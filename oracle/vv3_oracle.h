@@ -38,6 +38,13 @@ double v3_heat(const v3_params *P, const v3_fields *F); /* rot3dmg.f:882-905; tr
  * the two cylinders between nz/2-3 and nz/2+3; 0 for rot3dmg.f) */
 void v3_heat2(const v3_params *P, const v3_fields *F, double out[2]);
 void v3_update(const v3_params *P, v3_fields *F);       /* rot3dmg.f:964-1018 */
+/* Kinetic energy of the velocities (no counterpart in the reference; the
+ * build's definition): 1/2 dr dz dp sum over k = 0..np-1 of
+ *   sum_{j=1..nz, i=1..nr-1}   (r0 + i dr)       uel^2
+ * + sum_{j=1..nz, i=1..nr}     (r0 + (i-1/2) dr) vel^2
+ * + sum_{j=1..nz-1, i=1..nr}   (r0 + (i-1/2) dr) wel^2,
+ * each over the unknowns of its Poisson solve (poisson3, 2, 1). */
+double v3_energy(const v3_params *P, const v3_fields *F);
 void v3_rhsw(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1026-1048 */
 void v3_rhsv(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1066-1093 */
 void v3_rhsu(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1111-1134 */

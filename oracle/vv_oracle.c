@@ -410,6 +410,24 @@ void vo_heat(const vo_params *p, const double *th, double *trans1, double *trans
     *trans2 = t2;
 }
 
+/* kinetic energy, the build's definition (vv_oracle.h) */
+double vo_energy(const vo_params *p, const double *uel, const double *vel)
+{
+    const int nr = p->nr, np = p->np, ld = p->ld;
+    double s = 0.0;
+    for (int k = 0; k < np; ++k) {
+        for (int i = 1; i <= nr; ++i) {
+            const double w = p->geom ? p->r0 + ((double)i - 0.5) * p->dr : 1.0;
+            s += w * vel[IX(i, k)] * vel[IX(i, k)];
+        }
+        for (int i = 1; i <= nr - 1; ++i) {
+            const double w = p->geom ? p->r0 + (double)i * p->dr : 1.0;
+            s += w * uel[IX(i, k)] * uel[IX(i, k)];
+        }
+    }
+    return 0.5 * p->dr * p->dp * s;
+}
+
 /* rb.f:1629-1676 */
 void vo_rhsv(const vo_params *p, const double *Cn, double *vrh)
 {

@@ -72,6 +72,12 @@ void vo_pdma(double *a, double *b, double *c, double *d, double *e,
 void vo_cvorticity(const vo_params *p, vo_fields *f, int first);
 void vo_temperature(const vo_params *p, vo_fields *f, int first);
 void vo_heat(const vo_params *p, const double *th, double *trans1, double *trans2);
+/* Kinetic energy of the velocities (no counterpart in the reference; the
+ * build's definition): 1/2 dr dp sum over k = 0..np-1 of
+ *   sum_{i=1..nr} w_T(i) vel(i,k)^2 + sum_{i=1..nr-1} w_C(i) uel(i,k)^2,
+ * the unknowns of poisson2 / poisson3, with w = 1 in the box and the row
+ * radius in the annulus (w_T = r0 + (i-1/2) dr, w_C = r0 + i dr). */
+double vo_energy(const vo_params *p, const double *uel, const double *vel);
 void vo_rhsv(const vo_params *p, const double *Cn, double *vrh);
 void vo_rhsu(const vo_params *p, const double *Cn, double *urh);
 int  vo_update(const vo_params *p, vo_fields *f);
