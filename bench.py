@@ -318,7 +318,7 @@ def sweep_requests(n=64, seed=20261205, steps=2000):
     return out
 
 
-def serve_leg(n=64, steps=2000, slots=8):
+def serve_leg(n=64, steps=2000, slots=8, baseline=True):
     """The served sweep through the request loop (include/vvserve.h): the
     requests arrive on their schedule and are admitted by slot and device
     footprint; then the same requests one at a time (one slot, all queued
@@ -347,6 +347,8 @@ def serve_leg(n=64, steps=2000, slots=8):
                      "p99_s": float(np.percentile(lat, 99)), "peak_running": stats["peak_running"],
                      "peak_mem_GB": stats["peak_mem"] / 1e9, "failed": bad,
                      "not_converged": sum(1 for g in got if g["status"] == 1)}
+    if not baseline:
+        return res
     with Server(slots=1, mem_budget=64e9) as srv:
         t_start = time.perf_counter()
         for _, cfg, th0, st in reqs:
@@ -450,6 +452,9 @@ def main():
                     help="skip the direct-solve and large V-cycle measurements")
     ap.add_argument("--serve-only", action="store_true",
                     help="measure only the served mixed-size sweep and print its JSON object")
+    ap.add_argument("--slots", type=int, default=8, help="request-loop slots of the served sweep")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="with --serve-only: skip the one-at-a-time pass")
     ap.add_argument("--rot3d-only", action="store_true",
                     help="measure only the 3D step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -463,7 +468,7 @@ def main():
         print(json.dumps(rot3d_leg(cpu=not a.no_cpu)))
         return
     if a.serve_only:
-        print(json.dumps(serve_leg()))
+        print(json.dumps(serve_leg(slots=a.slots, baseline=not a.no_baseline)))
         return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
@@ -508,7 +513,7 @@ def main():
         line["alternatives"] = {"poisson_direct": direct_leg(a.workload, a.steps, a.warmup)}
         line["vcycle_large"] = vcycle_large_leg()
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
-        line["served_sweep"] = serve_leg()
+        line["served_sweep"] = serve_leg(slots=a.slots)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.workload)
     print(json.dumps(line))

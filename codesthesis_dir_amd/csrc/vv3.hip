@@ -29,7 +29,12 @@ struct vv3_run {
     double *heat = nullptr;              // flux terms (nz+2) x np, then the totals trans1, trans2
     double *trig = nullptr;              // VV3_MODEL_HOR: sin, cos of each plane's azimuth (2 x (np+1))
     double *kepart = nullptr;            // kinetic energy: partials [KE3_MAXB], then the total
+    double *dgbuf = nullptr;             // diagnostics, allocated at first use: plane sums (np),
+                                         // cos / sin table (2 x MODES_MAX x np), the 2 x MODES_MAX
+                                         // mode integrals, local Nusselt numbers (2 x (np+1))
 };
+
+constexpr int MODES_MAX = 16;
 
 constexpr int KE3_MAXB = 1024;
 
@@ -581,6 +586,58 @@ __global__ void __launch_bounds__(WG) k3_energy_sum(const double *part, int nb, 
     if (threadIdx.x == 0) *out = scale * s;
 }
 
+// Local Nusselt numbers at mid-height, sect360v2mg.f:318-325: a thread per
+// plane k = 0..np, in the reference's operation order (L = ln(r0) from the
+// host's C library, as the reference's dlog).
+__global__ void __launch_bounds__(WG) k3_local_nu(Geo g, const double *th, double rmax, double L, double *inner,
+                                                  double *outer)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > g.np) return;
+    const int nr = g.nr, jm = g.nz / 2;
+    const double dr = g.dr;
+    inner[k] = -(g.r0 * L * (-184.0 * X(th, 0, jm, k) + 225.0 * X(th, 1, jm, k) - 50.0 * X(th, 2, jm, k) +
+                             9.0 * X(th, 3, jm, k)) / (60.0 * dr));
+    outer[k] = -(rmax * L * (184.0 * X(th, nr + 1, jm, k) - 225.0 * X(th, nr, jm, k) + 50.0 * X(th, nr - 1, jm, k) -
+                             9.0 * X(th, nr - 2, jm, k)) / (60.0 * dr));
+}
+
+// Fourier-mode integrals after rot3dmghor.f:954-991.  The plane sums
+// P_k = sum_{i,j} f_i f_j (r0 + i dr) T(i,j,k) do not depend on the mode,
+// so workgroup k-1 forms P_k (k = 1..np) and k3_modes_dft takes the cos and
+// sin sums over k of each mode, one reduction per (mode, kind).
+__global__ void __launch_bounds__(WG) k3_plane_sums(Geo g, const double *th, double *psum)
+{
+    __shared__ double sh[WG / 64];
+    const int k = blockIdx.x + 1;
+    const int nr = g.nr, nz = g.nz, plane = g.ldi * g.ldj;
+    double sm = 0.0;
+    for (int t = threadIdx.x; t < plane; t += WG) {
+        const int j = t / g.ldi, i = t - j * g.ldi;
+        if (i > nr + 1 || j > nz + 1) continue;
+        double f = 1.0;
+        if (i == 0 || i == nr + 1) f *= 0.5;
+        if (j == 0 || j == nz + 1) f *= 0.5;
+        sm += (g.r0 + (double)i * g.dr) * X(th, i, j, k) * f;
+    }
+    sm = vv::block_sum(sm, sh);
+    if (threadIdx.x == 0) psum[blockIdx.x] = sm;
+}
+
+__global__ void __launch_bounds__(WG) k3_modes_dft(const double *psum, const double *tw, int np, int nmax,
+                                                   double scale, double *out)
+{
+    __shared__ double sh[WG / 64];
+    for (int q = 0; q < 2 * nmax; ++q) {   // q < nmax: cos of mode q+1; else sin of mode q-nmax+1
+        const double *w = tw + (size_t)((q < nmax) ? q : MODES_MAX + q - nmax) * np;
+        double sm = 0.0;
+        for (int k = threadIdx.x; k < np; k += WG) sm += psum[k] * w[k];
+        sm = vv::block_sum(sm, sh);
+        if (threadIdx.x == 0) out[q] = scale * sm;
+        __syncthreads();
+    }
+}
+
 struct RhsF {
     const double *An, *Bn, *Cn, *wel;
     double *wrh, *vrh, *urh;
@@ -795,6 +852,7 @@ int vv3_destroy(vv3_run *r)
     if (r->heat) (void)hipFree(r->heat);
     if (r->trig) (void)hipFree(r->trig);
     if (r->kepart) (void)hipFree(r->kepart);
+    if (r->dgbuf) (void)hipFree(r->dgbuf);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return VV_OK;
@@ -1018,6 +1076,66 @@ int vv3_energy(vv3_run *r, double *ke)
                        0.5 * r->dr * r->dz * r->dp, r->kepart + KE3_MAXB);
     VV_HIP(hipGetLastError());
     VV_HIP(hipMemcpyAsync(ke, r->kepart + KE3_MAXB, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+// the diagnostics buffer, allocated and tabulated at first use
+static int diag_buffer(vv3_run *r)
+{
+    if (r->dgbuf) return VV_OK;
+    const int np = r->np;
+    const size_t nw = (size_t)np + 2 * MODES_MAX * (size_t)np + 2 * MODES_MAX + 2 * ((size_t)np + 1);
+    std::vector<double> t(2 * MODES_MAX * (size_t)np);
+    for (int n = 1; n <= MODES_MAX; ++n)
+        for (int k = 1; k <= np; ++k) {
+            const double ang = (double)k * r->dp;   // ang=dble(k)*dp, rot3dmghor.f:970
+            t[(size_t)(n - 1) * np + (k - 1)] = std::cos((double)n * ang);
+            t[(size_t)(MODES_MAX + n - 1) * np + (k - 1)] = std::sin((double)n * ang);
+        }
+    if (hipMalloc(&r->dgbuf, nw * sizeof(double)) != hipSuccess) {
+        r->dgbuf = nullptr;
+        vv::set_error("vv3 diagnostics: device allocation failed");
+        return VV_ENOMEM;
+    }
+    VV_HIP(hipMemcpyAsync(r->dgbuf + np, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    return VV_OK;
+}
+
+int vv3_local_nu(vv3_run *r, double *inner, double *outer)
+{
+    if (!r || !inner || !outer) { vv::set_error("vv3_local_nu: null argument"); return VV_EINVAL; }
+    VV_HIP(hipSetDevice(r->device));
+    int st = diag_buffer(r);
+    if (st) return st;
+    const int np = r->np;
+    double *nu = r->dgbuf + (size_t)np + 2 * MODES_MAX * (size_t)np + 2 * MODES_MAX;
+    hipLaunchKernelGGL(k3_local_nu, dim3(nblk(np + 1, WG)), dim3(WG), 0, r->stream, geo(r),
+                       (const double *)r->f[VV3_TH], r->cfg.rmax, std::log(r->cfg.r0), nu, nu + np + 1);
+    VV_HIP(hipGetLastError());
+    VV_HIP(hipMemcpyAsync(inner, nu, (np + 1) * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipMemcpyAsync(outer, nu + np + 1, (np + 1) * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vv3_modes(vv3_run *r, int nmax, double *c, double *s)
+{
+    if (!r || !c || !s || nmax < 1 || nmax > MODES_MAX) {
+        vv::set_error("vv3_modes: need 1 <= nmax <= 16 and output arrays");
+        return VV_EINVAL;
+    }
+    VV_HIP(hipSetDevice(r->device));
+    int st = diag_buffer(r);
+    if (st) return st;
+    const int np = r->np;
+    double *psum = r->dgbuf, *tw = r->dgbuf + np, *out = r->dgbuf + np + 2 * MODES_MAX * (size_t)np;
+    hipLaunchKernelGGL(k3_plane_sums, dim3(np), dim3(WG), 0, r->stream, geo(r), (const double *)r->f[VV3_TH], psum);
+    hipLaunchKernelGGL(k3_modes_dft, dim3(1), dim3(WG), 0, r->stream, (const double *)psum, (const double *)tw, np,
+                       nmax, r->dr * r->dz * r->dp, out);
+    VV_HIP(hipGetLastError());
+    VV_HIP(hipMemcpyAsync(c, out, nmax * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipMemcpyAsync(s, out + nmax, nmax * sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }

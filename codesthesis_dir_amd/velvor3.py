@@ -69,6 +69,8 @@ def _bind():
     L.vv3_heat.argtypes = [vp, _P]
     L.vv3_fluxes.argtypes = [vp, _P, _P]
     L.vv3_energy.argtypes = [vp, _P]
+    L.vv3_local_nu.argtypes = [vp, _P, _P]
+    L.vv3_modes.argtypes = [vp, ctypes.c_int, _P, _P]
     L.vv3_rhs.argtypes = [vp, ctypes.c_int]
     L.vv3_poisson.argtypes = [vp, ctypes.c_int]
     L.vv3_update.argtypes = [vp]
@@ -145,6 +147,20 @@ class Annulus3D:
         t = ctypes.c_double()
         check(self._L.vv3_energy(self._h, ctypes.byref(t)), "vv3_energy")
         return t.value
+
+    def local_nu(self) -> tuple[np.ndarray, np.ndarray]:
+        """Local Nusselt numbers at mid-height of the inner and outer
+        cylinder, one per plane k = 0..np (vv3_local_nu)."""
+        a, b = np.empty(self.cfg.np + 1), np.empty(self.cfg.np + 1)
+        check(self._L.vv3_local_nu(self._h, a.ctypes.data_as(_P), b.ctypes.data_as(_P)), "vv3_local_nu")
+        return a, b
+
+    def modes(self, nmax: int = 6) -> tuple[np.ndarray, np.ndarray]:
+        """Fourier-mode volume integrals (cos, sin) of the temperature for
+        n = 1..nmax (vv3_modes)."""
+        c, s = np.empty(nmax), np.empty(nmax)
+        check(self._L.vv3_modes(self._h, nmax, c.ctypes.data_as(_P), s.ctypes.data_as(_P)), "vv3_modes")
+        return c, s
 
     def rhsw(self):
         check(self._L.vv3_rhs(self._h, 1), "vv3_rhs")

@@ -88,6 +88,22 @@ int vv3_fluxes(vv3_run *run, double *trans1, double *trans2);
  * each over the unknowns of its Poisson solve: 1/2 of the volume integral
  * of |u|^2 by the midpoint rule. */
 int vv3_energy(vv3_run *run, double *ke);
+/* Local Nusselt numbers at mid-height of both cylinders, one per plane
+ * k = 0..np (inner and outer hold np+1 values): the formula of
+ * sect360v2mg.f:318-325 at j = nz/2,
+ *   inner[k] = -r0 ln(r0) (-184 T0 + 225 T1 - 50 T2 + 9 T3) / (60 dr),
+ *   outer[k] = -rmax ln(r0) (184 T(nr+1) - 225 T(nr) + 50 T(nr-1) - 9 T(nr-2)) / (60 dr). */
+int vv3_local_nu(vv3_run *run, double *inner, double *outer);
+/* Azimuthal Fourier-mode volume integrals of the temperature after
+ * rot3dmghor.f:954-991 (subroutine aaaaaaaa), modes n = 1..nmax
+ * (1 <= nmax <= 16):
+ *   c[n-1] = dp dr dz sum_{k=1..np} sum_{i=0..nr+1} sum_{j=0..nz+1}
+ *            f_i f_j (r0 + i dr) T(i,j,k) cos(n k dp),  s[n-1] with sin,
+ * f = 1/2 on the wall rows i = 0, nr+1 and j = 0, nz+1.  The reference
+ * routine reads r0 from an uninitialised local and carries each plane's
+ * running sum into the next plane; the run's r0 and one sum per plane are
+ * used here. */
+int vv3_modes(vv3_run *run, int nmax, double *c, double *s);
 /* rhsw / rhsv / rhsu (rot3dmg.f:1026, 1066, 1111): which = 1 / 2 / 3 */
 int vv3_rhs(vv3_run *run, int which);
 /* poisson1 / poisson2 / poisson3 (rot3dmg.f:1216, 1325, 1434): solves the

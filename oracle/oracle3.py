@@ -40,6 +40,8 @@ def lib():
         _LIB.v3_heat.restype = ctypes.c_double
         _LIB.v3_heat2.restype = None
         _LIB.v3_energy.restype = ctypes.c_double
+        _LIB.v3_local_nu.restype = None
+        _LIB.v3_modes.restype = None
         _LIB.v3_residual.restype = ctypes.c_double
         _LIB.v3_poisson_exact.restype = ctypes.c_int
     return _LIB
@@ -90,6 +92,18 @@ class Run3:
     def energy(self):
         """Kinetic energy of the velocities (the build's definition)."""
         return lib().v3_energy(ctypes.byref(self.p), ctypes.byref(self.c))
+
+    def local_nu(self):
+        """(inner, outer) local Nusselt numbers at mid-height, k = 0..np."""
+        a, b = np.zeros(self.p.np + 1), np.zeros(self.p.np + 1)
+        lib().v3_local_nu(ctypes.byref(self.p), ctypes.byref(self.c), _P(a), _P(b))
+        return a, b
+
+    def modes(self, nmax=6):
+        """(c, s): Fourier-mode integrals n = 1..nmax of the temperature."""
+        c, s = np.zeros(nmax), np.zeros(nmax)
+        lib().v3_modes(ctypes.byref(self.p), ctypes.byref(self.c), nmax, _P(c), _P(s))
+        return c, s
 
     def residual(self, which, psi, rhs):
         res = np.zeros(self.shape)

@@ -516,6 +516,48 @@ double v3_energy(const v3_params *P, const v3_fields *F)
     return 0.5 * P->dr * P->dz * P->dp * s;
 }
 
+/* local Nusselt numbers, sect360v2mg.f:318-325 (vv3_oracle.h) */
+void v3_local_nu(const v3_params *P, const v3_fields *F, double *inner, double *outer)
+{
+    const int nr = P->nr, jm = P->nz / 2;
+    const double dr = P->dr, r0 = P->r0, rmax = P->rmax, L = log(r0);
+    const double *th = F->th;
+    for (int k = 0; k <= P->np; ++k) {
+        inner[k] = -(r0 * L * (-184.0 * X(th, 0, jm, k) + 225.0 * X(th, 1, jm, k) - 50.0 * X(th, 2, jm, k) +
+                               9.0 * X(th, 3, jm, k)) / (60.0 * dr));
+        outer[k] = -(rmax * L * (184.0 * X(th, nr + 1, jm, k) - 225.0 * X(th, nr, jm, k) +
+                                 50.0 * X(th, nr - 1, jm, k) - 9.0 * X(th, nr - 2, jm, k)) / (60.0 * dr));
+    }
+}
+
+/* Fourier-mode integrals after rot3dmghor.f:954-991 (vv3_oracle.h) */
+void v3_modes(const v3_params *P, const v3_fields *F, int nmax, double *c, double *s)
+{
+    const int nr = P->nr, nz = P->nz, np = P->np;
+    const double dr = P->dr, dz = P->dz, dp = P->dp;
+    for (int n = 1; n <= nmax; ++n) {
+        double sc = 0.0, ss = 0.0;
+        for (int k = 1; k <= np; ++k) {
+            const double ang = (double)k * dp;
+            double sumc = 0.0, sums = 0.0;
+            for (int i = 0; i <= nr + 1; ++i) {
+                const double r = P->r0 + (double)i * dr;
+                for (int j = 0; j <= nz + 1; ++j) {
+                    double fact = 1.0;
+                    if (i == 0 || i == nr + 1) fact = fact / 2.0;
+                    if (j == 0 || j == nz + 1) fact = fact / 2.0;
+                    sumc = sumc + r * X(F->th, i, j, k) * cos((double)n * ang) * fact;
+                    sums = sums + r * X(F->th, i, j, k) * sin((double)n * ang) * fact;
+                }
+            }
+            sc = sc + sumc * dr * dz * dp;
+            ss = ss + sums * dr * dz * dp;
+        }
+        c[n - 1] = sc;
+        s[n - 1] = ss;
+    }
+}
+
 /* rhsw, rot3dmg.f:1026-1048 */
 void v3_rhsw(const v3_params *P, v3_fields *F)
 {

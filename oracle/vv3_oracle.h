@@ -45,6 +45,21 @@ void v3_update(const v3_params *P, v3_fields *F);       /* rot3dmg.f:964-1018 */
  * + sum_{j=1..nz-1, i=1..nr}   (r0 + (i-1/2) dr) wel^2,
  * each over the unknowns of its Poisson solve (poisson3, 2, 1). */
 double v3_energy(const v3_params *P, const v3_fields *F);
+/* Local Nusselt numbers at mid-height (sect360v2mg.f:318-325): for each
+ * plane k = 0..np at j = nz/2,
+ *   inner[k] = -r0 ln(r0) (-184 T0 + 225 T1 - 50 T2 + 9 T3) / (60 dr),
+ *   outer[k] = -rmax ln(r0) (184 T(nr+1) - 225 T(nr) + 50 T(nr-1) - 9 T(nr-2)) / (60 dr),
+ * in the reference's operation order (ln(r0) from the C library). */
+void v3_local_nu(const v3_params *P, const v3_fields *F, double *inner, double *outer);
+/* Azimuthal Fourier-mode volume integrals of the temperature after
+ * rot3dmghor.f:954-991 (subroutine aaaaaaaa), n = 1..nmax:
+ *   c[n-1] = dp dr dz sum_{k=1..np} sum_{i=0..nr+1} sum_{j=0..nz+1}
+ *            f_i f_j (r0 + i dr) T(i,j,k) cos(n k dp),   s[n-1] with sin,
+ * f = 1/2 on the rows i = 0, nr+1 and j = 0, nz+1, 1 elsewhere.  The
+ * reference routine reads r0 from an uninitialised local (it lacks COMMON
+ * /radius/) and carries each plane's running sum, scaled by dr dz, into the
+ * next plane; the run's r0 and one sum per plane are used here. */
+void v3_modes(const v3_params *P, const v3_fields *F, int nmax, double *c, double *s);
 void v3_rhsw(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1026-1048 */
 void v3_rhsv(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1066-1093 */
 void v3_rhsu(const v3_params *P, v3_fields *F);         /* rot3dmg.f:1111-1134 */

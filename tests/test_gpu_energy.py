@@ -50,3 +50,26 @@ def test_energy_3d_matches_oracle(shape):
     assert abs(ke - want) <= 1e-13 * want, (ke, want)
     assert run.energy() == ke
     run.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 24), (33, 20, 160)])
+def test_local_nu_and_modes_match_oracle(shape):
+    """vv3_local_nu equals the oracle bit for bit (same operations per
+    plane); vv3_modes within 1e-12 of the oracle's sums, relative to the
+    sum of the plane sums' magnitudes (the GPU adds in another order)."""
+    from codesthesis_dir_amd.velvor3 import Annulus3D, Annulus3DConfig
+    nr, nz, np_ = shape
+    print(f"[gpu] diagnostics 3d {shape}", flush=True)
+    run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, dt=1e-4, Ra=1e4, Re=30.0, Ptl=1.0, Gr=0.0))
+    ora = O3.Run3(nr, nz, np_, 1e-4, 1e4, 30.0, 1.0, 0.0)
+    th = np.random.default_rng(np_ + 1).standard_normal(ora.shape)
+    run.put("th", th)
+    ora["th"] = th
+    a, b = run.local_nu()
+    oa, ob = ora.local_nu()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    c, s = run.modes(8)
+    oc, os_ = ora.modes(8)
+    scale = run.cfg.zmax / nz * 0.5 / nr * 2 * np.pi / np_ * np.sum(np.abs(th))
+    assert np.max(np.abs(c - oc)) <= 1e-12 * scale and np.max(np.abs(s - os_)) <= 1e-12 * scale
+    run.close()
