@@ -179,6 +179,15 @@ double *field_ptr(vv_run *r, int f)
     }
 }
 
+// the device recurrences divide by the pivots with a quotient form that
+// needs positive divisors
+bool positive(const std::vector<double> &v, size_t from)
+{
+    for (size_t j = from; j < v.size(); ++j)
+        if (!(v[j] > 0.0)) return false;
+    return true;
+}
+
 int upload_tab(vv_run *r, double **dst, const std::vector<double> &h)
 {
     VV_HIP(hipMalloc(dst, h.size() * sizeof(double)));
@@ -193,6 +202,7 @@ int build_tables(vv_run *r)
         const int n = nr - 1;
         std::vector<double> a(n, 1.0), b(n, 4.0), c(n, 1.0);
         TdmaTab t = make_tdma(a, b, c);
+        if (!positive(t.bp, 0)) { set_error("vv_create: non-positive tdma pivot"); return VV_EINVAL; }
         std::vector<double> h;
         h.insert(h.end(), t.fact.begin(), t.fact.end());
         h.insert(h.end(), t.bp.begin(), t.bp.end());
@@ -207,6 +217,7 @@ int build_tables(vv_run *r)
         b[0] = 1.0; c[0] = 2.0;
         a[n - 1] = 2.0; b[n - 1] = 1.0;
         TdmaTab t = make_tdma(a, b, c);
+        if (!positive(t.bp, 0)) { set_error("vv_create: non-positive tdma pivot"); return VV_EINVAL; }
         std::vector<double> h;
         h.insert(h.end(), t.fact.begin(), t.fact.end());
         h.insert(h.end(), t.bp.begin(), t.bp.end());
@@ -219,6 +230,7 @@ int build_tables(vv_run *r)
         const int n = np + 1;
         std::vector<double> a(n, 1.0), b(n, 4.0), c(n, 1.0);
         PtdmaTab t = make_ptdma(a, b, c);
+        if (!positive(t.b, 1)) { set_error("vv_create: non-positive ptdma pivot"); return VV_EINVAL; }
         std::vector<double> h;
         h.insert(h.end(), t.f1.begin(), t.f1.end());
         h.insert(h.end(), t.f2.begin(), t.f2.end());
@@ -327,8 +339,7 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         if ((st = upload_tab(r, &r->rC, rc))) return fail(st);
         if ((st = upload_tab(r, &r->rT, rt))) return fail(st);
     }
-    if (r->cfg.direct_cols <= 0)   // default: a V-cycle with a direct level of <= 128 columns
-        r->cfg.direct_cols = (cfg->np / 2 < 128) ? (cfg->np / 2 > 1 ? cfg->np / 2 : 1) : 128;
+    if (r->cfg.direct_cols <= 0) r->cfg.direct_cols = vv::default_direct_cols(cfg->np);
     r->nbx = field_blocks(r);
     {
         const size_t np_ = 2 * (size_t)r->nbx + 3 * (size_t)r->nbx + 2 * (size_t)vv::MG_NORM_MAXB;

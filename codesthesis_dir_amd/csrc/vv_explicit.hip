@@ -21,18 +21,20 @@ constexpr int WG = 256;
 
 __device__ __forceinline__ size_t ix(int i, int k, int ld) { return (size_t)k * ld + i; }
 
-// x / b, correctly rounded, from rb = 1/b correctly rounded (computed on the
-// host): q0 = x*rb is within one ulp of x/b, the remainder x - b*q0 is exact
-// in one FMA, and q0 + rem*rb rounds to x/b (Markstein's theorem for
-// division with a fused multiply-add; no overflow or underflow occurs for
-// these operands).  A zero remainder means q0 is exact, which also keeps the
-// sign of a zero quotient.  Three dependent operations replace the IEEE
-// division sequence on the recurrence's critical path, with the same result.
+// x / b for b > 0, correctly rounded, from rb = 1/b correctly rounded
+// (computed on the host): q0 = x*rb is within one ulp of x/b, the remainder
+// is exact in one FMA, and q0 + rem*rb rounds to x/b (Markstein's theorem
+// for division with a fused multiply-add; no overflow or underflow occurs for
+// these operands).  The remainder is formed as t = b*q0 - x and applied as
+// -t: when it is zero, t = +0 and q0 + (-0)*rb keeps q0 and the sign of a
+// zero quotient, so no select is needed.  Three dependent operations replace
+// the IEEE division sequence on the recurrence's critical path, with the same
+// result.  The elimination tables are checked for b > 0 when they are built.
 __device__ __forceinline__ double div_rb(double x, double b, double rb)
 {
     const double q0 = x * rb;
-    const double e = __builtin_fma(-b, q0, x);
-    return (e == 0.0) ? q0 : __builtin_fma(e, rb, q0);
+    const double t = __builtin_fma(b, q0, -x);
+    return __builtin_fma(-t, rb, q0);
 }
 
 // single-precision literals of rb.f:660-668 promoted to double

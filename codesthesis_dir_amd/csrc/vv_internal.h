@@ -71,6 +71,10 @@ struct MgLevelDev {
     double *xm;
     const double *twid;
 };
+// default of vv_config.direct_cols (0): the whole plane is the direct
+// level up to 1024 columns, a V-cycle down to 128 columns above
+inline int default_direct_cols(int np) { return np <= 1024 ? np : 128; }
+
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;
 

@@ -120,11 +120,24 @@ extern "C" {
 double vv_run_footprint(const vv_config *cfg)
 {
     if (!cfg || cfg->nr < 2 || cfg->np < 2) return 0.0;
-    // 24 fields of ld*(np+1) doubles (ld = nr+2 rounded up to 8), a
-    // multigrid hierarchy below 3 * (4/3) of a field per problem, tables
+    // 24 fields of ld*(np+1) doubles (ld = nr+2 rounded up to 8) and, per
+    // Poisson problem (two), the levels of the multigrid hierarchy: three
+    // arrays of n x ncol and a reduction table of at most 2 + 23 n words per
+    // smoothed level; the direct level also holds one such table per Fourier
+    // mode and its two ncol x ncol transform matrices
     const double ld = (double)(((cfg->nr + 2) + 7) / 8 * 8);
-    const double n = ld * (cfg->np + 1);
-    return 8.0 * (24.0 * n + 8.0 * n) + (1 << 20);
+    const double n = (double)cfg->nr;
+    const int dc = cfg->direct_cols > 0 ? cfg->direct_cols : vv::default_direct_cols(cfg->np);
+    double mg = 0.0;
+    for (int c = cfg->np;; c /= 2) {
+        const double tab = 2.0 + 23.0 * n;
+        if (c <= dc || c < 2) {
+            mg += 3.0 * n * c + (c / 2 + 1) * tab + 2.0 * (double)c * c;
+            break;
+        }
+        mg += 3.5 * n * c + tab;
+    }
+    return 8.0 * (24.0 * ld * (cfg->np + 1) + 2.0 * mg) + (1 << 20);
 }
 
 int vv_server_create(int device, int slots, double mem_budget, vv_server **out)

@@ -72,8 +72,11 @@ typedef struct vv_config {
     int    direct_cols;   /* phi coarsening stops at <= this many columns,
                              where the level is solved exactly (real DFT in
                              phi + one line solve per mode); >= np makes the
-                             whole solve direct.  0 selects min(128, np/2):
-                             a V-cycle with at least one smoothed level.    */
+                             whole solve direct.  0 selects the default:
+                             direct for np <= 1024 (one exact solve per
+                             Poisson call, measured 1.8-2.7x faster than the
+                             V-cycle at every grid up to 1025 x 513), a
+                             V-cycle down to 128 columns above.            */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */
