@@ -9,15 +9,6 @@
 
 namespace vv3 {
 
-// Workgroups are dispatched to the 8 XCDs (each with its own L2) in turn:
-// xcd_block maps the dispatch index b of nb workgroups to a logical index so
-// that each XCD works through one contiguous range, and neighbouring rows
-// and planes of a field are read through the same L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
-{
-    const unsigned x = b & 7u, q = b >> 3, per = nb >> 3, rem = nb & 7u;
-    return x * per + (x < rem ? x : rem) + q;
-}
 
 // grid and parameters handed to every kernel by value
 struct Geo {

@@ -17,6 +17,7 @@
 // residual of 1e-5, tolcon at rot3dmg.f:1562).
 
 #include "vv3_internal.h"
+#include "vv_gemm.h"
 
 #include <algorithm>
 #include <cmath>
@@ -27,105 +28,7 @@ namespace vv3 {
 
 namespace {
 
-constexpr int GK = 16;    // GEMM k step
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// C[q] = A[q] * B[q] in FP64 on the matrix cores, operands reached through
-// the functor: op.a(q, m, k), op.b(q, k, n) load, op.c(q, m, n, v) stores; C
-// is written with consecutive n in consecutive lanes.  A workgroup of WAVES
-// waves computes a (32 WR) x (32 WC) tile (WR x WC = WAVES), each wave a
-// 32 x 32 block as 2 x 2 v_mfma_f64_16x16x4f64 products.  MFMA lane layout:
-// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, element e of D in
-// lane l is D[l/16 + 4e][l%16].  The next k step's operands are loaded into
-// registers while the current one is multiplied.  AKFAST / BKFAST: the tile
-// loads run along k (operand contiguous in k) rather than along m / n.
-template <class Op, bool AKFAST, bool BKFAST, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k3_gemm(Op op)
-{
-    constexpr int WR = (WAVES == 4) ? 2 : 1, WC = WAVES / WR;
-    constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WAVES;
-    constexpr int LA = GK * TM / NT, LB = GK * TN / NT;   // tile elements per thread
-    // rows padded by one double: k-fast tile stores (consecutive threads one
-    // row apart) then fall on different banks
-    __shared__ double As[GK][TM + 1];
-    __shared__ double Bs[GK][TN + 1];
-    // logical tile order: m tiles fastest, then n tiles, then the batch, so
-    // the m tiles that share a B tile run together on one XCD
-    const unsigned nmt = gridDim.y, nnt = gridDim.x;
-    const unsigned lb = xcd_block((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
-                                  gridDim.x * gridDim.y * gridDim.z);
-    const int q = (int)(lb / (nmt * nnt));
-    const int m0 = (int)(lb % nmt) * TM, n0 = (int)((lb / nmt) % nnt) * TN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave / WC) * 32, wc = (wave % WC) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    d4 acc[2][2];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
-    double ra[LA], rb[LB];
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int t = 0; t < LA; ++t) {
-            const int e = tid + NT * t;
-            const int ka = AKFAST ? (e % GK) : (e / TM);
-            const int ma = AKFAST ? (e / GK) : (e % TM);
-            const int m = m0 + ma, k = k0 + ka;
-            ra[t] = (m < op.M && k < op.K) ? op.a(q, m, k) : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < LB; ++t) {
-            const int e = tid + NT * t;
-            const int kb = BKFAST ? (e % GK) : (e / TN);
-            const int nb = BKFAST ? (e / GK) : (e % TN);
-            const int n = n0 + nb, k = k0 + kb;
-            rb[t] = (n < op.N && k < op.K) ? op.b(q, k, n) : 0.0;
-        }
-    };
-    auto stash = [&]() {
-#pragma unroll
-        for (int t = 0; t < LA; ++t) {
-            const int e = tid + NT * t;
-            As[AKFAST ? (e % GK) : (e / TM)][AKFAST ? (e / GK) : (e % TM)] = ra[t];
-        }
-#pragma unroll
-        for (int t = 0; t < LB; ++t) {
-            const int e = tid + NT * t;
-            Bs[BKFAST ? (e % GK) : (e / TN)][BKFAST ? (e / GK) : (e % TN)] = rb[t];
-        }
-    };
-    fetch(0);
-    for (int k0 = 0; k0 < op.K; k0 += GK) {
-        stash();
-        __syncthreads();
-        if (k0 + GK < op.K) fetch(k0 + GK);
-#pragma unroll
-        for (int ks = 0; ks < GK; ks += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int r = 0; r < 2; ++r) a[r] = As[ks + lk][wr + 16 * r + li];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) b[c] = Bs[ks + lk][wc + 16 * c + li];
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + wr + 16 * r + lk + 4 * e;
-                const int n = n0 + wc + 16 * c + li;
-                if (m < op.M && n < op.N) op.c(q, m, n, acc[r][c][e]);
-            }
-}
+using vv::gemm_f64;
 
 // 1. spectra S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
 struct PhiFwd {
@@ -184,21 +87,6 @@ struct PhiInv {
         if (m + 1 == M) psi[off] = v;
     }
 };
-
-// 64 x 64 tiles of four waves when they fill the chip, else 32 x 32 tiles of
-// one wave (four times the workgroups for small planes)
-template <bool AKFAST, bool BKFAST, class Op>
-void gemm(const Op &op, int batch, hipStream_t s)
-{
-    const long long big = (long long)((op.N + 63) / 64) * ((op.M + 63) / 64) * batch;
-    if (big >= 256) {
-        dim3 grid((op.N + 63) / 64, (op.M + 63) / 64, batch);
-        hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST, 4>), grid, dim3(256), 0, s, op);
-    } else {
-        dim3 grid((op.N + 31) / 32, (op.M + 31) / 32, batch);
-        hipLaunchKernelGGL((k3_gemm<Op, AKFAST, BKFAST, 1>), grid, dim3(64), 0, s, op);
-    }
-}
 
 // ---------------------------------------------------------------- phi FFT
 // For np a power of two the phi transforms are radix-2 FFTs in LDS.  Two real
@@ -766,16 +654,16 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
         hipLaunchKernelGGL(k3_fft_fwd, fgrid, dim3(256), flds, s, rhs, P.S, (const double2 *)P.tw, np, bits, LP, i1,
                            j1, g.ldi, g.ldj, mt);
     else
-        gemm<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
-    gemm<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
+        gemm_f64<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
+    gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
     hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
-    gemm<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
+    gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
     if (P.fft)
         hipLaunchKernelGGL(k3_fft_inv, fgrid, dim3(256), flds, s, (const double *)P.S, psi, (const double2 *)P.tw, np,
                            bits, LP, i1, j1, g.ldi, g.ldj, mt);
     else
-        gemm<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+        gemm_f64<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }

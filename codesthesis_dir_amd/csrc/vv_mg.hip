@@ -23,6 +23,7 @@
 // converged, so several cycles can be queued between host checks.
 
 #include "vv_internal.h"
+#include "vv_gemm.h"
 #include "vv_reduce.h"
 
 #include <cmath>
@@ -563,6 +564,50 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
 }
 
+// The direct level's transforms as FP64 matrix-core GEMMs (gemm_f64), the
+// two problems as the batch.  Projection: X_q[m][i] = sum_k W[m][k] Y_q[k][i]
+// over the level's C columns k, Y the level's right-hand side or, in a
+// refinement cycle on level 0, its residual; synthesis: the level's iterate
+// u_q[k][i] = (or +=) 1/C sum_m Wi[k][m] X_q[m][i], stored only while the
+// problem runs.  The problems' row counts differ (nr, nr-1): N is the larger
+// and the accessors bound each problem's rows.
+struct DftFwd {
+    KArgs ka;
+    int M, N, K;
+    __device__ double a(int, int m, int k) const { return ka.P[0].L.twid[(size_t)m * K + k]; }
+    __device__ double b(int q, int k, int i) const
+    {
+        const KProb &P = ka.P[q];
+        if (i >= P.n) return 0.0;
+        if (ka.lev0 && ka.cycles[q] > 0) return resid_at(P, P.L, i, k);
+        return P.L.f[(size_t)k * P.n + i];
+    }
+    __device__ void c(int q, int m, int i, double v) const
+    {
+        const KProb &P = ka.P[q];
+        if (i < P.n) P.L.xm[(size_t)m * P.n + i] = v;
+    }
+};
+struct DftInv {
+    KArgs ka;
+    int M, N, K;
+    __device__ double a(int, int m, int k) const { return ka.P[0].L.twid[(size_t)K * K + (size_t)m * K + k]; }
+    __device__ double b(int q, int k, int i) const
+    {
+        const KProb &P = ka.P[q];
+        return (i < P.n) ? P.L.xm[(size_t)k * P.n + i] : 0.0;
+    }
+    __device__ void c(int q, int m, int i, double v) const
+    {
+        const KProb &P = ka.P[q];
+        if (i >= P.n || ka.done[q]) return;
+        double *d = P.L.u + cix(i, m, P.n);
+        const double w = v / (double)K;
+        if (!ka.lev0 || ka.cycles[q] == 0) *d = w;
+        else *d += w;
+    }
+};
+
 // max-norm of the level-0 residual: one point per thread, one partial per
 // workgroup (no atomics: same-address atomics from hundreds of workgroups
 // cost more than a second launch)
@@ -954,17 +999,24 @@ static int enqueue_cycle(vv_run *r)
         }
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
+        // matrix-core transforms from 1024 columns (measured: 1897 against
+        // 1652 steps/s at 1025 x 513); below, where the GEMM's sequence of
+        // k steps is latency-bound, the VALU kernel is faster (5860 against
+        // 4784 steps/s at 513 x 257)
+        const bool mfma = C >= 1024;
         prof_begin(r, &pe);
         // projection (read the level, write the mode lines) and line solves
         // (read and write the mode lines)
-        hipLaunchKernelGGL(k_mg_dft<0>, gd, dim3(256), 0, r->stream, a);
+        if (mfma) gemm_f64<true, false>(DftFwd{a, C, n, C}, 2, r->stream);
+        else hipLaunchKernelGGL(k_mg_dft<0>, gd, dim3(256), 0, r->stream, a);
         if (two)
             hipLaunchKernelGGL(k_mg_modes<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         else
             hipLaunchKernelGGL(k_mg_modes<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_DIRECT, pe, 32.0 * ntot * C);
         prof_begin(r, &pe);
-        hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
+        if (mfma) gemm_f64<true, false>(DftInv{a, C, n, C}, 2, r->stream);
+        else hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)

@@ -211,6 +211,35 @@ def test_poisson_coarse_solve_variants(direct_cols, geom):
     box.close()
 
 
+@pytest.mark.parametrize("geom", [0, 1])
+def test_poisson_direct_matrix_core_transforms(geom):
+    """At 1024 columns the direct level's transforms run on the matrix cores
+    (the default direct solve of the whole plane): the exact solve and, at
+    rtol 1e-12, its refinement cycles (projection of the residual, added
+    synthesis) against the oracle's exact solution."""
+    nr, np_ = 24, 1024
+    print(f"[gpu] direct solve, matrix-core transforms, geom={geom}", flush=True)
+    if geom:
+        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
+        dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
+    else:
+        box = _box(nr, np_, 2.0, rtol=1e-12, max_cycles=60)
+        dr, dp = 0.5 / nr, 2.0 / np_
+    rng = np.random.default_rng(12)
+    rhs = np.ascontiguousarray(rng.standard_normal((np_ + 1, nr + 2)))
+    rhs[0] = rhs[np_]
+    for which in (2, 3):
+        ni = nr + 2 if which == 2 else nr + 1
+        psi, psip = np.zeros_like(rhs), np.zeros_like(rhs)
+        st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
+        assert st == 0 and rel <= 1e-12 and cyc <= 3, (st, cyc, rel)
+        ex = np.zeros_like(psi)
+        O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
+        err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
+        assert err < 1e-10, (which, err)
+    box.close()
+
+
 @pytest.mark.parametrize("direct_cols", [16, 128])
 def test_trajectory_1000_steps_config1(direct_cols):
     """Periodic RB box 129x65 nodes (nr=64, np=128), 2:1 (width 1, height
