@@ -16,7 +16,6 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
     return x * per + (x < rem ? x : rem) + q;
 }
 
-constexpr int GK = 16;    // GEMM k step
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -29,7 +28,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // lane l is D[l/16 + 4e][l%16].  The next k step's operands are loaded into
 // registers while the current one is multiplied.  AKFAST / BKFAST: the tile
 // loads run along k (operand contiguous in k) rather than along m / n.
-template <class Op, bool AKFAST, bool BKFAST, int WAVES>
+template <class Op, bool AKFAST, bool BKFAST, int WAVES, int GK>
 __global__ void __launch_bounds__(64 * WAVES) k_gemm_f64(Op op)
 {
     constexpr int WR = (WAVES == 4) ? 2 : 1, WC = WAVES / WR;
@@ -117,17 +116,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_gemm_f64(Op op)
 }
 
 // 64 x 64 tiles of four waves when they fill the chip, else 32 x 32 tiles of
-// one wave (four times the workgroups for small planes)
-template <bool AKFAST, bool BKFAST, class Op>
+// one wave (four times the workgroups for small planes).  GK is the k step:
+// 16 by default; a deeper step (fewer, larger load rounds) suits short,
+// latency-bound k loops.
+template <bool AKFAST, bool BKFAST, class Op, int GK = 16>
 void gemm_f64(const Op &op, int batch, hipStream_t s)
 {
     const long long big = (long long)((op.N + 63) / 64) * ((op.M + 63) / 64) * batch;
     if (big >= 256) {
         dim3 grid((op.N + 63) / 64, (op.M + 63) / 64, batch);
-        hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 4>), grid, dim3(256), 0, s, op);
+        hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 4, GK>), grid, dim3(256), 0, s, op);
     } else {
         dim3 grid((op.N + 31) / 32, (op.M + 31) / 32, batch);
-        hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 1>), grid, dim3(64), 0, s, op);
+        hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 1, GK>), grid, dim3(64), 0, s, op);
     }
 }
 
