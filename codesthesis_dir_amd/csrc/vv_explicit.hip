@@ -114,7 +114,7 @@ __device__ __forceinline__ double rhs_p(const CompactArgs &a, int i, int j)
 // the right-hand sides of its T columns into LDS with coalesced loads, one
 // thread per column then runs the recurrences in the reference's order, and
 // the workgroup writes the solutions back.  LDS row stride nP is odd.
-template <int FAM>
+template <int FAM, bool TL>
 __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 {
     const int nr = a.nr, ld = a.ld, np = a.np;
@@ -127,11 +127,11 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     const int tc = (np + 1 - k0 < T) ? np + 1 - k0 : T;
     // the elimination factors go to LDS with the right-hand sides (when they
     // fit), so the recurrences below read nothing from global memory
-    if (Tab) {
+    if (TL) {
 #pragma unroll 8
         for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) Tab[idx] = tab[idx];
     }
-    const double *tb = Tab ? Tab : tab;
+    const double *tb = TL ? Tab : tab;
     const double *fact = tb, *bp = tb + n, *cc = tb + 2 * n, *rbp = tb + 3 * n;
 #pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
@@ -187,7 +187,7 @@ __device__ void tdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 // f1 | f2 | a | b | c | fA fB fC).  LDS holds entry j of row c at
 // (j-1)*T + c, so the per-row recurrences of consecutive threads touch
 // consecutive words.
-template <int FAM>
+template <int FAM, bool TL>
 __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
 {
     const int np = a.np, ld = a.ld;
@@ -199,11 +199,11 @@ __device__ void ptdma_tile(const CompactArgs &a, double *R, double *Tab, int T)
     const int tc = (rows + 1 - i0 < T) ? rows + 1 - i0 : T;
     double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
     // elimination factors to LDS when they fit (see tdma_tile)
-    if (Tab) {
+    if (TL) {
 #pragma unroll 8
         for (int idx = threadIdx.x; idx < 6 * s + 3; idx += blockDim.x) Tab[idx] = a.tab_p[idx];
     }
-    const double *tb = Tab ? Tab : a.tab_p;
+    const double *tb = TL ? Tab : a.tab_p;
 #pragma unroll 8
     for (int idx = threadIdx.x; idx < tc * n; idx += blockDim.x) {
         const int j = idx / tc, c = idx - j * tc;
@@ -282,16 +282,31 @@ struct CompactTiles {
 __global__ void __launch_bounds__(WG) k_compact(CompactArgs a, CompactTiles t)
 {
     extern __shared__ double R[];
-    double *Tab = t.tab_lds ? R + t.rwords : nullptr;
     const int fam = blockIdx.y;
     if (!(a.mask & (1 << fam))) return;
-    switch (fam) {
-    case 0: tdma_tile<0>(a, R, Tab, t.Tr); break;
-    case 1: tdma_tile<1>(a, R, Tab, t.Tr); break;
-    case 2: tdma_tile<2>(a, R, Tab, t.Tr); break;
-    case 3: ptdma_tile<3>(a, R, Tab, t.Tp); break;
-    case 4: ptdma_tile<4>(a, R, Tab, t.Tp); break;
-    case 5: ptdma_tile<5>(a, R, Tab, t.Tp); break;
+    // the table's place is a template parameter, so that the recurrences'
+    // table reads compile to LDS reads (a pointer chosen at run time between
+    // LDS and global memory would make them generic loads, whose every wait
+    // covers the global-memory counter as well)
+    if (t.tab_lds) {
+        double *Tab = R + t.rwords;
+        switch (fam) {
+        case 0: tdma_tile<0, true>(a, R, Tab, t.Tr); break;
+        case 1: tdma_tile<1, true>(a, R, Tab, t.Tr); break;
+        case 2: tdma_tile<2, true>(a, R, Tab, t.Tr); break;
+        case 3: ptdma_tile<3, true>(a, R, Tab, t.Tp); break;
+        case 4: ptdma_tile<4, true>(a, R, Tab, t.Tp); break;
+        case 5: ptdma_tile<5, true>(a, R, Tab, t.Tp); break;
+        }
+    } else {
+        switch (fam) {
+        case 0: tdma_tile<0, false>(a, R, nullptr, t.Tr); break;
+        case 1: tdma_tile<1, false>(a, R, nullptr, t.Tr); break;
+        case 2: tdma_tile<2, false>(a, R, nullptr, t.Tr); break;
+        case 3: ptdma_tile<3, false>(a, R, nullptr, t.Tp); break;
+        case 4: ptdma_tile<4, false>(a, R, nullptr, t.Tp); break;
+        case 5: ptdma_tile<5, false>(a, R, nullptr, t.Tp); break;
+        }
     }
 }
 
