@@ -17,6 +17,7 @@
 // residual of 1e-5, tolcon at rot3dmg.f:1562).
 
 #include "vv3_internal.h"
+#include "vv_fft.h"
 #include "vv_gemm.h"
 
 #include <algorithm>
@@ -28,6 +29,9 @@ namespace vv3 {
 
 namespace {
 
+using vv::bitrev;
+using vv::fft_pairs;
+using vv::fft_stages;
 using vv::gemm_f64;
 
 // 1. spectra S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
@@ -95,29 +99,6 @@ struct PhiInv {
 // (LP np complex values, at most 64 KB); k' = k mod np indexes the line, so
 // k = np is position 0.  Stage s pairs positions p and p + h (h = 2^s) with
 // the twiddle e^{-i 2 pi (p mod h)/(2h)} (conjugated for the inverse).
-
-__device__ __forceinline__ unsigned bitrev(unsigned x, int bits) { return __builtin_bitreverse32(x) >> (32 - bits); }
-
-template <bool INV>
-__device__ void fft_stages(double2 *z, const double2 *tw, int np, int bits, int LP)
-{
-    for (int s = 0; s < bits; ++s) {
-        const int h = 1 << s;
-        const int nb = LP * (np >> 1);
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const int pr = b / (np >> 1), t = b - pr * (np >> 1);
-            const int pos = t & (h - 1), grp = t >> s;
-            const int i0 = pr * np + grp * 2 * h + pos, i1 = i0 + h;
-            double2 w = tw[pos << (bits - 1 - s)];
-            if (INV) w.y = -w.y;
-            const double2 u = z[i0], v = z[i1];
-            const double2 vw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
-            z[i0] = make_double2(u.x + vw.x, u.y + vw.y);
-            z[i1] = make_double2(u.x - vw.x, u.y - vw.y);
-        }
-        __syncthreads();
-    }
-}
 
 // forward: S[m][u] = sum_k rhs(i,j,k) cos(2 pi m k/np) (m = 0..np/2) and
 // S[np/2+m][u] = sum_k rhs(i,j,k) sin(...) (0 < m < np/2), u = (j-1) i1 + (i-1)
@@ -193,9 +174,6 @@ __global__ void __launch_bounds__(256) k3_fft_inv(const double *S, double *psi, 
     }
 }
 
-// pairs of lines per workgroup: LP np complex values within 32 KB (64 KB for
-// the longest lines)
-inline int fft_pairs(int np) { return np >= 4096 ? 1 : std::max(1, 2048 / np); }
 
 // psip <- psi on the range the reference's routine writes, and psi <- 0 on
 // that range's wall points (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)

@@ -372,7 +372,7 @@ int vv_destroy(vv_run *r)
     double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                         r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
                         r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                        r->tab_p, r->mg_pool, r->diag, r->part, r->kepart};
+                        r->tab_p, r->mg_pool, r->mg_ftw, r->diag, r->part, r->kepart};
     for (double *f : fields)
         if (f) (void)hipFree(f);
     if (r->mg_state) (void)hipFree(r->mg_state);

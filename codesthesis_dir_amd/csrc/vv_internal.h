@@ -70,6 +70,10 @@ struct MgLevelDev {
     size_t tabsz;
     double *xm;
     const double *twid;
+    // direct level of 8 or more columns (a power of two): the FFT twiddles
+    // (cos, -sin)(2 pi t/ncol), t < ncol/2, in a buffer of their own; null
+    // elsewhere
+    const double *ftw;
 };
 // default of vv_config.direct_cols (0): the whole plane is the direct
 // level up to 1024 columns, a V-cycle down to 128 columns above
@@ -136,6 +140,7 @@ struct vv_run {
     // multigrid
     vv::MgBatchDev mg{};
     double *mg_pool = nullptr;
+    double *mg_ftw = nullptr;   // FFT twiddles of the direct level (see MgLevelDev::ftw)
     // solver state, one device block read back with a single copy:
     //   bytes 0-31 int done[2], cycles[2], unused[4];
     //   32-63 u64 unused[2], fnorm[2] (bits of non-negative doubles);

@@ -23,9 +23,10 @@
 // converged, so several cycles can be queued between host checks.
 
 #include "vv_internal.h"
-#include "vv_gemm.h"
+#include "vv_fft.h"
 #include "vv_reduce.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -51,6 +52,7 @@ enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
 struct KLev {
     double *u, *uo, *f;     // current iterate, output buffer of a sweep, rhs
     const double *lu, *twid;
+    const double2 *ftw;     // direct level: FFT twiddles (null: dense transforms)
     double *xm;
     double *dlt;            // black-line changes of the last pre-sweep (level L-1)
     double cphi;
@@ -564,49 +566,101 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
 }
 
-// The direct level's transforms as FP64 matrix-core GEMMs (gemm_f64), the
-// two problems as the batch.  Projection: X_q[m][i] = sum_k W[m][k] Y_q[k][i]
-// over the level's C columns k, Y the level's right-hand side or, in a
-// refinement cycle on level 0, its residual; synthesis: the level's iterate
-// u_q[k][i] = (or +=) 1/C sum_m Wi[k][m] X_q[m][i], stored only while the
-// problem runs.  The problems' row counts differ (nr, nr-1): N is the larger
-// and the accessors bound each problem's rows.
-struct DftFwd {
-    KArgs ka;
-    int M, N, K;
-    __device__ double a(int, int m, int k) const { return ka.P[0].L.twid[(size_t)m * K + k]; }
-    __device__ double b(int q, int k, int i) const
-    {
-        const KProb &P = ka.P[q];
-        if (i >= P.n) return 0.0;
-        if (ka.lev0 && ka.cycles[q] > 0) return resid_at(P, P.L, i, k);
-        return P.L.f[(size_t)k * P.n + i];
+// The direct level's transforms as radix-2 FFTs in LDS (C a power of two,
+// 8 or more columns): a workgroup transforms LP pairs of lines (rows i of the
+// level, two real lines per complex line), the problem is blockIdx.y.  The
+// projection writes the real lines of the modes, ml = 0 (m = 0), 2m-1 and 2m
+// (cos and sin of 0 < m < C/2), C-1 (m = C/2), i.e. X[ml][i] = sum_k W[ml][k]
+// Y[k][i] as the dense transform; Y is the level's right-hand side or, in a
+// refinement cycle on level 0, its residual.  The synthesis forms
+// (1/C) sum_m w_m (C_m cos + S_m sin) (w = 1 at m = 0, C/2, else 2) and
+// stores it into, or on refinement adds it to, the level's iterate.
+__global__ void __launch_bounds__(256) k_mg_fft_fwd(KArgs a, int bits, int LP)
+{
+    extern __shared__ double2 z[];
+    const int p = blockIdx.y;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, C = L.ncol, half = C >> 1, nl = 2 * LP;
+    const int ib = blockIdx.x * nl;
+    if (ib >= n || a.done[p]) return;   // uniform over the workgroup
+    const bool from_res = a.lev0 && a.cycles[p] > 0;
+    for (int e = threadIdx.x; e < nl * C; e += blockDim.x) {
+        const int l = e % nl, k = e / nl;
+        const int i = ib + l;
+        double v = 0.0;
+        if (i < n) v = from_res ? resid_at(P, L, i, k) : L.f[(size_t)k * n + i];
+        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * C + bitrev(k, bits)]);
+        zz[l & 1] = v;
     }
-    __device__ void c(int q, int m, int i, double v) const
-    {
-        const KProb &P = ka.P[q];
-        if (i < P.n) P.L.xm[(size_t)m * P.n + i] = v;
+    __syncthreads();
+    fft_stages<false>(z, L.ftw, C, bits, LP);
+    for (int e = threadIdx.x; e < nl * (half + 1); e += blockDim.x) {
+        const int l = e % nl, m = e / nl;
+        const int i = ib + l;
+        if (i >= n) continue;
+        const double2 zm = z[(l >> 1) * C + m], zn = z[(l >> 1) * C + ((C - m) & (C - 1))];
+        // A = (Z_m + conj Z_{C-m})/2, B = (Z_m - conj Z_{C-m})/(2i)
+        double re, im;
+        if ((l & 1) == 0) { re = 0.5 * (zm.x + zn.x); im = 0.5 * (zm.y - zn.y); }
+        else { re = 0.5 * (zm.y + zn.y); im = -0.5 * (zm.x - zn.x); }
+        if (m == 0) {
+            L.xm[i] = re;
+        } else if (m == half) {
+            L.xm[(size_t)(C - 1) * n + i] = re;
+        } else {
+            L.xm[(size_t)(2 * m - 1) * n + i] = re;    // sum y cos
+            L.xm[(size_t)(2 * m) * n + i] = -im;       // sum y sin
+        }
     }
-};
-struct DftInv {
-    KArgs ka;
-    int M, N, K;
-    __device__ double a(int, int m, int k) const { return ka.P[0].L.twid[(size_t)K * K + (size_t)m * K + k]; }
-    __device__ double b(int q, int k, int i) const
-    {
-        const KProb &P = ka.P[q];
-        return (i < P.n) ? P.L.xm[(size_t)k * P.n + i] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_mg_fft_inv(KArgs a, int bits, int LP)
+{
+    extern __shared__ double2 z[];
+    const int p = blockIdx.y;
+    const KProb &P = a.P[p];
+    const KLev &L = P.L;
+    const int n = P.n, C = L.ncol, half = C >> 1, nl = 2 * LP;
+    const int ib = blockIdx.x * nl;
+    if (ib >= n || a.done[p]) return;   // uniform over the workgroup
+    // Z_m = X_m + i Y_m of the pair, X_m = C_m - i S_m (m <= C/2), X_{C-m} = conj X_m
+    for (int e = threadIdx.x; e < LP * C; e += blockDim.x) {
+        const int pr = e % LP, m = e / LP;
+        const int mm = (m <= half) ? m : C - m;
+        const double sg = (m <= half) ? -1.0 : 1.0;
+        double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
+        for (int h2 = 0; h2 < 2; ++h2) {
+            const int i = ib + 2 * pr + h2;
+            if (i >= n) continue;
+            double c, sn = 0.0;
+            if (mm == 0) {
+                c = L.xm[i];
+            } else if (mm == half) {
+                c = L.xm[(size_t)(C - 1) * n + i];
+            } else {
+                c = L.xm[(size_t)(2 * mm - 1) * n + i];
+                sn = L.xm[(size_t)(2 * mm) * n + i];
+            }
+            if (h2 == 0) { xr = c; xi = sg * sn; } else { yr = c; yi = sg * sn; }
+        }
+        z[pr * C + bitrev(m, bits)] = make_double2(xr - yi, xi + yr);
     }
-    __device__ void c(int q, int m, int i, double v) const
-    {
-        const KProb &P = ka.P[q];
-        if (i >= P.n || ka.done[q]) return;
-        double *d = P.L.u + cix(i, m, P.n);
-        const double w = v / (double)K;
-        if (!ka.lev0 || ka.cycles[q] == 0) *d = w;
-        else *d += w;
+    __syncthreads();
+    fft_stages<true>(z, L.ftw, C, bits, LP);
+    const double inv = 1.0 / C;
+    const bool store = !a.lev0 || a.cycles[p] == 0;
+    for (int e = threadIdx.x; e < nl * C; e += blockDim.x) {
+        const int l = e % nl, q = e / nl;
+        const int i = ib + l;
+        if (i >= n) continue;
+        const double2 v = z[(l >> 1) * C + q];
+        const double x = ((l & 1) ? v.y : v.x) * inv;
+        double *d = L.u + cix(i, q, n);
+        if (store) *d = x;
+        else *d += x;
     }
-};
+}
 
 // max-norm of the level-0 residual: one point per thread, one partial per
 // workgroup (no atomics: same-address atomics from hundreds of workgroups
@@ -887,6 +941,7 @@ int mg_build(vv_run *r)
             std::memcpy(host.data() + off, H.tab.data(), H.tab.size() * sizeof(double));
             L.lu = r->mg_pool + off;
             L.twid = H.direct ? L.lu + (size_t)(H.ncol / 2 + 1) * H.tabsz : nullptr;
+            L.ftw = nullptr;
             off += H.tab.size();
             L.xm = nullptr;
             if (H.direct) { L.xm = r->mg_pool + off; off += (size_t)n * H.ncol; }
@@ -894,6 +949,23 @@ int mg_build(vv_run *r)
     }
     VV_HIP(hipMemcpyAsync(r->mg_pool, host.data(), total * sizeof(double), hipMemcpyHostToDevice,
                           r->stream));
+    {   // FFT twiddles of the direct level (both problems have its columns)
+        const int C = r->mg.prob[0].lev[nlev - 1].ncol;
+        std::vector<double> tw;
+        if (C >= 8 && (C & (C - 1)) == 0) {
+            const double pi = std::acos(-1.0);
+            tw.resize(C);
+            for (int t = 0; t < C / 2; ++t) {
+                tw[2 * t] = std::cos(2.0 * pi * t / C);
+                tw[2 * t + 1] = -std::sin(2.0 * pi * t / C);
+            }
+            VV_HIP(hipMalloc(&r->mg_ftw, tw.size() * sizeof(double)));
+            VV_HIP(hipMemcpyAsync(r->mg_ftw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  r->stream));
+            for (int p = 0; p < 2; ++p) r->mg.prob[p].lev[nlev - 1].ftw = r->mg_ftw;
+        }
+        VV_HIP(hipStreamSynchronize(r->stream));   // the host tables go out of scope
+    }
     r->mg.done = r->mg_flags;
     r->mg.cycles = r->mg_flags + 2;
     r->mg.fnorm = r->mg_bits + 2;
@@ -919,6 +991,7 @@ static KLev klev(const MgLevelDev &L, int cur)
     k.u = cur ? L.u2 : L.u;
     k.uo = cur ? L.u : L.u2;
     k.f = L.f; k.lu = L.lu; k.twid = L.twid; k.xm = L.xm; k.dlt = L.dlt;
+    k.ftw = reinterpret_cast<const double2 *>(L.ftw);
     k.cphi = L.cphi; k.ncol = L.ncol; k.nstep = L.nstep; k.tabsz = (int)L.tabsz;
     return k;
 }
@@ -999,15 +1072,21 @@ static int enqueue_cycle(vv_run *r)
         }
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
-        // matrix-core transforms from 1024 columns (measured: 1897 against
-        // 1652 steps/s at 1025 x 513); below, where the GEMM's sequence of
-        // k steps is latency-bound, the VALU kernel is faster (5860 against
-        // 4784 steps/s at 513 x 257)
-        const bool mfma = C >= 1024;
+        // FFTs where the level has twiddles (C >= 8, a power of two), the
+        // dense VALU transforms below.  LP line pairs per workgroup, few
+        // enough for about 256 workgroups (measured against LP = 2048/C:
+        // 8243 -> 10027 steps/s at 129 x 65, 5386 -> 5991 at 513 x 257;
+        // against the dense transforms 1902 -> 2896 steps/s at 1025 x 513;
+        // matrix-core GEMMs lose to both, their k loop being latency-bound)
+        const bool fft = B.prob[0].lev[L].ftw != nullptr;
+        const int fbits = fft ? __builtin_ctz((unsigned)C) : 0;
+        const int LP = std::max(1, std::min(fft_pairs(C), n / 256));
+        const dim3 fg((unsigned)((n + 2 * LP - 1) / (2 * LP)), 2);
+        const size_t flds = (size_t)LP * C * sizeof(double2);
         prof_begin(r, &pe);
         // projection (read the level, write the mode lines) and line solves
         // (read and write the mode lines)
-        if (mfma) gemm_f64<true, false>(DftFwd{a, C, n, C}, 2, r->stream);
+        if (fft) hipLaunchKernelGGL(k_mg_fft_fwd, fg, dim3(256), flds, r->stream, a, fbits, LP);
         else hipLaunchKernelGGL(k_mg_dft<0>, gd, dim3(256), 0, r->stream, a);
         if (two)
             hipLaunchKernelGGL(k_mg_modes<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
@@ -1015,7 +1094,7 @@ static int enqueue_cycle(vv_run *r)
             hipLaunchKernelGGL(k_mg_modes<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_DIRECT, pe, 32.0 * ntot * C);
         prof_begin(r, &pe);
-        if (mfma) gemm_f64<true, false>(DftInv{a, C, n, C}, 2, r->stream);
+        if (fft) hipLaunchKernelGGL(k_mg_fft_inv, fg, dim3(256), flds, r->stream, a, fbits, LP);
         else hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }

@@ -212,13 +212,14 @@ def test_poisson_coarse_solve_variants(direct_cols, geom):
 
 
 @pytest.mark.parametrize("geom", [0, 1])
-def test_poisson_direct_matrix_core_transforms(geom):
-    """At 1024 columns the direct level's transforms run on the matrix cores
-    (the default direct solve of the whole plane): the exact solve and, at
-    rtol 1e-12, its refinement cycles (projection of the residual, added
-    synthesis) against the oracle's exact solution."""
+def test_poisson_direct_fft_1024(geom):
+    """The default direct solve of the whole plane at 1024 columns (the
+    direct level's transforms as FFTs over the columns, one workgroup per
+    pair of lines): the exact solve and, at rtol 1e-12, its refinement
+    cycles (projection of the residual, added synthesis) against the
+    oracle's exact solution."""
     nr, np_ = 24, 1024
-    print(f"[gpu] direct solve, matrix-core transforms, geom={geom}", flush=True)
+    print(f"[gpu] direct solve, FFT transforms, np=1024, geom={geom}", flush=True)
     if geom:
         box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
         dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
