@@ -57,11 +57,11 @@ WORKLOADS = {
 DEFAULT_WORKLOAD = "annulus-513x257"
 
 
-def gpu_leg(name, steps, warmup, nu):
+def gpu_leg(name, steps, warmup, nu, direct_cols=0):
     from codesthesis_dir_amd import BoxConfig, PeriodicBox
     from codesthesis_dir_amd.ic import box_temperature
     desc, kw, seed, amp = WORKLOADS[name]
-    cfg = BoxConfig(rtol=1e-10, max_cycles=60, nu1=nu, nu2=nu, **kw)
+    cfg = BoxConfig(rtol=1e-10, max_cycles=60, nu1=nu, nu2=nu, direct_cols=direct_cols, **kw)
     box = PeriodicBox(cfg, device=0)
     box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, amp, seed))
     box.start()
@@ -230,7 +230,7 @@ def rot3d_routines(nr, nz, np_, reps=10):
     return out
 
 
-def rot3d_leg(cpu=True, warmup=3):
+def rot3d_leg(cpu=True, warmup=3, only=None):
     """The 3D step (vorticity A/B/C, temperature, heat, three exact Poisson
     solves, wall update) on the GPU at both grids, and the reference build
     itself (its multigrid Poisson to the shipped absolute 1e-5) on one core
@@ -241,6 +241,8 @@ def rot3d_leg(cpu=True, warmup=3):
     out = {"physics": "rot3dmg.f, Ra=1e7, Re=1000 (Ta=1e6), Pr=7, Gr=0, dt=1e-6, r0=0.5, rmax=1, zmax=0.5",
            "data": "synthetic: conduction + 8 seeded azimuthal modes (amplitude 1e-3), fluid at rest"}
     for name, (nr, nz, np_) in ROT3D.items():
+        if only and name != only:
+            continue
         run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, **P))
         for k, v in rot3d_state(nr, nz, np_).items():
             run.put(k, v)
@@ -251,8 +253,10 @@ def rot3d_leg(cpu=True, warmup=3):
         el = time.perf_counter() - t0
         run.close()
         out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
-        if nr == 128:
+        if nr == 128 and not only:
             out[name]["routines"] = rot3d_routines(nr, nz, np_)
+    if only:
+        return out
     name, (nr, nz, np_), n = HOR3D
     run = Annulus3D(Annulus3DConfig(nr=nr, nz=nz, np=np_, **HOR3D_PHYS))
     for k, v in hor3d_state(nr, nz, np_).items():
@@ -456,6 +460,10 @@ def main():
     ap.add_argument("--serve-only", action="store_true",
                     help="measure only the served mixed-size sweep and print its JSON object")
     ap.add_argument("--slots", type=int, default=8, help="request-loop slots of the served sweep")
+    ap.add_argument("--direct-cols", type=int, default=0,
+                    help="vv_config.direct_cols of the main leg (0: the library default)")
+    ap.add_argument("--rot3d-grid", default=None, choices=sorted(ROT3D),
+                    help="with --rot3d-only: time this grid alone (for profiling)")
     ap.add_argument("--no-baseline", action="store_true",
                     help="with --serve-only: skip the one-at-a-time pass")
     ap.add_argument("--rot3d-only", action="store_true",
@@ -468,14 +476,14 @@ def main():
               "is described in DESIGN.md and not implemented", file=sys.stderr)
         sys.exit(2)
     if a.rot3d_only:
-        print(json.dumps(rot3d_leg(cpu=not a.no_cpu)))
+        print(json.dumps(rot3d_leg(cpu=not a.no_cpu and not a.rot3d_grid, only=a.rot3d_grid)))
         return
     if a.serve_only:
         print(json.dumps(serve_leg(slots=a.slots, baseline=not a.no_baseline)))
         return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
-    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu)
+    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu, a.direct_cols)
     ms_per_step = wall_ms / a.steps
     # dominant kernel by device time in the profiled pass
     dom = max(prof, key=lambda k: prof[k]["ms"])
@@ -501,9 +509,11 @@ def main():
         "data": "synthetic: seeded linear-in-r temperature + uniform splitmix64 noise, fluid at rest",
         "config": {"workload": a.workload, "description": WORKLOADS[a.workload][0],
                    "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
-                   "poisson": "direct solve of the whole plane (real DFT in phi, one PCR line solve "
-                              "per mode; the default up to np = 1024), checked against 1e-10 each "
-                              "step; the V-cycle is under alternatives"},
+                   "poisson": ("direct solve of the whole plane (real FFT in phi, one PCR line solve "
+                               "per mode; the default up to np = 1024), checked against 1e-10 each "
+                               "step; the V-cycle is under alternatives") if cfg.direct_cols in (0, cfg.np)
+                              else f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening, direct level "
+                                   f"at {cfg.direct_cols} columns"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_ms * 1e3,
@@ -514,13 +524,22 @@ def main():
         "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
         "nusselt_last_step": nu,
     }
+    def done(leg):
+        print(f"[bench] {leg} done", file=sys.stderr, flush=True)
+
+    done("time step")
     if not a.no_extra:
         line["alternatives"] = {"poisson_vcycle": vcycle_leg(a.workload, a.steps, a.warmup, a.nu)}
+        done("V-cycle alternative")
         line["vcycle_large"] = vcycle_large_leg()
+        done("large V-cycle")
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
+        done("3D step")
         line["served_sweep"] = serve_leg(slots=a.slots)
+        done("served sweep")
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.workload)
+        done("CPU baseline")
     print(json.dumps(line))
 
 
