@@ -573,6 +573,14 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
     int status = VV_OK;
     int cyc[2] = {0, 0};
     double rel[2] = {0, 0};
+    // Deferred solves: with the whole plane direct, once a synchronised
+    // solve has converged in one cycle, later steps queue that cycle and a
+    // device-side check (k_mg_check) without waiting for the host; the
+    // check's sticky word is read once at the end of the call.  Profiled
+    // runs and the V-cycle synchronise every solve.
+    const bool direct_all = r->cfg.direct_cols >= r->np;
+    bool any_deferred = false;
+    VV_HIP(hipMemsetAsync(r->mg_flags + 4, 0, sizeof(int), r->stream));
     VV_HIP(hipEventRecord(r->ev0, r->stream));
     for (int s = 0; s < nsteps; ++s) {
         const int first = (r->steps == 0) ? 1 : 0;
@@ -581,9 +589,16 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
         if (!st) st = launch_heat(r, r->diag + 2 * (size_t)(r->steps % r->diag_cap));
         if (!st) st = launch_rhs_guess(r, 3, 5);
         if (st) return st;
-        st = mg_solve(r, 5, cyc, rel);
-        if (st < 0) return st;
-        if (st == VV_NOT_CONVERGED) status = VV_NOT_CONVERGED;
+        const bool defer = direct_all && !r->prof_on && r->last_cycles[0] == 1 && r->last_cycles[1] == 1;
+        if (defer) {
+            st = mg_enqueue_deferred(r);
+            if (st) return st;
+            any_deferred = true;
+        } else {
+            st = mg_solve(r, 5, cyc, rel);
+            if (st < 0) return st;
+            if (st == VV_NOT_CONVERGED) status = VV_NOT_CONVERGED;
+        }
         if ((st = launch_update(r, (int)(r->steps + 1), 1))) return st;
         rotate(r);
         r->steps += 1;
@@ -597,6 +612,22 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
     float ms = 0.f;
     VV_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
     r->last_ms = ms;
+    if (any_deferred) {   // the deferred solves' sticky check and the last solve's state
+        struct { int flags[8]; unsigned long long bits[4]; double rel[2]; } h;
+        VV_HIP(hipMemcpy(&h, r->mg_state, sizeof(h), hipMemcpyDeviceToHost));
+        cyc[0] = h.flags[2];
+        cyc[1] = h.flags[3];
+        rel[0] = h.rel[0];
+        rel[1] = h.rel[1];
+        if (h.flags[4]) {
+            r->last_cycles[0] = r->last_cycles[1] = 0;   // synchronise the next solve
+            if (h.flags[4] & 2) {
+                set_error("non-finite Poisson right-hand side");
+                return VV_ENONFINITE;
+            }
+            status = VV_NOT_CONVERGED;
+        }
+    }
     if (info) {
         std::memset(info, 0, sizeof(*info));
         if (r->steps > 0) {

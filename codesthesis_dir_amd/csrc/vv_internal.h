@@ -196,6 +196,7 @@ int launch_copy_prev(vv_run *r);
 // multigrid (vv_mg.hip)
 int mg_build(vv_run *r);
 int mg_solve(vv_run *r, int which, int *cycles, double *relres);
+int mg_enqueue_deferred(vv_run *r);
 int launch_mg_scatter_out(vv_run *r, int which);
 int launch_compact_mask(vv_run *r, int mask);
 int launch_heat_from(vv_run *r, const double *th, double *dst2);

@@ -1117,6 +1117,32 @@ static int enqueue_cycle(vv_run *r)
     return VV_OK;
 }
 
+// After a queued cycle: a problem whose relative residual is above rtol
+// (or NaN; bit 0) or whose right-hand side norm is not finite (bit 1) sets
+// the sticky word flags[4], which the host reads once per vv_step call.
+__global__ void k_mg_check(int *flags, const unsigned long long *fnorm, const double *relres, double rtol)
+{
+    if (threadIdx.x != 0) return;
+    int s = flags[4];
+    for (int p = 0; p < 2; ++p) {
+        if (!(relres[p] <= rtol)) s |= 1;
+        if (!isfinite(__longlong_as_double((long long)fnorm[p]))) s |= 2;
+    }
+    flags[4] = s;
+}
+
+// One cycle of both problems and the sticky check, with no host
+// synchronisation (the time loop's deferred solves)
+int mg_enqueue_deferred(vv_run *r)
+{
+    int st = enqueue_cycle(r);
+    if (st) return st;
+    hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, r->stream, r->mg_flags,
+                       (const unsigned long long *)r->mg.fnorm, (const double *)r->mg.relres, r->cfg.rtol);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
 // Runs cycles until both active problems are done.  Cycles are queued in
 // chunks sized from the previous solve, then the flags are read back.
 int mg_solve(vv_run *r, int which, int *cycles, double *relres)

@@ -377,3 +377,35 @@ def test_odd_sweep_count_is_rejected():
     with pytest.raises(L.VVError) as e:
         _box(16, 32, 1.0, nu1=1, nu2=2)
     assert e.value.code == L.VV_EINVAL
+
+
+def test_deferred_solves_report_a_nonfinite_rhs():
+    """After a first synchronised solve converged in one cycle, vv_step
+    queues its direct solves without waiting (a device check records
+    failures, read once per call): the fields equal those of a run whose
+    every solve is synchronised (per-kernel timing on), and a NaN that
+    reaches a later right-hand side is still reported as VV_ENONFINITE."""
+    from codesthesis_dir_amd import _lib as L
+    from codesthesis_dir_amd.ic import box_temperature
+    print("[gpu] deferred solves", flush=True)
+    nr, np_ = 32, 64
+    a = _box(nr, np_, 1.0)
+    b = _box(nr, np_, 1.0)
+    th0 = box_temperature(nr, np_, 0.5, 1.0, 1e-3, 3)
+    for box in (a, b):
+        box.put("th", th0)
+        box.start()
+    a.step(6)                       # one synchronised solve, five deferred
+    b.profile(True)                 # timed kernels: every solve synchronised
+    b.step(6)
+    b.profile(False)
+    for f in ("C", "th", "uel", "vel"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    th = a.get("th")
+    th[10, 5] = np.nan
+    a.put("th", th)
+    with pytest.raises(L.VVError) as e:
+        a.step(3)
+    assert e.value.code == L.VV_ENONFINITE
+    a.close()
+    b.close()
