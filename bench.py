@@ -7,7 +7,7 @@ A step is one pass of the time loop (compact-scheme DuFort-Frankel step of
 vorticity and temperature, both Poisson solves to 1e-10 relative residual,
 wall update, heat flux, running means) over the device-resident fields.  The
 Poisson solves take the library's default: the direct solve of the whole
-plane for np <= 1024 (the V-cycle is measured beside it).  The timed region runs K steps after W warm-up steps; inputs are
+plane for np <= 8192 (the V-cycle is measured beside it).  The timed region runs K steps after W warm-up steps; inputs are
 resident in HBM before it starts.  A second pass of K steps with per-kernel
 HIP events gives the dominant kernel's average duration and its achieved
 algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
@@ -88,7 +88,7 @@ MG_KERNELS = ("zebra", "restrict", "prolong", "norm", "direct")
 def vcycle_leg(name, steps, warmup, nu=1):
     """The same workload with the Poisson solves by the V-cycle: zebra
     sweeps on the levels of more than 128 columns, the direct level at 128
-    (the default is the direct solve of the whole plane up to np = 1024)."""
+    (the default is the direct solve of the whole plane up to np = 8192)."""
     from codesthesis_dir_amd import BoxConfig, PeriodicBox
     from codesthesis_dir_amd.ic import box_temperature
     desc, kw, seed, amp = WORKLOADS[name]
@@ -109,20 +109,34 @@ def vcycle_leg(name, steps, warmup, nu=1):
 def vcycle_large_leg(steps=6, nu=1):
     """V-cycle bandwidth on a large plane: periodic box, 2049 x 4097 nodes
     (nr=2048, np=4096), V(nu,nu) with five smoothed levels and the direct
-    level at 128 columns; per-kernel HIP events over `steps` time steps."""
+    level at 128 columns; per-kernel HIP events over `steps` time steps.
+    The same plane's time step with the default direct solve is measured
+    beside it."""
     from codesthesis_dir_amd import BoxConfig, PeriodicBox
     from codesthesis_dir_amd.ic import box_temperature
-    cfg = BoxConfig(nr=2048, np=4096, width=2.0, dt=1e-7, Ra=1e5, Pr=1.0, rtol=1e-10, max_cycles=60,
-                    nu1=nu, nu2=nu)
+    base = dict(nr=2048, np=4096, width=2.0, dt=1e-7, Ra=1e5, Pr=1.0, rtol=1e-10, max_cycles=60)
+    cfg = BoxConfig(nu1=nu, nu2=nu, direct_cols=128, **base)
     box = PeriodicBox(cfg, device=0)
     box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, 1e-3, 20261206))
     box.start()
     box.step(2)
+    t0 = time.perf_counter()
+    box.step(steps)
+    vstep = steps / (time.perf_counter() - t0)
     box.profile(True)
     info = box.step(steps)
     prof = box.profile_read()
     box.profile(False)
     box.close()
+    dcfg = BoxConfig(**base)
+    dbox = PeriodicBox(dcfg, device=0)
+    dbox.put("th", box_temperature(dcfg.nr, dcfg.np, dcfg.r0, dcfg.rmax, 1e-3, 20261206))
+    dbox.start()
+    dbox.step(2)
+    t0 = time.perf_counter()
+    dinfo = dbox.step(steps)
+    dstep = steps / (time.perf_counter() - t0)
+    dbox.close()
     vb = sum(prof[k]["bytes"] for k in MG_KERNELS)
     vt = sum(prof[k]["ms"] for k in MG_KERNELS)
     z = prof["zebra"]
@@ -132,7 +146,10 @@ def vcycle_large_leg(steps=6, nu=1):
             "zebra_gbs": z["bytes"] / (z["ms"] * 1e-3) / 1e9 if z["ms"] > 0 else None,
             "zebra_avg_launch_us": 1e3 * z["ms"] / max(z["launches"], 1),
             "vcycle_ms_per_step": vt / steps,
-            "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
+            "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
+            "vcycle_steps_per_s": vstep,
+            "direct_steps_per_s": dstep,
+            "direct_cycles_last_step": [dinfo.cycles_v, dinfo.cycles_u]}
 
 
 # 3D rotating annulus (rot3dmg.f): the reference program's largest grid
@@ -510,7 +527,7 @@ def main():
         "config": {"workload": a.workload, "description": WORKLOADS[a.workload][0],
                    "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
                    "poisson": ("direct solve of the whole plane (real FFT in phi, one PCR line solve "
-                               "per mode; the default up to np = 1024), checked against 1e-10 each "
+                               "per mode; the default up to np = 8192), checked against 1e-10 each "
                                "step; the V-cycle is under alternatives") if cfg.direct_cols in (0, cfg.np)
                               else f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening, direct level "
                                    f"at {cfg.direct_cols} columns"},

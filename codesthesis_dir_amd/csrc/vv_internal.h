@@ -76,8 +76,9 @@ struct MgLevelDev {
     const double *ftw;
 };
 // default of vv_config.direct_cols (0): the whole plane is the direct
-// level up to 1024 columns, a V-cycle down to 128 columns above
-inline int default_direct_cols(int np) { return np <= 1024 ? np : 128; }
+// level up to 8192 columns (measured faster than the V-cycle at every grid
+// from 65 x 33 to 8193 x 1025), a V-cycle down to 128 columns above
+inline int default_direct_cols(int np) { return np <= 8192 ? np : 128; }
 
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;

@@ -900,10 +900,14 @@ int mg_build(vv_run *r)
                     const double v = (ml & 1) ? std::cos(ang) : std::sin(ang);
                     return inv ? 2.0 * v : v;
                 };
-                for (int ml = 0; ml < c; ++ml)
-                    for (int k = 0; k < c; ++k) L.tab.push_back(w(ml, k, false));
-                for (int q = 0; q < c; ++q)
-                    for (int ml = 0; ml < c; ++ml) L.tab.push_back(w(ml, q, true));
+                // (dense weights only where the level has no FFT: fewer than
+                // 8 columns; 2 c^2 words would be 537 MB at c = 4096)
+                if (c < 8 || (c & (c - 1)) != 0) {
+                    for (int ml = 0; ml < c; ++ml)
+                        for (int k = 0; k < c; ++k) L.tab.push_back(w(ml, k, false));
+                    for (int q = 0; q < c; ++q)
+                        for (int ml = 0; ml < c; ++ml) L.tab.push_back(w(ml, q, true));
+                }
                 total += 3 * (size_t)n * c + L.tab.size();
             }
             lh[p].push_back(std::move(L));

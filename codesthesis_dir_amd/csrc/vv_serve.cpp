@@ -124,7 +124,8 @@ double vv_run_footprint(const vv_config *cfg)
     // Poisson problem (two), the levels of the multigrid hierarchy: three
     // arrays of n x ncol and a reduction table of at most 2 + 23 n words per
     // smoothed level; the direct level also holds one such table per Fourier
-    // mode and its two ncol x ncol transform matrices
+    // mode and its FFT twiddles (below 8 columns: two ncol x ncol transform
+    // matrices)
     const double ld = (double)(((cfg->nr + 2) + 7) / 8 * 8);
     const double n = (double)cfg->nr;
     const int dc = cfg->direct_cols > 0 ? cfg->direct_cols : vv::default_direct_cols(cfg->np);
@@ -132,7 +133,8 @@ double vv_run_footprint(const vv_config *cfg)
     for (int c = cfg->np;; c /= 2) {
         const double tab = 2.0 + 23.0 * n;
         if (c <= dc || c < 2) {
-            mg += 3.0 * n * c + (c / 2 + 1) * tab + 2.0 * (double)c * c;
+            const bool fft = c >= 8 && (c & (c - 1)) == 0;   // no dense transform weights
+            mg += 3.0 * n * c + (c / 2 + 1) * tab + (fft ? (double)c : 2.0 * (double)c * c);
             break;
         }
         mg += 3.5 * n * c + tab;

@@ -43,7 +43,7 @@ class BoxConfig:
     stats: int = 1
     geom: int = 0          # 0: periodic box (rb.f); 1: rotating annulus plane (ann.f)
     Re: float = 0.0        # rotational Reynolds number (annulus)
-    direct_cols: int = 0  # direct level at <= this many columns (0: direct for np <= 1024, else 128)
+    direct_cols: int = 0  # direct level at <= this many columns (0: direct for np <= 8192, else 128)
 
     def c(self) -> L.VVConfig:
         return L.VVConfig(self.nr, self.np, self.dt, self.r0, self.rmax, self.width, self.Ra,

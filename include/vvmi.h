@@ -73,10 +73,11 @@ typedef struct vv_config {
                              where the level is solved exactly (real DFT in
                              phi + one line solve per mode); >= np makes the
                              whole solve direct.  0 selects the default:
-                             direct for np <= 1024 (one exact solve per
-                             Poisson call, measured 1.8-2.7x faster than the
-                             V-cycle at every grid up to 1025 x 513), a
-                             V-cycle down to 128 columns above.            */
+                             direct for np <= 8192 (one exact solve per
+                             Poisson call, measured 1.8-3.4x faster than the
+                             V-cycle at every grid from 65 x 33 to
+                             8193 x 1025), a V-cycle down to 128 columns
+                             above.                                         */
 } vv_config;
 
 /* per-step diagnostics (device-resident until read) */
