@@ -261,8 +261,6 @@ void rotate(vv_run *r)
     r->thf = oT;
 }
 
-bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-
 }  // namespace
 
 extern "C" {
@@ -280,8 +278,8 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
 {
     if (!cfg || !out) { set_error("null argument"); return VV_EINVAL; }
     *out = nullptr;
-    if (cfg->nr < 8 || cfg->np < 8 || !pow2(cfg->np)) {
-        set_error("need nr >= 8 and np a power of two >= 8");
+    if (cfg->nr < 8 || cfg->np < 8) {
+        set_error("need nr >= 8 and np >= 8");
         return VV_EINVAL;
     }
     if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->Pr > 0) ||

@@ -14,10 +14,11 @@
 //     are first folded into a tridiagonal system); both colours of a sweep
 //     run in one launch (k_mg_zebra);
 //   - coarsening: phi only, full weighting / linear interpolation, down to
-//     direct_cols columns or fewer; that level is solved exactly: the
-//     operator is invariant along phi, so a real DFT over the level's
-//     columns decouples it into one line system per Fourier mode
-//     (k_mg_direct: project, reduce; k_mg_idft: synthesise).
+//     direct_cols columns or fewer, or to the first odd column count; that
+//     level is solved exactly: the operator is invariant along phi, so a
+//     real DFT over the level's columns decouples it into one line system
+//     per Fourier mode (k_mg_fft_fwd or k_mg_dft<0>: project; k_mg_modes:
+//     line solves; k_mg_fft_inv or k_mg_dft<1>: synthesise).
 // Both velocity problems of a run are batched into every launch
 // (blockIdx.y = problem).  Kernels return at once when their problem has
 // converged, so several cycles can be queued between host checks.
@@ -423,11 +424,11 @@ __global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
     L.f[cix(j, k, n)] = v;
 }
 
-// Exact solve of the coarsest level with C = ncol columns (a power of two):
-// the operator is invariant along phi, so the real DFT over the C columns
-// decouples it into C line systems D_r + lambda_m diag(rphi), one per real
-// Fourier line ml (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m;
-// ml = C-1: the Nyquist mode).  Three kernels: k_mg_dft<0> projects the
+// Exact solve of the coarsest level with C = ncol columns: the operator is
+// invariant along phi, so the real DFT over the C columns decouples it into
+// C line systems D_r + lambda_m diag(rphi), one per real Fourier line ml
+// (ml = 0: m = 0; ml = 2m-1 / 2m: cos / sin of mode m; for even C,
+// ml = C-1: the Nyquist mode m = C/2).  Three kernels: k_mg_dft<0> projects the
 // residual on the lines, k_mg_modes solves each line by parallel cyclic
 // reduction, k_mg_dft<1> synthesises the correction.  Below level 0 the
 // correction starts from zero, so the residual is f.  On level 0 (the whole
@@ -490,7 +491,7 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
         for (int r = 0; r < WPT; ++r) {
             const int idx = threadIdx.x + r * 256;
             const int oo = idx / DFT_TK, kk = idx - oo * DFT_TK;
-            wv[r] = (oo < TM && kk < tk) ? W[(size_t)(o0 + oo) * C + k0 + kk] : 0.0;
+            wv[r] = (oo < TM && o0 + oo < C && kk < tk) ? W[(size_t)(o0 + oo) * C + k0 + kk] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < YPT; ++r) Ys[r * 4 + w][lane] = yv[r];
@@ -514,8 +515,8 @@ __global__ void __launch_bounds__(256) k_mg_dft(KArgs a)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int oo = 4 * w + j;
-        if (oo >= TM) continue;
         const int o = o0 + oo;
+        if (oo >= TM || o >= C) continue;
         if (!INV) {
             L.xm[(size_t)o * n + i] = acc[j];
         } else {
@@ -821,7 +822,8 @@ int mg_build(vv_run *r)
     if (nr > MAXN) { set_error("nr exceeds the line length held in LDS (2048)"); return VV_EINVAL; }
     const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 1;
     int nlev = 1;
-    for (int c = np; c > Cd; c /= 2) ++nlev;
+    // phi coarsening halves the column count while it is even
+    for (int c = np; c > Cd && c % 2 == 0; c /= 2) ++nlev;
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
     const int nvec[2] = {nr, nr - 1};
     // host-side tables first, then one pool
@@ -894,14 +896,15 @@ int mg_build(vv_run *r)
                 // the device).  Angles are reduced mod c before the cos/sin.
                 auto w = [&](int ml, int k, bool inv) {
                     if (ml == 0) return 1.0;
-                    if (c >= 2 && ml == c - 1) return (k & 1) ? -1.0 : 1.0;
+                    if (c % 2 == 0 && ml == c - 1) return (k & 1) ? -1.0 : 1.0;   // Nyquist
                     const int mm = (ml + 1) / 2;
                     const double ang = 2.0 * pi * (double)(((long long)mm * k) % c) / c;
                     const double v = (ml & 1) ? std::cos(ang) : std::sin(ang);
                     return inv ? 2.0 * v : v;
                 };
                 // (dense weights only where the level has no FFT: fewer than
-                // 8 columns; 2 c^2 words would be 537 MB at c = 4096)
+                // 8 columns or not a power of two; 2 c^2 words would be
+                // 537 MB at c = 4096)
                 if (c < 8 || (c & (c - 1)) != 0) {
                     for (int ml = 0; ml < c; ++ml)
                         for (int k = 0; k < c; ++k) L.tab.push_back(w(ml, k, false));
@@ -1075,9 +1078,9 @@ static int enqueue_cycle(vv_run *r)
             prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * 2.0 * C);
         }
         const int TM = C < DFT_TM ? C : DFT_TM;
-        const dim3 gd(nblk(n, DFT_TN), 2, C / TM);
+        const dim3 gd(nblk(n, DFT_TN), 2, (C + TM - 1) / TM);
         // FFTs where the level has twiddles (C >= 8, a power of two), the
-        // dense VALU transforms below.  LP line pairs per workgroup, few
+        // dense VALU transforms otherwise.  LP line pairs per workgroup, few
         // enough for about 256 workgroups (measured against LP = 2048/C:
         // 8243 -> 10027 steps/s at 129 x 65, 5386 -> 5991 at 513 x 257;
         // against the dense transforms 1902 -> 2896 steps/s at 1025 x 513;

@@ -132,7 +132,7 @@ double vv_run_footprint(const vv_config *cfg)
     double mg = 0.0;
     for (int c = cfg->np;; c /= 2) {
         const double tab = 2.0 + 23.0 * n;
-        if (c <= dc || c < 2) {
+        if (c <= dc || c < 2 || (c & 1)) {   // phi coarsening ends at an odd count
             const bool fft = c >= 8 && (c & (c - 1)) == 0;   // no dense transform weights
             mg += 3.0 * n * c + (c / 2 + 1) * tab + (fft ? (double)c : 2.0 * (double)c * c);
             break;

@@ -57,7 +57,7 @@ def _random_state(run, seed):
 
 @pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False), (1100, 32, False),
-                                           (40, 64, False)])
+                                           (40, 64, False), (96, 336, False), (20, 45, True)])
 def test_explicit_step_bitexact(nr, np_, first, geom):
     import ctypes
     run, box = _pair(geom, nr, np_)
@@ -104,13 +104,18 @@ def test_explicit_step_bitexact(nr, np_, first, geom):
 
 @pytest.mark.parametrize("nr,np_,width", [(16, 32, 3.0), (64, 128, 1.0), (64, 128, 3.0),
                                           (128, 256, 1.0), (48, 64, 2.0), (64, 128, 0.0),
-                                          (256, 256, 0.0)])
+                                          (256, 256, 0.0), (96, 336, 3.0), (32, 100, 0.0),
+                                          (16, 45, 1.0)])
 @pytest.mark.parametrize("which", [2, 3])
 @pytest.mark.parametrize("direct_cols", [8, 128])
 def test_poisson_matches_exact_solution(nr, np_, width, which, direct_cols):
     """width 0 marks the annulus geometry (dp = 2 pi/np, r0 = 0.5).
-    direct_cols 8: multigrid down to 8 columns; 128: the default (two
-    levels at np = 256, a direct solve at np <= 128)."""
+    direct_cols 8: multigrid down to 8 columns or to the first odd column
+    count (336 -> 21, 100 -> 25, 45 at once); 128: a direct level of at
+    most 128 columns (two levels at np = 256 and 336, a direct solve at
+    np <= 128).  96 x 336 is the size of the reference's own input deck
+    (input.s; the box width is 3, rb.f:132-133); 100 and 45 columns are neither powers of two nor multiples
+    of the dense transform's 16-column tiles."""
     geom = 1 if width == 0.0 else 0
     if geom:
         box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0, direct_cols=direct_cols)
@@ -238,6 +243,66 @@ def test_poisson_direct_fft_1024(geom):
         O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
         err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
         assert err < 1e-10, (which, err)
+    box.close()
+
+
+@pytest.mark.parametrize("geom,nr,np_", [(0, 96, 336), (1, 96, 336), (0, 30, 45), (1, 40, 100)])
+def test_poisson_default_direct_any_np(geom, nr, np_):
+    """The default configuration at column counts that are not powers of
+    two: one direct solve of the whole plane with the dense transforms
+    (odd counts have no Nyquist line), against the oracle's exact solution,
+    and at rtol 1e-12 its refinement cycles."""
+    if geom:
+        box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
+        dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
+    else:
+        box = _box(nr, np_, 3.0, rtol=1e-12, max_cycles=60)
+        dr, dp = 0.5 / nr, 3.0 / np_
+    assert box.nlev == 1
+    rng = np.random.default_rng(np_)
+    k = np.arange(np_ + 1)[:, None]
+    rhs = np.ascontiguousarray(np.cos(2 * np.pi * (np_ // 2) * k / np_) * np.ones((1, nr + 2))
+                               + rng.standard_normal((np_ + 1, nr + 2)))
+    rhs[0] = rhs[np_]
+    for which in (2, 3):
+        ni = nr + 2 if which == 2 else nr + 1
+        psi, psip = np.zeros_like(rhs), np.zeros_like(rhs)
+        st, cyc, rel = box.poisson_host(which, psi, psip, rhs)
+        assert st == 0 and rel <= 1e-12 and cyc <= 3, (st, cyc, rel)
+        ex = np.zeros_like(psi)
+        O.poisson_exact(which, ex, np.zeros_like(ex), rhs, nr, np_, dr, dp, geom=geom, r0=0.5)
+        err = np.max(np.abs(psi[:, :ni] - ex[:, :ni])) / np.max(np.abs(ex))
+        assert err < 1e-10, (which, err)
+        assert np.array_equal(psi[0], psi[np_])
+    box.close()
+
+
+def test_trajectory_reference_deck_size():
+    """The reference's own input deck (input.s: nr = 96, np = 336,
+    Ra = 1e7, Pr = 0.71, dt = 1e-7; the box width is 3, rb.f:132-133),
+    seeded temperature, 200 steps with the default direct solves against the oracle with exact
+    solves: relative L2 of every field <= 1e-9, wall fluxes to 1e-10."""
+    nr, np_, width, dt, Ra, Pr = 96, 336, 3.0, 1e-7, 1e7, 0.71
+    run = O.Run(nr, np_, dt, Ra, Pr, width=width)
+    run.seed()
+    box = _box(nr, np_, width, dt=dt, Ra=Ra, Pr=Pr, rtol=1e-10, max_cycles=60)
+    box.put("th", run["th"])
+    box.start()
+    info = box.step(200)
+    assert info.status == 0
+    trans = []
+    for _ in range(200):
+        st, tr, ncl = run.step(mode=1)
+        assert st == 0
+        trans.append(tr.copy())
+    for name in ("th", "C", "uel", "vel", "Cp", "thp"):
+        g, o = box.get(name), run[name]
+        err = np.linalg.norm(g - o) / np.linalg.norm(o)
+        print(f"{name}: rel L2 {err:.3e}")
+        assert err <= 1e-9, (name, err)
+    hist = box.heat_history(200)
+    trans = np.array(trans)
+    assert np.max(np.abs(hist - trans) / np.abs(trans)) < 1e-10
     box.close()
 
 

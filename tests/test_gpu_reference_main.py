@@ -169,3 +169,25 @@ def test_periodic_box_main_program_on_the_gpu():
     for n in ("C", "U", "V"):
         assert errs[n] < 1e-5, (n, errs[n])
     assert em < 1e-5
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "rb_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_periodic_box_main_program_on_its_own_deck():
+    """velvor2dperiodicrbconv.f's main program on the reference's own input
+    deck (input.s: dt 1e-7, nr 96, np 336, Ra 1e7, Pr 0.71, pert 0,
+    amp 1, nval 3), 200 steps instead of 50000, output file names of this
+    test, through the 2D drop-in against the reference executable.  np = 336
+    is not a power of two: the drop-in's direct solve uses the dense
+    transforms over the 336 columns."""
+    nts = 200
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1.d-7", "96", "336", str(nts),
+                      "0.5d0,1.0d0,0.1d0", "1.d7", "0d0", "0.d0", "0.71d0", "0.d0", "1.d0", "3", "no",
+                      "n"]) + "\n"
+    eh, errs, em = _plane_main(os.path.join(REF, "rb_omp"), os.path.join(REF, "rb_main_gpu"), deck, nts)
+    print(f"rb main, input.s deck: heat series rel {eh:.3e}; fields {errs}; running means {em:.3e}")
+    assert eh < 1e-9
+    assert errs["T"] < 1e-9
+    for n in ("C", "U", "V"):
+        assert errs[n] < 1e-5, (n, errs[n])
+    assert em < 1e-5
