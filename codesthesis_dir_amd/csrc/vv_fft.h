@@ -2,7 +2,8 @@
 // 2D and 3D solvers (device code; included by the .hip files only).
 //
 // A workgroup holds LP pairs of real lines; each pair is one complex line of
-// n points (n a power of two, loaded in bit-reversed order).  Stage s pairs
+// n points (n a power of two, loaded in bit-reversed order; other even n
+// through the mixed-radix step below).  Stage s pairs
 // positions p and p + h (h = 2^s) with the twiddle e^{-i 2 pi (p mod h)/(2h)},
 // taken from tw[t] = (cos, -sin)(2 pi t/n), t < n/2 (conjugated for the
 // inverse).  Z_m and Z_{n-m} separate the pair's two real spectra.
@@ -35,6 +36,55 @@ __device__ void fft_stages(double2 *z, const double2 *tw, int np, int bits, int 
         }
         __syncthreads();
     }
+}
+
+// Mixed radix, n = b * n1 points (n1 = 2^bits, b odd > 1): point k of a line
+// is loaded into sub-line k mod b at position bitrev(k / b), fft_stages
+// transforms the LP*b sub-lines of n1 points (twiddles of n1), and
+// fft_combine forms, in place and in natural order,
+//   X_k = sum_{j<b} W_n^{jk} A_j[k mod n1],  W_n^t = twc[t] = (cos, -sin)(2 pi t/n),
+// which is the DFT of the line because W_n^{(b q + j) k} = W_{n1}^{q k} W_n^{j k}
+// (conjugated twiddles give the inverse).  b = 1 is the plain radix-2 line.
+__device__ __forceinline__ int mr_pos(int k, int b, int n1, int bits)
+{
+    if (b == 1) return (int)bitrev((unsigned)k, bits);
+    const int q = k / b;
+    return (k - q * b) * n1 + (int)bitrev((unsigned)q, bits);
+}
+
+// R outputs per thread: LP*n <= R*blockDim.x (checked by the host)
+template <bool INV, int R>
+__device__ void fft_combine(double2 *z, const double2 *twc, int n, int n1, int b, int LP)
+{
+    double2 acc[R];
+    const int total = LP * n;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = (int)threadIdx.x + r * (int)blockDim.x;
+        double re = 0.0, im = 0.0;
+        if (e < total) {
+            const int pr = e / n, k = e - pr * n;
+            const double2 *zl = z + pr * n + (k & (n1 - 1));
+            int t = 0;
+            for (int j = 0; j < b; ++j) {
+                double2 w = twc[t];
+                if (INV) w.y = -w.y;
+                const double2 v = zl[j * n1];
+                re += v.x * w.x - v.y * w.y;
+                im += v.x * w.y + v.y * w.x;
+                t += k;
+                if (t >= n) t -= n;
+            }
+        }
+        acc[r] = make_double2(re, im);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = (int)threadIdx.x + r * (int)blockDim.x;
+        if (e < total) z[e] = acc[r];
+    }
+    __syncthreads();
 }
 
 // pairs of lines per workgroup: LP np complex values within 32 KB (64 KB for

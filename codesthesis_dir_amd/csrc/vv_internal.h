@@ -70,15 +70,20 @@ struct MgLevelDev {
     size_t tabsz;
     double *xm;
     const double *twid;
-    // direct level of 8 or more columns (a power of two): the FFT twiddles
-    // (cos, -sin)(2 pi t/ncol), t < ncol/2, in a buffer of their own; null
-    // elsewhere
+    // direct level with FFTs (fft_cols): the twiddles (cos, -sin)(2 pi t/n1),
+    // t < n1/2, of its 2^bits = n1 point radix-2 sub-lines and, when ncol is
+    // not a power of two, (cos, -sin)(2 pi t/ncol), t < ncol, of the
+    // mixed-radix combination, in a buffer of their own; null elsewhere
     const double *ftw;
 };
 // default of vv_config.direct_cols (0): the whole plane is the direct
 // level up to 8192 columns (measured faster than the V-cycle at every grid
 // from 65 x 33 to 8193 x 1025), a V-cycle down to 128 columns above
 inline int default_direct_cols(int np) { return np <= 8192 ? np : 128; }
+
+// a direct level of c columns is transformed by FFTs (radix 2 times the odd
+// factor of c) when this holds, by dense transforms otherwise
+inline bool fft_cols(int c) { return c >= 8 && c % 2 == 0 && c <= 8192; }
 
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;

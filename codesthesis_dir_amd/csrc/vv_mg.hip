@@ -567,16 +567,19 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
 }
 
-// The direct level's transforms as radix-2 FFTs in LDS (C a power of two,
-// 8 or more columns): a workgroup transforms LP pairs of lines (rows i of the
-// level, two real lines per complex line), the problem is blockIdx.y.  The
+// The direct level's transforms as FFTs in LDS (C even, 8 <= C <= 8192):
+// radix 2 when C is a power of two (R = 0); otherwise C = b 2^bits with b odd,
+// b radix-2 sub-lines per line and the in-place combination of fft_combine
+// (R outputs per thread).  A workgroup transforms LP pairs of lines (rows i
+// of the level, two real lines per complex line), the problem is blockIdx.y.  The
 // projection writes the real lines of the modes, ml = 0 (m = 0), 2m-1 and 2m
 // (cos and sin of 0 < m < C/2), C-1 (m = C/2), i.e. X[ml][i] = sum_k W[ml][k]
 // Y[k][i] as the dense transform; Y is the level's right-hand side or, in a
 // refinement cycle on level 0, its residual.  The synthesis forms
 // (1/C) sum_m w_m (C_m cos + S_m sin) (w = 1 at m = 0, C/2, else 2) and
 // stores it into, or on refinement adds it to, the level's iterate.
-__global__ void __launch_bounds__(256) k_mg_fft_fwd(KArgs a, int bits, int LP)
+template <int R>
+__global__ void __launch_bounds__(256) k_mg_fft_fwd(KArgs a, int bits, int b, int LP)
 {
     extern __shared__ double2 z[];
     const int p = blockIdx.y;
@@ -591,16 +594,18 @@ __global__ void __launch_bounds__(256) k_mg_fft_fwd(KArgs a, int bits, int LP)
         const int i = ib + l;
         double v = 0.0;
         if (i < n) v = from_res ? resid_at(P, L, i, k) : L.f[(size_t)k * n + i];
-        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * C + bitrev(k, bits)]);
+        const int pos = R ? mr_pos(k, b, C / b, bits) : (int)bitrev(k, bits);
+        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * C + pos]);
         zz[l & 1] = v;
     }
     __syncthreads();
-    fft_stages<false>(z, L.ftw, C, bits, LP);
+    fft_stages<false>(z, L.ftw, C / b, bits, LP * b);
+    if (R) fft_combine<false, R ? R : 1>(z, L.ftw + (C / b) / 2, C, C / b, b, LP);
     for (int e = threadIdx.x; e < nl * (half + 1); e += blockDim.x) {
         const int l = e % nl, m = e / nl;
         const int i = ib + l;
         if (i >= n) continue;
-        const double2 zm = z[(l >> 1) * C + m], zn = z[(l >> 1) * C + ((C - m) & (C - 1))];
+        const double2 zm = z[(l >> 1) * C + m], zn = z[(l >> 1) * C + (m == 0 ? 0 : C - m)];
         // A = (Z_m + conj Z_{C-m})/2, B = (Z_m - conj Z_{C-m})/(2i)
         double re, im;
         if ((l & 1) == 0) { re = 0.5 * (zm.x + zn.x); im = 0.5 * (zm.y - zn.y); }
@@ -616,7 +621,8 @@ __global__ void __launch_bounds__(256) k_mg_fft_fwd(KArgs a, int bits, int LP)
     }
 }
 
-__global__ void __launch_bounds__(256) k_mg_fft_inv(KArgs a, int bits, int LP)
+template <int R>
+__global__ void __launch_bounds__(256) k_mg_fft_inv(KArgs a, int bits, int b, int LP)
 {
     extern __shared__ double2 z[];
     const int p = blockIdx.y;
@@ -645,10 +651,12 @@ __global__ void __launch_bounds__(256) k_mg_fft_inv(KArgs a, int bits, int LP)
             }
             if (h2 == 0) { xr = c; xi = sg * sn; } else { yr = c; yi = sg * sn; }
         }
-        z[pr * C + bitrev(m, bits)] = make_double2(xr - yi, xi + yr);
+        const int pos = R ? mr_pos(m, b, C / b, bits) : (int)bitrev(m, bits);
+        z[pr * C + pos] = make_double2(xr - yi, xi + yr);
     }
     __syncthreads();
-    fft_stages<true>(z, L.ftw, C, bits, LP);
+    fft_stages<true>(z, L.ftw, C / b, bits, LP * b);
+    if (R) fft_combine<true, R ? R : 1>(z, L.ftw + (C / b) / 2, C, C / b, b, LP);
     const double inv = 1.0 / C;
     const bool store = !a.lev0 || a.cycles[p] == 0;
     for (int e = threadIdx.x; e < nl * C; e += blockDim.x) {
@@ -903,9 +911,9 @@ int mg_build(vv_run *r)
                     return inv ? 2.0 * v : v;
                 };
                 // (dense weights only where the level has no FFT: fewer than
-                // 8 columns or not a power of two; 2 c^2 words would be
-                // 537 MB at c = 4096)
-                if (c < 8 || (c & (c - 1)) != 0) {
+                // 8 columns, odd or above 8192; 2 c^2 words would be 537 MB
+                // at c = 4096)
+                if (!fft_cols(c)) {
                     for (int ml = 0; ml < c; ++ml)
                         for (int k = 0; k < c; ++k) L.tab.push_back(w(ml, k, false));
                     for (int q = 0; q < c; ++q)
@@ -959,13 +967,20 @@ int mg_build(vv_run *r)
     {   // FFT twiddles of the direct level (both problems have its columns)
         const int C = r->mg.prob[0].lev[nlev - 1].ncol;
         std::vector<double> tw;
-        if (C >= 8 && (C & (C - 1)) == 0) {
+        if (fft_cols(C)) {
+            // radix-2 twiddles of the n1 = 2^bits point sub-lines, then (C not
+            // a power of two) the C twiddles of the mixed-radix combination
             const double pi = std::acos(-1.0);
-            tw.resize(C);
-            for (int t = 0; t < C / 2; ++t) {
-                tw[2 * t] = std::cos(2.0 * pi * t / C);
-                tw[2 * t + 1] = -std::sin(2.0 * pi * t / C);
+            const int n1 = C & -C;
+            for (int t = 0; t < n1 / 2; ++t) {
+                tw.push_back(std::cos(2.0 * pi * t / n1));
+                tw.push_back(-std::sin(2.0 * pi * t / n1));
             }
+            if (n1 != C)
+                for (int t = 0; t < C; ++t) {
+                    tw.push_back(std::cos(2.0 * pi * t / C));
+                    tw.push_back(-std::sin(2.0 * pi * t / C));
+                }
             VV_HIP(hipMalloc(&r->mg_ftw, tw.size() * sizeof(double)));
             VV_HIP(hipMemcpyAsync(r->mg_ftw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice,
                                   r->stream));
@@ -1079,7 +1094,7 @@ static int enqueue_cycle(vv_run *r)
         }
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, (C + TM - 1) / TM);
-        // FFTs where the level has twiddles (C >= 8, a power of two), the
+        // FFTs where the level has twiddles (even C, 8 <= C <= 8192), the
         // dense VALU transforms otherwise.  LP line pairs per workgroup, few
         // enough for about 256 workgroups (measured against LP = 2048/C:
         // 8243 -> 10027 steps/s at 129 x 65, 5386 -> 5991 at 513 x 257;
@@ -1087,13 +1102,19 @@ static int enqueue_cycle(vv_run *r)
         // matrix-core GEMMs lose to both, their k loop being latency-bound)
         const bool fft = B.prob[0].lev[L].ftw != nullptr;
         const int fbits = fft ? __builtin_ctz((unsigned)C) : 0;
+        const int fodd = C >> fbits;              // odd factor of C (1: radix 2 only)
         const int LP = std::max(1, std::min(fft_pairs(C), n / 256));
         const dim3 fg((unsigned)((n + 2 * LP - 1) / (2 * LP)), 2);
         const size_t flds = (size_t)LP * C * sizeof(double2);
+        // registers of the mixed-radix combination: LP C <= R 256
+        const int fR = (fodd == 1) ? 0 : (LP * C <= 8 * 256) ? 8 : 32;
+        if (fft && LP * C > 32 * 256) { set_error("FFT line longer than 8192 points"); return VV_EINVAL; }
         prof_begin(r, &pe);
         // projection (read the level, write the mode lines) and line solves
         // (read and write the mode lines)
-        if (fft) hipLaunchKernelGGL(k_mg_fft_fwd, fg, dim3(256), flds, r->stream, a, fbits, LP);
+        if (fft && fR == 0) hipLaunchKernelGGL(k_mg_fft_fwd<0>, fg, dim3(256), flds, r->stream, a, fbits, 1, LP);
+        else if (fft && fR == 8) hipLaunchKernelGGL(k_mg_fft_fwd<8>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
+        else if (fft) hipLaunchKernelGGL(k_mg_fft_fwd<32>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
         else hipLaunchKernelGGL(k_mg_dft<0>, gd, dim3(256), 0, r->stream, a);
         if (two)
             hipLaunchKernelGGL(k_mg_modes<2>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
@@ -1101,7 +1122,9 @@ static int enqueue_cycle(vv_run *r)
             hipLaunchKernelGGL(k_mg_modes<1>, dim3(C / 2 + 1, 2), dim3(zt), 0, r->stream, a);
         prof_end(r, VV_K_DIRECT, pe, 32.0 * ntot * C);
         prof_begin(r, &pe);
-        if (fft) hipLaunchKernelGGL(k_mg_fft_inv, fg, dim3(256), flds, r->stream, a, fbits, LP);
+        if (fft && fR == 0) hipLaunchKernelGGL(k_mg_fft_inv<0>, fg, dim3(256), flds, r->stream, a, fbits, 1, LP);
+        else if (fft && fR == 8) hipLaunchKernelGGL(k_mg_fft_inv<8>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
+        else if (fft) hipLaunchKernelGGL(k_mg_fft_inv<32>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
         else hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }

@@ -133,8 +133,11 @@ double vv_run_footprint(const vv_config *cfg)
     for (int c = cfg->np;; c /= 2) {
         const double tab = 2.0 + 23.0 * n;
         if (c <= dc || c < 2 || (c & 1)) {   // phi coarsening ends at an odd count
-            const bool fft = c >= 8 && (c & (c - 1)) == 0;   // no dense transform weights
-            mg += 3.0 * n * c + (c / 2 + 1) * tab + (fft ? (double)c : 2.0 * (double)c * c);
+            // FFT twiddles (no dense transform weights): 2^bits radix-2 words
+            // plus 2c for the mixed-radix combination
+            const bool fft = vv::fft_cols(c);
+            const double twd = (double)(c & -c) + ((c & -c) == c ? 0.0 : 2.0 * c);
+            mg += 3.0 * n * c + (c / 2 + 1) * tab + (fft ? twd : 2.0 * (double)c * c);
             break;
         }
         mg += 3.5 * n * c + tab;

@@ -58,9 +58,10 @@ enum vv_field {
 #define VV_GEOM_ANNULUS 1
 typedef struct vv_config {
     int    nr, np;        /* intervals across / along (both >= 8; any np:
-                             phi coarsening halves np while it is even, the
-                             direct level's transforms are FFTs for a power
-                             of two and dense transforms otherwise)       */
+                             phi coarsening halves np while it is even; the
+                             direct level's transforms are mixed-radix FFTs
+                             (radix 2 and the odd factor) for even column
+                             counts up to 8192, dense transforms otherwise) */
     double dt;
     double r0, rmax;      /* walls; dr = (rmax-r0)/nr                       */
     double width;         /* periodic length; dp = width/np (rb.f:132-133) */

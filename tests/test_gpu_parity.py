@@ -246,12 +246,16 @@ def test_poisson_direct_fft_1024(geom):
     box.close()
 
 
-@pytest.mark.parametrize("geom,nr,np_", [(0, 96, 336), (1, 96, 336), (0, 30, 45), (1, 40, 100)])
+@pytest.mark.parametrize("geom,nr,np_", [(0, 96, 336), (1, 96, 336), (0, 30, 45), (1, 40, 100),
+                                         (1, 240, 1792), (0, 24, 3072), (1, 16, 6144), (0, 20, 1200)])
 def test_poisson_default_direct_any_np(geom, nr, np_):
     """The default configuration at column counts that are not powers of
-    two: one direct solve of the whole plane with the dense transforms
-    (odd counts have no Nyquist line), against the oracle's exact solution,
-    and at rtol 1e-12 its refinement cycles."""
+    two: one direct solve of the whole plane, against the oracle's exact
+    solution, and at rtol 1e-12 its refinement cycles.  Even counts use the
+    mixed-radix FFT (336 = 21 * 16, 100 = 25 * 4, 1792 = 7 * 256 as in the
+    annulus deck input1e81e5pr1-1.s, 1200 = 75 * 16; 3072 and 6144 take
+    the 32-register combination), the odd 45 the dense transforms (no
+    Nyquist line)."""
     if geom:
         box = _annulus(nr, np_, rtol=1e-12, max_cycles=60, Re=100.0)
         dr, dp = 0.5 / nr, 2.0 * np.arccos(-1.0) / np_
