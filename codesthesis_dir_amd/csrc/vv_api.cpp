@@ -191,7 +191,9 @@ bool positive(const std::vector<double> &v, size_t from)
 int upload_tab(vv_run *r, double **dst, const std::vector<double> &h)
 {
     VV_HIP(hipMalloc(dst, h.size() * sizeof(double)));
-    VV_HIP(hipMemcpyAsync(*dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    // synchronous: the caller's vector is freed on return
+    VV_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    (void)r;
     return VV_OK;
 }
 

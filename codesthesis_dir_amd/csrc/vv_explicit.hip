@@ -11,6 +11,7 @@
 #include "vv_internal.h"
 #include "vv_reduce.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace vv {
@@ -731,12 +732,15 @@ static int compact_launch(vv_run *r, int mask)
     t.Tp = budget / (r->np + 1);
     if (t.Tp > 64) t.Tp = 64;
     if (t.Tp < 1) t.Tp = 1;
-    t.rwords = budget;
+    // a workgroup holds at least one whole line: Tp = 1 needs np+1 words,
+    // Tr = 1 nP words, either of which can exceed the budget
+    t.rwords = std::max(budget, std::max(t.Tr * nP, t.Tp * (r->np + 1)));
     // + the elimination tables: 4n words (r families, n <= nr), 6(np+2)+3 (phi)
     const int twords = (4 * r->nr > 6 * (r->np + 2) + 3) ? 4 * r->nr : 6 * (r->np + 2) + 3;
     // dynamic LDS is kept within 64 KB; larger tables stay in global memory
-    t.tab_lds = ((size_t)(budget + twords) * 8 <= 65536) ? 1 : 0;
-    const size_t lds = (size_t)(budget + (t.tab_lds ? twords : 0)) * 8;
+    t.tab_lds = ((size_t)(t.rwords + twords) * 8 <= 65536) ? 1 : 0;
+    const size_t lds = (size_t)(t.rwords + (t.tab_lds ? twords : 0)) * 8;
+    if (lds > 160 * 1024) { set_error("compact-scheme line longer than the LDS of a CU holds"); return VV_EINVAL; }
     const int gr = nblk(r->np + 1, t.Tr), gp = nblk(r->nr, t.Tp);
     dim3 grid(gr > gp ? gr : gp, 6);
     hipEvent_t pe_;

@@ -57,7 +57,8 @@ def _random_state(run, seed):
 
 @pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False), (1100, 32, False),
-                                           (40, 64, False), (96, 336, False), (20, 45, True)])
+                                           (40, 64, False), (96, 336, False), (20, 45, True),
+                                           (24, 1024, False), (12, 2048, True), (10, 4096, False)])
 def test_explicit_step_bitexact(nr, np_, first, geom):
     import ctypes
     run, box = _pair(geom, nr, np_)
