@@ -130,20 +130,11 @@ using namespace vv;
 
 namespace {
 
-struct Pending {
-    bool c = false, t = false;
-};
+using Pending = vv_run::Pend;
 
 // pending Robert filter: between the explicit step and update the filtered
 // current level lives in Cf/thf
-Pending &pending(vv_run *r)
-{
-    static thread_local std::vector<std::pair<vv_run *, Pending>> tab;
-    for (auto &e : tab)
-        if (e.first == r) return e.second;
-    tab.emplace_back(r, Pending{});
-    return tab.back().second;
-}
+Pending &pending(vv_run *r) { return r->pend; }
 
 int alloc_field(vv_run *r, double **p)
 {
@@ -367,19 +358,22 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
 int vv_destroy(vv_run *r)
 {
     if (!r) return VV_OK;
-    (void)hipSetDevice(r->device);
-    if (r->stream) (void)hipStreamSynchronize(r->stream);
-    double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
-                        r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
-                        r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                        r->tab_p, r->mg_pool, r->mg_ftw, r->diag, r->part, r->kepart};
-    for (double *f : fields)
-        if (f) (void)hipFree(f);
-    if (r->mg_state) (void)hipFree(r->mg_state);
-    for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
-    if (r->ev0) (void)hipEventDestroy(r->ev0);
-    if (r->ev1) (void)hipEventDestroy(r->ev1);
-    if (r->stream) (void)hipStreamDestroy(r->stream);
+    {   // the lock is released before the run is deleted
+        std::lock_guard<std::recursive_mutex> lk(r->mu);
+        (void)hipSetDevice(r->device);
+        if (r->stream) (void)hipStreamSynchronize(r->stream);
+        double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
+                            r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
+                            r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
+                            r->tab_p, r->mg_pool, r->mg_ftw, r->diag, r->part, r->kepart};
+        for (double *f : fields)
+            if (f) (void)hipFree(f);
+        if (r->mg_state) (void)hipFree(r->mg_state);
+        for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
+        if (r->ev0) (void)hipEventDestroy(r->ev0);
+        if (r->ev1) (void)hipEventDestroy(r->ev1);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+    }
     delete r;
     return VV_OK;
 }
@@ -387,6 +381,7 @@ int vv_destroy(vv_run *r)
 int vv_put(vv_run *r, int field, const double *host, int ld)
 {
     if (!r || !host || ld < r->nr + 2) { set_error("vv_put: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     double *d = field_ptr(r, field);
     if (!d) { set_error("vv_put: unknown field"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
@@ -400,6 +395,7 @@ int vv_put(vv_run *r, int field, const double *host, int ld)
 int vv_get(vv_run *r, int field, double *host, int ld)
 {
     if (!r || !host || ld < r->nr + 2) { set_error("vv_get: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     double *d = field_ptr(r, field);
     if (!d) { set_error("vv_get: unknown field"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
@@ -413,6 +409,7 @@ int vv_get(vv_run *r, int field, double *host, int ld)
 int vv_start(vv_run *r)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     VV_HIP(hipMemcpyAsync(r->Cp, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
     VV_HIP(hipMemcpyAsync(r->thp, r->th, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
@@ -428,6 +425,7 @@ int vv_start(vv_run *r)
 int vv_cvorticity(vv_run *r, int first)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_compact_mask(r, 1 | 2 | 8 | 16);
     if (!st) st = launch_df(r, first ? 1 : 0, 1, 0);
@@ -441,6 +439,7 @@ int vv_cvorticity(vv_run *r, int first)
 int vv_temperature(vv_run *r, int first)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_compact_mask(r, 4 | 32);
     if (!st) st = launch_df(r, first ? 1 : 0, 0, 1);
@@ -454,6 +453,7 @@ int vv_temperature(vv_run *r, int first)
 int vv_heat(vv_run *r, double *trans1, double *trans2)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     const double *th = pending(r).t ? r->thf : r->th;
     int st = launch_heat_from(r, th, r->diag);
@@ -469,6 +469,7 @@ int vv_heat(vv_run *r, double *trans1, double *trans2)
 int vv_energy(vv_run *r, double *ke)
 {
     if (!r || !ke) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     const int st = vv::launch_energy(r);
     if (st) return st;
@@ -480,6 +481,7 @@ int vv_energy(vv_run *r, double *ke)
 int vv_rhs(vv_run *r)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_rhs_guess(r, 3, 5);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -489,6 +491,7 @@ int vv_rhs(vv_run *r)
 int vv_poisson(vv_run *r, int which, int *cycles, double *relres)
 {
     if (!r || (which != 2 && which != 3 && which != 5)) { set_error("vv_poisson: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     const int w = (which == 5) ? 3 : (which == 2 ? 1 : 2);
     int st = launch_rhs_fields(r, w, which);
@@ -512,6 +515,7 @@ int vv_poisson_host(vv_run *r, int which, double *psi, double *psip, const doubl
         set_error("vv_poisson_host: bad argument");
         return VV_EINVAL;
     }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     int st;
     const int fpsi = (which == 2) ? VV_VEL : VV_UEL;
     const int fpsip = (which == 2) ? VV_VELP : VV_UELP;
@@ -549,6 +553,7 @@ static int read_update_partials(vv_run *r, double cfl[2], int *nclamp)
 int vv_update(vv_run *r, int *nclamp)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     Pending &pd = pending(r);
     if (!pd.c) VV_HIP(hipMemcpyAsync(r->Cf, r->C, r->fbytes, hipMemcpyDeviceToDevice, r->stream));
@@ -568,6 +573,7 @@ int vv_update(vv_run *r, int *nclamp)
 int vv_step(vv_run *r, int nsteps, vv_step_info *info)
 {
     if (!r || nsteps < 0) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     pending(r) = Pending{};
     int status = VV_OK;
@@ -656,6 +662,7 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
 int vv_heat_history(vv_run *r, double *trans, int n)
 {
     if (!r || !trans || n < 0) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     long long avail = r->steps < r->diag_cap ? r->steps : r->diag_cap;
     long long cnt = n < avail ? n : avail;
@@ -670,6 +677,7 @@ int vv_heat_history(vv_run *r, double *trans, int n)
 int vv_profile(vv_run *r, int enable)
 {
     if (!r) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     VV_HIP(hipStreamSynchronize(r->stream));
     int st = prof_resolve(r);
@@ -684,6 +692,7 @@ int vv_profile(vv_run *r, int enable)
 int vv_profile_read(vv_run *r, int kernel, long long *launches, double *ms, double *bytes)
 {
     if (!r || kernel < 0 || kernel >= VV_K_COUNT) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     VV_HIP(hipStreamSynchronize(r->stream));
     int st = prof_resolve(r);
@@ -697,6 +706,7 @@ int vv_profile_read(vv_run *r, int kernel, long long *launches, double *ms, doub
 int vv_last_step_ms(vv_run *r, double *ms)
 {
     if (!r || !ms) return VV_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     *ms = r->last_ms;
     return VV_OK;
 }

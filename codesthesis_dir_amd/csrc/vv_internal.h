@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -116,6 +117,13 @@ struct MgBatchDev {
 
 // ---------------------------------------------------------------- vv_run
 struct vv_run {
+    // every C entry point on a run holds this lock, so calls on one run from
+    // several threads are serialised (the reference's main program calls
+    // rhsv/poisson2 and rhsu/poisson3 in two OpenMP sections, rb.f:370-380)
+    std::recursive_mutex mu;
+    // Robert-filtered current levels waiting in Cf/thf between the explicit
+    // step (vv_cvorticity / vv_temperature) and vv_update
+    struct Pend { bool c = false, t = false; } pend;
     vv_config cfg;
     int device = 0;
     hipStream_t stream = nullptr;

@@ -18,8 +18,11 @@
  *   - All arithmetic is IEEE binary64.
  *   - Every function returns VV_OK (0) or a nonzero status; no C++
  *     exception crosses the boundary.  vv_last_error() returns a message.
- *   - A vv_run owns device storage for one simulation; calls on one run are
- *     not thread-safe, calls on different runs are.
+ *   - A vv_run owns device storage for one simulation.  Every call on a run
+ *     holds the run's lock, so calls on one run from several threads are
+ *     serialised (the reference's main program calls rhsv/poisson2 and
+ *     rhsu/poisson3 in two OpenMP sections, rb.f:370-380, which then reach
+ *     one run from two threads); calls on different runs run concurrently.
  */
 #ifndef VVMI_H
 #define VVMI_H

@@ -416,6 +416,47 @@ def test_annulus_against_reference_fixture(golden_dir, name, snap):
     box.close()
 
 
+def test_concurrent_calls_on_one_run():
+    """poisson2 and poisson3 on one run from two threads at once, as the
+    reference's main program calls them from two OpenMP sections
+    (rb.f:370-380): the run's lock serialises the calls, so every result is
+    bit-identical to the sequential one.  ctypes releases the interpreter
+    lock during the calls, so they do overlap."""
+    import threading
+    nr, np_ = 64, 128
+    box = _box(nr, np_, 3.0, rtol=1e-12, max_cycles=60)
+    rng = np.random.default_rng(5)
+    rhs = {w: np.ascontiguousarray(rng.standard_normal((np_ + 1, nr + 2))) for w in (2, 3)}
+    for w in (2, 3):
+        rhs[w][0] = rhs[w][np_]
+    psi0 = {w: np.ascontiguousarray(1e-3 * rng.standard_normal((np_ + 1, nr + 2))) for w in (2, 3)}
+
+    def solve(w, out):
+        psi, psip = psi0[w].copy(), 0.5 * psi0[w]
+        out[w] = (box.poisson_host(w, psi, psip, rhs[w].copy()), psi, psip)
+
+    ref = {}
+    for w in (2, 3):
+        solve(w, ref)
+    for rep in range(20):
+        got = {}
+        bar = threading.Barrier(2)
+
+        def run(w):
+            bar.wait()
+            solve(w, got)
+        th = [threading.Thread(target=run, args=(w,)) for w in (2, 3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for w in (2, 3):
+            assert got[w][0][0] == 0, (rep, w, got[w][0])
+            assert np.array_equal(got[w][1], ref[w][1]), (rep, w)
+            assert np.array_equal(got[w][2], ref[w][2]), (rep, w)
+    box.close()
+
+
 def test_nonfinite_rhs_is_an_error():
     from codesthesis_dir_amd import _lib as L
     nr, np_ = 16, 32

@@ -191,3 +191,33 @@ def test_periodic_box_main_program_on_its_own_deck():
     for n in ("C", "U", "V"):
         assert errs[n] < 1e-5, (n, errs[n])
     assert em < 1e-5
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "rb_main_gpu_omp")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_periodic_box_main_program_with_openmp_sections():
+    """The same main program built with OpenMP and run on 2 threads: its
+    sections call rhsv/poisson2 and rhsu/poisson3 at once (rb.f:370-380),
+    which reach one run of the drop-in from two threads.  The results match
+    the reference executable (one thread) as closely as the serial build's."""
+    nts = 200
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "32", "64", str(nts), "0.5 1.0 1.0",
+                      "1e5", "0.0", "0.0", "1.0", "0.0", "0.0", "1", "no", "n"]) + "\n"
+    env2 = dict(os.environ, OMP_NUM_THREADS="2")
+    with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
+        res = subprocess.run([os.path.join(REF, "rb_omp")], input=deck, text=True, capture_output=True, cwd=dr,
+                             timeout=300, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+        print(f"[gpu] rb_main_gpu_omp in {dg}, 2 threads", flush=True)
+        out = subprocess.run([os.path.join(REF, "rb_main_gpu_omp")], input=deck, text=True, capture_output=True,
+                             cwd=dg, timeout=300, env=env2)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+        hr, hg = _series(res.stdout, 2), _series(out.stdout, 2)
+        assert hr.shape == hg.shape == (nts // 50, 2)
+        errs = {n: _rel(_field(os.path.join(dg, n), 0), _field(os.path.join(dr, n), 0)) for n in ("T", "C", "U", "V")}
+    eh = _rel(hg, hr)
+    print(f"rb main, OpenMP sections: heat series rel {eh:.3e}; fields {errs}")
+    assert eh < 1e-9
+    assert errs["T"] < 1e-9
+    for n in ("C", "U", "V"):
+        assert errs[n] < 1e-5, (n, errs[n])
