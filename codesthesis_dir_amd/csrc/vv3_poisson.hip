@@ -30,8 +30,10 @@ namespace vv3 {
 namespace {
 
 using vv::bitrev;
+using vv::fft_combine;
 using vv::fft_pairs;
 using vv::fft_stages;
+using vv::mr_pos;
 using vv::gemm_f64;
 
 // 1. spectra S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
@@ -93,7 +95,10 @@ struct PhiInv {
 };
 
 // ---------------------------------------------------------------- phi FFT
-// For np a power of two the phi transforms are radix-2 FFTs in LDS.  Two real
+// For even np (8 <= np <= 4096) the phi transforms are FFTs in LDS: radix 2
+// for a power of two (R = 0), otherwise np = b 2^bits with b odd, b radix-2
+// sub-lines per line and the in-place combination of fft_combine (R outputs
+// per thread).  Two real
 // lines (consecutive i at one j) share one complex transform, z = a + i b;
 // Z_m and Z_{np-m} separate them afterwards.  A workgroup holds LP such pairs
 // (LP np complex values, at most 64 KB); k' = k mod np indexes the line, so
@@ -102,8 +107,9 @@ struct PhiInv {
 
 // forward: S[m][u] = sum_k rhs(i,j,k) cos(2 pi m k/np) (m = 0..np/2) and
 // S[np/2+m][u] = sum_k rhs(i,j,k) sin(...) (0 < m < np/2), u = (j-1) i1 + (i-1)
+template <int R>
 __global__ void __launch_bounds__(256) k3_fft_fwd(const double *rhs, double *S, const double2 *tw, int np, int bits,
-                                                   int LP, int i1, int j1, int ldi, int ldj, int mt)
+                                                   int b, int LP, int i1, int j1, int ldi, int ldj, int mt)
 {
     extern __shared__ double2 z[];
     const int j = blockIdx.y, ib = blockIdx.x * 2 * LP, nl = 2 * LP;
@@ -111,17 +117,19 @@ __global__ void __launch_bounds__(256) k3_fft_fwd(const double *rhs, double *S, 
         const int l = e % nl, kq = e / nl;
         const int i = ib + l, k = (kq == 0) ? np : kq;
         const double v = (i < i1) ? rhs[((size_t)k * ldj + (size_t)(j + 1)) * ldi + (size_t)(i + 1)] : 0.0;
-        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * np + bitrev(kq, bits)]);
+        const int pos = R ? mr_pos(kq, b, np / b, bits) : (int)bitrev(kq, bits);
+        double *zz = reinterpret_cast<double *>(&z[(l >> 1) * np + pos]);
         zz[l & 1] = v;
     }
     __syncthreads();
-    fft_stages<false>(z, tw, np, bits, LP);
+    fft_stages<false>(z, tw, np / b, bits, LP * b);
+    if (R) fft_combine<false, R ? R : 1>(z, tw + (np / b) / 2, np, np / b, b, LP);
     const int half = np >> 1;
     for (int e = threadIdx.x; e < nl * (half + 1); e += blockDim.x) {
         const int l = e % nl, m = e / nl;
         const int i = ib + l;
         if (i >= i1) continue;
-        const double2 zm = z[(l >> 1) * np + m], zn = z[(l >> 1) * np + ((np - m) & (np - 1))];
+        const double2 zm = z[(l >> 1) * np + m], zn = z[(l >> 1) * np + (m == 0 ? 0 : np - m)];
         // A = (Z_m + conj Z_{np-m})/2, B = (Z_m - conj Z_{np-m})/(2i)
         double re, im;
         if ((l & 1) == 0) { re = 0.5 * (zm.x + zn.x); im = 0.5 * (zm.y - zn.y); }
@@ -134,8 +142,9 @@ __global__ void __launch_bounds__(256) k3_fft_fwd(const double *rhs, double *S, 
 
 // inverse: psi(i,j,k) = (1/np) sum_m w_m (C_m cos + S_m sin), k = 1..np, and
 // k = 0 copied from k = np
+template <int R>
 __global__ void __launch_bounds__(256) k3_fft_inv(const double *S, double *psi, const double2 *tw, int np, int bits,
-                                                   int LP, int i1, int j1, int ldi, int ldj, int mt)
+                                                   int b, int LP, int i1, int j1, int ldi, int ldj, int mt)
 {
     extern __shared__ double2 z[];
     const int j = blockIdx.y, ib = blockIdx.x * 2 * LP, nl = 2 * LP, half = np >> 1;
@@ -153,10 +162,12 @@ __global__ void __launch_bounds__(256) k3_fft_inv(const double *S, double *psi, 
             const double sn = (mm > 0 && mm < half) ? S[(size_t)(half + mm) * mt + u] : 0.0;
             if (h2 == 0) { xr = c; xi = sg * sn; } else { yr = c; yi = sg * sn; }
         }
-        z[pr * np + bitrev(m, bits)] = make_double2(xr - yi, xi + yr);
+        const int pos = R ? mr_pos(m, b, np / b, bits) : (int)bitrev(m, bits);
+        z[pr * np + pos] = make_double2(xr - yi, xi + yr);
     }
     __syncthreads();
-    fft_stages<true>(z, tw, np, bits, LP);
+    fft_stages<true>(z, tw, np / b, bits, LP * b);
+    if (R) fft_combine<true, R ? R : 1>(z, tw + (np / b) / 2, np, np / b, b, LP);
     const double inv = 1.0 / np;
     for (int e = threadIdx.x; e < nl * np; e += blockDim.x) {
         const int l = e % nl, kq = e / nl;
@@ -560,13 +571,22 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
     }
     VV_HIP(upload(&P.W, W));
     VV_HIP(upload(&P.Wi, Wi));
-    P.fft = (np >= 8 && np <= 4096 && (np & (np - 1)) == 0);
+    P.fft = (np >= 8 && np <= 4096 && np % 2 == 0);
     if (P.fft) {
-        std::vector<double> tw(np);   // (cos, -sin)(2 pi t/np), t = 0..np/2-1
-        for (int t = 0; t < np / 2; ++t) {
-            tw[2 * t] = std::cos(2.0 * pi * t / np);
-            tw[2 * t + 1] = -std::sin(2.0 * pi * t / np);
+        // (cos, -sin)(2 pi t/n1), t < n1/2, for the n1 = 2^bits point
+        // radix-2 sub-lines, then (np not a power of two) (cos, -sin)(2 pi t/np),
+        // t < np, for the mixed-radix combination
+        const int n1 = np & -np;
+        std::vector<double> tw;
+        for (int t = 0; t < n1 / 2; ++t) {
+            tw.push_back(std::cos(2.0 * pi * t / n1));
+            tw.push_back(-std::sin(2.0 * pi * t / n1));
         }
+        if (n1 != np)
+            for (int t = 0; t < np; ++t) {
+                tw.push_back(std::cos(2.0 * pi * t / np));
+                tw.push_back(-std::sin(2.0 * pi * t / np));
+            }
         VV_HIP(upload(&P.tw, tw));
     }
     VV_HIP(upload(&P.lphi, lphi));
@@ -625,21 +645,40 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
     const int nm = P.nm, np = P.np, i1 = O.i1, j1 = O.j1, mt = O.m;
     hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((g.ldi * g.ldj + 255) / 256), np + 1), dim3(256), 0, s, g, which,
                        i1, j1, psi, psip);
-    const int bits = P.fft ? __builtin_ctz((unsigned)np) : 0, LP = fft_pairs(np);
+    // LP line pairs per workgroup, few enough for about 256 workgroups
+    const int bits = P.fft ? __builtin_ctz((unsigned)np) : 0;
+    const int LP = std::max(1, std::min(fft_pairs(np), (i1 * j1) / 512));
+    const int fodd = np >> bits;                      // odd factor of np (1: radix 2 only)
+    // registers of the mixed-radix combination: LP np <= R 256
+    const int fR = (fodd == 1) ? 0 : (LP * np <= 8 * 256) ? 8 : 32;
+    if (P.fft && LP * np > 32 * 256) { vv::set_error("phi FFT line longer than 8192 points"); return VV_EINVAL; }
     const dim3 fgrid((unsigned)((i1 + 2 * LP - 1) / (2 * LP)), (unsigned)j1);
     const size_t flds = (size_t)LP * np * sizeof(double2);
-    if (P.fft)
-        hipLaunchKernelGGL(k3_fft_fwd, fgrid, dim3(256), flds, s, rhs, P.S, (const double2 *)P.tw, np, bits, LP, i1,
-                           j1, g.ldi, g.ldj, mt);
+    const double2 *tw = (const double2 *)P.tw;
+    if (P.fft && fR == 0)
+        hipLaunchKernelGGL(k3_fft_fwd<0>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, 1, LP, i1, j1, g.ldi,
+                           g.ldj, mt);
+    else if (P.fft && fR == 8)
+        hipLaunchKernelGGL(k3_fft_fwd<8>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, fodd, LP, i1, j1, g.ldi,
+                           g.ldj, mt);
+    else if (P.fft)
+        hipLaunchKernelGGL(k3_fft_fwd<32>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, fodd, LP, i1, j1,
+                           g.ldi, g.ldj, mt);
     else
         gemm_f64<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
     gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
     hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
     gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
-    if (P.fft)
-        hipLaunchKernelGGL(k3_fft_inv, fgrid, dim3(256), flds, s, (const double *)P.S, psi, (const double2 *)P.tw, np,
-                           bits, LP, i1, j1, g.ldi, g.ldj, mt);
+    if (P.fft && fR == 0)
+        hipLaunchKernelGGL(k3_fft_inv<0>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, 1, LP, i1,
+                           j1, g.ldi, g.ldj, mt);
+    else if (P.fft && fR == 8)
+        hipLaunchKernelGGL(k3_fft_inv<8>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, fodd, LP,
+                           i1, j1, g.ldi, g.ldj, mt);
+    else if (P.fft)
+        hipLaunchKernelGGL(k3_fft_inv<32>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, fodd,
+                           LP, i1, j1, g.ldi, g.ldj, mt);
     else
         gemm_f64<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
     VV_HIP(hipGetLastError());

@@ -137,13 +137,13 @@ def test_explicit_routines_match_oracle(seed):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(NR, NZ, NP), (8, 10, 6), (12, 9, 16), (7, 6, 8)])
+@pytest.mark.parametrize("shape", [(NR, NZ, NP), (8, 10, 6), (12, 9, 16), (7, 6, 8), (8, 10, 12), (7, 6, 20)])
 def test_poisson_matches_exact_solve(which, shape):
     """poisson1/2/3 against the oracle's exact solve of the reference's
     residual operator (pinned in the oracle's tests), with the reference's
     output conventions (psip, walls, untouched points) bit for bit.  np = 16
-    and 8 take the FFT path of the phi transforms, np = 24 and 6 the dense
-    one."""
+    and 8 take the radix-2 FFT path of the phi transforms, np = 24, 12 and
+    20 the mixed-radix one (3 * 8, 3 * 4, 5 * 4), np = 6 the dense one."""
     nr, nz, np_ = shape
     run = _gpu(nr, nz, np_)
     ora = O3.Run3(nr, nz, np_, **PARAMS)
