@@ -580,9 +580,9 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
     int cyc[2] = {0, 0};
     double rel[2] = {0, 0};
     // Deferred solves: with the whole plane direct, once a synchronised
-    // solve has converged in one cycle, later steps queue that cycle and a
-    // device-side check (k_mg_check) without waiting for the host; the
-    // check's sticky word is read once at the end of the call.  Profiled
+    // solve has converged in one cycle, later steps queue that cycle with a
+    // device-side check in its stop test (k_mg_stop) without waiting for the
+    // host; the check's sticky word is read once at the end of the call.  Profiled
     // runs and the V-cycle synchronise every solve.
     const bool direct_all = r->cfg.direct_cols >= r->np;
     bool any_deferred = false;

@@ -699,6 +699,7 @@ struct StopArgs {
     const double *npart, *fpart;
     int nbn, nbr, maxc;
     double rtol;
+    int *check;    // deferred solves: the sticky word (else null)
 };
 
 // Stop test of one cycle (one workgroup): combines the residual partials of
@@ -740,6 +741,17 @@ __global__ void __launch_bounds__(256) k_mg_stop(StopArgs a)
         a.cycles[p] = c;
         a.relres[p] = rel;
         if (!(rel > a.rtol) || c >= a.maxc) a.done[p] = 1;
+    }
+    // deferred solves: a problem whose relative residual is above rtol (or
+    // NaN; bit 0) or whose right-hand side norm is not finite (bit 1) sets
+    // the sticky word, which the host reads once per vv_step call
+    if (a.check) {
+        int s = *a.check;
+        for (int p = 0; p < 2; ++p) {
+            if (!(a.relres[p] <= a.rtol)) s |= 1;
+            if (!isfinite(__longlong_as_double((long long)a.fnorm[p]))) s |= 2;
+        }
+        *a.check = s;
     }
 }
 
@@ -1048,7 +1060,7 @@ static KArgs kargs(const vv_run *r, int lev, int mode, const int *cur)
 // prolongation of a correction into the first post-sweep of the finer level
 // (ZB_PROLONG).  nu1 + nu2 is even, so every level is back in buffer 0 at
 // the end of the cycle and all cycles enqueue the same launches.
-static int enqueue_cycle(vv_run *r)
+static int enqueue_cycle(vv_run *r, int *check = nullptr)
 {
     vv::MgBatchDev &B = r->mg;
     const int L = B.prob[0].nlev - 1;
@@ -1139,6 +1151,7 @@ static int enqueue_cycle(vv_run *r)
     sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres;
     sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
+    sa.check = check;
     prof_begin(r, &pe);
     hipLaunchKernelGGL(k_mg_norm, dim3(nb, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), B.npart);
     hipLaunchKernelGGL(k_mg_stop, dim3(1), dim3(256), 0, r->stream, sa);
@@ -1147,30 +1160,11 @@ static int enqueue_cycle(vv_run *r)
     return VV_OK;
 }
 
-// After a queued cycle: a problem whose relative residual is above rtol
-// (or NaN; bit 0) or whose right-hand side norm is not finite (bit 1) sets
-// the sticky word flags[4], which the host reads once per vv_step call.
-__global__ void k_mg_check(int *flags, const unsigned long long *fnorm, const double *relres, double rtol)
-{
-    if (threadIdx.x != 0) return;
-    int s = flags[4];
-    for (int p = 0; p < 2; ++p) {
-        if (!(relres[p] <= rtol)) s |= 1;
-        if (!isfinite(__longlong_as_double((long long)fnorm[p]))) s |= 2;
-    }
-    flags[4] = s;
-}
-
-// One cycle of both problems and the sticky check, with no host
-// synchronisation (the time loop's deferred solves)
+// One cycle of both problems with the sticky check in its stop test, with
+// no host synchronisation (the time loop's deferred solves)
 int mg_enqueue_deferred(vv_run *r)
 {
-    int st = enqueue_cycle(r);
-    if (st) return st;
-    hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, r->stream, r->mg_flags,
-                       (const unsigned long long *)r->mg.fnorm, (const double *)r->mg.relres, r->cfg.rtol);
-    VV_HIP(hipGetLastError());
-    return VV_OK;
+    return enqueue_cycle(r, r->mg_flags + 4);
 }
 
 // Runs cycles until both active problems are done.  Cycles are queued in
