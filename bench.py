@@ -494,8 +494,8 @@ def main():
                     help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; the multi-GPU layout "
-              "is described in DESIGN.md and not implemented", file=sys.stderr)
+        print(f"bench.py: --gpus {a.gpus}: this build runs on one GPU; runs on several GPUs "
+              "(independent runs or requests sharded across devices) are not implemented", file=sys.stderr)
         sys.exit(2)
     if a.rot3d_only:
         print(json.dumps(rot3d_leg(cpu=not a.no_cpu and not a.rot3d_grid, only=a.rot3d_grid)))
