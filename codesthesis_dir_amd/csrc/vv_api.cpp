@@ -275,6 +275,12 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
         set_error("need nr >= 8 and np >= 8");
         return VV_EINVAL;
     }
+    // a workgroup of the compact scheme holds one whole line in LDS, and the
+    // line solves of the Poisson levels hold lines of nr rows
+    if (cfg->np > 16383 || cfg->nr > 2048) {
+        set_error("need np <= 16383 and nr <= 2048 (lines are held in LDS)");
+        return VV_EINVAL;
+    }
     if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->Pr > 0) ||
         (cfg->geom == VV_GEOM_BOX && !(cfg->width > 0)) ||
         (cfg->geom == VV_GEOM_ANNULUS && (!(cfg->r0 > 0) || cfg->Re == 0.0)) ||

@@ -60,7 +60,8 @@ enum vv_field {
 #define VV_GEOM_BOX     0
 #define VV_GEOM_ANNULUS 1
 typedef struct vv_config {
-    int    nr, np;        /* intervals across / along (both >= 8; any np:
+    int    nr, np;        /* intervals across / along (8 <= nr <= 2048,
+                             8 <= np <= 16383: lines are held in LDS; any np:
                              phi coarsening halves np while it is even; the
                              direct level's transforms are mixed-radix FFTs
                              (radix 2 and the odd factor) for even column

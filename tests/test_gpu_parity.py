@@ -493,6 +493,16 @@ def test_concurrent_calls_on_one_run():
     box.close()
 
 
+@pytest.mark.parametrize("nr,np_", [(16, 20000), (3000, 64), (4, 64), (16, 6)])
+def test_sizes_the_kernels_cannot_hold_are_rejected(nr, np_):
+    """vv_create refuses sizes whose lines do not fit in LDS (np above 16383,
+    nr above 2048) and sizes below 8, before any allocation."""
+    import codesthesis_dir_amd._lib as L
+    with pytest.raises(L.VVError) as e:
+        _box(nr, np_, 3.0)
+    assert e.value.code == L.VV_EINVAL
+
+
 def test_nonfinite_rhs_is_an_error():
     from codesthesis_dir_amd import _lib as L
     nr, np_ = 16, 32
