@@ -221,3 +221,35 @@ def test_periodic_box_main_program_with_openmp_sections():
     assert errs["T"] < 1e-9
     for n in ("C", "U", "V"):
         assert errs[n] < 1e-5, (n, errs[n])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "ann_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_annulus_main_program_on_its_own_deck(golden_dir):
+    """velvor2dnewsch3.4.f's main program on its shipped deck
+    input1e81e5pr1-1.s (nr 240, np 1792, Ra 1e8, Re 1e5, Pr 0.1), 50 steps
+    from the program's own start (the deck restarts from field files the
+    reference does not ship), through the annulus drop-in, against the
+    reference executable's output on the same deck
+    (ann_deck_240x1792.npz, written by make_golden_anndeck.py).  np = 1792 =
+    7 * 256 takes the mixed-radix FFT of the direct solve."""
+    import sys
+    sys.path.insert(0, golden_dir)
+    import make_golden_anndeck as M
+    g = np.load(os.path.join(golden_dir, "ann_deck_240x1792.npz"))
+    stride = int(g["stride"])
+    with tempfile.TemporaryDirectory() as d:
+        out = _run(os.path.join(REF, "ann_main_gpu"), M.DECK, d)
+        eh = _rel(_series(out, 2), g["heat"])
+        errs = {}
+        for f in ("T", "C", "U", "V"):
+            v = np.loadtxt(os.path.join(d, f))
+            assert v.size == int(g[f"n_{f}"]), f
+            errs[f] = float(np.max(np.abs(v[::stride] - g[f"s_{f}"])) / float(g[f"norm_{f}"]))
+        em = _rel(np.loadtxt(os.path.join(d, "zmtmtempvar.dat")), g["means"])
+    print(f"annulus main, input1e81e5pr1-1.s at 240x1792: heat {eh:.3e}; fields {errs}; running means {em:.3e}")
+    assert eh < 1e-9
+    assert errs["T"] < 1e-9
+    for n in ("C", "U", "V"):
+        assert errs[n] < 1e-4, (n, errs[n])
+    assert em < 1e-4
