@@ -92,14 +92,16 @@ def test_rot3dmg_main_program_on_the_gpu(grid):
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF, "r3h_main_gpu")),
                     reason="reference main programs not linked (reference sources were absent at build)")
-def test_rot3dmghor_main_program_on_the_gpu():
+@pytest.mark.parametrize("grid", [("16", "16", "32"), ("32", "32", "160")])
+def test_rot3dmghor_main_program_on_the_gpu(grid):
     """rot3dmghor.f's main program (the stationary horizontal annulus: its
     deck, initial condition, time loop with heat(trans1,trans2) and output
     files) through the drop-in built for it (libvvdropin3h.so) against the
-    reference executable.  The reference stops its solves at an absolute
+    reference executable, also at 32 x 32 x 160 (np = 5 * 32: the
+    mixed-radix FFT).  The reference stops its solves at an absolute
     residual of 1e-6 (rot3dmghor.f:1599)."""
     nts = 10
-    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "16", "16", "32", str(nts),
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", *grid, str(nts),
                       "0.5 1.0 0.5", "1e5", "100.0", "1e4", "1.0", "0.0", "0.1", "4", "no", "n"]) + "\n"
     with tempfile.TemporaryDirectory() as dr, tempfile.TemporaryDirectory() as dg:
         out_r = _run(os.path.join(REF, "r3h_ref"), deck, dr)
@@ -109,7 +111,7 @@ def test_rot3dmghor_main_program_on_the_gpu():
         eh = _rel(hg, hr)
         errs = {n: _rel(_field(os.path.join(dg, n), 2), _field(os.path.join(dr, n), 2))
                 for n in ("T", "A", "B", "C", "U", "V", "W")}
-    print(f"rot3dmghor main: heat series rel {eh:.3e}; fields {errs}")
+    print(f"rot3dmghor main {'x'.join(grid)}: heat series rel {eh:.3e}; fields {errs}")
     assert eh < 1e-9
     assert errs["T"] < 1e-9
     for n in ("A", "B", "C", "U", "V", "W"):
