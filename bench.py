@@ -347,7 +347,7 @@ def sweep_requests(n=64, seed=20261205, steps=2000):
     return out
 
 
-def serve_leg(n=64, steps=2000, slots=8, baseline=True):
+def serve_leg(n=64, steps=2000, slots=16, baseline=True):
     """The served sweep through the request loop (include/vvserve.h): the
     requests arrive on their schedule and are admitted by slot and device
     footprint; then the same requests one at a time (one slot, all queued
@@ -481,7 +481,7 @@ def main():
                     help="skip the V-cycle, large V-cycle, 3D and served-sweep measurements")
     ap.add_argument("--serve-only", action="store_true",
                     help="measure only the served mixed-size sweep and print its JSON object")
-    ap.add_argument("--slots", type=int, default=8, help="request-loop slots of the served sweep")
+    ap.add_argument("--slots", type=int, default=16, help="request-loop slots of the served sweep")
     ap.add_argument("--direct-cols", type=int, default=0,
                     help="vv_config.direct_cols of the main leg (0: the library default)")
     ap.add_argument("--rot3d-grid", default=None, choices=sorted(ROT3D),
