@@ -55,8 +55,8 @@ WORKLOADS = {
                     dict(geom=0, nr=512, np=256, width=1.0, dt=1e-6, Ra=1e5, Pr=1.0), 20261203, 1e-3),
     "box-97x337": ("periodic Rayleigh-Benard box (velvor2dperiodicrbconv.f) at the size and physics "
                    "of the reference's own input deck input.s: nr=96, np=336, Ra=1e7, Pr=0.71, "
-                   "dt=1e-7, the hard-coded width 3 (np is not a power of two: dense transforms "
-                   "over the 336 columns)",
+                   "dt=1e-7, the hard-coded width 3 (np is not a power of two: the direct solve's "
+                   "mixed-radix FFT, 336 = 21 x 16)",
                    dict(geom=0, nr=96, np=336, width=3.0, dt=1e-7, Ra=1e7, Pr=0.71), 20261204, 1e-3),
 }
 DEFAULT_WORKLOAD = "annulus-513x257"
