@@ -18,6 +18,9 @@
 #include <vector>
 
 struct vv3_run {
+    // every C entry point on a run holds this lock, so calls on one run from
+    // several threads are serialised
+    std::recursive_mutex mu;
     vv3_config cfg;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -844,16 +847,19 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
 int vv3_destroy(vv3_run *r)
 {
     if (!r) return VV_OK;
-    (void)hipSetDevice(r->device);
-    if (r->stream) (void)hipStreamSynchronize(r->stream);
-    for (int q = 0; q < VV3_NFIELD; ++q)
-        if (r->f[q]) (void)hipFree(r->f[q]);
-    poisson3_free(r->pois);
-    if (r->heat) (void)hipFree(r->heat);
-    if (r->trig) (void)hipFree(r->trig);
-    if (r->kepart) (void)hipFree(r->kepart);
-    if (r->dgbuf) (void)hipFree(r->dgbuf);
-    if (r->stream) (void)hipStreamDestroy(r->stream);
+    {   // the lock is released before the run is deleted
+        std::lock_guard<std::recursive_mutex> lk(r->mu);
+        (void)hipSetDevice(r->device);
+        if (r->stream) (void)hipStreamSynchronize(r->stream);
+        for (int q = 0; q < VV3_NFIELD; ++q)
+            if (r->f[q]) (void)hipFree(r->f[q]);
+        poisson3_free(r->pois);
+        if (r->heat) (void)hipFree(r->heat);
+        if (r->trig) (void)hipFree(r->trig);
+        if (r->kepart) (void)hipFree(r->kepart);
+        if (r->dgbuf) (void)hipFree(r->dgbuf);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+    }
     delete r;
     return VV_OK;
 }
@@ -894,12 +900,14 @@ static int copy3(vv3_run *r, int field, double *host, int ldi, int ldj, bool to_
 int vv3_put(vv3_run *r, int field, const double *host, int ldi, int ldj)
 {
     if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vv3_put: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     return copy3(r, field, const_cast<double *>(host), ldi, ldj, true);
 }
 
 int vv3_get(vv3_run *r, int field, double *host, int ldi, int ldj)
 {
     if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vv3_get: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     return copy3(r, field, host, ldi, ldj, false);
 }
 
@@ -1036,6 +1044,7 @@ static int launch_update(vv3_run *r)
 int vv3_vorticity(vv3_run *r, int which)
 {
     if (!r || which < 0 || which > 3) { vv::set_error("vv3_vorticity: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_explicit(r, which == 0 ? 7 : 1 << (which - 1));
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -1045,6 +1054,7 @@ int vv3_vorticity(vv3_run *r, int which)
 int vv3_temperature(vv3_run *r)
 {
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_explicit(r, 8);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -1054,6 +1064,7 @@ int vv3_temperature(vv3_run *r)
 int vv3_heat(vv3_run *r, double *trans1)
 {
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     const int st = launch_heat(r);
     if (st) return st;
@@ -1067,6 +1078,7 @@ int vv3_heat(vv3_run *r, double *trans1)
 int vv3_energy(vv3_run *r, double *ke)
 {
     if (!r || !ke) { vv::set_error("vv3_energy: null argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     const int nb = r->np < KE3_MAXB ? r->np : KE3_MAXB;
     const Geo g = geo(r);
@@ -1105,6 +1117,7 @@ static int diag_buffer(vv3_run *r)
 int vv3_local_nu(vv3_run *r, double *inner, double *outer)
 {
     if (!r || !inner || !outer) { vv::set_error("vv3_local_nu: null argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = diag_buffer(r);
     if (st) return st;
@@ -1125,6 +1138,7 @@ int vv3_modes(vv3_run *r, int nmax, double *c, double *s)
         vv::set_error("vv3_modes: need 1 <= nmax <= 16 and output arrays");
         return VV_EINVAL;
     }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = diag_buffer(r);
     if (st) return st;
@@ -1143,6 +1157,7 @@ int vv3_modes(vv3_run *r, int nmax, double *c, double *s)
 int vv3_fluxes(vv3_run *r, double *trans1, double *trans2)
 {
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     double t[2] = {0.0, 0.0};
     VV_HIP(hipMemcpyAsync(t, heat_total(r), sizeof(t), hipMemcpyDeviceToHost, r->stream));
@@ -1155,6 +1170,7 @@ int vv3_fluxes(vv3_run *r, double *trans1, double *trans2)
 int vv3_rhs(vv3_run *r, int which)
 {
     if (!r || which < 1 || which > 3) { vv::set_error("vv3_rhs: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_rhs(r, which);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -1164,6 +1180,7 @@ int vv3_rhs(vv3_run *r, int which)
 int vv3_poisson(vv3_run *r, int which)
 {
     if (!r || which < 1 || which > 3) { vv::set_error("vv3_poisson: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_poisson(r, which);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -1173,6 +1190,7 @@ int vv3_poisson(vv3_run *r, int which)
 int vv3_update(vv3_run *r)
 {
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     int st = launch_update(r);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
@@ -1182,6 +1200,7 @@ int vv3_update(vv3_run *r)
 int vv3_step(vv3_run *r, int n, double *trans1)
 {
     if (!r || n < 0) { vv::set_error("vv3_step: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
     for (int s = 0; s < n; ++s) {
         int st = launch_explicit(r, 15);

@@ -8,6 +8,9 @@
  * X(i,j,k) at host[(k*(nz+2) + j)*(nr+2) + i], i = 0..nr+1, j = 0..nz+1,
  * k = 0..np, as the reference's (0:kr,0:kz,0:kp) arrays with kr = nr+1,
  * kz = nz+1.  k = 0 and k = np are the same azimuth (both are kept).
+ *
+ * Every call on a run holds the run's lock, so calls on one run from
+ * several threads are serialised; calls on different runs run concurrently.
  */
 #ifndef VVMI3_H
 #define VVMI3_H
