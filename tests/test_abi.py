@@ -8,7 +8,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vvmi.h")
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h", "vvband.h")]
 LIB = os.path.join(ROOT, "codesthesis_dir_amd", "libvvmi.so")
 
 
