@@ -37,64 +37,54 @@ def _oracle(kind, rows, r):
     return out
 
 
-def _device(x, layout, torch):
-    """(nsys, n) device tensor with the given layout: 'rows' (system
-    contiguous), 'cols' (lines interleaved) or 'pad' (padded row pitch)."""
-    t = torch.from_numpy(x)
-    if layout == "rows":
-        return t.cuda()
-    if layout == "cols":
-        return t.t().contiguous().cuda().t()
-    big = torch.zeros((x.shape[0], x.shape[1] + 3), dtype=torch.float64)
-    big[:, :x.shape[1]] = t
-    return big.cuda()[:, :x.shape[1]]
-
-
 CASES = [(0, 1, 5), (0, 2, 33), (0, 63, 129), (0, 511, 300), (1, 3, 4), (1, 4, 9), (1, 129, 65),
          (1, 513, 257), (2, 2, 3), (2, 3, 17), (2, 64, 128), (2, 200, 41)]
+LAYOUTS = {"cols": dict(interleaved=True), "rows": dict(interleaved=False),
+           "pad": dict(interleaved=False, pad=3), "colpad": dict(interleaved=True, pad=5)}
 
 
-@pytest.mark.parametrize("layout", ["rows", "cols", "pad"])
+@pytest.mark.parametrize("layout", sorted(LAYOUTS))
 @pytest.mark.parametrize("kind,n,nsys", CASES)
 def test_band_solve_bitexact(kind, n, nsys, layout):
-    import torch
     from codesthesis_dir_amd import band
     print(f"[gpu] band kind={kind} n={n} nsys={nsys} layout={layout}", flush=True)
     rows, r = _systems(kind, n, nsys, seed=1000 * kind + n + nsys)
     want = _oracle(kind, rows, r)
-    dev = [_device(x, layout, torch) for x in rows]
-    rd = _device(r, layout, torch)
-    out = _device(np.zeros_like(r), layout, torch)
-    keep = [x.clone() for x in dev] + [rd.clone()]
     if kind == 2:
-        a, b, c, d, e = dev
-        band.solve(kind, a, b, c, rd, out, d=d, e=e)
+        a, b, c, d, e = rows
+        got = band.solve_host(kind, a, b, c, r, d=d, e=e, **LAYOUTS[layout])
     else:
-        a, b, c = dev
-        band.solve(kind, a, b, c, rd, out)
-    torch.cuda.synchronize()
-    got = out.cpu().numpy()
+        a, b, c = rows
+        got = band.solve_host(kind, a, b, c, r, **LAYOUTS[layout])
     assert np.array_equal(got, want), np.max(np.abs(got - want))
-    # the inputs are read only
-    for x, y in zip(dev + [rd], keep):
-        assert torch.equal(x, y)
 
 
-def test_band_solve_in_place_and_errors():
-    """x may equal r; bad sizes and kinds are refused with VV_EINVAL."""
-    import torch
+def test_band_solve_keeps_inputs_in_place_and_errors():
+    """The inputs are read only; x may equal r; bad sizes and kinds are
+    refused with VV_EINVAL."""
     from codesthesis_dir_amd import band
     from codesthesis_dir_amd._lib import VVError
-    print("[gpu] band in place + errors", flush=True)
-    rows, r = _systems(0, 40, 20, seed=7)
+    print("[gpu] band inputs kept, in place, errors", flush=True)
+    n, nsys = 40, 20
+    rows, r = _systems(0, n, nsys, seed=7)
     want = _oracle(0, rows, r)
-    a, b, c = (torch.from_numpy(x).cuda() for x in rows)
-    rd = torch.from_numpy(r).cuda()
-    band.solve(band.TDMA, a, b, c, rd, rd)
-    torch.cuda.synchronize()
-    assert np.array_equal(rd.cpu().numpy(), want)
-    small = torch.zeros((4, 2), dtype=torch.float64, device="cuda")
+    host = [np.ascontiguousarray(x.T) for x in rows + [r]]        # interleaved: [j, s]
+    dev = [band.DeviceArray.from_numpy(x) for x in host]
+    a, b, c, rd = dev
+    x = band.DeviceArray(n * nsys)
+    band.solve(band.TDMA, n, nsys, 1, nsys, a, b, c, rd, x)
+    band.hip().hipDeviceSynchronize()
+    assert np.array_equal(x.numpy().reshape(n, nsys).T, want)
+    for d, h in zip(dev, host):
+        assert np.array_equal(d.numpy(), h.ravel())
+    band.solve(band.TDMA, n, nsys, 1, nsys, a, b, c, rd, rd)
+    band.hip().hipDeviceSynchronize()
+    assert np.array_equal(rd.numpy().reshape(n, nsys).T, want)
     with pytest.raises(VVError):
-        band.solve(band.PTDMA, small, small, small, small, small)
+        band.solve(band.PTDMA, 2, nsys, 1, nsys, a, b, c, rd, x)
     with pytest.raises(VVError):
-        band.solve(7, small, small, small, small, small)
+        band.solve(7, n, nsys, 1, nsys, a, b, c, rd, x)
+    with pytest.raises(VVError):
+        band.solve(band.PDMA, n, nsys, 1, nsys, a, b, c, rd, x)      # d, e missing
+    for v in dev + [x]:
+        v.free()

@@ -1,0 +1,106 @@
+"""Config 4 on the GPU: the served mixed-size sweep.
+
+Twelve requests of the sweep itself (the benchmark's 64 seeded draws: sides
+65 ... 1025 x half height, Ra log-uniform in [1e4, 1e8], Pr in [0.7, 7], the
+periodic box of velvor2dperiodicrbconv.f or the rotating-annulus plane of
+velvor2dnewsch3.4.f with Re = sqrt(Ta)), one of each side and geometry plus
+the two largest Rayleigh numbers, each for its full 2000 steps at Poisson
+rtol 1e-10, are served eight at a time.
+
+* Each result equals the same run made alone through the plain API, bit for
+  bit (a run's state is private: fields, solver hierarchy and stream).
+* The three smallest of them (sides 65 and 129, among them Ra = 7e7 and
+  9.2e7) equal the oracle's run (its explicit step restates the reference
+  bit for bit; its Poisson solves are exact) to relative L2 <= 1e-9 in T and
+  v after the 2000 steps, with the wall fluxes to 1e-9.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+from bench import sweep_requests  # noqa: E402
+
+
+def _selection():
+    reqs = sweep_requests(64)
+    pick, seen = [], set()
+    for q, (_, cfg, _, _) in enumerate(reqs):
+        key = (cfg.np + 1, cfg.geom)
+        if key not in seen:
+            seen.add(key)
+            pick.append(q)
+    top = sorted(range(len(reqs)), key=lambda q: reqs[q][1].Ra)[-2:]
+    pick += [q for q in top if q not in pick]
+    return reqs, pick
+
+
+def _oracle_run(cfg, th0, steps):
+    r = O.Run(cfg.nr, cfg.np, cfg.dt, cfg.Ra, cfg.Pr, width=cfg.width, height=cfg.rmax - cfg.r0,
+              geom=cfg.geom, r0=cfg.r0, Re=cfg.Re, robert=cfg.robert)
+    r["th"][...] = th0
+    r.start()
+    trans = None
+    for _ in range(steps):
+        st, trans, _ = r.step(mode=1)
+        assert st == 0
+    return r, trans
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_served_sweep_matches_alone_and_oracle():
+    from codesthesis_dir_amd import PeriodicBox
+    from codesthesis_dir_amd.serve import Server
+    reqs, pick = _selection()
+    assert len(pick) >= 11
+    sides = {reqs[q][1].np + 1 for q in pick}
+    assert sides == {65, 129, 257, 513, 1025}
+    assert max(reqs[q][1].Ra for q in pick) > 9e7
+    print(f"[gpu] config4 serving {len(pick)} requests: {pick}", flush=True)
+    with Server(slots=8, mem_budget=64e9) as srv:
+        ids = {srv.submit(reqs[q][1], reqs[q][3], reqs[q][2], outputs=True): q for q in pick}
+        got = {}
+        for _ in ids:
+            g = srv.wait(timeout=300.0)
+            assert g is not None
+            got[ids[g["id"]]] = g
+        st = srv.stats()
+    assert st["peak_running"] >= 4
+    for q in pick:
+        _, cfg, th0, steps = reqs[q]
+        g = got[q]
+        print(f"[gpu] request {q}: side {cfg.np + 1} geom {cfg.geom} Ra {cfg.Ra:.3g}", flush=True)
+        assert g["status"] == 0, (q, g["status"], g["error"])
+        box = PeriodicBox(cfg, device=0)
+        box.put("th", th0)
+        box.start()
+        info = box.step(steps)
+        th, vel = box.get("th"), box.get("vel")
+        box.close()
+        assert info.status == 0
+        assert np.array_equal(g["th"], th[:, :cfg.nr + 2]), q
+        assert np.array_equal(g["vel"], vel[:, :cfg.nr + 2]), q
+        assert g["trans1"] == info.trans1 and g["trans2"] == info.trans2
+        assert np.all(np.isfinite(th))
+    small = [q for q in pick if reqs[q][1].np + 1 == 65] + \
+        [q for q in pick if reqs[q][1].np + 1 == 129 and reqs[q][1].Ra > 9e7]
+    assert len(small) >= 3
+    for q in small:
+        _, cfg, th0, steps = reqs[q]
+        print(f"[gpu] oracle for request {q}", flush=True)
+        ref, trans = _oracle_run(cfg, th0, steps)
+        g = got[q]
+        assert _rel(g["th"], ref["th"][:, :cfg.nr + 2]) <= 1e-9, q
+        assert _rel(g["vel"], ref["vel"][:, :cfg.nr + 2]) <= 1e-9, q
+        assert abs(g["trans1"] - trans[0]) <= 1e-9 * abs(trans[0]), q
+        assert abs(g["trans2"] - trans[1]) <= 1e-9 * abs(trans[1]), q
