@@ -17,6 +17,7 @@ import numpy as np
 from ._lib import check, lib
 
 TDMA, PTDMA, PDMA = 0, 1, 2
+NOALLOC_WORDS = 8388608
 _H2D, _D2H = 1, 2
 
 _ARGT = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong] + \
@@ -93,20 +94,21 @@ def _p(x):
 
 
 def solve(kind: int, n: int, nsys: int, ss: int, sr: int, a, b, c, r, x, d=None, e=None,
-          stream=None) -> int:
+          stream=None, noalloc: bool = False) -> int:
     """The C entry point on device arrays (DeviceArray or raw device
     addresses): row j of system s of each array at element s*ss + j*sr.
-    stream: a hipStream_t address (None: the null stream).  Returns after
-    the launch."""
-    fn = lib().vv_band_solve
+    stream: a hipStream_t address (None: the null stream).  noalloc selects
+    vv_band_solve_noalloc.  Returns after the launch."""
+    fn = lib().vv_band_solve_noalloc if noalloc else lib().vv_band_solve
     fn.argtypes = _ARGT
     fn.restype = ctypes.c_int
     code = fn(kind, n, nsys, ss, sr, _p(a), _p(b), _p(c), _p(d), _p(e), _p(r), _p(x),
               None if stream is None else ctypes.c_void_p(stream))
-    return check(code, "vv_band_solve")
+    return check(code, "vv_band_solve_noalloc" if noalloc else "vv_band_solve")
 
 
-def solve_host(kind: int, a, b, c, r, d=None, e=None, interleaved: bool = True, pad: int = 0) -> np.ndarray:
+def solve_host(kind: int, a, b, c, r, d=None, e=None, interleaved: bool = True, pad: int = 0,
+               noalloc: bool = False) -> np.ndarray:
     """Solves the (nsys, n) host systems on the GPU and returns x (nsys, n).
     interleaved stores the lines column-wise on the device (ss = 1, sr =
     nsys + pad), otherwise row-wise (ss = n + pad, sr = 1)."""
@@ -127,7 +129,8 @@ def solve_host(kind: int, a, b, c, r, d=None, e=None, interleaved: bool = True, 
 
     arrs = [None if v is None else lay(v) for v in (a, b, c, d, e, r)]
     x = DeviceArray(shape[0] * shape[1])
-    solve(kind, n, nsys, ss, sr, arrs[0], arrs[1], arrs[2], arrs[5], x, d=arrs[3], e=arrs[4])
+    solve(kind, n, nsys, ss, sr, arrs[0], arrs[1], arrs[2], arrs[5], x, d=arrs[3], e=arrs[4],
+          noalloc=noalloc)
     hip().hipDeviceSynchronize()
     out = x.numpy().reshape(shape)
     for v in arrs + [x]:

@@ -224,3 +224,46 @@ extern "C" int vv_band_solve(int kind, int n, int nsys, long long ss, long long 
     VV_HIP(fe);
     return VV_OK;
 }
+
+// ----------------------------------------------------- allocation-free form
+// The scratch rows of vv_band_solve_noalloc live in this array, one copy per
+// GPU for the life of the process.  Host code never passes its name to a
+// kernel: the device address of each GPU's copy is looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept in g_scr.
+namespace {
+static __device__ __align__(256) double scratch[VV_BAND_NOALLOC_WORDS];
+
+constexpr int MAX_DEV = 64;
+std::mutex g_scr_mu;
+double *g_scr[MAX_DEV] = {};
+}  // namespace
+
+extern "C" int vv_band_solve_noalloc(int kind, int n, int nsys, long long ss, long long sr, const double *a,
+                                     const double *b, const double *c, const double *d, const double *e,
+                                     const double *r, double *x, void *stream)
+{
+    int st = vv::band_check(kind, n, nsys, ss, sr, a, b, c, d, e, r, x);
+    if (st != VV_OK) return st;
+    const double words = (double)vv::band_words(kind) * (double)n * (double)nsys;
+    if (words > (double)VV_BAND_NOALLOC_WORDS) {
+        vv::set_error("vv_band_solve_noalloc: batch needs " + std::to_string((long long)words) +
+                      " scratch words, more than VV_BAND_NOALLOC_WORDS (" +
+                      std::to_string((long long)VV_BAND_NOALLOC_WORDS) + ")");
+        return VV_EINVAL;
+    }
+    if (nsys == 0) return VV_OK;
+    int dev = 0;
+    VV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEV) { vv::set_error("vv_band_solve_noalloc: device index out of range"); return VV_EINVAL; }
+    double *scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scr_mu);
+        if (!g_scr[dev]) {
+            void *p = nullptr;
+            VV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scr[dev] = (double *)p;
+        }
+        scr = g_scr[dev];
+    }
+    return vv::band_launch(kind, n, nsys, ss, sr, a, b, c, d, e, r, x, scr, (hipStream_t)stream);
+}

@@ -49,6 +49,23 @@ int vv_band_solve(int kind, int n, int nsys, long long ss, long long sr,
                   const double *d, const double *e, const double *r,
                   double *x, void *stream);
 
+/* The same solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The scratch rows live in one static device array per GPU of
+ * VV_BAND_NOALLOC_WORDS doubles (64 MiB), held from the first call on that
+ * GPU until the process ends; a batch needing more (words per row x n x nsys)
+ * is rejected with VV_EINVAL.  Every call on a GPU uses that GPU's one array,
+ * so calls that overlap on one GPU (different streams, or different host
+ * threads without ordering) race: order them on one stream.  Each call writes
+ * the scratch rows before it reads them, so nothing carries over between
+ * calls and nothing needs clearing. */
+#define VV_BAND_NOALLOC_WORDS 8388608
+int vv_band_solve_noalloc(int kind, int n, int nsys, long long ss, long long sr,
+                          const double *a, const double *b, const double *c,
+                          const double *d, const double *e, const double *r,
+                          double *x, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
