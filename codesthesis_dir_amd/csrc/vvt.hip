@@ -1,0 +1,1239 @@
+// vvt.hip -- the 3D horizontal-convection tank step on gfx950
+// (include/vvtank.h).
+//
+// Explicit part: the compact advection derivatives as batched line solves
+// (one thread per line, the elimination tables of the constant rows computed
+// once on the host in the order of the reference's tdma), then one thread per
+// point for the A, B, C and temperature updates, the right-hand sides and
+// the boundaries.  Every expression is evaluated in the operation order the
+// oracle restates (the library is compiled with -ffp-contract=off), so the
+// explicit results are bit-for-bit the oracle's.
+//
+// Poisson: each operator L = D_y + D_z + D_x (Cartesian, constant rows) is
+// solved exactly: its y rows have cosine modes (u, w: mirrored at the
+// stress-free side walls) or sine modes (v: zero there), applied as FP64
+// matrix-core GEMMs; the z rows are diagonalised once (the eigenbasis of
+// rot3dmg.f's z rows, shared with the annulus solver); what remains per
+// (y mode, z eigenvalue) is one banded system along x, eliminated by one
+// thread (Thomas, the closure rows' third coefficient folded in).
+
+#include "vv3_internal.h"
+#include "vv_gemm.h"
+#include "vv_reduce.h"
+
+#include "../../include/vvtank.h"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace vvt {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int KE_MAXB = 1024;
+
+struct TGeo {
+    int nr, nz, np, ldi, ldj;
+    double dt, dr, dz, dp, Re, Ptl, Gr;
+};
+
+#define X(a, i, j, k) (a)[(unsigned)(((k) * g.ldj + (j)) * g.ldi + (i))]
+
+inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------------- compact lines
+// One derivative job: field f along `axis` (0: x, 1: y, 2: z), end rows of
+// `type` (0: nodes equally spaced, velvor2dperiodicrbconv.f:613-620; 1: walls
+// half a spacing away, its 1085-1097), m unknowns, the lines' other two
+// indices in [1, n1] x [1, n2] (x lines (j, k), y lines (i, j), z lines
+// (i, k)), tables fact | bp | c (1-based, m+1 each).
+struct DJob {
+    const double *f;
+    double *d;
+    const double *tab;
+    int axis, type, m, n1, n2;
+    double dri;
+};
+struct DJobs {
+    DJob j[12];
+};
+
+__device__ __forceinline__ double line_rhs(int type, int m, int i, double dri, double fm3, double fm1, double f0,
+                                           double fp1, double f2, double f4)
+{
+    // f0 = f[i], fm1 = f[i-1], fp1 = f[i+1]; fm3 = f[m-3] and f4 = f[4] for
+    // the end rows of type 1, f2 = f[2] for row 1 of type 0
+    if (type == 0) {
+        if (i == 1) return 3.0 * (f2 - fm1) * dri - (-1.5 * fm1 + 2.0 * f0 - 0.5 * f2) * dri;
+        if (i == m) return 3.0 * (fp1 - fm1) * dri - (0.5 * fm1 - 2.0 * f0 + 1.5 * fp1) * dri;
+        return 3.0 * (fp1 - fm1) * dri;
+    }
+    if (i == 1) return dri * (0.533333333 * fm1 - 3.6 * f0 + 2.966666667 * fp1 + 0.1 * f4);
+    if (i == m) return dri * (-0.1 * fm3 - 2.966666667 * fm1 + 3.6 * f0 - 0.533333333 * fp1);
+    return 3.0 * (fp1 - fm1) * dri;
+}
+
+__global__ void __launch_bounds__(WG) k_t_deriv(TGeo g, DJobs jobs)
+{
+    const DJob J = jobs.j[blockIdx.y];
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= J.n1 * J.n2) return;
+    const int q = t / J.n1, p = t - q * J.n1;
+    const int P1 = p + 1, Q1 = q + 1;
+    unsigned base, s;
+    if (J.axis == 0) { base = (unsigned)((Q1 * g.ldj + P1) * g.ldi); s = 1u; }
+    else if (J.axis == 1) { base = (unsigned)(Q1 * g.ldi + P1); s = (unsigned)(g.ldi * g.ldj); }
+    else { base = (unsigned)(Q1 * g.ldj * g.ldi + P1); s = (unsigned)g.ldi; }
+    const double *__restrict__ f = J.f;
+    double *__restrict__ d = J.d;
+    const int m = J.m;
+    const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1);
+    const double f4 = (J.type == 1) ? f[base + 4u * s] : 0.0;
+    const double fm3 = (J.type == 1) ? f[base + (unsigned)(m - 3) * s] : 0.0;
+    double fa = f[base], fb = f[base + s], fc = f[base + 2u * s];   // f[i-1], f[i], f[i+1] at i = 1
+    const double f2 = fc;
+    double rp = 0.0;
+    for (int i = 1; i <= m; ++i) {
+        const double r = line_rhs(J.type, m, i, J.dri, fm3, fa, fb, fc, f2, f4);
+        rp = (i == 1) ? r : r - fact[i] * rp;
+        d[base + (unsigned)i * s] = rp;
+        fa = fb;
+        fb = fc;
+        if (i + 2 <= m + 1) fc = f[base + (unsigned)(i + 2) * s];
+    }
+    double y = rp / bp[m];
+    d[base + (unsigned)m * s] = y;
+    for (int i = m - 1; i >= 1; --i) {
+        const unsigned o = base + (unsigned)i * s;
+        y = (d[o] - cc[i] * y) / bp[i];
+        d[o] = y;
+    }
+}
+
+// ------------------------------------------------------- explicit step
+struct VF {
+    const double *A, *B, *C, *Ap, *Bp, *Cp, *th, *thp, *uel, *vel, *wel;
+    const double *dv[12];
+    double *An, *Bn, *Cn, *thn;
+};
+
+// Avorticity (rot3dmg.f:359-465, Cartesian): i = 1..nr, j = 1..nz-1,
+// k = 1..np-1
+__global__ void __launch_bounds__(WG) k_t_avort(TGeo g, VF F)
+{
+    const int nr = g.nr, nz = g.nz;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= nr * (nz - 1)) return;
+    const int jj = t / nr, i = 1 + (t - jj * nr), j = 1 + jj, k = 1 + blockIdx.y;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re, Gr = g.Gr;
+    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    const int im = i - 1, jp = j + 1, ka = k + 1, km = k - 1;
+    const double u = (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, k) +
+                      X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) *
+                     0.125;
+    const double v = (X(vel, i, jp, k) + X(vel, i, j, k)) * 0.5;
+    const double w = (X(wel, i, j, ka) + X(wel, i, j, k)) * 0.5;
+    const double BB = (X(B, i, j, k) + X(B, im, j, k) + X(B, i, j, ka) + X(B, im, j, ka)) * 0.25;
+    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, im, j, k) + X(C, im, jp, k)) * 0.25;
+    double Arra, cnr;
+    if (i == 1) {
+        Arra = (3.2 * X(A, i - 1, j, k) + 2.0 * X(A, i + 1, j, k) - 0.2 * X(A, i + 2, j, k)) / (dr * dr);
+        cnr = -2.5 / (dr * dr);
+    } else if (i == nr) {
+        Arra = (3.2 * X(A, i + 1, j, k) + 2.0 * X(A, i - 1, j, k) - 0.2 * X(A, i - 2, j, k)) / (dr * dr);
+        cnr = -2.5 / (dr * dr);
+    } else {
+        Arra = (X(A, i - 1, j, k) + X(A, i + 1, j, k)) / (dr * dr);
+        cnr = -1.0 / (dr * dr);
+    }
+    const double uelr = ((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
+                         (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25) /
+                        dr;
+    const double Azza = (X(A, i, j - 1, k) + X(A, i, j + 1, k)) / (dz * dz);
+    const double uelz = ((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
+                         (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25) /
+                        dz;
+    const double thh = (X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5 / dp;
+    const double Ahha = (X(A, i, j, ka) + X(A, i, j, km)) / (dp * dp);
+    const double uelh = ((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
+                         (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25) /
+                        dp;
+    const double cnz = -1.0 / (dz * dz);
+    const double cnh = -1.0 / (dp * dp);
+    const double Ax = X(F.dv[0], i, j, k), Ay = X(F.dv[1], i, j, k), Az = X(F.dv[2], i, j, k);
+    const double RHS1 = Arra + Ahha + Azza + X(A, i, j, k) / (Ptl * dt);
+    const double ULT = (u * Ax + v * Ay + w * Az - uelh * BB - uelz * CC) / Ptl;
+    const double RHS2 = -2.0 * Re * uelz - Gr * Ptl * thh;
+    const double cf1 = 1.0 / (Ptl * dt) - uelr / (2.0 * Ptl) - cnr - cnz - cnh;
+    const double cf2 = uelr / (2.0 * Ptl) + cnr + cnz + cnh;
+    X(F.An, i, j, k) = (cf2 * X(F.Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+}
+
+// Bvorticity (rot3dmg.f:495-590, Cartesian): i = 1..nr-1, j = 1..nz-1,
+// k = 1..np
+__global__ void __launch_bounds__(WG) k_t_bvort(TGeo g, VF F)
+{
+    const int nr = g.nr, nz = g.nz;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= (nr - 1) * (nz - 1)) return;
+    const int jj = t / (nr - 1), i = 1 + (t - jj * (nr - 1)), j = 1 + jj, k = 1 + blockIdx.y;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re, Gr = g.Gr;
+    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    const int ip = i + 1, jp = j + 1, km = k - 1, ka = k + 1;
+    const double u = (X(uel, i, j, k) + X(uel, i, jp, k)) * 0.5;
+    const double v = (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k) + X(vel, ip, j, km) +
+                      X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) *
+                     0.125;
+    const double w = (X(wel, i, j, k) + X(wel, ip, j, k)) * 0.5;
+    const double AA = (X(A, i, j, k) + X(A, ip, j, k) + X(A, i, j, km) + X(A, ip, j, km)) * 0.25;
+    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, i, j, km) + X(C, i, jp, km)) * 0.25;
+    const double Brra = (X(B, i - 1, j, k) + X(B, i + 1, j, k)) / (dr * dr);
+    const double velr = ((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
+                         (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25) /
+                        dr;
+    const double Bzza = (X(B, i, j - 1, k) + X(B, i, j + 1, k)) / (dz * dz);
+    const double velz = ((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
+                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25) /
+                        dz;
+    const double thr = (X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5 / dr;
+    const double Bhha = (X(B, i, j, ka) + X(B, i, j, km)) / (dp * dp);
+    const double velh = ((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
+                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25) /
+                        dp;
+    const double cnr = -1.0 / (dr * dr);
+    const double cnz = -1.0 / (dz * dz);
+    const double cnh = -1.0 / (dp * dp);
+    const double Bx = X(F.dv[3], i, j, k), By = X(F.dv[4], i, j, k), Bz = X(F.dv[5], i, j, k);
+    const double RHS1 = Brra + Bhha + Bzza + X(B, i, j, k) / (Ptl * dt);
+    const double ULT = (u * Bx + v * By + w * Bz - velr * AA - velz * CC) / Ptl;
+    const double RHS2 = -2.0 * Re * velz + Gr * Ptl * thr;
+    const double cf1 = 1.0 / (Ptl * dt) - velh / (2.0 * Ptl) - cnr - cnz - cnh;
+    const double cf2 = velh / (2.0 * Ptl) + cnr + cnz + cnh;
+    X(F.Bn, i, j, k) = (cf2 * X(F.Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+}
+
+// Cvorticity (rot3dmg.f:619-725, Cartesian): i = 1..nr-1, j = 1..nz,
+// k = 1..np-1
+__global__ void __launch_bounds__(WG) k_t_cvort(TGeo g, VF F)
+{
+    const int nr = g.nr, nz = g.nz;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= (nr - 1) * nz) return;
+    const int jj = t / (nr - 1), i = 1 + (t - jj * (nr - 1)), j = 1 + jj, k = 1 + blockIdx.y;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re;
+    const double *A = F.A, *B = F.B, *C = F.C, *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    const int ip = i + 1, jm = j - 1, km = k - 1, ka = k + 1;
+    const double u = (X(uel, i, j, ka) + X(uel, i, j, k)) * 0.5;
+    const double v = (X(vel, i, j, k) + X(vel, ip, j, k)) * 0.5;
+    const double w = (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, k) +
+                      X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) *
+                     0.125;
+    const double AA = (X(A, i, j, k) + X(A, i, jm, k) + X(A, ip, j, k) + X(A, ip, jm, k)) * 0.25;
+    const double BB = (X(B, i, j, k) + X(B, i, jm, k) + X(B, i, j, ka) + X(B, i, jm, ka)) * 0.25;
+    const double Crra = (X(C, i - 1, j, k) + X(C, i + 1, j, k)) / (dr * dr);
+    const double welr = ((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
+                         (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25) /
+                        dr;
+    double Czza, cnz;
+    if (j == 1) {
+        Czza = (3.2 * X(C, i, j - 1, k) + 2.0 * X(C, i, j + 1, k) - 0.2 * X(C, i, j + 2, k)) / (dz * dz);
+        cnz = -2.5 / (dz * dz);
+    } else if (j == nz) {
+        Czza = (3.2 * X(C, i, j + 1, k) + 2.0 * X(C, i, j - 1, k) - 0.2 * X(C, i, j - 2, k)) / (dz * dz);
+        cnz = -2.5 / (dz * dz);
+    } else {
+        Czza = (X(C, i, j - 1, k) + X(C, i, j + 1, k)) / (dz * dz);
+        cnz = -1.0 / (dz * dz);
+    }
+    const double welz = ((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
+                         (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25) /
+                        dz;
+    const double Chha = (X(C, i, j, ka) + X(C, i, j, km)) / (dp * dp);
+    const double welh = ((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
+                         (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25) /
+                        dp;
+    const double cnr = -1.0 / (dr * dr);
+    const double cnh = -1.0 / (dp * dp);
+    const double Cx = X(F.dv[6], i, j, k), Cy = X(F.dv[7], i, j, k), Cz = X(F.dv[8], i, j, k);
+    const double RHS1 = Crra + Chha + Czza + X(C, i, j, k) / (Ptl * dt);
+    const double ULT = (u * Cx + v * Cy + w * Cz - welr * AA - welh * BB) / Ptl;
+    const double RHS2 = -2.0 * Re * welz;
+    const double cf1 = 1.0 / (Ptl * dt) - welz / (2.0 * Ptl) - cnr - cnz - cnh;
+    const double cf2 = welz / (2.0 * Ptl) + cnr + cnz + cnh;
+    X(F.Cn, i, j, k) = (cf2 * X(F.Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+}
+
+// temperature (rot3dmg.f:754-871, Cartesian): i = 1..nr, j = 1..nz,
+// k = 1..np
+__global__ void __launch_bounds__(WG) k_t_temp(TGeo g, VF F)
+{
+    const int nr = g.nr, nz = g.nz;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= nr * nz) return;
+    const int jj = t / nr, i = 1 + (t - jj * nr), j = 1 + jj, k = 1 + blockIdx.y;
+    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
+    const double *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    const int im = i - 1, jm = j - 1, km = k - 1, ka = k + 1;
+    const double u = (X(uel, i, j, k) + X(uel, im, j, k)) * 0.5;
+    const double v = (X(vel, i, j, k) + X(vel, i, j, km)) * 0.5;
+    const double w = (X(wel, i, j, k) + X(wel, i, jm, k)) * 0.5;
+    double thrra, cnr, thzza, cnz;
+    if (i == 1) {
+        thrra = (3.2 * X(th, i - 1, j, k) + 2.0 * X(th, i + 1, j, k) - 0.2 * X(th, i + 2, j, k)) / (dr * dr);
+        cnr = -2.5 / (dr * dr);
+    } else if (i == nr) {
+        thrra = (3.2 * X(th, i + 1, j, k) + 2.0 * X(th, i - 1, j, k) - 0.2 * X(th, i - 2, j, k)) / (dr * dr);
+        cnr = -2.5 / (dr * dr);
+    } else {
+        thrra = (X(th, i - 1, j, k) + X(th, i + 1, j, k)) / (dr * dr);
+        cnr = -1.0 / (dr * dr);
+    }
+    if (j == 1) {
+        thzza = (3.2 * X(th, i, j - 1, k) + 2.0 * X(th, i, j + 1, k) - 0.2 * X(th, i, j + 2, k)) / (dz * dz);
+        cnz = -2.5 / (dz * dz);
+    } else if (j == nz) {
+        thzza = (3.2 * X(th, i, j + 1, k) + 2.0 * X(th, i, j - 1, k) - 0.2 * X(th, i, j - 2, k)) / (dz * dz);
+        cnz = -2.5 / (dz * dz);
+    } else {
+        thzza = (X(th, i, j - 1, k) + X(th, i, j + 1, k)) / (dz * dz);
+        cnz = -1.0 / (dz * dz);
+    }
+    const double thhha = (X(th, i, j, ka) + X(th, i, j, km)) / (dp * dp);
+    const double cnp = -1.0 / (dp * dp);
+    const double Tx = X(F.dv[9], i, j, k), Ty = X(F.dv[10], i, j, k), Tz = X(F.dv[11], i, j, k);
+    const double RHS1 = thrra + thhha + thzza + X(th, i, j, k) / dt;
+    const double ULT = u * Tx + v * Ty + w * Tz;
+    const double cf1 = 1.0 / dt - cnr - cnz - cnp;
+    const double cf2 = cnr + cnz + cnp;
+    X(F.thn, i, j, k) = (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
+}
+
+// heat(trans1,trans2), velvor2dhorconvtank.f:1589-1620: one thread per x
+// column sums its k range in the routine's order; k_t_heat_sum adds the
+// columns in order of i
+__global__ void __launch_bounds__(WG) k_t_heat(TGeo g, const double *th, double *cols)
+{
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const int i = 1 + blockIdx.x * WG + threadIdx.x;
+    if (i > nr) return;
+    const double dp = g.dp, dzi = 1.0 / g.dz;
+    double trans1 = 0.0, trans2 = 0.0;
+    for (int k = np / 2; k <= np - 1; ++k) {
+        const int ka = k + 1;
+        trans1 = trans1 +
+                 0.5 * dp *
+                     (-3.35238095 * X(th, i, 0, k) + 4.375 * X(th, i, 1, k) - 1.45833333 * X(th, i, 2, k) +
+                      0.525 * X(th, i, 3, k) - 0.08928571 * X(th, i, 4, k)) *
+                     dzi +
+                 0.5 * dp *
+                     (-3.35238095 * X(th, i, 0, ka) + 4.375 * X(th, i, 1, ka) - 1.45833333 * X(th, i, 2, ka) +
+                      0.525 * X(th, i, 3, ka) - 0.08928571 * X(th, i, 4, ka)) *
+                     dzi;
+        trans2 = trans2 +
+                 0.5 * dp *
+                     (-3.35238095 * X(th, i, nz + 1, k) + 4.375 * X(th, i, nz, k) - 1.45833333 * X(th, i, nz - 1, k) +
+                      0.525 * X(th, i, nz - 2, k) - 0.08928571 * X(th, i, nz - 3, k)) *
+                     dzi +
+                 0.5 * dp *
+                     (-3.35238095 * X(th, i, nz + 1, ka) + 4.375 * X(th, i, nz, ka) -
+                      1.45833333 * X(th, i, nz - 1, ka) + 0.525 * X(th, i, nz - 2, ka) -
+                      0.08928571 * X(th, i, nz - 3, ka)) *
+                     dzi;
+    }
+    cols[2 * (i - 1)] = trans1;
+    cols[2 * (i - 1) + 1] = trans2;
+}
+
+__global__ void k_t_heat_sum(int nr, double dr, const double *cols, double *out)
+{
+    if (threadIdx.x != 0) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        s1 = s1 + cols[2 * i] * dr;
+        s2 = s2 + cols[2 * i + 1] * dr;
+    }
+    out[0] = s1;
+    out[1] = s2;
+}
+
+// rhsw / rhsv / rhsu (rot3dmg.f:1026-1134, Cartesian)
+struct RF {
+    const double *An, *Bn, *Cn;
+    double *wrh, *vrh, *urh;
+};
+__global__ void __launch_bounds__(WG) k_t_rhs(TGeo g, RF F, int which)
+{
+    const int nr = g.nr, nz = g.nz;
+    const double dr = g.dr, dz = g.dz, dp = g.dp;
+    const int ni = (which == 3) ? nr - 1 : nr;
+    const int nj = (which == 1) ? nz - 1 : nz;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= ni * nj) return;
+    const int jj = t / ni, i = 1 + (t - jj * ni), j = 1 + jj, k = 1 + blockIdx.y;
+    const double *An = F.An, *Bn = F.Bn, *Cn = F.Cn;
+    if (which == 1) {
+        const double Ah = (X(An, i, j, k) - X(An, i, j, k - 1)) / dp;
+        const double Br = (X(Bn, i, j, k) - X(Bn, i - 1, j, k)) / dr;
+        double v;
+        if (i == 1) v = Ah - X(Bn, 1, j, k) / dr;
+        else if (i == nr) v = Ah + X(Bn, nr - 1, j, k) / dr;
+        else v = Ah - Br;
+        X(F.wrh, i, j, k) = v;
+    } else if (which == 2) {
+        const double Az = (X(An, i, j, k) - X(An, i, j - 1, k)) / dz;
+        const double Cr = (X(Cn, i, j, k) - X(Cn, i - 1, j, k)) / dr;
+        double v;
+        if (i == 1) v = X(Cn, 1, j, k) / dr;
+        else if (i == nr) v = -X(Cn, nr - 1, j, k) / dr;
+        else v = Cr;
+        if (j == 1) v = v - X(An, i, 1, k) / dz;
+        else if (j == nz) v = v + X(An, i, nz - 1, k) / dz;
+        else v = v - Az;
+        X(F.vrh, i, j, k) = v;
+    } else {
+        const double Bz = (X(Bn, i, j, k) - X(Bn, i, j - 1, k)) / dz;
+        const double Ch = (X(Cn, i, j, k) - X(Cn, i, j, k - 1)) / dp;
+        double v;
+        if (j == 1) v = X(Bn, i, 1, k) / dz - Ch;
+        else if (j == nz) v = -X(Bn, i, nz - 1, k) / dz - Ch;
+        else v = Bz - Ch;
+        X(F.urh, i, j, k) = v;
+    }
+}
+
+// ------------------------------------------------------------ update
+struct UF {
+    const double *uel, *vel, *wel;
+    double *An, *Bn, *Cn, *thn;
+};
+
+// wall vorticity at the z walls (rot3dmg.f:980-987) and x walls (989-996),
+// one thread per (wall point, k); blockIdx.y = k - 1 (k = 1..np)
+__global__ void __launch_bounds__(WG) k_t_upd_walls(TGeo g, UF F)
+{
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const double dr = g.dr, dz = g.dz;
+    const double *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    const int t = blockIdx.x * WG + threadIdx.x, k = 1 + blockIdx.y;
+    // t < nr: z walls of A (i = t+1) and B (i = t+1 < nr)
+    if (t < nr) {
+        const int i = t + 1;
+        if (k <= np - 1) {
+            X(F.An, i, 0, k) = -(3.0 * X(vel, i, 1, k) - X(vel, i, 2, k) / 3.0) / dz;
+            X(F.An, i, nz, k) = -(X(vel, i, nz - 1, k) / 3.0 - 3.0 * X(vel, i, nz, k)) / dz;
+        }
+        if (i <= nr - 1) {
+            X(F.Bn, i, 0, k) = (3.0 * X(uel, i, 1, k) - X(uel, i, 2, k) / 3.0) / dz;
+            X(F.Bn, i, nz, k) = (-3.0 * X(uel, i, nz, k) + X(uel, i, nz - 1, k) / 3.0) / dz;
+        }
+    } else if (t < nr + nz) {
+        const int j = t - nr + 1;   // 1..nz
+        if (j <= nz - 1) {
+            X(F.Bn, 0, j, k) = -(3.0 * X(wel, 1, j, k) - X(wel, 2, j, k) / 3.0) / dr;
+            X(F.Bn, nr, j, k) = -(X(wel, nr - 1, j, k) / 3.0 - 3.0 * X(wel, nr, j, k)) / dr;
+        }
+        if (k <= np - 1) {
+            X(F.Cn, 0, j, k) = (3.0 * X(vel, 1, j, k) - X(vel, 2, j, k) / 3.0) / dr;
+            X(F.Cn, nr, j, k) = (X(vel, nr - 1, j, k) / 3.0 - 3.0 * X(vel, nr, j, k)) / dr;
+        }
+    }
+}
+
+// side walls, stress free: A = C = 0 (velvor2dhorconvtank.f:1720-1727 for
+// C), B mirrored into the ghost planes; one thread per (i, j)
+__global__ void __launch_bounds__(WG) k_t_upd_side(TGeo g, UF F)
+{
+    const int np = g.np;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= g.ldi * g.ldj) return;
+    const int j = t / g.ldi, i = t - j * g.ldi;
+    X(F.An, i, j, 0) = 0.0;
+    X(F.An, i, j, np) = 0.0;
+    X(F.Cn, i, j, 0) = 0.0;
+    X(F.Cn, i, j, np) = 0.0;
+    X(F.Bn, i, j, 0) = X(F.Bn, i, j, 1);
+    X(F.Bn, i, j, np + 1) = X(F.Bn, i, j, np);
+}
+
+// temperature boundaries, velvor2dhorconvtank.f:1728-1744, in three phases
+// (each reads what the previous one wrote): 0: the insulated bottom and the
+// linear surface (i = 1..nr, k = 1..np); 1: the insulated x walls
+// (j = 0..nz+1, k = 1..np); 2: the insulated side walls (all i, j)
+__global__ void __launch_bounds__(WG) k_t_upd_temp(TGeo g, double *thn, int phase)
+{
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (phase == 0) {
+        if (t >= nr * np) return;
+        const int kk = t / nr, i = 1 + (t - kk * nr), k = 1 + kk;
+        X(thn, i, 0, k) =
+            -30.0 / 92.0 * (-3.75 * X(thn, i, 1, k) + 5.0 / 6.0 * X(thn, i, 2, k) - 3.0 / 20.0 * X(thn, i, 3, k));
+        X(thn, i, nz + 1, k) = (double)k / (double)np * 1.0;
+    } else if (phase == 1) {
+        if (t >= (nz + 2) * np) return;
+        const int kk = t / (nz + 2), j = t - kk * (nz + 2), k = 1 + kk;
+        X(thn, 0, j, k) =
+            -30.0 / 92.0 * (-3.75 * X(thn, 1, j, k) + 5.0 / 6.0 * X(thn, 2, j, k) - 3.0 / 20.0 * X(thn, 3, j, k));
+        X(thn, nr + 1, j, k) = -30.0 / 92.0 *
+                               (-3.75 * X(thn, nr, j, k) + 5.0 / 6.0 * X(thn, nr - 1, j, k) -
+                                3.0 / 20.0 * X(thn, nr - 2, j, k));
+    } else {
+        if (t >= g.ldi * g.ldj) return;
+        const int j = t / g.ldi, i = t - j * g.ldi;
+        X(thn, i, j, 0) = 2.0 / 1.5 * X(thn, i, j, 1) - 1.0 / 3.0 * X(thn, i, j, 2);
+        X(thn, i, j, np + 1) = 2.0 / 1.5 * X(thn, i, j, np) - 1.0 / 3.0 * X(thn, i, j, np - 1);
+    }
+}
+
+// rot3dmg.f:1003-1015 as a rotation of three buffers per quantity: the old
+// Ap buffer, which becomes the next An, receives a copy of An (so every
+// level holds what the reference's copies leave)
+struct CopyF {
+    const double *src[4];
+    double *dst[4];
+};
+__global__ void __launch_bounds__(WG) k_t_copy(CopyF F, size_t n)
+{
+    const size_t t = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (t >= n) return;
+    const int q = blockIdx.y;
+    F.dst[q][t] = F.src[q][t];
+}
+
+// ------------------------------------------------------------- Poisson
+// y transform: S[q][u] = sum_k W[q][k] rhs(i, j, k+1), u = (j-1) i1 + (i-1)
+struct YFwd {
+    const double *W;
+    int M, N, K;
+    const double *rhs;
+    double *S;
+    int i1, ldi, ldj;
+    __device__ double a(int, int m, int k) const { return W[(size_t)m * K + k]; }
+    __device__ double b(int, int k, int n) const
+    {
+        const int j = n / i1, i = n - j * i1;
+        return rhs[((size_t)(k + 1) * ldj + (size_t)(j + 1)) * ldi + (size_t)(i + 1)];
+    }
+    __device__ void c(int, int m, int n, double v) const { S[(size_t)m * N + n] = v; }
+};
+// z transform T[q][i][l] = sum_j S[q][j][i] Vinv[l][j]
+struct ZFwd {
+    const double *Vinv;
+    int M, N, K;
+    const double *S;
+    double *T;
+    int mt;
+    __device__ double a(int q, int m, int k) const { return S[(size_t)q * mt + (size_t)k * M + m]; }
+    __device__ double b(int, int k, int n) const { return Vinv[(size_t)n * K + k]; }
+    __device__ void c(int q, int m, int n, double v) const { T[(size_t)q * mt + (size_t)m * N + n] = v; }
+};
+// S[q][j][i] = sum_l V[j][l] T[q][i][l]
+struct ZInv {
+    const double *V;
+    int M, N, K;
+    const double *T;
+    double *S;
+    int mt;
+    __device__ double a(int, int m, int k) const { return V[(size_t)m * K + k]; }
+    __device__ double b(int q, int k, int n) const { return T[(size_t)q * mt + (size_t)n * K + k]; }
+    __device__ void c(int q, int m, int n, double v) const { S[(size_t)q * mt + (size_t)m * N + n] = v; }
+};
+// psi(i, j, k+1) = sum_q Wi[k][q] S[q][u]
+struct YInv {
+    const double *Wi;
+    int M, N, K;
+    const double *S;
+    double *psi;
+    int i1, ldi, ldj;
+    __device__ double a(int, int m, int k) const { return Wi[(size_t)m * K + k]; }
+    __device__ double b(int, int k, int n) const { return S[(size_t)k * N + n]; }
+    __device__ void c(int, int m, int n, double v) const
+    {
+        const int j = n / i1, i = n - j * i1;
+        psi[((size_t)(m + 1) * ldj + (size_t)(j + 1)) * ldi + (size_t)(i + 1)] = v;
+    }
+};
+
+// psip <- psi everywhere; psi <- 0 outside the unknowns
+__global__ void __launch_bounds__(WG) k_t_psi_pre(TGeo g, int i1, int j1, int k1, double *psi, double *psip)
+{
+    const int tp = blockIdx.x * WG + threadIdx.x;
+    if (tp >= g.ldi * g.ldj) return;
+    const int j = tp / g.ldi, i = tp - j * g.ldi, k = blockIdx.y;
+    const unsigned o = (unsigned)(k * g.ldi * g.ldj + tp);
+    psip[o] = psi[o];
+    if (!(i >= 1 && i <= i1 && j >= 1 && j <= j1 && k >= 1 && k <= k1)) psi[o] = 0.0;
+}
+
+// the ghost planes of u and w mirror their first and last cells
+__global__ void __launch_bounds__(WG) k_t_psi_post(TGeo g, double *psi)
+{
+    const int tp = blockIdx.x * WG + threadIdx.x;
+    if (tp >= g.ldi * g.ldj) return;
+    const unsigned pl = (unsigned)(g.ldi * g.ldj);
+    psi[tp] = psi[pl + tp];
+    psi[(unsigned)(g.np + 1) * pl + tp] = psi[(unsigned)g.np * pl + tp];
+}
+
+// per (y mode q, z eigenvalue l): (D_x + mu_q + lz_l) x = y along i, x
+// overwriting y in T (layout [q][i][l]).  Row 1 has an extra coefficient e1
+// two points inward, removed with row 2; row i1 has en, removed with the
+// eliminated row i1 - 2.  Thomas factors go to cp.
+__global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double *__restrict__ cp,
+                                                  const double *__restrict__ rt, const double *__restrict__ lz,
+                                                  const double *__restrict__ ly, int i1, int j1, int k1, double e1,
+                                                  double en)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k1 * j1) return;
+    const int q = t / j1, l = t - q * j1;
+    const size_t mt = (size_t)i1 * j1;
+    double *__restrict__ x = T + (size_t)q * mt + l;
+    double *__restrict__ c = cp + (size_t)q * mt + l;
+    const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1;
+    const double shift = ly[q] + lz[l];
+    double d0 = dia[0] + shift, s0 = sup[0], y = x[0];
+    if (e1 != 0.0) {
+        const double f = e1 / sup[1];
+        d0 -= f * sub[1];
+        s0 -= f * (dia[1] + shift);
+        y -= f * x[j1];
+    }
+    double cprev = s0 / d0;
+    y /= d0;
+    c[0] = cprev;
+    x[0] = y;
+    for (int i = 1; i < i1; ++i) {
+        const size_t o = (size_t)i * j1;
+        double a = sub[i], r = x[o];
+        if (i == i1 - 1 && en != 0.0) {
+            const size_t o2 = (size_t)(i - 2) * j1;
+            a -= en * c[o2];
+            r -= en * x[o2];
+        }
+        const double den = dia[i] + shift - a * cprev;
+        cprev = sup[i] / den;
+        y = (r - a * y) / den;
+        c[o] = cprev;
+        x[o] = y;
+    }
+    for (int i = i1 - 2; i >= 0; --i) {
+        const size_t o = (size_t)i * j1;
+        y = x[o] - c[o] * y;
+        x[o] = y;
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_t_energy(TGeo g, const double *uel, const double *vel, const double *wel,
+                                                  double *part)
+{
+    __shared__ double sh[WG / 64];
+    double s = 0.0;
+    const int pl = g.ldi * g.ldj;
+    const long long n = (long long)pl * (g.np + 2);
+    for (long long o = (long long)blockIdx.x * WG + threadIdx.x; o < n; o += (long long)gridDim.x * WG) {
+        const int k = (int)(o / pl), r = (int)(o - (long long)k * pl);
+        const int j = r / g.ldi, i = r - j * g.ldi;
+        const bool kc = k >= 1 && k <= g.np, kf = k >= 1 && k <= g.np - 1;
+        if (kc && i >= 1 && i <= g.nr - 1 && j >= 1 && j <= g.nz) s += uel[o] * uel[o];
+        if (kf && i >= 1 && i <= g.nr && j >= 1 && j <= g.nz) s += vel[o] * vel[o];
+        if (kc && i >= 1 && i <= g.nr && j >= 1 && j <= g.nz - 1) s += wel[o] * wel[o];
+    }
+    s = vv::block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(WG) k_t_energy_sum(const double *part, int nb, double scale, double *out)
+{
+    __shared__ double sh[WG / 64];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += WG) s += part[b];
+    s = vv::block_sum(s, sh);
+    if (threadIdx.x == 0) *out = scale * s;
+}
+
+}  // namespace
+}  // namespace vvt
+
+using namespace vvt;
+
+// one Poisson operator of the tank
+struct TankOp {
+    int i1 = 0, j1 = 0, k1 = 0, m = 0;
+    double e1 = 0.0, en = 0.0;
+    double *W = nullptr, *Wi = nullptr;   // k1 x k1: y modes (forward, inverse)
+    double *ly = nullptr;                 // k1: y eigenvalues / dp^2
+    double *V = nullptr, *Vinv = nullptr; // j1 x j1: z eigenvectors (row-major) and inverse
+    double *lz = nullptr;                 // j1: z eigenvalues / dz^2
+    double *rt = nullptr;                 // 3 x i1: x rows (sub, diagonal, super)
+};
+
+struct vvt_run {
+    std::recursive_mutex mu;
+    vvt_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nr = 0, nz = 0, np = 0, ldi = 0, ldj = 0;
+    double dr = 0, dz = 0, dp = 0;
+    size_t nf = 0;
+    double *f[VV3_NFIELD] = {nullptr};
+    double *dv[12] = {nullptr};
+    double *tabs = nullptr;       // compact elimination tables
+    size_t tab_off[12] = {0};
+    TankOp op[3];
+    double *S = nullptr, *T = nullptr, *cp = nullptr;   // Poisson work (kmax x mmax each)
+    double *heat = nullptr;       // 2 nr column sums, then trans1, trans2
+    double *kepart = nullptr;     // KE_MAXB partials, then the total
+    // profiling
+    bool prof = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long long prof_n[VVT_K_COUNT] = {0};
+    double prof_ms[VVT_K_COUNT] = {0}, prof_bytes[VVT_K_COUNT] = {0};
+};
+
+namespace {
+
+TGeo geo(const vvt_run *r)
+{
+    TGeo g;
+    g.nr = r->nr; g.nz = r->nz; g.np = r->np; g.ldi = r->ldi; g.ldj = r->ldj;
+    g.dt = r->cfg.dt; g.dr = r->dr; g.dz = r->dz; g.dp = r->dp;
+    g.Re = r->cfg.Re; g.Ptl = r->cfg.Ptl; g.Gr = r->cfg.Gr;
+    return g;
+}
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src)
+{
+    hipError_t e = hipMalloc(dst, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// the tables of a compact line, as the oracle (and the reference's tdma)
+// forms them: rows 1-4-1, type 1's end rows (1, 2) and (2, 1)
+void compact_tables(int type, int m, std::vector<double> &out)
+{
+    std::vector<double> a(m + 1, 1.0), b(m + 1, 4.0), c(m + 1, 1.0), fact(m + 1, 0.0);
+    if (type == 1) {
+        b[1] = 1.0;
+        c[1] = 2.0;
+        a[m] = 2.0;
+        b[m] = 1.0;
+    }
+    for (int j = 2; j <= m; ++j) {
+        fact[j] = a[j] / b[j - 1];
+        b[j] = b[j] - fact[j] * c[j - 1];
+    }
+    out.insert(out.end(), fact.begin(), fact.end());
+    out.insert(out.end(), b.begin(), b.end());
+    out.insert(out.end(), c.begin(), c.end());
+}
+
+// (field, axis) -> (type, m, n1, n2): the staggering of A, B, C, th
+struct DSpec {
+    int type, m, n1, n2;
+};
+DSpec dspec(const vvt_run *r, int d)
+{
+    const int nr = r->nr, nz = r->nz, np = r->np;
+    switch (d) {
+    case 0: return {1, nr, nz - 1, np - 1};        // A x: cells, lines (j, k)
+    case 1: return {0, np - 1, nr, nz - 1};        // A y: faces, lines (i, j)
+    case 2: return {0, nz - 1, nr, np - 1};        // A z: faces, lines (i, k)
+    case 3: return {0, nr - 1, nz - 1, np};        // B x: faces
+    case 4: return {0, np, nr - 1, nz - 1};        // B y: cells with ghost planes
+    case 5: return {0, nz - 1, nr - 1, np};        // B z: faces
+    case 6: return {0, nr - 1, nz, np - 1};        // C x: faces
+    case 7: return {0, np - 1, nr - 1, nz};        // C y: faces
+    case 8: return {1, nz, nr - 1, np - 1};        // C z: cells, walls half a spacing away
+    case 9: return {1, nr, nz, np};                // th x
+    case 10: return {0, np, nr, nz};               // th y
+    default: return {1, nz, nr, np};               // th z
+    }
+}
+
+int setup_poisson(vvt_run *r)
+{
+    using vv::set_error;
+    const double pi = std::acos(-1.0);
+    const int np = r->np;
+    const double hhr = r->dr * r->dr, hhz = r->dz * r->dz, hhp = r->dp * r->dp;
+    size_t mmax = 0, kmax = 0;
+    for (int w = 1; w <= 3; ++w) {
+        TankOp &O = r->op[w - 1];
+        O.i1 = (w == 3) ? r->nr - 1 : r->nr;
+        O.j1 = (w == 1) ? r->nz - 1 : r->nz;
+        O.k1 = (w == 2) ? np - 1 : np;
+        O.m = O.i1 * O.j1;
+        mmax = std::max(mmax, (size_t)O.m);
+        kmax = std::max(kmax, (size_t)O.k1);
+        // y modes: u, w mirrored at the side walls (cos(pi q (k - 1/2)/np),
+        // q = 0..np-1), v zero there (sin(pi q k/np), q = 1..np-1); each mode
+        // is an eigenvector of the y rows with eigenvalue 2 cos(pi q/np) - 2
+        const int k1 = O.k1;
+        std::vector<double> W((size_t)k1 * k1), Wi((size_t)k1 * k1), ly(k1);
+        for (int qi = 0; qi < k1; ++qi) {
+            const int q = (w == 2) ? qi + 1 : qi;
+            ly[qi] = (2.0 * std::cos(pi * q / np) - 2.0) / hhp;
+            const double norm = (w != 2 && q == 0) ? (double)np : 0.5 * np;
+            for (int k = 1; k <= k1; ++k) {
+                const double e = (w == 2) ? std::sin(pi * q * k / np) : std::cos(pi * q * (k - 0.5) / np);
+                W[(size_t)qi * k1 + (k - 1)] = e;
+                Wi[(size_t)(k - 1) * k1 + qi] = e / norm;
+            }
+        }
+        VV_HIP(upload(&O.W, W));
+        VV_HIP(upload(&O.Wi, Wi));
+        VV_HIP(upload(&O.ly, ly));
+        // z rows: rot3dmg.f's (Dirichlet for w, the wall closure for v, u)
+        std::vector<double> lam, V, Vinv;
+        if (!vv3::eigen_banded(vv3::z_matrix(w, O.j1), O.j1, lam, V, Vinv)) {
+            set_error("vvt_create: z eigen-decomposition of a Poisson operator failed");
+            return VV_EINVAL;
+        }
+        for (double &v : lam) v /= hhz;
+        VV_HIP(upload(&O.V, V));
+        VV_HIP(upload(&O.Vinv, Vinv));
+        VV_HIP(upload(&O.lz, lam));
+        // x rows: the wall closure (-2, 5/3, -0.2) for the cells of w and v,
+        // Dirichlet for the faces of u
+        const int i1 = O.i1;
+        std::vector<double> rt((size_t)3 * i1, 0.0);
+        for (int i = 0; i < i1; ++i) {
+            rt[i1 + i] = -2.0 / hhr;
+            if (i > 0) rt[i] = 1.0 / hhr;
+            if (i < i1 - 1) rt[2 * i1 + i] = 1.0 / hhr;
+        }
+        if (w != 3) {
+            rt[2 * i1 + 0] = 5.0 / 3.0 / hhr;
+            rt[i1 - 1] = 5.0 / 3.0 / hhr;
+            O.e1 = -0.2 / hhr;
+            O.en = -0.2 / hhr;
+        }
+        VV_HIP(upload(&O.rt, rt));
+    }
+    VV_HIP(hipMalloc(&r->S, kmax * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&r->T, kmax * mmax * sizeof(double)));
+    VV_HIP(hipMalloc(&r->cp, kmax * mmax * sizeof(double)));
+    return VV_OK;
+}
+
+// profiling brackets around one group of launches
+struct Prof {
+    vvt_run *r;
+    int id;
+    double bytes;
+    Prof(vvt_run *rr, int k, double b) : r(rr), id(k), bytes(b)
+    {
+        if (r->prof) (void)hipEventRecord(r->ev[0], r->stream);
+    }
+    ~Prof()
+    {
+        if (!r->prof) return;
+        (void)hipEventRecord(r->ev[1], r->stream);
+        (void)hipEventSynchronize(r->ev[1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r->ev[0], r->ev[1]);
+        r->prof_n[id] += 1;
+        r->prof_ms[id] += ms;
+        r->prof_bytes[id] += bytes;
+    }
+};
+
+double npts(const vvt_run *r) { return (double)r->nr * r->nz * r->np; }
+
+VF vfields(vvt_run *r)
+{
+    VF F;
+    F.A = r->f[VV3_A]; F.B = r->f[VV3_B]; F.C = r->f[VV3_C];
+    F.Ap = r->f[VV3_AP]; F.Bp = r->f[VV3_BP]; F.Cp = r->f[VV3_CP];
+    F.th = r->f[VV3_TH]; F.thp = r->f[VV3_THP];
+    F.uel = r->f[VV3_UEL]; F.vel = r->f[VV3_VEL]; F.wel = r->f[VV3_WEL];
+    for (int d = 0; d < 12; ++d) F.dv[d] = r->dv[d];
+    F.An = r->f[VV3_AN]; F.Bn = r->f[VV3_BN]; F.Cn = r->f[VV3_CN]; F.thn = r->f[VV3_THN];
+    return F;
+}
+
+int launch_derivatives(vvt_run *r)
+{
+    Prof p(r, VVT_K_DERIV, 8.0 * 24.0 * npts(r));
+    DJobs J;
+    int nmax = 0;
+    const double *src[4] = {r->f[VV3_A], r->f[VV3_B], r->f[VV3_C], r->f[VV3_TH]};
+    const double dri[3] = {1.0 / r->dr, 1.0 / r->dp, 1.0 / r->dz};
+    for (int d = 0; d < 12; ++d) {
+        const DSpec s = dspec(r, d);
+        J.j[d] = DJob{src[d / 3], r->dv[d], r->tabs + r->tab_off[d], d % 3, s.type, s.m, s.n1, s.n2, dri[d % 3]};
+        nmax = std::max(nmax, s.n1 * s.n2);
+    }
+    hipLaunchKernelGGL(k_t_deriv, dim3(nblk(nmax, WG), 12), dim3(WG), 0, r->stream, geo(r), J);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_vorticity(vvt_run *r)
+{
+    Prof p(r, VVT_K_VORT, 8.0 * 35.0 * npts(r));
+    const TGeo g = geo(r);
+    const VF F = vfields(r);
+    hipLaunchKernelGGL(k_t_avort, dim3(nblk((long long)r->nr * (r->nz - 1), WG), r->np - 1), dim3(WG), 0, r->stream,
+                       g, F);
+    hipLaunchKernelGGL(k_t_bvort, dim3(nblk((long long)(r->nr - 1) * (r->nz - 1), WG), r->np), dim3(WG), 0,
+                       r->stream, g, F);
+    hipLaunchKernelGGL(k_t_cvort, dim3(nblk((long long)(r->nr - 1) * r->nz, WG), r->np - 1), dim3(WG), 0, r->stream,
+                       g, F);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_temperature(vvt_run *r)
+{
+    Prof p(r, VVT_K_TEMP, 8.0 * 9.0 * npts(r));
+    hipLaunchKernelGGL(k_t_temp, dim3(nblk((long long)r->nr * r->nz, WG), r->np), dim3(WG), 0, r->stream, geo(r),
+                       vfields(r));
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+double *heat_out(vvt_run *r) { return r->heat + 2 * r->nr; }
+
+int launch_heat(vvt_run *r)
+{
+    Prof p(r, VVT_K_HEAT, 8.0 * 10.0 * r->nr * (r->np / 2));
+    hipLaunchKernelGGL(k_t_heat, dim3(nblk(r->nr, WG)), dim3(WG), 0, r->stream, geo(r),
+                       (const double *)r->f[VV3_TH], r->heat);
+    hipLaunchKernelGGL(k_t_heat_sum, dim3(1), dim3(64), 0, r->stream, r->nr, r->dr, (const double *)r->heat,
+                       heat_out(r));
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_rhs(vvt_run *r, int which)
+{
+    Prof p(r, VVT_K_RHS, 8.0 * 3.0 * npts(r));
+    RF F{r->f[VV3_AN], r->f[VV3_BN], r->f[VV3_CN], r->f[VV3_WRH], r->f[VV3_VRH], r->f[VV3_URH]};
+    const int ni = (which == 3) ? r->nr - 1 : r->nr, nj = (which == 1) ? r->nz - 1 : r->nz;
+    const int nk = (which == 2) ? r->np - 1 : r->np;
+    hipLaunchKernelGGL(k_t_rhs, dim3(nblk((long long)ni * nj, WG), nk), dim3(WG), 0, r->stream, geo(r), F, which);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_poisson(vvt_run *r, int which)
+{
+    const TankOp &O = r->op[which - 1];
+    const double flops = 2.0 * (2.0 * O.k1 * O.k1 * O.m) + 2.0 * (2.0 * O.j1 * O.j1 * O.i1 * O.k1);
+    Prof p(r, VVT_K_POISSON, flops);
+    const int psi = (which == 1) ? VV3_WEL : (which == 2) ? VV3_VEL : VV3_UEL;
+    const int psip = (which == 1) ? VV3_WELP : (which == 2) ? VV3_VELP : VV3_UELP;
+    const int rhs = (which == 1) ? VV3_WRH : (which == 2) ? VV3_VRH : VV3_URH;
+    const TGeo g = geo(r);
+    hipStream_t s = r->stream;
+    double *P = r->f[psi];
+    hipLaunchKernelGGL(k_t_psi_pre, dim3(nblk((long long)r->ldi * r->ldj, WG), r->np + 2), dim3(WG), 0, s, g, O.i1,
+                       O.j1, O.k1, P, r->f[psip]);
+    const int mt = O.m;
+    vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->S, O.i1, r->ldi, r->ldj}, 1, s);
+    vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->S, r->T, mt}, O.k1, s);
+    hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
+                       (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
+                       O.en);
+    vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->T, r->S, mt}, O.k1, s);
+    vv::gemm_f64<true, false>(YInv{O.Wi, O.k1, mt, O.k1, r->S, P, O.i1, r->ldi, r->ldj}, 1, s);
+    if (which != 2)
+        hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
+    VV_HIP(hipGetLastError());
+    return VV_OK;
+}
+
+int launch_update(vvt_run *r)
+{
+    Prof p(r, VVT_K_UPDATE, 8.0 * 8.0 * (double)r->nf);
+    const TGeo g = geo(r);
+    UF F{r->f[VV3_UEL], r->f[VV3_VEL], r->f[VV3_WEL], r->f[VV3_AN], r->f[VV3_BN], r->f[VV3_CN], r->f[VV3_THN]};
+    hipStream_t s = r->stream;
+    hipLaunchKernelGGL(k_t_upd_walls, dim3(nblk(r->nr + r->nz, WG), r->np), dim3(WG), 0, s, g, F);
+    const int plane = r->ldi * r->ldj;
+    hipLaunchKernelGGL(k_t_upd_side, dim3(nblk(plane, WG)), dim3(WG), 0, s, g, F);
+    hipLaunchKernelGGL(k_t_upd_temp, dim3(nblk((long long)r->nr * r->np, WG)), dim3(WG), 0, s, g, r->f[VV3_THN], 0);
+    hipLaunchKernelGGL(k_t_upd_temp, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, s, g,
+                       r->f[VV3_THN], 1);
+    hipLaunchKernelGGL(k_t_upd_temp, dim3(nblk(plane, WG)), dim3(WG), 0, s, g, r->f[VV3_THN], 2);
+    // Ap <- A <- An (and B, C, th): the old Ap buffer receives An, then the
+    // three buffers rotate
+    const int trip[4][3] = {{VV3_AP, VV3_A, VV3_AN}, {VV3_BP, VV3_B, VV3_BN}, {VV3_CP, VV3_C, VV3_CN},
+                            {VV3_THP, VV3_TH, VV3_THN}};
+    CopyF C;
+    for (int q = 0; q < 4; ++q) {
+        C.src[q] = r->f[trip[q][2]];
+        C.dst[q] = r->f[trip[q][0]];
+    }
+    hipLaunchKernelGGL(k_t_copy, dim3(nblk((long long)r->nf, WG), 4), dim3(WG), 0, s, C, r->nf);
+    VV_HIP(hipGetLastError());
+    for (auto &t : trip) {
+        double *pp = r->f[t[0]], *c = r->f[t[1]], *nn = r->f[t[2]];
+        r->f[t[0]] = c;
+        r->f[t[1]] = nn;
+        r->f[t[2]] = pp;
+    }
+    return VV_OK;
+}
+
+int copy_field(vvt_run *r, double *dev, double *host, int ldi, int ldj, bool to_dev)
+{
+    const int di = r->ldi, dj = r->ldj, nk = r->np + 2;
+    if (ldi == 0) ldi = di;
+    if (ldj == 0) ldj = dj;
+    if (ldi < di || ldj < dj) {
+        vv::set_error("vvt_put/vvt_get: ldi must be >= nr+2 and ldj >= nz+2");
+        return VV_EINVAL;
+    }
+    VV_HIP(hipSetDevice(r->device));
+    const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (ldi == di && ldj == dj) {
+        if (to_dev) VV_HIP(hipMemcpyAsync(dev, host, r->nf * sizeof(double), kind, r->stream));
+        else VV_HIP(hipMemcpyAsync(host, dev, r->nf * sizeof(double), kind, r->stream));
+    } else {
+        for (int k = 0; k < nk; ++k) {
+            double *h = host + (size_t)k * ldi * ldj;
+            double *d = dev + (size_t)k * di * dj;
+            if (to_dev)
+                VV_HIP(hipMemcpy2DAsync(d, di * sizeof(double), h, ldi * sizeof(double), di * sizeof(double), dj,
+                                        kind, r->stream));
+            else
+                VV_HIP(hipMemcpy2DAsync(h, ldi * sizeof(double), d, di * sizeof(double), di * sizeof(double), dj,
+                                        kind, r->stream));
+        }
+    }
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int one_step(vvt_run *r)
+{
+    int st;
+    if ((st = launch_derivatives(r))) return st;
+    if ((st = launch_vorticity(r))) return st;
+    if ((st = launch_temperature(r))) return st;
+    if ((st = launch_heat(r))) return st;
+    for (int w = 1; w <= 3; ++w) {
+        if ((st = launch_rhs(r, w))) return st;
+        if ((st = launch_poisson(r, w))) return st;
+    }
+    return launch_update(r);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vvt_create(const vvt_config *cfg, int device, vvt_run **out)
+{
+    using vv::set_error;
+    if (!cfg || !out) { set_error("vvt_create: null argument"); return VV_EINVAL; }
+    *out = nullptr;
+    if (cfg->nr < 6 || cfg->nz < 6 || cfg->np < 6 || cfg->nr > 4096 || cfg->nz > 4096 || cfg->np > 8192) {
+        set_error("vvt_create: need 6 <= nr, nz <= 4096 and 6 <= np <= 8192");
+        return VV_EINVAL;
+    }
+    const double nf = (double)(cfg->nr + 2) * (cfg->nz + 2) * (cfg->np + 2);
+    if (nf >= 4.0e9) { set_error("vvt_create: fields beyond 2^32 points"); return VV_EINVAL; }
+    if (!(cfg->dt > 0) || !(cfg->lx > 0) || !(cfg->lz > 0) || !(cfg->ly > 0) || !(cfg->Ptl > 0) ||
+        !std::isfinite(cfg->Re) || !std::isfinite(cfg->Gr)) {
+        set_error("vvt_create: bad physical parameters");
+        return VV_EINVAL;
+    }
+    VV_HIP(hipSetDevice(device));
+    vvt_run *r = new vvt_run();
+    r->cfg = *cfg;
+    r->device = device;
+    r->nr = cfg->nr; r->nz = cfg->nz; r->np = cfg->np;
+    r->ldi = cfg->nr + 2; r->ldj = cfg->nz + 2;
+    r->dr = cfg->lx / cfg->nr;
+    r->dz = cfg->lz / cfg->nz;
+    r->dp = cfg->ly / cfg->np;
+    r->nf = (size_t)r->ldi * r->ldj * (cfg->np + 2);
+    auto fail = [&](int s) { vvt_destroy(r); return s; };
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return fail(VV_EHIP);
+    for (int q = 0; q < VV3_NFIELD; ++q) {
+        if (hipMalloc(&r->f[q], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+    }
+    for (int d = 0; d < 12; ++d) {
+        if (hipMalloc(&r->dv[d], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemsetAsync(r->dv[d], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+    }
+    {
+        std::vector<double> tabs;
+        for (int d = 0; d < 12; ++d) {
+            const DSpec s = dspec(r, d);
+            r->tab_off[d] = tabs.size();
+            compact_tables(s.type, s.m, tabs);
+        }
+        if (upload(&r->tabs, tabs) != hipSuccess) return fail(VV_ENOMEM);
+    }
+    if (hipMalloc(&r->heat, (2 * (size_t)r->nr + 2) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMemsetAsync(r->heat, 0, (2 * (size_t)r->nr + 2) * sizeof(double), r->stream) != hipSuccess)
+        return fail(VV_EHIP);
+    if (hipMalloc(&r->kepart, (KE_MAXB + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipEventCreate(&r->ev[0]) != hipSuccess || hipEventCreate(&r->ev[1]) != hipSuccess) return fail(VV_EHIP);
+    {
+        const int st = setup_poisson(r);
+        if (st) return fail(st);
+    }
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
+    *out = r;
+    return VV_OK;
+}
+
+int vvt_destroy(vvt_run *r)
+{
+    if (!r) return VV_OK;
+    {
+        std::lock_guard<std::recursive_mutex> lk(r->mu);
+        (void)hipSetDevice(r->device);
+        if (r->stream) (void)hipStreamSynchronize(r->stream);
+        for (double *p : r->f)
+            if (p) (void)hipFree(p);
+        for (double *p : r->dv)
+            if (p) (void)hipFree(p);
+        for (auto &O : r->op)
+            for (double *p : {O.W, O.Wi, O.ly, O.V, O.Vinv, O.lz, O.rt})
+                if (p) (void)hipFree(p);
+        for (double *p : {r->tabs, r->S, r->T, r->cp, r->heat, r->kepart})
+            if (p) (void)hipFree(p);
+        for (hipEvent_t e : r->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+    }
+    delete r;
+    return VV_OK;
+}
+
+int vvt_put(vvt_run *r, int field, const double *host, int ldi, int ldj)
+{
+    if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vvt_put: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    return copy_field(r, r->f[field], const_cast<double *>(host), ldi, ldj, true);
+}
+
+int vvt_get(vvt_run *r, int field, double *host, int ldi, int ldj)
+{
+    if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vvt_get: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    return copy_field(r, r->f[field], host, ldi, ldj, false);
+}
+
+int vvt_get_derivative(vvt_run *r, int d, double *host)
+{
+    if (!r || !host || d < 0 || d >= 12) { vv::set_error("vvt_get_derivative: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    return copy_field(r, r->dv[d], host, 0, 0, false);
+}
+
+#define VVT_CALL(expr)                                      \
+    do {                                                    \
+        if (!r) { vv::set_error("null run"); return VV_EINVAL; } \
+        std::lock_guard<std::recursive_mutex> lk(r->mu);    \
+        VV_HIP(hipSetDevice(r->device));                    \
+        int st_ = (expr);                                   \
+        if (st_) return st_;                                \
+        VV_HIP(hipStreamSynchronize(r->stream));            \
+        return VV_OK;                                       \
+    } while (0)
+
+int vvt_derivatives(vvt_run *r) { VVT_CALL(launch_derivatives(r)); }
+int vvt_vorticity(vvt_run *r) { VVT_CALL(launch_vorticity(r)); }
+int vvt_temperature(vvt_run *r) { VVT_CALL(launch_temperature(r)); }
+int vvt_update(vvt_run *r) { VVT_CALL(launch_update(r)); }
+
+int vvt_rhs(vvt_run *r, int which)
+{
+    if (which < 1 || which > 3) { vv::set_error("vvt_rhs: which must be 1, 2 or 3"); return VV_EINVAL; }
+    VVT_CALL(launch_rhs(r, which));
+}
+
+int vvt_poisson(vvt_run *r, int which)
+{
+    if (which < 1 || which > 3) { vv::set_error("vvt_poisson: which must be 1, 2 or 3"); return VV_EINVAL; }
+    VVT_CALL(launch_poisson(r, which));
+}
+
+static int read_heat(vvt_run *r, double *trans1, double *trans2)
+{
+    double h[2];
+    VV_HIP(hipMemcpyAsync(h, heat_out(r), 2 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    if (trans1) *trans1 = h[0];
+    if (trans2) *trans2 = h[1];
+    return VV_OK;
+}
+
+int vvt_heat(vvt_run *r, double *trans1, double *trans2)
+{
+    if (!r) { vv::set_error("null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    VV_HIP(hipSetDevice(r->device));
+    const int st = launch_heat(r);
+    if (st) return st;
+    return read_heat(r, trans1, trans2);
+}
+
+int vvt_step(vvt_run *r, int n, double *trans1, double *trans2)
+{
+    if (!r || n < 0) { vv::set_error("vvt_step: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    VV_HIP(hipSetDevice(r->device));
+    for (int s = 0; s < n; ++s) {
+        const int st = one_step(r);
+        if (st) return st;
+    }
+    return read_heat(r, trans1, trans2);
+}
+
+int vvt_energy(vvt_run *r, double *ke)
+{
+    if (!r || !ke) { vv::set_error("vvt_energy: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    VV_HIP(hipSetDevice(r->device));
+    const int nb = std::min(KE_MAXB, std::max(1, nblk((long long)r->nf, WG)));
+    hipLaunchKernelGGL(k_t_energy, dim3(nb), dim3(WG), 0, r->stream, geo(r), (const double *)r->f[VV3_UEL],
+                       (const double *)r->f[VV3_VEL], (const double *)r->f[VV3_WEL], r->kepart);
+    hipLaunchKernelGGL(k_t_energy_sum, dim3(1), dim3(WG), 0, r->stream, (const double *)r->kepart, nb,
+                       0.5 * r->dr * r->dz * r->dp, r->kepart + KE_MAXB);
+    VV_HIP(hipGetLastError());
+    VV_HIP(hipMemcpyAsync(ke, r->kepart + KE_MAXB, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vvt_profile(vvt_run *r, int enable)
+{
+    if (!r) { vv::set_error("null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    if (enable) {
+        for (int k = 0; k < VVT_K_COUNT; ++k) {
+            r->prof_n[k] = 0;
+            r->prof_ms[k] = 0.0;
+            r->prof_bytes[k] = 0.0;
+        }
+    }
+    r->prof = enable != 0;
+    return VV_OK;
+}
+
+int vvt_profile_read(vvt_run *r, int kernel, long long *launches, double *ms, double *bytes)
+{
+    if (!r || kernel < 0 || kernel >= VVT_K_COUNT) { vv::set_error("vvt_profile_read: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    if (launches) *launches = r->prof_n[kernel];
+    if (ms) *ms = r->prof_ms[kernel];
+    if (bytes) *bytes = r->prof_bytes[kernel];
+    return VV_OK;
+}
+
+}  // extern "C"
