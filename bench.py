@@ -1,22 +1,27 @@
 """Benchmark of the velocity-vorticity time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
-    python bench.py --rot3d-only [--no-cpu]      # the 3D step (rot3dmg.f) alone
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-extra]
+    python bench.py --rot3d-only [--no-cpu]      # the 3D annulus step (rot3dmg.f) alone
+    python bench.py --serve-only                 # the served mixed-size sweep alone
 
-A step is one pass of the time loop (compact-scheme DuFort-Frankel step of
-vorticity and temperature, both Poisson solves to 1e-10 relative residual,
-wall update, heat flux, running means) over the device-resident fields.  The
-Poisson solves take the library's default: the direct solve of the whole
-plane for np <= 8192 (the V-cycle is measured beside it).  The timed region runs K steps after W warm-up steps; inputs are
-resident in HBM before it starts.  A second pass of K steps with per-kernel
-HIP events gives the dominant kernel's average duration and its achieved
-algorithmic bandwidth.  The CPU baseline (rank 0, N=1 only) times the
-reference's CPU path on a bounded sample of the same workload on this host.
-Unless --no-extra is given, the line also carries the V-cycle
-alternative, a large V-cycle measurement and the 3D step ("rot3d": rot3dmg.f
-at the reference program's largest grid, 33x33x161, at 129x129x257 and at
-257x257x513, with the reference build's own step timed on one core; and
-rot3dmghor.f, the stationary horizontal annulus, at 257x257x513).
+The default workload is config 5, the largest single-GPU configuration: the 3D
+horizontal-convection tank of 512 x 256 x 256 intervals (aspect 4:1:1),
+Ra = 1e9, Pr = 5, Ta = 1e8, uniform temperature plus seeded noise.  A step
+is one pass of its time loop (compact derivatives along the three axes,
+A/B/C and temperature updates, heat fluxes, the three exact Poisson solves,
+the wall update) over the device-resident fields.  The timed region runs K
+steps after W warm-up steps; inputs are resident in HBM before it starts.  A
+second pass of K steps with HIP events around every launch gives the
+dominant kernel's average duration and its achieved algorithmic bandwidth
+(or, for a transform GEMM, flop rate).  The CPU baseline (rank 0, N=1 only)
+times the oracle's step of the same workload on this host.  Unless
+--no-extra is given, the line also carries the 2D legs (the rotating-annulus
+plane at 513x257 with its V-cycle alternative, a large V-cycle), the 3D
+annulus of rot3dmg.f and the served sweep (config 4).
+
+The 2D workloads (--workload box-129x65, annulus-513x257, ...) keep their
+own headline form; every 2D leg refuses a run whose Poisson solves did not
+reach the tolerance (status != 0) and reports the final relative residuals.
 
 Prints one JSON line.
 """
@@ -59,7 +64,15 @@ WORKLOADS = {
                    "mixed-radix FFT, 336 = 21 x 16)",
                    dict(geom=0, nr=96, np=336, width=3.0, dt=1e-7, Ra=1e7, Pr=0.71), 20261204, 1e-3),
 }
-DEFAULT_WORKLOAD = "annulus-513x257"
+DEFAULT_WORKLOAD = "tank-512x256x256"
+
+
+def check_status(name, info):
+    """A run whose solves missed the tolerance (or met a non-finite
+    right-hand side) is not a measurement: stop with its status."""
+    if info.status != 0:
+        sys.exit(f"bench.py: {name}: Poisson status {info.status} (relres v {info.relres_v:.3g}, "
+                 f"u {info.relres_u:.3g}); no line is reported")
 
 
 def gpu_leg(name, steps, warmup, nu, direct_cols=0):
@@ -76,6 +89,7 @@ def gpu_leg(name, steps, warmup, nu, direct_cols=0):
     info = box.step(steps)          # returns after the stream is idle
     t1 = time.perf_counter()
     wall_ms = (t1 - t0) * 1e3
+    check_status(name, info)
     ev_ms = box.last_step_ms()
     # second pass with per-kernel events for the roofline
     box.profile(True)
@@ -106,7 +120,9 @@ def vcycle_leg(name, steps, warmup, nu=1):
     info = box.step(steps)
     el = time.perf_counter() - t0
     box.close()
+    check_status(name + " (V-cycle)", info)
     return {"value": steps / el, "unit": "time-steps/s", "direct_cols": cfg.direct_cols,
+            "relres_last_step": [info.relres_v, info.relres_u],
             "vcycle": f"V({nu},{nu}) zebra r-line, phi semi-coarsening",
             "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
 
@@ -133,6 +149,7 @@ def vcycle_large_leg(steps=6, nu=1):
     prof = box.profile_read()
     box.profile(False)
     box.close()
+    check_status("box-2049x4097 V-cycle", info)
     dcfg = BoxConfig(**base)
     dbox = PeriodicBox(dcfg, device=0)
     dbox.put("th", box_temperature(dcfg.nr, dcfg.np, dcfg.r0, dcfg.rmax, 1e-3, 20261206))
@@ -142,6 +159,7 @@ def vcycle_large_leg(steps=6, nu=1):
     dinfo = dbox.step(steps)
     dstep = steps / (time.perf_counter() - t0)
     dbox.close()
+    check_status("box-2049x4097 direct", dinfo)
     vb = sum(prof[k]["bytes"] for k in MG_KERNELS)
     vt = sum(prof[k]["ms"] for k in MG_KERNELS)
     z = prof["zebra"]
@@ -398,6 +416,79 @@ def serve_leg(n=64, steps=2000, slots=16, baseline=True):
     return res
 
 
+
+# config 5: the 3D horizontal-convection tank, 512 x 256 x 256 intervals
+# (np = 512 along the tank, nr = 256 across, nz = 256 vertical: aspect 4:1:1)
+TANK_WORKLOADS = {
+    "tank-512x256x256": (
+        "3D horizontal-convection tank (rot3dmg.f's step in Cartesian form, compact derivatives along "
+        "all three axes, velvor2dhorconvtank.f's boundaries: linear surface temperature, insulated walls, "
+        "stress-free side walls), 512 x 256 x 256 intervals (4:1:1), Ra = Gr Pr = 1e9, Pr = 5, "
+        "Ta = 1e8 (Re = 1e4, rotation about the vertical), dt = 1e-7; the three Poisson solves exact "
+        "(cosine/sine modes along the tank, z eigenbasis, banded solves across)",
+        dict(nr=256, nz=256, np=512, dt=1e-7, Re=1e4, Ptl=5.0, Gr=2e8, lx=1.0, lz=1.0, ly=4.0), 20261206, 1e-4),
+}
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def tank_state(cfg, seed, amp):
+    """T = 0.5 + U(-amp, amp) everywhere (seeded), fluid at rest; the first
+    update sets the walls."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    th = 0.5 + amp * (2.0 * rng.uniform(size=(cfg.np + 2, cfg.nz + 2, cfg.nr + 2)) - 1.0)
+    return th
+
+
+def tank_leg(name, steps, warmup):
+    from codesthesis_dir_amd.tank import FLOP_KERNELS, Tank, TankConfig
+    desc, kw, seed, amp = TANK_WORKLOADS[name]
+    cfg = TankConfig(**kw)
+    run = Tank(cfg)
+    th = tank_state(cfg, seed, amp)
+    run.put("th", th)
+    run.put("thp", th)
+    del th
+    if warmup:
+        run.step(warmup)
+    t0 = time.perf_counter()
+    h = run.step(steps)             # returns after the stream is idle
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    run.profile(True)
+    run.step(steps)
+    prof = run.profile_read()
+    run.profile(False)
+    ke = run.energy()
+    run.close()
+    for k, v in prof.items():
+        v["flop_kernel"] = k in FLOP_KERNELS
+    return cfg, wall_ms, prof, h, ke
+
+
+def tank_cpu_leg(name):
+    """One step of the same workload on this host's CPU, on one thread: the
+    oracle's explicit routines (C) and one of its three exact separable
+    solves (scipy DCT/DST and numpy eigenbases), in a process of its own."""
+    import numpy as np
+    desc, kw, seed, amp = TANK_WORKLOADS[name]
+    from codesthesis_dir_amd.tank import TankConfig
+    cfg = TankConfig(**kw)
+    th = tank_state(cfg, seed, amp)
+    with tempfile.TemporaryDirectory() as d:
+        np.savez(os.path.join(d, "s.npz"), nr=cfg.nr, nz=cfg.nz, np=cfg.np, dt=cfg.dt, Re=cfg.Re, Ptl=cfg.Ptl,
+                 Gr=cfg.Gr, lx=cfg.lx, lz=cfg.lz, ly=cfg.ly, f_th=th, f_thp=th)
+        del th
+        out = os.path.join(d, "o.json")
+        subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "oracle_tank.py"), os.path.join(d, "s.npz"),
+                        out], check=True, stdout=subprocess.DEVNULL)
+        r = json.load(open(out))
+    return {"value": 1.0 / r["step_s"], "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one step of the workload from its seeded start on 1 thread, timed in two parts: the "
+                      f"oracle's C restatement of all the explicit routines ({r['explicit_s']:.1f} s) and one "
+                      f"of the three equal-sized exact Poisson solves (scipy DCT-II/DST-I along the tank, "
+                      f"numpy eigenbases across; {r['solve_s']:.1f} s); step = explicit + 3 solves"}
+
+
 def cpu_leg(name, budget_s=15.0):
     """Time the reference CPU path on this host for a bounded sample."""
     desc, kw, seed, amp = WORKLOADS[name]
@@ -469,27 +560,79 @@ def read_traffic(kernel, workload):
     return None
 
 
+def leg_2d(a, name):
+    """The 2D headline form of workload `name` (the line of a 2D run)."""
+    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(name, a.steps, a.warmup, a.nu, a.direct_cols)
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_ms = d["ms"] / max(d["launches"], 1)
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    vcyc = {k: prof[k] for k in MG_KERNELS}
+    vc_ms = sum(v["ms"] for v in vcyc.values())
+    vc_bytes = sum(v["bytes"] for v in vcyc.values())
+    return {
+        "value": a.steps / (wall_ms * 1e-3),
+        "unit": "time-steps/s",
+        "steps": a.steps,
+        "ms_per_step": wall_ms / a.steps,
+        "config": {"workload": name, "description": WORKLOADS[name][0],
+                   "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
+                   "poisson": ("direct solve of the whole plane (real FFT in phi, one PCR line solve "
+                               "per mode; the default up to np = 8192), checked against 1e-10 each "
+                               "step") if cfg.direct_cols in (0, cfg.np)
+                              else f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening, direct level "
+                                   f"at {cfg.direct_cols} columns"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, name),
+                     "avg_launch_us": avg_ms * 1e3,
+                     "bytes_per_launch": d["bytes"] / max(d["launches"], 1)},
+        "vcycle_gbs": vc_bytes / (vc_ms * 1e-3) / 1e9 if vc_ms > 0 else None,
+        "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items() if v["launches"]},
+        "gpu_event_ms_per_step": ev_ms / a.steps,
+        "poisson_status": info.status,
+        "poisson_relres_last_step": [info.relres_v, info.relres_u],
+        "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
+        "nusselt_last_step": nu,
+    }
+
+
+def tank_roofline(prof, name):
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    n = max(d["launches"], 1)
+    if d["flop_kernel"]:
+        achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFS, "traffic": read_traffic(dom, name),
+                "avg_launch_us": 1e3 * d["ms"] / n, "flops_per_launch": d["flops"] / n,
+                "dtype": "f64 (v_mfma_f64_16x16x4f64)"}
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, name),
+            "avg_launch_us": 1e3 * d["ms"] / n, "bytes_per_launch": d["bytes"] / n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS) + sorted(TANK_WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nu", type=int, default=1, help="zebra sweeps before/after the coarse grid")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the V-cycle, large V-cycle, 3D and served-sweep measurements")
+                    help="skip the 2D, V-cycle, 3D annulus and served-sweep measurements")
     ap.add_argument("--serve-only", action="store_true",
                     help="measure only the served mixed-size sweep and print its JSON object")
     ap.add_argument("--slots", type=int, default=16, help="request-loop slots of the served sweep")
     ap.add_argument("--direct-cols", type=int, default=0,
-                    help="vv_config.direct_cols of the main leg (0: the library default)")
+                    help="vv_config.direct_cols of a 2D leg (0: the library default)")
     ap.add_argument("--rot3d-grid", default=None, choices=sorted(ROT3D),
                     help="with --rot3d-only: time this grid alone (for profiling)")
     ap.add_argument("--no-baseline", action="store_true",
                     help="with --serve-only: skip the one-at-a-time pass")
     ap.add_argument("--rot3d-only", action="store_true",
-                    help="measure only the 3D step (rot3dmg.f) and print its JSON object")
+                    help="measure only the 3D annulus step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -505,62 +648,48 @@ def main():
         return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
-    cfg, info, wall_ms, ev_ms, prof, nu = gpu_leg(a.workload, a.steps, a.warmup, a.nu, a.direct_cols)
-    ms_per_step = wall_ms / a.steps
-    # dominant kernel by device time in the profiled pass
-    dom = max(prof, key=lambda k: prof[k]["ms"])
-    d = prof[dom]
-    avg_ms = d["ms"] / max(d["launches"], 1)
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    vcyc = {k: prof[k] for k in MG_KERNELS}
-    vc_ms = sum(v["ms"] for v in vcyc.values())
-    vc_bytes = sum(v["bytes"] for v in vcyc.values())
-    traffic = read_traffic(dom, a.workload)
-    line = {
-        "metric": METRIC,
-        "value": a.steps / (wall_ms * 1e-3),
-        "unit": "time-steps/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: seeded linear-in-r temperature + uniform splitmix64 noise, fluid at rest",
-        "config": {"workload": a.workload, "description": WORKLOADS[a.workload][0],
-                   "nr": cfg.nr, "np": cfg.np, "poisson_rtol": cfg.rtol,
-                   "poisson": ("direct solve of the whole plane (real FFT in phi, one PCR line solve "
-                               "per mode; the default up to np = 8192), checked against 1e-10 each "
-                               "step; the V-cycle is under alternatives") if cfg.direct_cols in (0, cfg.np)
-                              else f"V({cfg.nu1},{cfg.nu2}) zebra r-line, phi semi-coarsening, direct level "
-                                   f"at {cfg.direct_cols} columns"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_us": avg_ms * 1e3,
-                     "bytes_per_launch": d["bytes"] / max(d["launches"], 1)},
-        "vcycle_gbs": vc_bytes / (vc_ms * 1e-3) / 1e9 if vc_ms > 0 else None,
-        "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items() if v["launches"]},
-        "gpu_event_ms_per_step": ev_ms / a.steps,
-        "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
-        "nusselt_last_step": nu,
-    }
+
     def done(leg):
         print(f"[bench] {leg} done", file=sys.stderr, flush=True)
 
-    done("time step")
+    line = {"metric": METRIC, "n_gpus": 1, "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64"}
+    if a.workload in TANK_WORKLOADS:
+        cfg, wall_ms, prof, heat, ke = tank_leg(a.workload, a.steps, a.warmup)
+        line.update({
+            "value": a.steps / (wall_ms * 1e-3),
+            "unit": "time-steps/s",
+            "steps": a.steps,
+            "ms_per_step": wall_ms / a.steps,
+            "data": "synthetic: uniform temperature 0.5 plus seeded U(-1e-4, 1e-4), fluid at rest",
+            "config": {"workload": a.workload, "description": TANK_WORKLOADS[a.workload][0],
+                       "nr": cfg.nr, "nz": cfg.nz, "np": cfg.np,
+                       "poisson": "exact solves every step (residual at rounding level)"},
+            "roofline": tank_roofline(prof, a.workload),
+            "kernel_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items() if v["launches"]},
+            "heat_last_step": list(heat),
+            "kinetic_energy": ke,
+        })
+        done("tank step")
+        if not a.no_extra:
+            line["annulus_513x257"] = leg_2d(a, "annulus-513x257")
+            done("2D annulus plane")
+    else:
+        line.update(leg_2d(a, a.workload))
+        line["data"] = "synthetic: seeded linear-in-r temperature + uniform splitmix64 noise, fluid at rest"
+        done("time step")
     if not a.no_extra:
-        line["alternatives"] = {"poisson_vcycle": vcycle_leg(a.workload, a.steps, a.warmup, a.nu)}
+        name2d = a.workload if a.workload in WORKLOADS else "annulus-513x257"
+        line["alternatives"] = {"poisson_vcycle": vcycle_leg(name2d, a.steps, a.warmup, a.nu)}
         done("V-cycle alternative")
         line["vcycle_large"] = vcycle_large_leg()
         done("large V-cycle")
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
-        done("3D step")
+        done("3D annulus step")
         line["served_sweep"] = serve_leg(slots=a.slots)
         done("served sweep")
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_leg(a.workload)
+        line["cpu_baseline"] = tank_cpu_leg(a.workload) if a.workload in TANK_WORKLOADS else cpu_leg(a.workload)
         done("CPU baseline")
     print(json.dumps(line))
 

@@ -49,17 +49,30 @@ inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 // `type` (0: nodes equally spaced, velvor2dperiodicrbconv.f:613-620; 1: walls
 // half a spacing away, its 1085-1097), m unknowns, the lines' other two
 // indices in [1, n1] x [1, n2] (x lines (j, k), y lines (i, j), z lines
-// (i, k)), tables fact | bp | c (1-based, m+1 each).
+// (i, k)), tables fact | bp | c | 1/bp (1-based, m+1 each), nl lines per
+// workgroup (x lines, staged in LDS).
 struct DJob {
     const double *f;
     double *d;
     const double *tab;
-    int axis, type, m, n1, n2;
+    int axis, type, m, n1, n2, nl;
     double dri;
 };
 struct DJobs {
     DJob j[12];
 };
+
+// x / b for a positive pivot b with rb = RN(1/b): q0 = x rb, t = b q0 - x
+// (exact, one fma), q = q0 - t rb: the correctly rounded quotient
+// (Markstein's correction), i.e. the IEEE quotient the reference's division
+// gives, in three dependent operations.  -t keeps the sign of a zero
+// quotient as IEEE division gives it.
+__device__ __forceinline__ double qdiv(double x, double b, double rb)
+{
+    const double q0 = x * rb;
+    const double t = __builtin_fma(b, q0, -x);
+    return __builtin_fma(-t, rb, q0);
+}
 
 __device__ __forceinline__ double line_rhs(int type, int m, int i, double dri, double fm3, double fm1, double f0,
                                            double fp1, double f2, double f4)
@@ -76,40 +89,174 @@ __device__ __forceinline__ double line_rhs(int type, int m, int i, double dri, d
     return 3.0 * (fp1 - fm1) * dri;
 }
 
-__global__ void __launch_bounds__(WG) k_t_deriv(TGeo g, DJobs jobs)
+// Lines along y and z: one thread per line, consecutive threads on
+// consecutive i (coalesced); the eliminated right-hand sides r' go to d and
+// are read back by the back substitution.  Rows are processed in groups of
+// U with their loads issued ahead of the dependent chain.
+__global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0)
 {
-    const DJob J = jobs.j[blockIdx.y];
+    const DJob J = jobs.j[j0 + blockIdx.y];
     const int t = blockIdx.x * WG + threadIdx.x;
     if (t >= J.n1 * J.n2) return;
     const int q = t / J.n1, p = t - q * J.n1;
     const int P1 = p + 1, Q1 = q + 1;
     unsigned base, s;
-    if (J.axis == 0) { base = (unsigned)((Q1 * g.ldj + P1) * g.ldi); s = 1u; }
-    else if (J.axis == 1) { base = (unsigned)(Q1 * g.ldi + P1); s = (unsigned)(g.ldi * g.ldj); }
+    if (J.axis == 1) { base = (unsigned)(Q1 * g.ldi + P1); s = (unsigned)(g.ldi * g.ldj); }
     else { base = (unsigned)(Q1 * g.ldj * g.ldi + P1); s = (unsigned)g.ldi; }
     const double *__restrict__ f = J.f;
     double *__restrict__ d = J.d;
-    const int m = J.m;
-    const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1);
-    const double f4 = (J.type == 1) ? f[base + 4u * s] : 0.0;
-    const double fm3 = (J.type == 1) ? f[base + (unsigned)(m - 3) * s] : 0.0;
-    double fa = f[base], fb = f[base + s], fc = f[base + 2u * s];   // f[i-1], f[i], f[i+1] at i = 1
-    const double f2 = fc;
+    const int m = J.m, type = J.type;
+    const double dri = J.dri;
+    const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
+    const double f4 = (type == 1) ? f[base + 4u * s] : 0.0;
+    const double fm3 = (type == 1) ? f[base + (unsigned)(m - 3) * s] : 0.0;
+    const double f2 = f[base + 2u * s];
+    double fa = f[base], fb = f[base + s];   // f[i-1], f[i]
     double rp = 0.0;
-    for (int i = 1; i <= m; ++i) {
-        const double r = line_rhs(J.type, m, i, J.dri, fm3, fa, fb, fc, f2, f4);
+    constexpr int U = 8;
+    int i = 1;
+    for (; i + U - 1 <= m; i += U) {
+        double nx[U];   // f[i+1 .. i+U]
+#pragma unroll
+        for (int u = 0; u < U; ++u) nx[u] = f[base + (unsigned)(i + 1 + u) * s];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ii = i + u;
+            const double r = line_rhs(type, m, ii, dri, fm3, fa, fb, nx[u], f2, f4);
+            rp = (ii == 1) ? r : r - fact[ii] * rp;
+            d[base + (unsigned)ii * s] = rp;
+            fa = fb;
+            fb = nx[u];
+        }
+    }
+    for (; i <= m; ++i) {
+        const double fc = f[base + (unsigned)(i + 1) * s];
+        const double r = line_rhs(type, m, i, dri, fm3, fa, fb, fc, f2, f4);
         rp = (i == 1) ? r : r - fact[i] * rp;
         d[base + (unsigned)i * s] = rp;
         fa = fb;
         fb = fc;
-        if (i + 2 <= m + 1) fc = f[base + (unsigned)(i + 2) * s];
     }
-    double y = rp / bp[m];
+    double y = qdiv(rp, bp[m], rbp[m]);
     d[base + (unsigned)m * s] = y;
-    for (int i = m - 1; i >= 1; --i) {
+    i = m - 1;
+    for (; i - U + 1 >= 1; i -= U) {
+        double rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) rr[u] = d[base + (unsigned)(i - u) * s];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ii = i - u;
+            y = qdiv(rr[u] - cc[ii] * y, bp[ii], rbp[ii]);
+            d[base + (unsigned)ii * s] = y;
+        }
+    }
+    for (; i >= 1; --i) {
         const unsigned o = base + (unsigned)i * s;
-        y = (d[o] - cc[i] * y) / bp[i];
+        y = qdiv(d[o] - cc[i] * y, bp[i], rbp[i]);
         d[o] = y;
+    }
+}
+
+// Lines along x (contiguous in memory): a workgroup stages nl lines, nodes
+// 0..m+1, in LDS with a row pitch sp = (m+2) | 1 (odd, so the threads' rows
+// fall on different banks) through coalesced loads, thread l < nl runs line
+// l's recurrence in LDS (its nodes are read into registers ahead of the
+// rows that overwrite them), and the interior rows go back coalesced.
+__global__ void __launch_bounds__(64) k_t_deriv_x(TGeo g, DJobs jobs)
+{
+    extern __shared__ double sh[];
+    const DJob J = jobs.j[3 * blockIdx.y];
+    const int m = J.m, nl = J.nl, type = J.type, sp = (m + 2) | 1;
+    const int tiles = (J.n1 + nl - 1) / nl;
+    const int q = blockIdx.x / tiles, p0 = (blockIdx.x - q * tiles) * nl;
+    if (q >= J.n2) return;
+    const int nlines = min(nl, J.n1 - p0);
+    const int tid = threadIdx.x;
+    // line l: (j, k) = (p0 + l + 1, q + 1), nodes contiguous from node 0
+    const unsigned base0 = (unsigned)(((q + 1) * g.ldj + (p0 + 1)) * g.ldi);
+    const double *__restrict__ f = J.f;
+    double *__restrict__ d = J.d;
+    const int len = m + 2, total = nlines * len;
+    // 8 loads in flight per thread before their LDS stores
+    for (int e0 = 0; e0 < total; e0 += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 64 + tid;
+            if (e < total) {
+                const int l = e / len, t = e - l * len;
+                v[u] = f[base0 + (unsigned)(l * g.ldi + t)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 64 + tid;
+            if (e < total) {
+                const int l = e / len, t = e - l * len;
+                sh[l * sp + t] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < nlines) {
+        double *L = sh + tid * sp;
+        const double dri = J.dri;
+        const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
+        const double f4 = (type == 1) ? L[4] : 0.0;
+        const double fm3 = (type == 1) ? L[m - 3] : 0.0;
+        const double f2 = L[2];
+        double fa = L[0], fb = L[1];   // f[i-1], f[i]
+        double rp = 0.0;
+        // rows in groups of U: the group's nodes f[i+1 .. i+U] are read
+        // before any of its rows overwrites L[i .. i+U-1]
+        constexpr int U = 8;
+        int i = 1;
+        for (; i + U - 1 <= m; i += U) {
+            double nx[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) nx[u] = L[i + 1 + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ii = i + u;
+                const double r = line_rhs(type, m, ii, dri, fm3, fa, fb, nx[u], f2, f4);
+                rp = (ii == 1) ? r : r - fact[ii] * rp;
+                L[ii] = rp;
+                fa = fb;
+                fb = nx[u];
+            }
+        }
+        for (; i <= m; ++i) {
+            const double fc = L[i + 1];
+            const double r = line_rhs(type, m, i, dri, fm3, fa, fb, fc, f2, f4);
+            rp = (i == 1) ? r : r - fact[i] * rp;
+            L[i] = rp;
+            fa = fb;
+            fb = fc;
+        }
+        double y = qdiv(rp, bp[m], rbp[m]);
+        L[m] = y;
+        i = m - 1;
+        for (; i - U + 1 >= 1; i -= U) {
+            double rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) rr[u] = L[i - u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ii = i - u;
+                y = qdiv(rr[u] - cc[ii] * y, bp[ii], rbp[ii]);
+                L[ii] = y;
+            }
+        }
+        for (; i >= 1; --i) {
+            y = qdiv(L[i] - cc[i] * y, bp[i], rbp[i]);
+            L[i] = y;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nlines * m; e += 64) {
+        const int l = e / m, t = 1 + (e - l * m);
+        d[base0 + (unsigned)(l * g.ldi + t)] = sh[l * sp + t];
     }
 }
 
@@ -311,40 +458,44 @@ __global__ void __launch_bounds__(WG) k_t_temp(TGeo g, VF F)
     X(F.thn, i, j, k) = (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
 }
 
-// heat(trans1,trans2), velvor2dhorconvtank.f:1589-1620: one thread per x
-// column sums its k range in the routine's order; k_t_heat_sum adds the
-// columns in order of i
-__global__ void __launch_bounds__(WG) k_t_heat(TGeo g, const double *th, double *cols)
+// heat(trans1,trans2), velvor2dhorconvtank.f:1589-1620.  Its k-sum adds, for
+// each k, the row's term at k and at k+1, each 0.5 dp (Fornberg row) dzi;
+// k_t_heat_terms evaluates every (i, k) term once, in parallel (the term at
+// k+1 of step k is the term at k of step k+1: the same operations), and
+// k_t_heat_cols adds them per x column in the routine's order.
+__global__ void __launch_bounds__(WG) k_t_heat_terms(TGeo g, const double *th, double *E)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
-    const int i = 1 + blockIdx.x * WG + threadIdx.x;
+    const int i = 1 + blockIdx.x * WG + threadIdx.x, kk = blockIdx.y, k = np / 2 + kk;
     if (i > nr) return;
     const double dp = g.dp, dzi = 1.0 / g.dz;
+    const size_t n = (size_t)(np - np / 2 + 1) * nr;   // rows k = np/2 .. np
+    E[(size_t)kk * nr + (i - 1)] = 0.5 * dp *
+                                    (-3.35238095 * X(th, i, 0, k) + 4.375 * X(th, i, 1, k) -
+                                     1.45833333 * X(th, i, 2, k) + 0.525 * X(th, i, 3, k) -
+                                     0.08928571 * X(th, i, 4, k)) *
+                                    dzi;
+    E[n + (size_t)kk * nr + (i - 1)] = 0.5 * dp *
+                                        (-3.35238095 * X(th, i, nz + 1, k) + 4.375 * X(th, i, nz, k) -
+                                         1.45833333 * X(th, i, nz - 1, k) + 0.525 * X(th, i, nz - 2, k) -
+                                         0.08928571 * X(th, i, nz - 3, k)) *
+                                        dzi;
+}
+
+__global__ void __launch_bounds__(WG) k_t_heat_cols(TGeo g, const double *E, double *cols)
+{
+    const int nr = g.nr, np = g.np;
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= nr) return;
+    const size_t n = (size_t)(np - np / 2 + 1) * nr;
     double trans1 = 0.0, trans2 = 0.0;
-    for (int k = np / 2; k <= np - 1; ++k) {
-        const int ka = k + 1;
-        trans1 = trans1 +
-                 0.5 * dp *
-                     (-3.35238095 * X(th, i, 0, k) + 4.375 * X(th, i, 1, k) - 1.45833333 * X(th, i, 2, k) +
-                      0.525 * X(th, i, 3, k) - 0.08928571 * X(th, i, 4, k)) *
-                     dzi +
-                 0.5 * dp *
-                     (-3.35238095 * X(th, i, 0, ka) + 4.375 * X(th, i, 1, ka) - 1.45833333 * X(th, i, 2, ka) +
-                      0.525 * X(th, i, 3, ka) - 0.08928571 * X(th, i, 4, ka)) *
-                     dzi;
-        trans2 = trans2 +
-                 0.5 * dp *
-                     (-3.35238095 * X(th, i, nz + 1, k) + 4.375 * X(th, i, nz, k) - 1.45833333 * X(th, i, nz - 1, k) +
-                      0.525 * X(th, i, nz - 2, k) - 0.08928571 * X(th, i, nz - 3, k)) *
-                     dzi +
-                 0.5 * dp *
-                     (-3.35238095 * X(th, i, nz + 1, ka) + 4.375 * X(th, i, nz, ka) -
-                      1.45833333 * X(th, i, nz - 1, ka) + 0.525 * X(th, i, nz - 2, ka) -
-                      0.08928571 * X(th, i, nz - 3, ka)) *
-                     dzi;
+    for (int kk = 0; kk < np - np / 2; ++kk) {
+        const size_t o = (size_t)kk * nr + i;
+        trans1 = trans1 + E[o] + E[o + nr];
+        trans2 = trans2 + E[n + o] + E[n + o + nr];
     }
-    cols[2 * (i - 1)] = trans1;
-    cols[2 * (i - 1) + 1] = trans2;
+    cols[2 * i] = trans1;
+    cols[2 * i + 1] = trans2;
 }
 
 __global__ void k_t_heat_sum(int nr, double dr, const double *cols, double *out)
@@ -557,6 +708,62 @@ struct YInv {
     }
 };
 
+// Even np: the y modes split by parity.  A mode's values at the mirrored
+// planes k and k' (k' = np+1-k for cells, np-k for faces) are equal or
+// opposite, so each transform is two half-size products: spectral row
+// q = 2p + b (b = 0, 1) of the forward takes the sums (b = 0) or the
+// differences (b = 1) of the mirrored planes k = 1..np/2 (for faces the
+// middle plane np/2 is its own mirror: its sum counts it twice, and its
+// weight is halved); the inverse forms E (b = 0) and O (b = 1) on the
+// planes k = 1..np/2, and psi(k) = E + O, psi(k') = E - O.
+struct YFwdSplit {
+    const double *W;        // [2][h][h]
+    int M, N, K;            // h = np/2, mt, h
+    const double *rhs;
+    double *S;
+    int i1, ldi, ldj, np, faces, k1;
+    __device__ double a(int b, int m, int k) const { return W[((size_t)b * M + m) * K + k]; }
+    __device__ double b(int b, int k, int n) const
+    {
+        const int j = n / i1, i = n - j * i1;
+        const size_t off = (size_t)(j + 1) * ldi + (size_t)(i + 1), pl = (size_t)ldj * ldi;
+        const int kk = k + 1, km = faces ? np - kk : np + 1 - kk;
+        const double x = rhs[(size_t)kk * pl + off], y = rhs[(size_t)km * pl + off];
+        return b ? x - y : x + y;
+    }
+    __device__ void c(int b, int m, int n, double v) const
+    {
+        const int q = 2 * m + b;
+        if (q < k1) S[(size_t)q * N + n] = v;
+    }
+};
+struct YInvSplit {
+    const double *Wi;       // [2][h][h]
+    int M, N, K;            // h, mt, h
+    const double *S;
+    double *EO;             // [2][h][mt]
+    int k1;
+    __device__ double a(int b, int m, int k) const { return Wi[((size_t)b * M + m) * K + k]; }
+    __device__ double b(int b, int k, int n) const
+    {
+        const int q = 2 * k + b;
+        return (q < k1) ? S[(size_t)q * N + n] : 0.0;
+    }
+    __device__ void c(int b, int m, int n, double v) const { EO[((size_t)b * M + m) * N + n] = v; }
+};
+__global__ void __launch_bounds__(WG) k_t_ycombine(TGeo g, const double *EO, double *psi, int i1, int mt, int faces)
+{
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= mt) return;
+    const int h = g.np / 2, m = blockIdx.y, kk = m + 1;
+    const int j = n / i1, i = n - j * i1;
+    const unsigned off = (unsigned)((j + 1) * g.ldi + (i + 1)), pl = (unsigned)(g.ldi * g.ldj);
+    const double E = EO[(size_t)m * mt + n], O = EO[((size_t)h + m) * mt + n];
+    const int km = faces ? g.np - kk : g.np + 1 - kk;
+    psi[(unsigned)kk * pl + off] = E + O;
+    if (km != kk) psi[(unsigned)km * pl + off] = E - O;
+}
+
 // psip <- psi everywhere; psi <- 0 outside the unknowns
 __global__ void __launch_bounds__(WG) k_t_psi_pre(TGeo g, int i1, int j1, int k1, double *psi, double *psip)
 {
@@ -606,7 +813,26 @@ __global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double 
     y /= d0;
     c[0] = cprev;
     x[0] = y;
-    for (int i = 1; i < i1; ++i) {
+    // rows 1 .. i1-2 in groups of U with their loads issued ahead
+    constexpr int U = 8;
+    int i = 1;
+    for (; i + U <= i1 - 1; i += U) {
+        double ru[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ru[u] = x[(size_t)(i + u) * j1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ii = i + u;
+            const size_t o = (size_t)ii * j1;
+            const double a = sub[ii];
+            const double den = dia[ii] + shift - a * cprev;
+            cprev = sup[ii] / den;
+            y = (ru[u] - a * y) / den;
+            c[o] = cprev;
+            x[o] = y;
+        }
+    }
+    for (; i < i1; ++i) {
         const size_t o = (size_t)i * j1;
         double a = sub[i], r = x[o];
         if (i == i1 - 1 && en != 0.0) {
@@ -620,7 +846,22 @@ __global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double 
         c[o] = cprev;
         x[o] = y;
     }
-    for (int i = i1 - 2; i >= 0; --i) {
+    i = i1 - 2;
+    for (; i - U + 1 >= 0; i -= U) {
+        double fu[U], xu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t o = (size_t)(i - u) * j1;
+            fu[u] = c[o];
+            xu[u] = x[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            y = xu[u] - fu[u] * y;
+            x[(size_t)(i - u) * j1] = y;
+        }
+    }
+    for (; i >= 0; --i) {
         const size_t o = (size_t)i * j1;
         y = x[o] - c[o] * y;
         x[o] = y;
@@ -664,7 +905,8 @@ using namespace vvt;
 struct TankOp {
     int i1 = 0, j1 = 0, k1 = 0, m = 0;
     double e1 = 0.0, en = 0.0;
-    double *W = nullptr, *Wi = nullptr;   // k1 x k1: y modes (forward, inverse)
+    double *W = nullptr, *Wi = nullptr;   // k1 x k1: y modes (forward, inverse); for even np the
+                                          // parity-split halves [2][np/2][np/2] instead
     double *ly = nullptr;                 // k1: y eigenvalues / dp^2
     double *V = nullptr, *Vinv = nullptr; // j1 x j1: z eigenvectors (row-major) and inverse
     double *lz = nullptr;                 // j1: z eigenvalues / dz^2
@@ -685,7 +927,7 @@ struct vvt_run {
     size_t tab_off[12] = {0};
     TankOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;   // Poisson work (kmax x mmax each)
-    double *heat = nullptr;       // 2 nr column sums, then trans1, trans2
+    double *heat = nullptr;       // 2 nr column sums, trans1, trans2, then the flux terms
     double *kepart = nullptr;     // KE_MAXB partials, then the total
     // profiling
     bool prof = false;
@@ -728,9 +970,12 @@ void compact_tables(int type, int m, std::vector<double> &out)
         fact[j] = a[j] / b[j - 1];
         b[j] = b[j] - fact[j] * c[j - 1];
     }
+    std::vector<double> rb(m + 1, 0.0);
+    for (int j = 1; j <= m; ++j) rb[j] = 1.0 / b[j];
     out.insert(out.end(), fact.begin(), fact.end());
     out.insert(out.end(), b.begin(), b.end());
     out.insert(out.end(), c.begin(), c.end());
+    out.insert(out.end(), rb.begin(), rb.end());
 }
 
 // (field, axis) -> (type, m, n1, n2): the staggering of A, B, C, th
@@ -776,15 +1021,38 @@ int setup_poisson(vvt_run *r)
         // is an eigenvector of the y rows with eigenvalue 2 cos(pi q/np) - 2
         const int k1 = O.k1;
         std::vector<double> W((size_t)k1 * k1), Wi((size_t)k1 * k1), ly(k1);
+        auto mode = [&](int q, int k) {   // mode q (0-based spectral row) at plane k
+            const int qq = (w == 2) ? q + 1 : q;
+            return (w == 2) ? std::sin(pi * qq * k / np) : std::cos(pi * qq * (k - 0.5) / np);
+        };
+        auto norm = [&](int q) { return (w != 2 && q == 0) ? (double)np : 0.5 * np; };
         for (int qi = 0; qi < k1; ++qi) {
             const int q = (w == 2) ? qi + 1 : qi;
             ly[qi] = (2.0 * std::cos(pi * q / np) - 2.0) / hhp;
-            const double norm = (w != 2 && q == 0) ? (double)np : 0.5 * np;
             for (int k = 1; k <= k1; ++k) {
-                const double e = (w == 2) ? std::sin(pi * q * k / np) : std::cos(pi * q * (k - 0.5) / np);
-                W[(size_t)qi * k1 + (k - 1)] = e;
-                Wi[(size_t)(k - 1) * k1 + qi] = e / norm;
+                W[(size_t)qi * k1 + (k - 1)] = mode(qi, k);
+                Wi[(size_t)(k - 1) * k1 + qi] = mode(qi, k) / norm(qi);
             }
+        }
+        if (np % 2 == 0) {
+            // the parity-split halves: forward row p of half b is spectral row
+            // q = 2p + b at the planes k = 1..np/2 (the faces' middle plane
+            // np/2 at half weight); inverse row k of half b holds the inverse
+            // weights of the rows q = 2p + b at plane k
+            const int h = np / 2;
+            std::vector<double> Ws((size_t)2 * h * h, 0.0), Wis((size_t)2 * h * h, 0.0);
+            for (int b = 0; b < 2; ++b)
+                for (int pp = 0; pp < h; ++pp) {
+                    const int q = 2 * pp + b;
+                    if (q >= k1) continue;
+                    for (int k = 1; k <= h; ++k) {
+                        const double e = mode(q, k);
+                        Ws[((size_t)b * h + pp) * h + (k - 1)] = (w == 2 && k == h) ? 0.5 * e : e;
+                        Wis[((size_t)b * h + (k - 1)) * h + pp] = e / norm(q);
+                    }
+                }
+            W.swap(Ws);
+            Wi.swap(Wis);
         }
         VV_HIP(upload(&O.W, W));
         VV_HIP(upload(&O.Wi, Wi));
@@ -858,41 +1126,77 @@ VF vfields(vvt_run *r)
     return F;
 }
 
+// x lines staged in LDS: at most 32 KB per workgroup
+int x_lines_per_group(int m) { return std::max(1, std::min(64, 4096 / ((m + 2) | 1))); }
+
 int launch_derivatives(vvt_run *r)
 {
-    Prof p(r, VVT_K_DERIV, 8.0 * 24.0 * npts(r));
     DJobs J;
-    int nmax = 0;
+    int nyz = 0, nx = 0;
+    double byz = 0.0, bx = 0.0;
     const double *src[4] = {r->f[VV3_A], r->f[VV3_B], r->f[VV3_C], r->f[VV3_TH]};
     const double dri[3] = {1.0 / r->dr, 1.0 / r->dp, 1.0 / r->dz};
+    size_t xlds = 0;
     for (int d = 0; d < 12; ++d) {
         const DSpec s = dspec(r, d);
-        J.j[d] = DJob{src[d / 3], r->dv[d], r->tabs + r->tab_off[d], d % 3, s.type, s.m, s.n1, s.n2, dri[d % 3]};
-        nmax = std::max(nmax, s.n1 * s.n2);
+        const int nl = (d % 3 == 0) ? x_lines_per_group(s.m) : 1;
+        J.j[d] = DJob{src[d / 3], r->dv[d], r->tabs + r->tab_off[d], d % 3, s.type, s.m, s.n1, s.n2, nl, dri[d % 3]};
+        const double b = 8.0 * (2.0 * s.m + 2.0) * s.n1 * s.n2;
+        if (d % 3 == 0) bx += b;
+        else byz += b;
+        if (d % 3 == 0) {
+            nx = std::max(nx, ((s.n1 + nl - 1) / nl) * s.n2);
+            xlds = std::max(xlds, (size_t)nl * ((s.m + 2) | 1) * sizeof(double));
+        } else {
+            nyz = std::max(nyz, s.n1 * s.n2);
+        }
     }
-    hipLaunchKernelGGL(k_t_deriv, dim3(nblk(nmax, WG), 12), dim3(WG), 0, r->stream, geo(r), J);
+    // the y and z jobs in one launch: jobs 1, 2, 4, 5, ... are reached as
+    // j0 + blockIdx.y with blockIdx.y over a compacted list
+    DJobs Jyz;
+    int c = 0;
+    for (int d = 0; d < 12; ++d)
+        if (d % 3 != 0) Jyz.j[c++] = J.j[d];
+    {
+        Prof p(r, VVT_K_DERIV_YZ, byz);
+        hipLaunchKernelGGL(k_t_deriv_yz, dim3(nblk(nyz, WG), c), dim3(WG), 0, r->stream, geo(r), Jyz, 0);
+    }
+    {
+        Prof p(r, VVT_K_DERIV_X, bx);
+        hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), xlds, r->stream, geo(r), J);
+    }
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
 int launch_vorticity(vvt_run *r)
 {
-    Prof p(r, VVT_K_VORT, 8.0 * 35.0 * npts(r));
     const TGeo g = geo(r);
     const VF F = vfields(r);
-    hipLaunchKernelGGL(k_t_avort, dim3(nblk((long long)r->nr * (r->nz - 1), WG), r->np - 1), dim3(WG), 0, r->stream,
-                       g, F);
-    hipLaunchKernelGGL(k_t_bvort, dim3(nblk((long long)(r->nr - 1) * (r->nz - 1), WG), r->np), dim3(WG), 0,
-                       r->stream, g, F);
-    hipLaunchKernelGGL(k_t_cvort, dim3(nblk((long long)(r->nr - 1) * r->nz, WG), r->np - 1), dim3(WG), 0, r->stream,
-                       g, F);
+    const double pa = (double)r->nr * (r->nz - 1) * (r->np - 1), pb = (double)(r->nr - 1) * (r->nz - 1) * r->np,
+                 pc = (double)(r->nr - 1) * r->nz * (r->np - 1);
+    {   // reads A, Ap, B, C, th, u, v, w and three derivatives; writes An
+        Prof p(r, VVT_K_AVORT, 8.0 * 12.0 * pa);
+        hipLaunchKernelGGL(k_t_avort, dim3(nblk((long long)r->nr * (r->nz - 1), WG), r->np - 1), dim3(WG), 0,
+                           r->stream, g, F);
+    }
+    {
+        Prof p(r, VVT_K_BVORT, 8.0 * 12.0 * pb);
+        hipLaunchKernelGGL(k_t_bvort, dim3(nblk((long long)(r->nr - 1) * (r->nz - 1), WG), r->np), dim3(WG), 0,
+                           r->stream, g, F);
+    }
+    {   // no temperature term
+        Prof p(r, VVT_K_CVORT, 8.0 * 11.0 * pc);
+        hipLaunchKernelGGL(k_t_cvort, dim3(nblk((long long)(r->nr - 1) * r->nz, WG), r->np - 1), dim3(WG), 0,
+                           r->stream, g, F);
+    }
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
 int launch_temperature(vvt_run *r)
 {
-    Prof p(r, VVT_K_TEMP, 8.0 * 9.0 * npts(r));
+    Prof p(r, VVT_K_TEMP, 8.0 * 9.0 * npts(r));   // th, thp, u, v, w, three derivatives; thn
     hipLaunchKernelGGL(k_t_temp, dim3(nblk((long long)r->nr * r->nz, WG), r->np), dim3(WG), 0, r->stream, geo(r),
                        vfields(r));
     VV_HIP(hipGetLastError());
@@ -903,9 +1207,12 @@ double *heat_out(vvt_run *r) { return r->heat + 2 * r->nr; }
 
 int launch_heat(vvt_run *r)
 {
-    Prof p(r, VVT_K_HEAT, 8.0 * 10.0 * r->nr * (r->np / 2));
-    hipLaunchKernelGGL(k_t_heat, dim3(nblk(r->nr, WG)), dim3(WG), 0, r->stream, geo(r),
-                       (const double *)r->f[VV3_TH], r->heat);
+    Prof p(r, VVT_K_HEAT, 8.0 * 10.0 * r->nr * (r->np / 2 + 1));
+    double *E = r->heat + 2 * (size_t)r->nr + 2;
+    hipLaunchKernelGGL(k_t_heat_terms, dim3(nblk(r->nr, WG), r->np - r->np / 2 + 1), dim3(WG), 0, r->stream, geo(r),
+                       (const double *)r->f[VV3_TH], E);
+    hipLaunchKernelGGL(k_t_heat_cols, dim3(nblk(r->nr, WG)), dim3(WG), 0, r->stream, geo(r), (const double *)E,
+                       r->heat);
     hipLaunchKernelGGL(k_t_heat_sum, dim3(1), dim3(64), 0, r->stream, r->nr, r->dr, (const double *)r->heat,
                        heat_out(r));
     VV_HIP(hipGetLastError());
@@ -926,26 +1233,56 @@ int launch_rhs(vvt_run *r, int which)
 int launch_poisson(vvt_run *r, int which)
 {
     const TankOp &O = r->op[which - 1];
-    const double flops = 2.0 * (2.0 * O.k1 * O.k1 * O.m) + 2.0 * (2.0 * O.j1 * O.j1 * O.i1 * O.k1);
-    Prof p(r, VVT_K_POISSON, flops);
     const int psi = (which == 1) ? VV3_WEL : (which == 2) ? VV3_VEL : VV3_UEL;
     const int psip = (which == 1) ? VV3_WELP : (which == 2) ? VV3_VELP : VV3_UELP;
     const int rhs = (which == 1) ? VV3_WRH : (which == 2) ? VV3_VRH : VV3_URH;
     const TGeo g = geo(r);
     hipStream_t s = r->stream;
     double *P = r->f[psi];
-    hipLaunchKernelGGL(k_t_psi_pre, dim3(nblk((long long)r->ldi * r->ldj, WG), r->np + 2), dim3(WG), 0, s, g, O.i1,
-                       O.j1, O.k1, P, r->f[psip]);
     const int mt = O.m;
-    vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->S, O.i1, r->ldi, r->ldj}, 1, s);
-    vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->S, r->T, mt}, O.k1, s);
-    hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
-                       (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
-                       O.en);
-    vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->T, r->S, mt}, O.k1, s);
-    vv::gemm_f64<true, false>(YInv{O.Wi, O.k1, mt, O.k1, r->S, P, O.i1, r->ldi, r->ldj}, 1, s);
-    if (which != 2)
-        hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
+    const double ftr_y = 2.0 * O.k1 * O.k1 * (double)mt, ftr_z = 2.0 * O.j1 * O.j1 * (double)O.i1 * O.k1;
+    {
+        Prof p(r, VVT_K_PSI_PRE, 8.0 * 3.0 * (double)r->nf);
+        hipLaunchKernelGGL(k_t_psi_pre, dim3(nblk((long long)r->ldi * r->ldj, WG), r->np + 2), dim3(WG), 0, s, g,
+                           O.i1, O.j1, O.k1, P, r->f[psip]);
+    }
+    const bool split = (r->np % 2 == 0);
+    const int h = r->np / 2;
+    {
+        Prof p(r, VVT_K_YFWD, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
+        if (split)
+            vv::gemm_f64<true, false>(YFwdSplit{O.W, h, mt, h, r->f[rhs], r->S, O.i1, r->ldi, r->ldj, r->np,
+                                                which == 2, O.k1}, 2, s);
+        else
+            vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->S, O.i1, r->ldi, r->ldj}, 1, s);
+    }
+    {
+        Prof p(r, VVT_K_ZFWD, ftr_z);
+        vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->S, r->T, mt}, O.k1, s);
+    }
+    {   // reads and writes T, writes the factors, reads them back
+        Prof p(r, VVT_K_XSOLVE, 8.0 * 5.0 * O.k1 * (double)mt);
+        hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
+                           (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
+                           O.en);
+    }
+    {
+        Prof p(r, VVT_K_ZINV, ftr_z);
+        vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->T, r->S, mt}, O.k1, s);
+    }
+    {
+        Prof p(r, VVT_K_YINV, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
+        if (split) {
+            // E and O go to T (free after the z inverse), then the planes
+            vv::gemm_f64<true, false>(YInvSplit{O.Wi, h, mt, h, r->S, r->T, O.k1}, 2, s);
+            hipLaunchKernelGGL(k_t_ycombine, dim3(nblk(mt, WG), h), dim3(WG), 0, s, g, (const double *)r->T, P, O.i1,
+                               mt, which == 2);
+        } else {
+            vv::gemm_f64<true, false>(YInv{O.Wi, O.k1, mt, O.k1, r->S, P, O.i1, r->ldi, r->ldj}, 1, s);
+        }
+        if (which != 2)
+            hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
+    }
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -1076,9 +1413,11 @@ int vvt_create(const vvt_config *cfg, int device, vvt_run **out)
         }
         if (upload(&r->tabs, tabs) != hipSuccess) return fail(VV_ENOMEM);
     }
-    if (hipMalloc(&r->heat, (2 * (size_t)r->nr + 2) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
-    if (hipMemsetAsync(r->heat, 0, (2 * (size_t)r->nr + 2) * sizeof(double), r->stream) != hipSuccess)
-        return fail(VV_EHIP);
+    {   // column sums (2 nr), trans1 and trans2, then the terms of rows k = np/2 .. np
+        const size_t nh = 2 * (size_t)r->nr + 2 + 2 * (size_t)(cfg->np - cfg->np / 2 + 1) * r->nr;
+        if (hipMalloc(&r->heat, nh * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+        if (hipMemsetAsync(r->heat, 0, nh * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+    }
     if (hipMalloc(&r->kepart, (KE_MAXB + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipEventCreate(&r->ev[0]) != hipSuccess || hipEventCreate(&r->ev[1]) != hipSuccess) return fail(VV_EHIP);
     {

@@ -17,7 +17,10 @@ from ._lib import check, lib
 from .velvor3 import FIELD_IDS, FIELDS  # noqa: F401  (the tank uses rot3dmg.f's fields)
 
 _P = ctypes.POINTER(ctypes.c_double)
-KERNELS = ["deriv", "vort", "temp", "heat", "rhs", "poisson", "update"]
+KERNELS = ["deriv_yz", "deriv_x", "avort", "bvort", "cvort", "temp", "heat", "rhs", "psi_pre", "yfwd", "zfwd",
+           "xsolve", "zinv", "yinv", "update"]
+# the work of these kernels is counted in floating-point operations, of the others in bytes
+FLOP_KERNELS = ("yfwd", "zfwd", "zinv", "yinv")
 
 
 class VVTConfig(ctypes.Structure):
@@ -149,5 +152,6 @@ class Tank:
             n, ms, b = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
             check(self._lib.vvt_profile_read(self._h, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
                   "vvt_profile_read")
-            out[name] = {"launches": n.value, "ms": ms.value, "bytes": b.value}
+            unit = "flops" if name in FLOP_KERNELS else "bytes"
+            out[name] = {"launches": n.value, "ms": ms.value, unit: b.value}
         return out

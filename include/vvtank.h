@@ -89,16 +89,27 @@ int vvt_step(vvt_run *run, int n, double *trans1, double *trans2);
  * the three solves (no counterpart in the reference) */
 int vvt_energy(vvt_run *run, double *ke);
 
-/* Per-kernel device time of the following vvt_step calls (HIP events on
- * the run's stream); kernel ids: */
+/* Per-kernel device time of the following calls (HIP events around each
+ * launch on the run's stream; no counterpart in the reference).  For each
+ * kernel: launches, summed milliseconds and the summed algorithmic work of
+ * those launches -- bytes of the fields read and written once each, or, for
+ * the transform GEMMs, floating-point operations (2 M N K). */
 enum vvt_kernel_id {
-    VVT_K_DERIV = 0,   /* compact derivatives                          */
-    VVT_K_VORT,        /* A, B, C updates                              */
-    VVT_K_TEMP,        /* temperature update                           */
-    VVT_K_HEAT,        /* heat fluxes                                  */
-    VVT_K_RHS,         /* right-hand sides                             */
-    VVT_K_POISSON,     /* the three solves (transforms and x solves)   */
-    VVT_K_UPDATE,      /* boundaries and the rotation copy             */
+    VVT_K_DERIV_YZ = 0,  /* compact derivatives along y and z (one thread per line) */
+    VVT_K_DERIV_X,       /* compact derivatives along x (lines staged in LDS)       */
+    VVT_K_AVORT,         /* A update                                                */
+    VVT_K_BVORT,         /* B update                                                */
+    VVT_K_CVORT,         /* C update                                                */
+    VVT_K_TEMP,          /* temperature update                                      */
+    VVT_K_HEAT,          /* heat fluxes                                             */
+    VVT_K_RHS,           /* right-hand sides                                        */
+    VVT_K_PSI_PRE,       /* psip <- psi, walls cleared                              */
+    VVT_K_YFWD,          /* y modes, forward (GEMM; work in flops)                  */
+    VVT_K_ZFWD,          /* z eigenbasis, forward (GEMM; flops)                     */
+    VVT_K_XSOLVE,        /* banded solves along x                                   */
+    VVT_K_ZINV,          /* z eigenbasis, inverse (GEMM; flops)                     */
+    VVT_K_YINV,          /* y modes, inverse (GEMM; flops)                          */
+    VVT_K_UPDATE,        /* boundaries and the rotation copy                        */
     VVT_K_COUNT
 };
 int vvt_profile(vvt_run *run, int enable);

@@ -175,3 +175,111 @@ class Tank:
             self.poisson(which)
         self.call("update")
         return h
+
+
+# ---------------------------------------------------------------- fast solve
+# The exact solve of the same operators at sizes the dense LU cannot reach:
+# along y the cosine modes (u, w) or sine modes (v) by scipy's DCT-II / DST-I,
+# along z and x the eigenbases of the operator's rows (numpy), then a
+# division by the sum of the three eigenvalues.  Pinned against the dense
+# solve in tests/test_oracle_tank.py.
+
+def _rows_matrix(kind: int, n: int, h2: float) -> np.ndarray:
+    """The rows of one direction as vt_apply forms them (kind 0 Dirichlet,
+    1 the no-slip closure, 2 mirrored ghosts)."""
+    M = np.zeros((n, n))
+    for t in range(n):
+        M[t, t] = -2.0
+        if t > 0:
+            M[t, t - 1] = 1.0
+        if t < n - 1:
+            M[t, t + 1] = 1.0
+    if kind == 1:
+        M[0, :3] = [-2.0, 5.0 / 3.0, -0.2]
+        M[-1, -3:] = [-0.2, 5.0 / 3.0, -2.0]
+    elif kind == 2:
+        M[0, 0] = M[-1, -1] = -1.0
+    return M / h2
+
+
+def _kinds(which):
+    return (0 if which == 3 else 1), (0 if which == 1 else 1), (0 if which == 2 else 2)
+
+
+class FastSolver:
+    """Exact separable solves of a Tank's three operators."""
+
+    def __init__(self, tank: "Tank"):
+        self.t = tank
+        self.ops = {}
+        p = tank.p
+        for which in (1, 2, 3):
+            i1, j1, k1 = tank.ranges(which)
+            kx, kz, ky = _kinds(which)
+            lx, Vx = np.linalg.eig(_rows_matrix(kx, i1, p.dr * p.dr))
+            lz, Vz = np.linalg.eig(_rows_matrix(kz, j1, p.dz * p.dz))
+            assert np.max(np.abs(lx.imag)) == 0.0 and np.max(np.abs(lz.imag)) == 0.0
+            lx, Vx, lz, Vz = lx.real, Vx.real, lz.real, Vz.real
+            q = np.arange(1, k1 + 1) if which == 2 else np.arange(k1)
+            ly = (2.0 * np.cos(np.pi * q / p.np) - 2.0) / (p.dp * p.dp)
+            self.ops[which] = (lx, Vx, np.linalg.inv(Vx), lz, Vz, np.linalg.inv(Vz), ly)
+
+    def solve(self, which, rhs):
+        import scipy.fft as sf
+        t = self.t
+        i1, j1, k1 = t.ranges(which)
+        lx, Vx, Vxi, lz, Vz, Vzi, ly = self.ops[which]
+        f = rhs[1:k1 + 1, 1:j1 + 1, 1:i1 + 1]
+        s = sf.dst(f, type=1, axis=0) if which == 2 else sf.dct(f, type=2, axis=0)
+        s = np.matmul(Vzi, s)                       # along j
+        s = np.matmul(s, Vxi.T)                     # along i
+        s /= (ly[:, None, None] + lz[None, :, None] + lx[None, None, :])
+        s = np.matmul(s, Vx.T)
+        s = np.matmul(Vz, s)
+        x = sf.idst(s, type=1, axis=0) if which == 2 else sf.idct(s, type=2, axis=0)
+        psi = np.zeros(t.shape)
+        psi[1:k1 + 1, 1:j1 + 1, 1:i1 + 1] = x
+        t.psi_bounds(which, psi)
+        return psi
+
+
+def time_step(npz_in: str, json_out: str) -> None:
+    """A bounded sample of one step of the oracle on the state in npz_in, on
+    one thread: the explicit routines of the whole step in C (derivatives,
+    A/B/C/T updates, heat, the three right-hand sides, update) and one of its
+    three exact fast solves (all three cost the same: the transforms of
+    equal-sized operators).  Writes the seconds of each part and the step
+    time they give (explicit + 3 solves)."""
+    import json
+    import time
+    from threadpoolctl import threadpool_limits
+    z = np.load(npz_in)
+    nr, nz, np_ = int(z["nr"]), int(z["nz"]), int(z["np"])
+    t = Tank(nr, nz, np_, float(z["dt"]), float(z["Re"]), float(z["Ptl"]), float(z["Gr"]), lx=float(z["lx"]),
+             lz=float(z["lz"]), ly=float(z["ly"]))
+    for n in FIELDS:
+        if "f_" + n in z:
+            t[n] = z["f_" + n]
+    with threadpool_limits(limits=1):
+        fs = FastSolver(t)
+        t0 = time.perf_counter()
+        t.derivatives()
+        for rt in ("avorticity", "bvorticity", "cvorticity", "temperature"):
+            t.call(rt)
+        h = t.heat()
+        for rhs in ("rhsw", "rhsv", "rhsu"):
+            t.call(rhs)
+        t.call("update")
+        t1 = time.perf_counter()
+        psi, psip, f = PSI[2]
+        t[psip] = t[psi]
+        t[psi] = fs.solve(2, t[f])
+        t2 = time.perf_counter()
+    with open(json_out, "w") as fo:
+        json.dump({"explicit_s": t1 - t0, "solve_s": t2 - t1, "step_s": (t1 - t0) + 3.0 * (t2 - t1),
+                   "heat": list(h)}, fo)
+
+
+if __name__ == "__main__":
+    import sys
+    time_step(sys.argv[1], sys.argv[2])
