@@ -8,7 +8,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vvmi.h")
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h", "vvband.h", "vvtank.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h", "vvband.h", "vvtank.h", "vvmerid.h")]
 LIB = os.path.join(ROOT, "codesthesis_dir_amd", "libvvmi.so")
 
 
@@ -16,7 +16,7 @@ def declared():
     names = set()
     for h in HEADERS:
         src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b(vv[3t]?_[a-z0-9_]+)\s*\(", src))
+        names |= set(re.findall(r"\b(vv[3tm]?_[a-z0-9_]+)\s*\(", src))
     return sorted(names)
 
 
