@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-extra]
     python bench.py --rot3d-only [--no-cpu]      # the 3D annulus step (rot3dmg.f) alone
     python bench.py --serve-only                 # the served mixed-size sweep alone
+    python bench.py --merid-only [--no-cpu]      # the meridional plane's 5000-step run alone
 
 The default workload is config 5, the largest single-GPU configuration: the 3D
 horizontal-convection tank of 512 x 256 x 256 intervals (aspect 4:1:1),
@@ -15,9 +16,10 @@ second pass of K steps with HIP events around every launch gives the
 dominant kernel's average duration and its achieved algorithmic bandwidth
 (or, for a transform GEMM, flop rate).  The CPU baseline (rank 0, N=1 only)
 times the oracle's step of the same workload on this host.  Unless
---no-extra is given, the line also carries the 2D legs (the rotating-annulus
-plane at 513x257 with its V-cycle alternative, a large V-cycle), the 3D
-annulus of rot3dmg.f and the served sweep (config 4).
+--no-extra is given, the line also carries the meridional plane with swirl
+on stretched nodes (config 2: its whole 5000-step run at 513 x 257), the 2D
+legs (the rotating-annulus plane at 513x257 with its V-cycle alternative, a
+large V-cycle), the 3D annulus of rot3dmg.f and the served sweep (config 4).
 
 The 2D workloads (--workload box-129x65, annulus-513x257, ...) keep their
 own headline form; every 2D leg refuses a run whose Poisson solves did not
@@ -465,6 +467,87 @@ def tank_leg(name, steps, warmup):
     return cfg, wall_ms, prof, h, ke
 
 
+# row f3 / config 2: the meridional (r,z) plane with swirl on tanh-clustered
+# nodes, 513 x 257 nodes, its whole 5000-step run timed
+MERID = ("merid-513x257",
+         "rotating-annulus meridional (r,z) plane with swirl (velocity-vorticity form of rotann2.f: "
+         "azimuthal vorticity, swirl, temperature, Coriolis and centrifugal coupling terms), 513 x 257 "
+         "nodes tanh-clustered toward all walls (beta 1.5), radius ratio 0.5, aspect 1, Ra = 1e8, "
+         "Ta = 1e5 (Re = sqrt(Ta)), Pr = 1, dt = 2e-8, differentially heated cylinders (T = 0 inside, 1 "
+         "outside), insulated lids; fourth-order compact advection derivatives on the uneven nodes, "
+         "three-point Fornberg weights elsewhere; both velocity solves exact every step",
+         dict(nr=512, nz=256, dt=2e-8, Ra=1e8, Re=1e5 ** 0.5, Pr=1.0, r0=0.5, rmax=1.0, H=0.5, beta=1.5),
+         20261203, 1e-4, 5000)
+
+
+def merid_leg(cpu=True, warmup=20, prof_steps=200, graphs=True):
+    """The whole 5000-step run of the meridional workload (timed from the
+    seeded start after `warmup` steps of a separate run), then
+    `prof_steps` steps with HIP events around every launch."""
+    from codesthesis_dir_amd.ic import merid_temperature
+    from codesthesis_dir_amd.merid import FLOP_KERNELS, Merid, MeridConfig
+    name, desc, kw, seed, amp, steps = MERID
+    cfg = MeridConfig(**kw)
+
+    def fresh():
+        run = Merid(cfg)
+        run.graphs(graphs)
+        th = merid_temperature(run.r, cfg.nz, amp, seed)
+        for f in ("th", "thp", "thn"):
+            run.put(f, th)
+        return run
+
+    run = fresh()
+    run.step(warmup)                # graph capture and first launches
+    run.close()
+    run = fresh()
+    t0 = time.perf_counter()
+    run.step(steps)                 # returns after the stream is idle
+    el = time.perf_counter() - t0
+    relres = [run.residual(1), run.residual(2)]
+    if not all(x <= 1e-10 for x in relres):
+        sys.exit(f"bench.py: {name}: velocity solve residuals {relres} above 1e-10; no line is reported")
+    heat = run.heat().tolist()
+    ke = run.energy()
+    run.profile(True)
+    run.step(prof_steps)
+    prof = run.profile_read()
+    run.profile(False)
+    run.close()
+    for k, v in prof.items():
+        v["flop_kernel"] = k in FLOP_KERNELS
+    out = {"value": steps / el, "unit": "time-steps/s", "steps": steps, "run_seconds": el,
+           "ms_per_step": 1e3 * el / steps,
+           "config": {"workload": name, "description": desc, "nr": cfg.nr, "nz": cfg.nz,
+                      "poisson": "exact separable solves every step (z eigenbasis, cyclic reduction along r)"},
+           "data": "synthetic: T linear in r plus seeded U(-1e-4, 1e-4), fluid at rest in the rotating frame",
+           "roofline": tank_roofline(prof, name),
+           "kernel_ms_per_step": {k: v["ms"] / prof_steps for k, v in prof.items() if v["launches"]},
+           "graph_vs_launches_ms_per_step": [1e3 * el / steps,
+                                             sum(v["ms"] for v in prof.values()) / prof_steps],
+           "poisson_relres_last_step": relres, "heat_last_step": heat, "kinetic_energy": ke}
+    if cpu:
+        out["cpu_baseline"] = merid_cpu_leg()
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def merid_cpu_leg(steps=100):
+    """`steps` steps of the meridional workload on this host's CPU, one
+    thread: the oracle's C restatement of the explicit part and its numpy
+    separable solves, in a process of its own."""
+    name, desc, kw, seed, amp, _ = MERID
+    c = dict(kw, seed=seed, steps=steps)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "o.json")
+        subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "oracle_merid.py"), json.dumps(c), out],
+                       check=True, stdout=subprocess.DEVNULL)
+        r = json.load(open(out))
+    return {"value": r["steps"] / r["seconds"], "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{r['steps']} whole steps of the same workload from its seeded start on 1 thread: the "
+                      f"oracle's C explicit part and its numpy exact solves ({r['seconds']:.1f} s)"}
+
+
 def tank_cpu_leg(name):
     """One step of the same workload on this host's CPU, on one thread: the
     oracle's explicit routines (C) and one of its three exact separable
@@ -631,6 +714,10 @@ def main():
                     help="with --rot3d-only: time this grid alone (for profiling)")
     ap.add_argument("--no-baseline", action="store_true",
                     help="with --serve-only: skip the one-at-a-time pass")
+    ap.add_argument("--merid-only", action="store_true",
+                    help="measure only the meridional plane's 5000-step run and print its JSON object")
+    ap.add_argument("--merid-launches", action="store_true",
+                    help="with --merid-only: launch each step's kernels one by one instead of replaying graphs")
     ap.add_argument("--rot3d-only", action="store_true",
                     help="measure only the 3D annulus step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -645,6 +732,9 @@ def main():
         return
     if a.serve_only:
         print(json.dumps(serve_leg(slots=a.slots, baseline=not a.no_baseline)))
+        return
+    if a.merid_only:
+        print(json.dumps(merid_leg(cpu=not a.no_cpu, graphs=not a.merid_launches)))
         return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
@@ -672,6 +762,8 @@ def main():
         })
         done("tank step")
         if not a.no_extra:
+            line["merid_513x257"] = merid_leg(cpu=not a.no_cpu)
+            done("meridional plane (5000 steps)")
             line["annulus_513x257"] = leg_2d(a, "annulus-513x257")
             done("2D annulus plane")
     else:

@@ -133,3 +133,117 @@ void gemm_f64(const Op &op, int batch, hipStream_t s)
 }
 
 }  // namespace vv
+
+namespace vv {
+
+// Small products with a long k (few output tiles, K in the hundreds): each
+// 32 x 32 output tile gets a workgroup of four waves, wave w accumulating
+// the k range [w kc, (w+1) kc) with its own LDS tiles; the four partial
+// tiles are summed through LDS at the end.  Four times the waves of the
+// one-wave tiles on the same output, each with a quarter of the k loop.
+template <class Op, bool AKFAST, bool BKFAST, int GK>
+__global__ void __launch_bounds__(256) k_gemm_f64_ks(Op op)
+{
+    constexpr int TM = 32, TN = 32, NT = 64;
+    constexpr int LA = GK * TM / NT, LB = GK * TN / NT;
+    __shared__ double As[4][GK][TM + 1];
+    __shared__ double Bs[4][GK][TN + 1];
+    __shared__ double Red[3][TM * TN];
+    const unsigned nmt = gridDim.y, nnt = gridDim.x;
+    const unsigned lb = xcd_block((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+                                  gridDim.x * gridDim.y * gridDim.z);
+    const int q = (int)(lb / (nmt * nnt));
+    const int m0 = (int)(lb % nmt) * TM, n0 = (int)((lb / nmt) % nnt) * TN;
+    const int wave = threadIdx.x >> 6, tid = threadIdx.x & 63, lane = tid;
+    const int li = lane & 15, lk = lane >> 4;
+    const int kc = ((op.K + 4 * GK - 1) / (4 * GK)) * GK;
+    const int kb = wave * kc, ke = min(op.K, kb + kc);
+    d4 acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[LA], rb[LB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int t = 0; t < LA; ++t) {
+            const int e = tid + NT * t;
+            const int ka = AKFAST ? (e % GK) : (e / TM);
+            const int ma = AKFAST ? (e / GK) : (e % TM);
+            const int m = m0 + ma, k = k0 + ka;
+            ra[t] = (m < op.M && k < ke) ? op.a(q, m, k) : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < LB; ++t) {
+            const int e = tid + NT * t;
+            const int kk = BKFAST ? (e % GK) : (e / TN);
+            const int nb = BKFAST ? (e / GK) : (e % TN);
+            const int n = n0 + nb, k = k0 + kk;
+            rb[t] = (n < op.N && k < ke) ? op.b(q, k, n) : 0.0;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int t = 0; t < LA; ++t) {
+            const int e = tid + NT * t;
+            As[wave][AKFAST ? (e % GK) : (e / TM)][AKFAST ? (e / GK) : (e % TM)] = ra[t];
+        }
+#pragma unroll
+        for (int t = 0; t < LB; ++t) {
+            const int e = tid + NT * t;
+            Bs[wave][BKFAST ? (e % GK) : (e / TN)][BKFAST ? (e / GK) : (e % TN)] = rb[t];
+        }
+    };
+    // every wave runs kc / GK rounds (the same count: the barriers match)
+    fetch(kb);
+    for (int k0 = kb; k0 < kb + kc; k0 += GK) {
+        stash();
+        __syncthreads();
+        if (k0 + GK < kb + kc) fetch(k0 + GK);
+#pragma unroll
+        for (int ks = 0; ks < GK; ks += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) a[r] = As[wave][ks + lk][16 * r + li];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) b[c] = Bs[wave][ks + lk][16 * c + li];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    Red[wave - 1][(16 * r + lk + 4 * e) * TN + 16 * c + li] = acc[r][c][e];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int mm = 16 * r + lk + 4 * e, nn = 16 * c + li;
+                    const double v = acc[r][c][e] + Red[0][mm * TN + nn] + Red[1][mm * TN + nn] + Red[2][mm * TN + nn];
+                    const int m = m0 + mm, n = n0 + nn;
+                    if (m < op.M && n < op.N) op.c(q, m, n, v);
+                }
+    }
+}
+
+template <bool AKFAST, bool BKFAST, class Op, int GK = 16>
+void gemm_f64_ksplit(const Op &op, int batch, hipStream_t s)
+{
+    dim3 grid((op.N + 31) / 32, (op.M + 31) / 32, batch);
+    hipLaunchKernelGGL((k_gemm_f64_ks<Op, AKFAST, BKFAST, GK>), grid, dim3(256), 0, s, op);
+}
+
+}  // namespace vv

@@ -2,8 +2,8 @@
 // gfx950 (include/vvmerid.h).
 //
 // Explicit part: the compact advection derivatives as batched line solves
-// (lines along r staged in LDS, lines along z one per thread, coalesced
-// across r), with the row tables of the uneven nodes and their elimination
+// (lines along r and along z staged in LDS with their row tables, one
+// thread per line), with the row tables of the uneven nodes and their elimination
 // factors formed once on the host; then one thread per node for the
 // DuFort-Frankel step, the right-hand sides and the walls.  Every
 // expression is evaluated in the operation order of the oracle (the library
@@ -38,9 +38,12 @@ namespace {
 constexpr int WG = 256;
 constexpr int MAXB = 1024;   // partials of the reductions
 
-// compact tables of one axis of n intervals: rows (alpha, beta, a, b, c) of
-// the nodes 0..n | fact | bp | 1/bp | end weights e0[5], en[5]
-inline size_t ctab_size(int n) { return 8 * (size_t)(n + 1) + 10; }
+// compact tables of one axis of n intervals: the forward rows (a, b, c of
+// the right-hand side, Thomas factor) 4 per node | the backward rows (beta,
+// b', 1/b', 0) 4 per node | alpha of row 1, beta of row n-1, the end
+// weights e0[5], en[5], 4 spare
+inline size_t ctab_size(int n) { return 8 * (size_t)(n + 1) + 16; }
+constexpr int LDS_WORDS = 8192;   // 64 KB of LDS per workgroup
 
 struct MGeo {
     int nr, nz, ld;
@@ -74,158 +77,169 @@ struct DF3 {
     double *d[6];         // w_r, w_z, v_r, v_z, th_r, th_z
 };
 
-// One line of n intervals, nodes L[0..n] (LDS, stride 1): on return L[1..n-1]
-// hold the compact derivative.  The group's nodes are read ahead of the rows
-// that overwrite them.
-__device__ __forceinline__ void line_lds(double *L, int n, const double *__restrict__ T)
+// One line of n intervals, nodes L[0..n] in LDS, in two passes with the
+// matching half of the table in LDS (tab: 4 words per row, misc: the end
+// weights).  The forward pass leaves the eliminated right-hand sides in
+// L[1..n-1]; the backward pass the derivative.  A group's nodes are read
+// ahead of the rows that overwrite them.
+__device__ __forceinline__ void line_fwd(double *L, int n, const double *tab, const double *misc)
 {
-    const double *R = T, *fact = T + 5 * (n + 1), *bp = T + 6 * (n + 1), *rbp = T + 7 * (n + 1);
-    const double *e0 = T + 8 * (n + 1), *en = e0 + 5;
+    const double *e0 = misc + 2, *en = misc + 7;
+    const double al1 = misc[0], bn1 = misc[1];
     const double d0 = e0[0] * L[0] + e0[1] * L[1] + e0[2] * L[2] + e0[3] * L[3] + e0[4] * L[4];
     const double dn = en[0] * L[n - 4] + en[1] * L[n - 3] + en[2] * L[n - 2] + en[3] * L[n - 1] + en[4] * L[n];
     double fa = L[0], fb = L[1], rp = 0.0;
-    constexpr int U = 8;
-    int i = 1;
-    auto row = [&](int ii, double fc) {
-        const double *q = R + 5 * ii;
-        double r = q[2] * fa + q[3] * fb + q[4] * fc;
-        if (ii == 1) r = r - q[0] * d0;
-        if (ii == n - 1) r = r - q[1] * dn;
-        rp = (ii == 1) ? r : r - fact[ii] * rp;
+    // the table and the nodes share the LDS with the rows being written: a
+    // group's reads are all issued before its first write
+    auto row = [&](int ii, const double *q, double fc) {
+        double r = q[0] * fa + q[1] * fb + q[2] * fc;
+        if (ii == 1) r = r - al1 * d0;
+        if (ii == n - 1) r = r - bn1 * dn;
+        rp = (ii == 1) ? r : r - q[3] * rp;
         L[ii] = rp;
         fa = fb;
         fb = fc;
     };
+    constexpr int U = 8;
+    int i = 1;
     for (; i + U - 1 <= n - 1; i += U) {
-        double nx[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) nx[u] = L[i + 1 + u];
-#pragma unroll
-        for (int u = 0; u < U; ++u) row(i + u, nx[u]);
-    }
-    for (; i <= n - 1; ++i) row(i, L[i + 1]);
-    double y = qdiv(rp, bp[n - 1], rbp[n - 1]);
-    L[n - 1] = y;
-    i = n - 2;
-    for (; i - U + 1 >= 1; i -= U) {
-        double rr[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) rr[u] = L[i - u];
+        double nx[U], qq[U][4];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int ii = i - u;
-            y = qdiv(rr[u] - R[5 * ii + 1] * y, bp[ii], rbp[ii]);
-            L[ii] = y;
+            nx[u] = L[i + 1 + u];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qq[u][k] = tab[4 * (i + u) + k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) row(i + u, qq[u], nx[u]);
+    }
+    for (; i <= n - 1; ++i) {
+        double qq[4];
+        for (int k = 0; k < 4; ++k) qq[k] = tab[4 * i + k];
+        row(i, qq, L[i + 1]);
+    }
+}
+
+__device__ __forceinline__ void line_bwd(double *L, int n, const double *tab)
+{
+    double y = qdiv(L[n - 1], tab[4 * (n - 1) + 1], tab[4 * (n - 1) + 2]);
+    L[n - 1] = y;
+    constexpr int U = 8;
+    int i = n - 2;
+    for (; i - U + 1 >= 1; i -= U) {
+        double rr[U], qq[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rr[u] = L[i - u];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) qq[u][k] = tab[4 * (i - u) + k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            y = qdiv(rr[u] - qq[u][0] * y, qq[u][1], qq[u][2]);
+            L[i - u] = y;
         }
     }
     for (; i >= 1; --i) {
-        y = qdiv(L[i] - R[5 * i + 1] * y, bp[i], rbp[i]);
+        const double *q = tab + 4 * i;
+        y = qdiv(L[i] - q[0] * y, q[1], q[2]);
         L[i] = y;
     }
 }
 
-// Workgroups [0, nbr): lines along r, nl per workgroup staged in LDS (row
-// pitch sp odd); workgroups nbr..: lines along z, one per thread, node i of
-// consecutive threads consecutive (the eliminated right-hand sides go to d
-// and are read back by the back substitution).
-__global__ void __launch_bounds__(64) k_m_deriv(MGeo g, DF3 F, int nbr, int nl)
+// dst[0..n) <- src[0..n) through a 64-thread workgroup, 16 loads in flight
+// per thread before their LDS stores
+__device__ __forceinline__ void lds_fill(double *dst, const double *__restrict__ src, int n, int tid)
+{
+    constexpr int B = 16;
+    for (int e0 = 0; e0 < n; e0 += B * 64) {
+        double v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int e = e0 + u * 64 + tid;
+            if (e < n) v[u] = src[e];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int e = e0 + u * 64 + tid;
+            if (e < n) dst[e] = v[u];
+        }
+    }
+}
+
+// Workgroups [0, nbr) take nlr lines along r (rows j, contiguous), the rest
+// nlz lines along z (columns i, loaded row by row and stored transposed);
+// each line sits in LDS with an odd pitch, one thread runs it, and the
+// derivative goes back the way the nodes came.
+__global__ void __launch_bounds__(64) k_m_deriv(MGeo g, DF3 F, int nbr, int nlr, int nlz)
 {
     extern __shared__ double sh[];
     const int tid = threadIdx.x;
-    const int nr = g.nr, nz = g.nz;
-    if ((int)blockIdx.x < nbr) {
-        const int tiles = (nz - 1 + nl - 1) / nl;
-        const int q = blockIdx.x / tiles, j0 = 1 + (blockIdx.x - q * tiles) * nl;
-        const int nlines = min(nl, nz - j0), len = nr + 1, sp = (nr + 1) | 1;
-        const double *__restrict__ f = F.f[q];
-        double *__restrict__ d = F.d[2 * q];
-        const unsigned base0 = (unsigned)(j0 * g.ld);
-        const int total = nlines * len;
-        for (int e0 = 0; e0 < total; e0 += 8 * 64) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * 64 + tid;
-                if (e < total) {
-                    const int l = e / len, t = e - l * len;
-                    v[u] = f[base0 + (unsigned)(l * g.ld + t)];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * 64 + tid;
-                if (e < total) {
-                    const int l = e / len, t = e - l * len;
-                    sh[l * sp + t] = v[u];
-                }
-            }
-        }
-        __syncthreads();
-        if (tid < nlines) line_lds(sh + tid * sp, nr, g.cr);
-        __syncthreads();
-        const int m = nr - 1;
-        for (int e = tid; e < nlines * m; e += 64) {
-            const int l = e / m, t = 1 + (e - l * m);
-            d[base0 + (unsigned)(l * g.ld + t)] = sh[l * sp + t];
-        }
-        return;
+    const bool rdir = (int)blockIdx.x < nbr;
+    const int n = rdir ? g.nr : g.nz;
+    const double *__restrict__ T = rdir ? g.cr : g.cz;
+    const int tw = 4 * (n + 1), sp = (n + 1) | 1;
+    double *tab = sh, *misc = sh + tw, *lines = sh + tw + 16;
+    int q, o0, nlines;
+    if (rdir) {
+        const int tiles = (g.nz - 1 + nlr - 1) / nlr;
+        q = blockIdx.x / tiles;
+        o0 = 1 + (blockIdx.x - q * tiles) * nlr;     // first row j
+        nlines = min(nlr, g.nz - o0);
+    } else {
+        const int b = blockIdx.x - nbr, tiles = (g.nr - 1 + nlz - 1) / nlz;
+        q = b / tiles;
+        o0 = 1 + (b - q * tiles) * nlz;              // first column i
+        nlines = min(nlz, g.nr - o0);
     }
-    const int t = (blockIdx.x - nbr) * 64 + tid;
-    const int ni = nr - 1;
-    if (t >= 3 * ni) return;
-    const int q = t / ni, i = 1 + (t - q * ni);
     const double *__restrict__ f = F.f[q];
-    double *__restrict__ d = F.d[2 * q + 1];
-    const unsigned s = (unsigned)g.ld;
-    const int n = nz;
-    const double *T = g.cz;
-    const double *R = T, *fact = T + 5 * (n + 1), *bp = T + 6 * (n + 1), *rbp = T + 7 * (n + 1);
-    const double *e0 = T + 8 * (n + 1), *en = e0 + 5;
-    const unsigned b = (unsigned)i;
-    const double d0 = e0[0] * f[b] + e0[1] * f[b + s] + e0[2] * f[b + 2u * s] + e0[3] * f[b + 3u * s] +
-                      e0[4] * f[b + 4u * s];
-    const double dn = en[0] * f[b + (unsigned)(n - 4) * s] + en[1] * f[b + (unsigned)(n - 3) * s] +
-                      en[2] * f[b + (unsigned)(n - 2) * s] + en[3] * f[b + (unsigned)(n - 1) * s] +
-                      en[4] * f[b + (unsigned)n * s];
-    double fa = f[b], fb = f[b + s], rp = 0.0;
-    auto row = [&](int jj, double fc) {
-        const double *qq = R + 5 * jj;
-        double r = qq[2] * fa + qq[3] * fb + qq[4] * fc;
-        if (jj == 1) r = r - qq[0] * d0;
-        if (jj == n - 1) r = r - qq[1] * dn;
-        rp = (jj == 1) ? r : r - fact[jj] * rp;
-        d[b + (unsigned)jj * s] = rp;
-        fa = fb;
-        fb = fc;
-    };
-    constexpr int U = 8;
-    int j = 1;
-    for (; j + U - 1 <= n - 1; j += U) {
-        double nx[U];
+    double *__restrict__ d = F.d[2 * q + (rdir ? 0 : 1)];
+    // element e of the tile is (major, minor) = (e / mn, e % mn): r tiles
+    // (line, node), z tiles (node, line); walked in steps of 64 without
+    // divisions
+    {
+        const int mn = rdir ? n + 1 : nlines, total = nlines * (n + 1);
+        const int q64 = 64 / mn, r64 = 64 - q64 * mn;
+        int ma = tid / mn, mi = tid - ma * mn;
+        for (int e0 = 0; e0 < total; e0 += 16 * 64) {
+            double v[16];
+            int li[16];
 #pragma unroll
-        for (int u = 0; u < U; ++u) nx[u] = f[b + (unsigned)(j + 1 + u) * s];
+            for (int u = 0; u < 16; ++u) {
+                li[u] = -1;
+                if (e0 + u * 64 + tid < total) {
+                    v[u] = rdir ? f[(unsigned)((o0 + ma) * g.ld + mi)] : f[(unsigned)(ma * g.ld + o0 + mi)];
+                    li[u] = rdir ? ma * sp + mi : mi * sp + ma;
+                }
+                ma += q64;
+                mi += r64;
+                if (mi >= mn) { mi -= mn; ++ma; }
+            }
 #pragma unroll
-        for (int u = 0; u < U; ++u) row(j + u, nx[u]);
-    }
-    for (; j <= n - 1; ++j) row(j, f[b + (unsigned)(j + 1) * s]);
-    double y = qdiv(rp, bp[n - 1], rbp[n - 1]);
-    d[b + (unsigned)(n - 1) * s] = y;
-    j = n - 2;
-    for (; j - U + 1 >= 1; j -= U) {
-        double rr[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) rr[u] = d[b + (unsigned)(j - u) * s];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int jj = j - u;
-            y = qdiv(rr[u] - R[5 * jj + 1] * y, bp[jj], rbp[jj]);
-            d[b + (unsigned)jj * s] = y;
+            for (int u = 0; u < 16; ++u)
+                if (li[u] >= 0) lines[li[u]] = v[u];
         }
     }
-    for (; j >= 1; --j) {
-        const unsigned o = b + (unsigned)j * s;
-        y = qdiv(d[o] - R[5 * j + 1] * y, bp[j], rbp[j]);
-        d[o] = y;
+    lds_fill(tab, T, tw, tid);
+    if (tid < 16) misc[tid] = T[2 * tw + tid];
+    __syncthreads();
+    if (tid < nlines) line_fwd(lines + tid * sp, n, tab, misc);
+    __syncthreads();
+    lds_fill(tab, T + tw, tw, tid);
+    __syncthreads();
+    if (tid < nlines) line_bwd(lines + tid * sp, n, tab);
+    __syncthreads();
+    // interior nodes back: r tiles (line, node 1..n-1), z tiles (node
+    // 1..n-1, line), walked as the loads
+    const int mn = rdir ? n - 1 : nlines, tot = nlines * (n - 1);
+    const int q64 = 64 / mn, r64 = 64 - q64 * mn;
+    int ma = tid / mn, mi = tid - ma * mn;
+    for (int e = tid; e < tot; e += 64) {
+        if (rdir) d[(unsigned)((o0 + ma) * g.ld + 1 + mi)] = lines[ma * sp + 1 + mi];
+        else d[(unsigned)((1 + ma) * g.ld + o0 + mi)] = lines[mi * sp + 1 + ma];
+        ma += q64;
+        mi += r64;
+        if (mi >= mn) { mi -= mn; ++ma; }
     }
 }
 
@@ -563,26 +577,36 @@ void compact_row(const double *x, int i, double *out)
     }
 }
 
-// the tables of a line of n intervals (layout of ctab_size): the Thomas
-// elimination of the rows alpha_i, 1, beta_i (i = 1..n-1) and the
-// five-point one-sided end derivatives
+// the tables of a line of n intervals (layout of ctab_size): the compact
+// rows, the Thomas elimination of the rows alpha_i, 1, beta_i (i = 1..n-1)
+// and the five-point one-sided end derivatives
 void compact_tables(int n, const double *x, double *T)
 {
     std::fill(T, T + ctab_size(n), 0.0);
-    double *R = T, *fact = T + 5 * (n + 1), *bp = T + 6 * (n + 1), *rbp = T + 7 * (n + 1);
-    double *e0 = T + 8 * (n + 1), *en = e0 + 5;
-    for (int i = 1; i <= n - 1; ++i) compact_row(x, i, R + 5 * i);
-    double c[10];
-    fornberg(x, 5, x[0], 1, c);
-    for (int q = 0; q < 5; ++q) e0[q] = c[5 + q];
-    fornberg(x + n - 4, 5, x[n], 1, c);
-    for (int q = 0; q < 5; ++q) en[q] = c[5 + q];
+    std::vector<double> R(5 * (size_t)(n + 1), 0.0), fact(n + 1, 0.0), bp(n + 1, 0.0);
+    for (int i = 1; i <= n - 1; ++i) compact_row(x, i, &R[5 * i]);
     bp[1] = 1.0;
     for (int i = 2; i <= n - 1; ++i) {
         fact[i] = R[5 * i + 0] / bp[i - 1];
         bp[i] = 1.0 - fact[i] * R[5 * (i - 1) + 1];
     }
-    for (int i = 1; i <= n - 1; ++i) rbp[i] = 1.0 / bp[i];
+    double *fw = T, *bw = T + 4 * (n + 1), *misc = T + 8 * (n + 1);
+    for (int i = 1; i <= n - 1; ++i) {
+        fw[4 * i + 0] = R[5 * i + 2];
+        fw[4 * i + 1] = R[5 * i + 3];
+        fw[4 * i + 2] = R[5 * i + 4];
+        fw[4 * i + 3] = fact[i];
+        bw[4 * i + 0] = R[5 * i + 1];
+        bw[4 * i + 1] = bp[i];
+        bw[4 * i + 2] = 1.0 / bp[i];
+    }
+    misc[0] = R[5 * 1 + 0];
+    misc[1] = R[5 * (n - 1) + 1];
+    double c[10];
+    fornberg(x, 5, x[0], 1, c);
+    for (int q = 0; q < 5; ++q) misc[2 + q] = c[5 + q];
+    fornberg(x + n - 4, 5, x[n], 1, c);
+    for (int q = 0; q < 5; ++q) misc[7 + q] = c[5 + q];
 }
 
 // three-point weights at the inner nodes: first derivative, then second
@@ -616,13 +640,14 @@ struct vvm_run {
     double *geo = nullptr;   // r nodes | wr | wz | cr | cz | hz | hr | heat weights
     MGeo g;
     const double *hz = nullptr, *hr = nullptr, *hw = nullptr;
-    int nl = 1, nbr = 0, nbz = 0;
+    int nlr = 1, nlz = 1, nbr = 0, nbz = 0;
     size_t dlds = 0;
     // Poisson
     double *V = nullptr, *Vi = nullptr, *lam = nullptr, *rt = nullptr;   // rt: [2][3][nr-1]
     double *G = nullptr, *X = nullptr;                                   // [2][nz-1][nr-1]
     double *part = nullptr;                                              // reductions, then results
     // graphs of one and of ten steps
+    bool use_graphs = true;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     // profiling
@@ -666,7 +691,8 @@ int launch_derivatives(vvm_run *r)
     F.f[2] = r->f[VVM_TH];
     for (int d = 0; d < 6; ++d) F.d[d] = r->dv[d];
     Prof P(r, VVM_K_DERIV, 9.0 * 8.0 * npts(r));
-    hipLaunchKernelGGL(k_m_deriv, dim3(r->nbr + r->nbz), dim3(64), r->dlds, r->stream, r->g, F, r->nbr, r->nl);
+    hipLaunchKernelGGL(k_m_deriv, dim3(r->nbr + r->nbz), dim3(64), r->dlds, r->stream, r->g, F, r->nbr, r->nlr,
+                       r->nlz);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -705,7 +731,7 @@ int launch_poisson(vvm_run *r, int which)
     {
         ZFwd op{r->Vi, M, N, M, src[k0], src[1], r->G, r->ld};
         Prof P(r, VVM_K_ZFWD, flops);
-        vv::gemm_f64<true, false>(op, batch, r->stream);
+        vv::gemm_f64_ksplit<true, false>(op, batch, r->stream);
         VV_HIP(hipGetLastError());
     }
     {
@@ -722,7 +748,7 @@ int launch_poisson(vvm_run *r, int which)
     {
         ZInv op{r->V, M, N, M, r->X, dst[k0], dst[1], r->ld};
         Prof P(r, VVM_K_ZINV, flops);
-        vv::gemm_f64<true, false>(op, batch, r->stream);
+        vv::gemm_f64_ksplit<true, false>(op, batch, r->stream);
         VV_HIP(hipGetLastError());
     }
     return VV_OK;
@@ -814,12 +840,15 @@ int setup(vvm_run *r, const double *rn, const double *zn)
     g.rn = r->geo + o_r; g.wr = r->geo + o_wr; g.wz = r->geo + o_wz;
     g.cr = r->geo + o_cr; g.cz = r->geo + o_cz;
     r->hz = r->geo + o_hz; r->hr = r->geo + o_hr; r->hw = r->geo + o_hw;
-    // derivative launch shape: r lines nl per workgroup within 64 KB of LDS
-    const int sp = (nr + 1) | 1;
-    r->nl = std::max(1, std::min(64, 8192 / sp));
-    r->nbr = 3 * ((nz - 1 + r->nl - 1) / r->nl);
-    r->nbz = nblk(3LL * (nr - 1), 64);
-    r->dlds = (size_t)r->nl * sp * sizeof(double);
+    // derivative launch shape: per direction the half table (4 words a node),
+    // 16 words of end weights and as many lines as fit in 64 KB of LDS
+    auto lines_fit = [](int n) { return (LDS_WORDS - 4 * (n + 1) - 16) / ((n + 1) | 1); };
+    r->nlr = std::max(1, std::min(64, lines_fit(nr)));
+    r->nlz = std::max(1, std::min(16, lines_fit(nz)));
+    r->nbr = 3 * ((nz - 1 + r->nlr - 1) / r->nlr);
+    r->nbz = 3 * ((nr - 1 + r->nlz - 1) / r->nlz);
+    r->dlds = sizeof(double) * std::max(4 * (size_t)(nr + 1) + 16 + (size_t)r->nlr * ((nr + 1) | 1),
+                                        4 * (size_t)(nz + 1) + 16 + (size_t)r->nlz * ((nz + 1) | 1));
     // Poisson: z eigenbasis, r rows of both operators
     const int M = nz - 1, N = nr - 1;
     std::vector<double> za(M), zb(M), zc(M);
@@ -876,8 +905,8 @@ int vvm_create(const vvm_config *cfg, const double *rn, const double *zn, int de
     using vv::set_error;
     if (!cfg || !rn || !zn || !out) { set_error("vvm_create: null argument"); return VV_EINVAL; }
     *out = nullptr;
-    if (cfg->nr < 8 || cfg->nz < 8 || cfg->nr > 1024 || cfg->nz > 2048) {
-        set_error("vvm_create: need 8 <= nr <= 1024 and 8 <= nz <= 2048");
+    if (cfg->nr < 8 || cfg->nz < 8 || cfg->nr > 1024 || cfg->nz > 1024) {
+        set_error("vvm_create: need 8 <= nr, nz <= 1024");
         return VV_EINVAL;
     }
     if (!(cfg->dt > 0) || !(cfg->Pr > 0) || !std::isfinite(cfg->Ra) || !std::isfinite(cfg->Re) ||
@@ -1011,7 +1040,7 @@ int vvm_step(vvm_run *r, int n)
     if (!r || n < 0) { vv::set_error("vvm_step: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    if (r->prof) {
+    if (r->prof || !r->use_graphs) {
         for (int s = 0; s < n; ++s) {
             const int st = launch_step(r);
             if (st) return st;
@@ -1030,6 +1059,14 @@ int vvm_step(vvm_run *r, int n)
         for (; left > 0; --left) VV_HIP(hipGraphLaunch(r->gexec[0], r->stream));
     }
     VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vvm_graphs(vvm_run *r, int enable)
+{
+    if (!r) { vv::set_error("null run"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    r->use_graphs = enable != 0;
     return VV_OK;
 }
 

@@ -36,3 +36,15 @@ def box_temperature(nr: int, np_: int, r0: float, rmax: float, amp: float, seed:
     th[:, 0] = 0.0
     th[:, nr + 1] = 1.0
     return th
+
+
+def merid_temperature(r: np.ndarray, nz: int, amp: float, seed: int) -> np.ndarray:
+    """Meridional plane (nodes r[0..nr], z rows 0..nz): temperature linear in
+    r between the cylinders (0 inside, 1 outside) plus U(-amp, amp) at the
+    inner nodes, drawn by numpy's default_rng(seed) as one (nz-1) x (nr-1)
+    block, j outer."""
+    nr = r.size - 1
+    th = np.broadcast_to((r - r[0]) / (r[-1] - r[0]), (nz + 1, nr + 1)).copy()
+    rng = np.random.default_rng(seed)
+    th[1:-1, 1:-1] += rng.uniform(-amp, amp, size=(nz - 1, nr - 1))
+    return th

@@ -78,6 +78,7 @@ def _bind():
     L.vvm_heat.argtypes = [vp, _P]
     L.vvm_energy.argtypes = [vp, _P]
     L.vvm_profile.argtypes = [vp, ctypes.c_int]
+    L.vvm_graphs.argtypes = [vp, ctypes.c_int]
     L.vvm_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _P, _P]
     L._vvm_bound = True
     return L
@@ -161,6 +162,11 @@ class Merid:
         e = ctypes.c_double()
         check(self._lib.vvm_energy(self._h, ctypes.byref(e)), "vvm_energy")
         return e.value
+
+    def graphs(self, enable: bool):
+        """Replay whole steps from HIP graphs (default) or launch them one
+        by one."""
+        check(self._lib.vvm_graphs(self._h, int(enable)), "vvm_graphs")
 
     def profile(self, enable: bool):
         check(self._lib.vvm_profile(self._h, int(enable)), "vvm_profile")

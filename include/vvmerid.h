@@ -38,7 +38,7 @@ extern "C" {
 typedef struct vvm_run vvm_run;
 
 typedef struct {
-    int    nr, nz;        /* intervals in r and z: 8 <= nr <= 1024, 8 <= nz <= 2048 */
+    int    nr, nz;        /* intervals in r and z: 8 <= nr, nz <= 1024          */
     double dt;
     double Ra, Re, Pr;    /* Rayleigh, Re = sqrt(Ta) (Re = 0: no rotation), Prandtl */
 } vvm_config;
@@ -81,8 +81,11 @@ int vvm_residual(vvm_run *run, int which, double *relres);
 /* walls of the new levels, then the time levels rotate */
 int vvm_update(vvm_run *run);
 /* n whole steps in the order derivatives, explicit, rhs, poisson 1 and 2,
- * update (replayed from a captured HIP graph) */
+ * update (replayed from captured HIP graphs of one and of ten steps) */
 int vvm_step(vvm_run *run, int n);
+/* enable = 0: vvm_step launches the kernels of each step one by one (the
+ * same kernels; for tools that trace launches); 1: graphs (the default) */
+int vvm_graphs(vvm_run *run, int enable);
 /* heat(trans) of rotann2.f:468: the fluxes dth/dr through the inner (out[0])
  * and outer (out[1]) cylinder, trapezoid-integrated over z */
 int vvm_heat(vvm_run *run, double out[2]);

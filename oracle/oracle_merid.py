@@ -185,18 +185,24 @@ class Merid:
             lam, V = lam.real[o], V.real[:, o]
             self._solver["z"] = (lam, V, np.linalg.inv(V))
         lam, V, Vi = self._solver["z"]
-        s, d, u = self.rows("r", which)
+        if which not in self._solver:
+            # Thomas factors along r, per z mode (they depend on the operator only)
+            s, d, u = self.rows("r", which)
+            n = d.size
+            bp = np.empty((lam.size, n))
+            fa = np.zeros((lam.size, n))
+            bp[:, 0] = d[0] + lam
+            for i in range(1, n):
+                fa[:, i] = s[i] / bp[:, i - 1]
+                bp[:, i] = d[i] + lam - fa[:, i] * u[i - 1]
+            self._solver[which] = (fa, bp, u)
+        fa, bp, u = self._solver[which]
         G = Vi @ rhs[1:-1, 1:-1]               # (modes, r)
-        # Thomas along r, vectorised over the modes
-        n = d.size
-        bp = np.empty((lam.size, n))
-        rp = np.empty((lam.size, n))
-        bp[:, 0] = d[0] + lam
+        n = bp.shape[1]
+        rp = np.empty_like(G)
         rp[:, 0] = G[:, 0]
         for i in range(1, n):
-            f = s[i] / bp[:, i - 1]
-            bp[:, i] = d[i] + lam - f * u[i - 1]
-            rp[:, i] = G[:, i] - f * rp[:, i - 1]
+            rp[:, i] = G[:, i] - fa[:, i] * rp[:, i - 1]
         X = np.empty_like(G)
         X[:, n - 1] = rp[:, n - 1] / bp[:, n - 1]
         for i in range(n - 2, -1, -1):
@@ -258,7 +264,8 @@ def time_steps(cfg_json: str, out_json: str):
     except Exception:  # pragma: no cover
         lim = None
     c = json.loads(cfg_json)
-    m = Merid(c["nr"], c["nz"], c["dt"], c["Ra"], c["Re"], c["Pr"], beta=c["beta"])
+    m = Merid(c["nr"], c["nz"], c["dt"], c["Ra"], c["Re"], c["Pr"], r0=c.get("r0", 0.5), rmax=c.get("rmax", 1.0),
+              H=c.get("H", 0.5), beta=c["beta"])
     initial_state(m, c["seed"])
     m.step()   # builds the z eigenbasis
     t0 = time.perf_counter()

@@ -118,19 +118,23 @@ def test_graph_replays_equal_eager_steps():
     print("[gpu] merid graph replays", flush=True)
     a, ora = _pair(48, 24, seed=None, dt=5e-8, Ra=1e8, Re=316.2, Pr=1.0)
     b, _ = _pair(48, 24, seed=None, dt=5e-8, Ra=1e8, Re=316.2, Pr=1.0)
-    _start(a, ora, 5)
-    _start(b, ora, 5)
-    a.profile(True)           # launched one by one
+    c, _ = _pair(48, 24, seed=None, dt=5e-8, Ra=1e8, Re=316.2, Pr=1.0)
+    for run in (a, b, c):
+        _start(run, ora, 5)
+    a.profile(True)           # launched one by one, timed
+    c.graphs(False)           # launched one by one
     for chunk in (3, 10, 1, 20, 7, 10):
         for _ in range(chunk):
             a.step(1)
         b.step(chunk)         # graphs of one and of ten steps
+        c.step(chunk)
         for n in ("w", "v", "th", "u", "ww", "wp"):
             assert np.array_equal(a.get(n), b.get(n)), (chunk, n)
+            assert np.array_equal(a.get(n), c.get(n)), (chunk, n)
     prof = a.profile_read()
     assert prof["deriv"]["launches"] == 51 and prof["zfwd"]["launches"] == 51
-    a.close()
-    b.close()
+    for run in (a, b, c):
+        run.close()
 
 
 def test_workload_grid_5000_steps():
