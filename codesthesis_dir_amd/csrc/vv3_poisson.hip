@@ -551,27 +551,32 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
     const int np = g.np, nm = np / 2 + 1;
     P.nm = nm;
     const double pi = std::acos(-1.0);
-    // phi tables: the angle of mode m at k is 2 pi ((m k) mod np) / np
-    // spectral rows: q = m (cos, m = 0..np/2), q = np/2 + m (sin, m = 1..np/2-1)
-    std::vector<double> W((size_t)np * np), Wi((size_t)np * np), lphi(nm);
-    for (int m = 0; m < nm; ++m) {
-        const double wgt = (m == 0 || 2 * m == np) ? 1.0 : 2.0;
-        lphi[m] = 2.0 * std::cos(2.0 * pi * m / np) - 2.0;
-        const bool sin_row = (m > 0 && 2 * m < np);
-        for (int k = 1; k <= np; ++k) {
-            const double ang = 2.0 * pi * (double)(((long long)m * k) % np) / np;
-            const double cs = std::cos(ang), sn = std::sin(ang);
-            W[(size_t)m * np + (k - 1)] = cs;
-            Wi[(size_t)(k - 1) * np + m] = wgt * cs / np;
-            if (sin_row) {
-                W[(size_t)(np / 2 + m) * np + (k - 1)] = sn;
-                Wi[(size_t)(k - 1) * np + np / 2 + m] = wgt * sn / np;
+    // phi transforms by the LDS FFT (even np up to 4096), else dense GEMMs
+    // with the tables W, Wi: the angle of mode m at k is 2 pi ((m k) mod
+    // np) / np; spectral rows q = m (cos, m = 0..np/2), q = np/2 + m (sin,
+    // m = 1..np/2-1)
+    P.fft = (np >= 8 && np <= 4096 && np % 2 == 0);
+    std::vector<double> lphi(nm);
+    for (int m = 0; m < nm; ++m) lphi[m] = 2.0 * std::cos(2.0 * pi * m / np) - 2.0;
+    if (!P.fft) {
+        std::vector<double> W((size_t)np * np), Wi((size_t)np * np);
+        for (int m = 0; m < nm; ++m) {
+            const double wgt = (m == 0 || 2 * m == np) ? 1.0 : 2.0;
+            const bool sin_row = (m > 0 && 2 * m < np);
+            for (int k = 1; k <= np; ++k) {
+                const double ang = 2.0 * pi * (double)(((long long)m * k) % np) / np;
+                const double cs = std::cos(ang), sn = std::sin(ang);
+                W[(size_t)m * np + (k - 1)] = cs;
+                Wi[(size_t)(k - 1) * np + m] = wgt * cs / np;
+                if (sin_row) {
+                    W[(size_t)(np / 2 + m) * np + (k - 1)] = sn;
+                    Wi[(size_t)(k - 1) * np + np / 2 + m] = wgt * sn / np;
+                }
             }
         }
+        VV_HIP(upload(&P.W, W));
+        VV_HIP(upload(&P.Wi, Wi));
     }
-    VV_HIP(upload(&P.W, W));
-    VV_HIP(upload(&P.Wi, Wi));
-    P.fft = (np >= 8 && np <= 4096 && np % 2 == 0);
     if (P.fft) {
         // (cos, -sin)(2 pi t/n1), t < n1/2, for the n1 = 2^bits point
         // radix-2 sub-lines, then (np not a power of two) (cos, -sin)(2 pi t/np),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -351,7 +352,12 @@ int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev)
     r->mg_flags = static_cast<int *>(r->mg_state);
     r->mg_bits = reinterpret_cast<unsigned long long *>(static_cast<char *>(r->mg_state) + 32);
     r->mg_relres = reinterpret_cast<double *>(static_cast<char *>(r->mg_state) + 64);
-    if ((st = mg_build(r))) return fail(st);
+    try {
+        if ((st = mg_build(r))) return fail(st);
+    } catch (const std::bad_alloc &) {
+        set_error("vv_create: host memory for the Poisson tables exhausted");
+        return fail(VV_ENOMEM);
+    }
     r->diag_cap = 1 << 16;
     if (hipMalloc(&r->diag, (size_t)r->diag_cap * 2 * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMalloc(&r->kepart, (vv::KE_MAXB + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
@@ -586,10 +592,11 @@ int vv_step(vv_run *r, int nsteps, vv_step_info *info)
     int cyc[2] = {0, 0};
     double rel[2] = {0, 0};
     // Deferred solves: with the whole plane direct, once a synchronised
-    // solve has converged in one cycle, later steps queue that cycle with a
-    // device-side check in its stop test (k_mg_stop) without waiting for the
-    // host; the check's sticky word is read once at the end of the call.  Profiled
-    // runs and the V-cycle synchronise every solve.
+    // solve has converged in one cycle, later steps queue that cycle and a
+    // refinement cycle (empty when the first converged) with a device-side
+    // check in the second's stop test (k_mg_stop) without waiting for the
+    // host; the check's sticky word is read once at the end of the call.
+    // Profiled runs and the V-cycle synchronise every solve.
     const bool direct_all = r->cfg.direct_cols >= r->np;
     bool any_deferred = false;
     VV_HIP(hipMemsetAsync(r->mg_flags + 4, 0, sizeof(int), r->stream));

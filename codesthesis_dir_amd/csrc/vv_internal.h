@@ -85,6 +85,8 @@ inline int default_direct_cols(int np) { return np <= 8192 ? np : 128; }
 // a direct level of c columns is transformed by FFTs (radix 2 times the odd
 // factor of c) when this holds, by dense transforms otherwise
 inline bool fft_cols(int c) { return c >= 8 && c % 2 == 0 && c <= 8192; }
+// the largest direct level solved with dense transforms (2 c^2 weights)
+constexpr int MG_DENSE_MAXC = 4096;
 
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_MAXPROB = 2;

@@ -843,8 +843,17 @@ int mg_build(vv_run *r)
     const int Cd = r->cfg.direct_cols > 0 ? r->cfg.direct_cols : 1;
     int nlev = 1;
     // phi coarsening halves the column count while it is even
-    for (int c = np; c > Cd && c % 2 == 0; c /= 2) ++nlev;
+    int cdir = np;
+    for (; cdir > Cd && cdir % 2 == 0; cdir /= 2) ++nlev;
     if (nlev > MG_MAXLEV) { set_error("too many multigrid levels"); return VV_EINVAL; }
+    // a direct level without FFTs (fewer than 8 columns, odd, or above 8192)
+    // holds 2 c^2 dense transform weights: bounded at c = 4096 (268 MB)
+    if (!fft_cols(cdir) && cdir > MG_DENSE_MAXC) {
+        set_error("the direct Poisson level would have " + std::to_string(cdir) +
+                  " columns without an FFT (odd np, or direct_cols above 8192): at most " +
+                  std::to_string(MG_DENSE_MAXC) + " columns are solved with dense transforms");
+        return VV_EINVAL;
+    }
     const int nvec[2] = {nr, nr - 1};
     // host-side tables first, then one pool
     struct LevHost { std::vector<double> tab; int S; double cphi; int ncol; int direct; size_t tabsz; };
@@ -1160,10 +1169,16 @@ static int enqueue_cycle(vv_run *r, int *check = nullptr)
     return VV_OK;
 }
 
-// One cycle of both problems with the sticky check in its stop test, with
-// no host synchronisation (the time loop's deferred solves)
+// The time loop's deferred solves, with no host synchronisation: one cycle
+// of both problems, then a refinement cycle whose stop test carries the
+// sticky check.  Every kernel of a cycle returns at once for a problem
+// already done, so the second cycle costs empty launches when the first
+// converged and refines on the device when it did not; only a problem still
+// above rtol after both sets the check.
 int mg_enqueue_deferred(vv_run *r)
 {
+    const int st = enqueue_cycle(r);
+    if (st) return st;
     return enqueue_cycle(r, r->mg_flags + 4);
 }
 

@@ -65,7 +65,11 @@ typedef struct vv_config {
                              phi coarsening halves np while it is even; the
                              direct level's transforms are mixed-radix FFTs
                              (radix 2 and the odd factor) for even column
-                             counts up to 8192, dense transforms otherwise) */
+                             counts up to 8192, dense transforms (2 c^2
+                             weights) otherwise, for at most 4096 columns:
+                             an np whose direct level would be larger, e.g.
+                             odd np above 4095 or 16382 = 2 x 8191, is
+                             rejected) */
     double dt;
     double r0, rmax;      /* walls; dr = (rmax-r0)/nr                       */
     double width;         /* periodic length; dp = width/np (rb.f:132-133) */

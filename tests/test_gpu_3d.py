@@ -175,11 +175,12 @@ def test_poisson_matches_exact_solve(which, shape):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10), (40, 36, 64), (160, 192, 32)])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (33, 65, 10), (40, 36, 64), (160, 192, 32), (8, 6, 3072)])
 def test_poisson_residual_at_larger_sizes(which, shape):
     """Grids beyond the oracle's dense solve: the solution satisfies the
     reference's residual operator (rot3dmg.f:1695-1723, 1927-1965,
-    2130-2165) to rounding, relative to the operator's scale."""
+    2130-2165) to rounding, relative to the operator's scale.  np = 3072
+    (3 x 1024) takes the FFT's 32-register mixed-radix combination."""
     nr, nz, np_ = shape
     run = _gpu(nr, nz, np_)
     ora = O3.Run3(nr, nz, np_, **PARAMS)

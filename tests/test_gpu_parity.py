@@ -58,7 +58,8 @@ def _random_state(run, seed):
 @pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("nr,np_,first", [(16, 32, True), (16, 32, False), (64, 128, False), (1100, 32, False),
                                            (40, 64, False), (96, 336, False), (20, 45, True),
-                                           (24, 1024, False), (12, 2048, True), (10, 4096, False)])
+                                           (24, 1024, False), (12, 2048, True), (10, 4096, False),
+                                           (16, 12288, False), (10, 16376, True)])
 def test_explicit_step_bitexact(nr, np_, first, geom):
     import ctypes
     run, box = _pair(geom, nr, np_)
@@ -282,6 +283,31 @@ def test_poisson_default_direct_any_np(geom, nr, np_):
     box.close()
 
 
+@pytest.mark.parametrize("nr,np_", [(16, 12288), (12, 16376)])
+def test_trajectory_above_8192_columns(nr, np_):
+    """Above 8192 columns the compact tiles no longer fit in LDS (their
+    tables move to global memory) and the default Poisson path is the
+    V-cycle with a 128-column direct level: 5 steps against the oracle with
+    exact solves, relative L2 of every field <= 1e-9."""
+    width, dt, Ra, Pr = 3.0, 1e-6, 1e5, 1.0
+    run = O.Run(nr, np_, dt, Ra, Pr, width=width)
+    run.seed()
+    box = _box(nr, np_, width, dt=dt, Ra=Ra, Pr=Pr, rtol=1e-12, max_cycles=60)
+    assert box.nlev > 1
+    box.put("th", run["th"])
+    box.start()
+    info = box.step(5)
+    assert info.status == 0
+    for _ in range(5):
+        st, tr, ncl = run.step(mode=1)
+        assert st == 0
+    for name in ("th", "C", "uel", "vel"):
+        g, o = box.get(name), run[name]
+        err = np.linalg.norm(g - o) / np.linalg.norm(o)
+        assert err <= 1e-9, (name, err)
+    box.close()
+
+
 def test_trajectory_reference_deck_size():
     """The reference's own input deck (input.s: nr = 96, np = 336,
     Ra = 1e7, Pr = 0.71, dt = 1e-7; the box width is 3, rb.f:132-133),
@@ -493,10 +519,12 @@ def test_concurrent_calls_on_one_run():
     box.close()
 
 
-@pytest.mark.parametrize("nr,np_", [(16, 20000), (3000, 64), (4, 64), (16, 6)])
+@pytest.mark.parametrize("nr,np_", [(16, 20000), (3000, 64), (4, 64), (16, 6), (16, 4097), (16, 16382)])
 def test_sizes_the_kernels_cannot_hold_are_rejected(nr, np_):
     """vv_create refuses sizes whose lines do not fit in LDS (np above 16383,
-    nr above 2048) and sizes below 8, before any allocation."""
+    nr above 2048), sizes below 8, and a direct level without an FFT of
+    more than 4096 columns (2 c^2 dense weights): an odd np above 4095, or
+    16382 = 2 x 8191, whose halving stops at 8191 columns."""
     import codesthesis_dir_amd._lib as L
     with pytest.raises(L.VVError) as e:
         _box(nr, np_, 3.0)
@@ -534,6 +562,48 @@ def test_odd_sweep_count_is_rejected():
     with pytest.raises(L.VVError) as e:
         _box(16, 32, 1.0, nu1=1, nu2=2)
     assert e.value.code == L.VV_EINVAL
+
+
+def test_deferred_solves_refine_a_cycle_that_misses_rtol():
+    """A deferred step whose first direct cycle misses rtol is refined on
+    the device by the queued second cycle: with rtol between the one-cycle
+    residuals of the steps, a run of deferred steps equals, bit for bit,
+    the run whose every solve is synchronised (and runs as many cycles as
+    it needs), and reports no failure."""
+    from codesthesis_dir_amd.ic import box_temperature
+    print("[gpu] deferred refinement", flush=True)
+    nr, np_ = 32, 64
+    th0 = box_temperature(nr, np_, 0.5, 1.0, 1e-3, 3)
+    probe = _box(nr, np_, 1.0, rtol=1e-10)
+    probe.put("th", th0)
+    probe.start()
+    probe.profile(True)
+    rel = []
+    for _ in range(12):
+        info = probe.step(1)
+        rel += [info.relres_v, info.relres_u]
+    probe.close()
+    rel = sorted(rel)
+    rtol = (rel[len(rel) // 4] * rel[3 * len(rel) // 4]) ** 0.5   # some steps need a second cycle
+    a = _box(nr, np_, 1.0, rtol=rtol, max_cycles=2)
+    b = _box(nr, np_, 1.0, rtol=rtol, max_cycles=2)
+    for box in (a, b):
+        box.put("th", th0)
+        box.start()
+    b.profile(True)                 # every solve synchronised
+    cycles, status = [], []
+    for _ in range(12):
+        info = b.step(1)
+        cycles += [info.cycles_v, info.cycles_u]
+        status.append(info.status)
+    assert max(cycles) == 2 and max(status) == 0
+    a.step(1)                       # synchronised; the rest deferred once a solve takes one cycle
+    info = a.step(11)
+    assert info.status == 0
+    for f in ("C", "th", "uel", "vel"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
 
 
 def test_deferred_solves_report_a_nonfinite_rhs():
