@@ -134,27 +134,41 @@ def vcycle_large_leg(steps=6, nu=1):
     (nr=2048, np=4096), V(nu,nu) with five smoothed levels and the direct
     level at 128 columns; per-kernel HIP events over `steps` time steps.
     The same plane's time step with the default direct solve is measured
-    beside it."""
+    beside it, and the V-cycle in mixed precision (FP64 defect correction
+    around an FP32 cycle): its time, its cycles and residual history, and
+    its fields against the FP64 V-cycle run after the same steps."""
+    import numpy as np
     from codesthesis_dir_amd import BoxConfig, PeriodicBox
     from codesthesis_dir_amd.ic import box_temperature
     base = dict(nr=2048, np=4096, width=2.0, dt=1e-7, Ra=1e5, Pr=1.0, rtol=1e-10, max_cycles=60)
-    cfg = BoxConfig(nu1=nu, nu2=nu, direct_cols=128, **base)
-    box = PeriodicBox(cfg, device=0)
-    box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, 1e-3, 20261206))
-    box.start()
-    box.step(2)
-    t0 = time.perf_counter()
-    box.step(steps)
-    vstep = steps / (time.perf_counter() - t0)
-    box.profile(True)
-    info = box.step(steps)
-    prof = box.profile_read()
-    box.profile(False)
-    box.close()
-    check_status("box-2049x4097 V-cycle", info)
+    th0 = box_temperature(base["nr"], base["np"], 0.5, 1.0, 1e-3, 20261206)
+
+    def vrun(prec):
+        cfg = BoxConfig(nu1=nu, nu2=nu, direct_cols=128, precision=prec, **base)
+        box = PeriodicBox(cfg, device=0)
+        box.put("th", th0)
+        box.start()
+        box.step(2)
+        t0 = time.perf_counter()
+        box.step(steps)
+        sps = steps / (time.perf_counter() - t0)
+        hist = [h.tolist() for h in box.solve_history()]
+        box.profile(True)
+        info = box.step(steps)
+        prof = box.profile_read()
+        box.profile(False)
+        fields = {f: box.get(f) for f in ("C", "th", "uel", "vel")}
+        box.close()
+        check_status("box-2049x4097 V-cycle" + (" (mixed)" if prec else ""), info)
+        return sps, hist, prof, info, fields
+
+    vstep, vhist, prof, info, f64 = vrun(0)
+    mstep, mhist, mprof, minfo, f32 = vrun(1)
+    err = {f: float(np.linalg.norm(f32[f] - f64[f]) / np.linalg.norm(f64[f])) for f in f64}
+    del f64, f32
     dcfg = BoxConfig(**base)
     dbox = PeriodicBox(dcfg, device=0)
-    dbox.put("th", box_temperature(dcfg.nr, dcfg.np, dcfg.r0, dcfg.rmax, 1e-3, 20261206))
+    dbox.put("th", th0)
     dbox.start()
     dbox.step(2)
     t0 = time.perf_counter()
@@ -165,6 +179,7 @@ def vcycle_large_leg(steps=6, nu=1):
     vb = sum(prof[k]["bytes"] for k in MG_KERNELS)
     vt = sum(prof[k]["ms"] for k in MG_KERNELS)
     z = prof["zebra"]
+    mt = sum(mprof[k]["ms"] for k in MG_KERNELS)
     return {"workload": "box-2049x4097 (nr=2048, np=4096), V(%d,%d), direct level 128 columns" % (nu, nu),
             "vcycle_gbs": vb / (vt * 1e-3) / 1e9 if vt > 0 else None,
             "vcycle_frac": (vb / (vt * 1e-3) / 1e9) / HBM_PEAK_GBS if vt > 0 else None,
@@ -172,7 +187,19 @@ def vcycle_large_leg(steps=6, nu=1):
             "zebra_avg_launch_us": 1e3 * z["ms"] / max(z["launches"], 1),
             "vcycle_ms_per_step": vt / steps,
             "poisson_cycles_last_step": [info.cycles_v, info.cycles_u],
+            "residual_history_last_step": vhist,
             "vcycle_steps_per_s": vstep,
+            "mixed_precision": {
+                "scheme": "FP64 defect correction (residual, u += e, direct 128-column level, stop test) around "
+                          "an FP32 V-cycle (zebra sweeps, restriction, prolongation)",
+                "steps_per_s": mstep, "vcycle_ms_per_step": mt / steps,
+                "zebra_ms_per_step": mprof["zebra"]["ms"] / steps,
+                "fp64_zebra_ms_per_step": z["ms"] / steps,
+                "poisson_cycles_last_step": [minfo.cycles_v, minfo.cycles_u],
+                "relres_last_step": [minfo.relres_v, minfo.relres_u],
+                "residual_history_last_step": mhist,
+                "field_rel_l2_vs_fp64_run": err,
+                "speedup_vs_fp64_vcycle": mstep / vstep},
             "direct_steps_per_s": dstep,
             "direct_cycles_last_step": [dinfo.cycles_v, dinfo.cycles_u]}
 
@@ -718,6 +745,8 @@ def main():
                     help="measure only the meridional plane's 5000-step run and print its JSON object")
     ap.add_argument("--merid-launches", action="store_true",
                     help="with --merid-only: launch each step's kernels one by one instead of replaying graphs")
+    ap.add_argument("--vcycle-only", action="store_true",
+                    help="measure only the large V-cycle (FP64, mixed precision, direct) and print its JSON object")
     ap.add_argument("--rot3d-only", action="store_true",
                     help="measure only the 3D annulus step (rot3dmg.f) and print its JSON object")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -732,6 +761,9 @@ def main():
         return
     if a.serve_only:
         print(json.dumps(serve_leg(slots=a.slots, baseline=not a.no_baseline)))
+        return
+    if a.vcycle_only:
+        print(json.dumps(vcycle_large_leg()))
         return
     if a.merid_only:
         print(json.dumps(merid_leg(cpu=not a.no_cpu, graphs=not a.merid_launches)))

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -377,9 +378,10 @@ int vv_destroy(vv_run *r)
         double *fields[] = {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel,
                             r->uelp, r->velp, r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC,
                             r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2, r->mV2, r->rC, r->rT, r->tab_r1, r->tab_rT,
-                            r->tab_p, r->mg_pool, r->mg_ftw, r->diag, r->part, r->kepart};
+                            r->tab_p, r->mg_pool, r->mg_ftw, r->mg_hist, r->diag, r->part, r->kepart};
         for (double *f : fields)
             if (f) (void)hipFree(f);
+        if (r->mg_pool32) (void)hipFree(r->mg_pool32);
         if (r->mg_state) (void)hipFree(r->mg_state);
         for (hipEvent_t e : r->ev_pool) (void)hipEventDestroy(e);
         if (r->ev0) (void)hipEventDestroy(r->ev0);
@@ -579,6 +581,27 @@ int vv_update(vv_run *r, int *nclamp)
     pd = Pending{};
     r->steps += 1;
     if (nclamp) *nclamp = nc;
+    return VV_OK;
+}
+
+int vv_solve_history(vv_run *r, double *relres_v, double *relres_u, int max, int *ncycles)
+{
+    if (!r || max < 0 || (max > 0 && (!relres_v || !relres_u))) { set_error("vv_solve_history: bad argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    int cyc[2] = {0, 0};
+    VV_HIP(hipMemcpy(cyc, r->mg.cycles, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> h(2 * vv::MG_HIST);
+    VV_HIP(hipMemcpy(h.data(), r->mg_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < max; ++k) {
+        relres_v[k] = (k < std::min(cyc[0], vv::MG_HIST)) ? h[k] : 0.0;
+        relres_u[k] = (k < std::min(cyc[1], vv::MG_HIST)) ? h[vv::MG_HIST + k] : 0.0;
+    }
+    if (ncycles) {
+        ncycles[0] = cyc[0];
+        ncycles[1] = cyc[1];
+    }
     return VV_OK;
 }
 

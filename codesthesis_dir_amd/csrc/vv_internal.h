@@ -76,6 +76,11 @@ struct MgLevelDev {
     // not a power of two, (cos, -sin)(2 pi t/ncol), t < ncol, of the
     // mixed-radix combination, in a buffer of their own; null elsewhere
     const double *ftw;
+    // mixed precision (VV_PREC_MIXED): the level's FP32 iterate buffers,
+    // right-hand side, black-line changes and PCR tables (smoothed levels);
+    // on the direct level only u32, the FP32 copy of its FP64 correction
+    float *u32 = nullptr, *u2_32 = nullptr, *f32 = nullptr, *dlt32 = nullptr;
+    const float *lu32 = nullptr;
 };
 // default of vv_config.direct_cols (0): the whole plane is the direct
 // level up to 8192 columns (measured faster than the V-cycle at every grid
@@ -96,10 +101,12 @@ struct MgProblemDev {
     int nlev;
     const double *sten;   // 5*n stencil of the r operator D_r (i-2..i+2)
     const double *rphi;   // n: row factor of the phi coupling (1/r^2; 1 in the box)
+    const float *sten32 = nullptr, *rphi32 = nullptr;   // their FP32 copies (mixed precision)
     MgLevelDev lev[MG_MAXLEV];
 };
 
 constexpr int MG_NORM_MAXB = 1024;   // workgroups of the residual-norm kernel
+constexpr int MG_HIST = 64;          // cycles of a solve whose residuals are kept
 
 // one batch of problems solved together (v and u of one run); host-side
 // description of the hierarchy and the solver state (device pointers)
@@ -110,6 +117,7 @@ struct MgBatchDev {
     int *cycles;          // [nprob]
     unsigned long long *fnorm;  // [nprob] max |f| bits (non-negative doubles)
     double *relres;       // [nprob]
+    double *hist;         // [nprob][MG_HIST] relres after each cycle
     double *npart;        // [nprob][MG_NORM_MAXB] per-workgroup residual maxima
     const double *fpart;  // [nprob][nbr] per-workgroup |f| maxima of the rhs kernel
     int nbr;
@@ -157,6 +165,8 @@ struct vv_run {
     vv::MgBatchDev mg{};
     double *mg_pool = nullptr;
     double *mg_ftw = nullptr;   // FFT twiddles of the direct level (see MgLevelDev::ftw)
+    float *mg_pool32 = nullptr; // FP32 hierarchy of the mixed-precision V-cycle
+    double *mg_hist = nullptr;  // [2][MG_HIST] relative residual after each cycle of the last solve
     // solver state, one device block read back with a single copy:
     //   bytes 0-31 int done[2], cycles[2], unused[4];
     //   32-63 u64 unused[2], fnorm[2] (bits of non-negative doubles);

@@ -49,81 +49,92 @@ constexpr int MAXS = 11;       // cyclic-reduction steps for n <= MAXN
 enum { ZB_PLAIN = 0, ZB_RESTRICT = 1, ZB_PROLONG = 2 };
 
 // Level and problem descriptors are kernel arguments (passed by value), so
-// no dependent load precedes a kernel's first data load.
-struct KLev {
-    double *u, *uo, *f;     // current iterate, output buffer of a sweep, rhs
-    const double *lu, *twid;
+// no dependent load precedes a kernel's first data load.  T is the type of
+// the smoothed levels: double, or float for the inner cycle of the mixed
+// precision defect correction (the direct level is double either way).
+template <class T>
+struct KLevT {
+    T *u, *uo, *f;          // current iterate, output buffer of a sweep, rhs
+    const T *lu;
+    const double *twid;
     const double2 *ftw;     // direct level: FFT twiddles (null: dense transforms)
     double *xm;
-    double *dlt;            // black-line changes of the last pre-sweep (level L-1)
+    T *dlt;                 // black-line changes of the last pre-sweep (level L-1)
     double cphi;
     int ncol, nstep, tabsz;
 };
-struct KProb {
-    const double *sten, *rphi;
+template <class T>
+struct KProbT {
+    const T *sten, *rphi;
     int n;
-    KLev L, F, Cc;     // the launch's level, the finer level, the coarser level
+    KLevT<T> L, F, Cc;      // the launch's level, the finer level, the coarser level
 };
-struct KArgs {
-    KProb P[MG_MAXPROB];
+template <class T>
+struct KArgsT {
+    KProbT<T> P[MG_MAXPROB];
     const int *done, *cycles;
     int mode, lev0;
     int store_delta;        // k_mg_zebra: store the black-line changes in L.dlt
 };
+using KLev = KLevT<double>;
+using KProb = KProbT<double>;
+using KArgs = KArgsT<double>;
 
 __device__ __forceinline__ size_t cix(int j, int q, int n) { return (size_t)q * n + j; }
 
 // residual f - (D_r u + cphi*rphi*(u_{q+1} - 2u + u_{q-1})) at (j, q)
-__device__ __forceinline__ double resid_at(const KProb &P, const KLev &L, int j, int q)
+template <class T>
+__device__ __forceinline__ T resid_at(const KProbT<T> &P, const KLevT<T> &L, int j, int q)
 {
     const int n = P.n, ncol = L.ncol;
-    const double *u = L.u;
-    const double *s = P.sten + 5 * j;
-    double lu = 0.0;
+    const T *u = L.u;
+    const T *s = P.sten + 5 * j;
+    T lu = 0;
 #pragma unroll
     for (int o = -2; o <= 2; ++o) {
         const int jj = j + o;
         if (jj >= 0 && jj < n && s[o + 2] != 0.0) lu += s[o + 2] * u[cix(jj, q, n)];
     }
-    const double uc = u[cix(j, q, n)];
+    const T uc = u[cix(j, q, n)];
     if (ncol > 1) {
         const int qm = (q == 0) ? ncol - 1 : q - 1;
         const int qp = (q == ncol - 1) ? 0 : q + 1;
-        lu += L.cphi * P.rphi[j] * (u[cix(j, qp, n)] - 2.0 * uc + u[cix(j, qm, n)]);
+        lu += (T)L.cphi * P.rphi[j] * (u[cix(j, qp, n)] - (T)2 * uc + u[cix(j, qm, n)]);
     }
     return L.f[cix(j, q, n)] - lu;
 }
 
 // Restricted residual of the finer level F at coarse line qc, row j (full
 // weighting along phi, centred on fine line 2qc+1).
-__device__ __forceinline__ double restricted_at(const KProb &P, const KLev &F, int j, int qc)
+template <class T>
+__device__ __forceinline__ T restricted_at(const KProbT<T> &P, const KLevT<T> &F, int j, int qc)
 {
     const int q0 = 2 * qc, q1 = 2 * qc + 1, q2 = (2 * qc + 2) % F.ncol;
-    const double r0 = resid_at(P, F, j, q0);
-    const double r1 = resid_at(P, F, j, q1);
-    const double r2 = resid_at(P, F, j, q2);
-    return 0.25 * r0 + 0.5 * r1 + 0.25 * r2;
+    const T r0 = resid_at(P, F, j, q0);
+    const T r1 = resid_at(P, F, j, q1);
+    const T r2 = resid_at(P, F, j, q2);
+    return (T)0.25 * r0 + (T)0.5 * r1 + (T)0.25 * r2;
 }
 
 // The reduction coefficients of one line operator for the rows a thread owns
 // (rows tid + k*nt, k < R): alpha and beta of each of the S steps and the
 // final 1/diagonal, in registers.
-template <int R>
-__device__ __forceinline__ void pcr_coef(const double *__restrict__ tab, int S, int n, double a[R][MAXS],
-                                         double b[R][MAXS], double di[R])
+template <int R, class T>
+__device__ __forceinline__ void pcr_coef(const T *__restrict__ tab, int S, int n, T a[R][MAXS],
+                                         T b[R][MAXS], T di[R])
 {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
+    const T *alpha = tab + 2, *beta = tab + 2 + (size_t)S * n, *dinv = tab + 2 + 2 * (size_t)S * n;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
         const bool in = i < n;
 #pragma unroll
         for (int t = 0; t < MAXS; ++t) {
-            a[k][t] = (in && t < S) ? alpha[(size_t)t * n + i] : 0.0;
-            b[k][t] = (in && t < S) ? beta[(size_t)t * n + i] : 0.0;
+            a[k][t] = (in && t < S) ? alpha[(size_t)t * n + i] : (T)0;
+            b[k][t] = (in && t < S) ? beta[(size_t)t * n + i] : (T)0;
         }
-        di[k] = in ? dinv[i] : 0.0;
+        di[k] = in ? dinv[i] : (T)0;
     }
 }
 
@@ -133,10 +144,10 @@ __device__ __forceinline__ void pcr_coef(const double *__restrict__ tab, int S, 
 // closure rows are folded here (tab[0], tab[1]), otherwise the caller has
 // folded them.  Solutions return in x[l][k].  The caller separates two calls
 // that reuse buf by a barrier.
-template <int R, int NL, bool FOLD>
-__device__ __forceinline__ void pcr_solve(const double *__restrict__ tab, int S, int n, double (*buf)[2][MAXN],
-                                          double rhs[NL][R], double x[NL][R], const double a[R][MAXS],
-                                          const double b[R][MAXS], const double di[R])
+template <int R, int NL, bool FOLD, class T>
+__device__ __forceinline__ void pcr_solve(const T *__restrict__ tab, int S, int n, T (*buf)[2][MAXN],
+                                          T rhs[NL][R], T x[NL][R], const T a[R][MAXS],
+                                          const T b[R][MAXS], const T di[R])
 {
     const int tid = threadIdx.x, nt = blockDim.x;
 #pragma unroll
@@ -149,11 +160,11 @@ __device__ __forceinline__ void pcr_solve(const double *__restrict__ tab, int S,
     __syncthreads();
     if (FOLD && n > 2) {   // r0 -= g0 r1, r_{n-1} -= gN r_{n-2}
         const bool o0 = (tid == 0), oN = (tid == (n - 1) % nt);
-        double r0[NL], rN[NL];
+        T r0[NL], rN[NL];
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
-            r0[l] = o0 ? buf[l][0][0] - tab[0] * buf[l][0][1] : 0.0;
-            rN[l] = oN ? buf[l][0][n - 1] - tab[1] * buf[l][0][n - 2] : 0.0;
+            r0[l] = o0 ? buf[l][0][0] - tab[0] * buf[l][0][1] : (T)0;
+            rN[l] = oN ? buf[l][0][n - 1] - tab[1] * buf[l][0][n - 2] : (T)0;
         }
         __syncthreads();
 #pragma unroll
@@ -174,7 +185,7 @@ __device__ __forceinline__ void pcr_solve(const double *__restrict__ tab, int S,
                 if (i < n) {
 #pragma unroll
                     for (int l = 0; l < NL; ++l) {
-                        double v = buf[l][cur][i];
+                        T v = buf[l][cur][i];
                         if (i >= s) v += a[k][t] * buf[l][cur][i - s];
                         if (i + s < n) v += b[k][t] * buf[l][cur][i + s];
                         buf[l][cur ^ 1][i] = v;
@@ -189,18 +200,18 @@ __device__ __forceinline__ void pcr_solve(const double *__restrict__ tab, int S,
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
 #pragma unroll
-        for (int l = 0; l < NL; ++l) x[l][k] = (i < n) ? buf[l][cur][i] * di[k] : 0.0;
+        for (int l = 0; l < NL; ++l) x[l][k] = (i < n) ? buf[l][cur][i] * di[k] : (T)0;
     }
 }
 
 // pcr_coef + pcr_solve for a single use of an operator
-template <int R, int NL, bool FOLD>
-__device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S, int n,
-                                          double (*buf)[2][MAXN], double rhs[NL][R], double x[NL][R])
+template <int R, int NL, bool FOLD, class T>
+__device__ __forceinline__ void pcr_lines(const T *__restrict__ tab, int S, int n,
+                                          T (*buf)[2][MAXN], T rhs[NL][R], T x[NL][R])
 {
-    double a[R][MAXS], b[R][MAXS], di[R];
-    pcr_coef<R>(tab, S, n, a, b, di);
-    pcr_solve<R, NL, FOLD>(tab, S, n, buf, rhs, x, a, b, di);
+    T a[R][MAXS], b[R][MAXS], di[R];
+    pcr_coef<R, T>(tab, S, n, a, b, di);
+    pcr_solve<R, NL, FOLD, T>(tab, S, n, buf, rhs, x, a, b, di);
 }
 
 // One zebra sweep (both colours) of one level, from buffer u to buffer uo.
@@ -223,12 +234,12 @@ __device__ __forceinline__ void pcr_lines(const double *__restrict__ tab, int S,
 // line equations and a red line e has the residual
 //   r_e = -c rphi (delta_{e-1} + delta_{e+1}),
 // so the restriction onto the coarsest level needs only these changes.
-template <int R>
-__global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
+template <int R, class T>
+__global__ void __launch_bounds__(1024) k_mg_zebra(KArgsT<T> a)
 {
     const int p = blockIdx.y;
-    const KProb &P = a.P[p];
-    const KLev &L = P.L;
+    const KProbT<T> &P = a.P[p];
+    const KLevT<T> &L = P.L;
     const int n = P.n, ncol = L.ncol;
     // With one row per thread the coefficients are loaded once for all line
     // groups; with two rows per thread they leave no registers for that (the
@@ -238,8 +249,8 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
     const int g0 = (int)blockIdx.x * G;
     if (ncol < 2 || 2 * g0 + 1 >= ncol) return;
     const int done = a.done[p];            // its latency overlaps the loads below
-    double ca[R][MAXS], cb[R][MAXS], cdi[R];
-    if constexpr (HOIST) pcr_coef<R>(L.lu, L.nstep, n, ca, cb, cdi);
+    T ca[R][MAXS], cb[R][MAXS], cdi[R];
+    if constexpr (HOIST) pcr_coef<R, T>(L.lu, L.nstep, n, ca, cb, cdi);
     if (done) return;                       // uniform over the workgroup
     for (int g = 0; g < G; ++g) {
     const int qb = 2 * (g0 + g) + 1;
@@ -249,37 +260,37 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
     const int qr1 = (qb + 1 == ncol) ? 0 : qb + 1;
     const int qbm = (qr0 == 0) ? ncol - 1 : qr0 - 1;          // black left of qr0
     const int qbp = (qr1 == ncol - 1) ? 0 : qr1 + 1;          // black right of qr1
-    __shared__ double buf[2][2][MAXN];
+    __shared__ T buf[2][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double c = L.cphi;
+    const T c = (T)L.cphi;
     const int mode = a.mode;
-    const double *u = L.u;
-    const double *e = P.Cc.u;
-    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
+    const T *u = L.u;
+    const T *e = P.Cc.u;
+    T rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rr[0][k] = rr[1][k] = 0.0;
-        fb[k] = fr0[k] = ubo[k] = 0.0;
+        rr[0][k] = rr[1][k] = 0;
+        fb[k] = fr0[k] = ubo[k] = 0;
         if (i >= n) continue;
-        const double rp = P.rphi[i];
+        const T rp = P.rphi[i];
         if (mode == ZB_RESTRICT) {
-            const double f0 = restricted_at(P, P.F, i, qr0);
-            const double f1 = restricted_at(P, P.F, i, qr1);
+            const T f0 = restricted_at(P, P.F, i, qr0);
+            const T f1 = restricted_at(P, P.F, i, qr1);
             fb[k] = restricted_at(P, P.F, i, qb);
             fr0[k] = f0;
             rr[0][k] = f0;
             rr[1][k] = f1;
         } else if (mode == ZB_PROLONG) {
-            const double um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
-            const double ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
-            const double up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
+            const T um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
+            const T ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
+            const T up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (um0 + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + up1);
             fb[k] = L.f[cix(i, qb, n)];
             ubo[k] = ub;
         } else {
-            const double ub = u[cix(i, qb, n)];
+            const T ub = u[cix(i, qb, n)];
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (u[cix(i, qbm, n)] + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + u[cix(i, qbp, n)]);
             fb[k] = L.f[cix(i, qb, n)];
@@ -296,15 +307,15 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
             }
         }
     }
-    if constexpr (HOIST) pcr_solve<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr, ca, cb, cdi);
-    else pcr_lines<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr);
+    if constexpr (HOIST) pcr_solve<R, 2, true, T>(L.lu, L.nstep, n, buf, rr, xr, ca, cb, cdi);
+    else pcr_lines<R, 2, true, T>(L.lu, L.nstep, n, buf, rr, xr);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : 0.0;
+        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : (T)0;
     }
-    if constexpr (HOIST) pcr_solve<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb, ca, cb, cdi);
-    else pcr_lines<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb);
+    if constexpr (HOIST) pcr_solve<R, 1, true, T>(L.lu, L.nstep, n, buf, rb, xb, ca, cb, cdi);
+    else pcr_lines<R, 1, true, T>(L.lu, L.nstep, n, buf, rb, xb);
     const bool sd = a.store_delta;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -321,12 +332,12 @@ __global__ void __launch_bounds__(1024) k_mg_zebra(KArgs a)
 // The same sweep with one line group per workgroup and the coefficients
 // loaded by each solve: for lines of more than 1024 rows (two rows per
 // thread), where holding the coefficients across both solves spills.
-template <int R>
-__global__ void __launch_bounds__(1024) k_mg_zebra1(KArgs a)
+template <int R, class T>
+__global__ void __launch_bounds__(1024) k_mg_zebra1(KArgsT<T> a)
 {
     const int p = blockIdx.y;
-    const KProb &P = a.P[p];
-    const KLev &L = P.L;
+    const KProbT<T> &P = a.P[p];
+    const KLevT<T> &L = P.L;
     const int n = P.n, ncol = L.ncol;
     const int qb = 2 * blockIdx.x + 1;
     if (ncol < 2 || qb >= ncol) return;
@@ -335,37 +346,37 @@ __global__ void __launch_bounds__(1024) k_mg_zebra1(KArgs a)
     const int qr1 = (qb + 1 == ncol) ? 0 : qb + 1;
     const int qbm = (qr0 == 0) ? ncol - 1 : qr0 - 1;          // black left of qr0
     const int qbp = (qr1 == ncol - 1) ? 0 : qr1 + 1;          // black right of qr1
-    __shared__ double buf[2][2][MAXN];
+    __shared__ T buf[2][2][MAXN];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const double c = L.cphi;
+    const T c = (T)L.cphi;
     const int mode = a.mode;
-    const double *u = L.u;
-    const double *e = P.Cc.u;
-    double rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
+    const T *u = L.u;
+    const T *e = P.Cc.u;
+    T rr[2][R], xr[2][R], rb[1][R], xb[1][R], fb[R], fr0[R], ubo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rr[0][k] = rr[1][k] = 0.0;
-        fb[k] = fr0[k] = ubo[k] = 0.0;
+        rr[0][k] = rr[1][k] = 0;
+        fb[k] = fr0[k] = ubo[k] = 0;
         if (i >= n) continue;
-        const double rp = P.rphi[i];
+        const T rp = P.rphi[i];
         if (mode == ZB_RESTRICT) {
-            const double f0 = restricted_at(P, P.F, i, qr0);
-            const double f1 = restricted_at(P, P.F, i, qr1);
+            const T f0 = restricted_at(P, P.F, i, qr0);
+            const T f1 = restricted_at(P, P.F, i, qr1);
             fb[k] = restricted_at(P, P.F, i, qb);
             fr0[k] = f0;
             rr[0][k] = f0;
             rr[1][k] = f1;
         } else if (mode == ZB_PROLONG) {
-            const double um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
-            const double ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
-            const double up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
+            const T um0 = u[cix(i, qbm, n)] + e[cix(i, qbm >> 1, n)];
+            const T ub = u[cix(i, qb, n)] + e[cix(i, qb >> 1, n)];
+            const T up1 = u[cix(i, qbp, n)] + e[cix(i, qbp >> 1, n)];
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (um0 + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + up1);
             fb[k] = L.f[cix(i, qb, n)];
             ubo[k] = ub;
         } else {
-            const double ub = u[cix(i, qb, n)];
+            const T ub = u[cix(i, qb, n)];
             rr[0][k] = L.f[cix(i, qr0, n)] - c * rp * (u[cix(i, qbm, n)] + ub);
             rr[1][k] = L.f[cix(i, qr1, n)] - c * rp * (ub + u[cix(i, qbp, n)]);
             fb[k] = L.f[cix(i, qb, n)];
@@ -383,13 +394,13 @@ __global__ void __launch_bounds__(1024) k_mg_zebra1(KArgs a)
             }
         }
     }
-    pcr_lines<R, 2, true>(L.lu, L.nstep, n, buf, rr, xr);
+    pcr_lines<R, 2, true, T>(L.lu, L.nstep, n, buf, rr, xr);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
-        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : 0.0;
+        rb[0][k] = (i < n) ? fb[k] - c * P.rphi[i] * (xr[0][k] + xr[1][k]) : (T)0;
     }
-    pcr_lines<R, 1, true>(L.lu, L.nstep, n, buf, rb, xb);
+    pcr_lines<R, 1, true, T>(L.lu, L.nstep, n, buf, rb, xb);
     const bool sd = a.store_delta;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -407,21 +418,65 @@ __global__ void __launch_bounds__(1024) k_mg_zebra1(KArgs a)
 // line 2k+1, whose red neighbours carry -c rphi (delta_{k-1} + delta_k) and
 // -c rphi (delta_k + delta_{k+1}), and the black lines have no residual, so
 // full weighting gives f_L = -c/4 rphi (delta_{k-1} + 2 delta_k + delta_{k+1}).
-__global__ void __launch_bounds__(256) k_mg_restrict(KArgs a)
+// T: the type of the finer level's changes; the coarsest level's f (fp)
+// is double.
+template <class T>
+__global__ void __launch_bounds__(256) k_mg_restrict(KArgsT<T> a, double *f0, double *f1)
 {
     const int p = blockIdx.y;
-    const KProb &P = a.P[p];
-    const KLev &L = P.L;
-    const int n = P.n, C = L.ncol;
+    const KProbT<T> &P = a.P[p];
+    const int n = P.n, C = P.F.ncol / 2;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n * C) return;
     const int done = a.done[p];
     const int k = (int)(t / n), j = (int)(t - (long long)k * n);
     const int km = (k == 0) ? C - 1 : k - 1, kp = (k == C - 1) ? 0 : k + 1;
-    const double *d = P.F.dlt;
-    const double v = -0.25 * P.F.cphi * P.rphi[j] * (d[cix(j, km, n)] + 2.0 * d[cix(j, k, n)] + d[cix(j, kp, n)]);
+    const T *d = P.F.dlt;
+    const double v = -0.25 * P.F.cphi * (double)P.rphi[j] *
+                     ((double)d[cix(j, km, n)] + 2.0 * (double)d[cix(j, k, n)] + (double)d[cix(j, kp, n)]);
     if (done) return;
-    L.f[cix(j, k, n)] = v;
+    (p ? f1 : f0)[cix(j, k, n)] = v;
+}
+
+// Mixed precision: the defect of the FP64 iterate on level 0, f - Lu, as the
+// FP32 right-hand side of the correction equation, whose iterate starts at
+// zero (FP32 level-0 buffers f32, u32 of each problem).
+struct MixPtr {
+    float *f[MG_MAXPROB], *u[MG_MAXPROB];
+    const double *src[MG_MAXPROB];
+    double *dst[MG_MAXPROB];
+    long long cnt[MG_MAXPROB];
+};
+__global__ void __launch_bounds__(256) k_mg_defect(KArgs a, MixPtr m)
+{
+    const int p = blockIdx.y;
+    if (a.done[p]) return;
+    const KProb &P = a.P[p];
+    const long long cnt = (long long)P.n * P.L.ncol;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / P.n), j = (int)(t - (long long)q * P.n);
+        m.f[p][t] = (float)resid_at(P, P.L, j, q);
+        m.u[p][t] = 0.0f;
+    }
+}
+// the direct level's FP64 correction -> its FP32 copy (src -> u)
+__global__ void __launch_bounds__(256) k_mg_to32(const int *done, MixPtr m)
+{
+    const int p = blockIdx.y;
+    if (done[p]) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m.cnt[p];
+         t += (long long)gridDim.x * blockDim.x)
+        m.u[p][t] = (float)m.src[p][t];
+}
+// u += e: the FP32 correction of level 0 added to the FP64 iterate (u -> dst)
+__global__ void __launch_bounds__(256) k_mg_correct(const int *done, MixPtr m)
+{
+    const int p = blockIdx.y;
+    if (done[p]) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m.cnt[p];
+         t += (long long)gridDim.x * blockDim.x)
+        m.dst[p][t] += (double)m.u[p][t];
 }
 
 // Exact solve of the coarsest level with C = ncol columns: the operator is
@@ -555,8 +610,8 @@ __global__ void __launch_bounds__(1024) k_mg_modes(KArgs a)
     }
     if (done) return;
     const double *tab = L.lu + (size_t)m * L.tabsz;
-    if (pair) pcr_lines<R, 2, true>(tab, L.nstep, n, buf, rhs, x);
-    else pcr_lines<R, 1, true>(tab, L.nstep, n, buf, rhs, x);
+    if (pair) pcr_lines<R, 2, true, double>(tab, L.nstep, n, buf, rhs, x);
+    else pcr_lines<R, 1, true, double>(tab, L.nstep, n, buf, rhs, x);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * nt;
@@ -696,6 +751,7 @@ struct StopArgs {
     int *done, *cycles;
     unsigned long long *fnorm;
     double *relres;
+    double *hist;      // [MG_MAXPROB][MG_HIST]: relres after each cycle
     const double *npart, *fpart;
     int nbn, nbr, maxc;
     double rtol;
@@ -740,6 +796,7 @@ __global__ void __launch_bounds__(256) k_mg_stop(StopArgs a)
         const int c = cy[p] + 1;
         a.cycles[p] = c;
         a.relres[p] = rel;
+        if (c <= MG_HIST) a.hist[p * MG_HIST + c - 1] = rel;
         if (!(rel > a.rtol) || c >= a.maxc) a.done[p] = 1;
     }
     // deferred solves: a problem whose relative residual is above rtol (or
@@ -985,6 +1042,51 @@ int mg_build(vv_run *r)
     }
     VV_HIP(hipMemcpyAsync(r->mg_pool, host.data(), total * sizeof(double), hipMemcpyHostToDevice,
                           r->stream));
+    std::vector<float> host32;
+    if (r->cfg.precision == VV_PREC_MIXED && nlev > 1) {
+        // the FP32 hierarchy of the inner cycle: stencil and row factors,
+        // per smoothed level u32, u2_32, f32 (n c each), the black-line
+        // changes of level nlev-2 and the PCR tables; the direct level's
+        // FP32 copy u32 of its correction
+        size_t t32 = 0;
+        for (int p = 0; p < 2; ++p) {
+            const int n = nvec[p];
+            t32 += 6 * (size_t)n;
+            for (int l = 0; l < nlev; ++l) {
+                const LevHost &H = lh[p][l];
+                if (H.direct) { t32 += (size_t)n * H.ncol; continue; }
+                t32 += 3 * (size_t)n * H.ncol + H.tab.size();
+                if (l == nlev - 2) t32 += (size_t)n * (H.ncol / 2);
+            }
+        }
+        VV_HIP(hipMalloc(&r->mg_pool32, t32 * sizeof(float)));
+        host32.assign(t32, 0.0f);
+        size_t o = 0;
+        auto put = [&](const double *src, size_t cnt) {
+            for (size_t q = 0; q < cnt; ++q) host32[o + q] = (float)src[q];
+            float *d = r->mg_pool32 + o;
+            o += cnt;
+            return d;
+        };
+        for (int p = 0; p < 2; ++p) {
+            const int n = nvec[p];
+            MgProblemDev &P = r->mg.prob[p];
+            P.sten32 = put(sten_h[p].data(), 5 * (size_t)n);
+            P.rphi32 = put(rphi_h[p].data(), (size_t)n);
+            for (int l = 0; l < nlev; ++l) {
+                MgLevelDev &L = P.lev[l];
+                const LevHost &H = lh[p][l];
+                L.u32 = r->mg_pool32 + o; o += (size_t)n * H.ncol;
+                if (H.direct) continue;
+                L.u2_32 = r->mg_pool32 + o; o += (size_t)n * H.ncol;
+                L.f32 = r->mg_pool32 + o; o += (size_t)n * H.ncol;
+                if (l == nlev - 2) { L.dlt32 = r->mg_pool32 + o; o += (size_t)n * (H.ncol / 2); }
+                L.lu32 = put(H.tab.data(), H.tab.size());
+            }
+        }
+        VV_HIP(hipMemcpyAsync(r->mg_pool32, host32.data(), t32 * sizeof(float), hipMemcpyHostToDevice,
+                              r->stream));
+    }
     {   // FFT twiddles of the direct level (both problems have its columns)
         const int C = r->mg.prob[0].lev[nlev - 1].ncol;
         std::vector<double> tw;
@@ -1013,6 +1115,9 @@ int mg_build(vv_run *r)
     r->mg.cycles = r->mg_flags + 2;
     r->mg.fnorm = r->mg_bits + 2;
     r->mg.relres = r->mg_relres;
+    VV_HIP(hipMalloc(&r->mg_hist, 2 * MG_HIST * sizeof(double)));
+    VV_HIP(hipMemsetAsync(r->mg_hist, 0, 2 * MG_HIST * sizeof(double), r->stream));
+    r->mg.hist = r->mg_hist;
     r->mg.npart = r->npart;
     r->mg.fpart = r->fpart;
     r->mg.nbr = r->nbx;
@@ -1062,6 +1167,41 @@ static KArgs kargs(const vv_run *r, int lev, int mode, const int *cur)
     return a;
 }
 
+// The FP32 descriptors of the mixed-precision inner cycle: the smoothed
+// levels' FP32 buffers and tables; the direct level appears only as the
+// coarser neighbour whose FP32 copy u32 the prolongation reads.
+static KLevT<float> klev32(const MgLevelDev &L, int cur)
+{
+    KLevT<float> k;
+    std::memset(&k, 0, sizeof(k));
+    k.u = cur ? L.u2_32 : L.u32;
+    k.uo = cur ? L.u32 : L.u2_32;
+    k.f = L.f32; k.lu = L.lu32; k.dlt = L.dlt32;
+    k.cphi = L.cphi; k.ncol = L.ncol; k.nstep = L.nstep; k.tabsz = (int)L.tabsz;
+    return k;
+}
+
+static KArgsT<float> kargs32(const vv_run *r, int lev, int mode, const int *cur)
+{
+    KArgsT<float> a;
+    std::memset(&a, 0, sizeof(a));
+    for (int p = 0; p < MG_MAXPROB; ++p) {
+        const MgProblemDev &Q = r->mg.prob[p];
+        KProbT<float> &P = a.P[p];
+        P.sten = Q.sten32;
+        P.rphi = Q.rphi32;
+        P.n = Q.n;
+        P.L = klev32(Q.lev[lev], cur[lev]);
+        if (lev > 0) P.F = klev32(Q.lev[lev - 1], cur[lev - 1]);
+        if (lev + 1 < Q.nlev) P.Cc = klev32(Q.lev[lev + 1], cur[lev + 1]);
+    }
+    a.done = r->mg.done;
+    a.cycles = r->mg.cycles;
+    a.mode = mode;
+    a.lev0 = (lev == 0);
+    return a;
+}
+
 // One V(nu1,nu2) cycle of both problems.  Levels 0..L-1 are smoothed by
 // zebra sweeps (both colours per launch, buffer u -> uo, so a level's
 // current buffer alternates), level L is solved exactly.  The restriction
@@ -1080,27 +1220,53 @@ static int enqueue_cycle(vv_run *r, int *check = nullptr)
     int cur[MG_MAXLEV] = {0};
     hipEvent_t pe;
     auto ncol = [&](int l) { return B.prob[0].lev[l].ncol; };
+    // mixed precision: the smoothed levels run the FP32 correction equation
+    const bool mixed = (r->cfg.precision == VV_PREC_MIXED) && L > 0;
+    const double wb = mixed ? 4.0 : 8.0;   // bytes of a smoothed level's word
     auto sweep = [&](int l, int mode, int store_delta) {
         // algorithmic words per point N of the level: plain 2.5 (read f and
         // the old black lines, write u); restriction 6 (read the finer u and
         // f, write f and u); prolongation 3 (plain + the coarse correction
         // at the black lines)
         const double w = (mode == ZB_RESTRICT) ? 6.0 : (mode == ZB_PROLONG) ? 3.0 : 2.5;
-        const double bytes = 8.0 * w * ntot * ncol(l);
-        KArgs a = kargs(r, l, mode, cur);
-        a.store_delta = store_delta;
+        const double bytes = wb * w * ntot * ncol(l);
         prof_begin(r, &pe);
         // line groups per workgroup: one on small levels, enough to keep
         // about 512 workgroups per problem on large ones
         const int nb = ncol(l) / 2, G = two ? 1 : (nb + 511) / 512;
         const dim3 grid((unsigned)((nb + G - 1) / G), 2);
-        if (two)
-            hipLaunchKernelGGL(k_mg_zebra1<2>, grid, dim3(zt), 0, r->stream, a);
-        else
-            hipLaunchKernelGGL(k_mg_zebra<1>, grid, dim3(zt), 0, r->stream, a);
+        if (mixed) {
+            KArgsT<float> a = kargs32(r, l, mode, cur);
+            a.store_delta = store_delta;
+            if (two)
+                hipLaunchKernelGGL((k_mg_zebra1<2, float>), grid, dim3(zt), 0, r->stream, a);
+            else
+                hipLaunchKernelGGL((k_mg_zebra<1, float>), grid, dim3(zt), 0, r->stream, a);
+        } else {
+            KArgs a = kargs(r, l, mode, cur);
+            a.store_delta = store_delta;
+            if (two)
+                hipLaunchKernelGGL((k_mg_zebra1<2, double>), grid, dim3(zt), 0, r->stream, a);
+            else
+                hipLaunchKernelGGL((k_mg_zebra<1, double>), grid, dim3(zt), 0, r->stream, a);
+        }
         prof_end(r, VV_K_ZEBRA, pe, bytes);
         cur[l] ^= 1;
     };
+    MixPtr mp;
+    std::memset(&mp, 0, sizeof(mp));
+    const long long cnt0 = (long long)n * ncol(0);
+    const int nbf = std::max(1, std::min(2048, nblk(cnt0, 256)));
+    if (mixed) {
+        // the defect of the FP64 iterate -> FP32 right-hand side, zero correction
+        for (int p = 0; p < 2; ++p) {
+            mp.f[p] = B.prob[p].lev[0].f32;
+            mp.u[p] = B.prob[p].lev[0].u32;
+        }
+        prof_begin(r, &pe);
+        hipLaunchKernelGGL(k_mg_defect, dim3(nbf, 2), dim3(256), 0, r->stream, kargs(r, 0, ZB_PLAIN, cur), mp);
+        prof_end(r, VV_K_NORM, pe, (16.0 + 8.0) * ntot * ncol(0));
+    }
     for (int l = 0; l < L; ++l)
         for (int s = 0; s < r->cfg.nu1; ++s)
             sweep(l, (l > 0 && s == 0) ? ZB_RESTRICT : ZB_PLAIN, (l == L - 1 && s == r->cfg.nu1 - 1) ? 1 : 0);
@@ -1110,8 +1276,14 @@ static int enqueue_cycle(vv_run *r, int *check = nullptr)
         const KArgs a = kargs(r, L, ZB_PLAIN, cur);
         if (L > 0) {
             prof_begin(r, &pe);
-            hipLaunchKernelGGL(k_mg_restrict, dim3(nblk((long long)n * C, 256), 2), dim3(256), 0, r->stream, a);
-            prof_end(r, VV_K_RESTRICT, pe, 8.0 * ntot * 2.0 * C);
+            const dim3 rg(nblk((long long)n * C, 256), 2);
+            if (mixed)
+                hipLaunchKernelGGL(k_mg_restrict<float>, rg, dim3(256), 0, r->stream, kargs32(r, L, ZB_PLAIN, cur),
+                                   B.prob[0].lev[L].f, B.prob[1].lev[L].f);
+            else
+                hipLaunchKernelGGL(k_mg_restrict<double>, rg, dim3(256), 0, r->stream, a, B.prob[0].lev[L].f,
+                                   B.prob[1].lev[L].f);
+            prof_end(r, VV_K_RESTRICT, pe, (wb + 8.0) * ntot * C);
         }
         const int TM = C < DFT_TM ? C : DFT_TM;
         const dim3 gd(nblk(n, DFT_TN), 2, (C + TM - 1) / TM);
@@ -1147,17 +1319,35 @@ static int enqueue_cycle(vv_run *r, int *check = nullptr)
         else if (fft && fR == 8) hipLaunchKernelGGL(k_mg_fft_inv<8>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
         else if (fft) hipLaunchKernelGGL(k_mg_fft_inv<32>, fg, dim3(256), flds, r->stream, a, fbits, fodd, LP);
         else hipLaunchKernelGGL(k_mg_dft<1>, gd, dim3(256), 0, r->stream, a);
+        if (mixed) {   // the correction's FP32 copy for the prolongation
+            for (int p = 0; p < 2; ++p) {
+                mp.src[p] = B.prob[p].lev[L].u;
+                mp.u[p] = B.prob[p].lev[L].u32;
+                mp.cnt[p] = (long long)B.prob[p].n * C;
+            }
+            hipLaunchKernelGGL(k_mg_to32, dim3(std::max(1, std::min(2048, nblk((long long)n * C, 256))), 2),
+                               dim3(256), 0, r->stream, (const int *)B.done, mp);
+        }
         prof_end(r, VV_K_PROLONG, pe, 24.0 * ntot * C);
     }
     for (int l = L - 1; l >= 0; --l)
         for (int s = 0; s < r->cfg.nu2; ++s) sweep(l, s == 0 ? ZB_PROLONG : ZB_PLAIN, 0);
+    if (mixed) {   // u += e in FP64 (level 0 is back in its first buffer)
+        for (int p = 0; p < 2; ++p) {
+            mp.u[p] = B.prob[p].lev[0].u32;
+            mp.dst[p] = B.prob[p].lev[0].u;
+            mp.cnt[p] = (long long)B.prob[p].n * ncol(0);
+        }
+        prof_begin(r, &pe);
+        hipLaunchKernelGGL(k_mg_correct, dim3(nbf, 2), dim3(256), 0, r->stream, (const int *)B.done, mp);
+        prof_end(r, VV_K_NORM, pe, 20.0 * ntot * ncol(0));
+    }
     const int nc0 = ncol(0);
-    const long long cnt0 = (long long)n * nc0;
     int nb = (int)((cnt0 + 255) / 256);    // one point per thread
     if (nb > MG_NORM_MAXB) nb = MG_NORM_MAXB;
     if (nb < 1) nb = 1;
     StopArgs sa;
-    sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres;
+    sa.done = B.done; sa.cycles = B.cycles; sa.fnorm = B.fnorm; sa.relres = B.relres; sa.hist = B.hist;
     sa.npart = B.npart; sa.fpart = B.fpart; sa.nbn = nb; sa.nbr = B.nbr;
     sa.maxc = r->cfg.max_cycles; sa.rtol = r->cfg.rtol;
     sa.check = check;

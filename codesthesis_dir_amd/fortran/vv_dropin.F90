@@ -46,6 +46,7 @@ module vv_dropin_c
     integer(c_int) :: max_cycles, nu1, nu2, stats, geom
     real(c_double) :: Re
     integer(c_int) :: direct_cols
+    integer(c_int) :: precision
   end type
 
   integer(c_int), parameter :: VV_C = 0, VV_CP = 1, VV_CN = 2, VV_TH = 3, VV_THP = 4, &
@@ -179,6 +180,7 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%Re = 0.0d0
 #endif
   cfg%direct_cols = 0
+  cfg%precision = 0
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl
 end subroutine

@@ -44,11 +44,12 @@ class BoxConfig:
     geom: int = 0          # 0: periodic box (rb.f); 1: rotating annulus plane (ann.f)
     Re: float = 0.0        # rotational Reynolds number (annulus)
     direct_cols: int = 0  # direct level at <= this many columns (0: direct for np <= 8192, else 128)
+    precision: int = 0    # 0: FP64 V-cycle; 1: FP64 defect correction around an FP32 V-cycle
 
     def c(self) -> L.VVConfig:
         return L.VVConfig(self.nr, self.np, self.dt, self.r0, self.rmax, self.width, self.Ra,
                           self.Pr, self.robert, self.rtol, self.max_cycles, self.nu1, self.nu2,
-                          self.stats, self.geom, self.Re, self.direct_cols)
+                          self.stats, self.geom, self.Re, self.direct_cols, self.precision)
 
 
 def AnnulusConfig(**kw) -> BoxConfig:
@@ -155,6 +156,14 @@ class PeriodicBox:
         self.steps += n
         info.status = st
         return info
+
+    def solve_history(self, max_cycles: int = 64):
+        """Relative residual after each cycle of the last solve of v and of
+        u: (v history, u history)."""
+        v, u = np.zeros(max_cycles), np.zeros(max_cycles)
+        n = (ctypes.c_int * 2)()
+        L.check(L.lib().vv_solve_history(self._run, _P(v), _P(u), max_cycles, n), "vv_solve_history")
+        return v[:min(n[0], max_cycles)], u[:min(n[1], max_cycles)]
 
     def heat_history(self, n: int) -> np.ndarray:
         out = np.zeros((n, 2))

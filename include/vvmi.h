@@ -90,7 +90,19 @@ typedef struct vv_config {
                              V-cycle at every grid from 65 x 33 to
                              8193 x 1025), a V-cycle down to 128 columns
                              above.                                         */
+    int    precision;     /* VV_PREC_FP64 (0): the V-cycle in FP64;
+                             VV_PREC_MIXED (1): FP64 defect correction around
+                             an FP32 V-cycle -- the residual f - Lu, the
+                             update u += e, the coarsest (direct) level, the
+                             stop test and every reduction in FP64; the zebra
+                             sweeps, the restriction and the prolongation of
+                             the correction equation in FP32.  Applies to
+                             V-cycles (direct_cols < np); a direct solve of
+                             the whole plane is FP64 either way.            */
 } vv_config;
+
+#define VV_PREC_FP64  0
+#define VV_PREC_MIXED 1
 
 /* per-step diagnostics (device-resident until read) */
 typedef struct vv_step_info {
@@ -161,6 +173,11 @@ int vv_poisson_host(vv_run *run, int which, double *psi, double *psip,
  * run in its forward-Euler form) without host copies; info (may be NULL)
  * receives the diagnostics of the last step. */
 int vv_step(vv_run *run, int nsteps, vv_step_info *info);
+/* The relative residual after each cycle of the last solve of v and of u,
+ * up to max entries each (0 past a solve's cycles; at most 64 are kept);
+ * ncycles[0], ncycles[1] = their cycle counts.  No counterpart in the
+ * reference. */
+int vv_solve_history(vv_run *run, double *relres_v, double *relres_u, int max, int *ncycles);
 
 /* Diagnostics of the steps run so far: trans1/trans2 per step (rb.f:384-388
  * multiplies them by (rmax-r0)/width to get the Nusselt numbers).
