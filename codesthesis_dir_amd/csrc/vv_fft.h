@@ -24,7 +24,7 @@ __device__ void fft_stages(double2 *z, const double2 *tw, int np, int bits, int 
         const int h = 1 << s;
         const int nb = LP * (np >> 1);
         for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const int pr = b / (np >> 1), t = b - pr * (np >> 1);
+            const int pr = b >> (bits - 1), t = b & ((np >> 1) - 1);   // np = 2^bits
             const int pos = t & (h - 1), grp = t >> s;
             const int i0 = pr * np + grp * 2 * h + pos, i1 = i0 + h;
             double2 w = tw[pos << (bits - 1 - s)];

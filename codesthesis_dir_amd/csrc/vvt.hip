@@ -11,13 +11,15 @@
 //
 // Poisson: each operator L = D_y + D_z + D_x (Cartesian, constant rows) is
 // solved exactly: its y rows have cosine modes (u, w: mirrored at the
-// stress-free side walls) or sine modes (v: zero there), applied as FP64
-// matrix-core GEMMs; the z rows are diagonalised once (the eigenbasis of
+// stress-free side walls) or sine modes (v: zero there), applied by FFTs in
+// LDS when np is a power of two (k_t_yfft_*) and as FP64 matrix-core GEMMs
+// otherwise; the z rows are diagonalised once (the eigenbasis of
 // rot3dmg.f's z rows, shared with the annulus solver); what remains per
 // (y mode, z eigenvalue) is one banded system along x, eliminated by one
 // thread (Thomas, the closure rows' third coefficient folded in).
 
 #include "vv3_internal.h"
+#include "vv_fft.h"
 #include "vv_gemm.h"
 #include "vv_reduce.h"
 
@@ -764,6 +766,182 @@ __global__ void __launch_bounds__(WG) k_t_ycombine(TGeo g, const double *EO, dou
     if (km != kk) psi[(unsigned)km * pl + off] = E - O;
 }
 
+// y modes by FFT (np = N a power of two): a mode transform of a line along y
+// is the 2N-point DFT of its extension -- the cells' lines (u, w: modes
+// cos(pi q (k - 1/2)/N), mirrored at the side walls) even about the walls,
+// y_j = x_{j+1}, y_{2N-1-j} = x_{j+1}, whose DFT is 2 e^{i pi q/(2N)} S_q;
+// the faces' lines (v: modes sin(pi q k/N), zero at the walls) odd,
+// y_k = x_k, y_{2N-k} = -x_k, y_0 = y_N = 0, whose DFT is -2i S_q.  Two real
+// lines n = 2p, 2p+1 go through one complex FFT (LP pairs per workgroup, in
+// LDS, the stages of vv_fft.h); the cells' spectra separate by conjugate
+// symmetry, the faces' fall apart into the real and imaginary parts.  The
+// inverse: the cells' DCT-III by the inverse DFT (times 1/(2N)) of the
+// conjugate-symmetric spectrum 2 e^{i pi q/(2N)} C_q, which is
+// sum_q C_q cos(pi q (k - 1/2)/N) / norm_q (norm_0 = N, else N/2); the
+// faces' DST-I being its own inverse (times 2/N).  tw: (cos, -sin)(2 pi t/(2N)), t < N; cq: (cos,
+// sin)(pi q/(2N)), q < N.
+using vv::bitrev;
+using vv::fft_stages;
+
+template <bool FACES>
+__global__ void __launch_bounds__(512) k_t_yfft_fwd(TGeo g, const double *__restrict__ rhs, double *__restrict__ S,
+                                                    const double2 *tw, const double2 *cq, int i1, int mt, int LP,
+                                                    int bits)
+{
+    extern __shared__ double2 z[];
+    const int N = g.np, N2 = 2 * N, nl = 2 * LP;
+    const int nb = blockIdx.x * nl;
+    const unsigned pl = (unsigned)(g.ldi * g.ldj);
+    double *zd = reinterpret_cast<double *>(z);
+    double2 *twl = z + LP * N2;   // the stages' twiddles, staged in LDS
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw[t];
+    if (FACES)
+        for (int e = threadIdx.x; e < nl * 2; e += blockDim.x) {   // y_0 = y_N = 0
+            const int l = e % nl, w = e / nl;
+            zd[2 * ((l >> 1) * N2 + (int)bitrev((unsigned)(w * N), bits)) + (l & 1)] = 0.0;
+        }
+    const int K = FACES ? N - 1 : N;
+    // element e = (k-1) nl + l: line l of plane k; the line's offset in the
+    // plane is fixed per thread (blockDim is a multiple of nl), and 16
+    // loads are in flight before their LDS stores
+    const int l = (int)threadIdx.x % nl, n = nb + l;
+    unsigned off = 0;
+    if (n < mt) {
+        const int j = n / i1, i = n - j * i1;
+        off = (unsigned)((j + 1) * g.ldi + (i + 1));
+    }
+    double *zl = zd + 2 * (l >> 1) * N2 + (l & 1);
+    constexpr int B = 16;
+    const int kstep = (int)blockDim.x / nl;
+    for (int k0 = 1 + (int)threadIdx.x / nl; k0 <= K; k0 += B * kstep) {
+        double x[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int k = k0 + u * kstep;
+            x[u] = (n < mt && k <= K) ? rhs[(unsigned)k * pl + off] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int k = k0 + u * kstep;
+            if (k > K) break;
+            if (FACES) {
+                zl[2 * (int)bitrev((unsigned)k, bits)] = x[u];
+                zl[2 * (int)bitrev((unsigned)(N2 - k), bits)] = -x[u];
+            } else {
+                zl[2 * (int)bitrev((unsigned)(k - 1), bits)] = x[u];
+                zl[2 * (int)bitrev((unsigned)(N2 - k), bits)] = x[u];
+            }
+        }
+    }
+    __syncthreads();
+    fft_stages<false>(z, twl, N2, bits, LP);
+    if (n >= mt) return;
+    const double2 *zp = z + (l >> 1) * N2;
+    for (int qi = (int)threadIdx.x / nl; qi < K; qi += kstep) {
+        double v;
+        if (FACES) {
+            const double2 Z = zp[qi + 1];
+            v = (l & 1) ? 0.5 * Z.x : -0.5 * Z.y;
+        } else {
+            const int q = qi;
+            const double2 Zm = zp[q], Zc = zp[q == 0 ? 0 : N2 - q];   // Z_q, Z_{2N-q}
+            double re, im;   // A = (Z_q + conj Z_{2N-q})/2 or B = (Z_q - conj Z_{2N-q})/(2i)
+            if ((l & 1) == 0) { re = 0.5 * (Zm.x + Zc.x); im = 0.5 * (Zm.y - Zc.y); }
+            else { re = 0.5 * (Zm.y + Zc.y); im = -0.5 * (Zm.x - Zc.x); }
+            const double2 c = cq[q];
+            v = 0.5 * (c.x * re + c.y * im);
+        }
+        S[(size_t)qi * mt + n] = v;
+    }
+}
+
+template <bool FACES>
+__global__ void __launch_bounds__(512) k_t_yfft_inv(TGeo g, const double *__restrict__ S, double *__restrict__ psi,
+                                                    const double2 *tw, const double2 *cq, int i1, int mt, int LP,
+                                                    int bits)
+{
+    extern __shared__ double2 z[];
+    const int N = g.np, N2 = 2 * N, nl = 2 * LP;
+    const int nb = blockIdx.x * nl;
+    const unsigned pl = (unsigned)(g.ldi * g.ldj);
+    double *zd = reinterpret_cast<double *>(z);
+    double2 *twl = z + LP * N2;   // the stages' twiddles, staged in LDS
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = tw[t];
+    if (FACES) {
+        for (int e = threadIdx.x; e < nl * 2; e += blockDim.x) {
+            const int l = e % nl, w = e / nl;
+            zd[2 * ((l >> 1) * N2 + (int)bitrev((unsigned)(w * N), bits)) + (l & 1)] = 0.0;
+        }
+        const int l = (int)threadIdx.x % nl, n = nb + l;
+        double *zl = zd + 2 * (l >> 1) * N2 + (l & 1);
+        constexpr int B = 16;
+        const int qstep = (int)blockDim.x / nl;
+        for (int q0 = 1 + (int)threadIdx.x / nl; q0 <= N - 1; q0 += B * qstep) {
+            double x[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int q = q0 + u * qstep;
+                x[u] = (n < mt && q <= N - 1) ? S[(size_t)(q - 1) * mt + n] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int q = q0 + u * qstep;
+                if (q > N - 1) break;
+                zl[2 * (int)bitrev((unsigned)q, bits)] = x[u];
+                zl[2 * (int)bitrev((unsigned)(N2 - q), bits)] = -x[u];
+            }
+        }
+    } else {
+        // Z_q = Y^a_q + i Y^b_q, Z_{2N-q} = conj Y^a_q + i conj Y^b_q,
+        // Y_q = 2 e^{i pi q/(2N)} C_q / norm_q, Z_N = 0
+        const int pr = (int)threadIdx.x % LP, na = nb + 2 * pr, nb1 = na + 1;
+        constexpr int B = 8;
+        const int qstep = (int)blockDim.x / LP;
+        for (int qq0 = (int)threadIdx.x / LP; qq0 < N; qq0 += B * qstep) {
+        double va[B], vb[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int q = qq0 + u * qstep;
+            va[u] = (na < mt && q < N) ? S[(size_t)q * mt + na] : 0.0;
+            vb[u] = (nb1 < mt && q < N) ? S[(size_t)q * mt + nb1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int q = qq0 + u * qstep;
+            if (q >= N) break;
+            const double ca = va[u], cb = vb[u];
+            const double2 c = cq[q];
+            const double ar = 2.0 * c.x * ca, ai = 2.0 * c.y * ca, br = 2.0 * c.x * cb, bi = 2.0 * c.y * cb;
+            double2 *zp = z + pr * N2;
+            zp[bitrev((unsigned)q, bits)] = make_double2(ar - bi, ai + br);
+            if (q > 0) zp[bitrev((unsigned)(N2 - q), bits)] = make_double2(ar + bi, br - ai);
+            else zp[bitrev((unsigned)N, bits)] = make_double2(0.0, 0.0);
+        }
+        }
+    }
+    __syncthreads();
+    if (FACES) fft_stages<false>(z, twl, N2, bits, LP);
+    else fft_stages<true>(z, twl, N2, bits, LP);
+    const int K = FACES ? N - 1 : N;
+    const double sc = FACES ? 1.0 / N : 1.0 / N2;
+    const int lo = (int)threadIdx.x % nl, no = nb + lo;
+    if (no >= mt) return;
+    const int jo = no / i1, io = no - jo * i1;
+    const unsigned off = (unsigned)((jo + 1) * g.ldi + (io + 1));
+    const double2 *zp = z + (lo >> 1) * N2;
+    for (int k = 1 + (int)threadIdx.x / nl; k <= K; k += (int)blockDim.x / nl) {
+        double v;
+        if (FACES) {
+            const double2 Z = zp[k];
+            v = (lo & 1) ? Z.x * sc : -Z.y * sc;
+        } else {
+            const double2 Z = zp[k - 1];
+            v = ((lo & 1) ? Z.y : Z.x) * sc;
+        }
+        psi[(unsigned)k * pl + off] = v;
+    }
+}
+
 // psip <- psi everywhere; psi <- 0 outside the unknowns
 __global__ void __launch_bounds__(WG) k_t_psi_pre(TGeo g, int i1, int j1, int k1, double *psi, double *psip)
 {
@@ -927,6 +1105,9 @@ struct vvt_run {
     size_t tab_off[12] = {0};
     TankOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;   // Poisson work (kmax x mmax each)
+    double2 *ytw = nullptr;       // y FFT: twiddles of 2np points (np), then (cos, sin)(pi q/(2np)) (np)
+    bool yfft = false;            // y modes by FFT (np a power of two, 8..2048)
+    int ylp = 1, ybits = 0;       // line pairs per workgroup, log2(2np)
     double *heat = nullptr;       // 2 nr column sums, trans1, trans2, then the flux terms
     double *kepart = nullptr;     // KE_MAXB partials, then the total
     // profiling
@@ -1008,6 +1189,20 @@ int setup_poisson(vvt_run *r)
     const int np = r->np;
     const double hhr = r->dr * r->dr, hhz = r->dz * r->dz, hhp = r->dp * r->dp;
     size_t mmax = 0, kmax = 0;
+    r->yfft = np >= 8 && np <= 2048 && (np & (np - 1)) == 0;
+    if (r->yfft) {
+        const int N2 = 2 * np;
+        r->ybits = __builtin_ctz((unsigned)N2);
+        // 8 lines a workgroup (64 B of each plane) in 64 KB of LDS, plus the
+        // twiddles: two workgroups per CU (16 lines in 128 KB measured slower)
+        r->ylp = std::max(1, std::min(4, 4096 / N2));
+        std::vector<double2> t(2 * (size_t)np);
+        for (int q = 0; q < np; ++q) {
+            t[q] = make_double2(std::cos(2.0 * pi * q / N2), -std::sin(2.0 * pi * q / N2));
+            t[np + q] = make_double2(std::cos(pi * q / N2), std::sin(pi * q / N2));
+        }
+        VV_HIP(upload(&r->ytw, t));
+    }
     for (int w = 1; w <= 3; ++w) {
         TankOp &O = r->op[w - 1];
         O.i1 = (w == 3) ? r->nr - 1 : r->nr;
@@ -1020,7 +1215,11 @@ int setup_poisson(vvt_run *r)
         // q = 0..np-1), v zero there (sin(pi q k/np), q = 1..np-1); each mode
         // is an eigenvector of the y rows with eigenvalue 2 cos(pi q/np) - 2
         const int k1 = O.k1;
-        std::vector<double> W((size_t)k1 * k1), Wi((size_t)k1 * k1), ly(k1);
+        std::vector<double> W, Wi, ly(k1);
+        if (!r->yfft) {
+            W.resize((size_t)k1 * k1);
+            Wi.resize((size_t)k1 * k1);
+        }
         auto mode = [&](int q, int k) {   // mode q (0-based spectral row) at plane k
             const int qq = (w == 2) ? q + 1 : q;
             return (w == 2) ? std::sin(pi * qq * k / np) : std::cos(pi * qq * (k - 0.5) / np);
@@ -1029,12 +1228,13 @@ int setup_poisson(vvt_run *r)
         for (int qi = 0; qi < k1; ++qi) {
             const int q = (w == 2) ? qi + 1 : qi;
             ly[qi] = (2.0 * std::cos(pi * q / np) - 2.0) / hhp;
+            if (r->yfft) continue;
             for (int k = 1; k <= k1; ++k) {
                 W[(size_t)qi * k1 + (k - 1)] = mode(qi, k);
                 Wi[(size_t)(k - 1) * k1 + qi] = mode(qi, k) / norm(qi);
             }
         }
-        if (np % 2 == 0) {
+        if (!r->yfft && np % 2 == 0) {
             // the parity-split halves: forward row p of half b is spectral row
             // q = 2p + b at the planes k = 1..np/2 (the faces' middle plane
             // np/2 at half weight); inverse row k of half b holds the inverse
@@ -1054,8 +1254,10 @@ int setup_poisson(vvt_run *r)
             W.swap(Ws);
             Wi.swap(Wis);
         }
-        VV_HIP(upload(&O.W, W));
-        VV_HIP(upload(&O.Wi, Wi));
+        if (!r->yfft) {
+            VV_HIP(upload(&O.W, W));
+            VV_HIP(upload(&O.Wi, Wi));
+        }
         VV_HIP(upload(&O.ly, ly));
         // z rows: rot3dmg.f's (Dirichlet for w, the wall closure for v, u)
         std::vector<double> lam, V, Vinv;
@@ -1248,7 +1450,19 @@ int launch_poisson(vvt_run *r, int which)
     }
     const bool split = (r->np % 2 == 0);
     const int h = r->np / 2;
-    {
+    const int LP = r->ylp, nbk = (mt + 2 * LP - 1) / (2 * LP);
+    const size_t flds = ((size_t)LP * 2 * r->np + r->np) * sizeof(double2);
+    const int yt = 512;    // threads: a multiple of the 2 LP lines (measured: 256 and 1024 slower)
+    const double2 *tw = r->ytw, *cq = r->ytw + r->np;
+    if (r->yfft) {   // reads the right-hand side's unknowns, writes the spectra
+        Prof p(r, VVT_K_YFFT_FWD, 8.0 * 2.0 * O.k1 * (double)mt);
+        if (which == 2)
+            hipLaunchKernelGGL(k_t_yfft_fwd<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->S,
+                               tw, cq, O.i1, mt, LP, r->ybits);
+        else
+            hipLaunchKernelGGL(k_t_yfft_fwd<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->S,
+                               tw, cq, O.i1, mt, LP, r->ybits);
+    } else {
         Prof p(r, VVT_K_YFWD, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
         if (split)
             vv::gemm_f64<true, false>(YFwdSplit{O.W, h, mt, h, r->f[rhs], r->S, O.i1, r->ldi, r->ldj, r->np,
@@ -1270,7 +1484,17 @@ int launch_poisson(vvt_run *r, int which)
         Prof p(r, VVT_K_ZINV, ftr_z);
         vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->T, r->S, mt}, O.k1, s);
     }
-    {
+    if (r->yfft) {
+        Prof p(r, VVT_K_YFFT_INV, 8.0 * 2.0 * O.k1 * (double)mt);
+        if (which == 2)
+            hipLaunchKernelGGL(k_t_yfft_inv<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->S, P, tw, cq,
+                               O.i1, mt, LP, r->ybits);
+        else
+            hipLaunchKernelGGL(k_t_yfft_inv<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->S, P, tw, cq,
+                               O.i1, mt, LP, r->ybits);
+        if (which != 2)
+            hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
+    } else {
         Prof p(r, VVT_K_YINV, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
         if (split) {
             // E and O go to T (free after the z inverse), then the planes
@@ -1445,6 +1669,7 @@ int vvt_destroy(vvt_run *r)
                 if (p) (void)hipFree(p);
         for (double *p : {r->tabs, r->S, r->T, r->cp, r->heat, r->kepart})
             if (p) (void)hipFree(p);
+        if (r->ytw) (void)hipFree(r->ytw);
         for (hipEvent_t e : r->ev)
             if (e) (void)hipEventDestroy(e);
         if (r->stream) (void)hipStreamDestroy(r->stream);

@@ -53,7 +53,8 @@ typedef struct {
 /* Allocates the fields, the derivative fields and the Poisson solver on
  * `device`: each of the three operators is solved exactly and separably
  * (cosine or sine modes along y, the eigenbasis of its z rows, a banded
- * solve along x per mode pair). */
+ * solve along x per mode pair).  The y modes are taken by FFTs in LDS when
+ * np is a power of two from 8 to 2048, by matrix-core GEMMs otherwise. */
 int vvt_create(const vvt_config *cfg, int device, vvt_run **out);
 int vvt_destroy(vvt_run *run);
 
@@ -110,6 +111,8 @@ enum vvt_kernel_id {
     VVT_K_ZINV,          /* z eigenbasis, inverse (GEMM; flops)                     */
     VVT_K_YINV,          /* y modes, inverse (GEMM; flops)                          */
     VVT_K_UPDATE,        /* boundaries and the rotation copy                        */
+    VVT_K_YFFT_FWD,      /* y modes, forward, by FFT (np a power of two; bytes)     */
+    VVT_K_YFFT_INV,      /* y modes, inverse, by FFT (bytes)                        */
     VVT_K_COUNT
 };
 int vvt_profile(vvt_run *run, int enable);
