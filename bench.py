@@ -422,7 +422,8 @@ def serve_leg(n=64, steps=2000, slots=16, baseline=True):
     res["served"] = {"slots": slots, "solves_per_s": n / wall, "wall_s": wall, "p50_s": float(np.percentile(lat, 50)),
                      "p99_s": float(np.percentile(lat, 99)), "peak_running": stats["peak_running"],
                      "peak_mem_GB": stats["peak_mem"] / 1e9, "failed": bad,
-                     "not_converged": sum(1 for g in got if g["status"] == 1)}
+                     "not_converged": sum(1 for g in got if g["status"] == 1),
+                     "runs_created": stats["created"], "runs_reused": stats["reused"]}
     if not baseline:
         return res
     with Server(slots=1, mem_budget=64e9) as srv:

@@ -392,6 +392,57 @@ int vv_destroy(vv_run *r)
     return VV_OK;
 }
 
+int vv_reset(vv_run *r, const vv_config *cfg)
+{
+    if (!r || !cfg) { set_error("vv_reset: null argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    if (!vv::same_shape(r->cfg, *cfg)) {
+        set_error("vv_reset: the configuration's grid, geometry or solver differs from the run's");
+        return VV_EINVAL;
+    }
+    if (!(cfg->dt > 0) || !(cfg->Pr > 0) || (cfg->geom == VV_GEOM_ANNULUS && cfg->Re == 0.0)) {
+        set_error("bad physical parameters");
+        return VV_EINVAL;
+    }
+    vv_config c = *cfg;
+    if (c.rtol <= 0) c.rtol = 1e-10;
+    if (c.max_cycles <= 0) c.max_cycles = 100;
+    if (c.robert < 0) c.robert = 0.02;
+    c.nu1 = r->cfg.nu1;
+    c.nu2 = r->cfg.nu2;
+    c.direct_cols = r->cfg.direct_cols;
+    VV_HIP(hipSetDevice(r->device));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    // every state buffer back to zero, as vv_create leaves it
+    for (double *f : {r->C, r->Cp, r->Cn, r->Cf, r->th, r->thp, r->thn, r->thf, r->uel, r->vel, r->uelp, r->velp,
+                      r->vrh, r->urh, r->dCr, r->dCh, r->dthrC, r->dthhC, r->dthrT, r->dthhT, r->mT2, r->mT, r->mU2,
+                      r->mV2})
+        VV_HIP(hipMemsetAsync(f, 0, r->fbytes, r->stream));
+    for (int p = 0; p < r->mg.nprob; ++p) {
+        const vv::MgProblemDev &P = r->mg.prob[p];
+        for (int l = 0; l < P.nlev; ++l) {
+            const vv::MgLevelDev &L = P.lev[l];
+            const size_t cnt = (size_t)P.n * L.ncol;
+            for (double *b : {L.u, L.f, L.u2, L.xm})
+                if (b) VV_HIP(hipMemsetAsync(b, 0, cnt * sizeof(double), r->stream));
+            if (L.dlt) VV_HIP(hipMemsetAsync(L.dlt, 0, cnt / 2 * sizeof(double), r->stream));
+            for (float *b : {L.u32, L.u2_32, L.f32})
+                if (b) VV_HIP(hipMemsetAsync(b, 0, cnt * sizeof(float), r->stream));
+            if (L.dlt32) VV_HIP(hipMemsetAsync(L.dlt32, 0, cnt / 2 * sizeof(float), r->stream));
+        }
+    }
+    VV_HIP(hipMemsetAsync(r->mg_state, 0, 128, r->stream));
+    if (r->mg_hist) VV_HIP(hipMemsetAsync(r->mg_hist, 0, 2 * vv::MG_HIST * sizeof(double), r->stream));
+    VV_HIP(hipMemsetAsync(r->diag, 0, (size_t)r->diag_cap * 2 * sizeof(double), r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    r->cfg = c;
+    r->steps = 0;
+    r->last_cycles[0] = r->last_cycles[1] = 0;
+    r->last_relres[0] = r->last_relres[1] = 0.0;
+    pending(r) = Pending{};
+    return VV_OK;
+}
+
 int vv_put(vv_run *r, int field, const double *host, int ld)
 {
     if (!r || !host || ld < r->nr + 2) { set_error("vv_put: bad argument"); return VV_EINVAL; }

@@ -87,6 +87,19 @@ struct MgLevelDev {
 // from 65 x 33 to 8193 x 1025), a V-cycle down to 128 columns above
 inline int default_direct_cols(int np) { return np <= 8192 ? np : 128; }
 
+// Whether a run created with configuration a can take configuration b
+// (vv_reset): the same grid, geometry and solver shape -- every table of a
+// run depends on these only; dt, Ra, Pr, Re, the Robert coefficient, the
+// stop and the statistics switch are kernel arguments.
+inline bool same_shape(const vv_config &a, const vv_config &b)
+{
+    auto dc = [](const vv_config &c) { return c.direct_cols > 0 ? c.direct_cols : default_direct_cols(c.np); };
+    auto nu = [](int v) { return v > 0 ? v : 1; };
+    return a.nr == b.nr && a.np == b.np && a.geom == b.geom && a.r0 == b.r0 && a.rmax == b.rmax &&
+           (a.geom != VV_GEOM_BOX || a.width == b.width) && dc(a) == dc(b) && nu(a.nu1) == nu(b.nu1) &&
+           nu(a.nu2) == nu(b.nu2) && a.precision == b.precision;
+}
+
 // a direct level of c columns is transformed by FFTs (radix 2 times the odd
 // factor of c) when this holds, by dense transforms otherwise
 inline bool fft_cols(int c) { return c >= 8 && c % 2 == 0 && c <= 8192; }

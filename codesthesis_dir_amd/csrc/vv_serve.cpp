@@ -1,6 +1,10 @@
 // vv_serve.cpp -- the request loop of include/vvserve.h: admission by slot
 // and device footprint, one worker thread per slot, each admitted run on its
-// own stream (vv_create), results posted to a completion queue.
+// own stream, results posted to a completion queue.  Runs stay resident after
+// their request: a later request of the same grid, geometry and solver takes
+// an idle run (vv_reset keeps its tables, buffers and stream) instead of
+// creating one; idle runs are destroyed when the budget needs their memory
+// or more than two per slot wait.
 
 #include "vv_internal.h"
 
@@ -24,11 +28,21 @@ struct Job {
     double t_submit;
 };
 
+struct Idle {
+    vv_run *r;
+    vv_config cfg;
+    double need;
+};
+
 }  // namespace
 
 struct vv_server {
     int device = 0;
     double budget = 0.0;
+    size_t max_idle = 0;
+    std::deque<Idle> idle;     // resident runs between requests, oldest first
+    double mem_idle = 0.0;     // their footprint (part of mem)
+    int reused = 0, created = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job> pending;
@@ -45,17 +59,34 @@ struct vv_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    // runs one admitted job start to finish on this thread
-    vv_result run(Job &j)
+    // an idle run that can take the job, or -1 (under mu)
+    int idle_match(const Job &j) const
+    {
+        for (size_t q = 0; q < idle.size(); ++q)
+            if (vv::same_shape(idle[q].cfg, j.req.cfg)) return (int)q;
+        return -1;
+    }
+    // the head of the queue can start: a resident run takes it, or the
+    // running runs leave room for a new one (idle runs can be evicted)
+    bool fits(const Job &j) const { return idle_match(j) >= 0 || mem - mem_idle + j.need <= budget; }
+
+    // runs one admitted job start to finish on this thread, on the resident
+    // run r (or a new one); r returns the run to keep (null: none)
+    vv_result run(Job &j, vv_run *&r)
     {
         vv_result res;
         std::memset(&res, 0, sizeof(res));
         res.id = j.id;
         res.user = j.req.user;
         res.t_submit = j.t_submit;
-        vv_run *r = nullptr;
         int nlev = 0;
-        int st = vv_create(&j.req.cfg, device, &r, &nlev);
+        int st;
+        if (r) {
+            st = vv_reset(r, &j.req.cfg);
+        } else {
+            st = vv_create(&j.req.cfg, device, &r, &nlev);
+            if (st) r = nullptr;
+        }
         if (!st && !j.th0.empty()) st = vv_put(r, VV_TH, j.th0.data(), j.req.ld);
         if (!st) st = vv_start(r);
         vv_step_info info;
@@ -74,7 +105,6 @@ struct vv_server {
             if (s3) st = s3;
         }
         if (st < 0) std::strncpy(res.error, vv_last_error(), sizeof(res.error) - 1);
-        if (r) vv_destroy(r);
         res.status = st;
         return res;
     }
@@ -83,32 +113,64 @@ struct vv_server {
     {
         for (;;) {
             Job j;
+            vv_run *r = nullptr;
+            double need = 0.0;          // the footprint this worker's run is counted with
+            std::vector<vv_run *> evict;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 // first come, first served: the head waits until it fits;
                 // after stop the queue is still drained
-                cv_work.wait(lk, [&] {
-                    return (!pending.empty() && mem + pending.front().need <= budget) || (stop && pending.empty());
-                });
+                cv_work.wait(lk, [&] { return (!pending.empty() && fits(pending.front())) || (stop && pending.empty()); });
                 if (pending.empty()) return;
                 j = std::move(pending.front());
                 pending.pop_front();
-                mem += j.need;
+                const int q = idle_match(j);
+                if (q >= 0) {           // a resident run of the same shape
+                    r = idle[q].r;
+                    need = idle[q].need;
+                    mem_idle -= need;
+                    idle.erase(idle.begin() + q);
+                    reused += 1;
+                } else {                // room for a new run: evict idle runs, oldest first
+                    while (mem + j.need > budget && !idle.empty()) {
+                        evict.push_back(idle.front().r);
+                        mem -= idle.front().need;
+                        mem_idle -= idle.front().need;
+                        idle.pop_front();
+                    }
+                    need = j.need;
+                    mem += need;
+                    created += 1;
+                }
                 running += 1;
                 if (running > peak_running) peak_running = running;
                 if (mem > peak_mem) peak_mem = mem;
             }
+            for (vv_run *e : evict) vv_destroy(e);
+            evict.clear();
             const double t_admit = now();
-            vv_result res = run(j);
+            vv_result res = run(j, r);
             res.t_admit = t_admit;
             res.t_done = now();
             {
                 std::lock_guard<std::mutex> lk(mu);
-                mem -= j.need;
                 running -= 1;
                 ndone += 1;
+                if (r) {                // stays resident
+                    idle.push_back(Idle{r, j.req.cfg, need});
+                    mem_idle += need;
+                    while (idle.size() > max_idle) {
+                        evict.push_back(idle.front().r);
+                        mem -= idle.front().need;
+                        mem_idle -= idle.front().need;
+                        idle.pop_front();
+                    }
+                } else {
+                    mem -= need;
+                }
                 done.push_back(res);
             }
+            for (vv_run *e : evict) vv_destroy(e);
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -155,6 +217,7 @@ int vv_server_create(int device, int slots, double mem_budget, vv_server **out)
     vv_server *s = new vv_server();
     s->device = device;
     s->budget = mem_budget;
+    s->max_idle = 2 * (size_t)slots;
     for (int q = 0; q < slots; ++q) s->workers.emplace_back([s, device] {
         (void)hipSetDevice(device);
         s->worker();
@@ -220,6 +283,9 @@ int vv_server_stats_read(vv_server *s, vv_server_stats *st)
     st->peak_running = s->peak_running;
     st->mem_in_use = s->mem;
     st->peak_mem = s->peak_mem;
+    st->resident = (int)s->idle.size();
+    st->reused = s->reused;
+    st->created = s->created;
     return VV_OK;
 }
 
@@ -232,6 +298,8 @@ int vv_server_destroy(vv_server *s)
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
+    (void)hipSetDevice(s->device);
+    for (Idle &e : s->idle) vv_destroy(e.r);
     delete s;
     return VV_OK;
 }

@@ -32,7 +32,8 @@ class VVResult(ctypes.Structure):
 
 class VVServerStats(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int), ("running", ctypes.c_int), ("done", ctypes.c_int),
-                ("peak_running", ctypes.c_int), ("mem_in_use", ctypes.c_double), ("peak_mem", ctypes.c_double)]
+                ("peak_running", ctypes.c_int), ("mem_in_use", ctypes.c_double), ("peak_mem", ctypes.c_double),
+                ("resident", ctypes.c_int), ("reused", ctypes.c_int), ("created", ctypes.c_int)]
 
 
 def _bind():
@@ -103,7 +104,8 @@ class Server:
         s = VVServerStats()
         L.check(self._lib.vv_server_stats_read(self._h, ctypes.byref(s)), "vv_server_stats_read")
         return {"queued": s.queued, "running": s.running, "done": s.done, "peak_running": s.peak_running,
-                "mem_in_use": s.mem_in_use, "peak_mem": s.peak_mem}
+                "mem_in_use": s.mem_in_use, "peak_mem": s.peak_mem, "resident": s.resident, "reused": s.reused,
+                "created": s.created}
 
     def close(self):
         if self._h:

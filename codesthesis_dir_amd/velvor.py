@@ -157,6 +157,14 @@ class PeriodicBox:
         info.status = st
         return info
 
+    def reset(self, cfg: "BoxConfig"):
+        """Re-arm the run for cfg (same grid, geometry and solver; new
+        physics): fields and solver state as after creation, tables kept."""
+        c = cfg.c()
+        L.check(L.lib().vv_reset(self._run, ctypes.byref(c)), "vv_reset")
+        self.cfg = cfg
+        self.steps = 0
+
     def solve_history(self, max_cycles: int = 64):
         """Relative residual after each cycle of the last solve of v and of
         u: (v history, u history)."""

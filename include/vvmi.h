@@ -124,6 +124,14 @@ int vv_device_count(int *n);
  * nlev (out, may be NULL) receives the number of multigrid levels. */
 int vv_create(const vv_config *cfg, int device, vv_run **out, int *nlev);
 int vv_destroy(vv_run *run);
+/* Re-arms a run for a new configuration of the same grid, geometry and
+ * solver (nr, np, geom, r0, rmax, width, direct_cols, nu1, nu2, precision;
+ * VV_EINVAL otherwise): the physics (dt, Ra, Pr, Re, robert), the stop and
+ * the statistics switch take the new values, every field and the solver
+ * state return to those of a fresh vv_create, and the tables are kept.  No
+ * counterpart in the reference; for callers that keep runs resident across
+ * requests (the served sweep). */
+int vv_reset(vv_run *run, const vv_config *cfg);
 
 /* Copy one field between a host array (leading dimension ld >= nr+2,
  * np+1 columns) and the run. */

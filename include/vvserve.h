@@ -9,7 +9,10 @@
  * admitted run steps on its own stream from its own worker thread, so the
  * work of different runs overlaps on the device.  A run's results do not
  * depend on what else is running: every run has its own fields, multigrid
- * hierarchy and stream.
+ * hierarchy and stream.  Runs stay resident between requests: a request of
+ * the same grid, geometry and solver as an idle run re-arms it (vv_reset)
+ * instead of creating one; idle runs (at most two per slot) are destroyed
+ * when the memory budget needs their footprint.
  */
 #ifndef VVSERVE_H
 #define VVSERVE_H
@@ -46,7 +49,10 @@ typedef struct {
 typedef struct {
     int queued, running, done;
     int peak_running;
-    double mem_in_use, peak_mem;        /* admitted footprint, bytes               */
+    double mem_in_use, peak_mem;        /* admitted footprint of the running and
+                                           the resident idle runs, bytes          */
+    int resident;        /* idle runs kept for later requests                      */
+    int reused, created; /* requests served by a resident run / by a new one       */
 } vv_server_stats;
 
 /* Device footprint a run of this configuration is admitted with (bytes): an

@@ -71,3 +71,52 @@ def test_admission_keeps_the_budget():
         if footprint(big) > 2.5 * f:
             with pytest.raises(VVError):
                 srv.submit(big, 10, th_big)
+
+
+def test_resident_runs_give_fresh_results():
+    """A run kept resident and re-armed for a request of the same grid
+    (vv_reset: new physics and initial state, tables kept) gives the
+    fields of a run created for it, bit for bit; a re-arm for another grid
+    is refused."""
+    from codesthesis_dir_amd import PeriodicBox
+    from codesthesis_dir_amd._lib import VVError
+    from codesthesis_dir_amd.serve import Server
+    from dataclasses import replace
+    reqs = _requests()
+    cfg_a, th_a = reqs[1]                               # annulus 32 x 64
+    cfg_b = replace(cfg_a, Ra=3e5, dt=1e-5, Re=40.0)    # same grid, other physics
+    th_b = th_a[::-1].copy()
+    fresh = {}
+    for name, cfg, th0 in (("a", cfg_a, th_a), ("b", cfg_b, th_b)):
+        box = PeriodicBox(cfg, device=0)
+        box.put("th", th0)
+        box.start()
+        info = box.step(30)
+        fresh[name] = (box.get("th"), box.get("vel"), info.trans1)
+        box.close()
+    # the same through one resident run: a, b, a
+    with Server(slots=1, mem_budget=8e9) as srv:
+        for cfg, th0 in ((cfg_a, th_a), (cfg_b, th_b), (cfg_a, th_a)):
+            srv.submit(cfg, 30, th0, outputs=True)
+        res = [srv.wait(timeout=120.0) for _ in range(3)]
+        st = srv.stats()
+    assert st["created"] == 1 and st["reused"] == 2 and st["resident"] == 1
+    for g, name in zip(res, ("a", "b", "a")):
+        th, vel, t1 = fresh[name]
+        assert g["status"] == 0
+        assert np.array_equal(g["th"], th[:, :cfg_a.nr + 2]), name
+        assert np.array_equal(g["vel"], vel[:, :cfg_a.nr + 2]), name
+        assert g["trans1"] == t1, name
+    # the plain API: a run re-armed after other steps equals a fresh one
+    box = PeriodicBox(cfg_b, device=0)
+    box.put("th", th_b)
+    box.start()
+    box.step(17)
+    box.reset(cfg_a)
+    box.put("th", th_a)
+    box.start()
+    info = box.step(30)
+    assert np.array_equal(box.get("th"), fresh["a"][0]) and info.trans1 == fresh["a"][2]
+    with pytest.raises(VVError):
+        box.reset(reqs[0][0])                           # another grid
+    box.close()
