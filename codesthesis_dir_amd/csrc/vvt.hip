@@ -92,10 +92,20 @@ __device__ __forceinline__ double line_rhs(int type, int m, int i, double dri, d
 }
 
 // Lines along y and z: one thread per line, consecutive threads on
-// consecutive i (coalesced); the eliminated right-hand sides r' go to d and
-// are read back by the back substitution.  Rows are processed in groups of
-// U with their loads issued ahead of the dependent chain.
-__global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0)
+// consecutive i (coalesced).  The lines (256-514 rows) are too long to keep
+// their eliminated right-hand sides r' in registers, so the forward sweep
+// keeps only every CK-th one (a checkpoint, in ckp: 1/CK of the line), and
+// the back substitution works segment by segment from the last: it reads
+// the segment's nodes again, recomputes its r' from the checkpoint before it
+// in registers -- the same operations on the same values, so the same bits
+// -- and substitutes back through them.  Per line that is two reads of the
+// nodes and one write of the derivative, against one read, a write and a
+// read of r' and the final write when r' went through d.  The forward rows
+// are processed in groups of U with their loads issued ahead of the chain.
+constexpr int CK = 32;
+
+__global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0, double *__restrict__ ckp, int nseg_max,
+                                                   int nls)
 {
     const DJob J = jobs.j[j0 + blockIdx.y];
     const int t = blockIdx.x * WG + threadIdx.x;
@@ -107,6 +117,7 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0)
     else { base = (unsigned)(Q1 * g.ldj * g.ldi + P1); s = (unsigned)g.ldi; }
     const double *__restrict__ f = J.f;
     double *__restrict__ d = J.d;
+    double *__restrict__ cp = ckp + (size_t)blockIdx.y * nseg_max * nls + t;   // segment sg at cp[sg * nls]
     const int m = J.m, type = J.type;
     const double dri = J.dri;
     const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
@@ -126,7 +137,7 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0)
             const int ii = i + u;
             const double r = line_rhs(type, m, ii, dri, fm3, fa, fb, nx[u], f2, f4);
             rp = (ii == 1) ? r : r - fact[ii] * rp;
-            d[base + (unsigned)ii * s] = rp;
+            if (ii % CK == 0) cp[(size_t)(ii / CK - 1) * nls] = rp;
             fa = fb;
             fb = nx[u];
         }
@@ -135,28 +146,39 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0)
         const double fc = f[base + (unsigned)(i + 1) * s];
         const double r = line_rhs(type, m, i, dri, fm3, fa, fb, fc, f2, f4);
         rp = (i == 1) ? r : r - fact[i] * rp;
-        d[base + (unsigned)i * s] = rp;
+        if (i % CK == 0) cp[(size_t)(i / CK - 1) * nls] = rp;
         fa = fb;
         fb = fc;
     }
-    double y = qdiv(rp, bp[m], rbp[m]);
-    d[base + (unsigned)m * s] = y;
-    i = m - 1;
-    for (; i - U + 1 >= 1; i -= U) {
-        double rr[U];
+    double y = 0.0;
+    for (int sg = (m + CK - 1) / CK - 1; sg >= 0; --sg) {
+        const int r0 = sg * CK + 1, r1 = min(r0 + CK - 1, m);
+        double rq = (sg == 0) ? 0.0 : cp[(size_t)(sg - 1) * nls];
+        double fv[CK + 2];   // nodes r0-1 .. r0+CK
 #pragma unroll
-        for (int u = 0; u < U; ++u) rr[u] = d[base + (unsigned)(i - u) * s];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int ii = i - u;
-            y = qdiv(rr[u] - cc[ii] * y, bp[ii], rbp[ii]);
-            d[base + (unsigned)ii * s] = y;
+        for (int u = 0; u < CK + 2; ++u) {
+            const int row = r0 - 1 + u;
+            fv[u] = (row <= r1 + 1) ? f[base + (unsigned)row * s] : 0.0;
         }
-    }
-    for (; i >= 1; --i) {
-        const unsigned o = base + (unsigned)i * s;
-        y = qdiv(d[o] - cc[i] * y, bp[i], rbp[i]);
-        d[o] = y;
+        double rr[CK];
+#pragma unroll
+        for (int u = 0; u < CK; ++u) {
+            const int ii = r0 + u;
+            rr[u] = 0.0;
+            if (ii <= r1) {
+                const double r = line_rhs(type, m, ii, dri, fm3, fv[u], fv[u + 1], fv[u + 2], f2, f4);
+                rq = (ii == 1) ? r : r - fact[ii] * rq;
+                rr[u] = rq;
+            }
+        }
+#pragma unroll
+        for (int u = CK - 1; u >= 0; --u) {
+            const int ii = r0 + u;
+            if (ii <= r1) {
+                y = (ii == m) ? qdiv(rr[u], bp[m], rbp[m]) : qdiv(rr[u] - cc[ii] * y, bp[ii], rbp[ii]);
+                d[base + (unsigned)ii * s] = y;
+            }
+        }
     }
 }
 
@@ -1105,6 +1127,8 @@ struct vvt_run {
     size_t tab_off[12] = {0};
     TankOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;   // Poisson work (kmax x mmax each)
+    double *ckp = nullptr;        // checkpoints of the y/z derivative lines: 8 jobs x ckp_seg x ckp_lines
+    int ckp_seg = 0, ckp_lines = 0;
     double2 *ytw = nullptr;       // y FFT: twiddles of 2np points (np), then (cos, sin)(pi q/(2np)) (np)
     bool yfft = false;            // y modes by FFT (np a power of two, 8..2048)
     int ylp = 1, ybits = 0;       // line pairs per workgroup, log2(2np)
@@ -1334,7 +1358,7 @@ int x_lines_per_group(int m) { return std::max(1, std::min(64, 4096 / ((m + 2) |
 int launch_derivatives(vvt_run *r)
 {
     DJobs J;
-    int nyz = 0, nx = 0;
+    int nyz = 0, nx = 0, maxm = 0;
     double byz = 0.0, bx = 0.0;
     const double *src[4] = {r->f[VV3_A], r->f[VV3_B], r->f[VV3_C], r->f[VV3_TH]};
     const double dri[3] = {1.0 / r->dr, 1.0 / r->dp, 1.0 / r->dz};
@@ -1351,6 +1375,7 @@ int launch_derivatives(vvt_run *r)
             xlds = std::max(xlds, (size_t)nl * ((s.m + 2) | 1) * sizeof(double));
         } else {
             nyz = std::max(nyz, s.n1 * s.n2);
+            maxm = std::max(maxm, s.m);
         }
     }
     // the y and z jobs in one launch: jobs 1, 2, 4, 5, ... are reached as
@@ -1361,7 +1386,12 @@ int launch_derivatives(vvt_run *r)
         if (d % 3 != 0) Jyz.j[c++] = J.j[d];
     {
         Prof p(r, VVT_K_DERIV_YZ, byz);
-        hipLaunchKernelGGL(k_t_deriv_yz, dim3(nblk(nyz, WG), c), dim3(WG), 0, r->stream, geo(r), Jyz, 0);
+        if (c > 8 || maxm > r->ckp_seg * CK || nyz > r->ckp_lines) {
+            vv::set_error("vvt: derivative checkpoints exceed their buffer");
+            return VV_EINVAL;
+        }
+        hipLaunchKernelGGL(k_t_deriv_yz, dim3(nblk(nyz, WG), c), dim3(WG), 0, r->stream, geo(r), Jyz, 0, r->ckp,
+                           r->ckp_seg, r->ckp_lines);
     }
     {
         Prof p(r, VVT_K_DERIV_X, bx);
@@ -1636,6 +1666,17 @@ int vvt_create(const vvt_config *cfg, int device, vvt_run **out)
             compact_tables(s.type, s.m, tabs);
         }
         if (upload(&r->tabs, tabs) != hipSuccess) return fail(VV_ENOMEM);
+        // checkpoints of the y and z lines (8 jobs)
+        int maxm = 0, nl = 0;
+        for (int d = 0; d < 12; ++d)
+            if (d % 3 != 0) {
+                const DSpec s = dspec(r, d);
+                maxm = std::max(maxm, s.m);
+                nl = std::max(nl, s.n1 * s.n2);
+            }
+        r->ckp_seg = (maxm + CK - 1) / CK;
+        r->ckp_lines = nl;
+        if (hipMalloc(&r->ckp, 8 * (size_t)r->ckp_seg * nl * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     }
     {   // column sums (2 nr), trans1 and trans2, then the terms of rows k = np/2 .. np
         const size_t nh = 2 * (size_t)r->nr + 2 + 2 * (size_t)(cfg->np - cfg->np / 2 + 1) * r->nr;
@@ -1670,6 +1711,7 @@ int vvt_destroy(vvt_run *r)
         for (double *p : {r->tabs, r->S, r->T, r->cp, r->heat, r->kepart})
             if (p) (void)hipFree(p);
         if (r->ytw) (void)hipFree(r->ytw);
+        if (r->ckp) (void)hipFree(r->ckp);
         for (hipEvent_t e : r->ev)
             if (e) (void)hipEventDestroy(e);
         if (r->stream) (void)hipStreamDestroy(r->stream);
