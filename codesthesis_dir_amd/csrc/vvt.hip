@@ -104,10 +104,20 @@ __device__ __forceinline__ double line_rhs(int type, int m, int i, double dri, d
 // are processed in groups of U with their loads issued ahead of the chain.
 constexpr int CK = 32;
 
+// TLDS: the job's tables (fact | b' | c | 1/b', 4 (m+1) words) are copied to
+// LDS first; lines longer than DYZ_LDS_MAXM read them from global memory
+constexpr int DYZ_LDS_MAXM = 2047;
+
+template <bool TLDS>
 __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0, double *__restrict__ ckp, int nseg_max,
                                                    int nls)
 {
+    extern __shared__ double tabl[];
     const DJob J = jobs.j[j0 + blockIdx.y];
+    if constexpr (TLDS) {
+        for (int e = threadIdx.x; e < 4 * (J.m + 1); e += WG) tabl[e] = J.tab[e];
+        __syncthreads();
+    }
     const int t = blockIdx.x * WG + threadIdx.x;
     if (t >= J.n1 * J.n2) return;
     const int q = t / J.n1, p = t - q * J.n1;
@@ -120,7 +130,8 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0, d
     double *__restrict__ cp = ckp + (size_t)blockIdx.y * nseg_max * nls + t;   // segment sg at cp[sg * nls]
     const int m = J.m, type = J.type;
     const double dri = J.dri;
-    const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
+    const double *T = TLDS ? tabl : J.tab;
+    const double *fact = T, *bp = T + (m + 1), *cc = T + 2 * (m + 1), *rbp = T + 3 * (m + 1);
     const double f4 = (type == 1) ? f[base + 4u * s] : 0.0;
     const double fm3 = (type == 1) ? f[base + (unsigned)(m - 3) * s] : 0.0;
     const double f2 = f[base + 2u * s];
@@ -182,105 +193,131 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0, d
     }
 }
 
-// Lines along x (contiguous in memory): a workgroup stages nl lines, nodes
-// 0..m+1, in LDS with a row pitch sp = (m+2) | 1 (odd, so the threads' rows
-// fall on different banks) through coalesced loads, thread l < nl runs line
-// l's recurrence in LDS (its nodes are read into registers ahead of the
-// rows that overwrite them), and the interior rows go back coalesced.
-__global__ void __launch_bounds__(64) k_t_deriv_x(TGeo g, DJobs jobs)
+// Lines along x (contiguous in memory): a workgroup of one wave takes 64
+// lines (consecutive j of one plane k), thread l line l.  The lines pass
+// through LDS in chunks of CX rows (nodes r0-1 .. r0+CX, odd pitch, loaded
+// coalesced along the lines): the forward sweep runs chunk by chunk and
+// keeps one checkpoint of the eliminated right-hand side per chunk (ckp, as
+// the y/z lines); the back substitution takes the chunks from the last,
+// reloads one, recomputes its r' from the checkpoint before it in registers
+// (the same operations, the same bits), substitutes back into the tile and
+// stores the chunk's rows coalesced.
+constexpr int CX = 32, XSP = CX + 3;   // rows per chunk; tile pitch (odd)
+
+__global__ void __launch_bounds__(64) k_t_deriv_x(TGeo g, DJobs jobs, double *__restrict__ ckp, int nseg_max, int nls)
 {
-    extern __shared__ double sh[];
+    __shared__ double sh[64 * XSP];
+    __shared__ double tb[4][CX];   // the chunk's table rows: fact, b', c, 1/b'
     const DJob J = jobs.j[3 * blockIdx.y];
-    const int m = J.m, nl = J.nl, type = J.type, sp = (m + 2) | 1;
-    const int tiles = (J.n1 + nl - 1) / nl;
-    const int q = blockIdx.x / tiles, p0 = (blockIdx.x - q * tiles) * nl;
+    const int m = J.m, type = J.type;
+    const int tiles = (J.n1 + 63) / 64;
+    const int q = blockIdx.x / tiles, p0 = (blockIdx.x - q * tiles) * 64;
     if (q >= J.n2) return;
-    const int nlines = min(nl, J.n1 - p0);
+    const int nlines = min(64, J.n1 - p0);
     const int tid = threadIdx.x;
     // line l: (j, k) = (p0 + l + 1, q + 1), nodes contiguous from node 0
     const unsigned base0 = (unsigned)(((q + 1) * g.ldj + (p0 + 1)) * g.ldi);
     const double *__restrict__ f = J.f;
     double *__restrict__ d = J.d;
-    const int len = m + 2, total = nlines * len;
-    // 8 loads in flight per thread before their LDS stores
-    for (int e0 = 0; e0 < total; e0 += 8 * 64) {
-        double v[8];
+    const bool act = tid < nlines;
+    const unsigned mybase = base0 + (unsigned)(tid * g.ldi);
+    double *__restrict__ cp = ckp + (size_t)blockIdx.y * nseg_max * nls + (size_t)q * J.n1 + p0 + tid;
+    const double dri = J.dri;
+    const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
+    const double f4 = (act && type == 1) ? f[mybase + 4u] : 0.0;
+    const double fm3 = (act && type == 1) ? f[mybase + (unsigned)(m - 3)] : 0.0;
+    const double f2 = act ? f[mybase + 2u] : 0.0;
+    const int nseg = (m + CX - 1) / CX;
+    double *L = sh + tid * XSP;
+    // nodes r0-1 .. r1+1 of chunk sg into the tile (coalesced: consecutive
+    // threads on consecutive nodes of a line)
+    auto load = [&](int sg) {
+        const int r0 = sg * CX + 1, r1 = min(r0 + CX - 1, m), len = r1 - r0 + 3;
+        const int total = nlines * len;
+        for (int e0 = 0; e0 < total; e0 += 8 * 64) {
+            double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * 64 + tid;
-            if (e < total) {
-                const int l = e / len, t = e - l * len;
-                v[u] = f[base0 + (unsigned)(l * g.ldi + t)];
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * 64 + tid;
+                if (e < total) {
+                    const int l = e / len, o = e - l * len;
+                    v[u] = f[base0 + (unsigned)(l * g.ldi + r0 - 1 + o)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * 64 + tid;
+                if (e < total) {
+                    const int l = e / len, o = e - l * len;
+                    sh[l * XSP + o] = v[u];
+                }
             }
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * 64 + tid;
-            if (e < total) {
-                const int l = e / len, t = e - l * len;
-                sh[l * sp + t] = v[u];
-            }
+        if (tid < CX) {   // table rows r0 .. r0+CX-1 (only rows <= r1 are read)
+            const int ii = min(r0 + tid, m);
+            tb[0][tid] = fact[ii];
+            tb[1][tid] = bp[ii];
+            tb[2][tid] = cc[ii];
+            tb[3][tid] = rbp[ii];
         }
+        __syncthreads();
+    };
+    double rp = 0.0;
+    for (int sg = 0; sg < nseg; ++sg) {
+        load(sg);
+        const int r0 = sg * CX + 1, r1 = min(r0 + CX - 1, m);
+        if (act) {
+            double fv[CX + 2];
+#pragma unroll
+            for (int u = 0; u < CX + 2; ++u) fv[u] = L[u];
+#pragma unroll
+            for (int u = 0; u < CX; ++u) {
+                const int ii = r0 + u;
+                if (ii <= r1) {
+                    const double r = line_rhs(type, m, ii, dri, fm3, fv[u], fv[u + 1], fv[u + 2], f2, f4);
+                    rp = (ii == 1) ? r : r - tb[0][u] * rp;
+                }
+            }
+            if (sg + 1 < nseg) cp[(size_t)sg * nls] = rp;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (tid < nlines) {
-        double *L = sh + tid * sp;
-        const double dri = J.dri;
-        const double *fact = J.tab, *bp = J.tab + (m + 1), *cc = J.tab + 2 * (m + 1), *rbp = J.tab + 3 * (m + 1);
-        const double f4 = (type == 1) ? L[4] : 0.0;
-        const double fm3 = (type == 1) ? L[m - 3] : 0.0;
-        const double f2 = L[2];
-        double fa = L[0], fb = L[1];   // f[i-1], f[i]
-        double rp = 0.0;
-        // rows in groups of U: the group's nodes f[i+1 .. i+U] are read
-        // before any of its rows overwrites L[i .. i+U-1]
-        constexpr int U = 8;
-        int i = 1;
-        for (; i + U - 1 <= m; i += U) {
-            double nx[U];
+    double y = 0.0;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        load(sg);
+        const int r0 = sg * CX + 1, r1 = min(r0 + CX - 1, m);
+        if (act) {
+            double rq = (sg == 0) ? 0.0 : cp[(size_t)(sg - 1) * nls];
+            double fv[CX + 2], rr[CX];
 #pragma unroll
-            for (int u = 0; u < U; ++u) nx[u] = L[i + 1 + u];
+            for (int u = 0; u < CX + 2; ++u) fv[u] = L[u];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int ii = i + u;
-                const double r = line_rhs(type, m, ii, dri, fm3, fa, fb, nx[u], f2, f4);
-                rp = (ii == 1) ? r : r - fact[ii] * rp;
-                L[ii] = rp;
-                fa = fb;
-                fb = nx[u];
+            for (int u = 0; u < CX; ++u) {
+                const int ii = r0 + u;
+                rr[u] = 0.0;
+                if (ii <= r1) {
+                    const double r = line_rhs(type, m, ii, dri, fm3, fv[u], fv[u + 1], fv[u + 2], f2, f4);
+                    rq = (ii == 1) ? r : r - tb[0][u] * rq;
+                    rr[u] = rq;
+                }
+            }
+#pragma unroll
+            for (int u = CX - 1; u >= 0; --u) {
+                const int ii = r0 + u;
+                if (ii <= r1) {
+                    y = (ii == m) ? qdiv(rr[u], tb[1][u], tb[3][u])
+                                  : qdiv(rr[u] - tb[2][u] * y, tb[1][u], tb[3][u]);
+                    L[u + 1] = y;   // row ii sits at tile position ii - r0 + 1
+                }
             }
         }
-        for (; i <= m; ++i) {
-            const double fc = L[i + 1];
-            const double r = line_rhs(type, m, i, dri, fm3, fa, fb, fc, f2, f4);
-            rp = (i == 1) ? r : r - fact[i] * rp;
-            L[i] = rp;
-            fa = fb;
-            fb = fc;
+        __syncthreads();
+        const int len = r1 - r0 + 1, total = nlines * len;
+        for (int e = tid; e < total; e += 64) {
+            const int l = e / len, o = e - l * len;
+            d[base0 + (unsigned)(l * g.ldi + r0 + o)] = sh[l * XSP + 1 + o];
         }
-        double y = qdiv(rp, bp[m], rbp[m]);
-        L[m] = y;
-        i = m - 1;
-        for (; i - U + 1 >= 1; i -= U) {
-            double rr[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) rr[u] = L[i - u];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int ii = i - u;
-                y = qdiv(rr[u] - cc[ii] * y, bp[ii], rbp[ii]);
-                L[ii] = y;
-            }
-        }
-        for (; i >= 1; --i) {
-            y = qdiv(L[i] - cc[i] * y, bp[i], rbp[i]);
-            L[i] = y;
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < nlines * m; e += 64) {
-        const int l = e / m, t = 1 + (e - l * m);
-        d[base0 + (unsigned)(l * g.ldi + t)] = sh[l * sp + t];
+        __syncthreads();
     }
 }
 
@@ -1352,27 +1389,24 @@ VF vfields(vvt_run *r)
     return F;
 }
 
-// x lines staged in LDS: at most 32 KB per workgroup
-int x_lines_per_group(int m) { return std::max(1, std::min(64, 4096 / ((m + 2) | 1))); }
-
 int launch_derivatives(vvt_run *r)
 {
     DJobs J;
-    int nyz = 0, nx = 0, maxm = 0;
+    int nyz = 0, nx = 0, maxm = 0, maxmx = 0, nlx = 0;
     double byz = 0.0, bx = 0.0;
     const double *src[4] = {r->f[VV3_A], r->f[VV3_B], r->f[VV3_C], r->f[VV3_TH]};
     const double dri[3] = {1.0 / r->dr, 1.0 / r->dp, 1.0 / r->dz};
-    size_t xlds = 0;
     for (int d = 0; d < 12; ++d) {
         const DSpec s = dspec(r, d);
-        const int nl = (d % 3 == 0) ? x_lines_per_group(s.m) : 1;
+        const int nl = (d % 3 == 0) ? 64 : 1;
         J.j[d] = DJob{src[d / 3], r->dv[d], r->tabs + r->tab_off[d], d % 3, s.type, s.m, s.n1, s.n2, nl, dri[d % 3]};
         const double b = 8.0 * (2.0 * s.m + 2.0) * s.n1 * s.n2;
         if (d % 3 == 0) bx += b;
         else byz += b;
         if (d % 3 == 0) {
             nx = std::max(nx, ((s.n1 + nl - 1) / nl) * s.n2);
-            xlds = std::max(xlds, (size_t)nl * ((s.m + 2) | 1) * sizeof(double));
+            maxmx = std::max(maxmx, s.m);
+            nlx = std::max(nlx, s.n1 * s.n2);
         } else {
             nyz = std::max(nyz, s.n1 * s.n2);
             maxm = std::max(maxm, s.m);
@@ -1390,12 +1424,22 @@ int launch_derivatives(vvt_run *r)
             vv::set_error("vvt: derivative checkpoints exceed their buffer");
             return VV_EINVAL;
         }
-        hipLaunchKernelGGL(k_t_deriv_yz, dim3(nblk(nyz, WG), c), dim3(WG), 0, r->stream, geo(r), Jyz, 0, r->ckp,
-                           r->ckp_seg, r->ckp_lines);
+        if (maxm <= DYZ_LDS_MAXM)
+            hipLaunchKernelGGL(k_t_deriv_yz<true>, dim3(nblk(nyz, WG), c), dim3(WG),
+                               4 * (size_t)(maxm + 1) * sizeof(double), r->stream, geo(r), Jyz, 0, r->ckp,
+                               r->ckp_seg, r->ckp_lines);
+        else
+            hipLaunchKernelGGL(k_t_deriv_yz<false>, dim3(nblk(nyz, WG), c), dim3(WG), 0, r->stream, geo(r), Jyz, 0,
+                               r->ckp, r->ckp_seg, r->ckp_lines);
     }
     {
         Prof p(r, VVT_K_DERIV_X, bx);
-        hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), xlds, r->stream, geo(r), J);
+        if ((maxmx + CX - 1) / CX > r->ckp_seg || nlx > r->ckp_lines) {
+            vv::set_error("vvt: derivative checkpoints exceed their buffer");
+            return VV_EINVAL;
+        }
+        hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), 0, r->stream, geo(r), J, r->ckp, r->ckp_seg,
+                           r->ckp_lines);
     }
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1666,15 +1710,15 @@ int vvt_create(const vvt_config *cfg, int device, vvt_run **out)
             compact_tables(s.type, s.m, tabs);
         }
         if (upload(&r->tabs, tabs) != hipSuccess) return fail(VV_ENOMEM);
-        // checkpoints of the y and z lines (8 jobs)
-        int maxm = 0, nl = 0;
-        for (int d = 0; d < 12; ++d)
-            if (d % 3 != 0) {
-                const DSpec s = dspec(r, d);
-                maxm = std::max(maxm, s.m);
-                nl = std::max(nl, s.n1 * s.n2);
-            }
-        r->ckp_seg = (maxm + CK - 1) / CK;
+        // checkpoints of the derivative lines (y and z: 8 jobs, one per CK
+        // rows; then x: 4 jobs, one per CX rows, in the same buffer)
+        int segs = 0, nl = 0;
+        for (int d = 0; d < 12; ++d) {
+            const DSpec s = dspec(r, d);
+            segs = std::max(segs, (s.m + ((d % 3 == 0) ? CX : CK) - 1) / ((d % 3 == 0) ? CX : CK));
+            nl = std::max(nl, s.n1 * s.n2);
+        }
+        r->ckp_seg = segs;
         r->ckp_lines = nl;
         if (hipMalloc(&r->ckp, 8 * (size_t)r->ckp_seg * nl * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     }

@@ -66,6 +66,20 @@ def test_explicit_routines_bitexact(shape):
     run.close()
 
 
+@pytest.mark.parametrize("shape", [(6, 6, 2100), (40, 70, 6)])
+def test_derivatives_long_lines_bitexact(shape):
+    """Lines longer than the tables staged in LDS (y lines of 2100 cells) and
+    x / z lines spanning several checkpoint segments, bit for bit."""
+    nr, nz, np_ = shape
+    print(f"[gpu] tank derivatives {shape}", flush=True)
+    run, ora = _pair(nr, nz, np_, seed=5 + nr + nz + np_)
+    run.derivatives()
+    d = ora.derivatives()
+    for q in range(12):
+        assert np.array_equal(run.derivative(q), d[q]), q
+    run.close()
+
+
 @pytest.mark.parametrize("which", [1, 2, 3])
 @pytest.mark.parametrize("shape", [(8, 6, 10), (7, 9, 6)])
 def test_poisson_matches_dense_solve(which, shape):
