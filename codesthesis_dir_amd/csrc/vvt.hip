@@ -3,9 +3,10 @@
 //
 // Explicit part: the compact advection derivatives as batched line solves
 // (one thread per line, the elimination tables of the constant rows computed
-// once on the host in the order of the reference's tdma), then one thread per
-// point for the A, B, C and temperature updates, the right-hand sides and
-// the boundaries.  Every expression is evaluated in the operation order the
+// once on the host in the order of the reference's tdma), then the A, B, C
+// and temperature updates in one pass (one thread per (i, j) column marching
+// along y), one thread per point for the right-hand sides and the
+// boundaries.  Every expression is evaluated in the operation order the
 // oracle restates (the library is compiled with -ffp-contract=off), so the
 // explicit results are bit-for-bit the oracle's.
 //
@@ -328,195 +329,235 @@ struct VF {
     double *An, *Bn, *Cn, *thn;
 };
 
-// Avorticity (rot3dmg.f:359-465, Cartesian): i = 1..nr, j = 1..nz-1,
-// k = 1..np-1
-__global__ void __launch_bounds__(WG) k_t_avort(TGeo g, VF F)
+// The explicit updates, fused: Avorticity (rot3dmg.f:359-465; i = 1..nr,
+// j = 1..nz-1, k = 1..np-1), Bvorticity (495-590; i = 1..nr-1, j = 1..nz-1,
+// k = 1..np), Cvorticity (619-725; i = 1..nr-1, j = 1..nz, k = 1..np-1) and
+// temperature (754-871; i = 1..nr, j = 1..nz, k = 1..np), in Cartesian form.
+// MASK 1 runs the three vorticity updates, 2 the temperature, 3 all four:
+// they read only the current and previous time levels, so one pass reads
+// the fields the four share once.
+//
+// One thread per (i, j) column marches over KC consecutive y planes.  The
+// values a column needs at several planes (its own node and the few
+// neighbours the staggered averages take across y) stay in registers as the
+// march moves on: each is loaded once per plane, not at k-1, k and k+1.
+// Every expression is the routine's own, in its order (same bits).
+//
+// Tiles go to the XCDs in contiguous ranges (the hardware deals linear
+// workgroup ids to the 8 XCDs in turn): the row blocks of a chunk of planes
+// run together on one XCD, so the rows j-1 and j+1 a block reads were just
+// fetched into that XCD's L2 by its neighbours.
+constexpr int KC = 16;
+
+__device__ __forceinline__ void xcd_tile(unsigned nbx, unsigned nby, int &bx, int &by)
 {
-    const int nr = g.nr, nz = g.nz;
-    const int t = blockIdx.x * WG + threadIdx.x;
-    if (t >= nr * (nz - 1)) return;
-    const int jj = t / nr, i = 1 + (t - jj * nr), j = 1 + jj, k = 1 + blockIdx.y;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re, Gr = g.Gr;
-    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
-    const int im = i - 1, jp = j + 1, ka = k + 1, km = k - 1;
-    const double u = (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, k) +
-                      X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) *
-                     0.125;
-    const double v = (X(vel, i, jp, k) + X(vel, i, j, k)) * 0.5;
-    const double w = (X(wel, i, j, ka) + X(wel, i, j, k)) * 0.5;
-    const double BB = (X(B, i, j, k) + X(B, im, j, k) + X(B, i, j, ka) + X(B, im, j, ka)) * 0.25;
-    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, im, j, k) + X(C, im, jp, k)) * 0.25;
-    double Arra, cnr;
-    if (i == 1) {
-        Arra = (3.2 * X(A, i - 1, j, k) + 2.0 * X(A, i + 1, j, k) - 0.2 * X(A, i + 2, j, k)) / (dr * dr);
-        cnr = -2.5 / (dr * dr);
-    } else if (i == nr) {
-        Arra = (3.2 * X(A, i + 1, j, k) + 2.0 * X(A, i - 1, j, k) - 0.2 * X(A, i - 2, j, k)) / (dr * dr);
-        cnr = -2.5 / (dr * dr);
-    } else {
-        Arra = (X(A, i - 1, j, k) + X(A, i + 1, j, k)) / (dr * dr);
-        cnr = -1.0 / (dr * dr);
-    }
-    const double uelr = ((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
-                         (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25) /
-                        dr;
-    const double Azza = (X(A, i, j - 1, k) + X(A, i, j + 1, k)) / (dz * dz);
-    const double uelz = ((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
-                         (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25) /
-                        dz;
-    const double thh = (X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5 / dp;
-    const double Ahha = (X(A, i, j, ka) + X(A, i, j, km)) / (dp * dp);
-    const double uelh = ((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
-                         (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25) /
-                        dp;
-    const double cnz = -1.0 / (dz * dz);
-    const double cnh = -1.0 / (dp * dp);
-    const double Ax = X(F.dv[0], i, j, k), Ay = X(F.dv[1], i, j, k), Az = X(F.dv[2], i, j, k);
-    const double RHS1 = Arra + Ahha + Azza + X(A, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Ax + v * Ay + w * Az - uelh * BB - uelz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * uelz - Gr * Ptl * thh;
-    const double cf1 = 1.0 / (Ptl * dt) - uelr / (2.0 * Ptl) - cnr - cnz - cnh;
-    const double cf2 = uelr / (2.0 * Ptl) + cnr + cnz + cnh;
-    X(F.An, i, j, k) = (cf2 * X(F.Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+    const unsigned T = nbx * nby, L = blockIdx.y * nbx + blockIdx.x;
+    const unsigned x = L & 7u, q = L >> 3, base = T >> 3, rem = T & 7u;
+    const unsigned W = x * base + (x < rem ? x : rem) + q;
+    by = (int)(W / nbx);
+    bx = (int)(W - (unsigned)by * nbx);
 }
 
-// Bvorticity (rot3dmg.f:495-590, Cartesian): i = 1..nr-1, j = 1..nz-1,
-// k = 1..np
-__global__ void __launch_bounds__(WG) k_t_bvort(TGeo g, VF F)
+// The divisors the updates share, with their correctly rounded reciprocals
+// (computed on the host: uniform, they stay in scalar registers); x / b is
+// evaluated as qdiv(x, b, 1/b), the same quotient.  c25r and c25z are the
+// closure rows' -2.5 / dr^2 and -2.5 / dz^2.
+struct XDiv {
+    double dr2, rdr2, dz2, rdz2, dp2, rdp2, pdt, rpdt, ptl2, rptl2, rdr, rdz, rdp, rptl, rdt, c25r, c25z;
+};
+
+XDiv xdiv(const TGeo &g)
 {
-    const int nr = g.nr, nz = g.nz;
-    const int t = blockIdx.x * WG + threadIdx.x;
-    if (t >= (nr - 1) * (nz - 1)) return;
-    const int jj = t / (nr - 1), i = 1 + (t - jj * (nr - 1)), j = 1 + jj, k = 1 + blockIdx.y;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re, Gr = g.Gr;
-    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
-    const int ip = i + 1, jp = j + 1, km = k - 1, ka = k + 1;
-    const double u = (X(uel, i, j, k) + X(uel, i, jp, k)) * 0.5;
-    const double v = (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k) + X(vel, ip, j, km) +
-                      X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) *
-                     0.125;
-    const double w = (X(wel, i, j, k) + X(wel, ip, j, k)) * 0.5;
-    const double AA = (X(A, i, j, k) + X(A, ip, j, k) + X(A, i, j, km) + X(A, ip, j, km)) * 0.25;
-    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, i, j, km) + X(C, i, jp, km)) * 0.25;
-    const double Brra = (X(B, i - 1, j, k) + X(B, i + 1, j, k)) / (dr * dr);
-    const double velr = ((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25) /
-                        dr;
-    const double Bzza = (X(B, i, j - 1, k) + X(B, i, j + 1, k)) / (dz * dz);
-    const double velz = ((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25) /
-                        dz;
-    const double thr = (X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5 / dr;
-    const double Bhha = (X(B, i, j, ka) + X(B, i, j, km)) / (dp * dp);
-    const double velh = ((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                         (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25) /
-                        dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnz = -1.0 / (dz * dz);
-    const double cnh = -1.0 / (dp * dp);
-    const double Bx = X(F.dv[3], i, j, k), By = X(F.dv[4], i, j, k), Bz = X(F.dv[5], i, j, k);
-    const double RHS1 = Brra + Bhha + Bzza + X(B, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Bx + v * By + w * Bz - velr * AA - velz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * velz + Gr * Ptl * thr;
-    const double cf1 = 1.0 / (Ptl * dt) - velh / (2.0 * Ptl) - cnr - cnz - cnh;
-    const double cf2 = velh / (2.0 * Ptl) + cnr + cnz + cnh;
-    X(F.Bn, i, j, k) = (cf2 * X(F.Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+    XDiv d;
+    d.dr2 = g.dr * g.dr; d.dz2 = g.dz * g.dz; d.dp2 = g.dp * g.dp;
+    d.pdt = g.Ptl * g.dt; d.ptl2 = 2.0 * g.Ptl;
+    d.rdr2 = 1.0 / d.dr2; d.rdz2 = 1.0 / d.dz2; d.rdp2 = 1.0 / d.dp2; d.rpdt = 1.0 / d.pdt; d.rptl2 = 1.0 / d.ptl2;
+    d.rdr = 1.0 / g.dr; d.rdz = 1.0 / g.dz; d.rdp = 1.0 / g.dp; d.rptl = 1.0 / g.Ptl; d.rdt = 1.0 / g.dt;
+    d.c25r = -2.5 / d.dr2; d.c25z = -2.5 / d.dz2;
+    return d;
 }
 
-// Cvorticity (rot3dmg.f:619-725, Cartesian): i = 1..nr-1, j = 1..nz,
-// k = 1..np-1
-__global__ void __launch_bounds__(WG) k_t_cvort(TGeo g, VF F)
+template <int MASK>
+__global__ void __launch_bounds__(WG) k_t_explicit(TGeo g, VF F, XDiv q)
 {
-    const int nr = g.nr, nz = g.nz;
-    const int t = blockIdx.x * WG + threadIdx.x;
-    if (t >= (nr - 1) * nz) return;
-    const int jj = t / (nr - 1), i = 1 + (t - jj * (nr - 1)), j = 1 + jj, k = 1 + blockIdx.y;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re;
-    const double *A = F.A, *B = F.B, *C = F.C, *uel = F.uel, *vel = F.vel, *wel = F.wel;
-    const int ip = i + 1, jm = j - 1, km = k - 1, ka = k + 1;
-    const double u = (X(uel, i, j, ka) + X(uel, i, j, k)) * 0.5;
-    const double v = (X(vel, i, j, k) + X(vel, ip, j, k)) * 0.5;
-    const double w = (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, k) +
-                      X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) *
-                     0.125;
-    const double AA = (X(A, i, j, k) + X(A, i, jm, k) + X(A, ip, j, k) + X(A, ip, jm, k)) * 0.25;
-    const double BB = (X(B, i, j, k) + X(B, i, jm, k) + X(B, i, j, ka) + X(B, i, jm, ka)) * 0.25;
-    const double Crra = (X(C, i - 1, j, k) + X(C, i + 1, j, k)) / (dr * dr);
-    const double welr = ((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
-                         (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25) /
-                        dr;
-    double Czza, cnz;
-    if (j == 1) {
-        Czza = (3.2 * X(C, i, j - 1, k) + 2.0 * X(C, i, j + 1, k) - 0.2 * X(C, i, j + 2, k)) / (dz * dz);
-        cnz = -2.5 / (dz * dz);
-    } else if (j == nz) {
-        Czza = (3.2 * X(C, i, j + 1, k) + 2.0 * X(C, i, j - 1, k) - 0.2 * X(C, i, j - 2, k)) / (dz * dz);
-        cnz = -2.5 / (dz * dz);
-    } else {
-        Czza = (X(C, i, j - 1, k) + X(C, i, j + 1, k)) / (dz * dz);
-        cnz = -1.0 / (dz * dz);
-    }
-    const double welz = ((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
-                         (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25) /
-                        dz;
-    const double Chha = (X(C, i, j, ka) + X(C, i, j, km)) / (dp * dp);
-    const double welh = ((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
-                         (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25) /
-                        dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnh = -1.0 / (dp * dp);
-    const double Cx = X(F.dv[6], i, j, k), Cy = X(F.dv[7], i, j, k), Cz = X(F.dv[8], i, j, k);
-    const double RHS1 = Crra + Chha + Czza + X(C, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Cx + v * Cy + w * Cz - welr * AA - welh * BB) / Ptl;
-    const double RHS2 = -2.0 * Re * welz;
-    const double cf1 = 1.0 / (Ptl * dt) - welz / (2.0 * Ptl) - cnr - cnz - cnh;
-    const double cf2 = welz / (2.0 * Ptl) + cnr + cnz + cnh;
-    X(F.Cn, i, j, k) = (cf2 * X(F.Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
-}
-
-// temperature (rot3dmg.f:754-871, Cartesian): i = 1..nr, j = 1..nz,
-// k = 1..np
-__global__ void __launch_bounds__(WG) k_t_temp(TGeo g, VF F)
-{
-    const int nr = g.nr, nz = g.nz;
-    const int t = blockIdx.x * WG + threadIdx.x;
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    int bx, kc;
+    xcd_tile(gridDim.x, gridDim.y, bx, kc);
+    const int t = bx * WG + threadIdx.x;
     if (t >= nr * nz) return;
-    const int jj = t / nr, i = 1 + (t - jj * nr), j = 1 + jj, k = 1 + blockIdx.y;
-    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
-    const double *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
-    const int im = i - 1, jm = j - 1, km = k - 1, ka = k + 1;
-    const double u = (X(uel, i, j, k) + X(uel, im, j, k)) * 0.5;
-    const double v = (X(vel, i, j, k) + X(vel, i, j, km)) * 0.5;
-    const double w = (X(wel, i, j, k) + X(wel, i, jm, k)) * 0.5;
-    double thrra, cnr, thzza, cnz;
-    if (i == 1) {
-        thrra = (3.2 * X(th, i - 1, j, k) + 2.0 * X(th, i + 1, j, k) - 0.2 * X(th, i + 2, j, k)) / (dr * dr);
-        cnr = -2.5 / (dr * dr);
-    } else if (i == nr) {
-        thrra = (3.2 * X(th, i + 1, j, k) + 2.0 * X(th, i - 1, j, k) - 0.2 * X(th, i - 2, j, k)) / (dr * dr);
-        cnr = -2.5 / (dr * dr);
-    } else {
-        thrra = (X(th, i - 1, j, k) + X(th, i + 1, j, k)) / (dr * dr);
-        cnr = -1.0 / (dr * dr);
+    const int jj = t / nr, i = 1 + (t - jj * nr), j = 1 + jj;
+    const int k0 = 1 + kc * KC, k1 = min(np, k0 + KC - 1);
+    const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt, Ptl = g.Ptl, Re = g.Re, Gr = g.Gr;
+    const double dr2 = q.dr2, dz2 = q.dz2, dp2 = q.dp2, pdt = q.pdt, ptl2 = q.ptl2;
+    const double rdr2 = q.rdr2, rdz2 = q.rdz2, rdp2 = q.rdp2, rpdt = q.rpdt, rptl2 = q.rptl2;
+    const double rdr = q.rdr, rdz = q.rdz, rdp = q.rdp, rptl = q.rptl, rdt = q.rdt;
+    const double *__restrict__ A = F.A, *__restrict__ B = F.B, *__restrict__ C = F.C, *__restrict__ th = F.th;
+    const double *__restrict__ uel = F.uel, *__restrict__ vel = F.vel, *__restrict__ wel = F.wel;
+    const bool doA = (MASK & 1) && j <= nz - 1;
+    const bool doB = (MASK & 1) && i <= nr - 1 && j <= nz - 1;
+    const bool doC = (MASK & 1) && i <= nr - 1;
+    const unsigned P = (unsigned)(g.ldi * g.ldj), li = (unsigned)g.ldi;
+    const unsigned o = (unsigned)j * li + (unsigned)i;   // (i, j) within a plane
+    // in-plane offsets of the neighbours
+    const unsigned oim = o - 1u, oip = o + 1u, ojm = o - li, ojp = o + li;
+    const unsigned oimjp = ojp - 1u, oipjp = ojp + 1u, oipjm = ojm + 1u;
+
+    // registers carried along y: <field><position><plane>, plane M = k-1,
+    // K = k, N = k+1; positions 0 = (i, j), and the neighbours named
+    unsigned b = (unsigned)(k0 - 1) * P;   // plane k0-1
+    double aM = A[b + o], apM = A[b + oip];
+    double bM = B[b + o], cM = C[b + o], cjM = C[b + ojp], tM = th[b + o];
+    double vM00 = vel[b + o], vM01 = vel[b + ojp], vM10 = vel[b + oip], vM11 = vel[b + oipjp];
+    b += P;   // plane k0
+    double aK = A[b + o], bK = B[b + o], bmK = B[b + oim], bjK = B[b + ojm];
+    double cK = C[b + o], tK = th[b + o], tjK = th[b + ojp];
+    double uK00 = uel[b + o], uK01 = uel[b + ojp], uK10 = uel[b + oim], uK11 = uel[b + oimjp];
+    double wK00 = wel[b + o], wK10 = wel[b + oip], wK0m = wel[b + ojm], wK1m = wel[b + oipjm];
+
+    for (int k = k0; k <= k1; ++k) {
+        const unsigned bk = (unsigned)k * P, bn = bk + P;
+        // the planes the march brings in
+        const double aN = A[bn + o], apK = A[bk + oip];
+        const double bN = B[bn + o], bmN = B[bn + oim], bjN = B[bn + ojm];
+        const double cN = C[bn + o], cjK = C[bk + ojp];
+        const double tN = th[bn + o], tjN = th[bn + ojp];
+        const double uN00 = uel[bn + o], uN01 = uel[bn + ojp], uN10 = uel[bn + oim], uN11 = uel[bn + oimjp];
+        const double vK00 = vel[bk + o], vK01 = vel[bk + ojp], vK10 = vel[bk + oip], vK11 = vel[bk + oipjp];
+        const double wN00 = wel[bn + o], wN10 = wel[bn + oip], wN0m = wel[bn + ojm], wN1m = wel[bn + oipjm];
+
+        if (doA && k <= np - 1) {
+            const double u = (uK00 + uK01 + uN00 + uN01 + uK10 + uK11 + uN10 + uN11) * 0.125;
+            const double v = (vK01 + vK00) * 0.5;
+            const double w = (wN00 + wK00) * 0.5;
+            const double BB = (bK + bmK + bN + bmN) * 0.25;
+            const double CC = (cK + cjK + C[bk + oim] + C[bk + oimjp]) * 0.25;
+            double Arra, cnr;
+            if (i == 1) {
+                Arra = qdiv(3.2 * A[bk + oim] + 2.0 * apK - 0.2 * A[bk + o + 2u], dr2, rdr2);
+                cnr = q.c25r;
+            } else if (i == nr) {
+                Arra = qdiv(3.2 * apK + 2.0 * A[bk + oim] - 0.2 * A[bk + o - 2u], dr2, rdr2);
+                cnr = q.c25r;
+            } else {
+                Arra = qdiv(A[bk + oim] + apK, dr2, rdr2);
+                cnr = -rdr2;
+            }
+            const double uelr = qdiv((uK00 + uK01 + uN00 + uN01) * 0.25 - (uK10 + uK11 + uN10 + uN11) * 0.25, dr, rdr);
+            const double Azza = qdiv(A[bk + ojm] + A[bk + ojp], dz2, rdz2);
+            const double uelz = qdiv((uK01 + uN01 + uK11 + uN11) * 0.25 - (uK00 + uN00 + uK10 + uN10) * 0.25, dz, rdz);
+            const double thh = qdiv((tN + tjN - tK - tjK) * 0.5, dp, rdp);
+            const double Ahha = qdiv(aN + aM, dp2, rdp2);
+            const double uelh = qdiv((uN00 + uN01 + uN10 + uN11) * 0.25 - (uK00 + uK01 + uK10 + uK11) * 0.25, dp, rdp);
+            const double cnz = -rdz2;
+            const double cnh = -rdp2;
+            const double Ax = F.dv[0][bk + o], Ay = F.dv[1][bk + o], Az = F.dv[2][bk + o];
+            const double RHS1 = Arra + Ahha + Azza + qdiv(aK, pdt, rpdt);
+            const double ULT = qdiv(u * Ax + v * Ay + w * Az - uelh * BB - uelz * CC, Ptl, rptl);
+            const double RHS2 = -2.0 * Re * uelz - Gr * Ptl * thh;
+            const double cf1 = rpdt - qdiv(uelr, ptl2, rptl2) - cnr - cnz - cnh;
+            const double cf2 = qdiv(uelr, ptl2, rptl2) + cnr + cnz + cnh;
+            F.An[bk + o] = (cf2 * F.Ap[bk + o] - ULT - RHS2 + RHS1) / cf1;
+        }
+        if (doB) {
+            const double u = (uK00 + uK01) * 0.5;
+            const double v = (vM00 + vM01 + vK00 + vK01 + vM10 + vM11 + vK10 + vK11) * 0.125;
+            const double w = (wK00 + wK10) * 0.5;
+            const double AA = (aK + apK + aM + apM) * 0.25;
+            const double CC = (cK + cjK + cM + cjM) * 0.25;
+            const double Brra = qdiv(bmK + B[bk + oip], dr2, rdr2);
+            const double velr = qdiv((vM10 + vM11 + vK10 + vK11) * 0.25 - (vM00 + vM01 + vK00 + vK01) * 0.25, dr, rdr);
+            const double Bzza = qdiv(bjK + B[bk + ojp], dz2, rdz2);
+            const double velz = qdiv((vM01 + vM11 + vK01 + vK11) * 0.25 - (vM00 + vM10 + vK00 + vK10) * 0.25, dz, rdz);
+            const double thr = qdiv((th[bk + oipjp] + th[bk + oip] - tjK - tK) * 0.5, dr, rdr);
+            const double Bhha = qdiv(bN + bM, dp2, rdp2);
+            const double velh = qdiv((vK00 + vK10 + vK01 + vK11) * 0.25 - (vM00 + vM10 + vM01 + vM11) * 0.25, dp, rdp);
+            const double cnr = -rdr2;
+            const double cnz = -rdz2;
+            const double cnh = -rdp2;
+            const double Bx = F.dv[3][bk + o], By = F.dv[4][bk + o], Bz = F.dv[5][bk + o];
+            const double RHS1 = Brra + Bhha + Bzza + qdiv(bK, pdt, rpdt);
+            const double ULT = qdiv(u * Bx + v * By + w * Bz - velr * AA - velz * CC, Ptl, rptl);
+            const double RHS2 = -2.0 * Re * velz + Gr * Ptl * thr;
+            const double cf1 = rpdt - qdiv(velh, ptl2, rptl2) - cnr - cnz - cnh;
+            const double cf2 = qdiv(velh, ptl2, rptl2) + cnr + cnz + cnh;
+            F.Bn[bk + o] = (cf2 * F.Bp[bk + o] - ULT - RHS2 + RHS1) / cf1;
+        }
+        if (doC && k <= np - 1) {
+            const double u = (uN00 + uK00) * 0.5;
+            const double v = (vK00 + vK10) * 0.5;
+            const double w = (wK00 + wK10 + wN00 + wN10 + wK0m + wK1m + wN0m + wN1m) * 0.125;
+            const double AA = (aK + A[bk + ojm] + apK + A[bk + oipjm]) * 0.25;
+            const double BB = (bK + bjK + bN + bjN) * 0.25;
+            const double Crra = qdiv(C[bk + oim] + C[bk + oip], dr2, rdr2);
+            const double welr = qdiv((wK10 + wN10 + wK1m + wN1m) * 0.25 - (wK00 + wN00 + wK0m + wN0m) * 0.25, dr, rdr);
+            double Czza, cnz;
+            if (j == 1) {
+                Czza = qdiv(3.2 * C[bk + ojm] + 2.0 * cjK - 0.2 * C[bk + ojp + li], dz2, rdz2);
+                cnz = q.c25z;
+            } else if (j == nz) {
+                Czza = qdiv(3.2 * cjK + 2.0 * C[bk + ojm] - 0.2 * C[bk + ojm - li], dz2, rdz2);
+                cnz = q.c25z;
+            } else {
+                Czza = qdiv(C[bk + ojm] + cjK, dz2, rdz2);
+                cnz = -rdz2;
+            }
+            const double welz = qdiv((wK00 + wK10 + wN00 + wN10) * 0.25 - (wK0m + wK1m + wN0m + wN1m) * 0.25, dz, rdz);
+            const double Chha = qdiv(cN + cM, dp2, rdp2);
+            const double welh = qdiv((wN00 + wN10 + wN0m + wN1m) * 0.25 - (wK00 + wK10 + wK0m + wK1m) * 0.25, dp, rdp);
+            const double cnr = -rdr2;
+            const double cnh = -rdp2;
+            const double Cx = F.dv[6][bk + o], Cy = F.dv[7][bk + o], Cz = F.dv[8][bk + o];
+            const double RHS1 = Crra + Chha + Czza + qdiv(cK, pdt, rpdt);
+            const double ULT = qdiv(u * Cx + v * Cy + w * Cz - welr * AA - welh * BB, Ptl, rptl);
+            const double RHS2 = -2.0 * Re * welz;
+            const double cf1 = rpdt - qdiv(welz, ptl2, rptl2) - cnr - cnz - cnh;
+            const double cf2 = qdiv(welz, ptl2, rptl2) + cnr + cnz + cnh;
+            F.Cn[bk + o] = (cf2 * F.Cp[bk + o] - ULT - RHS2 + RHS1) / cf1;
+        }
+        if (MASK & 2) {
+            const double u = (uK00 + uK10) * 0.5;
+            const double v = (vK00 + vM00) * 0.5;
+            const double w = (wK00 + wK0m) * 0.5;
+            const double thim = th[bk + oim], thip = th[bk + oip], thjm = th[bk + ojm];
+            double thrra, cnr, thzza, cnz;
+            if (i == 1) {
+                thrra = qdiv(3.2 * thim + 2.0 * thip - 0.2 * th[bk + o + 2u], dr2, rdr2);
+                cnr = q.c25r;
+            } else if (i == nr) {
+                thrra = qdiv(3.2 * thip + 2.0 * thim - 0.2 * th[bk + o - 2u], dr2, rdr2);
+                cnr = q.c25r;
+            } else {
+                thrra = qdiv(thim + thip, dr2, rdr2);
+                cnr = -rdr2;
+            }
+            if (j == 1) {
+                thzza = qdiv(3.2 * thjm + 2.0 * tjK - 0.2 * th[bk + ojp + li], dz2, rdz2);
+                cnz = q.c25z;
+            } else if (j == nz) {
+                thzza = qdiv(3.2 * tjK + 2.0 * thjm - 0.2 * th[bk + ojm - li], dz2, rdz2);
+                cnz = q.c25z;
+            } else {
+                thzza = qdiv(thjm + tjK, dz2, rdz2);
+                cnz = -rdz2;
+            }
+            const double thhha = qdiv(tN + tM, dp2, rdp2);
+            const double cnp = -rdp2;
+            const double Tx = F.dv[9][bk + o], Ty = F.dv[10][bk + o], Tz = F.dv[11][bk + o];
+            const double RHS1 = thrra + thhha + thzza + qdiv(tK, dt, rdt);
+            const double ULT = u * Tx + v * Ty + w * Tz;
+            const double cf1 = rdt - cnr - cnz - cnp;
+            const double cf2 = cnr + cnz + cnp;
+            F.thn[bk + o] = (cf2 * F.thp[bk + o] - ULT + RHS1) / cf1;
+        }
+        // move the march on one plane
+        aM = aK; aK = aN; apM = apK;
+        bM = bK; bK = bN; bmK = bmN; bjK = bjN;
+        cM = cK; cK = cN; cjM = cjK;
+        tM = tK; tK = tN; tjK = tjN;
+        uK00 = uN00; uK01 = uN01; uK10 = uN10; uK11 = uN11;
+        vM00 = vK00; vM01 = vK01; vM10 = vK10; vM11 = vK11;
+        wK00 = wN00; wK10 = wN10; wK0m = wN0m; wK1m = wN1m;
     }
-    if (j == 1) {
-        thzza = (3.2 * X(th, i, j - 1, k) + 2.0 * X(th, i, j + 1, k) - 0.2 * X(th, i, j + 2, k)) / (dz * dz);
-        cnz = -2.5 / (dz * dz);
-    } else if (j == nz) {
-        thzza = (3.2 * X(th, i, j + 1, k) + 2.0 * X(th, i, j - 1, k) - 0.2 * X(th, i, j - 2, k)) / (dz * dz);
-        cnz = -2.5 / (dz * dz);
-    } else {
-        thzza = (X(th, i, j - 1, k) + X(th, i, j + 1, k)) / (dz * dz);
-        cnz = -1.0 / (dz * dz);
-    }
-    const double thhha = (X(th, i, j, ka) + X(th, i, j, km)) / (dp * dp);
-    const double cnp = -1.0 / (dp * dp);
-    const double Tx = X(F.dv[9], i, j, k), Ty = X(F.dv[10], i, j, k), Tz = X(F.dv[11], i, j, k);
-    const double RHS1 = thrra + thhha + thzza + X(th, i, j, k) / dt;
-    const double ULT = u * Tx + v * Ty + w * Tz;
-    const double cf1 = 1.0 / dt - cnr - cnz - cnp;
-    const double cf2 = cnr + cnz + cnp;
-    X(F.thn, i, j, k) = (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
 }
 
 // heat(trans1,trans2), velvor2dhorconvtank.f:1589-1620.  Its k-sum adds, for
@@ -1445,36 +1486,21 @@ int launch_derivatives(vvt_run *r)
     return VV_OK;
 }
 
-int launch_vorticity(vvt_run *r)
+// mask 1: A, B, C; 2: temperature; 3: all four
+int launch_explicit(vvt_run *r, int mask)
 {
+    // algorithmic words per point: each field read or written once -- A, B,
+    // C, their previous levels, th, u, v, w and nine derivatives, three
+    // outputs (mask 1); th, thp, u, v, w, three derivatives, thn (mask 2)
+    const double words = mask == 3 ? 27.0 : (mask == 1 ? 22.0 : 9.0);
+    Prof p(r, VVT_K_EXPLICIT, 8.0 * words * npts(r));
+    const dim3 grid(nblk((long long)r->nr * r->nz, WG), (r->np + KC - 1) / KC);
     const TGeo g = geo(r);
     const VF F = vfields(r);
-    const double pa = (double)r->nr * (r->nz - 1) * (r->np - 1), pb = (double)(r->nr - 1) * (r->nz - 1) * r->np,
-                 pc = (double)(r->nr - 1) * r->nz * (r->np - 1);
-    {   // reads A, Ap, B, C, th, u, v, w and three derivatives; writes An
-        Prof p(r, VVT_K_AVORT, 8.0 * 12.0 * pa);
-        hipLaunchKernelGGL(k_t_avort, dim3(nblk((long long)r->nr * (r->nz - 1), WG), r->np - 1), dim3(WG), 0,
-                           r->stream, g, F);
-    }
-    {
-        Prof p(r, VVT_K_BVORT, 8.0 * 12.0 * pb);
-        hipLaunchKernelGGL(k_t_bvort, dim3(nblk((long long)(r->nr - 1) * (r->nz - 1), WG), r->np), dim3(WG), 0,
-                           r->stream, g, F);
-    }
-    {   // no temperature term
-        Prof p(r, VVT_K_CVORT, 8.0 * 11.0 * pc);
-        hipLaunchKernelGGL(k_t_cvort, dim3(nblk((long long)(r->nr - 1) * r->nz, WG), r->np - 1), dim3(WG), 0,
-                           r->stream, g, F);
-    }
-    VV_HIP(hipGetLastError());
-    return VV_OK;
-}
-
-int launch_temperature(vvt_run *r)
-{
-    Prof p(r, VVT_K_TEMP, 8.0 * 9.0 * npts(r));   // th, thp, u, v, w, three derivatives; thn
-    hipLaunchKernelGGL(k_t_temp, dim3(nblk((long long)r->nr * r->nz, WG), r->np), dim3(WG), 0, r->stream, geo(r),
-                       vfields(r));
+    const XDiv q = xdiv(g);
+    if (mask == 1) hipLaunchKernelGGL(k_t_explicit<1>, grid, dim3(WG), 0, r->stream, g, F, q);
+    else if (mask == 2) hipLaunchKernelGGL(k_t_explicit<2>, grid, dim3(WG), 0, r->stream, g, F, q);
+    else hipLaunchKernelGGL(k_t_explicit<3>, grid, dim3(WG), 0, r->stream, g, F, q);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -1652,8 +1678,7 @@ int one_step(vvt_run *r)
 {
     int st;
     if ((st = launch_derivatives(r))) return st;
-    if ((st = launch_vorticity(r))) return st;
-    if ((st = launch_temperature(r))) return st;
+    if ((st = launch_explicit(r, 3))) return st;
     if ((st = launch_heat(r))) return st;
     for (int w = 1; w <= 3; ++w) {
         if ((st = launch_rhs(r, w))) return st;
@@ -1797,8 +1822,8 @@ int vvt_get_derivative(vvt_run *r, int d, double *host)
     } while (0)
 
 int vvt_derivatives(vvt_run *r) { VVT_CALL(launch_derivatives(r)); }
-int vvt_vorticity(vvt_run *r) { VVT_CALL(launch_vorticity(r)); }
-int vvt_temperature(vvt_run *r) { VVT_CALL(launch_temperature(r)); }
+int vvt_vorticity(vvt_run *r) { VVT_CALL(launch_explicit(r, 1)); }
+int vvt_temperature(vvt_run *r) { VVT_CALL(launch_explicit(r, 2)); }
 int vvt_update(vvt_run *r) { VVT_CALL(launch_update(r)); }
 
 int vvt_rhs(vvt_run *r, int which)

@@ -18,7 +18,7 @@ from .velvor3 import FIELD_IDS, FIELDS  # noqa: F401  (the tank uses rot3dmg.f's
 
 _P = ctypes.POINTER(ctypes.c_double)
 KERNELS = ["deriv_yz", "deriv_x", "avort", "bvort", "cvort", "temp", "heat", "rhs", "psi_pre", "yfwd", "zfwd",
-           "xsolve", "zinv", "yinv", "update", "yfft_fwd", "yfft_inv"]
+           "xsolve", "zinv", "yinv", "update", "yfft_fwd", "yfft_inv", "explicit"]
 # the work of these kernels is counted in floating-point operations, of the others in bytes
 FLOP_KERNELS = ("yfwd", "zfwd", "zinv", "yinv")
 

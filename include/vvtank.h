@@ -98,10 +98,10 @@ int vvt_energy(vvt_run *run, double *ke);
 enum vvt_kernel_id {
     VVT_K_DERIV_YZ = 0,  /* compact derivatives along y and z (one thread per line) */
     VVT_K_DERIV_X,       /* compact derivatives along x (lines staged in LDS)       */
-    VVT_K_AVORT,         /* A update                                                */
-    VVT_K_BVORT,         /* B update                                                */
-    VVT_K_CVORT,         /* C update                                                */
-    VVT_K_TEMP,          /* temperature update                                      */
+    VVT_K_AVORT,         /* not used: the A, B, C and temperature updates run       */
+    VVT_K_BVORT,         /*   fused, as VVT_K_EXPLICIT                              */
+    VVT_K_CVORT,         /*                                                         */
+    VVT_K_TEMP,          /*                                                         */
     VVT_K_HEAT,          /* heat fluxes                                             */
     VVT_K_RHS,           /* right-hand sides                                        */
     VVT_K_PSI_PRE,       /* psip <- psi, walls cleared                              */
@@ -113,6 +113,8 @@ enum vvt_kernel_id {
     VVT_K_UPDATE,        /* boundaries and the rotation copy                        */
     VVT_K_YFFT_FWD,      /* y modes, forward, by FFT (np a power of two; bytes)     */
     VVT_K_YFFT_INV,      /* y modes, inverse, by FFT (bytes)                        */
+    VVT_K_EXPLICIT,      /* A, B, C and temperature updates (one pass over the      */
+                         /*   fields they read)                                     */
     VVT_K_COUNT
 };
 int vvt_profile(vvt_run *run, int enable);

@@ -66,6 +66,25 @@ def test_explicit_routines_bitexact(shape):
     run.close()
 
 
+@pytest.mark.parametrize("shape", [(8, 6, 10), (13, 9, 40), (24, 16, 17)])
+def test_step_explicit_part_bitexact(shape):
+    """A whole step runs the four updates in one pass: after one step the
+    new A, B, C and th at their updated points (the solves and the update
+    touch only velocities, walls and ghosts) and the heat fluxes are the
+    oracle's, bit for bit."""
+    nr, nz, np_ = shape
+    print(f"[gpu] tank step explicit {shape}", flush=True)
+    run, ora = _pair(nr, nz, np_, seed=11 + nr + 5 * nz + np_)
+    tg = run.step(1)
+    to = ora.step()
+    assert tg == to
+    inner = {"A": np.s_[1:np_, 1:nz, 1:nr + 1], "B": np.s_[1:np_ + 1, 1:nz, 1:nr],
+             "C": np.s_[1:np_, 1:nz + 1, 1:nr], "th": np.s_[1:np_ + 1, 1:nz + 1, 1:nr + 1]}
+    for n, sl in inner.items():
+        assert np.array_equal(run.get(n)[sl], ora[n][sl]), n
+    run.close()
+
+
 @pytest.mark.parametrize("shape", [(6, 6, 2100), (40, 70, 6)])
 def test_derivatives_long_lines_bitexact(shape):
     """Lines longer than the tables staged in LDS (y lines of 2100 cells) and
