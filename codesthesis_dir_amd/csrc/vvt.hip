@@ -1146,6 +1146,117 @@ __global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double 
     }
 }
 
+// The same solves without the factor array: the forward sweep keeps only
+// the state (c, y) entering every XCK-th row (in LDS, per thread), and the
+// back substitution takes the segments from the last: it reads the
+// segment's right-hand sides again, repeats their elimination in registers
+// from the state before it -- the same operations, so the same bits -- and
+// substitutes back.  T is read twice and written once (against reading it
+// twice, writing it twice and the factors once each).  Lines of up to
+// XCK * XSEG_MAX rows.
+constexpr int XCK = 16;
+constexpr int XSEG_MAX = 64;
+
+__global__ void __launch_bounds__(64) k_t_xsolve_ck(double *__restrict__ T, const double *__restrict__ rt,
+                                                     const double *__restrict__ lz, const double *__restrict__ ly,
+                                                     int i1, int j1, int k1, double e1, double en)
+{
+    extern __shared__ double xck[];   // (c, y) entering segment sg at xck[(2 sg + w) 64 + lane]
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k1 * j1) return;   // no barriers below
+    const int q = t / j1, l = t - q * j1;
+    const size_t mt = (size_t)i1 * j1;
+    double *__restrict__ x = T + (size_t)q * mt + l;
+    double *ck = xck + threadIdx.x;
+    const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1;
+    const double shift = ly[q] + lz[l];
+    const int nseg = (i1 + XCK - 1) / XCK;
+    // row 0, its third coefficient folded in with row 1
+    double d0 = dia[0] + shift, s0 = sup[0];
+    const double x0 = x[0], x1 = x[j1];
+    double f0 = 0.0;
+    if (e1 != 0.0) {
+        f0 = e1 / sup[1];
+        d0 -= f0 * sub[1];
+        s0 -= f0 * (dia[1] + shift);
+    }
+    // one elimination row: a and r as given (the last row's fold applied by the caller)
+    auto row = [&](int i, double a, double r, double &c, double &y) {
+        const double den = dia[i] + shift - a * c;
+        c = sup[i] / den;
+        y = (r - a * y) / den;
+    };
+    auto row0 = [&](double &c, double &y) {
+        y = x0;
+        if (e1 != 0.0) y -= f0 * x1;
+        c = s0 / d0;
+        y /= d0;
+    };
+    // forward sweep: rows in groups of XCK with their loads issued ahead
+    double c, y, c3 = 0.0, y3 = 0.0;   // (c, y) of row i1-3 for the last row's fold
+    row0(c, y);
+    if (i1 == 3) { c3 = c; y3 = y; }
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int r0 = sg * XCK;
+        if (sg > 0) {
+            ck[(2 * sg) * 64] = c;
+            ck[(2 * sg + 1) * 64] = y;
+        }
+        double ru[XCK];
+#pragma unroll
+        for (int u = 0; u < XCK; ++u) ru[u] = (r0 + u < i1) ? x[(size_t)(r0 + u) * j1] : 0.0;
+#pragma unroll
+        for (int u = 0; u < XCK; ++u) {
+            const int i = r0 + u;
+            if (i == 0 || i >= i1) continue;
+            double a = sub[i], r = ru[u];
+            if (i == i1 - 1 && en != 0.0) {
+                a -= en * c3;
+                r -= en * y3;
+            }
+            row(i, a, r, c, y);
+            if (i == i1 - 3) { c3 = c; y3 = y; }
+        }
+    }
+    // back substitution, segment by segment from the last
+    double xn = 0.0;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int r0 = sg * XCK;
+        double ru[XCK], cv[XCK], yv[XCK];
+#pragma unroll
+        for (int u = 0; u < XCK; ++u) ru[u] = (r0 + u < i1) ? x[(size_t)(r0 + u) * j1] : 0.0;
+        if (sg > 0) {
+            c = ck[(2 * sg) * 64];
+            y = ck[(2 * sg + 1) * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < XCK; ++u) {
+            const int i = r0 + u;
+            if (i < i1) {
+                if (i == 0) {
+                    row0(c, y);
+                } else {
+                    double a = sub[i], r = ru[u];
+                    if (i == i1 - 1 && en != 0.0) {
+                        a -= en * c3;
+                        r -= en * y3;
+                    }
+                    row(i, a, r, c, y);
+                }
+            }
+            cv[u] = c;
+            yv[u] = y;
+        }
+#pragma unroll
+        for (int u = XCK - 1; u >= 0; --u) {
+            const int i = r0 + u;
+            if (i >= i1) continue;
+            xn = (i == i1 - 1) ? yv[u] : yv[u] - cv[u] * xn;
+            x[(size_t)i * j1] = xn;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(WG) k_t_energy(TGeo g, const double *uel, const double *vel, const double *wel,
                                                   double *part)
 {
@@ -1574,11 +1685,17 @@ int launch_poisson(vvt_run *r, int which)
         Prof p(r, VVT_K_ZFWD, ftr_z);
         vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->S, r->T, mt}, O.k1, s);
     }
-    {   // reads and writes T, writes the factors, reads them back
-        Prof p(r, VVT_K_XSOLVE, 8.0 * 5.0 * O.k1 * (double)mt);
-        hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
-                           (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
-                           O.en);
+    {   // reads T and writes it (algorithmic: once each)
+        Prof p(r, VVT_K_XSOLVE, 8.0 * 2.0 * O.k1 * (double)mt);
+        const int nseg = (O.i1 + XCK - 1) / XCK;
+        if (nseg <= XSEG_MAX)
+            hipLaunchKernelGGL(k_t_xsolve_ck, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64),
+                               (size_t)2 * nseg * 64 * sizeof(double), s, r->T, (const double *)O.rt,
+                               (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1, O.en);
+        else
+            hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
+                               (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1,
+                               O.e1, O.en);
     }
     {
         Prof p(r, VVT_K_ZINV, ftr_z);

@@ -100,8 +100,10 @@ def test_derivatives_long_lines_bitexact(shape):
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(8, 6, 10), (7, 9, 6)])
+@pytest.mark.parametrize("shape", [(8, 6, 10), (7, 9, 6), (35, 6, 8), (40, 6, 7)])
 def test_poisson_matches_dense_solve(which, shape):
+    """Including x lines of several 16-row segments (35 and 40 cells), with
+    y modes by FFT (np = 8) and by GEMM."""
     nr, nz, np_ = shape
     print(f"[gpu] tank poisson {which} {shape}", flush=True)
     run, ora = _pair(nr, nz, np_, seed=which)
