@@ -1066,108 +1066,29 @@ __global__ void __launch_bounds__(WG) k_t_psi_post(TGeo g, double *psi)
 // per (y mode q, z eigenvalue l): (D_x + mu_q + lz_l) x = y along i, x
 // overwriting y in T (layout [q][i][l]).  Row 1 has an extra coefficient e1
 // two points inward, removed with row 2; row i1 has en, removed with the
-// eliminated row i1 - 2.  Thomas factors go to cp.
-__global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double *__restrict__ cp,
-                                                  const double *__restrict__ rt, const double *__restrict__ lz,
-                                                  const double *__restrict__ ly, int i1, int j1, int k1, double e1,
-                                                  double en)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= k1 * j1) return;
-    const int q = t / j1, l = t - q * j1;
-    const size_t mt = (size_t)i1 * j1;
-    double *__restrict__ x = T + (size_t)q * mt + l;
-    double *__restrict__ c = cp + (size_t)q * mt + l;
-    const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1;
-    const double shift = ly[q] + lz[l];
-    double d0 = dia[0] + shift, s0 = sup[0], y = x[0];
-    if (e1 != 0.0) {
-        const double f = e1 / sup[1];
-        d0 -= f * sub[1];
-        s0 -= f * (dia[1] + shift);
-        y -= f * x[j1];
-    }
-    double cprev = s0 / d0;
-    y /= d0;
-    c[0] = cprev;
-    x[0] = y;
-    // rows 1 .. i1-2 in groups of U with their loads issued ahead
-    constexpr int U = 8;
-    int i = 1;
-    for (; i + U <= i1 - 1; i += U) {
-        double ru[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) ru[u] = x[(size_t)(i + u) * j1];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int ii = i + u;
-            const size_t o = (size_t)ii * j1;
-            const double a = sub[ii];
-            const double den = dia[ii] + shift - a * cprev;
-            cprev = sup[ii] / den;
-            y = (ru[u] - a * y) / den;
-            c[o] = cprev;
-            x[o] = y;
-        }
-    }
-    for (; i < i1; ++i) {
-        const size_t o = (size_t)i * j1;
-        double a = sub[i], r = x[o];
-        if (i == i1 - 1 && en != 0.0) {
-            const size_t o2 = (size_t)(i - 2) * j1;
-            a -= en * c[o2];
-            r -= en * x[o2];
-        }
-        const double den = dia[i] + shift - a * cprev;
-        cprev = sup[i] / den;
-        y = (r - a * y) / den;
-        c[o] = cprev;
-        x[o] = y;
-    }
-    i = i1 - 2;
-    for (; i - U + 1 >= 0; i -= U) {
-        double fu[U], xu[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t o = (size_t)(i - u) * j1;
-            fu[u] = c[o];
-            xu[u] = x[o];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            y = xu[u] - fu[u] * y;
-            x[(size_t)(i - u) * j1] = y;
-        }
-    }
-    for (; i >= 0; --i) {
-        const size_t o = (size_t)i * j1;
-        y = x[o] - c[o] * y;
-        x[o] = y;
-    }
-}
-
-// The same solves without the factor array: the forward sweep keeps only
-// the state (c, y) entering every XCK-th row (in LDS, per thread), and the
-// back substitution takes the segments from the last: it reads the
-// segment's right-hand sides again, repeats their elimination in registers
-// from the state before it -- the same operations, so the same bits -- and
-// substitutes back.  T is read twice and written once (against reading it
-// twice, writing it twice and the factors once each).  Lines of up to
-// XCK * XSEG_MAX rows.
+// eliminated row i1 - 2.  Thomas elimination, one thread per line (threads
+// on consecutive l: coalesced), without a factor array: the forward sweep
+// keeps only the state (c, y) entering every XCK-th row (in ckp, 2/XCK of
+// T), and the back substitution takes the segments from the last: it reads
+// the segment's right-hand sides again, repeats their elimination in
+// registers from the state before it -- the same operations, so the same
+// bits -- and substitutes back.  T is read twice and written once (against
+// reading it twice, writing it twice and the factors once each).
 constexpr int XCK = 16;
-constexpr int XSEG_MAX = 64;
 
-__global__ void __launch_bounds__(64) k_t_xsolve_ck(double *__restrict__ T, const double *__restrict__ rt,
-                                                     const double *__restrict__ lz, const double *__restrict__ ly,
-                                                     int i1, int j1, int k1, double e1, double en)
+__global__ void __launch_bounds__(64) k_t_xsolve(double *__restrict__ T, double *__restrict__ ckp,
+                                                     const double *__restrict__ rt, const double *__restrict__ lz,
+                                                     const double *__restrict__ ly, int i1, int j1, int k1, double e1,
+                                                     double en)
 {
-    extern __shared__ double xck[];   // (c, y) entering segment sg at xck[(2 sg + w) 64 + lane]
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= k1 * j1) return;   // no barriers below
+    const int nlines = k1 * j1;
+    if (t >= nlines) return;
     const int q = t / j1, l = t - q * j1;
     const size_t mt = (size_t)i1 * j1;
     double *__restrict__ x = T + (size_t)q * mt + l;
-    double *ck = xck + threadIdx.x;
+    // (c, y) entering segment sg at ck[(2 sg + w) nlines]
+    double *__restrict__ ck = ckp + t;
     const double *sub = rt, *dia = rt + i1, *sup = rt + 2 * i1;
     const double shift = ly[q] + lz[l];
     const int nseg = (i1 + XCK - 1) / XCK;
@@ -1199,8 +1120,8 @@ __global__ void __launch_bounds__(64) k_t_xsolve_ck(double *__restrict__ T, cons
     for (int sg = 0; sg < nseg; ++sg) {
         const int r0 = sg * XCK;
         if (sg > 0) {
-            ck[(2 * sg) * 64] = c;
-            ck[(2 * sg + 1) * 64] = y;
+            ck[(size_t)(2 * sg) * nlines] = c;
+            ck[(size_t)(2 * sg + 1) * nlines] = y;
         }
         double ru[XCK];
 #pragma unroll
@@ -1226,8 +1147,8 @@ __global__ void __launch_bounds__(64) k_t_xsolve_ck(double *__restrict__ T, cons
 #pragma unroll
         for (int u = 0; u < XCK; ++u) ru[u] = (r0 + u < i1) ? x[(size_t)(r0 + u) * j1] : 0.0;
         if (sg > 0) {
-            c = ck[(2 * sg) * 64];
-            y = ck[(2 * sg + 1) * 64];
+            c = ck[(size_t)(2 * sg) * nlines];
+            y = ck[(size_t)(2 * sg + 1) * nlines];
         }
 #pragma unroll
         for (int u = 0; u < XCK; ++u) {
@@ -1687,15 +1608,10 @@ int launch_poisson(vvt_run *r, int which)
     }
     {   // reads T and writes it (algorithmic: once each)
         Prof p(r, VVT_K_XSOLVE, 8.0 * 2.0 * O.k1 * (double)mt);
-        const int nseg = (O.i1 + XCK - 1) / XCK;
-        if (nseg <= XSEG_MAX)
-            hipLaunchKernelGGL(k_t_xsolve_ck, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64),
-                               (size_t)2 * nseg * 64 * sizeof(double), s, r->T, (const double *)O.rt,
-                               (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1, O.en);
-        else
-            hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
-                               (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1,
-                               O.e1, O.en);
+        // checkpoints in cp: 2 ceil(i1/XCK) k1 j1 <= i1 k1 j1 words (i1 >= 5)
+        hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
+                           (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
+                           O.en);
     }
     {
         Prof p(r, VVT_K_ZINV, ftr_z);
