@@ -15,12 +15,14 @@
 #include "vv_reduce.h"
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 struct vv3_run {
     // every C entry point on a run holds this lock, so calls on one run from
     // several threads are serialised
     std::recursive_mutex mu;
+    vv::PinnedStage stage;   // host copies (vv3_put / vv3_get)
     vv3_config cfg;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -876,24 +878,34 @@ static int copy3(vv3_run *r, int field, double *host, int ldi, int ldj, bool to_
     }
     VV_HIP(hipSetDevice(r->device));
     double *dev = r->f[field];
-    const hipMemcpyKind kind = to_dev ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
-    if (ldi == di && ldj == dj) {
-        if (to_dev) VV_HIP(hipMemcpyAsync(dev, host, r->nf * sizeof(double), kind, r->stream));
-        else VV_HIP(hipMemcpyAsync(host, dev, r->nf * sizeof(double), kind, r->stream));
-    } else {
-        // one pitched copy per k plane (rows of nr+2 doubles)
-        for (int k = 0; k < nk; ++k) {
-            double *h = host + (size_t)k * ldi * ldj;
-            double *d = dev + (size_t)k * di * dj;
-            if (to_dev)
-                VV_HIP(hipMemcpy2DAsync(d, di * sizeof(double), h, ldi * sizeof(double), di * sizeof(double), dj,
-                                        kind, r->stream));
-            else
-                VV_HIP(hipMemcpy2DAsync(h, ldi * sizeof(double), d, di * sizeof(double), di * sizeof(double), dj,
-                                        kind, r->stream));
+    // planes k = 0..np through the pinned stage in the device layout (rows
+    // of nr+2 doubles, di x dj per plane): one DMA copy
+    const size_t words = (size_t)di * dj * nk;
+    double *st = r->stage.get(words);
+    if (!st) { vv::set_error("vv3_put/vv3_get: pinned staging buffer"); return VV_ENOMEM; }
+    auto rows = [&](bool pack) {
+        if (ldi == di && ldj == dj) {
+            if (pack) std::memcpy(st, host, words * sizeof(double));
+            else std::memcpy(host, st, words * sizeof(double));
+            return;
         }
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < dj; ++j) {
+                double *h = host + ((size_t)k * ldj + j) * ldi;
+                double *s = st + ((size_t)k * dj + j) * di;
+                if (pack) std::memcpy(s, h, (size_t)di * sizeof(double));
+                else std::memcpy(h, s, (size_t)di * sizeof(double));
+            }
+    };
+    if (to_dev) {
+        rows(true);
+        VV_HIP(hipMemcpyAsync(dev, st, words * sizeof(double), hipMemcpyHostToDevice, r->stream));
+        VV_HIP(hipStreamSynchronize(r->stream));
+    } else {
+        VV_HIP(hipMemcpyAsync(st, dev, words * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+        VV_HIP(hipStreamSynchronize(r->stream));
+        rows(false);
     }
-    VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
 

@@ -450,9 +450,12 @@ int vv_put(vv_run *r, int field, const double *host, int ld)
     double *d = field_ptr(r, field);
     if (!d) { set_error("vv_put: unknown field"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
-    VV_HIP(hipMemcpy2DAsync(d, (size_t)r->ld * sizeof(double), host, (size_t)ld * sizeof(double),
-                            (size_t)(r->nr + 2) * sizeof(double), r->ncols, hipMemcpyHostToDevice,
-                            r->stream));
+    const size_t w = (size_t)r->nr + 2;
+    double *st = r->stage.get(w * r->ncols);
+    if (!st) { set_error("vv_put: pinned staging buffer"); return VV_ENOMEM; }
+    for (int c = 0; c < r->ncols; ++c) std::memcpy(st + c * w, host + (size_t)c * ld, w * sizeof(double));
+    VV_HIP(hipMemcpy2DAsync(d, (size_t)r->ld * sizeof(double), st, w * sizeof(double), w * sizeof(double), r->ncols,
+                            hipMemcpyHostToDevice, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
@@ -464,10 +467,13 @@ int vv_get(vv_run *r, int field, double *host, int ld)
     double *d = field_ptr(r, field);
     if (!d) { set_error("vv_get: unknown field"); return VV_EINVAL; }
     VV_HIP(hipSetDevice(r->device));
-    VV_HIP(hipMemcpy2DAsync(host, (size_t)ld * sizeof(double), d, (size_t)r->ld * sizeof(double),
-                            (size_t)(r->nr + 2) * sizeof(double), r->ncols, hipMemcpyDeviceToHost,
-                            r->stream));
+    const size_t w = (size_t)r->nr + 2;
+    double *st = r->stage.get(w * r->ncols);
+    if (!st) { set_error("vv_get: pinned staging buffer"); return VV_ENOMEM; }
+    VV_HIP(hipMemcpy2DAsync(st, w * sizeof(double), d, (size_t)r->ld * sizeof(double), w * sizeof(double), r->ncols,
+                            hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
+    for (int c = 0; c < r->ncols; ++c) std::memcpy(host + (size_t)c * ld, st + c * w, w * sizeof(double));
     return VV_OK;
 }
 

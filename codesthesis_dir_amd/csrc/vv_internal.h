@@ -136,6 +136,33 @@ struct MgBatchDev {
     int nbr;
 };
 
+// Pinned host staging for the field copies between a caller's pitched
+// host arrays and the device: rows are packed into (or unpacked from) it on
+// the CPU, and one DMA moves it (copies from pageable memory with a pitch
+// go row by row through the runtime's own staging).  Grown on demand, held
+// until the run is destroyed; every copy holds the run's lock.
+struct PinnedStage {
+    double *p = nullptr;
+    size_t n = 0;
+    double *get(size_t words)
+    {
+        if (words <= n) return p;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&p), words * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return nullptr;
+        }
+        n = words;
+        return p;
+    }
+    ~PinnedStage()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 }  // namespace vv
 
 // ---------------------------------------------------------------- vv_run
@@ -144,6 +171,7 @@ struct vv_run {
     // several threads are serialised (the reference's main program calls
     // rhsv/poisson2 and rhsu/poisson3 in two OpenMP sections, rb.f:370-380)
     std::recursive_mutex mu;
+    vv::PinnedStage stage;   // host copies (vv_put / vv_get)
     // Robert-filtered current levels waiting in Cf/thf between the explicit
     // step (vv_cvorticity / vv_temperature) and vv_update
     struct Pend { bool c = false, t = false; } pend;

@@ -5,11 +5,26 @@
 ! libvvmi.so) in place of the reference's own subroutines.  The routine
 ! names and argument lists are the reference's; the COMMON blocks are the
 ! reference's, dimensioned with the caller's PARAMETERs VV_KR and VV_KP
-! (defaults 290 and 1610 as in velvor2dperiodicrbconv.f:54).  Each routine
-! copies its inputs from COMMON to the device run, calls the HIP library and
-! copies its outputs back, so the caller's view of COMMON is the same as with
-! the reference routines.  The device run is created by poissoninit from
-! COMMON /num/ /val/ /radius/.
+! (defaults 290 and 1610 as in velvor2dperiodicrbconv.f:54).  The device run
+! is created by poissoninit from COMMON /num/ /val/ /radius/.
+!
+! Residency (the default).  The device run keeps the state between calls and
+! every field carries two coherence bits: host-newer (put before the field's
+! next use; all fields start so) and device-newer.  A routine puts only its
+! host-newer inputs and leaves its outputs on the device; update, the last
+! routine of a step, then copies back the fields the reference's time loop
+! reads between steps -- C, th, uel and vel: its running means, CFL check
+! and output files (velvor2dperiodicrbconv.f:398-470) -- so a step moves
+! four fields and the scalars (fluxes, clamp count) instead of every
+! routine's inputs and outputs.  poisson2/3 solve on the device when their
+! arguments are the COMMON fields and the array rhsv/rhsu filled, and fall
+! back to the host copies otherwise.  Callers that read other fields
+! between steps call vv_dropin_sync (every device-newer field to COMMON);
+! callers that write a field the step uses between calls declare it with
+! vv_dropin_host_dirty.  vv_dropin_residency(0), or VV_DROPIN_RESIDENT=0 in
+! the environment at poissoninit, selects full copies: each routine then
+! copies its inputs from COMMON and its outputs back, and the caller's view
+! of COMMON is the reference routines' after every call.
 !
 ! Replaced routines (velvor2dperiodicrbconv.f):
 !   Cvorticity1 584, Cvorticity 786, temperature1 1056, temperature 1220,
@@ -96,6 +111,13 @@ module vv_dropin_c
       type(c_ptr), value :: run
       integer(c_int), intent(out) :: nclamp
     end function
+    integer(c_int) function vv_poisson(run, which, cycles, relres) bind(C, name="vv_poisson")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: run
+      integer(c_int), value :: which
+      integer(c_int), intent(out) :: cycles(2)
+      real(c_double), intent(out) :: relres(2)
+    end function
     integer(c_int) function vv_poisson_host(run, which, psi, psip, rhs, ld, cycles, relres) &
          bind(C, name="vv_poisson_host")
       import :: c_int, c_ptr, c_double
@@ -115,6 +137,14 @@ module vv_dropin_c
   real(c_double), save :: g_width = 3.0d0        ! velvor2dperiodicrbconv.f:132
   real(c_double), save :: g_rtol = 1.0d-10
   integer(c_int), parameter :: g_ld = VV_KR + 1
+  ! coherence bits per field id (VV_C .. VV_URH), the residency switch, the
+  ! arrays rhsv / rhsu last filled on the device, and the fields update
+  ! returns to COMMON
+  logical, save :: g_resident = .true.
+  logical, save :: g_hdirty(0:11) = .true.
+  logical, save :: g_ddirty(0:11) = .false.
+  integer(c_intptr_t), save :: g_rhs_loc(0:11) = 0
+  integer(c_int), parameter :: g_mirror(4) = [VV_C, VV_TH, VV_UEL, VV_VEL]
 
 contains
 
@@ -183,6 +213,60 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%precision = 0
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl
+  call vv_dropin_env_residency()
+  g_hdirty = .true.
+  g_ddirty = .false.
+  g_rhs_loc = 0
+end subroutine
+
+subroutine vv_dropin_env_residency()
+  use vv_dropin_c
+  implicit none
+  character(len=8) :: val
+  integer :: st
+  call get_environment_variable('VV_DROPIN_RESIDENT', val, status=st)
+  if (st == 0) g_resident = (val(1:1) /= '0')
+end subroutine
+
+! mode 1: residency (the default); 0: full copies.  Switching to copies
+! first returns every device-newer field to COMMON; switching to residency
+! takes COMMON as the newer copy of every field.
+subroutine vv_dropin_residency(mode)
+  use vv_dropin_c
+  implicit none
+  integer, intent(in) :: mode
+  if (mode == 0 .and. g_resident .and. c_associated(g_run)) call vv_dropin_sync()
+  if (mode /= 0 .and. .not. g_resident) g_hdirty = .true.
+  g_resident = (mode /= 0)
+end subroutine
+
+! every device-newer COMMON field back to the host
+subroutine vv_dropin_sync()
+  use vv_dropin_c
+  implicit none
+  if (.not. g_resident) return
+  call vv_dropin_get_state([VV_C, VV_CP, VV_CN, VV_TH, VV_THP, VV_THN, VV_UEL, VV_VEL, VV_UELP, VV_VELP], 10)
+end subroutine
+
+! the caller has written COMMON field `field` (a VV_* id) since the last call
+subroutine vv_dropin_host_dirty(field)
+  use vv_dropin_c
+  implicit none
+  integer, intent(in) :: field
+  g_hdirty(field) = .true.
+  g_ddirty(field) = .false.
+end subroutine
+
+subroutine vv_dropin_mark_device(names, n)
+  use vv_dropin_c
+  implicit none
+  integer, intent(in) :: n
+  integer(c_int), intent(in) :: names(n)
+  integer :: j
+  do j = 1, n
+    g_ddirty(names(j)) = .true.
+    g_hdirty(names(j)) = .false.
+  end do
 end subroutine
 
 ! ---------------------------------------------------------- explicit step
@@ -202,6 +286,7 @@ subroutine vv_dropin_put_state(names, n)
   common /sol3/ uel, vel, wel, uelp, velp, welp
   integer :: j
   do j = 1, n
+    if (g_resident .and. .not. g_hdirty(names(j))) cycle
     select case (names(j))
     case (VV_C);    call vv_check(vv_put(g_run, VV_C, C, g_ld), 'put C')
     case (VV_CP);   call vv_check(vv_put(g_run, VV_CP, Cp, g_ld), 'put Cp')
@@ -211,7 +296,10 @@ subroutine vv_dropin_put_state(names, n)
     case (VV_THN);  call vv_check(vv_put(g_run, VV_THN, thn, g_ld), 'put thn')
     case (VV_UEL);  call vv_check(vv_put(g_run, VV_UEL, uel, g_ld), 'put uel')
     case (VV_VEL);  call vv_check(vv_put(g_run, VV_VEL, vel, g_ld), 'put vel')
+    case (VV_UELP); call vv_check(vv_put(g_run, VV_UELP, uelp, g_ld), 'put uelp')
+    case (VV_VELP); call vv_check(vv_put(g_run, VV_VELP, velp, g_ld), 'put velp')
     end select
+    g_hdirty(names(j)) = .false.
   end do
 end subroutine
 
@@ -231,6 +319,7 @@ subroutine vv_dropin_get_state(names, n)
   common /sol3/ uel, vel, wel, uelp, velp, welp
   integer :: j
   do j = 1, n
+    if (g_resident .and. .not. g_ddirty(names(j))) cycle
     select case (names(j))
     case (VV_C);    call vv_check(vv_get(g_run, VV_C, C, g_ld), 'get C')
     case (VV_CP);   call vv_check(vv_get(g_run, VV_CP, Cp, g_ld), 'get Cp')
@@ -238,7 +327,12 @@ subroutine vv_dropin_get_state(names, n)
     case (VV_TH);   call vv_check(vv_get(g_run, VV_TH, th, g_ld), 'get th')
     case (VV_THP);  call vv_check(vv_get(g_run, VV_THP, thp, g_ld), 'get thp')
     case (VV_THN);  call vv_check(vv_get(g_run, VV_THN, thn, g_ld), 'get thn')
+    case (VV_UEL);  call vv_check(vv_get(g_run, VV_UEL, uel, g_ld), 'get uel')
+    case (VV_VEL);  call vv_check(vv_get(g_run, VV_VEL, vel, g_ld), 'get vel')
+    case (VV_UELP); call vv_check(vv_get(g_run, VV_UELP, uelp, g_ld), 'get uelp')
+    case (VV_VELP); call vv_check(vv_get(g_run, VV_VELP, velp, g_ld), 'get velp')
     end select
+    g_ddirty(names(j)) = .false.
   end do
 end subroutine
 
@@ -258,6 +352,10 @@ subroutine vv_dropin_cvort(first)
   real(c_double), allocatable :: w(:,:,:)
   call vv_dropin_put_state([VV_C, VV_CP, VV_TH, VV_UEL, VV_VEL], 5)
   call vv_check(vv_cvorticity(g_run, first), 'Cvorticity')
+  if (g_resident) then
+    call vv_dropin_mark_device([VV_CN, VV_C], 2)
+    return
+  end if
   allocate (w(0:VV_KR,1,0:VV_KP))
   call vv_check(vv_get(g_run, VV_CN, w, g_ld), 'get Cn')
   Cn(1:nr-1,1,0:np) = w(1:nr-1,1,0:np)
@@ -285,6 +383,10 @@ subroutine vv_dropin_temp(first)
   real(c_double), allocatable :: w(:,:,:)
   call vv_dropin_put_state([VV_TH, VV_THP, VV_UEL, VV_VEL], 4)
   call vv_check(vv_temperature(g_run, first), 'temperature')
+  if (g_resident) then
+    call vv_dropin_mark_device([VV_THN, VV_TH], 2)
+    return
+  end if
   allocate (w(0:VV_KR,1,0:VV_KP))
   call vv_check(vv_get(g_run, VV_THN, w, g_ld), 'get thn')
   thn(1:nr,1,0:np) = w(1:nr,1,0:np)
@@ -319,6 +421,13 @@ subroutine vv_dropin_rhs(field, out)
   real(c_double), allocatable :: w(:,:,:)
   call vv_dropin_put_state([VV_CN], 1)
   call vv_check(vv_rhs(g_run), 'rhs')
+  if (g_resident) then
+    ! vrh / urh stay on the device; poisson2 / poisson3 find them there
+    ! when they are passed this array
+    call vv_dropin_mark_device([field], 1)
+    g_rhs_loc(field) = loc(out)
+    return
+  end if
   allocate (w(0:VV_KR,1,0:VV_KP))
   call vv_check(vv_get(g_run, field, w, g_ld), 'get rhs')
   if (field == VV_VRH) then
@@ -343,6 +452,44 @@ subroutine rhsu(urh)
   call vv_dropin_rhs(VV_URH, urh)
 end subroutine
 
+! poisson2 / poisson3: on the device-resident fields when psi, psip are the
+! COMMON fields and rhs the array rhsv / rhsu last filled, else through the
+! host arrays (device-newer COMMON fields returned first, COMMON the newer
+! copy afterwards)
+subroutine vv_dropin_solve(which, fpsi, fpsip, frhs, psi, psip, rhs, cyc, rel, st)
+  use vv_dropin_c
+  implicit none
+  integer(c_int), intent(in) :: which, fpsi, fpsip, frhs
+  real(c_double) :: psi(0:VV_KR,1,0:VV_KP), psip(0:VV_KR,1,0:VV_KP), rhs(0:VV_KR,1,0:VV_KP)
+  integer(c_int), intent(out) :: cyc(2), st
+  real(c_double), intent(out) :: rel(2)
+  real(c_double) :: uel(0:VV_KR,1,0:VV_KP), vel(0:VV_KR,1,0:VV_KP), wel(0:VV_KR,1,0:VV_KP)
+  real(c_double) :: uelp(0:VV_KR,1,0:VV_KP), velp(0:VV_KR,1,0:VV_KP), welp(0:VV_KR,1,0:VV_KP)
+  common /sol3/ uel, vel, wel, uelp, velp, welp
+  integer(c_intptr_t) :: lpsi, lpsip
+  if (which == 2) then
+    lpsi = loc(vel)
+    lpsip = loc(velp)
+  else
+    lpsi = loc(uel)
+    lpsip = loc(uelp)
+  end if
+  if (g_resident .and. loc(psi) == lpsi .and. loc(psip) == lpsip .and. loc(rhs) == g_rhs_loc(frhs) &
+      .and. g_ddirty(frhs)) then
+    call vv_dropin_put_state([fpsi, fpsip], 2)
+    st = vv_poisson(g_run, which, cyc, rel)
+    if (st >= 0) call vv_dropin_mark_device([fpsi, fpsip], 2)
+    return
+  end if
+  if (g_resident) call vv_dropin_get_state([fpsi, fpsip], 2)
+  st = vv_poisson_host(g_run, which, psi, psip, rhs, g_ld, cyc, rel)
+  if (g_resident) then
+    call vv_dropin_host_dirty(int(fpsi))
+    call vv_dropin_host_dirty(int(fpsip))
+    call vv_dropin_host_dirty(int(frhs))
+  end if
+end subroutine
+
 subroutine poisson2(psi, psip, rhs, nr, nz, np, dr, dz, dp, nlev)
   use vv_dropin_c
   implicit none
@@ -351,7 +498,7 @@ subroutine poisson2(psi, psip, rhs, nr, nz, np, dr, dz, dp, nlev)
   real(c_double) :: psi(0:VV_KR,1,0:VV_KP), psip(0:VV_KR,1,0:VV_KP), rhs(0:VV_KR,1,0:VV_KP)
   integer(c_int) :: cyc(2), st
   real(c_double) :: rel(2)
-  st = vv_poisson_host(g_run, 2_c_int, psi, psip, rhs, g_ld, cyc, rel)
+  call vv_dropin_solve(2_c_int, VV_VEL, VV_VELP, VV_VRH, psi, psip, rhs, cyc, rel, st)
   call vv_check(st, 'poisson2')
   if (st == 1) then          ! the reference stops after 500 sweeps (line 1843)
     write (*, *) 'poisson2: not converged, relative residual ', rel(1)
@@ -367,7 +514,7 @@ subroutine poisson3(psi, psip, rhs, nr, nz, np, dr, dz, dp, nlev)
   real(c_double) :: psi(0:VV_KR,1,0:VV_KP), psip(0:VV_KR,1,0:VV_KP), rhs(0:VV_KR,1,0:VV_KP)
   integer(c_int) :: cyc(2), st
   real(c_double) :: rel(2)
-  st = vv_poisson_host(g_run, 3_c_int, psi, psip, rhs, g_ld, cyc, rel)
+  call vv_dropin_solve(3_c_int, VV_UEL, VV_UELP, VV_URH, psi, psip, rhs, cyc, rel, st)
   call vv_check(st, 'poisson3')
   if (st == 1) then
     write (*, *) 'poisson3: not converged, relative residual ', rel(1)
@@ -391,6 +538,11 @@ subroutine update
   call vv_dropin_put_state([VV_C, VV_CN, VV_TH, VV_THN, VV_VEL, VV_UEL], 6)
   call vv_check(vv_update(g_run, ncl), 'update')
   if (ncl > 0) write (*, *) 'WARNING! Temp damped'
+  if (g_resident) then
+    call vv_dropin_mark_device([VV_C, VV_CP, VV_CN, VV_TH, VV_THP, VV_THN], 6)
+    call vv_dropin_get_state(g_mirror, 4)
+    return
+  end if
   call vv_dropin_get_state([VV_C, VV_CP, VV_TH, VV_THP], 4)
   ! the reference leaves the new level in Cn/thn as well (walls, clamp)
   Cn(0,1,0:np) = C(0,1,0:np)

@@ -5,13 +5,26 @@
 ! libvvmi.so) in place of the reference's own subroutines.  The routine
 ! names and argument lists are the reference's; the COMMON blocks are the
 ! reference's, dimensioned with the caller's PARAMETERs VV3_KR, VV3_KZ,
-! VV3_KP (defaults 33, 33, 160 as in rot3dmg.f:23).  Each routine copies the
-! fields it reads from COMMON (or its arguments) to the device run, calls
-! the HIP library and copies what it writes back, so the caller's view of
-! COMMON is the same as with the reference routines.  poissoninit creates
+! VV3_KP (defaults 33, 33, 160 as in rot3dmg.f:23).  poissoninit creates
 ! the device run from COMMON /num/ /val/ /radius/, with the caller's dr, dz,
 ! dp.  The three Poisson solves are exact (the reference iterates its
 ! multigrid to an absolute residual of 1e-5, tolcon at rot3dmg.f:1562).
+!
+! Residency (the default).  The device run keeps the state between calls;
+! every field carries two coherence bits, host-newer (put before its next
+! use; all fields start so) and device-newer.  A routine puts only its
+! host-newer inputs and leaves its outputs on the device; update, the last
+! routine of a step, copies back the fields the reference program writes
+! out -- A, B, C, th, uel, vel, wel (rot3dmg.f:279-319; its time loop reads
+! none, only the flux) -- so a step moves seven fields and the flux instead
+! of about ninety copies.  The solves run on the device when their
+! arguments are the COMMON fields and the array rhsw/rhsv/rhsu filled, and
+! through the host arrays otherwise.  vv3_dropin_sync returns every
+! device-newer COMMON field; vv3_dropin_host_dirty declares a field the
+! caller wrote between calls; vv3_dropin_residency(0), or
+! VV_DROPIN_RESIDENT=0 in the environment at poissoninit, selects full
+! copies of every routine's inputs and outputs (the caller's view of COMMON
+! is then the reference routines' after every call).
 !
 ! Replaced routines (rot3dmg.f): Avorticity 341, Bvorticity 477,
 ! Cvorticity 601, temperature 736, heat 882, update 964, rhsw 1026,
@@ -108,6 +121,13 @@ module vv_dropin3_c
 
   type(c_ptr), save :: g_run = c_null_ptr
   integer(c_int), parameter :: g_ldi = VV3_KR + 1, g_ldj = VV3_KZ + 1
+  ! coherence bits per field id, the residency switch, the arrays
+  ! rhsw / rhsv / rhsu last filled on the device, the fields update returns
+  logical, save :: g_resident = .true.
+  logical, save :: g_hdirty(0:20) = .true.
+  logical, save :: g_ddirty(0:20) = .false.
+  integer(c_intptr_t), save :: g_rhs_loc(0:20) = 0
+  integer(c_int), parameter :: g_mirror(7) = [F_A, F_B, F_C, F_TH, F_UEL, F_VEL, F_WEL]
 
 contains
 
@@ -174,47 +194,195 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
   cfg%model = VV3_HOR
   call vv3_check(vv3_create(cfg, 0_c_int, g_run), 'poissoninit')
   nlev = 1
+  call vv3_dropin_env_residency()
+  g_hdirty = .true.
+  g_ddirty = .false.
+  g_rhs_loc = 0
+end subroutine
+
+subroutine vv3_dropin_env_residency()
+  use vv_dropin3_c
+  implicit none
+  character(len=8) :: val
+  integer :: st
+  call get_environment_variable('VV_DROPIN_RESIDENT', val, status=st)
+  if (st == 0) g_resident = (val(1:1) /= '0')
+end subroutine
+
+! mode 1: residency (the default); 0: full copies.  Switching to copies
+! first returns every device-newer field to COMMON; switching to residency
+! takes COMMON as the newer copy of every field.
+subroutine vv3_dropin_residency(mode)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: mode
+  if (mode == 0 .and. g_resident .and. c_associated(g_run)) call vv3_dropin_sync()
+  if (mode /= 0 .and. .not. g_resident) g_hdirty = .true.
+  g_resident = (mode /= 0)
+end subroutine
+
+! every device-newer COMMON field back to the host
+subroutine vv3_dropin_sync()
+  use vv_dropin3_c
+  implicit none
+  if (.not. g_resident) return
+  call vv3_dropin_get_state([F_AN, F_A, F_AP, F_BN, F_B, F_BP, F_CN, F_C, F_CP, F_THN, F_TH, F_THP, &
+                             F_UEL, F_VEL, F_WEL, F_UELP, F_VELP, F_WELP], 18)
+end subroutine
+
+! the caller has written COMMON field `field` (an F_* id) since the last call
+subroutine vv3_dropin_host_dirty(field)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: field
+  g_hdirty(field) = .true.
+  g_ddirty(field) = .false.
+end subroutine
+
+subroutine vv3_dropin_mark_device(names, n)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: n
+  integer(c_int), intent(in) :: names(n)
+  integer :: j
+  do j = 1, n
+    g_ddirty(names(j)) = .true.
+    g_hdirty(names(j)) = .false.
+  end do
+end subroutine
+
+! COMMON fields to the device: host-newer ones when resident, all otherwise
+subroutine vv3_dropin_put_state(names, n)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: n
+  integer(c_int), intent(in) :: names(n)
+#include "vv_dropin3_state.inc"
+  integer :: j
+  do j = 1, n
+    if (g_resident .and. .not. g_hdirty(names(j))) cycle
+    select case (names(j))
+    case (F_AN);   call put(F_AN, An)
+    case (F_A);    call put(F_A, A)
+    case (F_AP);   call put(F_AP, Ap)
+    case (F_BN);   call put(F_BN, Bn)
+    case (F_B);    call put(F_B, B)
+    case (F_BP);   call put(F_BP, Bp)
+    case (F_CN);   call put(F_CN, Cn)
+    case (F_C);    call put(F_C, C)
+    case (F_CP);   call put(F_CP, Cp)
+    case (F_THN);  call put(F_THN, thn)
+    case (F_TH);   call put(F_TH, th)
+    case (F_THP);  call put(F_THP, thp)
+    case (F_UEL);  call put(F_UEL, uel)
+    case (F_VEL);  call put(F_VEL, vel)
+    case (F_WEL);  call put(F_WEL, wel)
+    case (F_UELP); call put(F_UELP, uelp)
+    case (F_VELP); call put(F_VELP, velp)
+    case (F_WELP); call put(F_WELP, welp)
+    end select
+    g_hdirty(names(j)) = .false.
+  end do
+end subroutine
+
+! COMMON fields from the device: device-newer ones when resident, all otherwise
+subroutine vv3_dropin_get_state(names, n)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: n
+  integer(c_int), intent(in) :: names(n)
+#include "vv_dropin3_state.inc"
+  integer :: j
+  do j = 1, n
+    if (g_resident .and. .not. g_ddirty(names(j))) cycle
+    select case (names(j))
+    case (F_AN);   call get(F_AN, An)
+    case (F_A);    call get(F_A, A)
+    case (F_AP);   call get(F_AP, Ap)
+    case (F_BN);   call get(F_BN, Bn)
+    case (F_B);    call get(F_B, B)
+    case (F_BP);   call get(F_BP, Bp)
+    case (F_CN);   call get(F_CN, Cn)
+    case (F_C);    call get(F_C, C)
+    case (F_CP);   call get(F_CP, Cp)
+    case (F_THN);  call get(F_THN, thn)
+    case (F_TH);   call get(F_TH, th)
+    case (F_THP);  call get(F_THP, thp)
+    case (F_UEL);  call get(F_UEL, uel)
+    case (F_VEL);  call get(F_VEL, vel)
+    case (F_WEL);  call get(F_WEL, wel)
+    case (F_UELP); call get(F_UELP, uelp)
+    case (F_VELP); call get(F_VELP, velp)
+    case (F_WELP); call get(F_WELP, welp)
+    end select
+    g_ddirty(names(j)) = .false.
+  end do
+end subroutine
+
+! a routine's outputs: left on the device when resident, copied back otherwise
+subroutine vv3_dropin_out(names, n)
+  use vv_dropin3_c
+  implicit none
+  integer, intent(in) :: n
+  integer(c_int), intent(in) :: names(n)
+  if (g_resident) then
+    call vv3_dropin_mark_device(names, n)
+  else
+    call vv3_dropin_get_state(names, n)
+  end if
+end subroutine
+
+! a right-hand side (not in COMMON): put when the host copy is newer; left
+! on the device (its array recorded for the solve) or copied back
+subroutine vv3_dropin_rhs(which, f, x)
+  use vv_dropin3_c
+  implicit none
+  integer(c_int), intent(in) :: which, f
+  real(c_double) :: x(*)
+  if (.not. g_resident .or. g_hdirty(f)) then
+    call put(f, x)
+    g_hdirty(f) = .false.
+  end if
+  call vv3_check(vv3_rhs(g_run, which), 'rhs')
+  if (g_resident) then
+    call vv3_dropin_mark_device([f], 1)
+    g_rhs_loc(f) = loc(x)
+  else
+    call get(f, x)
+  end if
 end subroutine
 
 ! ------------------------------------------------------- explicit routines
 subroutine Avorticity
   use vv_dropin3_c
   implicit none
-#include "vv_dropin3_state.inc"
-  call put(F_A, A); call put(F_B, B); call put(F_C, C); call put(F_AP, Ap); call put(F_TH, th)
-  call put(F_UEL, uel); call put(F_VEL, vel); call put(F_WEL, wel); call put(F_AN, An)
+  call vv3_dropin_put_state([F_A, F_B, F_C, F_AP, F_TH, F_UEL, F_VEL, F_WEL, F_AN], 9)
   call vv3_check(vv3_vorticity(g_run, 1_c_int), 'Avorticity')
-  call get(F_AN, An)
+  call vv3_dropin_out([F_AN], 1)
 end subroutine
 
 subroutine Bvorticity
   use vv_dropin3_c
   implicit none
-#include "vv_dropin3_state.inc"
-  call put(F_A, A); call put(F_B, B); call put(F_C, C); call put(F_BP, Bp); call put(F_TH, th)
-  call put(F_UEL, uel); call put(F_VEL, vel); call put(F_WEL, wel); call put(F_BN, Bn)
+  call vv3_dropin_put_state([F_A, F_B, F_C, F_BP, F_TH, F_UEL, F_VEL, F_WEL, F_BN], 9)
   call vv3_check(vv3_vorticity(g_run, 2_c_int), 'Bvorticity')
-  call get(F_BN, Bn)
+  call vv3_dropin_out([F_BN], 1)
 end subroutine
 
 subroutine Cvorticity
   use vv_dropin3_c
   implicit none
-#include "vv_dropin3_state.inc"
-  call put(F_A, A); call put(F_B, B); call put(F_C, C); call put(F_CP, Cp); call put(F_TH, th)
-  call put(F_UEL, uel); call put(F_VEL, vel); call put(F_WEL, wel); call put(F_CN, Cn)
+  call vv3_dropin_put_state([F_A, F_B, F_C, F_CP, F_TH, F_UEL, F_VEL, F_WEL, F_CN], 9)
   call vv3_check(vv3_vorticity(g_run, 3_c_int), 'Cvorticity')
-  call get(F_CN, Cn)
+  call vv3_dropin_out([F_CN], 1)
 end subroutine
 
 subroutine temperature
   use vv_dropin3_c
   implicit none
-#include "vv_dropin3_state.inc"
-  call put(F_TH, th); call put(F_THP, thp); call put(F_UEL, uel); call put(F_VEL, vel)
-  call put(F_WEL, wel); call put(F_THN, thn)
+  call vv3_dropin_put_state([F_TH, F_THP, F_UEL, F_VEL, F_WEL, F_THN], 6)
   call vv3_check(vv3_temperature(g_run), 'temperature')
-  call get(F_THN, thn)
+  call vv3_dropin_out([F_THN], 1)
 end subroutine
 
 #if VV3_HOR
@@ -223,8 +391,7 @@ subroutine heat(trans1, trans2)
   use vv_dropin3_c
   implicit none
   real(c_double) :: trans1, trans2
-#include "vv_dropin3_state.inc"
-  call put(F_TH, th)
+  call vv3_dropin_put_state([F_TH], 1)
   call vv3_check(vv3_heat(g_run, trans1), 'heat')
   call vv3_check(vv3_fluxes(g_run, trans1, trans2), 'heat')
 end subroutine
@@ -233,8 +400,7 @@ subroutine heat(trans1)
   use vv_dropin3_c
   implicit none
   real(c_double) :: trans1
-#include "vv_dropin3_state.inc"
-  call put(F_TH, th)
+  call vv3_dropin_put_state([F_TH], 1)
   call vv3_check(vv3_heat(g_run, trans1), 'heat')
 end subroutine
 #endif
@@ -243,30 +409,24 @@ subroutine rhsw(wrh)
   use vv_dropin3_c
   implicit none
   real(c_double) :: wrh(0:VV3_KR,0:VV3_KZ,0:VV3_KP)
-#include "vv_dropin3_state.inc"
-  call put(F_AN, An); call put(F_BN, Bn); call put(F_WRH, wrh)
-  call vv3_check(vv3_rhs(g_run, 1_c_int), 'rhsw')
-  call get(F_WRH, wrh)
+  call vv3_dropin_put_state([F_AN, F_BN], 2)
+  call vv3_dropin_rhs(1_c_int, F_WRH, wrh)
 end subroutine
 
 subroutine rhsv(vrh)
   use vv_dropin3_c
   implicit none
   real(c_double) :: vrh(0:VV3_KR,0:VV3_KZ,0:VV3_KP)
-#include "vv_dropin3_state.inc"
-  call put(F_AN, An); call put(F_CN, Cn); call put(F_VRH, vrh)
-  call vv3_check(vv3_rhs(g_run, 2_c_int), 'rhsv')
-  call get(F_VRH, vrh)
+  call vv3_dropin_put_state([F_AN, F_CN], 2)
+  call vv3_dropin_rhs(2_c_int, F_VRH, vrh)
 end subroutine
 
 subroutine rhsu(urh)
   use vv_dropin3_c
   implicit none
   real(c_double) :: urh(0:VV3_KR,0:VV3_KZ,0:VV3_KP)
-#include "vv_dropin3_state.inc"
-  call put(F_BN, Bn); call put(F_CN, Cn); call put(F_WEL, wel); call put(F_URH, urh)
-  call vv3_check(vv3_rhs(g_run, 3_c_int), 'rhsu')
-  call get(F_URH, urh)
+  call vv3_dropin_put_state([F_BN, F_CN, F_WEL], 3)
+  call vv3_dropin_rhs(3_c_int, F_URH, urh)
 end subroutine
 
 ! ------------------------------------------------------------ the solves
@@ -275,9 +435,36 @@ subroutine vv3_dropin_solve(which, fpsi, fpsip, frhs, psi, psip, rhs)
   implicit none
   integer(c_int), intent(in) :: which, fpsi, fpsip, frhs
   real(c_double) :: psi(*), psip(*), rhs(*)
+#include "vv_dropin3_state.inc"
+  integer(c_intptr_t) :: lpsi, lpsip
+  if (which == 1) then
+    lpsi = loc(wel)
+    lpsip = loc(welp)
+  else if (which == 2) then
+    lpsi = loc(vel)
+    lpsip = loc(velp)
+  else
+    lpsi = loc(uel)
+    lpsip = loc(uelp)
+  end if
+  if (g_resident .and. loc(psi) == lpsi .and. loc(psip) == lpsip .and. loc(rhs) == g_rhs_loc(frhs) &
+      .and. g_ddirty(frhs)) then
+    call vv3_dropin_put_state([fpsi, fpsip], 2)
+    call vv3_check(vv3_poisson(g_run, which), 'poisson')
+    call vv3_dropin_mark_device([fpsi, fpsip], 2)
+    return
+  end if
+  ! through the host arrays: device-newer COMMON fields returned first,
+  ! COMMON the newer copy afterwards
+  if (g_resident) call vv3_dropin_get_state([fpsi, fpsip], 2)
   call put(fpsi, psi); call put(fpsip, psip); call put(frhs, rhs)
   call vv3_check(vv3_poisson(g_run, which), 'poisson')
   call get(fpsi, psi); call get(fpsip, psip)
+  if (g_resident) then
+    call vv3_dropin_host_dirty(int(fpsi))
+    call vv3_dropin_host_dirty(int(fpsip))
+    call vv3_dropin_host_dirty(int(frhs))
+  end if
 end subroutine
 
 subroutine poisson1(psi, psip, rhs, nr, nz, np, dr, dz, dp, nlev)
@@ -308,15 +495,9 @@ end subroutine
 subroutine update
   use vv_dropin3_c
   implicit none
-#include "vv_dropin3_state.inc"
-  call put(F_AN, An); call put(F_A, A); call put(F_AP, Ap)
-  call put(F_BN, Bn); call put(F_B, B); call put(F_BP, Bp)
-  call put(F_CN, Cn); call put(F_C, C); call put(F_CP, Cp)
-  call put(F_THN, thn); call put(F_TH, th); call put(F_THP, thp)
-  call put(F_UEL, uel); call put(F_VEL, vel); call put(F_WEL, wel)
+  call vv3_dropin_put_state([F_AN, F_A, F_AP, F_BN, F_B, F_BP, F_CN, F_C, F_CP, F_THN, F_TH, F_THP, &
+                             F_UEL, F_VEL, F_WEL], 15)
   call vv3_check(vv3_update(g_run), 'update')
-  call get(F_AN, An); call get(F_A, A); call get(F_AP, Ap)
-  call get(F_BN, Bn); call get(F_B, B); call get(F_BP, Bp)
-  call get(F_CN, Cn); call get(F_C, C); call get(F_CP, Cp)
-  call get(F_THN, thn); call get(F_TH, th); call get(F_THP, thp)
+  call vv3_dropin_out([F_AN, F_A, F_AP, F_BN, F_B, F_BP, F_CN, F_C, F_CP, F_THN, F_TH, F_THP], 12)
+  if (g_resident) call vv3_dropin_get_state(g_mirror, 7)
 end subroutine

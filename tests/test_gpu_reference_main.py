@@ -118,6 +118,37 @@ def test_rot3dmghor_main_program_on_the_gpu(grid):
         assert errs[n] < 1e-4, (n, errs[n])
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "r3_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_rot3dmg_main_program_resident_and_full_copies():
+    """rot3dmg.f's main program at 32 x 32 x 160, 40 steps, through the 3D
+    drop-in with its default residency (a step returns A, B, C, th, uel,
+    vel, wel) and with full copies (VV_DROPIN_RESIDENT=0): the printed flux
+    series and every file the program writes are the same bit for bit; the
+    wall time of each run is printed."""
+    import time
+    nts = 40
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1e-5", "32", "32", "160", str(nts),
+                      "0.5 1.0 0.5", "1e5", "100.0", "0.0", "1.0", "0.0", "0.1", "4", "no", "n"]) + "\n"
+    exe = os.path.join(REF, "r3_main_gpu")
+    out, secs, files = {}, {}, {}
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d0:
+        for mode, d in (("1", d1), ("0", d0)):
+            print(f"[gpu] r3_main_gpu VV_DROPIN_RESIDENT={mode} in {d}", flush=True)
+            t = time.perf_counter()
+            res = subprocess.run([exe], input=deck, text=True, capture_output=True, cwd=d, timeout=300,
+                                 env=dict(os.environ, VV_DROPIN_RESIDENT=mode))
+            secs[mode] = time.perf_counter() - t
+            assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+            out[mode] = _series(res.stdout, 1)
+            files[mode] = {n: open(os.path.join(d, n), "rb").read() for n in ("T", "A", "B", "C", "U", "V", "W")}
+    print(f"rot3dmg main 32x32x160, {nts} steps: resident {secs['1']:.2f} s, full copies {secs['0']:.2f} s")
+    assert out["1"].shape == (nts, 1)
+    assert np.array_equal(out["1"], out["0"])
+    for n, b in files["1"].items():
+        assert b == files["0"][n], n
+
+
 def _plane_main(ref_exe, gpu_exe, deck, nts, formatted=False):
     """Runs a 2D program's executable and its main program on the GPU with
     one deck; returns the heat-series error, the field-file errors and the
@@ -196,6 +227,39 @@ def test_periodic_box_main_program_on_its_own_deck():
     for n in ("C", "U", "V"):
         assert errs[n] < 1e-5, (n, errs[n])
     assert em < 1e-5
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "rb_main_gpu")),
+                    reason="reference main programs not linked (reference sources were absent at build)")
+def test_periodic_box_main_program_resident_and_full_copies():
+    """The input.s deck (96 x 336), 1000 steps, through the drop-in with its
+    default residency (a step returns C, th, uel and vel to COMMON) and with
+    full copies of every routine's inputs and outputs (VV_DROPIN_RESIDENT=0):
+    the printed heat series and every file the program writes are the same
+    bit for bit; the wall time of each run is printed."""
+    import time
+    nts = 1000
+    deck = "\n".join(["A", "B", "C", "U", "V", "W", "T", "no", "1.d-7", "96", "336", str(nts),
+                      "0.5d0,1.0d0,0.1d0", "1.d7", "0d0", "0.d0", "0.71d0", "0.d0", "1.d0", "3", "no",
+                      "n"]) + "\n"
+    exe = os.path.join(REF, "rb_main_gpu")
+    out, secs, files = {}, {}, {}
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d0:
+        for mode, d in (("1", d1), ("0", d0)):
+            print(f"[gpu] rb_main_gpu VV_DROPIN_RESIDENT={mode} in {d}", flush=True)
+            t = time.perf_counter()
+            res = subprocess.run([exe], input=deck, text=True, capture_output=True, cwd=d, timeout=300,
+                                 env=dict(os.environ, VV_DROPIN_RESIDENT=mode))
+            secs[mode] = time.perf_counter() - t
+            assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+            out[mode] = _series(res.stdout, 2)
+            files[mode] = {n: open(os.path.join(d, n), "rb").read()
+                           for n in ("T", "A", "B", "C", "U", "V", "W", "zmtmtempvar.dat")}
+    print(f"rb main, input.s deck, {nts} steps: resident {secs['1']:.2f} s, full copies {secs['0']:.2f} s")
+    assert out["1"].shape == (nts // 50, 2)
+    assert np.array_equal(out["1"], out["0"])
+    for n, b in files["1"].items():
+        assert b == files["0"][n], n
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF, "rb_main_gpu_omp")),
