@@ -108,40 +108,72 @@ struct Rad {
     }
 };
 
-// Avorticity at (i, j, k), rot3dmg.f:359-465; M = 1: rot3dmghor.f:373-483
-template <int M>
-__device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
+// The routines read the fields through a view: fv.x<field, di, dj, dk>() is
+// the value at (i + di, j + dj, k + dk), k + dk taken periodically (km, ka).
+// GView reads it from memory; MView (below) carries the values a marching
+// column reuses from plane to plane.  Both give the same numbers, so the
+// routines' results do not depend on the view.
+constexpr int FA = 0, FB = 1, FC = 2, FTH = 3, FU = 4, FV = 5, FW = 6, FAP = 7, FBP = 8, FCP = 9, FTHP = 10;
+
+template <int FLD>
+__device__ __forceinline__ const double *fptr(const Fields &F)
 {
-    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
+    if constexpr (FLD == FA) return F.A;
+    else if constexpr (FLD == FB) return F.B;
+    else if constexpr (FLD == FC) return F.C;
+    else if constexpr (FLD == FTH) return F.th;
+    else if constexpr (FLD == FU) return F.uel;
+    else if constexpr (FLD == FV) return F.vel;
+    else if constexpr (FLD == FW) return F.wel;
+    else if constexpr (FLD == FAP) return F.Ap;
+    else if constexpr (FLD == FBP) return F.Bp;
+    else if constexpr (FLD == FCP) return F.Cp;
+    else return F.thp;
+}
+
+struct GView {
+    const Fields &F;
+    unsigned b[3];   // offset of (i, j) in the planes km, k, ka
+    int li;          // row pitch
+    template <int FLD, int DI, int DJ, int DK>
+    __device__ double x() const
+    {
+        return fptr<FLD>(F)[b[DK + 1] + (unsigned)(DJ * li + DI)];
+    }
+};
+
+#define XV(f, di, dj, dk) fv.template x<f, di, dj, dk>()
+
+// Avorticity at (i, j, k), rot3dmg.f:359-465; M = 1: rot3dmghor.f:373-483
+template <int M, class V>
+__device__ double avort(const Geo &g, const Kc &K, const V &fv, int i, int j, int k)
+{
     const int nr = g.nr, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
-    const int im = i - 1, jp = j + 1;
-    const int km = (k == 0) ? np - 1 : k - 1;
-    const int ka = (k == np) ? 1 : k + 1;
     const Rad R(g.r0 + ((double)i - 0.5) * dr);
-    const double u = (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, k) +
-                      X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.125;
-    const double v = (X(vel, i, jp, k) + X(vel, i, j, k)) * 0.5;
-    const double w = (X(wel, i, j, ka) + X(wel, i, j, k)) * 0.5;
-    const double tem = (X(th, i, j, k) + X(th, i, j, ka) + X(th, i, jp, k) + X(th, i, jp, ka)) * 0.25;
-    const double BB = (X(B, i, j, k) + X(B, im, j, k) + X(B, i, j, ka) + X(B, im, j, ka)) * 0.25;
-    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, im, j, k) + X(C, im, jp, k)) * 0.25;
-    const double Ac = X(A, i, j, k);
+    const double u = (XV(FU, 0, 0, 0) + XV(FU, 0, 1, 0) + XV(FU, 0, 0, 1) + XV(FU, 0, 1, 1) + XV(FU, -1, 0, 0) +
+                      XV(FU, -1, 1, 0) + XV(FU, -1, 0, 1) + XV(FU, -1, 1, 1)) * 0.125;
+    const double v = (XV(FV, 0, 1, 0) + XV(FV, 0, 0, 0)) * 0.5;
+    const double w = (XV(FW, 0, 0, 1) + XV(FW, 0, 0, 0)) * 0.5;
+    const double tem = (XV(FTH, 0, 0, 0) + XV(FTH, 0, 0, 1) + XV(FTH, 0, 1, 0) + XV(FTH, 0, 1, 1)) * 0.25;
+    const double BB = (XV(FB, 0, 0, 0) + XV(FB, -1, 0, 0) + XV(FB, 0, 0, 1) + XV(FB, -1, 0, 1)) * 0.25;
+    const double CC = (XV(FC, 0, 0, 0) + XV(FC, 0, 1, 0) + XV(FC, -1, 0, 0) + XV(FC, -1, 1, 0)) * 0.25;
+    const double Ac = XV(FA, 0, 0, 0);
     double Arra, Ara, Ar, cnr;
     if (i == 1) {
-        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
-        Arra = dv(3.2 * Am + 2.0 * A1 - 0.2 * X(A, i + 2, j, k), K.dr2);
+        const double Am = XV(FA, -1, 0, 0), A1 = XV(FA, 1, 0, 0);
+        Arra = dv(3.2 * Am + 2.0 * A1 - 0.2 * XV(FA, 2, 0, 0), K.dr2);
         Ara = dv(dv(-4.0 * Am, K.three) + dv(A1, K.three), K.dr);
         Ar = dv(dv(-4.0 * Am, K.three) + Ac + dv(A1, K.three), K.dr);
         cnr = K.cnr_w + dv(K.hdr, R.r);
     } else if (i == nr) {
-        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
-        Arra = dv(3.2 * A1 + 2.0 * Am - 0.2 * X(A, i - 2, j, k), K.dr2);
+        const double Am = XV(FA, -1, 0, 0), A1 = XV(FA, 1, 0, 0);
+        Arra = dv(3.2 * A1 + 2.0 * Am - 0.2 * XV(FA, -2, 0, 0), K.dr2);
         Ara = dv(dv(-Am, K.three) + dv(4.0 * A1, K.three), K.dr);
         Ar = dv(dv(-4.0 * Am, K.three) + Ac + dv(A1, K.three), K.dr);
         cnr = K.cnr_w - dv(K.hdr, R.r);
     } else {
-        const double Am = X(A, i - 1, j, k), A1 = X(A, i + 1, j, k);
+        const double Am = XV(FA, -1, 0, 0), A1 = XV(FA, 1, 0, 0);
         Arra = dv(Am + A1, K.dr2);
         Ara = dv((A1 - Am) * 0.5, K.dr);
         Ar = Ara;
@@ -151,27 +183,27 @@ __device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j
             else if (u < 0) Ar = dv(A1 - Ac, K.dr);
         }
     }
-    const double uelr = dv((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
-                           (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25, K.dr);
-    const double Azza = dv(X(A, i, j - 1, k) + X(A, i, j + 1, k), K.dz2);
-    double Az = dv((X(A, i, j + 1, k) - X(A, i, j - 1, k)) * 0.5, K.dz);
+    const double uelr = dv((XV(FU, 0, 0, 0) + XV(FU, 0, 1, 0) + XV(FU, 0, 0, 1) + XV(FU, 0, 1, 1)) * 0.25 -
+                           (XV(FU, -1, 0, 0) + XV(FU, -1, 1, 0) + XV(FU, -1, 0, 1) + XV(FU, -1, 1, 1)) * 0.25, K.dr);
+    const double Azza = dv(XV(FA, 0, -1, 0) + XV(FA, 0, 1, 0), K.dz2);
+    double Az = dv((XV(FA, 0, 1, 0) - XV(FA, 0, -1, 0)) * 0.5, K.dz);
     if (fabs(dv(w * dz, K.ptl)) > 2) {
-        if (w > 0) Az = dv(Ac - X(A, i, j - 1, k), K.dz);
-        else if (w < 0) Az = dv(X(A, i, j + 1, k) - Ac, K.dz);
+        if (w > 0) Az = dv(Ac - XV(FA, 0, -1, 0), K.dz);
+        else if (w < 0) Az = dv(XV(FA, 0, 1, 0) - Ac, K.dz);
     }
-    const double uelz = dv((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
-                           (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25, K.dz);
-    const double thz = dv((X(th, i, jp, k) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5, K.dz);
-    const double thh = dv((X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5, K.dp);
-    const double Ahha = dv(X(A, i, j, ka) + X(A, i, j, km), K.dp2);
-    double Ah = dv((X(A, i, j, ka) - X(A, i, j, km)) * 0.5, K.dp);
+    const double uelz = dv((XV(FU, 0, 1, 0) + XV(FU, 0, 1, 1) + XV(FU, -1, 1, 0) + XV(FU, -1, 1, 1)) * 0.25 -
+                           (XV(FU, 0, 0, 0) + XV(FU, 0, 0, 1) + XV(FU, -1, 0, 0) + XV(FU, -1, 0, 1)) * 0.25, K.dz);
+    const double thz = dv((XV(FTH, 0, 1, 0) + XV(FTH, 0, 1, 1) - XV(FTH, 0, 0, 0) - XV(FTH, 0, 0, 1)) * 0.5, K.dz);
+    const double thh = dv((XV(FTH, 0, 0, 1) + XV(FTH, 0, 1, 1) - XV(FTH, 0, 0, 0) - XV(FTH, 0, 1, 0)) * 0.5, K.dp);
+    const double Ahha = dv(XV(FA, 0, 0, 1) + XV(FA, 0, 0, -1), K.dp2);
+    double Ah = dv((XV(FA, 0, 0, 1) - XV(FA, 0, 0, -1)) * 0.5, K.dp);
     if (fabs(dv(v * dp * R.r.v, K.ptl)) > 2) {
-        if (v > 0) Ah = dv(Ac - X(A, i, j, km), K.dp);
-        else if (v < 0) Ah = dv(X(A, i, j, ka) - Ac, K.dp);
+        if (v > 0) Ah = dv(Ac - XV(FA, 0, 0, -1), K.dp);
+        else if (v < 0) Ah = dv(XV(FA, 0, 0, 1) - Ac, K.dp);
     }
-    const double Bh = dv((X(B, i, j, ka) + X(B, im, j, ka) - X(B, i, j, k) - X(B, im, j, k)) * 0.5, K.dp);
-    const double uelh = dv((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
-                           (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25, K.dp);
+    const double Bh = dv((XV(FB, 0, 0, 1) + XV(FB, -1, 0, 1) - XV(FB, 0, 0, 0) - XV(FB, -1, 0, 0)) * 0.5, K.dp);
+    const double uelh = dv((XV(FU, 0, 0, 1) + XV(FU, 0, 1, 1) + XV(FU, -1, 0, 1) + XV(FU, -1, 1, 1)) * 0.25 -
+                           (XV(FU, 0, 0, 0) + XV(FU, 0, 1, 0) + XV(FU, -1, 0, 0) + XV(FU, -1, 1, 0)) * 0.25, K.dp);
     const double cnz = K.cnz_int;
     const double cnh = dv(K.m1dp2, R.r2);
     const double RHS1 = Arra + dv(Ara, R.r) + dv(Ahha, R.r2) + Azza - dv(2.0 * Bh, R.r2) + dv(Ac, K.ptldt);
@@ -181,57 +213,53 @@ __device__ double avort(const Geo &g, const Kc &K, const Fields &F, int i, int j
     else RHS2 = -2.0 * Re * uelz + dv(dvs(2.0 * Ra * (tem * uelz + thz * u), K.re), K.ptl) - dv(K.grptl, R.r) * thh;
     const double cf1 = K.ptldt_inv - dv(uelr, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
     const double cf2 = dv(uelr, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
-    return (cf2 * X(F.Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+    return (cf2 * XV(FAP, 0, 0, 0) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // Bvorticity at (i, j, k), rot3dmg.f:495-590; M = 1: rot3dmghor.f:514-613
-template <int M>
-__device__ double bvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
+template <int M, class V>
+__device__ double bvort(const Geo &g, const Kc &K, const V &fv, int i, int j, int k)
 {
-    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
-    const int ip = i + 1, jp = j + 1;
-    const int km = (k == 0) ? np - 1 : k - 1;
-    const int ka = (k == np) ? 1 : k + 1;
     const Rad R(g.r0 + (double)i * dr);
     const double r = R.r.v;
-    const double u = (X(uel, i, j, k) + X(uel, i, jp, k)) * 0.5;
-    const double v = (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k) + X(vel, ip, j, km) +
-                      X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.125;
-    const double w = (X(wel, i, j, k) + X(wel, ip, j, k)) * 0.5;
-    const double tem = (X(th, i, jp, k) + X(th, ip, jp, k) + X(th, i, j, k) + X(th, ip, j, k)) * 0.25;
-    const double AA = (X(A, i, j, k) + X(A, ip, j, k) + X(A, i, j, km) + X(A, ip, j, km)) * 0.25;
-    const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, i, j, km) + X(C, i, jp, km)) * 0.25;
-    const double Bc = X(B, i, j, k);
-    const double Brra = dv(X(B, i - 1, j, k) + X(B, i + 1, j, k), K.dr2);
-    const double Bra = dv((X(B, i + 1, j, k) - X(B, i - 1, j, k)) * 0.5, K.dr);
+    const double u = (XV(FU, 0, 0, 0) + XV(FU, 0, 1, 0)) * 0.5;
+    const double v = (XV(FV, 0, 0, -1) + XV(FV, 0, 1, -1) + XV(FV, 0, 0, 0) + XV(FV, 0, 1, 0) + XV(FV, 1, 0, -1) +
+                      XV(FV, 1, 1, -1) + XV(FV, 1, 0, 0) + XV(FV, 1, 1, 0)) * 0.125;
+    const double w = (XV(FW, 0, 0, 0) + XV(FW, 1, 0, 0)) * 0.5;
+    const double tem = (XV(FTH, 0, 1, 0) + XV(FTH, 1, 1, 0) + XV(FTH, 0, 0, 0) + XV(FTH, 1, 0, 0)) * 0.25;
+    const double AA = (XV(FA, 0, 0, 0) + XV(FA, 1, 0, 0) + XV(FA, 0, 0, -1) + XV(FA, 1, 0, -1)) * 0.25;
+    const double CC = (XV(FC, 0, 0, 0) + XV(FC, 0, 1, 0) + XV(FC, 0, 0, -1) + XV(FC, 0, 1, -1)) * 0.25;
+    const double Bc = XV(FB, 0, 0, 0);
+    const double Brra = dv(XV(FB, -1, 0, 0) + XV(FB, 1, 0, 0), K.dr2);
+    const double Bra = dv((XV(FB, 1, 0, 0) - XV(FB, -1, 0, 0)) * 0.5, K.dr);
     double Br = Bra;
     if (fabs(dv(u * dr, K.ptl)) > 2) {
-        if (u > 0) Br = dv(Bc - X(B, i - 1, j, k), K.dr);
-        else if (u < 0) Br = dv(X(B, i + 1, j, k) - Bc, K.dr);
+        if (u > 0) Br = dv(Bc - XV(FB, -1, 0, 0), K.dr);
+        else if (u < 0) Br = dv(XV(FB, 1, 0, 0) - Bc, K.dr);
     }
-    const double velr = dv((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
-                           (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25, K.dr);
-    const double Bzza = dv(X(B, i, j - 1, k) + X(B, i, j + 1, k), K.dz2);
-    double Bz = dv((X(B, i, j + 1, k) - X(B, i, j - 1, k)) * 0.5, K.dz);
+    const double velr = dv((XV(FV, 1, 0, -1) + XV(FV, 1, 1, -1) + XV(FV, 1, 0, 0) + XV(FV, 1, 1, 0)) * 0.25 -
+                           (XV(FV, 0, 0, -1) + XV(FV, 0, 1, -1) + XV(FV, 0, 0, 0) + XV(FV, 0, 1, 0)) * 0.25, K.dr);
+    const double Bzza = dv(XV(FB, 0, -1, 0) + XV(FB, 0, 1, 0), K.dz2);
+    double Bz = dv((XV(FB, 0, 1, 0) - XV(FB, 0, -1, 0)) * 0.5, K.dz);
     if (fabs(dv(w * dz, K.ptl)) > 2) {
-        if (w > 0) Bz = dv(Bc - X(B, i, j - 1, k), K.dz);
-        else if (w < 0) Bz = dv(X(B, i, j + 1, k) - Bc, K.dz);
+        if (w > 0) Bz = dv(Bc - XV(FB, 0, -1, 0), K.dz);
+        else if (w < 0) Bz = dv(XV(FB, 0, 1, 0) - Bc, K.dz);
     }
-    const double velz = dv((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                           (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25, K.dz);
-    const double thz = dv((X(th, i, jp, k) + X(th, ip, jp, k) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5, K.dz);
-    const double thr = dv((X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5, K.dr);
-    const double Bhha = dv(X(B, i, j, ka) + X(B, i, j, km), K.dp2);
-    double Bh = dv((X(B, i, j, ka) - X(B, i, j, km)) * 0.5, K.dp);
+    const double velz = dv((XV(FV, 0, 1, -1) + XV(FV, 1, 1, -1) + XV(FV, 0, 1, 0) + XV(FV, 1, 1, 0)) * 0.25 -
+                           (XV(FV, 0, 0, -1) + XV(FV, 1, 0, -1) + XV(FV, 0, 0, 0) + XV(FV, 1, 0, 0)) * 0.25, K.dz);
+    const double thz = dv((XV(FTH, 0, 1, 0) + XV(FTH, 1, 1, 0) - XV(FTH, 0, 0, 0) - XV(FTH, 1, 0, 0)) * 0.5, K.dz);
+    const double thr = dv((XV(FTH, 1, 1, 0) + XV(FTH, 1, 0, 0) - XV(FTH, 0, 1, 0) - XV(FTH, 0, 0, 0)) * 0.5, K.dr);
+    const double Bhha = dv(XV(FB, 0, 0, 1) + XV(FB, 0, 0, -1), K.dp2);
+    double Bh = dv((XV(FB, 0, 0, 1) - XV(FB, 0, 0, -1)) * 0.5, K.dp);
     if (fabs(dv(v * dp * r, K.ptl)) > 2) {
-        if (v > 0) Bh = dv(Bc - X(B, i, j, km), K.dp);
-        else if (v < 0) Bh = dv(X(B, i, j, ka) - Bc, K.dp);
+        if (v > 0) Bh = dv(Bc - XV(FB, 0, 0, -1), K.dp);
+        else if (v < 0) Bh = dv(XV(FB, 0, 0, 1) - Bc, K.dp);
     }
-    const double Ah = dv((X(A, i, j, k) + X(A, ip, j, k) - X(A, i, j, km) - X(A, ip, j, km)) * 0.5, K.dp);
-    const double velh = dv((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
-                           (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25, K.dp);
+    const double Ah = dv((XV(FA, 0, 0, 0) + XV(FA, 1, 0, 0) - XV(FA, 0, 0, -1) - XV(FA, 1, 0, -1)) * 0.5, K.dp);
+    const double velh = dv((XV(FV, 0, 0, 0) + XV(FV, 1, 0, 0) + XV(FV, 0, 1, 0) + XV(FV, 1, 1, 0)) * 0.25 -
+                           (XV(FV, 0, 0, -1) + XV(FV, 1, 0, -1) + XV(FV, 0, 1, -1) + XV(FV, 1, 1, -1)) * 0.25, K.dp);
     const double cnr = K.cnr_int;
     const double cnz = K.cnz_int;
     const double cnh = dv(K.m1dp2, R.r2);
@@ -243,52 +271,48 @@ __device__ double bvort(const Geo &g, const Kc &K, const Fields &F, int i, int j
     const double uvh = dv(u, R.r) + dv(velh, R.r);
     const double cf1 = K.ptldt_inv - dv(uvh, K.ptl2) + dv(1.0, R.r2h) - cnr - cnz - cnh;
     const double cf2 = dv(uvh, K.ptl2) - dv(1.0, R.r2h) + cnr + cnz + cnh;
-    return (cf2 * X(F.Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+    return (cf2 * XV(FBP, 0, 0, 0) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // Cvorticity at (i, j, k), rot3dmg.f:619-725; M = 1: rot3dmghor.f:643-754
-template <int M>
-__device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
+template <int M, class V>
+__device__ double cvort(const Geo &g, const Kc &K, const V &fv, int i, int j, int k)
 {
-    const double *A = F.A, *B = F.B, *C = F.C, *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, Ra = g.Ra, Re = g.Re;
-    const int ip = i + 1, jm = j - 1;
-    const int km = (k == 0) ? np - 1 : k - 1;
-    const int ka = (k == np) ? 1 : k + 1;
     const Rad R(g.r0 + (double)i * dr);
     const double r = R.r.v;
-    const double u = (X(uel, i, j, ka) + X(uel, i, j, k)) * 0.5;
-    const double v = (X(vel, i, j, k) + X(vel, ip, j, k)) * 0.5;
-    const double w = (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, k) +
-                      X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.125;
-    const double AA = (X(A, i, j, k) + X(A, i, jm, k) + X(A, ip, j, k) + X(A, ip, jm, k)) * 0.25;
-    const double BB = (X(B, i, j, k) + X(B, i, jm, k) + X(B, i, j, ka) + X(B, i, jm, ka)) * 0.25;
-    const double tem = (X(th, i, j, k) + X(th, ip, j, k) + X(th, ip, j, ka) + X(th, i, j, ka)) * 0.25;
-    const double Cc = X(C, i, j, k);
-    const double Crra = dv(X(C, i - 1, j, k) + X(C, i + 1, j, k), K.dr2);
-    const double Cra = dv((X(C, i + 1, j, k) - X(C, i - 1, j, k)) * 0.5, K.dr);
+    const double u = (XV(FU, 0, 0, 1) + XV(FU, 0, 0, 0)) * 0.5;
+    const double v = (XV(FV, 0, 0, 0) + XV(FV, 1, 0, 0)) * 0.5;
+    const double w = (XV(FW, 0, 0, 0) + XV(FW, 1, 0, 0) + XV(FW, 0, 0, 1) + XV(FW, 1, 0, 1) + XV(FW, 0, -1, 0) +
+                      XV(FW, 1, -1, 0) + XV(FW, 0, -1, 1) + XV(FW, 1, -1, 1)) * 0.125;
+    const double AA = (XV(FA, 0, 0, 0) + XV(FA, 0, -1, 0) + XV(FA, 1, 0, 0) + XV(FA, 1, -1, 0)) * 0.25;
+    const double BB = (XV(FB, 0, 0, 0) + XV(FB, 0, -1, 0) + XV(FB, 0, 0, 1) + XV(FB, 0, -1, 1)) * 0.25;
+    const double tem = (XV(FTH, 0, 0, 0) + XV(FTH, 1, 0, 0) + XV(FTH, 1, 0, 1) + XV(FTH, 0, 0, 1)) * 0.25;
+    const double Cc = XV(FC, 0, 0, 0);
+    const double Crra = dv(XV(FC, -1, 0, 0) + XV(FC, 1, 0, 0), K.dr2);
+    const double Cra = dv((XV(FC, 1, 0, 0) - XV(FC, -1, 0, 0)) * 0.5, K.dr);
     double Cr = Cra;
     if (fabs(dv(u * dr, K.ptl)) > 2) {
-        if (u > 0) Cr = dv(Cc - X(C, i - 1, j, k), K.dr);
-        else if (u < 0) Cr = dv(X(C, i + 1, j, k) - Cc, K.dr);
+        if (u > 0) Cr = dv(Cc - XV(FC, -1, 0, 0), K.dr);
+        else if (u < 0) Cr = dv(XV(FC, 1, 0, 0) - Cc, K.dr);
     }
-    const double welr = dv((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
-                           (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25, K.dr);
-    const double thr = dv((X(th, ip, j, k) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5, K.dr);
+    const double welr = dv((XV(FW, 1, 0, 0) + XV(FW, 1, 0, 1) + XV(FW, 1, -1, 0) + XV(FW, 1, -1, 1)) * 0.25 -
+                           (XV(FW, 0, 0, 0) + XV(FW, 0, 0, 1) + XV(FW, 0, -1, 0) + XV(FW, 0, -1, 1)) * 0.25, K.dr);
+    const double thr = dv((XV(FTH, 1, 0, 0) + XV(FTH, 1, 0, 1) - XV(FTH, 0, 0, 0) - XV(FTH, 0, 0, 1)) * 0.5, K.dr);
     double Czza, Cz, cnz;
     if (j == 1) {
-        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
-        Czza = dv(3.2 * Cm + 2.0 * C1 - 0.2 * X(C, i, j + 2, k), K.dz2);
+        const double Cm = XV(FC, 0, -1, 0), C1 = XV(FC, 0, 1, 0);
+        Czza = dv(3.2 * Cm + 2.0 * C1 - 0.2 * XV(FC, 0, 2, 0), K.dz2);
         Cz = dv(dv(-4.0 * Cm, K.three) + Cc + dv(C1, K.three), K.dz);
         cnz = K.cnz_w;
     } else if (j == nz) {
-        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
-        Czza = dv(3.2 * C1 + 2.0 * Cm - 0.2 * X(C, i, j - 2, k), K.dz2);
+        const double Cm = XV(FC, 0, -1, 0), C1 = XV(FC, 0, 1, 0);
+        Czza = dv(3.2 * C1 + 2.0 * Cm - 0.2 * XV(FC, 0, -2, 0), K.dz2);
         Cz = dv(dv(-Cm, K.three) - Cc + dv(4.0 * C1, K.three), K.dz);
         cnz = K.cnz_w;
     } else {
-        const double Cm = X(C, i, j - 1, k), C1 = X(C, i, j + 1, k);
+        const double Cm = XV(FC, 0, -1, 0), C1 = XV(FC, 0, 1, 0);
         Czza = dv(Cm + C1, K.dz2);
         Cz = dv((C1 - Cm) * 0.5, K.dz);
         if (fabs(dv(w * dz, K.ptl)) > 2) {
@@ -297,17 +321,17 @@ __device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j
         }
         cnz = K.cnz_int;
     }
-    const double welz = dv((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
-                           (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25, K.dz);
-    const double Chha = dv(X(C, i, j, ka) + X(C, i, j, km), K.dp2);
-    double Ch = dv((X(C, i, j, ka) - X(C, i, j, km)) * 0.5, K.dp);
+    const double welz = dv((XV(FW, 0, 0, 0) + XV(FW, 1, 0, 0) + XV(FW, 0, 0, 1) + XV(FW, 1, 0, 1)) * 0.25 -
+                           (XV(FW, 0, -1, 0) + XV(FW, 1, -1, 0) + XV(FW, 0, -1, 1) + XV(FW, 1, -1, 1)) * 0.25, K.dz);
+    const double Chha = dv(XV(FC, 0, 0, 1) + XV(FC, 0, 0, -1), K.dp2);
+    double Ch = dv((XV(FC, 0, 0, 1) - XV(FC, 0, 0, -1)) * 0.5, K.dp);
     if (fabs(dv(v * dp * r, K.ptl)) > 2) {
-        if (v > 0) Ch = dv(Cc - X(C, i, j, km), K.dp);
-        else if (v < 0) Ch = dv(X(C, i, j, ka) - Cc, K.dp);
+        if (v > 0) Ch = dv(Cc - XV(FC, 0, 0, -1), K.dp);
+        else if (v < 0) Ch = dv(XV(FC, 0, 0, 1) - Cc, K.dp);
     }
-    const double welh = dv((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
-                           (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25, K.dp);
-    const double thh = dv((X(th, i, j, ka) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5, K.dp);
+    const double welh = dv((XV(FW, 0, 0, 1) + XV(FW, 1, 0, 1) + XV(FW, 0, -1, 1) + XV(FW, 1, -1, 1)) * 0.25 -
+                           (XV(FW, 0, 0, 0) + XV(FW, 1, 0, 0) + XV(FW, 0, -1, 0) + XV(FW, 1, -1, 0)) * 0.25, K.dp);
+    const double thh = dv((XV(FTH, 0, 0, 1) + XV(FTH, 1, 0, 1) - XV(FTH, 0, 0, 0) - XV(FTH, 1, 0, 0)) * 0.5, K.dp);
     const double cnr = K.cnr_int;
     const double cnh = dv(K.m1dp2, R.r2);
     const double RHS1 = Crra + dv(Cra, R.r) + dv(Chha, R.r2) + Czza + dv(Cc, K.ptldt);
@@ -319,18 +343,15 @@ __device__ double cvort(const Geo &g, const Kc &K, const Fields &F, int i, int j
         RHS2 = -2.0 * Re * welz - Ra * thh + dv(dvs(2.0 * Ra * (tem * welz - thr * u - dv(thh * v, R.r)), K.re), K.ptl);
     const double cf1 = K.ptldt_inv - dv(welz, K.ptl2) - cnr - cnz - cnh;
     const double cf2 = dv(welz, K.ptl2) + cnr + cnz + cnh;
-    return (cf2 * X(F.Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
+    return (cf2 * XV(FCP, 0, 0, 0) - ULT - RHS2 + RHS1) / cf1;
 }
 
 // temperature at (i, j, k), rot3dmg.f:754-871
-__device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int j, int k)
+template <class V>
+__device__ double temper(const Geo &g, const Kc &K, const V &fv, int i, int j, int k)
 {
-    const double *th = F.th, *uel = F.uel, *vel = F.vel, *wel = F.wel;
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp;
-    const int im = i - 1, jm = j - 1;
-    const int km = (k == 0) ? np - 1 : k - 1;
-    const int ka = (k == np) ? 1 : k + 1;
     const Rad R(g.r0 + ((double)i - 0.5) * dr);
     const double r = R.r.v;
     double u, v, w;
@@ -339,26 +360,26 @@ __device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int 
         v = 0.0;
         w = 0.0;
     } else {
-        u = (X(uel, i, j, k) + X(uel, im, j, k)) * 0.5;
-        v = (X(vel, i, j, k) + X(vel, i, j, km)) * 0.5;
-        w = (X(wel, i, j, k) + X(wel, i, jm, k)) * 0.5;
+        u = (XV(FU, 0, 0, 0) + XV(FU, -1, 0, 0)) * 0.5;
+        v = (XV(FV, 0, 0, 0) + XV(FV, 0, 0, -1)) * 0.5;
+        w = (XV(FW, 0, 0, 0) + XV(FW, 0, -1, 0)) * 0.5;
     }
-    const double Tc = X(th, i, j, k);
+    const double Tc = XV(FTH, 0, 0, 0);
     double thrra, thra, thr, cnr;
     if (i == 1) {
-        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
-        thrra = dv(3.2 * Tm + 2.0 * T1 - 0.2 * X(th, i + 2, j, k), K.dr2);
+        const double Tm = XV(FTH, -1, 0, 0), T1 = XV(FTH, 1, 0, 0);
+        thrra = dv(3.2 * Tm + 2.0 * T1 - 0.2 * XV(FTH, 2, 0, 0), K.dr2);
         thra = dv(dv(-4.0 * Tm, K.three) + dv(T1, K.three), K.dr);
         thr = dv(dv(-4.0 * Tm, K.three) + Tc + dv(T1, K.three), K.dr);
         cnr = K.cnr_w + dv(K.hdr, R.r);
     } else if (i == nr) {
-        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
-        thrra = dv(3.2 * T1 + 2.0 * Tm - 0.2 * X(th, i - 2, j, k), K.dr2);
+        const double Tm = XV(FTH, -1, 0, 0), T1 = XV(FTH, 1, 0, 0);
+        thrra = dv(3.2 * T1 + 2.0 * Tm - 0.2 * XV(FTH, -2, 0, 0), K.dr2);
         thra = dv(dv(-Tm, K.three) + dv(4.0 * T1, K.three), K.dr);
         thr = dv(dv(-Tm, K.three) - Tc + dv(4.0 * T1, K.three), K.dr);
         cnr = K.cnr_w - dv(K.hdr, R.r);
     } else {
-        const double Tm = X(th, i - 1, j, k), T1 = X(th, i + 1, j, k);
+        const double Tm = XV(FTH, -1, 0, 0), T1 = XV(FTH, 1, 0, 0);
         thrra = dv(Tm + T1, K.dr2);
         thra = dv((T1 - Tm) * 0.5, K.dr);
         thr = thra;
@@ -370,25 +391,25 @@ __device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int 
     }
     double thzza, thz, cnz;
     if (j == 1) {
-        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
-        thzza = dv(3.2 * Tm + 2.0 * T1 - 0.2 * X(th, i, j + 2, k), K.dz2);
+        const double Tm = XV(FTH, 0, -1, 0), T1 = XV(FTH, 0, 1, 0);
+        thzza = dv(3.2 * Tm + 2.0 * T1 - 0.2 * XV(FTH, 0, 2, 0), K.dz2);
         thz = dv(dv(-4.0 * Tm, K.three) + Tc + dv(T1, K.three), K.dz);
         cnz = K.cnz_w;
     } else if (j == nz) {
-        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
-        thzza = dv(3.2 * T1 + 2.0 * Tm - 0.2 * X(th, i, j - 2, k), K.dz2);
+        const double Tm = XV(FTH, 0, -1, 0), T1 = XV(FTH, 0, 1, 0);
+        thzza = dv(3.2 * T1 + 2.0 * Tm - 0.2 * XV(FTH, 0, -2, 0), K.dz2);
         thz = dv(dv(-Tm, K.three) - Tc + dv(4.0 * T1, K.three), K.dz);
         cnz = K.cnz_w;
     } else if (j == 0) {
-        thzza = dv(8.0 * X(th, i, j + 1, k), K.dz2);
+        thzza = dv(8.0 * XV(FTH, 0, 1, 0), K.dz2);
         thz = 0.0;
         cnz = K.cnz_t;
     } else if (j == nz + 1) {
-        thzza = dv(8.0 * X(th, i, j - 1, k), K.dz2);
+        thzza = dv(8.0 * XV(FTH, 0, -1, 0), K.dz2);
         thz = 0.0;
         cnz = K.cnz_t;
     } else {
-        const double Tm = X(th, i, j - 1, k), T1 = X(th, i, j + 1, k);
+        const double Tm = XV(FTH, 0, -1, 0), T1 = XV(FTH, 0, 1, 0);
         thzza = dv(Tm + T1, K.dz2);
         thz = dv((T1 - Tm) * 0.5, K.dz);
         if (fabs(dv(w * dz, K.ptl)) > 2) {
@@ -397,43 +418,223 @@ __device__ double temper(const Geo &g, const Kc &K, const Fields &F, int i, int 
         }
         cnz = K.cnz_int;
     }
-    const double thhha = dv(X(th, i, j, ka) + X(th, i, j, km), K.dp2);
-    double thh = dv((X(th, i, j, ka) - X(th, i, j, km)) * 0.5, K.dp);
+    const double thhha = dv(XV(FTH, 0, 0, 1) + XV(FTH, 0, 0, -1), K.dp2);
+    double thh = dv((XV(FTH, 0, 0, 1) - XV(FTH, 0, 0, -1)) * 0.5, K.dp);
     if (fabs(dv(v * dp * r, K.ptl)) > 2) {
-        if (v > 0) thh = dv(Tc - X(th, i, j, km), K.dp);
-        else if (v < 0) thh = dv(X(th, i, j, ka) - Tc, K.dp);
+        if (v > 0) thh = dv(Tc - XV(FTH, 0, 0, -1), K.dp);
+        else if (v < 0) thh = dv(XV(FTH, 0, 0, 1) - Tc, K.dp);
     }
     const double cnp = dv(K.m1dp2, R.r2);
     const double RHS1 = thrra + dv(thra, R.r) + dv(thhha, R.r2) + thzza + dv(Tc, K.dt);
     const double ULT = u * thr + dv(v * thh, R.r) + w * thz;
     const double cf1 = K.dt_inv - cnr - cnz - cnp;
     const double cf2 = cnr + cnz + cnp;
-    return (cf2 * X(F.thp, i, j, k) - ULT + RHS1) / cf1;
+    return (cf2 * XV(FTHP, 0, 0, 0) - ULT + RHS1) / cf1;
+}
+
+// The vorticity routines' reads while a column marches in k (MView): the
+// values a column needs at several planes -- its node and the neighbours the
+// staggered averages take across k -- stay in registers (K: plane k, M: its
+// km, N: its ka), each loaded once per plane; the others are read at plane
+// k.  A member's name gives the field and the offsets in i and j (m1 = -1).
+// The set is that of the reads of avort, bvort and cvort.
+struct MView {
+    const Fields &F;
+    unsigned o, li, bk = 0;   // (i, j) within a plane; row pitch; plane k
+    double a00M, a00K, a00N, a10M, a10K, bm10K, bm10N, b0m1K;
+    double b0m1N, b00M, b00K, b00N, c00M, c00K, c00N, c01M;
+    double c01K, th00K, th00N, th01K, th01N, th10K, th10N, um10K;
+    double um10N, um11K, um11N, u00K, u00N, u01K, u01N, v00M;
+    double v00K, v01M, v01K, v10M, v10K, v11M, v11K, w0m1K;
+    double w0m1N, w00K, w00N, w1m1K, w1m1N, w10K, w10N;
+    __device__ MView(const Fields &F_, unsigned o_, unsigned li_) : F(F_), o(o_), li(li_) {}
+    template <int FLD, int DI, int DJ, int DK>
+    __device__ double x() const
+    {
+        if constexpr (FLD == FA && DI == 0 && DJ == 0)
+            return DK == -1 ? a00M : DK == 0 ? a00K : a00N;
+        else if constexpr (FLD == FA && DI == 1 && DJ == 0)
+            return DK == -1 ? a10M : a10K;
+        else if constexpr (FLD == FB && DI == -1 && DJ == 0)
+            return DK == 0 ? bm10K : bm10N;
+        else if constexpr (FLD == FB && DI == 0 && DJ == -1)
+            return DK == 0 ? b0m1K : b0m1N;
+        else if constexpr (FLD == FB && DI == 0 && DJ == 0)
+            return DK == -1 ? b00M : DK == 0 ? b00K : b00N;
+        else if constexpr (FLD == FC && DI == 0 && DJ == 0)
+            return DK == -1 ? c00M : DK == 0 ? c00K : c00N;
+        else if constexpr (FLD == FC && DI == 0 && DJ == 1)
+            return DK == -1 ? c01M : c01K;
+        else if constexpr (FLD == FTH && DI == 0 && DJ == 0)
+            return DK == 0 ? th00K : th00N;
+        else if constexpr (FLD == FTH && DI == 0 && DJ == 1)
+            return DK == 0 ? th01K : th01N;
+        else if constexpr (FLD == FTH && DI == 1 && DJ == 0)
+            return DK == 0 ? th10K : th10N;
+        else if constexpr (FLD == FU && DI == -1 && DJ == 0)
+            return DK == 0 ? um10K : um10N;
+        else if constexpr (FLD == FU && DI == -1 && DJ == 1)
+            return DK == 0 ? um11K : um11N;
+        else if constexpr (FLD == FU && DI == 0 && DJ == 0)
+            return DK == 0 ? u00K : u00N;
+        else if constexpr (FLD == FU && DI == 0 && DJ == 1)
+            return DK == 0 ? u01K : u01N;
+        else if constexpr (FLD == FV && DI == 0 && DJ == 0)
+            return DK == -1 ? v00M : v00K;
+        else if constexpr (FLD == FV && DI == 0 && DJ == 1)
+            return DK == -1 ? v01M : v01K;
+        else if constexpr (FLD == FV && DI == 1 && DJ == 0)
+            return DK == -1 ? v10M : v10K;
+        else if constexpr (FLD == FV && DI == 1 && DJ == 1)
+            return DK == -1 ? v11M : v11K;
+        else if constexpr (FLD == FW && DI == 0 && DJ == -1)
+            return DK == 0 ? w0m1K : w0m1N;
+        else if constexpr (FLD == FW && DI == 0 && DJ == 0)
+            return DK == 0 ? w00K : w00N;
+        else if constexpr (FLD == FW && DI == 1 && DJ == -1)
+            return DK == 0 ? w1m1K : w1m1N;
+        else if constexpr (FLD == FW && DI == 1 && DJ == 0)
+            return DK == 0 ? w10K : w10N;
+        else {
+            static_assert(DK == 0, "a read across k without a register");
+            return fptr<FLD>(F)[bk + o + (unsigned)(DJ * (int)li + DI)];
+        }
+    }
+    // the march starts at plane k0 (offset b0) after km(k0) (offset bm)
+    __device__ void start(unsigned bm, unsigned b0)
+    {
+        a00M = F.A[bm + o];
+        a00K = F.A[b0 + o];
+        a10M = F.A[bm + o + (unsigned)(0 * (int)li + 1)];
+        bm10K = F.B[b0 + o + (unsigned)(0 * (int)li + -1)];
+        b0m1K = F.B[b0 + o + (unsigned)(-1 * (int)li + 0)];
+        b00M = F.B[bm + o];
+        b00K = F.B[b0 + o];
+        c00M = F.C[bm + o];
+        c00K = F.C[b0 + o];
+        c01M = F.C[bm + o + (unsigned)(1 * (int)li + 0)];
+        th00K = F.th[b0 + o];
+        th01K = F.th[b0 + o + (unsigned)(1 * (int)li + 0)];
+        th10K = F.th[b0 + o + (unsigned)(0 * (int)li + 1)];
+        um10K = F.uel[b0 + o + (unsigned)(0 * (int)li + -1)];
+        um11K = F.uel[b0 + o + (unsigned)(1 * (int)li + -1)];
+        u00K = F.uel[b0 + o];
+        u01K = F.uel[b0 + o + (unsigned)(1 * (int)li + 0)];
+        v00M = F.vel[bm + o];
+        v01M = F.vel[bm + o + (unsigned)(1 * (int)li + 0)];
+        v10M = F.vel[bm + o + (unsigned)(0 * (int)li + 1)];
+        v11M = F.vel[bm + o + (unsigned)(1 * (int)li + 1)];
+        w0m1K = F.wel[b0 + o + (unsigned)(-1 * (int)li + 0)];
+        w00K = F.wel[b0 + o];
+        w1m1K = F.wel[b0 + o + (unsigned)(-1 * (int)li + 1)];
+        w10K = F.wel[b0 + o + (unsigned)(0 * (int)li + 1)];
+    }
+    // plane k (offset bk_) with its ka (offset bn)
+    __device__ void advance(unsigned bk_, unsigned bn)
+    {
+        bk = bk_;
+        a00N = F.A[bn + o];
+        a10K = F.A[bk + o + (unsigned)(0 * (int)li + 1)];
+        bm10N = F.B[bn + o + (unsigned)(0 * (int)li + -1)];
+        b0m1N = F.B[bn + o + (unsigned)(-1 * (int)li + 0)];
+        b00N = F.B[bn + o];
+        c00N = F.C[bn + o];
+        c01K = F.C[bk + o + (unsigned)(1 * (int)li + 0)];
+        th00N = F.th[bn + o];
+        th01N = F.th[bn + o + (unsigned)(1 * (int)li + 0)];
+        th10N = F.th[bn + o + (unsigned)(0 * (int)li + 1)];
+        um10N = F.uel[bn + o + (unsigned)(0 * (int)li + -1)];
+        um11N = F.uel[bn + o + (unsigned)(1 * (int)li + -1)];
+        u00N = F.uel[bn + o];
+        u01N = F.uel[bn + o + (unsigned)(1 * (int)li + 0)];
+        v00K = F.vel[bk + o];
+        v01K = F.vel[bk + o + (unsigned)(1 * (int)li + 0)];
+        v10K = F.vel[bk + o + (unsigned)(0 * (int)li + 1)];
+        v11K = F.vel[bk + o + (unsigned)(1 * (int)li + 1)];
+        w0m1N = F.wel[bn + o + (unsigned)(-1 * (int)li + 0)];
+        w00N = F.wel[bn + o];
+        w1m1N = F.wel[bn + o + (unsigned)(-1 * (int)li + 1)];
+        w10N = F.wel[bn + o + (unsigned)(0 * (int)li + 1)];
+    }
+    __device__ void rotate()
+    {
+        a00M = a00K;
+        a00K = a00N;
+        a10M = a10K;
+        bm10K = bm10N;
+        b0m1K = b0m1N;
+        b00M = b00K;
+        b00K = b00N;
+        c00M = c00K;
+        c00K = c00N;
+        c01M = c01K;
+        th00K = th00N;
+        th01K = th01N;
+        th10K = th10N;
+        um10K = um10N;
+        um11K = um11N;
+        u00K = u00N;
+        u01K = u01N;
+        v00M = v00K;
+        v01M = v01K;
+        v10M = v10K;
+        v11M = v11K;
+        w0m1K = w0m1N;
+        w00K = w00N;
+        w1m1K = w1m1N;
+        w10K = w10N;
+    }
+};
+
+// A, B and C in one pass, marching: one thread per (i, j) column over KC3
+// consecutive planes, the values reused across planes carried in registers
+// (MView); tiles dealt to the XCDs in contiguous ranges (the hardware deals
+// linear workgroup ids to the 8 XCDs in turn), so the rows j-1 and j+1 a
+// block reads were just fetched into that XCD's L2 by its neighbours.
+constexpr int KC3 = 8;
+constexpr int MARCH_MIN = 16384;   // columns (nr nz) from which the march pays
+
+template <int M>
+__global__ void __launch_bounds__(WG) k3_explicit_abc(Geo g, Kc K, Fields F)
+{
+    const int nr = g.nr, nz = g.nz, np = g.np;
+    const unsigned T = gridDim.x * gridDim.y, L = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned x = L & 7u, q = L >> 3, base = T >> 3, rem = T & 7u;
+    const unsigned Wt = x * base + (x < rem ? x : rem) + q;
+    const int kc = (int)(Wt / gridDim.x), bx = (int)(Wt - (unsigned)kc * gridDim.x);
+    const int t = bx * WG + threadIdx.x;
+    if (t >= nr * nz) return;
+    const int jj = t / nr;
+    const int i = 1 + (t - jj * nr), j = 1 + jj;
+    const int k0 = kc * KC3, k1 = min(np, k0 + KC3 - 1);
+    const unsigned pl = (unsigned)(g.ldi * g.ldj);
+    MView fv(F, (unsigned)(j * g.ldi + i), (unsigned)g.ldi);
+    fv.start((unsigned)(k0 == 0 ? np - 1 : k0 - 1) * pl, (unsigned)k0 * pl);
+    for (int k = k0; k <= k1; ++k) {
+        fv.advance((unsigned)k * pl, (unsigned)(k == np ? 1 : k + 1) * pl);
+        const unsigned e = (unsigned)k * pl + fv.o;
+        if (j <= nz - 1) F.An[e] = avort<M>(g, K, fv, i, j, k);
+        if (i <= nr - 1 && j <= nz - 1) F.Bn[e] = bvort<M>(g, K, fv, i, j, k);
+        if (i <= nr - 1) F.Cn[e] = cvort<M>(g, K, fv, i, j, k);
+        fv.rotate();
+    }
 }
 
 // One routine (W = 0: A, 1: B, 2: C, 3: T) per launch, or (W = 4) A, B and
-// C together, which read the same fields; blockIdx.y = k, a thread per
-// (i, j) point of the routine's loop range (for W = 4 the union i = 1..nr,
-// j = 1..nz, each routine evaluated where its own range holds).  A kernel per
-// routine set keeps its constants and field pointers within the scalar
-// registers.  M is the model (enum vv3_model).
+// C together; blockIdx.y = k, a thread per (i, j) point of the routine's
+// loop range (for W = 4 the union i = 1..nr, j = 1..nz, each routine
+// evaluated where its own range holds).  A kernel per routine set keeps its
+// constants and field pointers within the scalar registers.  M is the model
+// (enum vv3_model).  W = 4 serves planes of fewer than MARCH_MIN columns,
+// where marching leaves too few threads to fill the chip.
 template <int W, int M>
 __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
 {
     const int nr = g.nr, nz = g.nz;
-    if constexpr (W == 4) {
-        const int t = blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= nr * nz) return;
-        const int jj = t / nr;
-        const int i = 1 + (t - jj * nr), j = 1 + jj, k = blockIdx.y;
-        if (j <= nz - 1) X(F.An, i, j, k) = avort<M>(g, K, F, i, j, k);
-        if (i <= nr - 1 && j <= nz - 1) X(F.Bn, i, j, k) = bvort<M>(g, K, F, i, j, k);
-        if (i <= nr - 1) X(F.Cn, i, j, k) = cvort<M>(g, K, F, i, j, k);
-        return;
-    }
     constexpr int which = W;
     int i0 = 1, ni, j0 = 1, nj;
-    if (which == 0) { ni = nr; nj = nz - 1; }
+    if (which == 4) { ni = nr; nj = nz; }
+    else if (which == 0) { ni = nr; nj = nz - 1; }
     else if (which == 1) { ni = nr - 1; nj = nz - 1; }
     else if (which == 2) { ni = nr - 1; nj = nz; }
     else { ni = nr; j0 = 0; nj = nz + 2; }
@@ -441,10 +642,19 @@ __global__ void __launch_bounds__(WG) k3_explicit(Geo g, Kc K, Fields F)
     if (t >= ni * nj) return;
     const int jj = t / ni;
     const int i = i0 + (t - jj * ni), j = j0 + jj, k = blockIdx.y;
-    if constexpr (W == 0) X(F.An, i, j, k) = avort<M>(g, K, F, i, j, k);
-    else if constexpr (W == 1) X(F.Bn, i, j, k) = bvort<M>(g, K, F, i, j, k);
-    else if constexpr (W == 2) X(F.Cn, i, j, k) = cvort<M>(g, K, F, i, j, k);
-    else X(F.thn, i, j, k) = temper(g, K, F, i, j, k);
+    const int km = (k == 0) ? g.np - 1 : k - 1, ka = (k == g.np) ? 1 : k + 1;
+    const unsigned o = (unsigned)(j * g.ldi + i), pl = (unsigned)(g.ldi * g.ldj);
+    const GView fv{F, {(unsigned)km * pl + o, (unsigned)k * pl + o, (unsigned)ka * pl + o}, g.ldi};
+    if constexpr (W == 4) {
+        if (j <= nz - 1) X(F.An, i, j, k) = avort<M>(g, K, fv, i, j, k);
+        if (i <= nr - 1 && j <= nz - 1) X(F.Bn, i, j, k) = bvort<M>(g, K, fv, i, j, k);
+        if (i <= nr - 1) X(F.Cn, i, j, k) = cvort<M>(g, K, fv, i, j, k);
+        return;
+    }
+    if constexpr (W == 0) X(F.An, i, j, k) = avort<M>(g, K, fv, i, j, k);
+    else if constexpr (W == 1) X(F.Bn, i, j, k) = bvort<M>(g, K, fv, i, j, k);
+    else if constexpr (W == 2) X(F.Cn, i, j, k) = cvort<M>(g, K, fv, i, j, k);
+    else X(F.thn, i, j, k) = temper(g, K, fv, i, j, k);
 }
 
 // heat, rot3dmg.f:882-905.  The flux terms of every (j, k) in parallel,
@@ -992,7 +1202,11 @@ static int launch_explicit_m(vv3_run *r, int mask)
     const long long pc = (long long)(r->nr - 1) * r->nz, pt = (long long)r->nr * (r->nz + 2);
     if ((mask & 7) == 7) {
         const long long pu = (long long)r->nr * r->nz;
-        hipLaunchKernelGGL((k3_explicit<4, M>), dim3(nblk(pu, WG), np1), dim3(WG), 0, r->stream, g, K, F);
+        if (pu >= MARCH_MIN)
+            hipLaunchKernelGGL((k3_explicit_abc<M>), dim3(nblk(pu, WG), (np1 + KC3 - 1) / KC3), dim3(WG), 0,
+                               r->stream, g, K, F);
+        else
+            hipLaunchKernelGGL((k3_explicit<4, M>), dim3(nblk(pu, WG), np1), dim3(WG), 0, r->stream, g, K, F);
     } else {
         if (mask & 1)
             hipLaunchKernelGGL((k3_explicit<0, M>), dim3(nblk(pa, WG), np1), dim3(WG), 0, r->stream, g, K, F);

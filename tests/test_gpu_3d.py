@@ -107,6 +107,29 @@ def test_vorticity_in_one_launch_equals_separate_calls():
     run.close()
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 10), (130, 131, 18)])
+def test_marching_vorticity_matches_oracle(shape):
+    """Planes of 16384 columns and more take the marching kernel (a column
+    carries the values it reuses across k in registers, chunks of 8 planes,
+    the periodic neighbours km of k = 0 and ka of k = np at chunk edges):
+    A, B and C bit for bit against the oracle, with velocities large enough
+    to take the hybrid-upwind branches."""
+    nr, nz, np_ = shape
+    print(f"[gpu] 3D marching vorticity {shape}", flush=True)
+    run = _gpu(nr, nz, np_)
+    ora = O3.Run3(nr, nz, np_, **PARAMS)
+    f = _state(np.random.default_rng(nr + np_), ora.shape, 150.0)
+    for n, a in f.items():
+        run.put(n, a)
+        ora[n] = a
+    run.vorticity()
+    for rt in ("avorticity", "bvorticity", "cvorticity"):
+        ora.call(rt)
+    for n in ("An", "Bn", "Cn"):
+        assert np.array_equal(run.get(n), ora[n]), n
+    run.close()
+
+
 @pytest.mark.parametrize("seed", [5, 6])
 def test_explicit_routines_match_oracle(seed):
     """A second grid shape (nr != nz, np not a multiple of 8) against the
