@@ -1232,6 +1232,9 @@ struct vvt_run {
     double dr = 0, dz = 0, dp = 0;
     size_t nf = 0;
     double *f[VV3_NFIELD] = {nullptr};
+    // per field: zero outside its solve's unknowns, ghost planes aside (the
+    // fields start at zero; a put clears the flag); see launch_poisson
+    bool clean[VV3_NFIELD] = {};
     double *dv[12] = {nullptr};
     double *tabs = nullptr;       // compact elimination tables
     size_t tab_off[12] = {0};
@@ -1575,11 +1578,21 @@ int launch_poisson(vvt_run *r, int which)
     double *P = r->f[psi];
     const int mt = O.m;
     const double ftr_y = 2.0 * O.k1 * O.k1 * (double)mt, ftr_z = 2.0 * O.j1 * O.j1 * (double)O.i1 * O.k1;
-    {
+    // psip <- psi and psi <- 0 outside the unknowns: when psip's buffer is
+    // already zero there (it held psi before the previous solve), the two
+    // buffers swap -- psip becomes the old psi, psi a buffer the solve then
+    // fills (unknowns, then the ghost planes) -- with no copy at all
+    if (r->clean[psip]) {
+        std::swap(r->f[psi], r->f[psip]);
+        std::swap(r->clean[psi], r->clean[psip]);
+    } else {
         Prof p(r, VVT_K_PSI_PRE, 8.0 * 3.0 * (double)r->nf);
         hipLaunchKernelGGL(k_t_psi_pre, dim3(nblk((long long)r->ldi * r->ldj, WG), r->np + 2), dim3(WG), 0, s, g,
                            O.i1, O.j1, O.k1, P, r->f[psip]);
+        r->clean[psip] = r->clean[psi];
+        r->clean[psi] = true;
     }
+    P = r->f[psi];
     const bool split = (r->np % 2 == 0);
     const int h = r->np / 2;
     const int LP = r->ylp, nbk = (mt + 2 * LP - 1) / (2 * LP);
@@ -1755,6 +1768,7 @@ int vvt_create(const vvt_config *cfg, int device, vvt_run **out)
     for (int q = 0; q < VV3_NFIELD; ++q) {
         if (hipMalloc(&r->f[q], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
         if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
+        r->clean[q] = true;
     }
     for (int d = 0; d < 12; ++d) {
         if (hipMalloc(&r->dv[d], r->nf * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
@@ -1826,6 +1840,7 @@ int vvt_put(vvt_run *r, int field, const double *host, int ldi, int ldj)
 {
     if (!r || !host || field < 0 || field >= VV3_NFIELD) { vv::set_error("vvt_put: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
+    r->clean[field] = false;
     return copy_field(r, r->f[field], const_cast<double *>(host), ldi, ldj, true);
 }
 
