@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(WG) k_t_deriv_yz(TGeo g, DJobs jobs, int j0, d
 // reloads one, recomputes its r' from the checkpoint before it in registers
 // (the same operations, the same bits), substitutes back into the tile and
 // stores the chunk's rows coalesced.
-constexpr int CX = 32, XSP = CX + 3;   // rows per chunk; tile pitch (odd)
+constexpr int CX = 20, XSP = CX + 3;   // rows per chunk; tile pitch (odd)
 
 __global__ void __launch_bounds__(64) k_t_deriv_x(TGeo g, DJobs jobs, double *__restrict__ ckp, int nseg_max, int nls)
 {
