@@ -129,6 +129,37 @@ def vcycle_leg(name, steps, warmup, nu=1):
             "poisson_cycles_last_step": [info.cycles_v, info.cycles_u]}
 
 
+def lines_scan_leg(name, steps, warmup):
+    """The same workload with the compact schemes' line solves in their scan
+    form (vv_config.lines = 1: a wavefront per line, the affine recurrences
+    composed across the lanes; equal to the default's results to rounding,
+    not bit for bit), timed as the default; the compact kernel's device time
+    per launch from a second, profiled pass, for both forms."""
+    from codesthesis_dir_amd import BoxConfig, PeriodicBox
+    from codesthesis_dir_amd.ic import box_temperature
+    desc, kw, seed, amp = WORKLOADS[name]
+    out = {}
+    for lines in (1, 0):
+        cfg = BoxConfig(rtol=1e-10, max_cycles=60, lines=lines, **kw)
+        box = PeriodicBox(cfg, device=0)
+        box.put("th", box_temperature(cfg.nr, cfg.np, cfg.r0, cfg.rmax, amp, seed))
+        box.start()
+        box.step(warmup)
+        t0 = time.perf_counter()
+        info = box.step(steps)
+        el = time.perf_counter() - t0
+        check_status(name + f" (lines={lines})", info)
+        box.profile(True)
+        box.step(steps)
+        p = box.profile_read()["compact"]
+        box.profile(False)
+        box.close()
+        out[lines] = (steps / el, 1e3 * p["ms"] / max(p["launches"], 1))
+    return {"value": out[1][0], "unit": "time-steps/s", "lines": "scan (vv_config.lines = 1)",
+            "compact_us_per_launch": out[1][1], "reference_lines_same_box": {"value": out[0][0],
+            "compact_us_per_launch": out[0][1]}}
+
+
 def vcycle_large_leg(steps=6, nu=1):
     """V-cycle bandwidth on a large plane: periodic box, 2049 x 4097 nodes
     (nr=2048, np=4096), V(nu,nu) with five smoothed levels and the direct
@@ -805,8 +836,9 @@ def main():
         done("time step")
     if not a.no_extra:
         name2d = a.workload if a.workload in WORKLOADS else "annulus-513x257"
-        line["alternatives"] = {"poisson_vcycle": vcycle_leg(name2d, a.steps, a.warmup, a.nu)}
-        done("V-cycle alternative")
+        line["alternatives"] = {"poisson_vcycle": vcycle_leg(name2d, a.steps, a.warmup, a.nu),
+                                "lines_scan": lines_scan_leg(name2d, a.steps, a.warmup)}
+        done("V-cycle and scan-line alternatives")
         line["vcycle_large"] = vcycle_large_leg()
         done("large V-cycle")
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)

@@ -28,7 +28,8 @@ class VVConfig(ctypes.Structure):
                 ("Ra", ctypes.c_double), ("Pr", ctypes.c_double), ("robert", ctypes.c_double),
                 ("rtol", ctypes.c_double), ("max_cycles", ctypes.c_int), ("nu1", ctypes.c_int),
                 ("nu2", ctypes.c_int), ("stats", ctypes.c_int), ("geom", ctypes.c_int),
-                ("Re", ctypes.c_double), ("direct_cols", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("Re", ctypes.c_double), ("direct_cols", ctypes.c_int), ("precision", ctypes.c_int),
+                ("lines", ctypes.c_int)]
 
 VV_GEOM_BOX, VV_GEOM_ANNULUS = 0, 1
 

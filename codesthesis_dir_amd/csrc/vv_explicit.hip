@@ -311,6 +311,215 @@ __global__ void __launch_bounds__(WG) k_compact(CompactArgs a, CompactTiles t)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same six line families with the recurrences evaluated in parallel
+// (vv_config.lines = VV_LINES_SCAN): one wavefront per line, each lane a
+// chunk of m consecutive rows.  Every recurrence of the reference's
+// elimination is affine in its predecessor (y_j = r_j - f_j y_{j-1}; the
+// back substitution x_j = p_j + q_j x_{j+1}), so a chunk composes its rows
+// into one affine map, an inclusive scan of those maps across the 64 lanes
+// (6 shuffle steps) gives every chunk its entry value, and the chunk then
+// runs its rows from there.  The same elimination factors and pivots as the
+// sequential form; only the association of the products changes, so the
+// results agree to rounding, not bit for bit.  SM: the rows a lane holds
+// (at most 32: lines of up to 2048 rows), a template parameter so that the
+// chunk stays in registers.
+constexpr int SM_MAX = 32;
+
+// composes (a, b) after the maps of the lanes below (inclusive scan up)
+__device__ __forceinline__ void scan_up(double &a, double &b)
+{
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double pa = __shfl_up(a, d, 64), pb = __shfl_up(b, d, 64);
+        if (lane >= d) {
+            a = a + b * pa;
+            b = b * pb;
+        }
+    }
+}
+// composes (a, b) before the maps of the lanes above (inclusive scan down)
+__device__ __forceinline__ void scan_down(double &a, double &b)
+{
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double na = __shfl_down(a, d, 64), nb = __shfl_down(b, d, 64);
+        if (lane + d < 64) {
+            a = a + b * na;
+            b = b * nb;
+        }
+    }
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Thomas along r (rows 0..n-1 of column k), the tables of tdma_tile
+template <int FAM, int SM>
+__device__ void tdma_scan(const CompactArgs &a, int k)
+{
+    const int nr = a.nr, ld = a.ld;
+    const int n = (FAM == 2) ? nr : nr - 1;
+    const double *tab = (FAM == 2) ? a.tab_rT : a.tab_r1;
+    double *out = (FAM == 0) ? a.dCr : (FAM == 1) ? a.dthrC : a.dthrT;
+    const double *fact = tab, *bp = tab + n, *cc = tab + 2 * n, *rbp = tab + 3 * n;
+    const int lane = (int)(threadIdx.x & 63), m = (n + 63) / 64, j0 = lane * m;
+    double v[SM];
+    // the forward elimination's map of this chunk: y_out = fa + fb y_in
+    double fa = 0.0, fb = 1.0;
+#pragma unroll
+    for (int u = 0; u < SM; ++u) {
+        const int j = j0 + u;
+        if (u < m && j < n) {
+            v[u] = rhs_r<FAM>(a, j + 1, k);
+            if (j == 0) { fa = v[u]; fb = 0.0; }
+            else { fa = v[u] - fact[j] * fa; fb = -fact[j] * fb; }
+        }
+    }
+    scan_up(fa, fb);
+    double y = __shfl_up(fa, 1, 64);   // the row before the chunk (row 0 resets it)
+#pragma unroll
+    for (int u = 0; u < SM; ++u) {
+        const int j = j0 + u;
+        if (u < m && j < n) {
+            y = (j == 0) ? v[u] : v[u] - fact[j] * y;
+            v[u] = y;
+        }
+    }
+    // back substitution: x_j = p_j + q_j x_{j+1}, the last row q = 0
+    double ba = 0.0, bb = 1.0;
+#pragma unroll
+    for (int u = SM - 1; u >= 0; --u) {
+        const int j = j0 + u;
+        if (u < m && j < n) {
+            const double p = v[u] * rbp[j], q = (j == n - 1) ? 0.0 : -cc[j] * rbp[j];
+            ba = p + q * ba;
+            bb = q * bb;
+        }
+    }
+    scan_down(ba, bb);
+    double x = __shfl_down(ba, 1, 64);   // the row after the chunk (the last row ends it)
+#pragma unroll
+    for (int u = SM - 1; u >= 0; --u) {
+        const int j = j0 + u;
+        if (u < m && j < n) {
+            const double p = v[u] * rbp[j], q = (j == n - 1) ? 0.0 : -cc[j] * rbp[j];
+            x = p + q * x;
+            out[ix(j + 1, k, ld)] = x;
+        }
+    }
+}
+
+// cyclic solve along phi (entries 1..n of row i), the tables of ptdma_tile
+template <int FAM, int SM>
+__device__ void ptdma_scan(const CompactArgs &a, int i)
+{
+    const int np = a.np, ld = a.ld;
+    const int n = np + 1, s = n + 1;
+    double *out = (FAM == 3) ? a.dCh : (FAM == 4) ? a.dthhC : a.dthhT;
+    const double *tb = a.tab_p;
+    const double *f1 = tb, *f2 = tb + s, *aa = tb + 2 * s, *bbt = tb + 3 * s, *cc = tb + 4 * s;
+    const double fA = tb[5 * s], fB = tb[5 * s + 1], fC = tb[5 * s + 2];
+    const double *rbb = tb + 5 * s + 3;
+    // entries 1..n-2 in chunks; n-1 and n are closed afterwards
+    const int ne = n - 2, lane = (int)(threadIdx.x & 63), m = (ne + 63) / 64, j0 = 1 + lane * m;
+    double v[SM];
+    double fa = 0.0, fb = 1.0;
+#pragma unroll
+    for (int u = 0; u < SM; ++u) {
+        const int j = j0 + u;
+        if (u < m && j <= ne) {
+            v[u] = rhs_p<FAM>(a, i, j);
+            if (j == 1) { fa = v[u]; fb = 0.0; }
+            else { fa = v[u] - f1[j] * fa; fb = -f1[j] * fb; }
+        }
+    }
+    scan_up(fa, fb);
+    double rm = __shfl_up(fa, 1, 64);
+    double sum = 0.0;   // this chunk's part of sum_j f2_j rm_{j-1}
+#pragma unroll
+    for (int u = 0; u < SM; ++u) {
+        const int j = j0 + u;
+        if (u < m && j <= ne) {
+            if (j == 1) rm = v[u];
+            else {
+                sum += f2[j] * rm;
+                rm = v[u] - f1[j] * rm;
+            }
+            v[u] = rm;
+        }
+    }
+    sum = wave_sum(sum);
+    // rm_{n-2}: the last lane's chunk that holds entry n-2
+    const int lastlane = (ne - 1) / m;
+    const double rm2 = __shfl(rm, lastlane, 64);
+    const double Rn = rhs_p<FAM>(a, i, n), Rn1 = rhs_p<FAM>(a, i, n - 1);
+    const double rn1 = Rn1 - fA * rm2;
+    double rn = Rn - sum;
+    rn = rn - fB * rm2;
+    rn = rn - fC * rn1;
+    const double yn = div_rb(rn, bbt[n], rbb[n]);
+    const double yn1 = div_rb(rn1 - cc[n - 1] * yn, bbt[n - 1], rbb[n - 1]);
+    // back substitution: y_j = p_j + q_j y_{j+1}, p_j = (rm_j - a_j yn)/b_j
+    double ba = 0.0, bb = 1.0;
+#pragma unroll
+    for (int u = SM - 1; u >= 0; --u) {
+        const int j = j0 + u;
+        if (u < m && j <= ne) {
+            const double p = (v[u] - aa[j] * yn) * rbb[j], q = -cc[j] * rbb[j];
+            ba = p + q * ba;
+            bb = q * bb;
+        }
+    }
+    scan_down(ba, bb);
+    // entry after the chunk: the next lane's first, or y_{n-1} past the last
+    double y = __shfl_down(ba, 1, 64), yb = __shfl_down(bb, 1, 64);
+    if (j0 + m > ne) { y = yn1; yb = 0.0; }
+    else y = y + yb * yn1;   // the next lanes' maps end at entry n-1
+#pragma unroll
+    for (int u = SM - 1; u >= 0; --u) {
+        const int j = j0 + u;
+        if (u < m && j <= ne) {
+            const double p = (v[u] - aa[j] * yn) * rbb[j], q = -cc[j] * rbb[j];
+            y = p + q * y;
+            out[ix(i, j - 1, ld)] = y;
+        }
+    }
+    if (lane == 0) {
+        out[ix(i, n - 2, ld)] = yn1;
+        out[ix(i, n - 1, ld)] = yn;
+    }
+}
+
+template <int SM>
+__global__ void __launch_bounds__(WG) k_compact_scan(CompactArgs a)
+{
+    const int fam = blockIdx.y;
+    if (!(a.mask & (1 << fam))) return;
+    const int line = blockIdx.x * (WG / 64) + (int)(threadIdx.x >> 6);
+    if (fam < 3) {
+        if (line > a.np) return;   // whole waves leave together: the shuffles see full waves
+        switch (fam) {
+        case 0: tdma_scan<0, SM>(a, line); break;
+        case 1: tdma_scan<1, SM>(a, line); break;
+        default: tdma_scan<2, SM>(a, line); break;
+        }
+    } else {
+        const int rows = (fam == 5) ? a.nr : a.nr - 1;
+        if (line >= rows) return;
+        switch (fam) {
+        case 3: ptdma_scan<3, SM>(a, 1 + line); break;
+        case 4: ptdma_scan<4, SM>(a, 1 + line); break;
+        default: ptdma_scan<5, SM>(a, 1 + line); break;
+        }
+    }
+}
+
 struct DFArgs {
     const double *C, *Cp, *th, *thp, *uel, *vel;
     const double *dCr, *dCh, *dthrC, *dthhC, *dthrT, *dthhT;
@@ -743,9 +952,21 @@ static int compact_launch(vv_run *r, int mask)
     if (lds > 160 * 1024) { set_error("compact-scheme line longer than the LDS of a CU holds"); return VV_EINVAL; }
     const int gr = nblk(r->np + 1, t.Tr), gp = nblk(r->nr, t.Tp);
     dim3 grid(gr > gp ? gr : gp, 6);
+    // the scan form: a wavefront per line, lines of up to 64 SM_MAX rows
+    const bool scan = r->cfg.lines == VV_LINES_SCAN && r->nr <= 64 * SM_MAX && r->np - 1 <= 64 * SM_MAX;
+    const int mrow = std::max((r->nr + 63) / 64, (r->np - 1 + 63) / 64);   // rows per lane
     hipEvent_t pe_;
     prof_begin(r, &pe_);
-    hipLaunchKernelGGL(k_compact, grid, dim3(WG), lds, r->stream, a, t);
+    if (scan) {
+        const int lines = (r->np + 1 > r->nr) ? r->np + 1 : r->nr;
+        const dim3 g(nblk(lines, WG / 64), 6);
+        if (mrow <= 4) hipLaunchKernelGGL(k_compact_scan<4>, g, dim3(WG), 0, r->stream, a);
+        else if (mrow <= 8) hipLaunchKernelGGL(k_compact_scan<8>, g, dim3(WG), 0, r->stream, a);
+        else if (mrow <= 16) hipLaunchKernelGGL(k_compact_scan<16>, g, dim3(WG), 0, r->stream, a);
+        else hipLaunchKernelGGL(k_compact_scan<32>, g, dim3(WG), 0, r->stream, a);
+    } else {
+        hipLaunchKernelGGL(k_compact, grid, dim3(WG), lds, r->stream, a, t);
+    }
     // read C and th, write six derivative fields
     prof_end(r, VV_K_COMPACT, pe_, 8.0 * 8.0 * (double)(r->nr + 2) * (r->np + 1));
     VV_HIP(hipGetLastError());

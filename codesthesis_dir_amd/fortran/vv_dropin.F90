@@ -62,6 +62,7 @@ module vv_dropin_c
     real(c_double) :: Re
     integer(c_int) :: direct_cols
     integer(c_int) :: precision
+    integer(c_int) :: lines
   end type
 
   integer(c_int), parameter :: VV_C = 0, VV_CP = 1, VV_CN = 2, VV_TH = 3, VV_THP = 4, &
@@ -211,6 +212,7 @@ subroutine poissoninit(nr, nz, np, dr, dz, dp, nlev)
 #endif
   cfg%direct_cols = 0
   cfg%precision = 0
+  cfg%lines = 0
   call vv_check(vv_create(cfg, 0_c_int, g_run, nl), 'poissoninit')
   nlev = nl
   call vv_dropin_env_residency()

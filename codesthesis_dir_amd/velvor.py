@@ -45,11 +45,12 @@ class BoxConfig:
     Re: float = 0.0        # rotational Reynolds number (annulus)
     direct_cols: int = 0  # direct level at <= this many columns (0: direct for np <= 8192, else 128)
     precision: int = 0    # 0: FP64 V-cycle; 1: FP64 defect correction around an FP32 V-cycle
+    lines: int = 0        # 0: the reference's line recurrences (bit-exact); 1: parallel scan (to rounding)
 
     def c(self) -> L.VVConfig:
         return L.VVConfig(self.nr, self.np, self.dt, self.r0, self.rmax, self.width, self.Ra,
                           self.Pr, self.robert, self.rtol, self.max_cycles, self.nu1, self.nu2,
-                          self.stats, self.geom, self.Re, self.direct_cols, self.precision)
+                          self.stats, self.geom, self.Re, self.direct_cols, self.precision, self.lines)
 
 
 def AnnulusConfig(**kw) -> BoxConfig:

@@ -99,10 +99,22 @@ typedef struct vv_config {
                              the correction equation in FP32.  Applies to
                              V-cycles (direct_cols < np); a direct solve of
                              the whole plane is FP64 either way.            */
+    int    lines;         /* the compact schemes' line solves (tdma, ptdma):
+                             VV_LINES_REFERENCE (0): the reference's
+                             recurrences in its order, bit for bit;
+                             VV_LINES_SCAN (1): one wavefront per line, the
+                             same elimination factors with the affine
+                             recurrences composed by a scan across the
+                             lanes -- equal to rounding, not bit for bit
+                             (lines of up to 2048 rows; longer ones take
+                             the reference form).  No counterpart in the
+                             reference.                                     */
 } vv_config;
 
 #define VV_PREC_FP64  0
 #define VV_PREC_MIXED 1
+#define VV_LINES_REFERENCE 0
+#define VV_LINES_SCAN      1
 
 /* per-step diagnostics (device-resident until read) */
 typedef struct vv_step_info {
