@@ -5,7 +5,7 @@
     python bench.py --serve-only                 # the served mixed-size sweep alone
     python bench.py --merid-only [--no-cpu]      # the meridional plane's 5000-step run alone
 
-The default workload is config 5, the largest single-GPU configuration: the 3D
+The default workload is the largest single-GPU one: the 3D
 horizontal-convection tank of 512 x 256 x 256 intervals (aspect 4:1:1),
 Ra = 1e9, Pr = 5, Ta = 1e8, uniform temperature plus seeded noise.  A step
 is one pass of its time loop (compact derivatives along the three axes,
@@ -17,9 +17,9 @@ dominant kernel's average duration and its achieved algorithmic bandwidth
 (or, for a transform GEMM, flop rate).  The CPU baseline (rank 0, N=1 only)
 times the oracle's step of the same workload on this host.  Unless
 --no-extra is given, the line also carries the meridional plane with swirl
-on stretched nodes (config 2: its whole 5000-step run at 513 x 257), the 2D
+on stretched nodes (its whole 5000-step run at 513 x 257), the 2D
 legs (the rotating-annulus plane at 513x257 with its V-cycle alternative, a
-large V-cycle), the 3D annulus of rot3dmg.f and the served sweep (config 4).
+large V-cycle), the 3D annulus of rot3dmg.f and the served sweep.
 
 The 2D workloads (--workload box-129x65, annulus-513x257, ...) keep their
 own headline form; every 2D leg refuses a run whose Poisson solves did not
@@ -478,7 +478,7 @@ def serve_leg(n=64, steps=2000, slots=16, baseline=True):
 
 
 
-# config 5: the 3D horizontal-convection tank, 512 x 256 x 256 intervals
+# the 3D horizontal-convection tank, 512 x 256 x 256 intervals
 # (np = 512 along the tank, nr = 256 across, nz = 256 vertical: aspect 4:1:1)
 TANK_WORKLOADS = {
     "tank-512x256x256": (
@@ -526,7 +526,7 @@ def tank_leg(name, steps, warmup):
     return cfg, wall_ms, prof, h, ke
 
 
-# row f3 / config 2: the meridional (r,z) plane with swirl on tanh-clustered
+# the meridional (r,z) plane with swirl on tanh-clustered
 # nodes, 513 x 257 nodes, its whole 5000-step run timed
 MERID = ("merid-513x257",
          "rotating-annulus meridional (r,z) plane with swirl (velocity-vorticity form of rotann2.f: "

@@ -374,7 +374,7 @@ def test_unsteady_high_ra_time_averaged_nusselt():
 
 
 @pytest.mark.parametrize("direct_cols", [16, 128])
-def test_trajectory_1000_steps_config1(direct_cols):
+def test_trajectory_1000_steps_box_129x65(direct_cols):
     """Periodic RB box 129x65 nodes (nr=64, np=128), 2:1 (width 1, height
     0.5), Ra=1e5, Pr=1, dt=1e-5, seeded temperature, 1000 steps with the
     Poisson solves at 1e-10 relative residual.  Relative L2 of every field

@@ -1,11 +1,11 @@
-"""Config 3 on the GPU: rot3dmg.f's rotating annulus (periodic in phi) at its
+"""The 3D rotating annulus on the GPU: rot3dmg.f's annulus (periodic in phi) at its
 physics, Ra = 1e7, Ta = 1e6 (Re = 1000), Pr = 7, Gr = 0, dt = 1e-6, started
 from conduction plus eight seeded azimuthal modes of amplitude 1e-3.
 
 * At a small grid the whole step is compared with the oracle (its explicit
   routines restate rot3dmg.f bit for bit and its exact solves zero the
   reference's residual operators, tests/test_oracle3.py).
-* At the config's own 129 x 129 x 257 grid (nr = nz = 128, np = 256) the
+* At its 129 x 129 x 257 grid (nr = nz = 128, np = 256) the
   oracle's dense solves are out of reach, so the run is checked through
   properties that fix the result independently of the size: every solve of
   the last step satisfies the reference's residual operator
@@ -61,11 +61,11 @@ def _oracle_step(ora):
     return t
 
 
-def test_config3_physics_trajectory_matches_oracle():
-    """20 steps at config 3's physics and initial state on a 13 x 11 x 16
+def test_rot3d_physics_trajectory_matches_oracle():
+    """20 steps at this physics and initial state on a 13 x 11 x 16
     grid: heat flux and every field within 1e-10 of the oracle."""
     nr, nz, np_ = 12, 10, 16
-    print(f"[gpu] config3 trajectory {nr}x{nz}x{np_}", flush=True)
+    print(f"[gpu] rot3d trajectory {nr}x{nz}x{np_}", flush=True)
     run, ora = _gpu(nr, nz, np_), _oracle(nr, nz, np_)
     for k, v in rot3d_state(nr, nz, np_).items():
         run.put(k, v)
@@ -81,11 +81,11 @@ def test_config3_physics_trajectory_matches_oracle():
     run.close()
 
 
-def test_config3_full_grid_properties():
-    """50 steps on config 3's grid (129 x 129 x 257), then one more step
+def test_rot3d_129x129x257_grid_properties():
+    """50 steps on the 129 x 129 x 257 grid, then one more step
     checked through grid-independent properties."""
     nr, nz, np_ = 128, 128, 256
-    print(f"[gpu] config3 full grid {nr}x{nz}x{np_}", flush=True)
+    print(f"[gpu] rot3d full grid {nr}x{nz}x{np_}", flush=True)
     run = _gpu(nr, nz, np_)
     for k, v in rot3d_state(nr, nz, np_).items():
         run.put(k, v)

@@ -1,4 +1,4 @@
-"""Config 4 on the GPU: the served mixed-size sweep.
+"""The served mixed-size sweep on the GPU.
 
 Twelve requests of the sweep itself (the benchmark's 64 seeded draws: sides
 65 ... 1025 x half height, Ra log-uniform in [1e4, 1e8], Pr in [0.7, 7], the
@@ -66,7 +66,7 @@ def test_served_sweep_matches_alone_and_oracle():
     sides = {reqs[q][1].np + 1 for q in pick}
     assert sides == {65, 129, 257, 513, 1025}
     assert max(reqs[q][1].Ra for q in pick) > 9e7
-    print(f"[gpu] config4 serving {len(pick)} requests: {pick}", flush=True)
+    print(f"[gpu] served sweep {len(pick)} requests: {pick}", flush=True)
     with Server(slots=8, mem_budget=64e9) as srv:
         ids = {srv.submit(reqs[q][1], reqs[q][3], reqs[q][2], outputs=True): q for q in pick}
         got = {}
