@@ -9,7 +9,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libvvmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["vv_explicit.hip", "vv_mg.hip", "vv3.hip", "vv3_poisson.hip", "vv_band.hip", "vvt.hip", "vvm.hip", "vv_api.cpp", "vv_serve.cpp"]
+SOURCES = ["vv_explicit.hip", "vv_mg.hip", "vv3.hip", "vv3_poisson.hip", "vv_band.hip", "vvt.hip", "vvm.hip", "vvs.hip", "vv_api.cpp", "vv_serve.cpp"]
 # -ffp-contract=off keeps the explicit-step arithmetic identical to the
 # reference's operation order; packed FP32 instructions are disabled.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

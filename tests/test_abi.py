@@ -8,7 +8,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vvmi.h")
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h", "vvband.h", "vvtank.h", "vvmerid.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("vvmi.h", "vvmi3.h", "vvserve.h", "vvband.h", "vvtank.h", "vvmerid.h", "vvsector.h")]
 LIB = os.path.join(ROOT, "codesthesis_dir_amd", "libvvmi.so")
 
 
@@ -16,7 +16,7 @@ def declared():
     names = set()
     for h in HEADERS:
         src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b(vv[3tm]?_[a-z0-9_]+)\s*\(", src))
+        names |= set(re.findall(r"\b(vv[3tms]?_[a-z0-9_]+)\s*\(", src))
     return sorted(names)
 
 
@@ -74,3 +74,12 @@ def test_fortran_dropin3_exports_reference_routine_names(name):
     for n in ("poissoninit_", "poisson1_", "poisson2_", "poisson3_", "avorticity_", "bvorticity_",
               "cvorticity_", "temperature_", "heat_", "rhsw_", "rhsv_", "rhsu_", "update_"):
         assert hasattr(lib, n), n
+
+
+def test_sector_header_declares_entry_points():
+    names = declared()
+    for n in ("vvs_create", "vvs_destroy", "vvs_put", "vvs_get", "vvs_vorticity", "vvs_temperature", "vvs_heat",
+              "vvs_rhs", "vvs_poisson", "vvs_update", "vvs_step"):
+        assert n in names
+    src = open(HEADERS[-1]).read()
+    assert src.count("sect360v2mg.f:") >= 8
