@@ -4,6 +4,7 @@
     python bench.py --rot3d-only [--no-cpu]      # the 3D annulus step (rot3dmg.f) alone
     python bench.py --serve-only                 # the served mixed-size sweep alone
     python bench.py --merid-only [--no-cpu]      # the meridional plane's 5000-step run alone
+    python bench.py --sector-only                # the barrier sector (sect360v2mg.f) alone
 
 The default workload is the largest single-GPU one: the 3D
 horizontal-convection tank of 512 x 256 x 256 intervals (aspect 4:1:1),
@@ -387,6 +388,85 @@ def rot3d_leg(cpu=True, warmup=3, only=None):
                       "multigrid to the shipped absolute residual 1e-5, from the same start"}
         g = out["rot3d-33x33x161"]["steps_per_s"]
         out["gpu_over_cpu_reference_33x33x161"] = g / out["cpu_reference_33x33x161"]["value"]
+    return out
+
+
+# sect360v2mg.f (the rotating annulus with a radial barrier) at the
+# reference's PARAMETER grid (kr = kz = 33, kp = 161), 129x129x257 and
+# 257x257x513, with rot3dmg.f's rotating-annulus physics
+SECTOR = {
+    "sector-33x33x161": (32, 32, 160),
+    "sector-129x129x257": (128, 128, 256),
+    "sector-257x257x513": (256, 256, 512),
+}
+SECTOR_STEPS = {"sector-33x33x161": 100, "sector-129x129x257": 20, "sector-257x257x513": 5}
+
+
+def sector_state(nr, nz, np_, r0=0.5, rmax=1.0, amp=1e-3, nval=2):
+    """sect360v2mg.f's initial temperature (sect360v2mg.f:94-123): the
+    conduction profile 1 - ln r / ln r0 (rmax = 1), a mode nval of amplitude
+    amp, and the cold line th(nr/4, j, np/2) - 0.2, j = 1..nz; fluid at rest."""
+    import numpy as np
+    dr = (rmax - r0) / nr
+    r = r0 + (np.arange(nr + 2) - 0.5) * dr
+    r[0], r[-1] = r0, rmax
+    phi = 8.0 * np.arctan(1.0) * np.arange(np_ + 2) / np_
+    th = (1.0 - np.log(r) / np.log(r0))[None, None, :] + \
+        amp * ((rmax - r) * (r - r0))[None, None, :] * np.cos(nval * phi)[:, None, None]
+    th = np.broadcast_to(th, (np_ + 2, nz + 2, nr + 2)).copy()
+    th[np_ // 2, 1:nz + 1, nr // 4] -= 0.2
+    return {"th": th, "thp": th.copy()}
+
+
+def sector_leg(warmup=3, only=None, reps=10):
+    """The barrier-sector step (include/vvsector.h: A/B/C and temperature in
+    one launch, heat, three exact Poisson solves, wall update) at the three
+    grids; at 129x129x257 also each routine's time, each call synchronised.
+    sect360v2mg.f does not compile here (its include files are absent), so
+    there is no reference timing."""
+    from codesthesis_dir_amd.sector import Sector, SectorConfig
+    P = {k: v for k, v in ROT3D_PHYS.items()}
+    out = {"physics": "sect360v2mg.f (radial barrier), Ra=1e7, Re=1000 (Ta=1e6), Pr=7, Gr=0, dt=1e-6, r0=0.5, "
+                      "rmax=1, zmax=0.5",
+           "data": "synthetic: the reference's start (conduction, mode 2 of amplitude 1e-3, the cold line at "
+                   "nr/4, np/2), fluid at rest"}
+    for name, (nr, nz, np_) in SECTOR.items():
+        if only and name != only:
+            continue
+        run = Sector(SectorConfig(nr=nr, nz=nz, np=np_, **P))
+        for k, v in sector_state(nr, nz, np_).items():
+            run.put(k, v)
+        run.step(warmup)
+        n = SECTOR_STEPS[name]
+        t0 = time.perf_counter()
+        t1 = run.step(n)
+        el = time.perf_counter() - t0
+        out[name] = {"steps_per_s": n / el, "ms_per_step": 1e3 * el / n, "steps": n, "trans1_last": t1}
+        if nr == 128 and not only:
+            N = (nr + 2) * (nz + 2) * (np_ + 2)
+            calls = {"vorticity_abc": (lambda: run.vorticity(0), 27), "temperature": (run.temperature, 6),
+                     "rhsw": (lambda: run.rhs(1), 3), "rhsv": (lambda: run.rhs(2), 3),
+                     "rhsu": (lambda: run.rhs(3), 4), "update": (run.update, 24)}
+            rt = {}
+            for key, (f, w) in calls.items():
+                f()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    f()
+                ms = 1e3 * (time.perf_counter() - t0) / reps
+                rt[key] = {"ms": ms, "GB_s": 8.0 * w * N / (ms * 1e-3) / 1e9}
+            for which in (1, 2, 3):
+                run.poisson(which)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    run.poisson(which)
+                ms = 1e3 * (time.perf_counter() - t0) / reps
+                k1 = np_ - 1 if which == 2 else np_
+                m = (nr - (which == 3)) * (nz - (which == 1))
+                flops = 2 * (2.0 * k1 * k1 * m) + 2 * (2.0 * nz * nz * nr * k1)
+                rt[f"poisson{which}"] = {"ms": ms, "transform_TFLOP_s": flops / (ms * 1e-3) / 1e12}
+            out[name]["routines"] = rt
+        run.close()
     return out
 
 
@@ -781,6 +861,10 @@ def main():
                     help="measure only the large V-cycle (FP64, mixed precision, direct) and print its JSON object")
     ap.add_argument("--rot3d-only", action="store_true",
                     help="measure only the 3D annulus step (rot3dmg.f) and print its JSON object")
+    ap.add_argument("--sector-only", action="store_true",
+                    help="measure only the barrier-sector step (sect360v2mg.f) and print its JSON object")
+    ap.add_argument("--sector-grid", default=None, choices=sorted(SECTOR),
+                    help="with --sector-only: time this grid alone (for profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device copy of this size (counter calibration)")
     a = ap.parse_args()
@@ -790,6 +874,9 @@ def main():
         sys.exit(2)
     if a.rot3d_only:
         print(json.dumps(rot3d_leg(cpu=not a.no_cpu and not a.rot3d_grid, only=a.rot3d_grid)))
+        return
+    if a.sector_only:
+        print(json.dumps(sector_leg(only=a.sector_grid)))
         return
     if a.serve_only:
         print(json.dumps(serve_leg(slots=a.slots, baseline=not a.no_baseline)))
@@ -843,6 +930,8 @@ def main():
         done("large V-cycle")
         line["rot3d"] = rot3d_leg(cpu=not a.no_cpu)
         done("3D annulus step")
+        line["sector"] = sector_leg()
+        done("barrier-sector step")
         line["served_sweep"] = serve_leg(slots=a.slots)
         done("served sweep")
     if not a.no_cpu:

@@ -549,25 +549,26 @@ __global__ void __launch_bounds__(256) ks_rhs(Geo g, SF F)
     else d_rhsu(g, F, i, j, k);
 }
 
-// heat, sect360v2mg.f:840-881: the flux of each plane k = 0..np+1 (a thread
-// each, its z sum in the reference's order) ...
-__global__ void __launch_bounds__(64) ks_heat_planes(Geo g, const double *__restrict__ th, double *__restrict__ tr)
+// heat, sect360v2mg.f:840-881: the flux of each plane k = 0..np+1, a
+// workgroup each: its z terms in parallel into LDS, then their sum in the
+// reference's order by one thread ...
+__global__ void __launch_bounds__(256) ks_heat_planes(Geo g, const double *__restrict__ th, double *__restrict__ tr)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > g.np + 1) return;
-    const int nz = g.nz;
+    extern __shared__ double term[];   // nz+2 terms
+    const int k = blockIdx.x, nz = g.nz;
     const double dr = g.dr, dz = g.dz;
-    double trans = dz * (-184.0 * X(th, 0, 0, k) + 225.0 * X(th, 1, 0, k) - 50.0 * X(th, 2, 0, k) +
-                         9.0 * X(th, 3, 0, k)) / (240.0 * dr);
-    trans = trans + dz * (-184.0 * X(th, 0, 1, k) + 225.0 * X(th, 1, 1, k) - 50.0 * X(th, 2, 1, k) +
-                          9.0 * X(th, 3, 1, k)) / (80.0 * dr);
-    for (int j = 2; j <= nz - 1; ++j)
-        trans = trans + dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) +
-                              9.0 * X(th, 3, j, k)) / (60.0 * dr);
-    trans = trans + dz * (-184.0 * X(th, 0, nz, k) + 225.0 * X(th, 1, nz, k) - 50.0 * X(th, 2, nz, k) +
-                          9.0 * X(th, 3, nz, k)) / (80.0 * dr);
-    trans = trans + dz * (-184.0 * X(th, 0, nz + 1, k) + 225.0 * X(th, 1, nz + 1, k) -
-                          50.0 * X(th, 2, nz + 1, k) + 9.0 * X(th, 3, nz + 1, k)) / (240.0 * dr);
+    for (int j = threadIdx.x; j <= nz + 1; j += blockDim.x) {
+        const double w = (j == 0 || j == nz + 1) ? 240.0 : (j == 1 || j == nz) ? 80.0 : 60.0;
+        term[j] = dz * (-184.0 * X(th, 0, j, k) + 225.0 * X(th, 1, j, k) - 50.0 * X(th, 2, j, k) +
+                        9.0 * X(th, 3, j, k)) / (w * dr);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double trans = term[0];
+    trans = trans + term[1];
+    for (int j = 2; j <= nz - 1; ++j) trans = trans + term[j];
+    trans = trans + term[nz];
+    trans = trans + term[nz + 1];
     tr[k] = trans;
 }
 
@@ -769,7 +770,25 @@ __global__ void __launch_bounds__(64) ks_rsolve(double *__restrict__ T, double *
     y /= d0;
     c[0] = cprev;
     x[0] = y;
-    for (int i = 1; i < i1; ++i) {
+    // rows 1 .. i1-2 in groups of U with their loads issued ahead
+    constexpr int U = 8;
+    int i = 1;
+    for (; i + U <= i1 - 1; i += U) {
+        double rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) rv[u] = x[(size_t)(i + u) * j1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t o = (size_t)(i + u) * j1;
+            const double a = sub[i + u];
+            const double den = diag(i + u) - a * cprev;
+            cprev = sup[i + u] / den;
+            y = (rv[u] - a * y) / den;
+            c[o] = cprev;
+            x[o] = y;
+        }
+    }
+    for (; i < i1; ++i) {
         const size_t o = (size_t)i * j1;
         double a = sub[i], rv = x[o];
         if (i == i1 - 1 && en != 0.0) {
@@ -783,7 +802,22 @@ __global__ void __launch_bounds__(64) ks_rsolve(double *__restrict__ T, double *
         c[o] = cprev;
         x[o] = y;
     }
-    for (int i = i1 - 2; i >= 0; --i) {
+    i = i1 - 2;
+    for (; i - U + 1 >= 0; i -= U) {
+        double fu[U], xu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t o = (size_t)(i - u) * j1;
+            fu[u] = c[o];
+            xu[u] = x[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            y = xu[u] - fu[u] * y;
+            x[(size_t)(i - u) * j1] = y;
+        }
+    }
+    for (; i >= 0; --i) {
         const size_t o = (size_t)i * j1;
         y = x[o] - c[o] * y;
         x[o] = y;
@@ -981,8 +1015,8 @@ static int launch_explicit(vvs_run *r, int mask)
 
 static int launch_heat(vvs_run *r)
 {
-    hipLaunchKernelGGL(ks_heat_planes, dim3((unsigned)((r->g.np + 2 + 63) / 64)), dim3(64), 0, r->stream, r->g,
-                       (const double *)r->f[VV3_TH], r->heat);
+    hipLaunchKernelGGL(ks_heat_planes, dim3((unsigned)(r->g.np + 2)), dim3(256), (size_t)(r->g.nz + 2) * sizeof(double),
+                       r->stream, r->g, (const double *)r->f[VV3_TH], r->heat);
     hipLaunchKernelGGL(ks_heat_sum, dim3(1), dim3(64), 0, r->stream, r->g, r->heat);
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1061,8 +1095,8 @@ int vvs_create(const vv3_config *cfg, int device, vvs_run **out)
     using vv::set_error;
     if (!cfg || !out) { set_error("vvs_create: null argument"); return VV_EINVAL; }
     *out = nullptr;
-    if (cfg->nr < 4 || cfg->nz < 4 || cfg->np < 4 || cfg->np > VVS_NP_MAX) {
-        set_error("vvs_create: need nr >= 4, nz >= 4 and 4 <= np <= VVS_NP_MAX");
+    if (cfg->nr < 4 || cfg->nz < 4 || cfg->nz > VVS_NZ_MAX || cfg->np < 4 || cfg->np > VVS_NP_MAX) {
+        set_error("vvs_create: need nr >= 4, 4 <= nz <= VVS_NZ_MAX and 4 <= np <= VVS_NP_MAX");
         return VV_EINVAL;
     }
     if (!(cfg->dt > 0) || !(cfg->rmax > cfg->r0) || !(cfg->r0 > 0) || !(cfg->zmax > 0) || !(cfg->Ptl > 0) ||

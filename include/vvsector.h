@@ -24,13 +24,15 @@ extern "C" {
 
 typedef struct vvs_run vvs_run;
 
-/* largest np (the phi transforms are dense tables of np^2 doubles) */
+/* largest np (the phi transforms are dense tables of np^2 doubles) and nz
+ * (a plane's heat-flux terms are summed in LDS) */
 #define VVS_NP_MAX 2048
+#define VVS_NZ_MAX 4096
 
 /* Allocates the fields (zero) and the Poisson solver on `device` (the
  * reference builds its multigrid levels in poissoninit,
- * sect360v2mg.f:1302).  Needs nr, nz >= 4 and 4 <= np <= VVS_NP_MAX
- * (VV_EINVAL otherwise). */
+ * sect360v2mg.f:1302).  Needs nr >= 4, 4 <= nz <= VVS_NZ_MAX and
+ * 4 <= np <= VVS_NP_MAX (VV_EINVAL otherwise). */
 int vvs_create(const vv3_config *cfg, int device, vvs_run **out);
 int vvs_destroy(vvs_run *run);
 

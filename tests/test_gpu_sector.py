@@ -14,10 +14,16 @@ tests/test_oracle_sector.py):
   residual operator to rounding and the explicit routines of the next step
   equal the oracle's bit for bit.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle_sector as OS
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from bench import sector_state  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -40,21 +46,6 @@ def _pair(nr, nz, np_, seed, scale=1.0, phys=PHYS):
         ora[n] = a
         run.put(n, a)
     return run, ora
-
-
-def sector_state(nr, nz, np_, r0=0.5, rmax=1.0, amp=1e-3, nval=2):
-    """sect360v2mg.f's initial temperature (sect360v2mg.f:94-123): the
-    conduction profile 1 - ln r / ln r0 (rmax = 1), a mode nval of
-    amplitude amp, and the cold line th(nr/4, j, np/2) - 0.2, j = 1..nz"""
-    dr = (rmax - r0) / nr
-    r = r0 + (np.arange(nr + 2) - 0.5) * dr
-    r[0], r[-1] = r0, rmax
-    phi = 8.0 * np.arctan(1.0) * np.arange(np_ + 2) / np_
-    th = (1.0 - np.log(r) / np.log(r0))[None, None, :] + \
-        amp * ((rmax - r) * (r - r0))[None, None, :] * np.cos(nval * phi)[:, None, None]
-    th = np.broadcast_to(th, (np_ + 2, nz + 2, nr + 2)).copy()
-    th[np_ // 2, 1:nz + 1, nr // 4] -= 0.2
-    return {"th": th, "thp": th.copy()}
 
 
 @pytest.mark.parametrize("shape", [(6, 6, 8), (9, 7, 13), (16, 12, 24)])
