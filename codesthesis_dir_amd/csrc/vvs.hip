@@ -17,7 +17,9 @@
 // difference with zero-flux faces (w, u: eigenvectors cos(m pi (k-1/2)/np),
 // m = 0..np-1) or with zero values on the faces (v: sin(m pi k/np),
 // m = 1..np-1), both with eigenvalues 2 cos(m pi/np) - 2, so a solve is
-//   1. the phi transform as a GEMM with the eigenvector table,
+//   1. the phi transform: by FFT in LDS for np a power of two (8..2048, the
+//      tank's y transforms, vv_ytrans.h), else a GEMM with the eigenvector
+//      table,
 //   2. the eigenbasis of the z rows (a batched GEMM),
 //   3. one banded system in r per (phi mode, z eigenvalue), a thread each,
 //   4. the inverse z and phi transforms,
@@ -26,6 +28,7 @@
 
 #include "vv3_internal.h"
 #include "vv_gemm.h"
+#include "vv_ytrans.h"
 #include "../../include/vvsector.h"
 
 #include <cmath>
@@ -62,6 +65,8 @@ struct SOp {
 struct SPoisson {
     SOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;
+    bool fft = false;     // phi transforms by FFT (np a power of two, 8..2048) instead of the W / Wi GEMMs
+    vvt::YTrans yt;
 };
 
 namespace {
@@ -880,6 +885,11 @@ int spoisson_setup(SPoisson &P, const Geo &g)
     const double pi = std::acos(-1.0);
     const double hhz = g.dz * g.dz, hhp = g.dp * g.dp;
     size_t smax = 0, cmax = 0;
+    P.fft = vvt::ytrans_fits(g.np);
+    if (P.fft) {
+        const int st = vvt::ytrans_setup(P.yt, g.np, g.ldi, g.ldj);
+        if (st) return st;
+    }
     for (int w = 1; w <= 3; ++w) {
         SOp &O = P.op[w - 1];
         ranges(g, w, O.i1, O.j1, O.k1);
@@ -917,8 +927,10 @@ int spoisson_setup(SPoisson &P, const Geo &g)
             return VV_EINVAL;
         }
         for (double &v : lam) v /= hhz;
-        VV_HIP(upload(&O.W, W));
-        VV_HIP(upload(&O.Wi, Wi));
+        if (!P.fft) {
+            VV_HIP(upload(&O.W, W));
+            VV_HIP(upload(&O.Wi, Wi));
+        }
         VV_HIP(upload(&O.lphi, lphi));
         VV_HIP(upload(&O.V, V));
         VV_HIP(upload(&O.Vinv, Vinv));
@@ -938,6 +950,7 @@ void spoisson_free(SPoisson &P)
     for (double *p : {P.S, P.T, P.cp})
         if (p) (void)hipFree(p);
     P.S = P.T = P.cp = nullptr;
+    vvt::ytrans_free(P.yt);
     for (auto &O : P.op) {
         for (double *p : {O.W, O.Wi, O.lphi, O.V, O.Vinv, O.lz, O.rt})
             if (p) (void)hipFree(p);
@@ -952,12 +965,14 @@ int spoisson_launch(const SPoisson &P, const Geo &g, int which, double *psi, dou
     const int i1 = O.i1, j1 = O.j1, k1 = O.k1, mt = O.m;
     const dim3 pg((unsigned)((g.ldi * g.ldj + 255) / 256), (unsigned)(g.np + 2));
     hipLaunchKernelGGL(ks_psi_pre, pg, dim3(256), 0, s, g, which, (const double *)psi, psip);
-    vv::gemm_f64<true, false>(PhiFwd{O.W, k1, mt, k1, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
+    if (P.fft) vvt::ytrans_fwd(P.yt, which == 2, rhs, P.S, i1, mt, s);
+    else vv::gemm_f64<true, false>(PhiFwd{O.W, k1, mt, k1, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
     vv::gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, k1, s);
     hipLaunchKernelGGL(ks_rsolve, dim3((unsigned)((k1 * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)O.lphi, i1, j1, k1, O.e1, O.en);
     vv::gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, k1, s);
-    vv::gemm_f64<true, false>(PhiInv{O.Wi, k1, mt, k1, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+    if (P.fft) vvt::ytrans_inv(P.yt, which == 2, P.S, psi, i1, mt, s);
+    else vv::gemm_f64<true, false>(PhiInv{O.Wi, k1, mt, k1, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
     hipLaunchKernelGGL(ks_psi_post, pg, dim3(256), 0, s, g, which, psi);
     VV_HIP(hipGetLastError());
     return VV_OK;

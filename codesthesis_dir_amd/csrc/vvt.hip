@@ -23,6 +23,7 @@
 #include "vv_fft.h"
 #include "vv_gemm.h"
 #include "vv_reduce.h"
+#include "vv_ytrans.h"
 
 #include "../../include/vvtank.h"
 
@@ -1207,6 +1208,73 @@ __global__ void __launch_bounds__(WG) k_t_energy_sum(const double *part, int nb,
 }
 
 }  // namespace
+
+// the mode transforms of vv_ytrans.h: the kernels above with the tank's
+// launch shape (2 LP lines a workgroup of 512 threads, LP = min(4, 2048/np))
+bool ytrans_fits(int np) { return np >= 8 && np <= 2048 && (np & (np - 1)) == 0; }
+
+int ytrans_setup(YTrans &T, int np, int ldi, int ldj)
+{
+    if (!ytrans_fits(np)) {
+        vv::set_error("mode transform by FFT: np must be a power of two, 8..2048");
+        return VV_EINVAL;
+    }
+    const double pi = std::acos(-1.0);
+    const int N2 = 2 * np;
+    T.np = np;
+    T.ldi = ldi;
+    T.ldj = ldj;
+    T.bits = __builtin_ctz((unsigned)N2);
+    T.lp = std::max(1, std::min(4, 4096 / N2));
+    std::vector<double2> t(2 * (size_t)np);
+    for (int q = 0; q < np; ++q) {
+        t[q] = make_double2(std::cos(2.0 * pi * q / N2), -std::sin(2.0 * pi * q / N2));
+        t[np + q] = make_double2(std::cos(pi * q / N2), std::sin(pi * q / N2));
+    }
+    VV_HIP(hipMalloc(&T.tw, t.size() * sizeof(double2)));
+    VV_HIP(hipMemcpy(T.tw, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+    return VV_OK;
+}
+
+void ytrans_free(YTrans &T)
+{
+    if (T.tw) (void)hipFree(T.tw);
+    T.tw = nullptr;
+}
+
+static TGeo ytrans_geo(const YTrans &T)
+{
+    TGeo g{};
+    g.np = T.np;
+    g.ldi = T.ldi;
+    g.ldj = T.ldj;
+    return g;
+}
+
+void ytrans_fwd(const YTrans &T, bool faces, const double *rhs, double *S, int i1, int mt, hipStream_t s)
+{
+    const int LP = T.lp, nbk = (mt + 2 * LP - 1) / (2 * LP);
+    const size_t flds = ((size_t)LP * 2 * T.np + T.np) * sizeof(double2);
+    const TGeo g = ytrans_geo(T);
+    const double2 *tw = T.tw, *cq = T.tw + T.np;
+    if (faces)
+        hipLaunchKernelGGL(k_t_yfft_fwd<true>, dim3(nbk), dim3(512), flds, s, g, rhs, S, tw, cq, i1, mt, LP, T.bits);
+    else
+        hipLaunchKernelGGL(k_t_yfft_fwd<false>, dim3(nbk), dim3(512), flds, s, g, rhs, S, tw, cq, i1, mt, LP, T.bits);
+}
+
+void ytrans_inv(const YTrans &T, bool faces, const double *S, double *psi, int i1, int mt, hipStream_t s)
+{
+    const int LP = T.lp, nbk = (mt + 2 * LP - 1) / (2 * LP);
+    const size_t flds = ((size_t)LP * 2 * T.np + T.np) * sizeof(double2);
+    const TGeo g = ytrans_geo(T);
+    const double2 *tw = T.tw, *cq = T.tw + T.np;
+    if (faces)
+        hipLaunchKernelGGL(k_t_yfft_inv<true>, dim3(nbk), dim3(512), flds, s, g, S, psi, tw, cq, i1, mt, LP, T.bits);
+    else
+        hipLaunchKernelGGL(k_t_yfft_inv<false>, dim3(nbk), dim3(512), flds, s, g, S, psi, tw, cq, i1, mt, LP, T.bits);
+}
+
 }  // namespace vvt
 
 using namespace vvt;

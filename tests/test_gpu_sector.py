@@ -85,8 +85,9 @@ def test_single_routines_match_the_fused_launch():
 
 
 @pytest.mark.parametrize("which", [1, 2, 3])
-@pytest.mark.parametrize("shape", [(6, 5, 8), (9, 7, 12), (12, 6, 9)])
+@pytest.mark.parametrize("shape", [(6, 5, 8), (9, 7, 12), (12, 6, 9), (8, 6, 32)])
 def test_poisson_matches_dense_solve(which, shape):
+    """np = 8, 32: the phi transforms by FFT; 12, 9: by GEMM."""
     nr, nz, np_ = shape
     print(f"[gpu] sector poisson {which} {shape}", flush=True)
     run, ora = _pair(nr, nz, np_, seed=which + nr)
