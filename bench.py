@@ -446,7 +446,7 @@ def sector_leg(warmup=3, only=None, reps=10):
             N = (nr + 2) * (nz + 2) * (np_ + 2)
             calls = {"vorticity_abc": (lambda: run.vorticity(0), 27), "temperature": (run.temperature, 6),
                      "rhsw": (lambda: run.rhs(1), 3), "rhsv": (lambda: run.rhs(2), 3),
-                     "rhsu": (lambda: run.rhs(3), 4), "update": (run.update, 24)}
+                     "rhsu": (lambda: run.rhs(3), 4), "update": (run.update, 8)}
             rt = {}
             for key, (f, w) in calls.items():
                 f()

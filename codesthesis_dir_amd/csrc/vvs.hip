@@ -625,23 +625,21 @@ __global__ void __launch_bounds__(256) ks_update_walls(Geo g, SF F)
     else if (i == nr + 1) X(F.thn, nr + 1, j, k) = 1.0;
 }
 
-// ... then the time levels rotate over every point (sect360v2mg.f:981-993)
-struct Rot {
-    double *Ap, *A, *An, *Bp, *B, *Bn, *Cp, *C, *Cn, *thp, *th, *thn;
+// ... then the time levels rotate (sect360v2mg.f:981-993): Ap <- A <- An by
+// exchanging the buffers, and the old Ap buffer, which becomes An, receives
+// a copy of the new A, so every level holds what the reference's copies
+// leave (An keeps its values where no routine writes)
+struct Copy4 {
+    const double *src[4];
+    double *dst[4];
 };
 
-__global__ void __launch_bounds__(256) ks_rotate(Rot R, size_t n)
+__global__ void __launch_bounds__(256) ks_copy4(Copy4 C, size_t n)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    R.Ap[t] = R.A[t];
-    R.A[t] = R.An[t];
-    R.Bp[t] = R.B[t];
-    R.B[t] = R.Bn[t];
-    R.Cp[t] = R.C[t];
-    R.C[t] = R.Cn[t];
-    R.thp[t] = R.th[t];
-    R.th[t] = R.thn[t];
+    const int q = blockIdx.y;
+    C.dst[q][t] = C.src[q][t];
 }
 
 // ---------------------------------------------------------------- Poisson
@@ -1060,9 +1058,18 @@ static int launch_update(vvs_run *r)
 {
     const SF F = sfields(r);
     hipLaunchKernelGGL(ks_update_walls, pgrid(r), dim3(256), 0, r->stream, r->g, F);
-    Rot R{r->f[VV3_AP], r->f[VV3_A], r->f[VV3_AN], r->f[VV3_BP], r->f[VV3_B], r->f[VV3_BN],
-          r->f[VV3_CP], r->f[VV3_C], r->f[VV3_CN], r->f[VV3_THP], r->f[VV3_TH], r->f[VV3_THN]};
-    hipLaunchKernelGGL(ks_rotate, dim3((unsigned)((r->nf + 255) / 256)), dim3(256), 0, r->stream, R, r->nf);
+    static const int LV[4][3] = {{VV3_AP, VV3_A, VV3_AN}, {VV3_BP, VV3_B, VV3_BN}, {VV3_CP, VV3_C, VV3_CN},
+                                 {VV3_THP, VV3_TH, VV3_THN}};
+    Copy4 C;
+    for (int q = 0; q < 4; ++q) {
+        double *old_p = r->f[LV[q][0]];
+        r->f[LV[q][0]] = r->f[LV[q][1]];
+        r->f[LV[q][1]] = r->f[LV[q][2]];
+        r->f[LV[q][2]] = old_p;
+        C.src[q] = r->f[LV[q][1]];
+        C.dst[q] = r->f[LV[q][2]];
+    }
+    hipLaunchKernelGGL(ks_copy4, dim3((unsigned)((r->nf + 255) / 256), 4), dim3(256), 0, r->stream, C, r->nf);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
