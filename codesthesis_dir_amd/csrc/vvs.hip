@@ -9,8 +9,9 @@
 // Explicit part: one thread per point for Avorticity / Bvorticity /
 // Cvorticity / temperature, the Poisson right-hand sides and the wall
 // update, evaluating the reference's expressions in its operation order
-// (the library is compiled with -ffp-contract=off): the results equal the
-// reference's arithmetic bit for bit.
+// (the library is compiled with -ffp-contract=off): the results equal those
+// of the C restatement of the reference's routines bit for bit (the
+// reference itself does not compile without its include files).
 //
 // Poisson: the three operators of relax1/2/3 (sect360v2mg.f:1817-1848,
 // 2034-2065, 2233-2261) are separable.  Their phi rows are the second
@@ -24,7 +25,7 @@
 //   3. one banded system in r per (phi mode, z eigenvalue), a thread each,
 //   4. the inverse z and phi transforms,
 // exact to rounding where the reference iterates multigrid cycles to an
-// absolute residual of 1e-5 (tolcon, sect360v2mg.f:1669).
+// absolute residual of 1e-5 (tolcon, sect360v2mg.f:1668).
 
 #include "vv3_internal.h"
 #include "vv_gemm.h"

@@ -61,7 +61,7 @@ int vvs_rhs(vvs_run *run, int which);
  * for wel / vel / uel from wrh / vrh / urh exactly; psip <- old psi, then
  * the walls and the mirrored ghost values the reference writes.  The
  * reference iterates to an absolute residual of 1e-5 (tolcon,
- * sect360v2mg.f:1669); this solve is exact to rounding. */
+ * sect360v2mg.f:1668); this solve is exact to rounding. */
 int vvs_poisson(vvs_run *run, int which);
 /* update (sect360v2mg.f:936): wall vorticity on the end walls, the
  * cylinders and the barrier's faces (sect360v2mg.f:964-974), wall
