@@ -587,6 +587,22 @@ __global__ void __launch_bounds__(64) ks_heat_sum(Geo g, double *tr)
     tr[g.np + 2] = s;
 }
 
+// Local Nusselt numbers at mid-height, the output of sect360v2mg.f:318-325:
+// a thread per plane k = 0..np, in the reference's operation order (L =
+// ln(r0) from the host's C library, as the reference's dlog)
+__global__ void __launch_bounds__(64) ks_local_nu(Geo g, const double *__restrict__ th, double rmax, double L,
+                                                  double *__restrict__ inner, double *__restrict__ outer)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > g.np) return;
+    const int nr = g.nr, jm = g.nz / 2;
+    const double dr = g.dr;
+    inner[k] = -(g.r0 * L * (-184.0 * X(th, 0, jm, k) + 225.0 * X(th, 1, jm, k) - 50.0 * X(th, 2, jm, k) +
+                             9.0 * X(th, 3, jm, k)) / (60.0 * dr));
+    outer[k] = -(rmax * L * (184.0 * X(th, nr + 1, jm, k) - 225.0 * X(th, nr, jm, k) + 50.0 * X(th, nr - 1, jm, k) -
+                             9.0 * X(th, nr - 2, jm, k)) / (60.0 * dr));
+}
+
 // update, sect360v2mg.f:936-979: wall vorticity of the end walls, the
 // cylinders and the barrier's faces, wall temperatures (no point is written
 // twice) ...
@@ -993,6 +1009,7 @@ struct vvs_run {
     double *f[VV3_NFIELD] = {nullptr};
     vvs::SPoisson pois;
     double *heat = nullptr;              // plane fluxes k = 0..np+1, then trans1
+    double *nu = nullptr;                // local Nusselt numbers: inner, outer (np+1 each)
 };
 
 using namespace vvs;
@@ -1147,6 +1164,7 @@ int vvs_create(const vv3_config *cfg, int device, vvs_run **out)
         if (hipMemsetAsync(r->f[q], 0, r->nf * sizeof(double), r->stream) != hipSuccess) return fail(VV_EHIP);
     }
     if (hipMalloc(&r->heat, (size_t)(cfg->np + 3) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
+    if (hipMalloc(&r->nu, 2 * (size_t)(cfg->np + 1) * sizeof(double)) != hipSuccess) return fail(VV_ENOMEM);
     if (hipMemsetAsync(r->heat, 0, (size_t)(cfg->np + 3) * sizeof(double), r->stream) != hipSuccess)
         return fail(VV_EHIP);
     {
@@ -1169,6 +1187,7 @@ int vvs_destroy(vvs_run *r)
             if (r->f[q]) (void)hipFree(r->f[q]);
         spoisson_free(r->pois);
         if (r->heat) (void)hipFree(r->heat);
+        if (r->nu) (void)hipFree(r->nu);
         if (r->stream) (void)hipStreamDestroy(r->stream);
     }
     delete r;
@@ -1216,6 +1235,21 @@ int vvs_heat(vvs_run *r, double *trans1)
     VV_HIP(hipSetDevice(r->device));
     VVS_TRY(launch_heat(r));
     VV_HIP(hipMemcpyAsync(trans1, r->heat + r->g.np + 2, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipStreamSynchronize(r->stream));
+    return VV_OK;
+}
+
+int vvs_local_nu(vvs_run *r, double *inner, double *outer)
+{
+    if (!r || !inner || !outer) { vv::set_error("vvs_local_nu: null argument"); return VV_EINVAL; }
+    std::lock_guard<std::recursive_mutex> lk(r->mu);
+    VV_HIP(hipSetDevice(r->device));
+    const int np = r->g.np;
+    hipLaunchKernelGGL(ks_local_nu, dim3((unsigned)((np + 1 + 63) / 64)), dim3(64), 0, r->stream, r->g,
+                       (const double *)r->f[VV3_TH], r->cfg.rmax, std::log(r->cfg.r0), r->nu, r->nu + np + 1);
+    VV_HIP(hipGetLastError());
+    VV_HIP(hipMemcpyAsync(inner, r->nu, (np + 1) * sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    VV_HIP(hipMemcpyAsync(outer, r->nu + np + 1, (np + 1) * sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }

@@ -53,6 +53,7 @@ def _bind():
     for fn in ("vvs_vorticity", "vvs_rhs", "vvs_poisson"):
         getattr(L, fn).argtypes = [vp, ctypes.c_int]
     L.vvs_heat.argtypes = [vp, _P]
+    L.vvs_local_nu.argtypes = [vp, _P, _P]
     L.vvs_step.argtypes = [vp, ctypes.c_int, _P]
     L._vvs_bound = True
     return L
@@ -102,6 +103,12 @@ class Sector:
         t = ctypes.c_double()
         check(self._lib.vvs_heat(self._h, ctypes.byref(t)), "vvs_heat")
         return t.value
+
+    def local_nu(self):
+        """(inner, outer) local Nusselt numbers at mid-height, k = 0..np."""
+        a, b = np.empty(self.cfg.np + 1), np.empty(self.cfg.np + 1)
+        check(self._lib.vvs_local_nu(self._h, a.ctypes.data_as(_P), b.ctypes.data_as(_P)), "vvs_local_nu")
+        return a, b
 
     def rhs(self, which: int):
         check(self._lib.vvs_rhs(self._h, which), "vvs_rhs")

@@ -54,6 +54,12 @@ int vvs_temperature(vvs_run *run);
 /* heat (sect360v2mg.f:840): the inner-wall flux trans1 over every plane
  * k = 0..np+1 */
 int vvs_heat(vvs_run *run, double *trans1);
+/* Local Nusselt numbers at mid-height of both cylinders, one per plane
+ * k = 0..np (inner and outer hold np+1 values), as the reference writes
+ * them with its fields (sect360v2mg.f:318-325), at j = nz/2:
+ *   inner[k] = -r0 ln(r0) (-184 T0 + 225 T1 - 50 T2 + 9 T3) / (60 dr),
+ *   outer[k] = -rmax ln(r0) (184 T(nr+1) - 225 T(nr) + 50 T(nr-1) - 9 T(nr-2)) / (60 dr). */
+int vvs_local_nu(vvs_run *run, double *inner, double *outer);
 /* rhsw / rhsv / rhsu (sect360v2mg.f:999, 1034, 1066): which = 1 / 2 / 3 */
 int vvs_rhs(vvs_run *run, int which);
 /* poisson1 / poisson2 / poisson3 (sect360v2mg.f:1357, 1462, 1559): solves

@@ -185,3 +185,23 @@ def test_sector_129x129x257_grid_properties():
         ora[psi] = got
         ora[psip] = pp
     run.close()
+
+
+def test_local_nu_matches_the_pinned_formula():
+    """vvs_local_nu equals the local Nusselt numbers of the 3D oracle
+    (v3_local_nu, the same formula of sect360v2mg.f:318-325, pinned by
+    conduction in tests/test_oracle_diagnostics.py) on the same temperature,
+    planes k = 0..np, bit for bit"""
+    import oracle3 as O3
+    nr, nz, np_ = 12, 10, 16
+    print("[gpu] sector local Nu", flush=True)
+    run = _gpu(nr, nz, np_)
+    th = sector_state(nr, nz, np_)["th"]
+    th = th + 1e-3 * np.random.default_rng(4).standard_normal(th.shape)
+    run.put("th", th)
+    a, b = run.local_nu()
+    ora = O3.Run3(nr, nz, np_ + 1, **PHYS)
+    ora["th"] = th
+    oa, ob = ora.local_nu()
+    assert np.array_equal(a, oa[:np_ + 1]) and np.array_equal(b, ob[:np_ + 1])
+    run.close()
