@@ -107,6 +107,44 @@ def test_poisson_matches_dense_solve(which, shape):
     run.close()
 
 
+@pytest.mark.parametrize("which", [1, 2, 3])
+@pytest.mark.parametrize("shape", [(4, 4, 2048), (6, 5, 1024), (5, 4, 4), (7, 5, 5)])
+def test_poisson_residual_at_size_limits(which, shape):
+    """Where the dense solve is out of reach (np = 1024, 2048: the FFT path
+    at its largest sizes) and at the smallest np (4, 5: the GEMM path): the
+    reference's residual operator vanishes to rounding, psip is the old psi
+    and the barrier planes are as the reference writes them."""
+    nr, nz, np_ = shape
+    print(f"[gpu] sector poisson limits {which} {shape}", flush=True)
+    run, ora = _pair(nr, nz, np_, seed=17 + which + np_)
+    psi, psip, rhs = OS.PSI[which]
+    old = ora[psi].copy()
+    run.poisson(which)
+    got = run.get(psi)
+    m, _ = ora.residual(which, got, ora[rhs])
+    h = min(ora.p.dr, ora.p.dz, ora.p.r0 * ora.p.dp)
+    scale = np.max(np.abs(ora[rhs])) + np.max(np.abs(got)) * 8.0 / h ** 2
+    assert m <= 1e-13 * scale, (m, scale)
+    ie, je, ke = {1: (nr + 1, nz, np_ + 1), 2: (nr + 1, nz + 1, np_), 3: (nr, nz + 1, np_ + 1)}[which]
+    assert np.array_equal(run.get(psip)[:ke + 1, :je + 1, :ie + 1], old[:ke + 1, :je + 1, :ie + 1])
+    if which == 2:
+        assert np.all(got[0] == 0.0) and np.all(got[np_] == 0.0)
+    else:
+        i1 = nr if which == 1 else nr - 1
+        j1 = nz - 1 if which == 1 else nz
+        assert np.array_equal(got[0, 1:j1 + 1, 1:i1 + 1], -got[1, 1:j1 + 1, 1:i1 + 1])
+        assert np.array_equal(got[np_ + 1, 1:j1 + 1, 1:i1 + 1], -got[np_, 1:j1 + 1, 1:i1 + 1])
+    run.close()
+
+
+def test_create_rejects_out_of_range_sizes():
+    from codesthesis_dir_amd._lib import VVError
+    from codesthesis_dir_amd.sector import Sector, SectorConfig
+    for nr, nz, np_ in ((3, 6, 8), (6, 3, 8), (6, 6, 3), (6, 6, 2049), (6, 4097, 8)):
+        with pytest.raises(VVError):
+            Sector(SectorConfig(nr=nr, nz=nz, np=np_, **PHYS))
+
+
 def _oracle_run(nr, nz, np_, phys):
     ora = OS.Sector(nr, nz, np_, **phys)
     for k, v in sector_state(nr, nz, np_).items():
