@@ -9,7 +9,9 @@
 // Explicit part: one thread per point for Avorticity / Bvorticity /
 // Cvorticity / temperature, the Poisson right-hand sides and the wall
 // update, evaluating the reference's expressions in its operation order
-// (the library is compiled with -ffp-contract=off): the results equal those
+// (the library is compiled with -ffp-contract=off; divisions by constants
+// and radii through their correctly rounded reciprocals, which returns the
+// IEEE quotient): the results equal those
 // of the C restatement of the reference's routines bit for bit (the
 // reference itself does not compile without its include files).
 //
@@ -51,6 +53,52 @@ struct SF {
     double *An, *Bn, *Cn, *thn, *urh, *vrh, *wrh;
 };
 
+// A positive divisor with its correctly rounded reciprocal.  x / b is
+// evaluated as q0 = x rb, t = fma(b, q0, -x) = b q0 - x (exact),
+// q = fma(-t, rb, q0): for rb = RN(1/b) this is the correctly rounded
+// quotient (Markstein's correction), the IEEE quotient of the reference's
+// division, in three instructions.  Forming -t gives a zero quotient the
+// sign IEEE division gives it.
+struct Dv {
+    double v, r;
+};
+__device__ __forceinline__ double operator/(double x, Dv b)
+{
+    const double q0 = x * b.r;
+    const double t = __builtin_fma(b.v, q0, -x);
+    return __builtin_fma(-t, b.r, q0);
+}
+// the same for a divisor of either sign (the exact case kept apart)
+struct Dvs {
+    double v, r;
+};
+__device__ __forceinline__ double operator/(double x, Dvs b)
+{
+    const double q0 = x * b.r;
+    const double e = __builtin_fma(-b.v, q0, x);
+    return (e == 0.0) ? q0 : __builtin_fma(e, b.r, q0);
+}
+
+// the constant divisors of the explicit routines, each formed on the host as
+// the reference's expression forms it (dr*dr, Ptl*dt, 2*Ptl, 2*dp)
+struct SK {
+    Dv dr, dz, dp, dr2, dz2, dp2, three, ptl, ptl2, ptldt, dt, dp2x;
+    Dvs re;
+};
+
+// a point's radius r with r*r and 2*(r*r) as divisors; RN(1/(2 q)) is
+// RN(1/q)/2 exactly
+struct Dr {
+    Dv r, r2, r2h;
+    __device__ explicit Dr(double rv)
+    {
+        const double q = rv * rv, iq = 1.0 / q;
+        r = Dv{rv, 1.0 / rv};
+        r2 = Dv{q, iq};
+        r2h = Dv{2.0 * q, 0.5 * iq};
+    }
+};
+
 // one sector Poisson operator (1: w, 2: v, 3: u): unknowns i = 1..i1,
 // j = 1..j1, k = 1..k1
 struct SOp {
@@ -77,22 +125,23 @@ namespace {
 
 // thh of the row kk of temperature's interior branch (sect360v2mg.f:793-805):
 // the value the reference's row k = np reuses
-__device__ __forceinline__ double thh_row(const Geo &g, const SF &F, int i, int j, int kk)
+__device__ __forceinline__ double thh_row(const Geo &g, const SF &F, const SK &K, int i, int j, int kk)
 {
     const double *th = F.th, *vel = F.vel;
     const double dp = g.dp;
     const double r = g.r0 + ((double)i - 0.5) * g.dr;
+    const Dr R(r);
     const double v = (j == 0 || j == g.nz + 1) ? 0.0 : (X(vel, i, j, kk) + X(vel, i, j, kk - 1)) * 0.5;
-    double thh = (X(th, i, j, kk + 1) - X(th, i, j, kk - 1)) / (2.0 * dp);
-    const double Pe = fabs(v * dp * r / g.Ptl);
+    double thh = (X(th, i, j, kk + 1) - X(th, i, j, kk - 1)) / K.dp2x;
+    const double Pe = fabs(v * dp * r / K.ptl);
     if (Pe > 2) {
-        if (v > 0) thh = (X(th, i, j, kk) - X(th, i, j, kk - 1)) / dp;
-        else if (v < 0) thh = (X(th, i, j, kk + 1) - X(th, i, j, kk)) / dp;
+        if (v > 0) thh = (X(th, i, j, kk) - X(th, i, j, kk - 1)) / K.dp;
+        else if (v < 0) thh = (X(th, i, j, kk + 1) - X(th, i, j, kk)) / K.dp;
     }
     return thh;
 }
 
-__device__ __forceinline__ void d_avort(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_avort(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -105,6 +154,7 @@ __device__ __forceinline__ void d_avort(const Geo &g, const SF &F, int i, int j,
 
     const int im = i - 1, jp = j + 1, km = k - 1, ka = k + 1;
     const double r = g.r0 + ((double)i - 0.5) * dr;
+    const Dr R(r);
     const double u = (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka) +
                       X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) *
                      0.125;
@@ -115,68 +165,68 @@ __device__ __forceinline__ void d_avort(const Geo &g, const SF &F, int i, int j,
     const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, im, j, k) + X(C, im, jp, k)) * 0.25;
     double Arra, Ara, Ar, cnr;
     if (i == 1) {
-        Arra = (3.2 * X(A, i - 1, j, k) + 2.0 * X(A, i + 1, j, k) - 0.2 * X(A, i + 2, j, k)) / (dr * dr);
-        Ara = (-4.0 * X(A, i - 1, j, k) / 3.0 + X(A, i + 1, j, k) / 3.0) / dr;
-        Ar = (-4.0 * X(A, i - 1, j, k) / 3.0 + X(A, i, j, k) + X(A, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) + 0.5 / dr / r;
+        Arra = (3.2 * X(A, i - 1, j, k) + 2.0 * X(A, i + 1, j, k) - 0.2 * X(A, i + 2, j, k)) / K.dr2;
+        Ara = (-4.0 * X(A, i - 1, j, k) / K.three + X(A, i + 1, j, k) / K.three) / K.dr;
+        Ar = (-4.0 * X(A, i - 1, j, k) / K.three + X(A, i, j, k) + X(A, i + 1, j, k) / K.three) / K.dr;
+        cnr = -2.5 / K.dr2 + 0.5 / K.dr / R.r;
     } else if (i == nr) {
-        Arra = (3.2 * X(A, i + 1, j, k) + 2.0 * X(A, i - 1, j, k) - 0.2 * X(A, i - 2, j, k)) / (dr * dr);
-        Ara = (-X(A, i - 1, j, k) / 3.0 + 4.0 * X(A, i + 1, j, k) / 3.0) / dr;
-        Ar = (-X(A, i - 1, j, k) / 3.0 - X(A, i, j, k) + 4.0 * X(A, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) - 0.5 / dr / r;
+        Arra = (3.2 * X(A, i + 1, j, k) + 2.0 * X(A, i - 1, j, k) - 0.2 * X(A, i - 2, j, k)) / K.dr2;
+        Ara = (-X(A, i - 1, j, k) / K.three + 4.0 * X(A, i + 1, j, k) / K.three) / K.dr;
+        Ar = (-X(A, i - 1, j, k) / K.three - X(A, i, j, k) + 4.0 * X(A, i + 1, j, k) / K.three) / K.dr;
+        cnr = -2.5 / K.dr2 - 0.5 / K.dr / R.r;
     } else {
-        Arra = (X(A, i - 1, j, k) + X(A, i + 1, j, k)) / (dr * dr);
-        Ara = (X(A, i + 1, j, k) - X(A, i - 1, j, k)) * 0.5 / dr;
+        Arra = (X(A, i - 1, j, k) + X(A, i + 1, j, k)) / K.dr2;
+        Ara = (X(A, i + 1, j, k) - X(A, i - 1, j, k)) * 0.5 / K.dr;
         Ar = Ara;
-        cnr = -1.0 / (dr * dr);
-        const double Pe = fabs(u * dr / Ptl);
+        cnr = -1.0 / K.dr2;
+        const double Pe = fabs(u * dr / K.ptl);
         if (Pe > 2) {
-            if (u > 0) Ar = (X(A, i, j, k) - X(A, i - 1, j, k)) / dr;
-            else if (u < 0) Ar = (X(A, i + 1, j, k) - X(A, i, j, k)) / dr;
+            if (u > 0) Ar = (X(A, i, j, k) - X(A, i - 1, j, k)) / K.dr;
+            else if (u < 0) Ar = (X(A, i + 1, j, k) - X(A, i, j, k)) / K.dr;
         }
     }
     const double uelr = ((X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, i, j, ka) + X(uel, i, jp, ka)) * 0.25 -
                          (X(uel, im, j, k) + X(uel, im, jp, k) + X(uel, im, j, ka) + X(uel, im, jp, ka)) *
                              0.25) /
                         dr;
-    const double Azza = (X(A, i, j - 1, k) + X(A, i, j + 1, k)) / (dz * dz);
-    double Az = (X(A, i, j + 1, k) - X(A, i, j - 1, k)) * 0.5 / dz;
+    const double Azza = (X(A, i, j - 1, k) + X(A, i, j + 1, k)) / K.dz2;
+    double Az = (X(A, i, j + 1, k) - X(A, i, j - 1, k)) * 0.5 / K.dz;
     {
-        const double Pe = fabs(w * dz / Ptl);
+        const double Pe = fabs(w * dz / K.ptl);
         if (Pe > 2) {
-            if (w > 0) Az = (X(A, i, j, k) - X(A, i, j - 1, k)) / dz;
-            else if (w < 0) Az = (X(A, i, j + 1, k) - X(A, i, j, k)) / dz;
+            if (w > 0) Az = (X(A, i, j, k) - X(A, i, j - 1, k)) / K.dz;
+            else if (w < 0) Az = (X(A, i, j + 1, k) - X(A, i, j, k)) / K.dz;
         }
     }
     const double uelz = ((X(uel, i, jp, k) + X(uel, i, jp, ka) + X(uel, im, jp, k) + X(uel, im, jp, ka)) * 0.25 -
                          (X(uel, i, j, k) + X(uel, i, j, ka) + X(uel, im, j, k) + X(uel, im, j, ka)) * 0.25) /
                         dz;
-    const double thz = (X(th, i, jp, k) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / dz;
-    const double thh = (X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5 / dp;
-    const double Ahha = (X(A, i, j, ka) + X(A, i, j, km)) / (dp * dp);
-    double Ah = (X(A, i, j, ka) - X(A, i, j, km)) * 0.5 / dp;
+    const double thz = (X(th, i, jp, k) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / K.dz;
+    const double thh = (X(th, i, j, ka) + X(th, i, jp, ka) - X(th, i, j, k) - X(th, i, jp, k)) * 0.5 / K.dp;
+    const double Ahha = (X(A, i, j, ka) + X(A, i, j, km)) / K.dp2;
+    double Ah = (X(A, i, j, ka) - X(A, i, j, km)) * 0.5 / K.dp;
     {
-        const double Pe = fabs(v * dp * r / Ptl);
+        const double Pe = fabs(v * dp * r / K.ptl);
         if (Pe > 2) {
-            if (v > 0) Ah = (X(A, i, j, k) - X(A, i, j, km)) / dp;
-            else if (v < 0) Ah = (X(A, i, j, ka) - X(A, i, j, k)) / dp;
+            if (v > 0) Ah = (X(A, i, j, k) - X(A, i, j, km)) / K.dp;
+            else if (v < 0) Ah = (X(A, i, j, ka) - X(A, i, j, k)) / K.dp;
         }
     }
-    const double Bh = (X(B, i, j, ka) + X(B, im, j, ka) - X(B, i, j, k) - X(B, im, j, k)) * 0.5 / dp;
+    const double Bh = (X(B, i, j, ka) + X(B, im, j, ka) - X(B, i, j, k) - X(B, im, j, k)) * 0.5 / K.dp;
     const double uelh = ((X(uel, i, j, ka) + X(uel, i, jp, ka) + X(uel, im, j, ka) + X(uel, im, jp, ka)) * 0.25 -
                          (X(uel, i, j, k) + X(uel, i, jp, k) + X(uel, im, j, k) + X(uel, im, jp, k)) * 0.25) /
                         dp;
-    const double cnz = -1.0 / (dz * dz);
-    const double cnh = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = Arra + Ara / r + Ahha / (r * r) + Azza - 2.0 * Bh / (r * r) + X(A, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Ar + v * Ah / r + w * Az - uelh * BB / r - uelz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * uelz + 2.0 * Ra * (tem * uelz + thz * u) / Re / Ptl - Gr * Ptl / r * thh;
-    const double cf1 = 1.0 / (Ptl * dt) - uelr / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
-    const double cf2 = uelr / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
+    const double cnz = -1.0 / K.dz2;
+    const double cnh = -1.0 / K.dp2 / R.r2;
+    const double RHS1 = Arra + Ara / R.r + Ahha / R.r2 + Azza - 2.0 * Bh / R.r2 + X(A, i, j, k) / K.ptldt;
+    const double ULT = (u * Ar + v * Ah / R.r + w * Az - uelh * BB / R.r - uelz * CC) / K.ptl;
+    const double RHS2 = -2.0 * Re * uelz + 2.0 * Ra * (tem * uelz + thz * u) / K.re / K.ptl - Gr * Ptl / R.r * thh;
+    const double cf1 = 1.0 / K.ptldt - uelr / K.ptl2 + 1.0 / R.r2h - cnr - cnz - cnh;
+    const double cf2 = uelr / K.ptl2 - 1.0 / R.r2h + cnr + cnz + cnh;
     XW(F.An, i, j, k) = (cf2 * X(Ap, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
-__device__ __forceinline__ void d_bvort(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_bvort(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -189,6 +239,7 @@ __device__ __forceinline__ void d_bvort(const Geo &g, const SF &F, int i, int j,
 
     const int ip = i + 1, jp = j + 1, km = k - 1, ka = k + 1;
     const double r = g.r0 + (double)i * dr;
+    const Dr R(r);
     const double u = (X(uel, i, j, k) + X(uel, i, jp, k)) * 0.5;
     const double v = (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k) +
                       X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) *
@@ -197,68 +248,68 @@ __device__ __forceinline__ void d_bvort(const Geo &g, const SF &F, int i, int j,
     const double tem = (X(th, i, jp, k) + X(th, ip, jp, k) + X(th, i, j, k) + X(th, ip, j, k)) * 0.25;
     const double AA = (X(A, i, j, k) + X(A, ip, j, k) + X(A, i, j, km) + X(A, ip, j, km)) * 0.25;
     const double CC = (X(C, i, j, k) + X(C, i, jp, k) + X(C, i, j, km) + X(C, i, jp, km)) * 0.25;
-    const double Brra = (X(B, i - 1, j, k) + X(B, i + 1, j, k)) / (dr * dr);
-    const double Bra = (X(B, i + 1, j, k) - X(B, i - 1, j, k)) * 0.5 / dr;
+    const double Brra = (X(B, i - 1, j, k) + X(B, i + 1, j, k)) / K.dr2;
+    const double Bra = (X(B, i + 1, j, k) - X(B, i - 1, j, k)) * 0.5 / K.dr;
     double Br = Bra;
     {
-        const double Pe = fabs(u * dr / Ptl);
+        const double Pe = fabs(u * dr / K.ptl);
         if (Pe > 2) {
-            if (u > 0) Br = (X(B, i, j, k) - X(B, i - 1, j, k)) / dr;
-            else if (u < 0) Br = (X(B, i + 1, j, k) - X(B, i, j, k)) / dr;
+            if (u > 0) Br = (X(B, i, j, k) - X(B, i - 1, j, k)) / K.dr;
+            else if (u < 0) Br = (X(B, i + 1, j, k) - X(B, i, j, k)) / K.dr;
         }
     }
     const double velr = ((X(vel, ip, j, km) + X(vel, ip, jp, km) + X(vel, ip, j, k) + X(vel, ip, jp, k)) * 0.25 -
                          (X(vel, i, j, km) + X(vel, i, jp, km) + X(vel, i, j, k) + X(vel, i, jp, k)) * 0.25) /
                         dr;
-    const double Bzza = (X(B, i, j - 1, k) + X(B, i, j + 1, k)) / (dz * dz);
-    double Bz = (X(B, i, j + 1, k) - X(B, i, j - 1, k)) * 0.5 / dz;
+    const double Bzza = (X(B, i, j - 1, k) + X(B, i, j + 1, k)) / K.dz2;
+    double Bz = (X(B, i, j + 1, k) - X(B, i, j - 1, k)) * 0.5 / K.dz;
     {
-        const double Pe = fabs(w * dz / Ptl);
+        const double Pe = fabs(w * dz / K.ptl);
         if (Pe > 2) {
-            if (w > 0) Bz = (X(B, i, j, k) - X(B, i, j - 1, k)) / dz;
-            else if (w < 0) Bz = (X(B, i, j + 1, k) - X(B, i, j, k)) / dz;
+            if (w > 0) Bz = (X(B, i, j, k) - X(B, i, j - 1, k)) / K.dz;
+            else if (w < 0) Bz = (X(B, i, j + 1, k) - X(B, i, j, k)) / K.dz;
         }
     }
     const double velz = ((X(vel, i, jp, km) + X(vel, ip, jp, km) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
                          (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, j, k) + X(vel, ip, j, k)) * 0.25) /
                         dz;
-    const double thz = (X(th, i, jp, k) + X(th, ip, jp, k) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / dz;
-    const double thr = (X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5 / dr;
+    const double thz = (X(th, i, jp, k) + X(th, ip, jp, k) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / K.dz;
+    const double thr = (X(th, ip, jp, k) + X(th, ip, j, k) - X(th, i, jp, k) - X(th, i, j, k)) * 0.5 / K.dr;
     double Bhha, Bh, cnh;
     if (k == 1) {
-        Bhha = (3.2 * X(B, i, j, 0) + 2.0 * X(B, i, j, 2) - 0.2 * X(B, i, j, 3)) / (dp * dp);
-        Bh = (-4.0 / 3.0 * X(B, i, j, 0) + X(B, i, j, 1) + X(B, i, j, 2) / 3.0) / dp;
-        cnh = -2.5 / (dp * dp) / (r * r);
+        Bhha = (3.2 * X(B, i, j, 0) + 2.0 * X(B, i, j, 2) - 0.2 * X(B, i, j, 3)) / K.dp2;
+        Bh = (-4.0 / K.three * X(B, i, j, 0) + X(B, i, j, 1) + X(B, i, j, 2) / K.three) / K.dp;
+        cnh = -2.5 / K.dp2 / R.r2;
     } else if (k == np) {
-        Bhha = (-0.2 * X(B, i, j, np - 2) + 2.0 * X(B, i, j, np - 1) + 3.2 * X(B, i, j, np + 1)) / (dp * dp);
-        Bh = (-X(B, i, j, np - 1) / 3.0 - X(B, i, j, np) + 4.0 / 3.0 * X(B, i, j, np + 1)) / dp;
-        cnh = -2.5 / (dp * dp) / (r * r);
+        Bhha = (-0.2 * X(B, i, j, np - 2) + 2.0 * X(B, i, j, np - 1) + 3.2 * X(B, i, j, np + 1)) / K.dp2;
+        Bh = (-X(B, i, j, np - 1) / K.three - X(B, i, j, np) + 4.0 / K.three * X(B, i, j, np + 1)) / K.dp;
+        cnh = -2.5 / K.dp2 / R.r2;
     } else {
-        Bhha = (X(B, i, j, ka) + X(B, i, j, km)) / (dp * dp);
-        Bh = (X(B, i, j, ka) - X(B, i, j, km)) * 0.5 / dp;
-        cnh = -1.0 / (dp * dp) / (r * r);
-        const double Pe = fabs(v * dp * r / Ptl);
+        Bhha = (X(B, i, j, ka) + X(B, i, j, km)) / K.dp2;
+        Bh = (X(B, i, j, ka) - X(B, i, j, km)) * 0.5 / K.dp;
+        cnh = -1.0 / K.dp2 / R.r2;
+        const double Pe = fabs(v * dp * r / K.ptl);
         if (Pe > 2) {
-            if (v > 0) Bh = (X(B, i, j, k) - X(B, i, j, km)) / dp;
-            else if (v < 0) Bh = (X(B, i, j, ka) - X(B, i, j, k)) / dp;
+            if (v > 0) Bh = (X(B, i, j, k) - X(B, i, j, km)) / K.dp;
+            else if (v < 0) Bh = (X(B, i, j, ka) - X(B, i, j, k)) / K.dp;
         }
     }
-    const double Ah = (X(A, i, j, k) + X(A, ip, j, k) - X(A, i, j, km) - X(A, ip, j, km)) * 0.5 / dp;
+    const double Ah = (X(A, i, j, k) + X(A, ip, j, k) - X(A, i, j, km) - X(A, ip, j, km)) * 0.5 / K.dp;
     const double velh = ((X(vel, i, j, k) + X(vel, ip, j, k) + X(vel, i, jp, k) + X(vel, ip, jp, k)) * 0.25 -
                          (X(vel, i, j, km) + X(vel, ip, j, km) + X(vel, i, jp, km) + X(vel, ip, jp, km)) * 0.25) /
                         dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnz = -1.0 / (dz * dz);
-    const double RHS1 = Brra + Bra / r + Bhha / (r * r) + Bzza + 2.0 * Ah / (r * r) + X(B, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Br + v * Bh / r + w * Bz - (velr - v / r) * AA - velz * CC) / Ptl;
-    const double RHS2 = -2.0 * Re * velz + r * Ra * thz + 2.0 * Ra * (tem * velz + thz * v) / Re / Ptl +
+    const double cnr = -1.0 / K.dr2;
+    const double cnz = -1.0 / K.dz2;
+    const double RHS1 = Brra + Bra / R.r + Bhha / R.r2 + Bzza + 2.0 * Ah / R.r2 + X(B, i, j, k) / K.ptldt;
+    const double ULT = (u * Br + v * Bh / R.r + w * Bz - (velr - v / R.r) * AA - velz * CC) / K.ptl;
+    const double RHS2 = -2.0 * Re * velz + r * Ra * thz + 2.0 * Ra * (tem * velz + thz * v) / K.re / K.ptl +
                         Gr * Ptl * thr;
-    const double cf1 = 1.0 / (Ptl * dt) - (u / r + velh / r) / (2.0 * Ptl) + 1.0 / (2.0 * (r * r)) - cnr - cnz - cnh;
-    const double cf2 = (u / r + velh / r) / (2.0 * Ptl) - 1.0 / (2.0 * (r * r)) + cnr + cnz + cnh;
+    const double cf1 = 1.0 / K.ptldt - (u / R.r + velh / R.r) / K.ptl2 + 1.0 / R.r2h - cnr - cnz - cnh;
+    const double cf2 = (u / R.r + velh / R.r) / K.ptl2 - 1.0 / R.r2h + cnr + cnz + cnh;
     XW(F.Bn, i, j, k) = (cf2 * X(Bp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
-__device__ __forceinline__ void d_cvort(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_cvort(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -271,6 +322,7 @@ __device__ __forceinline__ void d_cvort(const Geo &g, const SF &F, int i, int j,
 
     const int ip = i + 1, im = i - 1, jm = j - 1, km = k - 1, ka = k + 1;
     const double r = g.r0 + (double)i * dr;
+    const Dr R(r);
     const double u = (X(uel, i, j, ka) + X(uel, i, j, k)) * 0.5;
     const double v = (X(vel, i, j, k) + X(vel, ip, j, k)) * 0.5;
     const double w = (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka) +
@@ -279,68 +331,68 @@ __device__ __forceinline__ void d_cvort(const Geo &g, const SF &F, int i, int j,
     const double AA = (X(A, i, j, k) + X(A, i, jm, k) + X(A, ip, j, k) + X(A, ip, jm, k)) * 0.25;
     const double BB = (X(B, i, j, k) + X(B, i, jm, k) + X(B, i, j, ka) + X(B, i, jm, ka)) * 0.25;
     const double tem = (X(th, i, j, k) + X(th, ip, j, k) + X(th, ip, j, ka) + X(th, i, j, ka)) * 0.25;
-    const double Crra = (X(C, i - 1, j, k) + X(C, i + 1, j, k)) / (dr * dr);
-    const double Cra = (X(C, i + 1, j, k) - X(C, i - 1, j, k)) * 0.5 / dr;
+    const double Crra = (X(C, i - 1, j, k) + X(C, i + 1, j, k)) / K.dr2;
+    const double Cra = (X(C, i + 1, j, k) - X(C, i - 1, j, k)) * 0.5 / K.dr;
     double Cr = Cra;
     {
-        const double Pe = fabs(u * dr / Ptl);
+        const double Pe = fabs(u * dr / K.ptl);
         if (Pe > 2) {
-            if (u > 0) Cr = (X(C, i, j, k) - X(C, i - 1, j, k)) / dr;
-            else if (u < 0) Cr = (X(C, i + 1, j, k) - X(C, i, j, k)) / dr;
+            if (u > 0) Cr = (X(C, i, j, k) - X(C, i - 1, j, k)) / K.dr;
+            else if (u < 0) Cr = (X(C, i + 1, j, k) - X(C, i, j, k)) / K.dr;
         }
     }
     const double welr = ((X(wel, ip, j, k) + X(wel, ip, j, ka) + X(wel, ip, jm, k) + X(wel, ip, jm, ka)) * 0.25 -
                          (X(wel, i, j, k) + X(wel, i, j, ka) + X(wel, i, jm, k) + X(wel, i, jm, ka)) * 0.25) /
                         dr;
     /* uelr (sect360v2mg.f:623-624) is computed by the reference and unused */
-    const double thr = (X(th, ip, j, k) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / dr;
+    const double thr = (X(th, ip, j, k) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, i, j, ka)) * 0.5 / K.dr;
     double Czza, Cz, cnz;
     if (j == 1) {
-        Czza = (3.2 * X(C, i, j - 1, k) + 2.0 * X(C, i, j + 1, k) - 0.2 * X(C, i, j + 2, k)) / (dz * dz);
-        Cz = (-4.0 * X(C, i, j - 1, k) / 3.0 + X(C, i, j, k) + X(C, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        Czza = (3.2 * X(C, i, j - 1, k) + 2.0 * X(C, i, j + 1, k) - 0.2 * X(C, i, j + 2, k)) / K.dz2;
+        Cz = (-4.0 * X(C, i, j - 1, k) / K.three + X(C, i, j, k) + X(C, i, j + 1, k) / K.three) / K.dz;
+        cnz = -2.5 / K.dz2;
     } else if (j == nz) {
-        Czza = (3.2 * X(C, i, j + 1, k) + 2.0 * X(C, i, j - 1, k) - 0.2 * X(C, i, j - 2, k)) / (dz * dz);
-        Cz = (-X(C, i, j - 1, k) / 3.0 - X(C, i, j, k) + 4.0 * X(C, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        Czza = (3.2 * X(C, i, j + 1, k) + 2.0 * X(C, i, j - 1, k) - 0.2 * X(C, i, j - 2, k)) / K.dz2;
+        Cz = (-X(C, i, j - 1, k) / K.three - X(C, i, j, k) + 4.0 * X(C, i, j + 1, k) / K.three) / K.dz;
+        cnz = -2.5 / K.dz2;
     } else {
-        Czza = (X(C, i, j - 1, k) + X(C, i, j + 1, k)) / (dz * dz);
-        Cz = (X(C, i, j + 1, k) - X(C, i, j - 1, k)) * 0.5 / dz;
-        const double Pe = fabs(w * dz / Ptl);
+        Czza = (X(C, i, j - 1, k) + X(C, i, j + 1, k)) / K.dz2;
+        Cz = (X(C, i, j + 1, k) - X(C, i, j - 1, k)) * 0.5 / K.dz;
+        const double Pe = fabs(w * dz / K.ptl);
         if (Pe > 2) {
-            if (w > 0) Cz = (X(C, i, j, k) - X(C, i, j - 1, k)) / dz;
-            else if (w < 0) Cz = (X(C, i, j + 1, k) - X(C, i, j, k)) / dz;
+            if (w > 0) Cz = (X(C, i, j, k) - X(C, i, j - 1, k)) / K.dz;
+            else if (w < 0) Cz = (X(C, i, j + 1, k) - X(C, i, j, k)) / K.dz;
         }
-        cnz = -1.0 / (dz * dz);
+        cnz = -1.0 / K.dz2;
     }
     const double welz = ((X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, j, ka) + X(wel, ip, j, ka)) * 0.25 -
                          (X(wel, i, jm, k) + X(wel, ip, jm, k) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25) /
                         dz;
-    const double Chha = (X(C, i, j, ka) + X(C, i, j, km)) / (dp * dp);
-    double Ch = (X(C, i, j, ka) - X(C, i, j, km)) * 0.5 / dp;
+    const double Chha = (X(C, i, j, ka) + X(C, i, j, km)) / K.dp2;
+    double Ch = (X(C, i, j, ka) - X(C, i, j, km)) * 0.5 / K.dp;
     {
-        const double Pe = fabs(v * dp * r / Ptl);
+        const double Pe = fabs(v * dp * r / K.ptl);
         if (Pe > 2) {
-            if (v > 0) Ch = (X(C, i, j, k) - X(C, i, j, km)) / dp;
-            else if (v < 0) Ch = (X(C, i, j, ka) - X(C, i, j, k)) / dp;
+            if (v > 0) Ch = (X(C, i, j, k) - X(C, i, j, km)) / K.dp;
+            else if (v < 0) Ch = (X(C, i, j, ka) - X(C, i, j, k)) / K.dp;
         }
     }
     const double welh = ((X(wel, i, j, ka) + X(wel, ip, j, ka) + X(wel, i, jm, ka) + X(wel, ip, jm, ka)) * 0.25 -
                          (X(wel, i, j, k) + X(wel, ip, j, k) + X(wel, i, jm, k) + X(wel, ip, jm, k)) * 0.25) /
                         dp;
     /* velh (sect360v2mg.f:667-668) is computed by the reference and unused */
-    const double thh = (X(th, i, j, ka) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / dp;
-    const double cnr = -1.0 / (dr * dr);
-    const double cnh = -1.0 / (dp * dp) / (r * r);
-    const double RHS1 = Crra + Cra / r + Chha / (r * r) + Czza + X(C, i, j, k) / (Ptl * dt);
-    const double ULT = (u * Cr + v * Ch / r + w * Cz - welr * AA - welh * BB / r) / Ptl;
-    const double RHS2 = -2.0 * Re * welz - Ra * thh + 2.0 * Ra * (tem * welz - thr * u - thh * v / r) / Re / Ptl;
-    const double cf1 = 1.0 / (Ptl * dt) - welz / (2.0 * Ptl) - cnr - cnz - cnh;
-    const double cf2 = welz / (2.0 * Ptl) + cnr + cnz + cnh;
+    const double thh = (X(th, i, j, ka) + X(th, ip, j, ka) - X(th, i, j, k) - X(th, ip, j, k)) * 0.5 / K.dp;
+    const double cnr = -1.0 / K.dr2;
+    const double cnh = -1.0 / K.dp2 / R.r2;
+    const double RHS1 = Crra + Cra / R.r + Chha / R.r2 + Czza + X(C, i, j, k) / K.ptldt;
+    const double ULT = (u * Cr + v * Ch / R.r + w * Cz - welr * AA - welh * BB / R.r) / K.ptl;
+    const double RHS2 = -2.0 * Re * welz - Ra * thh + 2.0 * Ra * (tem * welz - thr * u - thh * v / R.r) / K.re / K.ptl;
+    const double cf1 = 1.0 / K.ptldt - welz / K.ptl2 - cnr - cnz - cnh;
+    const double cf2 = welz / K.ptl2 + cnr + cnz + cnh;
     XW(F.Cn, i, j, k) = (cf2 * X(Cp, i, j, k) - ULT - RHS2 + RHS1) / cf1;
 }
 
-__device__ __forceinline__ void d_temper(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_temper(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -351,10 +403,11 @@ __device__ __forceinline__ void d_temper(const Geo &g, const SF &F, int i, int j
     (void)A; (void)B; (void)C; (void)Ap; (void)Bp; (void)Cp; (void)th; (void)thp; (void)uel; (void)vel; (void)wel;
     (void)An; (void)Bn; (void)Cn;
     // the row k = np advects with the thh of the row k = np-1 (see above)
-    double thh = (k == np) ? thh_row(g, F, i, j, np - 1) : 0.0;
+    double thh = (k == np) ? thh_row(g, F, K, i, j, np - 1) : 0.0;
 
     const int im = i - 1, jm = j - 1, km = k - 1, ka = k + 1;
     const double r = g.r0 + ((double)i - 0.5) * dr;
+    const Dr R(r);
     double u, v, w;
     if (j == 0 || j == nz + 1 || k == 0 || k == np + 1) {
         u = 0.0;
@@ -367,88 +420,88 @@ __device__ __forceinline__ void d_temper(const Geo &g, const SF &F, int i, int j
     }
     double thrra, thra, thr, cnr;
     if (i == 1) {
-        thrra = (3.2 * X(th, i - 1, j, k) + 2.0 * X(th, i + 1, j, k) - 0.2 * X(th, i + 2, j, k)) / (dr * dr);
-        thra = (-4.0 * X(th, i - 1, j, k) / 3.0 + X(th, i + 1, j, k) / 3.0) / dr;
-        thr = (-4.0 * X(th, i - 1, j, k) / 3.0 + X(th, i, j, k) + X(th, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) + 0.5 / dr / r;
+        thrra = (3.2 * X(th, i - 1, j, k) + 2.0 * X(th, i + 1, j, k) - 0.2 * X(th, i + 2, j, k)) / K.dr2;
+        thra = (-4.0 * X(th, i - 1, j, k) / K.three + X(th, i + 1, j, k) / K.three) / K.dr;
+        thr = (-4.0 * X(th, i - 1, j, k) / K.three + X(th, i, j, k) + X(th, i + 1, j, k) / K.three) / K.dr;
+        cnr = -2.5 / K.dr2 + 0.5 / K.dr / R.r;
     } else if (i == nr) {
-        thrra = (3.2 * X(th, i + 1, j, k) + 2.0 * X(th, i - 1, j, k) - 0.2 * X(th, i - 2, j, k)) / (dr * dr);
-        thra = (-X(th, i - 1, j, k) / 3.0 + 4.0 * X(th, i + 1, j, k) / 3.0) / dr;
-        thr = (-X(th, i - 1, j, k) / 3.0 - X(th, i, j, k) + 4.0 * X(th, i + 1, j, k) / 3.0) / dr;
-        cnr = -2.5 / (dr * dr) - 0.5 / dr / r;
+        thrra = (3.2 * X(th, i + 1, j, k) + 2.0 * X(th, i - 1, j, k) - 0.2 * X(th, i - 2, j, k)) / K.dr2;
+        thra = (-X(th, i - 1, j, k) / K.three + 4.0 * X(th, i + 1, j, k) / K.three) / K.dr;
+        thr = (-X(th, i - 1, j, k) / K.three - X(th, i, j, k) + 4.0 * X(th, i + 1, j, k) / K.three) / K.dr;
+        cnr = -2.5 / K.dr2 - 0.5 / K.dr / R.r;
     } else {
-        thrra = (X(th, i - 1, j, k) + X(th, i + 1, j, k)) / (dr * dr);
-        thra = (X(th, i + 1, j, k) - X(th, i - 1, j, k)) * 0.5 / dr;
+        thrra = (X(th, i - 1, j, k) + X(th, i + 1, j, k)) / K.dr2;
+        thra = (X(th, i + 1, j, k) - X(th, i - 1, j, k)) * 0.5 / K.dr;
         thr = thra;
-        const double Pe = fabs(u * dr / Ptl);
+        const double Pe = fabs(u * dr / K.ptl);
         if (Pe > 2) {
-            if (u > 0) thr = (X(th, i, j, k) - X(th, i - 1, j, k)) / dr;
-            else if (u < 0) thr = (X(th, i + 1, j, k) - X(th, i, j, k)) / dr;
+            if (u > 0) thr = (X(th, i, j, k) - X(th, i - 1, j, k)) / K.dr;
+            else if (u < 0) thr = (X(th, i + 1, j, k) - X(th, i, j, k)) / K.dr;
         }
-        cnr = -1.0 / (dr * dr);
+        cnr = -1.0 / K.dr2;
     }
     double thzza, thz, cnz;
     if (j == 1) {
-        thzza = (3.2 * X(th, i, j - 1, k) + 2.0 * X(th, i, j + 1, k) - 0.2 * X(th, i, j + 2, k)) / (dz * dz);
-        thz = (-4.0 * X(th, i, j - 1, k) / 3.0 + X(th, i, j, k) + X(th, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        thzza = (3.2 * X(th, i, j - 1, k) + 2.0 * X(th, i, j + 1, k) - 0.2 * X(th, i, j + 2, k)) / K.dz2;
+        thz = (-4.0 * X(th, i, j - 1, k) / K.three + X(th, i, j, k) + X(th, i, j + 1, k) / K.three) / K.dz;
+        cnz = -2.5 / K.dz2;
     } else if (j == nz) {
-        thzza = (3.2 * X(th, i, j + 1, k) + 2.0 * X(th, i, j - 1, k) - 0.2 * X(th, i, j - 2, k)) / (dz * dz);
-        thz = (-X(th, i, j - 1, k) / 3.0 - X(th, i, j, k) + 4.0 * X(th, i, j + 1, k) / 3.0) / dz;
-        cnz = -2.5 / (dz * dz);
+        thzza = (3.2 * X(th, i, j + 1, k) + 2.0 * X(th, i, j - 1, k) - 0.2 * X(th, i, j - 2, k)) / K.dz2;
+        thz = (-X(th, i, j - 1, k) / K.three - X(th, i, j, k) + 4.0 * X(th, i, j + 1, k) / K.three) / K.dz;
+        cnz = -2.5 / K.dz2;
     } else if (j == 0) {
-        thzza = 8.0 * X(th, i, j + 1, k) / (dz * dz);
+        thzza = 8.0 * X(th, i, j + 1, k) / K.dz2;
         thz = 0.0;
-        cnz = -4.0 / (dz * dz);
+        cnz = -4.0 / K.dz2;
     } else if (j == nz + 1) {
-        thzza = 8.0 * X(th, i, j - 1, k) / (dz * dz);
+        thzza = 8.0 * X(th, i, j - 1, k) / K.dz2;
         thz = 0.0;
-        cnz = -4.0 / (dz * dz);
+        cnz = -4.0 / K.dz2;
     } else {
-        thzza = (X(th, i, j - 1, k) + X(th, i, j + 1, k)) / (dz * dz);
-        thz = (X(th, i, j + 1, k) - X(th, i, j - 1, k)) * 0.5 / dz;
-        const double Pe = fabs(w * dz / Ptl);
+        thzza = (X(th, i, j - 1, k) + X(th, i, j + 1, k)) / K.dz2;
+        thz = (X(th, i, j + 1, k) - X(th, i, j - 1, k)) * 0.5 / K.dz;
+        const double Pe = fabs(w * dz / K.ptl);
         if (Pe > 2) {
-            if (w > 0) thz = (X(th, i, j, k) - X(th, i, j - 1, k)) / dz;
-            else if (w < 0) thz = (X(th, i, j + 1, k) - X(th, i, j, k)) / dz;
+            if (w > 0) thz = (X(th, i, j, k) - X(th, i, j - 1, k)) / K.dz;
+            else if (w < 0) thz = (X(th, i, j + 1, k) - X(th, i, j, k)) / K.dz;
         }
-        cnz = -1.0 / (dz * dz);
+        cnz = -1.0 / K.dz2;
     }
     double thhha, cnp;
     if (k == 1) {
-        thhha = (3.2 * X(th, i, j, 0) + 2.0 * X(th, i, j, 2) - 0.2 * X(th, i, j, 3)) / (dp * dp);
-        thh = (-4.0 * X(th, i, j, 0) / 3.0 + X(th, i, j, 1) + X(th, i, j, 2) / 3.0) / dp;
-        cnp = -2.5 / (dp * dp) / (r * r);
+        thhha = (3.2 * X(th, i, j, 0) + 2.0 * X(th, i, j, 2) - 0.2 * X(th, i, j, 3)) / K.dp2;
+        thh = (-4.0 * X(th, i, j, 0) / K.three + X(th, i, j, 1) + X(th, i, j, 2) / K.three) / K.dp;
+        cnp = -2.5 / K.dp2 / R.r2;
     } else if (k == np) {
-        thhha = (-0.2 * X(th, i, j, np - 2) + 2.0 * X(th, i, j, np - 1) + 3.2 * X(th, i, j, np + 1)) / (dp * dp);
-        thz = (-X(th, i, j, np - 1) / 3.0 - X(th, i, j, np) + 4.0 * X(th, i, j, np + 1) / 3.0) / dp;
-        cnp = -2.5 / (dp * dp) / (r * r);
+        thhha = (-0.2 * X(th, i, j, np - 2) + 2.0 * X(th, i, j, np - 1) + 3.2 * X(th, i, j, np + 1)) / K.dp2;
+        thz = (-X(th, i, j, np - 1) / K.three - X(th, i, j, np) + 4.0 * X(th, i, j, np + 1) / K.three) / K.dp;
+        cnp = -2.5 / K.dp2 / R.r2;
     } else if (k == 0) {
-        thhha = 8.0 * X(th, i, j, 1) / (dp * dp);
+        thhha = 8.0 * X(th, i, j, 1) / K.dp2;
         thh = 0.0;
-        cnp = -4.0 / (dp * dp) / (r * r);
+        cnp = -4.0 / K.dp2 / R.r2;
     } else if (k == np + 1) {
-        thhha = 8.0 * X(th, i, j, np) / (dp * dp);
+        thhha = 8.0 * X(th, i, j, np) / K.dp2;
         thh = 0.0;
-        cnp = -4.0 / (dp * dp) / (r * r);
+        cnp = -4.0 / K.dp2 / R.r2;
     } else {
-        thhha = (X(th, i, j, ka) + X(th, i, j, km)) / (dp * dp);
-        thh = (X(th, i, j, ka) - X(th, i, j, km)) / (2.0 * dp);
-        const double Pe = fabs(v * dp * r / Ptl);
+        thhha = (X(th, i, j, ka) + X(th, i, j, km)) / K.dp2;
+        thh = (X(th, i, j, ka) - X(th, i, j, km)) / K.dp2x;
+        const double Pe = fabs(v * dp * r / K.ptl);
         if (Pe > 2) {
-            if (v > 0) thh = (X(th, i, j, k) - X(th, i, j, km)) / dp;
-            else if (v < 0) thh = (X(th, i, j, ka) - X(th, i, j, k)) / dp;
+            if (v > 0) thh = (X(th, i, j, k) - X(th, i, j, km)) / K.dp;
+            else if (v < 0) thh = (X(th, i, j, ka) - X(th, i, j, k)) / K.dp;
         }
-        cnp = -1.0 / (dp * dp) / (r * r);
+        cnp = -1.0 / K.dp2 / R.r2;
     }
-    const double RHS1 = thrra + thra / r + thhha / (r * r) + thzza + X(th, i, j, k) / dt;
-    const double ULT = u * thr + v * thh / r + w * thz;
-    const double cf1 = 1.0 / dt - cnr - cnz - cnp;
+    const double RHS1 = thrra + thra / R.r + thhha / R.r2 + thzza + X(th, i, j, k) / K.dt;
+    const double ULT = u * thr + v * thh / R.r + w * thz;
+    const double cf1 = 1.0 / K.dt - cnr - cnz - cnp;
     const double cf2 = cnr + cnz + cnp;
     XW(F.thn, i, j, k) = (cf2 * X(thp, i, j, k) - ULT + RHS1) / cf1;
 }
 
-__device__ __forceinline__ void d_rhsw(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_rhsw(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -461,19 +514,20 @@ __device__ __forceinline__ void d_rhsw(const Geo &g, const SF &F, int i, int j, 
 
     const int km = k - 1;
     const double r = g.r0 + ((double)i - 0.5) * dr;
-    const double Ah = (X(An, i, j, k) - X(An, i, j, km)) / dp;
-    const double Br = (X(Bn, i, j, k) - X(Bn, i - 1, j, k)) / dr;
+    const Dr R(r);
+    const double Ah = (X(An, i, j, k) - X(An, i, j, km)) / K.dp;
+    const double Br = (X(Bn, i, j, k) - X(Bn, i - 1, j, k)) / K.dr;
     double v;
-    if (i == 1) v = -X(Bn, 1, j, k) / dr - 0.5 * X(Bn, 1, j, k) / r;
-    else if (i == nr) v = X(Bn, nr - 1, j, k) / dr - 0.5 * X(Bn, nr - 1, j, k) / r;
-    else v = -Br - 0.5 * (X(Bn, i, j, k) + X(Bn, i - 1, j, k)) / r;
-    if (k == 1) v = v + X(An, i, j, k) / r / dp;
-    else if (k == np) v = v - X(An, i, j, km) / r / dp;
-    else v = v + Ah / r;
+    if (i == 1) v = -X(Bn, 1, j, k) / K.dr - 0.5 * X(Bn, 1, j, k) / R.r;
+    else if (i == nr) v = X(Bn, nr - 1, j, k) / K.dr - 0.5 * X(Bn, nr - 1, j, k) / R.r;
+    else v = -Br - 0.5 * (X(Bn, i, j, k) + X(Bn, i - 1, j, k)) / R.r;
+    if (k == 1) v = v + X(An, i, j, k) / R.r / K.dp;
+    else if (k == np) v = v - X(An, i, j, km) / R.r / K.dp;
+    else v = v + Ah / R.r;
     XW(F.wrh, i, j, k) = v;
 }
 
-__device__ __forceinline__ void d_rhsv(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_rhsv(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -485,19 +539,21 @@ __device__ __forceinline__ void d_rhsv(const Geo &g, const SF &F, int i, int j, 
     (void)An; (void)Bn; (void)Cn;
 
     const double r = g.r0 + ((double)i - 0.5) * dr;
-    const double Az = (X(An, i, j, k) - X(An, i, j - 1, k)) / dz;
-    const double Cr = (X(Cn, i, j, k) - X(Cn, i - 1, j, k)) / dr;
+
+    const Dr R(r);
+    const double Az = (X(An, i, j, k) - X(An, i, j - 1, k)) / K.dz;
+    const double Cr = (X(Cn, i, j, k) - X(Cn, i - 1, j, k)) / K.dr;
     double v;
-    if (i == 1) v = X(Cn, 1, j, k) / dr + X(Cn, 1, j, k) / r;
-    else if (i == nr) v = -X(Cn, nr - 1, j, k) / dr + X(Cn, nr - 1, j, k) / r;
-    else v = Cr + (X(Cn, i, j, k) + X(Cn, i - 1, j, k)) / r;
-    if (j == 1) v = v - X(An, i, 1, k) / dz;
-    else if (j == nz) v = v + X(An, i, nz - 1, k) / dz;
+    if (i == 1) v = X(Cn, 1, j, k) / K.dr + X(Cn, 1, j, k) / R.r;
+    else if (i == nr) v = -X(Cn, nr - 1, j, k) / K.dr + X(Cn, nr - 1, j, k) / R.r;
+    else v = Cr + (X(Cn, i, j, k) + X(Cn, i - 1, j, k)) / R.r;
+    if (j == 1) v = v - X(An, i, 1, k) / K.dz;
+    else if (j == nz) v = v + X(An, i, nz - 1, k) / K.dz;
     else v = v - Az;
     XW(F.vrh, i, j, k) = v;
 }
 
-__device__ __forceinline__ void d_rhsu(const Geo &g, const SF &F, int i, int j, int k)
+__device__ __forceinline__ void d_rhsu(const Geo &g, const SF &F, const SK &K, int i, int j, int k)
 {
     const int nr = g.nr, nz = g.nz, np = g.np;
     const double dr = g.dr, dz = g.dz, dp = g.dp, dt = g.dt;
@@ -510,17 +566,18 @@ __device__ __forceinline__ void d_rhsu(const Geo &g, const SF &F, int i, int j, 
 
     const int km = k - 1;
     const double r = g.r0 + (double)i * dr;
-    const double Bz = (X(Bn, i, j, k) - X(Bn, i, j - 1, k)) / dz;
+    const Dr R(r);
+    const double Bz = (X(Bn, i, j, k) - X(Bn, i, j - 1, k)) / K.dz;
     const double welz = 0.5 * (X(wel, i, j, k) + X(wel, i + 1, j, k) - X(wel, i, j - 1, k) -
-                               X(wel, i + 1, j - 1, k)) / dz;
-    const double Ch = (X(Cn, i, j, k) - X(Cn, i, j, km)) / dp;
+                               X(wel, i + 1, j - 1, k)) / K.dz;
+    const double Ch = (X(Cn, i, j, k) - X(Cn, i, j, km)) / K.dp;
     double v;
-    if (j == 1) v = X(Bn, i, 1, k) / dz - 2.0 * welz / r;
-    else if (j == nz) v = -X(Bn, i, nz - 1, k) / dz - 2.0 * welz / r;
-    else v = Bz - 2.0 * welz / r;
-    if (k == 1) v = v - X(Cn, i, j, k) / r / dp;
-    else if (k == np) v = v + X(Cn, i, j, k - 1) / r / dp;
-    else v = v - Ch / r;
+    if (j == 1) v = X(Bn, i, 1, k) / K.dz - 2.0 * welz / R.r;
+    else if (j == nz) v = -X(Bn, i, nz - 1, k) / K.dz - 2.0 * welz / R.r;
+    else v = Bz - 2.0 * welz / R.r;
+    if (k == 1) v = v - X(Cn, i, j, k) / R.r / K.dp;
+    else if (k == np) v = v + X(Cn, i, j, k - 1) / R.r / K.dp;
+    else v = v - Ch / R.r;
     XW(F.urh, i, j, k) = v;
 }
 
@@ -528,21 +585,21 @@ __device__ __forceinline__ void d_rhsu(const Geo &g, const SF &F, int i, int j, 
 // x over the (i, j) plane, y over the planes k = 0..np+1.  MASK: 1 A, 2 B,
 // 4 C, 8 temperature (all read only the old levels)
 template <int MASK>
-__global__ void __launch_bounds__(256) ks_explicit(Geo g, SF F)
+__global__ void __launch_bounds__(256) ks_explicit(Geo g, SF F, SK K)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.ldi * g.ldj) return;
     const int j = t / g.ldi, i = t - j * g.ldi, k = blockIdx.y;
     const int nr = g.nr, nz = g.nz, np = g.np;
-    if ((MASK & 1) && i >= 1 && i <= nr && j >= 1 && j <= nz - 1 && k >= 1 && k <= np - 1) d_avort(g, F, i, j, k);
-    if ((MASK & 2) && i >= 1 && i <= nr - 1 && j >= 1 && j <= nz - 1 && k >= 1 && k <= np) d_bvort(g, F, i, j, k);
-    if ((MASK & 4) && i >= 1 && i <= nr - 1 && j >= 1 && j <= nz && k >= 1 && k <= np - 1) d_cvort(g, F, i, j, k);
-    if ((MASK & 8) && i >= 1 && i <= nr) d_temper(g, F, i, j, k);
+    if ((MASK & 1) && i >= 1 && i <= nr && j >= 1 && j <= nz - 1 && k >= 1 && k <= np - 1) d_avort(g, F, K, i, j, k);
+    if ((MASK & 2) && i >= 1 && i <= nr - 1 && j >= 1 && j <= nz - 1 && k >= 1 && k <= np) d_bvort(g, F, K, i, j, k);
+    if ((MASK & 4) && i >= 1 && i <= nr - 1 && j >= 1 && j <= nz && k >= 1 && k <= np - 1) d_cvort(g, F, K, i, j, k);
+    if ((MASK & 8) && i >= 1 && i <= nr) d_temper(g, F, K, i, j, k);
 }
 
 // rhsw / rhsv / rhsu on their unknowns
 template <int WHICH>
-__global__ void __launch_bounds__(256) ks_rhs(Geo g, SF F)
+__global__ void __launch_bounds__(256) ks_rhs(Geo g, SF F, SK K)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.ldi * g.ldj) return;
@@ -550,9 +607,9 @@ __global__ void __launch_bounds__(256) ks_rhs(Geo g, SF F)
     const int i1 = (WHICH == 3) ? g.nr - 1 : g.nr, j1 = (WHICH == 1) ? g.nz - 1 : g.nz;
     const int k1 = (WHICH == 2) ? g.np - 1 : g.np;
     if (i < 1 || i > i1 || j < 1 || j > j1 || k < 1 || k > k1) return;
-    if (WHICH == 1) d_rhsw(g, F, i, j, k);
-    else if (WHICH == 2) d_rhsv(g, F, i, j, k);
-    else d_rhsu(g, F, i, j, k);
+    if (WHICH == 1) d_rhsw(g, F, K, i, j, k);
+    else if (WHICH == 2) d_rhsv(g, F, K, i, j, k);
+    else d_rhsu(g, F, K, i, j, k);
 }
 
 // heat, sect360v2mg.f:840-881: the flux of each plane k = 0..np+1, a
@@ -1008,6 +1065,7 @@ struct vvs_run {
     size_t nf = 0;                       // doubles per field: ldi ldj (np+2)
     double *f[VV3_NFIELD] = {nullptr};
     vvs::SPoisson pois;
+    vvs::SK K;                           // the explicit routines' constant divisors
     double *heat = nullptr;              // plane fluxes k = 0..np+1, then trans1
     double *nu = nullptr;                // local Nusselt numbers: inner, outer (np+1 each)
 };
@@ -1033,12 +1091,12 @@ static int launch_explicit(vvs_run *r, int mask)
     const SF F = sfields(r);
     const dim3 gr = pgrid(r);
     switch (mask) {
-    case 1: hipLaunchKernelGGL(ks_explicit<1>, gr, dim3(256), 0, r->stream, r->g, F); break;
-    case 2: hipLaunchKernelGGL(ks_explicit<2>, gr, dim3(256), 0, r->stream, r->g, F); break;
-    case 4: hipLaunchKernelGGL(ks_explicit<4>, gr, dim3(256), 0, r->stream, r->g, F); break;
-    case 7: hipLaunchKernelGGL(ks_explicit<7>, gr, dim3(256), 0, r->stream, r->g, F); break;
-    case 8: hipLaunchKernelGGL(ks_explicit<8>, gr, dim3(256), 0, r->stream, r->g, F); break;
-    default: hipLaunchKernelGGL(ks_explicit<15>, gr, dim3(256), 0, r->stream, r->g, F); break;
+    case 1: hipLaunchKernelGGL(ks_explicit<1>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
+    case 2: hipLaunchKernelGGL(ks_explicit<2>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
+    case 4: hipLaunchKernelGGL(ks_explicit<4>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
+    case 7: hipLaunchKernelGGL(ks_explicit<7>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
+    case 8: hipLaunchKernelGGL(ks_explicit<8>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
+    default: hipLaunchKernelGGL(ks_explicit<15>, gr, dim3(256), 0, r->stream, r->g, F, r->K); break;
     }
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1057,9 +1115,9 @@ static int launch_rhs(vvs_run *r, int which)
 {
     const SF F = sfields(r);
     const dim3 gr = pgrid(r);
-    if (which == 1) hipLaunchKernelGGL(ks_rhs<1>, gr, dim3(256), 0, r->stream, r->g, F);
-    else if (which == 2) hipLaunchKernelGGL(ks_rhs<2>, gr, dim3(256), 0, r->stream, r->g, F);
-    else hipLaunchKernelGGL(ks_rhs<3>, gr, dim3(256), 0, r->stream, r->g, F);
+    if (which == 1) hipLaunchKernelGGL(ks_rhs<1>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
+    else if (which == 2) hipLaunchKernelGGL(ks_rhs<2>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
+    else hipLaunchKernelGGL(ks_rhs<3>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -1156,6 +1214,15 @@ int vvs_create(const vv3_config *cfg, int device, vvs_run **out)
     g.dp = (cfg->dp > 0.0) ? cfg->dp : 2.0 * std::acos(-1.0) / cfg->np;
     g.dt = cfg->dt; g.Ra = cfg->Ra; g.Re = cfg->Re; g.Ptl = cfg->Ptl; g.Gr = cfg->Gr; g.r0 = cfg->r0;
     g.model = 0;
+    {   // the divisors as the reference's expressions form them, with RN(1/b)
+        auto dv = [](double v) { return Dv{v, 1.0 / v}; };
+        SK &K = r->K;
+        K.dr = dv(g.dr); K.dz = dv(g.dz); K.dp = dv(g.dp);
+        K.dr2 = dv(g.dr * g.dr); K.dz2 = dv(g.dz * g.dz); K.dp2 = dv(g.dp * g.dp);
+        K.three = dv(3.0); K.ptl = dv(g.Ptl); K.ptl2 = dv(2.0 * g.Ptl); K.ptldt = dv(g.Ptl * g.dt);
+        K.dt = dv(g.dt); K.dp2x = dv(2.0 * g.dp);
+        K.re = Dvs{g.Re, 1.0 / g.Re};
+    }
     r->nf = (size_t)g.ldi * g.ldj * (cfg->np + 2);
     auto fail = [&](int s) { vvs_destroy(r); return s; };
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return fail(VV_EHIP);
