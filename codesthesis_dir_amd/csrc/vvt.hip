@@ -1308,6 +1308,18 @@ struct vvt_run {
     size_t tab_off[12] = {0};
     TankOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;   // Poisson work (kmax x mmax each)
+    // v's right-hand side and solve run beside w's and u's (one_step): a
+    // second stream with work arrays of its own, and the fork / join events
+    hipStream_t stream2 = nullptr;
+    double *S2 = nullptr, *T2 = nullptr, *cp2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    // the x derivatives run on the second stream beside the y and z ones,
+    // with checkpoints of their own (4 jobs)
+    double *ckpx = nullptr;
+    hipEvent_t forkd = nullptr, joind = nullptr;
+    // the stream and work arrays launch_rhs / launch_poisson use
+    hipStream_t cur = nullptr;
+    double *cS = nullptr, *cT = nullptr, *ccp = nullptr;
     double *ckp = nullptr;        // checkpoints of the y/z derivative lines: 8 jobs x ckp_seg x ckp_lines
     int ckp_seg = 0, ckp_lines = 0;
     double2 *ytw = nullptr;       // y FFT: twiddles of 2np points (np), then (cos, sin)(pi q/(2np)) (np)
@@ -1494,6 +1506,24 @@ int setup_poisson(vvt_run *r)
     VV_HIP(hipMalloc(&r->S, kmax * mmax * sizeof(double)));
     VV_HIP(hipMalloc(&r->T, kmax * mmax * sizeof(double)));
     VV_HIP(hipMalloc(&r->cp, kmax * mmax * sizeof(double)));
+    r->cur = r->stream; r->cS = r->S; r->cT = r->T; r->ccp = r->cp;
+    // the second solve stream: without its memory the solves run in turn
+    const size_t w2 = kmax * mmax * sizeof(double);
+    if (hipMalloc(&r->S2, w2) == hipSuccess && hipMalloc(&r->T2, w2) == hipSuccess &&
+        hipMalloc(&r->cp2, w2) == hipSuccess &&
+        hipMalloc(&r->ckpx, 4 * (size_t)r->ckp_seg * r->ckp_lines * sizeof(double)) == hipSuccess &&
+        hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreateWithFlags(&r->fork, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&r->join, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&r->forkd, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&r->joind, hipEventDisableTiming) == hipSuccess)
+        return VV_OK;
+    (void)hipGetLastError();
+    for (double **p : {&r->S2, &r->T2, &r->cp2, &r->ckpx})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (r->stream2) { (void)hipStreamDestroy(r->stream2); r->stream2 = nullptr; }
+    for (hipEvent_t *e : {&r->fork, &r->join, &r->forkd, &r->joind})
+        if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
     return VV_OK;
 }
 
@@ -1562,6 +1592,10 @@ int launch_derivatives(vvt_run *r)
     int c = 0;
     for (int d = 0; d < 12; ++d)
         if (d % 3 != 0) Jyz.j[c++] = J.j[d];
+    if (r->stream2 && !r->prof) {   // the x jobs start from the state the y and z jobs read
+        VV_HIP(hipEventRecord(r->forkd, r->stream));
+        VV_HIP(hipStreamWaitEvent(r->stream2, r->forkd, 0));
+    }
     {
         Prof p(r, VVT_K_DERIV_YZ, byz);
         if (c > 8 || maxm > r->ckp_seg * CK || nyz > r->ckp_lines) {
@@ -1582,8 +1616,17 @@ int launch_derivatives(vvt_run *r)
             vv::set_error("vvt: derivative checkpoints exceed their buffer");
             return VV_EINVAL;
         }
-        hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), 0, r->stream, geo(r), J, r->ckp, r->ckp_seg,
-                           r->ckp_lines);
+        if (r->stream2 && !r->prof) {
+            // on the second stream (from the state the y and z jobs also
+            // read), joined before the explicit updates read the derivatives
+            hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), 0, r->stream2, geo(r), J, r->ckpx, r->ckp_seg,
+                               r->ckp_lines);
+            VV_HIP(hipEventRecord(r->joind, r->stream2));
+            VV_HIP(hipStreamWaitEvent(r->stream, r->joind, 0));
+        } else {
+            hipLaunchKernelGGL(k_t_deriv_x, dim3(nx, 4), dim3(64), 0, r->stream, geo(r), J, r->ckp, r->ckp_seg,
+                               r->ckp_lines);
+        }
     }
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1630,7 +1673,7 @@ int launch_rhs(vvt_run *r, int which)
     RF F{r->f[VV3_AN], r->f[VV3_BN], r->f[VV3_CN], r->f[VV3_WRH], r->f[VV3_VRH], r->f[VV3_URH]};
     const int ni = (which == 3) ? r->nr - 1 : r->nr, nj = (which == 1) ? r->nz - 1 : r->nz;
     const int nk = (which == 2) ? r->np - 1 : r->np;
-    hipLaunchKernelGGL(k_t_rhs, dim3(nblk((long long)ni * nj, WG), nk), dim3(WG), 0, r->stream, geo(r), F, which);
+    hipLaunchKernelGGL(k_t_rhs, dim3(nblk((long long)ni * nj, WG), nk), dim3(WG), 0, r->cur, geo(r), F, which);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
@@ -1642,7 +1685,7 @@ int launch_poisson(vvt_run *r, int which)
     const int psip = (which == 1) ? VV3_WELP : (which == 2) ? VV3_VELP : VV3_UELP;
     const int rhs = (which == 1) ? VV3_WRH : (which == 2) ? VV3_VRH : VV3_URH;
     const TGeo g = geo(r);
-    hipStream_t s = r->stream;
+    hipStream_t s = r->cur;
     double *P = r->f[psi];
     const int mt = O.m;
     const double ftr_y = 2.0 * O.k1 * O.k1 * (double)mt, ftr_z = 2.0 * O.j1 * O.j1 * (double)O.i1 * O.k1;
@@ -1670,41 +1713,41 @@ int launch_poisson(vvt_run *r, int which)
     if (r->yfft) {   // reads the right-hand side's unknowns, writes the spectra
         Prof p(r, VVT_K_YFFT_FWD, 8.0 * 2.0 * O.k1 * (double)mt);
         if (which == 2)
-            hipLaunchKernelGGL(k_t_yfft_fwd<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->S,
+            hipLaunchKernelGGL(k_t_yfft_fwd<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->cS,
                                tw, cq, O.i1, mt, LP, r->ybits);
         else
-            hipLaunchKernelGGL(k_t_yfft_fwd<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->S,
+            hipLaunchKernelGGL(k_t_yfft_fwd<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->f[rhs], r->cS,
                                tw, cq, O.i1, mt, LP, r->ybits);
     } else {
         Prof p(r, VVT_K_YFWD, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
         if (split)
-            vv::gemm_f64<true, false>(YFwdSplit{O.W, h, mt, h, r->f[rhs], r->S, O.i1, r->ldi, r->ldj, r->np,
+            vv::gemm_f64<true, false>(YFwdSplit{O.W, h, mt, h, r->f[rhs], r->cS, O.i1, r->ldi, r->ldj, r->np,
                                                 which == 2, O.k1}, 2, s);
         else
-            vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->S, O.i1, r->ldi, r->ldj}, 1, s);
+            vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->cS, O.i1, r->ldi, r->ldj}, 1, s);
     }
     {
         Prof p(r, VVT_K_ZFWD, ftr_z);
-        vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->S, r->T, mt}, O.k1, s);
+        vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->cS, r->cT, mt}, O.k1, s);
     }
     {   // reads T and writes it (algorithmic: once each)
         Prof p(r, VVT_K_XSOLVE, 8.0 * 2.0 * O.k1 * (double)mt);
         // checkpoints in cp: 2 ceil(i1/XCK) k1 j1 <= i1 k1 j1 words (i1 >= 5)
-        hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->T, r->cp,
+        hipLaunchKernelGGL(k_t_xsolve, dim3(nblk((long long)O.k1 * O.j1, 64)), dim3(64), 0, s, r->cT, r->ccp,
                            (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
                            O.en);
     }
     {
         Prof p(r, VVT_K_ZINV, ftr_z);
-        vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->T, r->S, mt}, O.k1, s);
+        vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->cT, r->cS, mt}, O.k1, s);
     }
     if (r->yfft) {
         Prof p(r, VVT_K_YFFT_INV, 8.0 * 2.0 * O.k1 * (double)mt);
         if (which == 2)
-            hipLaunchKernelGGL(k_t_yfft_inv<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->S, P, tw, cq,
+            hipLaunchKernelGGL(k_t_yfft_inv<true>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->cS, P, tw, cq,
                                O.i1, mt, LP, r->ybits);
         else
-            hipLaunchKernelGGL(k_t_yfft_inv<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->S, P, tw, cq,
+            hipLaunchKernelGGL(k_t_yfft_inv<false>, dim3(nbk), dim3(yt), flds, s, g, (const double *)r->cS, P, tw, cq,
                                O.i1, mt, LP, r->ybits);
         if (which != 2)
             hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
@@ -1712,11 +1755,11 @@ int launch_poisson(vvt_run *r, int which)
         Prof p(r, VVT_K_YINV, split ? 2.0 * (2.0 * h * h * (double)mt) : ftr_y);
         if (split) {
             // E and O go to T (free after the z inverse), then the planes
-            vv::gemm_f64<true, false>(YInvSplit{O.Wi, h, mt, h, r->S, r->T, O.k1}, 2, s);
-            hipLaunchKernelGGL(k_t_ycombine, dim3(nblk(mt, WG), h), dim3(WG), 0, s, g, (const double *)r->T, P, O.i1,
+            vv::gemm_f64<true, false>(YInvSplit{O.Wi, h, mt, h, r->cS, r->cT, O.k1}, 2, s);
+            hipLaunchKernelGGL(k_t_ycombine, dim3(nblk(mt, WG), h), dim3(WG), 0, s, g, (const double *)r->cT, P, O.i1,
                                mt, which == 2);
         } else {
-            vv::gemm_f64<true, false>(YInv{O.Wi, O.k1, mt, O.k1, r->S, P, O.i1, r->ldi, r->ldj}, 1, s);
+            vv::gemm_f64<true, false>(YInv{O.Wi, O.k1, mt, O.k1, r->cS, P, O.i1, r->ldi, r->ldj}, 1, s);
         }
         if (which != 2)
             hipLaunchKernelGGL(k_t_psi_post, dim3(nblk((long long)r->ldi * r->ldj, WG)), dim3(WG), 0, s, g, P);
@@ -1794,9 +1837,29 @@ int one_step(vvt_run *r)
     if ((st = launch_derivatives(r))) return st;
     if ((st = launch_explicit(r, 3))) return st;
     if ((st = launch_heat(r))) return st;
-    for (int w = 1; w <= 3; ++w) {
-        if ((st = launch_rhs(r, w))) return st;
-        if ((st = launch_poisson(r, w))) return st;
+    if (r->stream2 && !r->prof) {
+        // v's right-hand side reads An and Cn only (rot3dmg.f:1066-1093), so
+        // it and v's solve run on the second stream with their own work
+        // arrays while w's and u's (u's right-hand side reads the new w) run
+        // on the first; the update waits for both
+        VV_HIP(hipEventRecord(r->fork, r->stream));
+        VV_HIP(hipStreamWaitEvent(r->stream2, r->fork, 0));
+        r->cur = r->stream2; r->cS = r->S2; r->cT = r->T2; r->ccp = r->cp2;
+        st = launch_rhs(r, 2);
+        if (!st) st = launch_poisson(r, 2);
+        r->cur = r->stream; r->cS = r->S; r->cT = r->T; r->ccp = r->cp;
+        if (st) return st;
+        VV_HIP(hipEventRecord(r->join, r->stream2));
+        for (int w = 1; w <= 3; w += 2) {
+            if ((st = launch_rhs(r, w))) return st;
+            if ((st = launch_poisson(r, w))) return st;
+        }
+        VV_HIP(hipStreamWaitEvent(r->stream, r->join, 0));
+    } else {
+        for (int w = 1; w <= 3; ++w) {
+            if ((st = launch_rhs(r, w))) return st;
+            if ((st = launch_poisson(r, w))) return st;
+        }
     }
     return launch_update(r);
 }
@@ -1892,8 +1955,12 @@ int vvt_destroy(vvt_run *r)
         for (auto &O : r->op)
             for (double *p : {O.W, O.Wi, O.ly, O.V, O.Vinv, O.lz, O.rt})
                 if (p) (void)hipFree(p);
-        for (double *p : {r->tabs, r->S, r->T, r->cp, r->heat, r->kepart})
+        if (r->stream2) (void)hipStreamSynchronize(r->stream2);
+        for (double *p : {r->tabs, r->S, r->T, r->cp, r->S2, r->T2, r->cp2, r->ckpx, r->heat, r->kepart})
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : {r->fork, r->join, r->forkd, r->joind})
+            if (e) (void)hipEventDestroy(e);
+        if (r->stream2) (void)hipStreamDestroy(r->stream2);
         if (r->ytw) (void)hipFree(r->ytw);
         if (r->ckp) (void)hipFree(r->ckp);
         for (hipEvent_t e : r->ev)
