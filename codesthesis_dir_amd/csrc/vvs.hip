@@ -114,6 +114,7 @@ struct SOp {
 struct SPoisson {
     SOp op[3];
     double *S = nullptr, *T = nullptr, *cp = nullptr;
+    double *S2 = nullptr, *T2 = nullptr, *cp2 = nullptr;   // a second set: v's solve beside w's and u's
     bool fft = false;     // phi transforms by FFT (np a power of two, 8..2048) instead of the W / Wi GEMMs
     vvt::YTrans yt;
 };
@@ -1014,14 +1015,21 @@ int spoisson_setup(SPoisson &P, const Geo &g)
     VV_HIP(hipMalloc(&P.S, smax * sizeof(double)));
     VV_HIP(hipMalloc(&P.T, smax * sizeof(double)));
     VV_HIP(hipMalloc(&P.cp, cmax * sizeof(double)));
+    // the second set is optional: without it the solves run in turn
+    if (hipMalloc(&P.S2, smax * sizeof(double)) != hipSuccess || hipMalloc(&P.T2, smax * sizeof(double)) != hipSuccess ||
+        hipMalloc(&P.cp2, cmax * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        for (double **q : {&P.S2, &P.T2, &P.cp2})
+            if (*q) { (void)hipFree(*q); *q = nullptr; }
+    }
     return VV_OK;
 }
 
 void spoisson_free(SPoisson &P)
 {
-    for (double *p : {P.S, P.T, P.cp})
+    for (double *p : {P.S, P.T, P.cp, P.S2, P.T2, P.cp2})
         if (p) (void)hipFree(p);
-    P.S = P.T = P.cp = nullptr;
+    P.S = P.T = P.cp = P.S2 = P.T2 = P.cp2 = nullptr;
     vvt::ytrans_free(P.yt);
     for (auto &O : P.op) {
         for (double *p : {O.W, O.Wi, O.lphi, O.V, O.Vinv, O.lz, O.rt})
@@ -1031,20 +1039,21 @@ void spoisson_free(SPoisson &P)
 }
 
 int spoisson_launch(const SPoisson &P, const Geo &g, int which, double *psi, double *psip, const double *rhs,
-                    hipStream_t s)
+                    hipStream_t s, bool second)
 {
     const SOp &O = P.op[which - 1];
+    double *S = second ? P.S2 : P.S, *T = second ? P.T2 : P.T, *cp = second ? P.cp2 : P.cp;
     const int i1 = O.i1, j1 = O.j1, k1 = O.k1, mt = O.m;
     const dim3 pg((unsigned)((g.ldi * g.ldj + 255) / 256), (unsigned)(g.np + 2));
     hipLaunchKernelGGL(ks_psi_pre, pg, dim3(256), 0, s, g, which, (const double *)psi, psip);
-    if (P.fft) vvt::ytrans_fwd(P.yt, which == 2, rhs, P.S, i1, mt, s);
-    else vv::gemm_f64<true, false>(PhiFwd{O.W, k1, mt, k1, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
-    vv::gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, k1, s);
-    hipLaunchKernelGGL(ks_rsolve, dim3((unsigned)((k1 * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
+    if (P.fft) vvt::ytrans_fwd(P.yt, which == 2, rhs, S, i1, mt, s);
+    else vv::gemm_f64<true, false>(PhiFwd{O.W, k1, mt, k1, rhs, S, i1, g.ldi, g.ldj}, 1, s);
+    vv::gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, k1, s);
+    hipLaunchKernelGGL(ks_rsolve, dim3((unsigned)((k1 * j1 + 63) / 64)), dim3(64), 0, s, T, cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)O.lphi, i1, j1, k1, O.e1, O.en);
-    vv::gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, k1, s);
-    if (P.fft) vvt::ytrans_inv(P.yt, which == 2, P.S, psi, i1, mt, s);
-    else vv::gemm_f64<true, false>(PhiInv{O.Wi, k1, mt, k1, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+    vv::gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, k1, s);
+    if (P.fft) vvt::ytrans_inv(P.yt, which == 2, S, psi, i1, mt, s);
+    else vv::gemm_f64<true, false>(PhiInv{O.Wi, k1, mt, k1, S, psi, i1, g.ldi, g.ldj}, 1, s);
     hipLaunchKernelGGL(ks_psi_post, pg, dim3(256), 0, s, g, which, psi);
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1066,6 +1075,8 @@ struct vvs_run {
     double *f[VV3_NFIELD] = {nullptr};
     vvs::SPoisson pois;
     vvs::SK K;                           // the explicit routines' constant divisors
+    hipStream_t stream2 = nullptr;       // heat and v's right-hand side and solve, beside w's and u's
+    hipEvent_t fork = nullptr, join = nullptr;
     double *heat = nullptr;              // plane fluxes k = 0..np+1, then trans1
     double *nu = nullptr;                // local Nusselt numbers: inner, outer (np+1 each)
 };
@@ -1102,32 +1113,32 @@ static int launch_explicit(vvs_run *r, int mask)
     return VV_OK;
 }
 
-static int launch_heat(vvs_run *r)
+static int launch_heat(vvs_run *r, hipStream_t s)
 {
     hipLaunchKernelGGL(ks_heat_planes, dim3((unsigned)(r->g.np + 2)), dim3(256), (size_t)(r->g.nz + 2) * sizeof(double),
-                       r->stream, r->g, (const double *)r->f[VV3_TH], r->heat);
-    hipLaunchKernelGGL(ks_heat_sum, dim3(1), dim3(64), 0, r->stream, r->g, r->heat);
+                       s, r->g, (const double *)r->f[VV3_TH], r->heat);
+    hipLaunchKernelGGL(ks_heat_sum, dim3(1), dim3(64), 0, s, r->g, r->heat);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
-static int launch_rhs(vvs_run *r, int which)
+static int launch_rhs(vvs_run *r, int which, hipStream_t s)
 {
     const SF F = sfields(r);
     const dim3 gr = pgrid(r);
-    if (which == 1) hipLaunchKernelGGL(ks_rhs<1>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
-    else if (which == 2) hipLaunchKernelGGL(ks_rhs<2>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
-    else hipLaunchKernelGGL(ks_rhs<3>, gr, dim3(256), 0, r->stream, r->g, F, r->K);
+    if (which == 1) hipLaunchKernelGGL(ks_rhs<1>, gr, dim3(256), 0, s, r->g, F, r->K);
+    else if (which == 2) hipLaunchKernelGGL(ks_rhs<2>, gr, dim3(256), 0, s, r->g, F, r->K);
+    else hipLaunchKernelGGL(ks_rhs<3>, gr, dim3(256), 0, s, r->g, F, r->K);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
-static int launch_poisson(vvs_run *r, int which)
+static int launch_poisson(vvs_run *r, int which, hipStream_t s, bool second = false)
 {
     static const int PSI[3][3] = {{VV3_WEL, VV3_WELP, VV3_WRH}, {VV3_VEL, VV3_VELP, VV3_VRH},
                                   {VV3_UEL, VV3_UELP, VV3_URH}};
     const int *p = PSI[which - 1];
-    return spoisson_launch(r->pois, r->g, which, r->f[p[0]], r->f[p[1]], r->f[p[2]], r->stream);
+    return spoisson_launch(r->pois, r->g, which, r->f[p[0]], r->f[p[1]], r->f[p[2]], s, second);
 }
 
 static int launch_update(vvs_run *r)
@@ -1238,6 +1249,10 @@ int vvs_create(const vv3_config *cfg, int device, vvs_run **out)
         const int st = spoisson_setup(r->pois, g);
         if (st) return fail(st);
     }
+    if (r->pois.S2 && (hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking) != hipSuccess ||
+                       hipEventCreateWithFlags(&r->fork, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&r->join, hipEventDisableTiming) != hipSuccess))
+        return fail(VV_EHIP);
     if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
     *out = r;
     return VV_OK;
@@ -1252,8 +1267,12 @@ int vvs_destroy(vvs_run *r)
         if (r->stream) (void)hipStreamSynchronize(r->stream);
         for (int q = 0; q < VV3_NFIELD; ++q)
             if (r->f[q]) (void)hipFree(r->f[q]);
+        if (r->stream2) (void)hipStreamSynchronize(r->stream2);
         spoisson_free(r->pois);
         if (r->heat) (void)hipFree(r->heat);
+        for (hipEvent_t e : {r->fork, r->join})
+            if (e) (void)hipEventDestroy(e);
+        if (r->stream2) (void)hipStreamDestroy(r->stream2);
         if (r->nu) (void)hipFree(r->nu);
         if (r->stream) (void)hipStreamDestroy(r->stream);
     }
@@ -1300,7 +1319,7 @@ int vvs_heat(vvs_run *r, double *trans1)
     if (!r || !trans1) { vv::set_error("vvs_heat: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    VVS_TRY(launch_heat(r));
+    VVS_TRY(launch_heat(r, r->stream));
     VV_HIP(hipMemcpyAsync(trans1, r->heat + r->g.np + 2, sizeof(double), hipMemcpyDeviceToHost, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
@@ -1326,7 +1345,7 @@ int vvs_rhs(vvs_run *r, int which)
     if (!r || which < 1 || which > 3) { vv::set_error("vvs_rhs: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    VVS_TRY(launch_rhs(r, which));
+    VVS_TRY(launch_rhs(r, which, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
@@ -1336,7 +1355,7 @@ int vvs_poisson(vvs_run *r, int which)
     if (!r || which < 1 || which > 3) { vv::set_error("vvs_poisson: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    VVS_TRY(launch_poisson(r, which));
+    VVS_TRY(launch_poisson(r, which, r->stream));
     VV_HIP(hipStreamSynchronize(r->stream));
     return VV_OK;
 }
@@ -1358,13 +1377,29 @@ int vvs_step(vvs_run *r, int n, double *trans1)
     VV_HIP(hipSetDevice(r->device));
     for (int s = 0; s < n; ++s) {
         VVS_TRY(launch_explicit(r, 15));
-        VVS_TRY(launch_heat(r));
-        VVS_TRY(launch_rhs(r, 1));
-        VVS_TRY(launch_poisson(r, 1));
-        VVS_TRY(launch_rhs(r, 2));
-        VVS_TRY(launch_poisson(r, 2));
-        VVS_TRY(launch_rhs(r, 3));
-        VVS_TRY(launch_poisson(r, 3));
+        if (r->stream2) {
+            // the heat flux (it reads th) and v's right-hand side (An and Cn
+            // only, sect360v2mg.f:1034-1063) and solve on the second stream
+            // with the second work arrays, w's and u's (u's right-hand side
+            // reads the new w) on the first; the update waits for both
+            VV_HIP(hipEventRecord(r->fork, r->stream));
+            VV_HIP(hipStreamWaitEvent(r->stream2, r->fork, 0));
+            VVS_TRY(launch_heat(r, r->stream2));
+            VVS_TRY(launch_rhs(r, 2, r->stream2));
+            VVS_TRY(launch_poisson(r, 2, r->stream2, true));
+            VV_HIP(hipEventRecord(r->join, r->stream2));
+            VVS_TRY(launch_rhs(r, 1, r->stream));
+            VVS_TRY(launch_poisson(r, 1, r->stream));
+            VVS_TRY(launch_rhs(r, 3, r->stream));
+            VVS_TRY(launch_poisson(r, 3, r->stream));
+            VV_HIP(hipStreamWaitEvent(r->stream, r->join, 0));
+        } else {
+            VVS_TRY(launch_heat(r, r->stream));
+            for (int w = 1; w <= 3; ++w) {
+                VVS_TRY(launch_rhs(r, w, r->stream));
+                VVS_TRY(launch_poisson(r, w, r->stream));
+            }
+        }
         VVS_TRY(launch_update(r));
     }
     if (trans1)

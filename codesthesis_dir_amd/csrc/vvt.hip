@@ -1657,11 +1657,11 @@ int launch_heat(vvt_run *r)
 {
     Prof p(r, VVT_K_HEAT, 8.0 * 10.0 * r->nr * (r->np / 2 + 1));
     double *E = r->heat + 2 * (size_t)r->nr + 2;
-    hipLaunchKernelGGL(k_t_heat_terms, dim3(nblk(r->nr, WG), r->np - r->np / 2 + 1), dim3(WG), 0, r->stream, geo(r),
+    hipLaunchKernelGGL(k_t_heat_terms, dim3(nblk(r->nr, WG), r->np - r->np / 2 + 1), dim3(WG), 0, r->cur, geo(r),
                        (const double *)r->f[VV3_TH], E);
-    hipLaunchKernelGGL(k_t_heat_cols, dim3(nblk(r->nr, WG)), dim3(WG), 0, r->stream, geo(r), (const double *)E,
+    hipLaunchKernelGGL(k_t_heat_cols, dim3(nblk(r->nr, WG)), dim3(WG), 0, r->cur, geo(r), (const double *)E,
                        r->heat);
-    hipLaunchKernelGGL(k_t_heat_sum, dim3(1), dim3(64), 0, r->stream, r->nr, r->dr, (const double *)r->heat,
+    hipLaunchKernelGGL(k_t_heat_sum, dim3(1), dim3(64), 0, r->cur, r->nr, r->dr, (const double *)r->heat,
                        heat_out(r));
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1836,16 +1836,17 @@ int one_step(vvt_run *r)
     int st;
     if ((st = launch_derivatives(r))) return st;
     if ((st = launch_explicit(r, 3))) return st;
-    if ((st = launch_heat(r))) return st;
     if (r->stream2 && !r->prof) {
-        // v's right-hand side reads An and Cn only (rot3dmg.f:1066-1093), so
-        // it and v's solve run on the second stream with their own work
-        // arrays while w's and u's (u's right-hand side reads the new w) run
-        // on the first; the update waits for both
+        // the heat fluxes (they read th) and v's right-hand side (it reads
+        // An and Cn only, rot3dmg.f:1066-1093) and solve run on the second
+        // stream, v's solve with its own work arrays, while w's and u's (u's
+        // right-hand side reads the new w) run on the first; the update
+        // waits for both
         VV_HIP(hipEventRecord(r->fork, r->stream));
         VV_HIP(hipStreamWaitEvent(r->stream2, r->fork, 0));
         r->cur = r->stream2; r->cS = r->S2; r->cT = r->T2; r->ccp = r->cp2;
-        st = launch_rhs(r, 2);
+        st = launch_heat(r);
+        if (!st) st = launch_rhs(r, 2);
         if (!st) st = launch_poisson(r, 2);
         r->cur = r->stream; r->cS = r->S; r->cT = r->T; r->ccp = r->cp;
         if (st) return st;
@@ -1856,6 +1857,7 @@ int one_step(vvt_run *r)
         }
         VV_HIP(hipStreamWaitEvent(r->stream, r->join, 0));
     } else {
+        if ((st = launch_heat(r))) return st;
         for (int w = 1; w <= 3; ++w) {
             if ((st = launch_rhs(r, w))) return st;
             if ((st = launch_poisson(r, w))) return st;
