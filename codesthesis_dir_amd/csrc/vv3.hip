@@ -31,6 +31,8 @@ struct vv3_run {
     size_t nf = 0;                       // doubles per field
     double *f[VV3_NFIELD] = {nullptr};   // device fields (rotated by pointer swap)
     vv3::Poisson3 pois;                  // the three Poisson operators
+    hipStream_t stream2 = nullptr;       // heat and v's right-hand side and solve, beside w's and u's
+    hipEvent_t fork = nullptr, join = nullptr;
     double *heat = nullptr;              // flux terms (nz+2) x np, then the totals trans1, trans2
     double *trig = nullptr;              // VV3_MODEL_HOR: sin, cos of each plane's azimuth (2 x (np+1))
     double *kepart = nullptr;            // kinetic energy: partials [KE3_MAXB], then the total
@@ -1051,6 +1053,10 @@ int vv3_create(const vv3_config *cfg, int device, vv3_run **out)
         const int st = poisson3_setup(r->pois, geo(r));
         if (st) return fail(st);
     }
+    if (r->pois.S2 && (hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking) != hipSuccess ||
+                       hipEventCreateWithFlags(&r->fork, hipEventDisableTiming) != hipSuccess ||
+                       hipEventCreateWithFlags(&r->join, hipEventDisableTiming) != hipSuccess))
+        return fail(VV_EHIP);
     if (hipStreamSynchronize(r->stream) != hipSuccess) return fail(VV_EHIP);
     *out = r;
     return VV_OK;
@@ -1063,9 +1069,13 @@ int vv3_destroy(vv3_run *r)
         std::lock_guard<std::recursive_mutex> lk(r->mu);
         (void)hipSetDevice(r->device);
         if (r->stream) (void)hipStreamSynchronize(r->stream);
+        if (r->stream2) (void)hipStreamSynchronize(r->stream2);
         for (int q = 0; q < VV3_NFIELD; ++q)
             if (r->f[q]) (void)hipFree(r->f[q]);
         poisson3_free(r->pois);
+        for (hipEvent_t e : {r->fork, r->join})
+            if (e) (void)hipEventDestroy(e);
+        if (r->stream2) (void)hipStreamDestroy(r->stream2);
         if (r->heat) (void)hipFree(r->heat);
         if (r->trig) (void)hipFree(r->trig);
         if (r->kepart) (void)hipFree(r->kepart);
@@ -1147,20 +1157,20 @@ static Fields fields(vv3_run *r)
 // heat's terms and sums; the total lands at heat_total(r)
 static double *heat_total(vv3_run *r) { return r->heat + (size_t)(r->nz + 3) * r->np; }
 
-static int launch_heat(vv3_run *r)
+static int launch_heat(vv3_run *r, hipStream_t s)
 {
     const Geo g = geo(r);
     double *hs = r->heat;
     if (g.model == VV3_MODEL_HOR) {
-        hipLaunchKernelGGL(k3_heat_hor_terms, dim3(nblk(r->np, WG)), dim3(WG), 0, r->stream, g,
+        hipLaunchKernelGGL(k3_heat_hor_terms, dim3(nblk(r->np, WG)), dim3(WG), 0, s, g,
                            (const double *)r->f[VV3_TH], hs);
-        hipLaunchKernelGGL(k3_heat_hor_sum, dim3(1), dim3(128), 0, r->stream, g, (const double *)hs, heat_total(r));
+        hipLaunchKernelGGL(k3_heat_hor_sum, dim3(1), dim3(128), 0, s, g, (const double *)hs, heat_total(r));
         VV_HIP(hipGetLastError());
         return VV_OK;
     }
-    hipLaunchKernelGGL(k3_heat_terms, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, r->stream, g,
+    hipLaunchKernelGGL(k3_heat_terms, dim3(nblk((long long)(r->nz + 2) * r->np, WG)), dim3(WG), 0, s, g,
                        (const double *)r->f[VV3_TH], hs);
-    hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), r->np * sizeof(double), r->stream, g, (const double *)hs,
+    hipLaunchKernelGGL(k3_heat_sum, dim3(1), dim3(WG), r->np * sizeof(double), s, g, (const double *)hs,
                        heat_total(r));
     VV_HIP(hipGetLastError());
     return VV_OK;
@@ -1227,23 +1237,23 @@ static int launch_explicit(vv3_run *r, int mask)
     return (r->cfg.model == VV3_MODEL_HOR) ? launch_explicit_m<1>(r, mask) : launch_explicit_m<0>(r, mask);
 }
 
-static int launch_rhs(vv3_run *r, int which)
+static int launch_rhs(vv3_run *r, int which, hipStream_t s)
 {
     RhsF F;
     F.An = r->f[VV3_AN]; F.Bn = r->f[VV3_BN]; F.Cn = r->f[VV3_CN]; F.wel = r->f[VV3_WEL];
     F.wrh = r->f[VV3_WRH]; F.vrh = r->f[VV3_VRH]; F.urh = r->f[VV3_URH];
     const long long cnt = (long long)r->nr * r->nz;
-    hipLaunchKernelGGL(k3_rhs, dim3(nblk(cnt, WG), r->np + 1), dim3(WG), 0, r->stream, geo(r), F, which);
+    hipLaunchKernelGGL(k3_rhs, dim3(nblk(cnt, WG), r->np + 1), dim3(WG), 0, s, geo(r), F, which);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
 
-static int launch_poisson(vv3_run *r, int which)
+static int launch_poisson(vv3_run *r, int which, hipStream_t s, bool second = false)
 {
     const int rhs = (which == 1) ? VV3_WRH : (which == 2) ? VV3_VRH : VV3_URH;
     const int psi = (which == 1) ? VV3_WEL : (which == 2) ? VV3_VEL : VV3_UEL;
     const int psip = (which == 1) ? VV3_WELP : (which == 2) ? VV3_VELP : VV3_UELP;
-    return poisson3_launch(r->pois, geo(r), which, r->f[psi], r->f[psip], r->f[rhs], r->stream);
+    return poisson3_launch(r->pois, geo(r), which, r->f[psi], r->f[psip], r->f[rhs], s, second);
 }
 
 static int launch_update(vv3_run *r)
@@ -1292,7 +1302,7 @@ int vv3_heat(vv3_run *r, double *trans1)
     if (!r) { vv::set_error("vv3: null run"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    const int st = launch_heat(r);
+    const int st = launch_heat(r, r->stream);
     if (st) return st;
     double t = 0.0;
     VV_HIP(hipMemcpyAsync(&t, heat_total(r), sizeof(double), hipMemcpyDeviceToHost, r->stream));
@@ -1398,7 +1408,7 @@ int vv3_rhs(vv3_run *r, int which)
     if (!r || which < 1 || which > 3) { vv::set_error("vv3_rhs: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    int st = launch_rhs(r, which);
+    int st = launch_rhs(r, which, r->stream);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
     return st;
 }
@@ -1408,7 +1418,7 @@ int vv3_poisson(vv3_run *r, int which)
     if (!r || which < 1 || which > 3) { vv::set_error("vv3_poisson: bad argument"); return VV_EINVAL; }
     std::lock_guard<std::recursive_mutex> lk(r->mu);
     VV_HIP(hipSetDevice(r->device));
-    int st = launch_poisson(r, which);
+    int st = launch_poisson(r, which, r->stream);
     if (!st) VV_HIP(hipStreamSynchronize(r->stream));
     return st;
 }
@@ -1430,13 +1440,29 @@ int vv3_step(vv3_run *r, int n, double *trans1)
     VV_HIP(hipSetDevice(r->device));
     for (int s = 0; s < n; ++s) {
         int st = launch_explicit(r, 15);
-        if (!st) st = launch_heat(r);
-        if (!st) st = launch_rhs(r, 1);
-        if (!st) st = launch_poisson(r, 1);
-        if (!st) st = launch_rhs(r, 2);
-        if (!st) st = launch_poisson(r, 2);
-        if (!st) st = launch_rhs(r, 3);
-        if (!st) st = launch_poisson(r, 3);
+        if (!st && r->stream2) {
+            // the heat flux (it reads th) and v's right-hand side (An and Cn
+            // only, rot3dmg.f:1066-1093) and solve on the second stream with
+            // the second work arrays, w's and u's (u's right-hand side reads
+            // the new w) on the first; the update waits for both
+            VV_HIP(hipEventRecord(r->fork, r->stream));
+            VV_HIP(hipStreamWaitEvent(r->stream2, r->fork, 0));
+            st = launch_heat(r, r->stream2);
+            if (!st) st = launch_rhs(r, 2, r->stream2);
+            if (!st) st = launch_poisson(r, 2, r->stream2, true);
+            if (!st) VV_HIP(hipEventRecord(r->join, r->stream2));
+            if (!st) st = launch_rhs(r, 1, r->stream);
+            if (!st) st = launch_poisson(r, 1, r->stream);
+            if (!st) st = launch_rhs(r, 3, r->stream);
+            if (!st) st = launch_poisson(r, 3, r->stream);
+            if (!st) VV_HIP(hipStreamWaitEvent(r->stream, r->join, 0));
+        } else {
+            if (!st) st = launch_heat(r, r->stream);
+            for (int w = 1; w <= 3 && !st; ++w) {
+                st = launch_rhs(r, w, r->stream);
+                if (!st) st = launch_poisson(r, w, r->stream);
+            }
+        }
         if (!st) st = launch_update(r);
         if (st) return st;
     }

@@ -42,6 +42,7 @@ struct Poisson3 {
     double *S = nullptr;     // np x mmax: phi spectra, then the z-inverse result
     double *T = nullptr;     // np x mmax: z spectra, (i, l) with l fastest
     double *cp = nullptr;    // nm x mmax: elimination factors of the r solves
+    double *S2 = nullptr, *T2 = nullptr, *cp2 = nullptr;   // a second set (a solve beside another), or none
     double *tw = nullptr;    // np/2 complex (cos, -sin)(2 pi t/np): FFT twiddles (np a power of two)
     bool fft = false;        // phi transforms by FFT (np a power of two, 8..4096) instead of GEMM
 };
@@ -67,7 +68,8 @@ int poisson3_setup(Poisson3 &P, const Geo &g);
 void poisson3_free(Poisson3 &P);
 // solve L psi = rhs for operator `which`; psip <- old psi and walls 0 on the
 // range the reference writes (rot3dmg.f:1294-1318, 1400-1424, 1509-1533)
+// second: with the second set of work arrays (P.S2 != nullptr)
 int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, double *psip, const double *rhs,
-                    hipStream_t s);
+                    hipStream_t s, bool second = false);
 
 }  // namespace vv3

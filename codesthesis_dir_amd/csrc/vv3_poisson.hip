@@ -628,14 +628,21 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
     VV_HIP(hipMalloc(&P.S, (size_t)np * mmax * sizeof(double)));
     VV_HIP(hipMalloc(&P.T, (size_t)np * mmax * sizeof(double)));
     VV_HIP(hipMalloc(&P.cp, (size_t)nm * mmax * sizeof(double)));
+    if (hipMalloc(&P.S2, (size_t)np * mmax * sizeof(double)) != hipSuccess ||
+        hipMalloc(&P.T2, (size_t)np * mmax * sizeof(double)) != hipSuccess ||
+        hipMalloc(&P.cp2, (size_t)nm * mmax * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        for (double **q : {&P.S2, &P.T2, &P.cp2})
+            if (*q) { (void)hipFree(*q); *q = nullptr; }
+    }
     return VV_OK;
 }
 
 void poisson3_free(Poisson3 &P)
 {
-    for (double *p : {P.W, P.Wi, P.lphi, P.S, P.T, P.cp, P.tw})
+    for (double *p : {P.W, P.Wi, P.lphi, P.S, P.T, P.cp, P.S2, P.T2, P.cp2, P.tw})
         if (p) (void)hipFree(p);
-    P.W = P.Wi = P.lphi = P.S = P.T = P.cp = P.tw = nullptr;
+    P.W = P.Wi = P.lphi = P.S = P.T = P.cp = P.S2 = P.T2 = P.cp2 = P.tw = nullptr;
     for (auto &O : P.op) {
         for (double *p : {O.V, O.Vinv, O.rt, O.lz})
             if (p) (void)hipFree(p);
@@ -644,9 +651,10 @@ void poisson3_free(Poisson3 &P)
 }
 
 int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, double *psip, const double *rhs,
-                    hipStream_t s)
+                    hipStream_t s, bool second)
 {
     const PoissonOp &O = P.op[which - 1];
+    double *const S = second ? P.S2 : P.S, *const T = second ? P.T2 : P.T, *const cpw = second ? P.cp2 : P.cp;
     const int nm = P.nm, np = P.np, i1 = O.i1, j1 = O.j1, mt = O.m;
     hipLaunchKernelGGL(k3_psi_pre, dim3((unsigned)((g.ldi * g.ldj + 255) / 256), np + 1), dim3(256), 0, s, g, which,
                        i1, j1, psi, psip);
@@ -661,31 +669,31 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
     const size_t flds = (size_t)LP * np * sizeof(double2);
     const double2 *tw = (const double2 *)P.tw;
     if (P.fft && fR == 0)
-        hipLaunchKernelGGL(k3_fft_fwd<0>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, 1, LP, i1, j1, g.ldi,
+        hipLaunchKernelGGL(k3_fft_fwd<0>, fgrid, dim3(256), flds, s, rhs, S, tw, np, bits, 1, LP, i1, j1, g.ldi,
                            g.ldj, mt);
     else if (P.fft && fR == 8)
-        hipLaunchKernelGGL(k3_fft_fwd<8>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, fodd, LP, i1, j1, g.ldi,
+        hipLaunchKernelGGL(k3_fft_fwd<8>, fgrid, dim3(256), flds, s, rhs, S, tw, np, bits, fodd, LP, i1, j1, g.ldi,
                            g.ldj, mt);
     else if (P.fft)
-        hipLaunchKernelGGL(k3_fft_fwd<32>, fgrid, dim3(256), flds, s, rhs, P.S, tw, np, bits, fodd, LP, i1, j1,
+        hipLaunchKernelGGL(k3_fft_fwd<32>, fgrid, dim3(256), flds, s, rhs, S, tw, np, bits, fodd, LP, i1, j1,
                            g.ldi, g.ldj, mt);
     else
-        gemm_f64<true, false>(PhiFwd{P.W, np, mt, np, rhs, P.S, i1, g.ldi, g.ldj}, 1, s);
-    gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, P.S, P.T, mt}, np, s);
-    hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, P.T, P.cp,
+        gemm_f64<true, false>(PhiFwd{P.W, np, mt, np, rhs, S, i1, g.ldi, g.ldj}, 1, s);
+    gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, np, s);
+    hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, T, cpw,
                        (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
-    gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, P.T, P.S, mt}, np, s);
+    gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, np, s);
     if (P.fft && fR == 0)
-        hipLaunchKernelGGL(k3_fft_inv<0>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, 1, LP, i1,
+        hipLaunchKernelGGL(k3_fft_inv<0>, fgrid, dim3(256), flds, s, (const double *)S, psi, tw, np, bits, 1, LP, i1,
                            j1, g.ldi, g.ldj, mt);
     else if (P.fft && fR == 8)
-        hipLaunchKernelGGL(k3_fft_inv<8>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, fodd, LP,
+        hipLaunchKernelGGL(k3_fft_inv<8>, fgrid, dim3(256), flds, s, (const double *)S, psi, tw, np, bits, fodd, LP,
                            i1, j1, g.ldi, g.ldj, mt);
     else if (P.fft)
-        hipLaunchKernelGGL(k3_fft_inv<32>, fgrid, dim3(256), flds, s, (const double *)P.S, psi, tw, np, bits, fodd,
+        hipLaunchKernelGGL(k3_fft_inv<32>, fgrid, dim3(256), flds, s, (const double *)S, psi, tw, np, bits, fodd,
                            LP, i1, j1, g.ldi, g.ldj, mt);
     else
-        gemm_f64<true, false>(PhiInv{P.Wi, np, mt, np, P.S, psi, i1, g.ldi, g.ldj}, 1, s);
+        gemm_f64<true, false>(PhiInv{P.Wi, np, mt, np, S, psi, i1, g.ldi, g.ldj}, 1, s);
     VV_HIP(hipGetLastError());
     return VV_OK;
 }
