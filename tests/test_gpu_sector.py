@@ -243,3 +243,23 @@ def test_local_nu_matches_the_pinned_formula():
     oa, ob = ora.local_nu()
     assert np.array_equal(a, oa[:np_ + 1]) and np.array_equal(b, ob[:np_ + 1])
     run.close()
+
+
+def test_trajectory_200_steps_matches_oracle():
+    """200 steps at 8 x 6 x 8 from the reference's start (with the oracle's
+    dense solves each step): heat flux and fields within 1e-9."""
+    nr, nz, np_ = 8, 6, 8
+    print("[gpu] sector 200-step trajectory", flush=True)
+    run = _gpu(nr, nz, np_)
+    ora = _oracle_run(nr, nz, np_, PHYS)
+    for k, v in sector_state(nr, nz, np_).items():
+        run.put(k, v)
+    for s in range(200):
+        tg = run.step(1)
+        to = ora.step()
+        assert abs(tg - to) <= 1e-9 * abs(to), (s, tg, to)
+    for n in ("A", "B", "C", "th", "uel", "vel", "wel"):
+        got, want = run.get(n), ora[n]
+        scale = max(np.max(np.abs(want)), 1e-300)
+        assert np.max(np.abs(got - want)) <= 1e-9 * scale, n
+    run.close()
