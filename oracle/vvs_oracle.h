@@ -24,11 +24,11 @@
 extern "C" {
 #endif
 
-void vs_avorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:343-450 */
-void vs_bvorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:460-567 */
-void vs_cvorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:577-682 */
-void vs_temperature(const v3_params *P, v3_fields *F);  /* sect360v2mg.f:692-825 */
-double vs_heat(const v3_params *P, const v3_fields *F); /* sect360v2mg.f:840-879; trans1 */
+void vs_avorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:343-452 */
+void vs_bvorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:460-569 */
+void vs_cvorticity(const v3_params *P, v3_fields *F);   /* sect360v2mg.f:577-684 */
+void vs_temperature(const v3_params *P, v3_fields *F);  /* sect360v2mg.f:692-831 */
+double vs_heat(const v3_params *P, const v3_fields *F); /* sect360v2mg.f:840-881; trans1 */
 void vs_update(const v3_params *P, v3_fields *F);       /* sect360v2mg.f:936-996 */
 void vs_rhsw(const v3_params *P, v3_fields *F);         /* sect360v2mg.f:999-1031 */
 void vs_rhsv(const v3_params *P, v3_fields *F);         /* sect360v2mg.f:1034-1063 */
