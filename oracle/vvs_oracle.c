@@ -17,7 +17,7 @@
 #define X(a, i, j, k) ((a)[IX(i, j, k)])
 
 /* Avorticity, sect360v2mg.f:343-452: k = 1..np-1 (no wrap), the outer-wall
- * row i = nr with the one-sided Ar of dum15 (sect360v2mg.f:1269-1276) */
+ * row i = nr with the one-sided Ar of dum15 (sect360v2mg.f:1268-1275) */
 void vs_avorticity(const v3_params *P, v3_fields *F)
 {
     const int nr = P->nr, nz = P->nz, np = P->np;
@@ -103,7 +103,7 @@ void vs_avorticity(const v3_params *P, v3_fields *F)
 }
 
 /* Bvorticity, sect360v2mg.f:460-569: k = 1..np, the barrier rows k = 1 and
- * k = np with the wall closures of sect360v2mg.f:527-535 */
+ * k = np with the wall closures of sect360v2mg.f:523-532 */
 void vs_bvorticity(const v3_params *P, v3_fields *F)
 {
     const int nr = P->nr, nz = P->nz, np = P->np;

@@ -72,7 +72,7 @@ def test_rhs_w_u_interior_match_rot3dmg():
         got, want = s[n][sl], a[n][sl]
         assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want)), n
     # at k = 1 the sector leaves out the barrier plane's An and Cn
-    # (sect360v2mg.f:1022-1023, 1091-1092): rot3dmg.f's formula with those zero
+    # (sect360v2mg.f:1020-1021, 1090-1091): rot3dmg.f's formula with those zero
     s2, a2 = _pair(nr, nz, np_, seed=5)
     for f in (s2, a2):
         f["An"][0] = 0.0
