@@ -124,7 +124,7 @@ namespace {
 #define X(a, i, j, k) ((a)[((size_t)(k) * g.ldj + (size_t)(j)) * g.ldi + (size_t)(i)])
 #define XW(a, i, j, k) X(a, i, j, k)
 
-// thh of the row kk of temperature's interior branch (sect360v2mg.f:793-805):
+// thh of the row kk of temperature's interior branch (sect360v2mg.f:799-809):
 // the value the reference's row k = np reuses
 __device__ __forceinline__ double thh_row(const Geo &g, const SF &F, const SK &K, int i, int j, int kk)
 {
