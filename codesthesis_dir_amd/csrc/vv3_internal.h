@@ -63,6 +63,19 @@ std::vector<double> z_matrix(int which, int j1);
 // the decomposition does not reproduce the matrix.
 bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam, std::vector<double> &V,
                   std::vector<double> &Vinv);
+// The z matrices are persymmetric (the same read from the other wall), so
+// each eigenvector is even or odd under j -> n-1-j, and so is the matching
+// row of V^-1.  The transforms then split into two half-size products: the
+// even modes take the sums x_j + x_{n-1-j} (and the middle row of odd n),
+// the odd modes the differences; back, E (even modes) and O (odd modes) give
+// x_j = E_j + O_j and x_{n-1-j} = E_j - O_j.  z_split orders the eigenpairs
+// even first (he = n - n/2 of them), then odd (n/2), permuting lam, and
+// forms the half tables, he = n - n/2 square each, zero-padded:
+// F[b][l][j] = V^-1[mode l of half b][j], G[b][j][l] = V[j][mode l of half b]
+// (b = 0 even, 1 odd; j < he, resp. n/2).  False (lam unchanged) if a mode
+// is not of one parity to 1e-9 of its largest entry.
+bool z_split(int n, std::vector<double> &lam, const std::vector<double> &V, const std::vector<double> &Vinv,
+             std::vector<double> &F, std::vector<double> &G);
 
 int poisson3_setup(Poisson3 &P, const Geo &g);
 void poisson3_free(Poisson3 &P);

@@ -544,6 +544,44 @@ bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam,
     return e1 <= 1e-11 * R && e2 <= 1e-10;
 }
 
+bool z_split(int n, std::vector<double> &lam, const std::vector<double> &V, const std::vector<double> &Vinv,
+             std::vector<double> &F, std::vector<double> &G)
+{
+    const int h = n / 2, he = n - h;
+    std::vector<int> ev, od;
+    for (int l = 0; l < n; ++l) {
+        // parity of column l of V and of row l of V^-1 (the same, since
+        // the reversal J commutes with the matrix: J V = V D, V^-1 J = D V^-1)
+        double s = 0.0, d = 0.0, nv = 0.0, si = 0.0, di = 0.0, ni = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double a = V[(size_t)j * n + l], b = V[(size_t)(n - 1 - j) * n + l];
+            const double c = Vinv[(size_t)l * n + j], e = Vinv[(size_t)l * n + (n - 1 - j)];
+            s = std::max(s, std::fabs(a + b)); d = std::max(d, std::fabs(a - b)); nv = std::max(nv, std::fabs(a));
+            si = std::max(si, std::fabs(c + e)); di = std::max(di, std::fabs(c - e)); ni = std::max(ni, std::fabs(c));
+        }
+        if (d <= 1e-9 * nv && di <= 1e-9 * ni) ev.push_back(l);
+        else if (s <= 1e-9 * nv && si <= 1e-9 * ni) od.push_back(l);
+        else return false;
+    }
+    if ((int)ev.size() != he || (int)od.size() != h) return false;
+    std::vector<double> lp(n);
+    F.assign((size_t)2 * he * he, 0.0);
+    G.assign((size_t)2 * he * he, 0.0);
+    for (int b = 0; b < 2; ++b) {
+        const std::vector<int> &P = b ? od : ev;
+        const int nj = b ? h : he;   // odd modes vanish on the middle row of odd n
+        for (int l = 0; l < (int)P.size(); ++l) {
+            lp[(size_t)b * he + l] = lam[P[l]];
+            for (int j = 0; j < nj; ++j) {
+                F[((size_t)b * he + l) * he + j] = Vinv[(size_t)P[l] * n + j];
+                G[((size_t)b * he + j) * he + l] = V[(size_t)j * n + P[l]];
+            }
+        }
+    }
+    lam.swap(lp);
+    return true;
+}
+
 int poisson3_setup(Poisson3 &P, const Geo &g)
 {
     using vv::set_error;

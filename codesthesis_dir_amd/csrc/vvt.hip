@@ -795,6 +795,65 @@ struct ZInv {
     __device__ double b(int q, int k, int n) const { return T[(size_t)q * mt + (size_t)n * K + k]; }
     __device__ void c(int q, int m, int n, double v) const { S[(size_t)q * mt + (size_t)m * N + n] = v; }
 };
+// The z transforms split by parity (the z rows read the same from either
+// wall): batch 2q + b, b = 0 the even modes, 1 the odd ones, he = n - n/2
+// rows and modes per half.  Forward: T[q][i][b he + l] = sum_j F[b][l][j]
+// (S[q][j][i] +- S[q][n-1-j][i]) over j < n/2, the middle row of odd n
+// joining the even half alone.
+struct ZFwdSplit {
+    const double *F;        // [2][he][he]
+    int M, N, K;            // i1, he, he
+    const double *S;
+    double *T;
+    int mt, n, h;
+    __device__ double a(int bq, int m, int k) const
+    {
+        const int b = bq & 1;
+        const double *X = S + (size_t)(bq >> 1) * mt + m;
+        if (k < h) {
+            const double x = X[(size_t)k * M], y = X[(size_t)(n - 1 - k) * M];
+            return b ? x - y : x + y;
+        }
+        return b ? 0.0 : X[(size_t)k * M];
+    }
+    __device__ double b(int bq, int k, int n_) const { return F[((size_t)(bq & 1) * K + n_) * K + k]; }
+    __device__ void c(int bq, int m, int n_, double v) const
+    {
+        const int b = bq & 1;
+        if (b == 0 || n_ < h) T[(size_t)(bq >> 1) * mt + (size_t)m * n + (size_t)b * K + n_] = v;
+    }
+};
+// Inverse: E (b = 0) on rows j < he, O (b = 1) on rows n-1-j, j < n/2, of
+// S[q]; k_t_zcombine then forms E + O and E - O.
+struct ZInvSplit {
+    const double *G;        // [2][he][he]
+    int M, N, K;            // he, i1, he
+    const double *T;
+    double *S;
+    int mt, n, h;
+    __device__ double a(int bq, int m, int k) const { return G[((size_t)(bq & 1) * M + m) * K + k]; }
+    __device__ double b(int bq, int k, int n_) const
+    {
+        const int b = bq & 1;
+        return (b == 0 || k < h) ? T[(size_t)(bq >> 1) * mt + (size_t)n_ * n + (size_t)b * K + k] : 0.0;
+    }
+    __device__ void c(int bq, int m, int n_, double v) const
+    {
+        const int b = bq & 1;
+        if (b == 0) S[(size_t)(bq >> 1) * mt + (size_t)m * N + n_] = v;
+        else if (m < h) S[(size_t)(bq >> 1) * mt + (size_t)(n - 1 - m) * N + n_] = v;
+    }
+};
+__global__ void __launch_bounds__(WG) k_t_zcombine(double *S, int i1, int n, int mt)
+{
+    const int t = blockIdx.x * WG + threadIdx.x, h = n / 2;
+    if (t >= h * i1) return;
+    const int j = t / i1, i = t - j * i1;
+    double *X = S + (size_t)blockIdx.y * mt + i;
+    const double e = X[(size_t)j * i1], o = X[(size_t)(n - 1 - j) * i1];
+    X[(size_t)j * i1] = e + o;
+    X[(size_t)(n - 1 - j) * i1] = e - o;
+}
 // psi(i, j, k+1) = sum_q Wi[k][q] S[q][u]
 struct YInv {
     const double *Wi;
@@ -1286,8 +1345,10 @@ struct TankOp {
     double *W = nullptr, *Wi = nullptr;   // k1 x k1: y modes (forward, inverse); for even np the
                                           // parity-split halves [2][np/2][np/2] instead
     double *ly = nullptr;                 // k1: y eigenvalues / dp^2
-    double *V = nullptr, *Vinv = nullptr; // j1 x j1: z eigenvectors (row-major) and inverse
-    double *lz = nullptr;                 // j1: z eigenvalues / dz^2
+    double *V = nullptr, *Vinv = nullptr; // j1 x j1: z eigenvectors (row-major) and inverse; with
+                                          // zsplit the parity halves [2][he][he] instead (G, F)
+    double *lz = nullptr;                 // j1: z eigenvalues / dz^2 (zsplit: even modes first)
+    bool zsplit = false;
     double *rt = nullptr;                 // 3 x i1: x rows (sub, diagonal, super)
 };
 
@@ -1483,8 +1544,10 @@ int setup_poisson(vvt_run *r)
             return VV_EINVAL;
         }
         for (double &v : lam) v /= hhz;
-        VV_HIP(upload(&O.V, V));
-        VV_HIP(upload(&O.Vinv, Vinv));
+        std::vector<double> Fz, Gz;
+        O.zsplit = vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
+        VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.lz, lam));
         // x rows: the wall closure (-2, 5/3, -0.2) for the cells of w and v,
         // Dirichlet for the faces of u
@@ -1726,7 +1789,12 @@ int launch_poisson(vvt_run *r, int which)
         else
             vv::gemm_f64<true, false>(YFwd{O.W, O.k1, mt, O.k1, r->f[rhs], r->cS, O.i1, r->ldi, r->ldj}, 1, s);
     }
-    {
+    const int zh = O.j1 / 2, zhe = O.j1 - zh;
+    const double ftr_zs = 2.0 * ((double)zhe * zhe + (double)zh * zh) * O.i1 * O.k1;
+    if (O.zsplit) {
+        Prof p(r, VVT_K_ZFWD, ftr_zs);
+        vv::gemm_f64<false, true>(ZFwdSplit{O.Vinv, O.i1, zhe, zhe, r->cS, r->cT, mt, O.j1, zh}, 2 * O.k1, s);
+    } else {
         Prof p(r, VVT_K_ZFWD, ftr_z);
         vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->cS, r->cT, mt}, O.k1, s);
     }
@@ -1737,7 +1805,13 @@ int launch_poisson(vvt_run *r, int which)
                            (const double *)O.rt, (const double *)O.lz, (const double *)O.ly, O.i1, O.j1, O.k1, O.e1,
                            O.en);
     }
-    {
+    if (O.zsplit) {
+        Prof p(r, VVT_K_ZINV, ftr_zs);
+        vv::gemm_f64<true, true>(ZInvSplit{O.V, zhe, O.i1, zhe, r->cT, r->cS, mt, O.j1, zh}, 2 * O.k1, s);
+        if (zh > 0)
+            hipLaunchKernelGGL(k_t_zcombine, dim3(nblk((long long)zh * O.i1, WG), O.k1), dim3(WG), 0, s, r->cS, O.i1,
+                               O.j1, mt);
+    } else {
         Prof p(r, VVT_K_ZINV, ftr_z);
         vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->cT, r->cS, mt}, O.k1, s);
     }
