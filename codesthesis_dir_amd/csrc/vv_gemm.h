@@ -28,11 +28,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // lane l is D[l/16 + 4e][l%16].  The next k step's operands are loaded into
 // registers while the current one is multiplied.  AKFAST / BKFAST: the tile
 // loads run along k (operand contiguous in k) rather than along m / n.
-template <class Op, bool AKFAST, bool BKFAST, int WAVES, int GK>
+// RB x CB: the 16 x 16 blocks of a wave (2 x 2 by default; wider waves
+// load each operand element for more products).
+template <class Op, bool AKFAST, bool BKFAST, int WAVES, int GK, int RB = 2, int CB = 2>
 __global__ void __launch_bounds__(64 * WAVES) k_gemm_f64(Op op)
 {
     constexpr int WR = (WAVES == 4) ? 2 : 1, WC = WAVES / WR;
-    constexpr int TM = 32 * WR, TN = 32 * WC, NT = 64 * WAVES;
+    constexpr int TM = 16 * RB * WR, TN = 16 * CB * WC, NT = 64 * WAVES;
     constexpr int LA = GK * TM / NT, LB = GK * TN / NT;   // tile elements per thread
     // rows padded by one double: k-fast tile stores (consecutive threads one
     // row apart) then fall on different banks
@@ -46,13 +48,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_gemm_f64(Op op)
     const int q = (int)(lb / (nmt * nnt));
     const int m0 = (int)(lb % nmt) * TM, n0 = (int)((lb / nmt) % nnt) * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave / WC) * 32, wc = (wave % WC) * 32;
+    const int wr = (wave / WC) * 16 * RB, wc = (wave % WC) * 16 * CB;
     const int li = lane & 15, lk = lane >> 4;
-    d4 acc[2][2];
+    d4 acc[RB][CB];
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < CB; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
     double ra[LA], rb[LB];
     auto fetch = [&](int k0) {
 #pragma unroll
@@ -91,22 +93,22 @@ __global__ void __launch_bounds__(64 * WAVES) k_gemm_f64(Op op)
         if (k0 + GK < op.K) fetch(k0 + GK);
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4) {
-            double a[2], b[2];
+            double a[RB], b[CB];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) a[r] = As[ks + lk][wr + 16 * r + li];
+            for (int r = 0; r < RB; ++r) a[r] = As[ks + lk][wr + 16 * r + li];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) b[c] = Bs[ks + lk][wc + 16 * c + li];
+            for (int c = 0; c < CB; ++c) b[c] = Bs[ks + lk][wc + 16 * c + li];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < RB; ++r)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+                for (int c = 0; c < CB; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < CB; ++c)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + wr + 16 * r + lk + 4 * e;
@@ -130,6 +132,16 @@ void gemm_f64(const Op &op, int batch, hipStream_t s)
         dim3 grid((op.N + 31) / 32, (op.M + 31) / 32, batch);
         hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 1, GK>), grid, dim3(64), 0, s, op);
     }
+}
+
+// Tiles of (32 RB) x (32 CB) for four waves, for products whose operand
+// loads bound them (a short k loop over a large operand).
+template <bool AKFAST, bool BKFAST, int RB, int CB, class Op, int GK = 16>
+void gemm_f64_tile(const Op &op, int batch, hipStream_t s)
+{
+    constexpr int TM = 32 * RB, TN = 32 * CB;
+    dim3 grid((op.N + TN - 1) / TN, (op.M + TM - 1) / TM, batch);
+    hipLaunchKernelGGL((k_gemm_f64<Op, AKFAST, BKFAST, 4, GK, RB, CB>), grid, dim3(256), 0, s, op);
 }
 
 }  // namespace vv

@@ -808,13 +808,12 @@ struct ZFwdSplit {
     int mt, n, h;
     __device__ double a(int bq, int m, int k) const
     {
+        // branch-free: the middle row of odd n (k = h) reads itself twice
         const int b = bq & 1;
+        const bool pair = k < h;
         const double *X = S + (size_t)(bq >> 1) * mt + m;
-        if (k < h) {
-            const double x = X[(size_t)k * M], y = X[(size_t)(n - 1 - k) * M];
-            return b ? x - y : x + y;
-        }
-        return b ? 0.0 : X[(size_t)k * M];
+        const double x = X[(size_t)k * M], y = X[(size_t)(pair ? n - 1 - k : k) * M];
+        return b ? (pair ? x - y : 0.0) : (pair ? x + y : x);
     }
     __device__ double b(int bq, int k, int n_) const { return F[((size_t)(bq & 1) * K + n_) * K + k]; }
     __device__ void c(int bq, int m, int n_, double v) const
@@ -1793,7 +1792,10 @@ int launch_poisson(vvt_run *r, int which)
     const double ftr_zs = 2.0 * ((double)zhe * zhe + (double)zh * zh) * O.i1 * O.k1;
     if (O.zsplit) {
         Prof p(r, VVT_K_ZFWD, ftr_zs);
-        vv::gemm_f64<false, true>(ZFwdSplit{O.Vinv, O.i1, zhe, zhe, r->cS, r->cT, mt, O.j1, zh}, 2 * O.k1, s);
+        // 64 x 128 tiles: one tile holds all of a half's modes for he <= 128,
+        // so each workgroup loads its rows of S once
+        vv::gemm_f64_tile<false, true, 2, 4>(ZFwdSplit{O.Vinv, O.i1, zhe, zhe, r->cS, r->cT, mt, O.j1, zh}, 2 * O.k1,
+                                             s);
     } else {
         Prof p(r, VVT_K_ZFWD, ftr_z);
         vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->cS, r->cT, mt}, O.k1, s);
@@ -1807,7 +1809,7 @@ int launch_poisson(vvt_run *r, int which)
     }
     if (O.zsplit) {
         Prof p(r, VVT_K_ZINV, ftr_zs);
-        vv::gemm_f64<true, true>(ZInvSplit{O.V, zhe, O.i1, zhe, r->cT, r->cS, mt, O.j1, zh}, 2 * O.k1, s);
+        vv::gemm_f64_tile<true, true, 4, 2>(ZInvSplit{O.V, zhe, O.i1, zhe, r->cT, r->cS, mt, O.j1, zh}, 2 * O.k1, s);
         if (zh > 0)
             hipLaunchKernelGGL(k_t_zcombine, dim3(nblk((long long)zh * O.i1, WG), O.k1), dim3(WG), 0, s, r->cS, O.i1,
                                O.j1, mt);
