@@ -30,6 +30,8 @@ struct PoissonOp {
     double *Vinv = nullptr;      // j1 x j1, row-major
     double *rt = nullptr;        // 4 x i1: sub, diagonal, super, 1/(dp^2 r^2)
     double *lz = nullptr;        // j1 eigenvalues of Z
+    bool zsplit = false;         // V, Vinv hold the parity halves G, F of z_split; lz
+                                 // the even modes first
 };
 
 struct Poisson3 {
@@ -76,6 +78,11 @@ bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam,
 // is not of one parity to 1e-9 of its largest entry.
 bool z_split(int n, std::vector<double> &lam, const std::vector<double> &V, const std::vector<double> &Vinv,
              std::vector<double> &F, std::vector<double> &G);
+// Whether a solver with n z rows takes the split transforms: from 192 rows
+// (measured faster than the whole products at 255 and 256 rows, slower at
+// 127 and 128 on tiles of 128 modes); VV_ZSPLIT=1 in the environment at
+// create splits at every size the rows allow, VV_ZSPLIT=0 never.
+bool zsplit_wanted(int n);
 
 int poisson3_setup(Poisson3 &P, const Geo &g);
 void poisson3_free(Poisson3 &P);

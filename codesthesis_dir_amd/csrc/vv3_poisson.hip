@@ -19,9 +19,11 @@
 #include "vv3_internal.h"
 #include "vv_fft.h"
 #include "vv_gemm.h"
+#include "vv_zsplit.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #pragma clang fp contract(fast)
 
@@ -582,6 +584,14 @@ bool z_split(int n, std::vector<double> &lam, const std::vector<double> &V, cons
     return true;
 }
 
+bool zsplit_wanted(int n)
+{
+    const char *e = std::getenv("VV_ZSPLIT");
+    if (e && e[0] == '0') return false;
+    if (e && e[0] == '1') return true;
+    return n >= 192;
+}
+
 int poisson3_setup(Poisson3 &P, const Geo &g)
 {
     using vv::set_error;
@@ -658,8 +668,10 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
             return VV_EINVAL;
         }
         for (double &v : lam) v /= hhz;
-        VV_HIP(upload(&O.V, V));
-        VV_HIP(upload(&O.Vinv, Vinv));
+        std::vector<double> Fz, Gz;
+        O.zsplit = zsplit_wanted(O.j1) && z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
+        VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.rt, rt));
         VV_HIP(upload(&O.lz, lam));
     }
@@ -717,10 +729,12 @@ int poisson3_launch(const Poisson3 &P, const Geo &g, int which, double *psi, dou
                            g.ldi, g.ldj, mt);
     else
         gemm_f64<true, false>(PhiFwd{P.W, np, mt, np, rhs, S, i1, g.ldi, g.ldj}, 1, s);
-    gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, np, s);
+    if (O.zsplit) vv::zsplit_fwd(O.Vinv, S, T, i1, j1, np, s);
+    else gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, np, s);
     hipLaunchKernelGGL(k3_rsolve, dim3((unsigned)((nm * j1 + 63) / 64)), dim3(64), 0, s, T, cpw,
                        (const double *)O.rt, (const double *)O.lz, (const double *)P.lphi, i1, j1, np, O.e1, O.en);
-    gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, np, s);
+    if (O.zsplit) vv::zsplit_inv(O.V, T, S, i1, j1, np, s);
+    else gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, np, s);
     if (P.fft && fR == 0)
         hipLaunchKernelGGL(k3_fft_inv<0>, fgrid, dim3(256), flds, s, (const double *)S, psi, tw, np, bits, 1, LP, i1,
                            j1, g.ldi, g.ldj, mt);

@@ -32,6 +32,7 @@
 #include "vv3_internal.h"
 #include "vv_gemm.h"
 #include "vv_ytrans.h"
+#include "vv_zsplit.h"
 #include "../../include/vvsector.h"
 
 #include <cmath>
@@ -108,6 +109,7 @@ struct SOp {
     double *Wi = nullptr;        // k1 x k1: inverse, row = plane
     double *lphi = nullptr;      // k1 mode eigenvalues 2 cos(m pi/np) - 2
     double *V = nullptr, *Vinv = nullptr, *lz = nullptr;   // z eigenbasis
+    bool zsplit = false;         // V, Vinv: the parity halves of vv3::z_split; lz even modes first
     double *rt = nullptr;        // 4 x i1: sub, diagonal, super, 1/(dp^2 r^2)
 };
 
@@ -1005,8 +1007,10 @@ int spoisson_setup(SPoisson &P, const Geo &g)
             VV_HIP(upload(&O.Wi, Wi));
         }
         VV_HIP(upload(&O.lphi, lphi));
-        VV_HIP(upload(&O.V, V));
-        VV_HIP(upload(&O.Vinv, Vinv));
+        std::vector<double> Fz, Gz;
+        O.zsplit = vv3::zsplit_wanted(O.j1) && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
+        VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.lz, lam));
         VV_HIP(upload(&O.rt, rt));
         smax = std::max(smax, (size_t)k1 * O.m);
@@ -1048,10 +1052,12 @@ int spoisson_launch(const SPoisson &P, const Geo &g, int which, double *psi, dou
     hipLaunchKernelGGL(ks_psi_pre, pg, dim3(256), 0, s, g, which, (const double *)psi, psip);
     if (P.fft) vvt::ytrans_fwd(P.yt, which == 2, rhs, S, i1, mt, s);
     else vv::gemm_f64<true, false>(PhiFwd{O.W, k1, mt, k1, rhs, S, i1, g.ldi, g.ldj}, 1, s);
-    vv::gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, k1, s);
+    if (O.zsplit) vv::zsplit_fwd(O.Vinv, S, T, i1, j1, k1, s);
+    else vv::gemm_f64<false, true>(ZFwd{O.Vinv, i1, j1, j1, S, T, mt}, k1, s);
     hipLaunchKernelGGL(ks_rsolve, dim3((unsigned)((k1 * j1 + 63) / 64)), dim3(64), 0, s, T, cp,
                        (const double *)O.rt, (const double *)O.lz, (const double *)O.lphi, i1, j1, k1, O.e1, O.en);
-    vv::gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, k1, s);
+    if (O.zsplit) vv::zsplit_inv(O.V, T, S, i1, j1, k1, s);
+    else vv::gemm_f64<true, true>(ZInv{O.V, j1, i1, j1, T, S, mt}, k1, s);
     if (P.fft) vvt::ytrans_inv(P.yt, which == 2, S, psi, i1, mt, s);
     else vv::gemm_f64<true, false>(PhiInv{O.Wi, k1, mt, k1, S, psi, i1, g.ldi, g.ldj}, 1, s);
     hipLaunchKernelGGL(ks_psi_post, pg, dim3(256), 0, s, g, which, psi);

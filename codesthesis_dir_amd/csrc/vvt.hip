@@ -24,6 +24,7 @@
 #include "vv_gemm.h"
 #include "vv_reduce.h"
 #include "vv_ytrans.h"
+#include "vv_zsplit.h"
 
 #include "../../include/vvtank.h"
 
@@ -795,64 +796,6 @@ struct ZInv {
     __device__ double b(int q, int k, int n) const { return T[(size_t)q * mt + (size_t)n * K + k]; }
     __device__ void c(int q, int m, int n, double v) const { S[(size_t)q * mt + (size_t)m * N + n] = v; }
 };
-// The z transforms split by parity (the z rows read the same from either
-// wall): batch 2q + b, b = 0 the even modes, 1 the odd ones, he = n - n/2
-// rows and modes per half.  Forward: T[q][i][b he + l] = sum_j F[b][l][j]
-// (S[q][j][i] +- S[q][n-1-j][i]) over j < n/2, the middle row of odd n
-// joining the even half alone.
-struct ZFwdSplit {
-    const double *F;        // [2][he][he]
-    int M, N, K;            // i1, he, he
-    const double *S;
-    double *T;
-    int mt, n, h;
-    __device__ double a(int bq, int m, int k) const
-    {
-        // branch-free: the middle row of odd n (k = h) reads itself twice
-        const int b = bq & 1;
-        const bool pair = k < h;
-        const double *X = S + (size_t)(bq >> 1) * mt + m;
-        const double x = X[(size_t)k * M], y = X[(size_t)(pair ? n - 1 - k : k) * M];
-        return b ? (pair ? x - y : 0.0) : (pair ? x + y : x);
-    }
-    __device__ double b(int bq, int k, int n_) const { return F[((size_t)(bq & 1) * K + n_) * K + k]; }
-    __device__ void c(int bq, int m, int n_, double v) const
-    {
-        const int b = bq & 1;
-        if (b == 0 || n_ < h) T[(size_t)(bq >> 1) * mt + (size_t)m * n + (size_t)b * K + n_] = v;
-    }
-};
-// Inverse: E (b = 0) on rows j < he, O (b = 1) on rows n-1-j, j < n/2, of
-// S[q]; k_t_zcombine then forms E + O and E - O.
-struct ZInvSplit {
-    const double *G;        // [2][he][he]
-    int M, N, K;            // he, i1, he
-    const double *T;
-    double *S;
-    int mt, n, h;
-    __device__ double a(int bq, int m, int k) const { return G[((size_t)(bq & 1) * M + m) * K + k]; }
-    __device__ double b(int bq, int k, int n_) const
-    {
-        const int b = bq & 1;
-        return (b == 0 || k < h) ? T[(size_t)(bq >> 1) * mt + (size_t)n_ * n + (size_t)b * K + k] : 0.0;
-    }
-    __device__ void c(int bq, int m, int n_, double v) const
-    {
-        const int b = bq & 1;
-        if (b == 0) S[(size_t)(bq >> 1) * mt + (size_t)m * N + n_] = v;
-        else if (m < h) S[(size_t)(bq >> 1) * mt + (size_t)(n - 1 - m) * N + n_] = v;
-    }
-};
-__global__ void __launch_bounds__(WG) k_t_zcombine(double *S, int i1, int n, int mt)
-{
-    const int t = blockIdx.x * WG + threadIdx.x, h = n / 2;
-    if (t >= h * i1) return;
-    const int j = t / i1, i = t - j * i1;
-    double *X = S + (size_t)blockIdx.y * mt + i;
-    const double e = X[(size_t)j * i1], o = X[(size_t)(n - 1 - j) * i1];
-    X[(size_t)j * i1] = e + o;
-    X[(size_t)(n - 1 - j) * i1] = e - o;
-}
 // psi(i, j, k+1) = sum_q Wi[k][q] S[q][u]
 struct YInv {
     const double *Wi;
@@ -1544,7 +1487,7 @@ int setup_poisson(vvt_run *r)
         }
         for (double &v : lam) v /= hhz;
         std::vector<double> Fz, Gz;
-        O.zsplit = vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        O.zsplit = vv3::zsplit_wanted(O.j1) && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
         VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
         VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.lz, lam));
@@ -1792,10 +1735,7 @@ int launch_poisson(vvt_run *r, int which)
     const double ftr_zs = 2.0 * ((double)zhe * zhe + (double)zh * zh) * O.i1 * O.k1;
     if (O.zsplit) {
         Prof p(r, VVT_K_ZFWD, ftr_zs);
-        // 64 x 128 tiles: one tile holds all of a half's modes for he <= 128,
-        // so each workgroup loads its rows of S once
-        vv::gemm_f64_tile<false, true, 2, 4>(ZFwdSplit{O.Vinv, O.i1, zhe, zhe, r->cS, r->cT, mt, O.j1, zh}, 2 * O.k1,
-                                             s);
+        vv::zsplit_fwd(O.Vinv, r->cS, r->cT, O.i1, O.j1, O.k1, s);
     } else {
         Prof p(r, VVT_K_ZFWD, ftr_z);
         vv::gemm_f64<false, true>(ZFwd{O.Vinv, O.i1, O.j1, O.j1, r->cS, r->cT, mt}, O.k1, s);
@@ -1809,10 +1749,7 @@ int launch_poisson(vvt_run *r, int which)
     }
     if (O.zsplit) {
         Prof p(r, VVT_K_ZINV, ftr_zs);
-        vv::gemm_f64_tile<true, true, 4, 2>(ZInvSplit{O.V, zhe, O.i1, zhe, r->cT, r->cS, mt, O.j1, zh}, 2 * O.k1, s);
-        if (zh > 0)
-            hipLaunchKernelGGL(k_t_zcombine, dim3(nblk((long long)zh * O.i1, WG), O.k1), dim3(WG), 0, s, r->cS, O.i1,
-                               O.j1, mt);
+        vv::zsplit_inv(O.V, r->cT, r->cS, O.i1, O.j1, O.k1, s);
     } else {
         Prof p(r, VVT_K_ZINV, ftr_z);
         vv::gemm_f64<true, true>(ZInv{O.V, O.j1, O.i1, O.j1, r->cT, r->cS, mt}, O.k1, s);
