@@ -140,15 +140,16 @@ __global__ void __launch_bounds__(256) k_zinv_pair(const double *G, const double
             }
 }
 
-// The forward: for he > 64 on 64 x 128 tiles, so that for he <= 128 one
-// tile holds all of a half's modes and each workgroup loads its rows of S
-// once; smaller he: the usual tiles.  2 nq <= 65535.  The inverse:
-// k_zinv_pair, nq <= 65535.
+// The forward: for he > 64 on 128 x 128 tiles (for he <= 128 one tile holds
+// all of a half's modes; measured faster than 64 x 128, 64 x 64 and
+// 128 x 64 tiles and than one workgroup per block of both halves); smaller
+// he: the usual tiles.  2 nq <= 65535.  The inverse: k_zinv_pair,
+// nq <= 65535.
 inline void zsplit_fwd(const double *F, const double *S, double *T, int i1, int n, int nq, hipStream_t s)
 {
     const int h = n / 2, he = n - h;
     const ZFwdSplit op{F, i1, he, he, S, T, i1 * n, n, h};
-    if (he > 64) gemm_f64_tile<false, true, 2, 4>(op, 2 * nq, s);
+    if (he > 64) gemm_f64_tile<false, true, 4, 4>(op, 2 * nq, s);
     else gemm_f64<false, true>(op, 2 * nq, s);
 }
 
