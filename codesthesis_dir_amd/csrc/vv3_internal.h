@@ -78,10 +78,10 @@ bool eigen_banded(const std::vector<double> &M, int n, std::vector<double> &lam,
 // is not of one parity to 1e-9 of its largest entry.
 bool z_split(int n, std::vector<double> &lam, const std::vector<double> &V, const std::vector<double> &Vinv,
              std::vector<double> &F, std::vector<double> &G);
-// Whether a solver with n z rows takes the split transforms: from 192 rows
-// (measured faster than the whole products at 255 and 256 rows, slower at
-// 127 and 128 on tiles of 128 modes); VV_ZSPLIT=1 in the environment at
-// create splits at every size the rows allow, VV_ZSPLIT=0 never.
+// Whether a solver with n z rows takes the split transforms: from 96 rows
+// (measured faster than the whole products at 127, 128, 255 and 256 rows,
+// about 1 % slower at 31 and 32); VV_ZSPLIT=1 in the environment at create
+// splits at every size the rows allow, VV_ZSPLIT=0 never.
 bool zsplit_wanted(int n);
 
 int poisson3_setup(Poisson3 &P, const Geo &g);

@@ -589,7 +589,7 @@ bool zsplit_wanted(int n)
     const char *e = std::getenv("VV_ZSPLIT");
     if (e && e[0] == '0') return false;
     if (e && e[0] == '1') return true;
-    return n >= 192;
+    return n >= 96;
 }
 
 int poisson3_setup(Poisson3 &P, const Geo &g)

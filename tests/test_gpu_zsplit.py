@@ -1,6 +1,6 @@
 """The z transforms split by parity (codesthesis_dir_amd/csrc/vv_zsplit.h) in
 the three 3D solvers: the tank, the annulus of rot3dmg.f and the barrier
-sector.  By default the split runs from 192 z rows; VV_ZSPLIT=1 forces it at
+sector.  By default the split runs from 96 z rows; VV_ZSPLIT=1 forces it at
 every size, so that the solve tests of the three solvers (against the
 oracles' dense or exact solves, and the reference's residual operators on
 grids beyond them) run on it:
