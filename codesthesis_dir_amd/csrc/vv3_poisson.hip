@@ -669,7 +669,7 @@ int poisson3_setup(Poisson3 &P, const Geo &g)
         }
         for (double &v : lam) v /= hhz;
         std::vector<double> Fz, Gz;
-        O.zsplit = zsplit_wanted(O.j1) && z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        O.zsplit = zsplit_wanted(O.j1) && 2 * np <= 65535 && z_split(O.j1, lam, V, Vinv, Fz, Gz);
         VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
         VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.rt, rt));

@@ -1008,7 +1008,7 @@ int spoisson_setup(SPoisson &P, const Geo &g)
         }
         VV_HIP(upload(&O.lphi, lphi));
         std::vector<double> Fz, Gz;
-        O.zsplit = vv3::zsplit_wanted(O.j1) && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        O.zsplit = vv3::zsplit_wanted(O.j1) && 2 * k1 <= 65535 && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
         VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
         VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.lz, lam));

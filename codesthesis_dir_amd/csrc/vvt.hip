@@ -1487,7 +1487,7 @@ int setup_poisson(vvt_run *r)
         }
         for (double &v : lam) v /= hhz;
         std::vector<double> Fz, Gz;
-        O.zsplit = vv3::zsplit_wanted(O.j1) && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
+        O.zsplit = vv3::zsplit_wanted(O.j1) && 2 * O.k1 <= 65535 && vv3::z_split(O.j1, lam, V, Vinv, Fz, Gz);
         VV_HIP(upload(&O.V, O.zsplit ? Gz : V));
         VV_HIP(upload(&O.Vinv, O.zsplit ? Fz : Vinv));
         VV_HIP(upload(&O.lz, lam));
