@@ -325,6 +325,8 @@ def rot3d_routines(nr, nz, np_, reps=10):
         for _ in range(reps):
             f()
         ms = 1e3 * (time.perf_counter() - t0) / reps
+        # the dense phi and whole z products' flops: an equivalent rate where the solver
+        # uses FFTs or splits the z products by parity
         flops = 2 * (2.0 * np_ * np_ * m) + 2 * (2.0 * nz * nz * nr * np_)
         out[f"poisson{which}"] = {"ms": ms, "transform_TFLOP_s": flops / (ms * 1e-3) / 1e12}
     run.close()
@@ -462,7 +464,8 @@ def sector_leg(warmup=3, only=None, reps=10):
                     run.poisson(which)
                 ms = 1e3 * (time.perf_counter() - t0) / reps
                 k1 = np_ - 1 if which == 2 else np_
-                # the z transforms' flops (the phi transforms are FFTs at a power-of-two np)
+                # the whole z products' flops (the phi transforms are FFTs at a power-of-two
+                # np): an equivalent rate where the solver splits them by parity (half the flops)
                 flops = 2 * (2.0 * nz * nz * nr * k1)
                 rt[f"poisson{which}"] = {"ms": ms, "z_transform_TFLOP_s": flops / (ms * 1e-3) / 1e12}
             out[name]["routines"] = rt
